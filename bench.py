@@ -1,0 +1,150 @@
+"""Benchmark of the run! + build_edges hot path on one MI355X.
+
+Workload (one step = one pass of the hot path over one batch): the 3D SPH
+block, 1,000,000 FP32 points on a 100^3 lattice (h = 0.01, U(+-0.1h) jitter),
+radius 2h; every step adds N(0, (0.01h)^2) per coordinate (the snapshots are
+generated on the device before the timed region), then update_point_set +
+run! + build_edges on the self pair.  The metric is edges/s of build_edges
+with a bit-exact edge set; the line reports the exact two-pass path and, when
+available, the learned single pass.
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "octopus.jl_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="exact", choices=["exact", "learned"])
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU oracle baseline (0 = skip)")
+    ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M points)")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import octohip
+    from octohip import datasets
+
+    h = 0.01
+    pts, r = datasets.sph_lattice(side=a.side, h=h)
+    n = pts.shape[0]
+    dev = torch.device("cuda", 0)
+    base = torch.from_numpy(pts).to(dev)
+    total_steps = a.warmup + a.steps
+    g = torch.Generator(device=dev)
+    g.manual_seed(2)
+    snaps = [base]
+    for _ in range(total_steps):
+        snaps.append((snaps[-1].double() + torch.randn(base.shape, generator=g, device=dev, dtype=torch.float64)
+                      * (0.01 * h)).float().contiguous())
+    snaps = snaps[1:]
+    torch.cuda.synchronize()
+
+    tns = octohip.TNS()
+    octohip.set_search_radius(tns, float(r))
+    sid = octohip.add_point_set(tns, snaps[0])
+    octohip.set_active_search(tns, sid, sid)
+
+    def step(k):
+        octohip.update_point_set(tns, sid, snaps[k])
+        octohip.run(tns)
+        e = octohip.build_edges(tns, sid, sid, mode=a.mode)
+        return e.senders.shape[0]
+
+    for k in range(a.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    octohip.profile(tns, True)
+    octohip.profile_read(tns, reset=True)
+    edges = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.warmup, total_steps):
+        edges += step(k)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = octohip.profile_read(tns, reset=True)
+    octohip.profile(tns, False)
+    sec = t1 - t0
+    ms_step = 1e3 * sec / a.steps
+    e_avg = edges / a.steps
+
+    # dominant kernel: edge emission (fill) in exact mode, the single pass in learned mode
+    kname = "fill" if a.mode == "exact" else "single"
+    kms, kcalls = prof[kname]
+    k_avg_s = (kms / max(kcalls, 1)) / 1e3
+    alg_bytes = 24.0 * e_avg + 12.0 * n  # 24 B per emitted edge + 12 B per query point read
+    achieved = alg_bytes / k_avg_s / 1e9 if k_avg_s > 0 else 0.0
+    run_ms = (prof["bounds"][0] + prof["build"][0]) / a.steps
+    edges_ms = sum(prof[p][0] for p in ("leaf_lists", "count", "scan", "fill", "single")) / a.steps
+    step_bytes = 16.0 * n + 24.0 * e_avg + 12.0 * n  # coords + perm write, edge output, query reads (tree bytes excluded)
+
+    cpu = None
+    if a.cpu_steps > 0:
+        import oracle
+
+        oracle.build()
+        threads = oracle.max_threads()
+        host = [s.cpu().numpy() for s in snaps[a.warmup:a.warmup + a.cpu_steps]]
+        ce = 0
+        c0 = time.perf_counter()
+        for hp in host:
+            t = oracle.run(hp, float(r))
+            ce += oracle.build_edges(t, hp).n_edges
+        c1 = time.perf_counter()
+        cpu = {"value": ce / (c1 - c0), "unit": "edges/s", "cores": threads, "kind": "port",
+               "sample": f"{len(host)} steps of the same 1M-point SPH workload (run! + build_edges, "
+                         f"C oracle, OpenMP {threads} threads), {c1 - c0:.1f} s"}
+
+    out = {
+        "metric": "edges/s, learned vs exact-count build_edges, 1M pts 3D, bit-exact edges",
+        "value": edges / sec,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"SPH block: {n} FP32 points on a {a.side}^3 lattice, h=0.01, jitter 0.1h, "
+                               f"radius 2h, N(0,(0.01h)^2) step displacement; update + run! + build_edges per step",
+                   "mode": a.mode, "points": n, "edges_per_step": e_avg, "radius": float(r)},
+        "breakdown_ms_per_step": {"run": run_ms, "build_edges": edges_ms,
+                                  **{k: v[0] / a.steps for k, v in prof.items() if v[1]}},
+        "points_per_s": n * a.steps / sec,
+        "hbm_fraction_step": (step_bytes / (ms_step / 1e3)) / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_edges<EDGE_FILL>" if a.mode == "exact" else "k_edges_single",
+                     "kernel_avg_ms": k_avg_s * 1e3, "alg_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+/*
+ * octohip: C ABI of the MI355X fixed-radius neighbour search engine.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device addresses on
+ * the context's device.  Every function returns OCT_OK (0) or a negative
+ * status; oct_last_error() gives the message.  A context is not reentrant:
+ * all work of one context is ordered on its stream, and the host waits only
+ * where a host value is returned.
+ *
+ * Each entry point names the reference interface it replaces
+ * (una-auxme/Octopus.jl, file:line).  Index conventions follow index_base
+ * (1 for a Julia caller, 0 for C / Python): set ids, point indices, perm
+ * entries, node ids in node_first/node_last/node_children and
+ * senders/receivers all carry it; -1 stays the empty-child sentinel.
+ */
+#ifndef OCTOHIP_H
+#define OCTOHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; the front-ends map them to the reference's exception types */
+#define OCT_OK 0
+#define OCT_ERR_ARGUMENT (-1)     /* ArgumentError   (src/api.jl:25,184,190) */
+#define OCT_ERR_DIMENSION (-2)    /* DimensionMismatch (src/api.jl:97,137-138) */
+#define OCT_ERR_STATE (-3)        /* ErrorException  (src/api.jl:306,452) */
+#define OCT_ERR_DEVICE (-4)       /* HIP runtime failure */
+#define OCT_ERR_CAPACITY (-5)     /* size beyond a documented limit (E >= 2^31) */
+
+/* build_edges modes */
+#define OCT_EDGES_EXACT 0         /* count pass, scan, fill pass (two traversals) */
+#define OCT_EDGES_LEARNED 1       /* single pass sized by the per-leaf count predictor */
+
+/* edge order */
+#define OCT_ORDER_LEAF 0          /* receiver runs in Morton (tree leaf) order */
+#define OCT_ORDER_RECEIVER 1      /* receivers ascending, as the reference writes them */
+
+typedef struct oct_ctx oct_ctx;
+
+/* Flat-COO edges of one active pair (src/types.jl:84-100 EdgeBuffer).
+ * Device arrays owned by the context, valid until the next build on the pair
+ * (the reference aliases its EdgeBuffer the same way, test/contracts.jl:223-240). */
+typedef struct oct_edges {
+    int32_t* senders;          /* E, target index j */
+    int32_t* receivers;        /* E, query index i */
+    float* rel_displacement;   /* 3*E, (q_i - t_j) / r, xyz per edge */
+    float* rel_dist_norm;      /* E, |q_i - t_j| / r */
+    int64_t n_edges;
+} oct_edges;
+
+/* Tree of one point set, reference layout (src/types.jl:23-42), device arrays
+ * owned by the context, valid until the next oct_run. */
+typedef struct oct_tree {
+    float* node_bounds_min;    /* 3*M */
+    float* node_bounds_max;    /* 3*M */
+    int32_t* node_first;       /* M, inclusive first position into perm */
+    int32_t* node_last;        /* M, inclusive last position */
+    int32_t* node_children;    /* 8*M, octant order, -1 empty; a leaf has all 8 empty */
+    int32_t* permutation;      /* N, sorted position -> point index */
+    int32_t n_nodes;
+    int32_t n_leaves;
+    int64_t n_points;
+    float origin[3];
+} oct_tree;
+
+/* Borrowed device handle for user kernels (ext/OctopusCUDAExt.jl:23-35
+ * DeviceView); pass it to oct_for_each_neighbor_device (octohip_device.h).
+ * Invalidated by the next oct_run. */
+typedef struct oct_view {
+    const float* coords_q;     /* query coords, 3*Nq */
+    const float* coords_t;     /* target coords, 3*Nt */
+    const float* bounds_min;   /* reference-layout tree, as in oct_tree */
+    const float* bounds_max;
+    const int32_t* node_first;
+    const int32_t* node_last;
+    const int32_t* node_children;
+    const int32_t* perm;
+    const void* nodes;         /* packed traversal records of the target tree */
+    const float* sorted_t;     /* 4*Nt Morton-sorted target coords, w = point index bits */
+    int32_t n_nodes;
+    int32_t root;              /* first node with more than one child (or the lone leaf) */
+    float radius;
+    float radius_sq;
+    int64_t n_q;
+    int32_t index_base;
+    int32_t pad;
+} oct_view;
+
+/* TNS(T; ndims) (src/types.jl:167-183).  ndims must be 3; device is a HIP
+ * ordinal.  The context creates its own stream unless oct_set_stream is used. */
+int oct_create(oct_ctx** out, int device, int ndims, int index_base);
+int oct_destroy(oct_ctx* ctx);
+const char* oct_last_error(const oct_ctx* ctx);
+/* Order all work of the context after/before other work on this stream. */
+int oct_set_stream(oct_ctx* ctx, void* hip_stream);
+
+/* set_search_radius! (src/api.jl:24-29): r > 0, marks every set dirty. */
+int oct_set_search_radius(oct_ctx* ctx, float r);
+/* set_refit_mode! (src/api.jl:43-47): keep keys and perm between builds and
+ * try the almost-sorted path first; marks every set dirty. */
+int oct_set_refit_mode(oct_ctx* ctx, int on);
+/* target_leaf_size (src/types.jl:177, fixed to 32 there); 1..256. */
+int oct_set_leaf_size(oct_ctx* ctx, int leaf_size);
+
+/* add_point_set! (src/api.jl:79-109): coords is a caller-owned device array of
+ * n points, xyz interleaved (the bytes of a column-major (3, n) matrix); it is
+ * referenced, never copied.  Writes the set id. */
+int oct_add_point_set(oct_ctx* ctx, const float* coords, int64_t n, int* set_id);
+/* update_point_set! (src/api.jl:131-140): same n required. */
+int oct_update_point_set(oct_ctx* ctx, int set_id, const float* coords, int64_t n);
+/* resize_point_set! (src/api.jl:111-129): n may change. */
+int oct_resize_point_set(oct_ctx* ctx, int set_id, const float* coords, int64_t n);
+/* set_active_search! (src/api.jl:142-165): directed pair, idempotent. */
+int oct_set_active_search(oct_ctx* ctx, int q, int t);
+
+/* run! (src/api.jl:183-250, ext/OctopusCUDAExt.jl:214-248): rebuild every
+ * dirty set; invalidates all edge buffers. */
+int oct_run(oct_ctx* ctx);
+/* The built tree of a set (tns.trees[id], tns.permutation[id]). */
+int oct_get_tree(oct_ctx* ctx, int set_id, oct_tree* out);
+
+/* build_edges / build_edges! (src/api.jl:304-337,
+ * ext/OctopusCUDAExt.jl:345-415): flat-COO edges of the active pair (q, t);
+ * the self pair (q == t) excludes j == i.  mode: OCT_EDGES_*; order:
+ * OCT_ORDER_* (OCT_ORDER_RECEIVER needs OCT_EDGES_EXACT). */
+int oct_build_edges(oct_ctx* ctx, int q, int t, int mode, int order, oct_edges* out);
+
+/* device_view (src/api.jl:451-454, ext/OctopusCUDAExt.jl:37-56). */
+int oct_device_view(oct_ctx* ctx, int q, int t, oct_view* out);
+
+/* Per-query neighbour counts through the device iterator of octohip_device.h,
+ * self not filtered (the reference's test kernel, test/query_cuda.jl:40-56).
+ * counts is a device array of view->n_q int32. */
+int oct_count_neighbors_device(oct_ctx* ctx, const oct_view* view, int32_t* counts);
+
+/* Pullback of build_edges_diff (ext/OctopusChainRulesCoreExt.jl:137-192,
+ * ext/OctopusCUDAChainRulesCoreExt.jl:56-167): grad (3*n_points, device) is
+ * overwritten with dL/dcoords.  d_disp (3*E) or d_dist (E) may be NULL for a
+ * zero tangent.  Edge indices carry the context's index_base. */
+int oct_build_edges_backward(oct_ctx* ctx, int64_t n_points, const oct_edges* edges,
+                             const float* d_disp, const float* d_dist, float radius, float* grad);
+
+/* Kernel timing by phase with HIP events on the context's stream (off by
+ * default).  oct_profile_read waits for the stream, then writes the summed
+ * milliseconds and launch counts of the OCT_PROF_NPHASES phases below. */
+#define OCT_PROF_BOUNDS 0      /* origin / extent reduction */
+#define OCT_PROF_BUILD 1       /* keys, radix sort, topology, refit */
+#define OCT_PROF_LEAF_LISTS 2  /* candidate target leaves per query leaf */
+#define OCT_PROF_COUNT 3       /* per-query edge counts */
+#define OCT_PROF_SCAN 4        /* exclusive scan of the counts */
+#define OCT_PROF_FILL 5        /* edge emission */
+#define OCT_PROF_BACKWARD 6    /* build_edges_diff adjoint */
+#define OCT_PROF_SINGLE 7      /* learned single pass */
+#define OCT_PROF_NPHASES 8
+int oct_profile(oct_ctx* ctx, int enable);
+int oct_profile_read(oct_ctx* ctx, int reset, double* ms, int64_t* calls);
+
+/* Block until all work queued on the context's stream is done. */
+int oct_synchronize(oct_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,584 @@
+// run! on the GPU: origin/extent reduction, Morton keys, stable LSD radix
+// sort, octree topology, bounds refit.
+//
+// Reference behaviour (src/api.jl:194-250, src/cpu/*.jl): origin = per-axis
+// min - Float32(1e-6); key = 21-bit-per-axis Morton code of
+// clamp(floor((p - origin) / r)); perm sorts the keys; BFS octree where a node
+// splits on the 3 key bits of its level, is a leaf iff count <= leaf size or
+// level == 21, children in octant order and node ids in BFS order; node bounds
+// are the exact min/max of the node's points.
+//
+// The topology here is not built level by level.  For sorted keys sk, let
+// m_k = number of levels sk[k-1] and sk[k] share (m_0 = -1).  Position S
+// starts the level-L prefix range iff m_S < L.  The node at level L starting
+// at S exists iff its parent range has more than T points, and the prefix
+// ranges shrink with L, so the nodes starting at S are exactly levels
+// m_S < L <= lambda(S) (when S = 0 or the parent range is big), where
+//   lambda(S) = min(21, max(m_S, min{m_j : S < j <= S + T}) + 1)
+// is the level at which the range starting at S first holds <= T points.
+// Counting those (level, S) pairs per block, one exclusive scan over
+// (level, block) gives every node its BFS id directly, since BFS order is
+// level-major and Morton-ordered within a level.  Four launches build the
+// whole topology regardless of depth.
+#include "internal.hpp"
+
+namespace oct {
+
+// =============================================================== bounds
+
+constexpr int BND_THREADS = 256;
+
+// Per-block ordered-uint min/max of the coordinates; the last block to finish
+// folds the partials and derives origin, 1/r and the live key bits.
+__global__ void __launch_bounds__(BND_THREADS)
+k_bounds(const float* __restrict__ coords, int64_t n, float radius, uint32_t* __restrict__ partials,
+         BuildParams* __restrict__ prm) {
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+    uint32_t mx[3] = {0u, 0u, 0u};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        for (int d = 0; d < 3; ++d) {
+            uint32_t o = f2o(coords[3 * i + d]);
+            mn[d] = o < mn[d] ? o : mn[d];
+            mx[d] = o > mx[d] ? o : mx[d];
+        }
+    }
+    __shared__ uint32_t red[6][BND_THREADS / OCT_WAVE];
+    const int w = threadIdx.x / OCT_WAVE;
+    for (int d = 0; d < 3; ++d) {
+        uint32_t a = wave_min_u(mn[d]);
+        uint32_t b = wave_max_u(mx[d]);
+        if (lane_id() == 0) { red[d][w] = a; red[3 + d][w] = b; }
+    }
+    __syncthreads();
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        uint32_t v[6];
+        for (int d = 0; d < 6; ++d) v[d] = red[d][0];
+        for (int k = 1; k < BND_THREADS / OCT_WAVE; ++k)
+            for (int d = 0; d < 3; ++d) {
+                v[d] = red[d][k] < v[d] ? red[d][k] : v[d];
+                v[3 + d] = red[3 + d][k] > v[3 + d] ? red[3 + d][k] : v[3 + d];
+            }
+        for (int d = 0; d < 6; ++d) partials[6 * blockIdx.x + d] = v[d];
+        __threadfence();
+        uint32_t ticket = atomicAdd(&prm->done, 1u);
+        last = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    for (unsigned b = 0; b < gridDim.x; ++b)
+        for (int d = 0; d < 3; ++d) {
+            uint32_t a = (uint32_t)load_coherent((const int*)&partials[6 * b + d]);
+            uint32_t c = (uint32_t)load_coherent((const int*)&partials[6 * b + 3 + d]);
+            v[d] = a < v[d] ? a : v[d];
+            v[3 + d] = c > v[3 + d] ? c : v[3 + d];
+        }
+    const float inv_cs = 1.0f / radius;  // one(T) / cell_size
+    uint32_t cmax = 0;
+    for (int d = 0; d < 3; ++d) {
+        prm->omin[d] = v[d];
+        prm->omax[d] = v[3 + d];
+        const float o = o2f(v[d]) - 1e-6f;  // ox - T(1e-6)
+        prm->origin[d] = o;
+        const uint32_t c = floor_clamp((o2f(v[3 + d]) - o) * inv_cs, 0x1fffffu);
+        cmax = c > cmax ? c : cmax;
+    }
+    prm->inv_cs = inv_cs;
+    prm->cmax = cmax;
+    prm->key_bits = cmax == 0 ? 0 : 3 * (32 - __clz((int)cmax));
+    prm->done = 0;  // ready for the next build
+}
+
+// ================================================================ keys
+
+__global__ void __launch_bounds__(256)
+k_keys(const float* __restrict__ coords, int64_t n, const BuildParams* __restrict__ prm,
+       uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float ox = prm->origin[0], oy = prm->origin[1], oz = prm->origin[2];
+    const float inv = prm->inv_cs;
+    const float px = coords[3 * i], py = coords[3 * i + 1], pz = coords[3 * i + 2];
+    const uint32_t ix = floor_clamp((px - ox) * inv, 0x1fffffu);
+    const uint32_t iy = floor_clamp((py - oy) * inv, 0x1fffffu);
+    const uint32_t iz = floor_clamp((pz - oz) * inv, 0x1fffffu);
+    keys[i] = morton3(ix, iy, iz);
+    vals[i] = (int32_t)i;
+}
+
+// ========================================================== radix sort
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per tile
+constexpr int RS_WAVES = RS_THREADS / OCT_WAVE;
+constexpr int RS_BITS = 8;
+constexpr int RS_BUCKETS = 1 << RS_BITS;
+
+// Tile digit histograms, stored digit-major (hist[d * ntiles + tile]) so one
+// exclusive scan yields every (digit, tile) output offset.
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift, uint32_t* __restrict__ hist,
+          int ntiles) {
+    __shared__ uint32_t h[RS_BUCKETS];
+    for (int d = threadIdx.x; d < RS_BUCKETS; d += RS_THREADS) h[d] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int i = 0; i < RS_ITEMS; ++i) {
+        const int64_t k = base + (int64_t)i * RS_THREADS + threadIdx.x;
+        if (k < n) atomicAdd(&h[(uint32_t)(keys[k] >> shift) & (RS_BUCKETS - 1)], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < RS_BUCKETS; d += RS_THREADS) hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
+}
+
+// Stable scatter of one tile.  Items are ranked in index order with wave
+// ballots (one "same digit" peer mask per item), staged in LDS in digit order,
+// then written out so consecutive lanes write consecutive addresses of one
+// digit run.  The last pass also writes perm (+ index base) and the
+// Morton-sorted coordinates with the original index in w.
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_scatter(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
+             uint64_t* __restrict__ kout, int32_t* __restrict__ vout, int64_t n, int shift,
+             const uint32_t* __restrict__ hist_off, int ntiles, int last_pass,
+             int32_t* __restrict__ perm_out, float4* __restrict__ scoord,
+             const float* __restrict__ coords, int index_base) {
+    __shared__ uint64_t skey[RS_TILE];
+    __shared__ int32_t sval[RS_TILE];
+    __shared__ uint32_t wcount[RS_WAVES][RS_BUCKETS];
+    __shared__ uint32_t dstart[RS_BUCKETS];
+    __shared__ uint32_t gstart[RS_BUCKETS];
+
+    const int lane = lane_id();
+    const int w = threadIdx.x / OCT_WAVE;
+    for (int d = threadIdx.x; d < RS_BUCKETS; d += RS_THREADS) {
+        for (int k = 0; k < RS_WAVES; ++k) wcount[k][d] = 0;
+        gstart[d] = hist_off[(int64_t)d * ntiles + blockIdx.x];
+    }
+    __syncthreads();
+
+    const int64_t tile_base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t wave_base = tile_base + (int64_t)w * (RS_ITEMS * OCT_WAVE);
+    uint64_t key[RS_ITEMS];
+    int32_t val[RS_ITEMS];
+    uint32_t lpos[RS_ITEMS];  // position within (wave, digit)
+    for (int t = 0; t < RS_ITEMS; ++t) {
+        const int64_t k = wave_base + (int64_t)t * OCT_WAVE + lane;
+        const bool valid = k < n;
+        key[t] = valid ? kin[k] : 0ull;
+        val[t] = valid ? vin[k] : 0;
+        const uint32_t d = (uint32_t)(key[t] >> shift) & (RS_BUCKETS - 1);
+        uint64_t peers = ballot(valid);
+        for (int b = 0; b < RS_BITS; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)mbcnt(peers);
+        const uint32_t old = wcount[w][d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) wcount[w][d] = old + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        lpos[t] = old + rank;
+    }
+    __syncthreads();
+    // digit starts inside the tile and per-wave offsets inside each digit
+    if (threadIdx.x < RS_BUCKETS) {
+        const int d = threadIdx.x;
+        uint32_t run = 0;
+        for (int k = 0; k < RS_WAVES; ++k) {
+            const uint32_t c = wcount[k][d];
+            wcount[k][d] = run;
+            run += c;
+        }
+        dstart[d] = run;  // temporarily the digit total
+    }
+    __syncthreads();
+    if (threadIdx.x < OCT_WAVE) {
+        // exclusive scan of the 256 digit totals by one wave (4 per lane)
+        uint32_t t4[4], s = 0;
+        for (int j = 0; j < 4; ++j) { t4[j] = dstart[4 * lane + j]; s += t4[j]; }
+        uint32_t incl = (uint32_t)wave_incl_scan((int)s);
+        uint32_t pre = incl - s;
+        for (int j = 0; j < 4; ++j) { dstart[4 * lane + j] = pre; pre += t4[j]; }
+    }
+    __syncthreads();
+    for (int t = 0; t < RS_ITEMS; ++t) {
+        const int64_t k = wave_base + (int64_t)t * OCT_WAVE + lane;
+        if (k < n) {
+            const uint32_t d = (uint32_t)(key[t] >> shift) & (RS_BUCKETS - 1);
+            const uint32_t p = dstart[d] + wcount[w][d] + lpos[t];
+            skey[p] = key[t];
+            sval[p] = val[t];
+        }
+    }
+    __syncthreads();
+    const int64_t tile_n = n - tile_base < RS_TILE ? n - tile_base : RS_TILE;
+    for (int p = threadIdx.x; p < tile_n; p += RS_THREADS) {
+        const uint64_t kk = skey[p];
+        const int32_t vv = sval[p];
+        const uint32_t d = (uint32_t)(kk >> shift) & (RS_BUCKETS - 1);
+        const int64_t g = (int64_t)gstart[d] + (p - (int64_t)dstart[d]);
+        kout[g] = kk;
+        if (!last_pass) {
+            vout[g] = vv;
+        } else {
+            perm_out[g] = vv + index_base;
+            const float* c = coords + 3 * (int64_t)vv;
+            scoord[g] = make_float4(c[0], c[1], c[2], __int_as_float(vv));
+        }
+    }
+}
+
+// ============================================================ topology
+
+constexpr int TOPO_THREADS = 256;
+constexpr int TOPO_MAX_T = 256;  // largest leaf size the window supports
+
+// Per sorted position: m (shared levels with the previous key), lambda, and
+// whether nodes start here; per-block counts of nodes per level (rows
+// 0..21) and of leaves (row 22).
+__global__ void __launch_bounds__(TOPO_THREADS)
+k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, uint16_t* __restrict__ topo,
+             uint32_t* __restrict__ hist, int nblk) {
+    __shared__ int8_t m[TOPO_THREADS + 2 * TOPO_MAX_T + 1];
+    __shared__ uint32_t wc[TOPO_THREADS / OCT_WAVE][OCT_NLEVELS + 1];
+    const int64_t base = (int64_t)blockIdx.x * TOPO_THREADS;
+    const int64_t lo = base - T;  // m[i] holds m_{lo + i}
+    const int span = TOPO_THREADS + 2 * T + 1;
+    for (int i = threadIdx.x; i < span; i += TOPO_THREADS) {
+        const int64_t j = lo + i;
+        int8_t v = -1;
+        if (j > 0 && j < n) v = (int8_t)shared_levels(sk[j - 1], sk[j]);
+        else if (j >= n) v = -2;  // past the end: never read as a real value
+        m[i] = v;
+    }
+    __syncthreads();
+    const int64_t k = base + threadIdx.x;
+    const int li = threadIdx.x + T;
+    int mk = -1, lam = -1;
+    bool emit = false;
+    if (k < n) {
+        mk = m[li];  // -1 at k == 0
+        // lambda: first level whose range starting at k holds <= T points
+        int mw = 127;
+        if (k + T >= n) {
+            mw = -1;
+        } else {
+            for (int j = 1; j <= T; ++j) mw = min(mw, (int)m[li + j]);
+        }
+        lam = min(OCT_MAX_LEVEL, max(mk, mw) + 1);
+        if (k == 0) {
+            emit = true;
+        } else if (lam > mk) {
+            // parent range (level mk, contains k) must hold more than T points
+            const int P = mk;
+            int a = T + 1, b = T + 1;
+            for (int j = 1; j <= T; ++j) {
+                if (k - j < 0) break;
+                if ((int)m[li - j] < P || k - j == 0) { a = j; break; }
+            }
+            for (int j = 1; j <= T; ++j) {
+                if (k + j >= n) { b = j; break; }
+                if ((int)m[li + j] < P) { b = j; break; }
+            }
+            emit = (a + b) > T;
+        }
+    }
+    // m is stored +1 so k == 0 (m = -1) fits the low 5 bits
+    if (k < n) topo[k] = (uint16_t)((mk + 1) | (lam << 5) | (emit ? (1 << 10) : 0));
+    const int w = threadIdx.x / OCT_WAVE;
+    for (int L = 0; L < OCT_NLEVELS; ++L) {
+        const uint64_t b = ballot(emit && mk < L && L <= lam);
+        if (lane_id() == 0) wc[w][L] = (uint32_t)__popcll(b);
+    }
+    const uint64_t bl = ballot(emit);
+    if (lane_id() == 0) wc[w][OCT_NLEVELS] = (uint32_t)__popcll(bl);
+    __syncthreads();
+    if (threadIdx.x <= OCT_NLEVELS) {
+        uint32_t s = 0;
+        for (int q = 0; q < TOPO_THREADS / OCT_WAVE; ++q) s += wc[q][threadIdx.x];
+        hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = s;
+    }
+}
+
+// first j in [lo, hi) with pred(j) true, pred monotone false..true; hi if none
+template <typename P>
+__device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
+    while (lo < hi) {
+        int64_t mid = lo + ((hi - lo) >> 1);
+        if (pred(mid)) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Emits every node starting at each sorted position: reference-layout
+// arrays (node_first/last/children with the index base, bounds), packed
+// records, parent links, and leaf bounds.  Nodes covering all n points get
+// the global bounds; the rest of the internal bounds come from k_refit.
+__global__ void __launch_bounds__(TOPO_THREADS)
+k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, int64_t n, int T,
+            const uint16_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
+            BuildParams* __restrict__ prm, int index_base, NodeRec* __restrict__ nodes,
+            int32_t* __restrict__ nfirst, int32_t* __restrict__ nlast, int32_t* __restrict__ nchild,
+            float* __restrict__ bmin, float* __restrict__ bmax, int32_t* __restrict__ parent,
+            int32_t* __restrict__ visit, int32_t* __restrict__ leaf_ids) {
+    __shared__ uint32_t wc[TOPO_THREADS / OCT_WAVE][OCT_NLEVELS + 1];
+    const int64_t k = (int64_t)blockIdx.x * TOPO_THREADS + threadIdx.x;
+    int mk = -1, lam = -1;
+    bool emit = false;
+    if (k < n) {
+        const uint16_t t = topo[k];
+        mk = (int)(t & 31) - 1;
+        lam = (t >> 5) & 31;
+        emit = (t >> 10) & 1;
+    }
+    const int w = threadIdx.x / OCT_WAVE;
+    const int lane = lane_id();
+    uint32_t rank_in_wave[OCT_NLEVELS + 1];
+    for (int L = 0; L <= OCT_NLEVELS; ++L) {
+        const bool ind = L < OCT_NLEVELS ? (emit && mk < L && L <= lam) : emit;
+        const uint64_t b = ballot(ind);
+        rank_in_wave[L] = (uint32_t)mbcnt(b);
+        if (lane == 0) wc[w][L] = (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // total node count M = start of the leaf row in the scanned table
+        // (written by every block with the same value; also pick up leaves)
+        if (blockIdx.x == 0) {
+            const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
+            prm->n_nodes = M;
+        }
+    }
+    if (!emit) return;
+    auto wave_off = [&](int L) {
+        uint32_t s = 0;
+        for (int q = 0; q < w; ++q) s += wc[q][L];
+        return s;
+    };
+    const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
+    auto node_id = [&](int L) -> int32_t {
+        return (int32_t)(hoff[(int64_t)L * nblk + blockIdx.x] + wave_off(L) + rank_in_wave[L]);
+    };
+    const uint64_t key = sk[k];
+    const int Lfirst = mk + 1;  // 0 at k == 0
+    for (int L = Lfirst; L <= lam; ++L) {
+        const int32_t id = node_id(L);
+        visit[id] = 0;
+        if (L == Lfirst && k == 0 && L == 0) parent[id] = -1;
+        if (L > Lfirst) parent[id] = node_id(L - 1);
+        int64_t E;  // exclusive end of the range
+        if (L == lam) {
+            // leaf: the range ends at the first j > k with m_j < lam
+            int64_t e = -1;
+            for (int j = 1; j <= T && k + j <= n; ++j) {
+                if (k + j == n) { e = n; break; }
+                const uint16_t tj = topo[k + j];
+                if ((int)(tj & 31) - 1 < lam) { e = k + j; break; }
+            }
+            if (e < 0) {
+                // level-21 leaf holding more than T equal keys
+                e = lower_bound_pred(k + T + 1, n, [&](int64_t j) { return sk[j] != key; });
+            }
+            E = e;
+        } else {
+            const int sh = 63 - 3 * L;  // prefix_L(key) = key >> sh (sh = 63 at L = 0)
+            const uint64_t pre = sh >= 64 ? 0ull : (sh == 63 ? 0ull : key >> sh);
+            E = lower_bound_pred(k + 1, n, [&](int64_t j) {
+                const uint64_t pj = (sh >= 63) ? 0ull : (sk[j] >> sh);
+                return pj != pre;
+            });
+        }
+        const int32_t cnt = (int32_t)(E - k);
+        nfirst[id] = (int32_t)k + index_base;
+        nlast[id] = (int32_t)(E - 1) + index_base;
+        NodeRec rec;
+        if (L == lam) {
+            for (int o = 0; o < 8; ++o) nchild[8 * (int64_t)id + o] = -1;
+            uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+            uint32_t hi3[3] = {0u, 0u, 0u};
+            for (int64_t j = k; j < E; ++j) {
+                const float4 p = scoord[j];
+                const float v[3] = {p.x, p.y, p.z};
+                for (int d = 0; d < 3; ++d) {
+                    const uint32_t o = f2o(v[d]);
+                    lo3[d] = o < lo3[d] ? o : lo3[d];
+                    hi3[d] = o > hi3[d] ? o : hi3[d];
+                }
+            }
+            rec.lo = make_float4(o2f(lo3[0]), o2f(lo3[1]), o2f(lo3[2]), __int_as_float((int)k));
+            rec.hi = make_float4(o2f(hi3[0]), o2f(hi3[1]), o2f(hi3[2]), __int_as_float(cnt));
+            const int32_t lr = (int32_t)(hoff[(int64_t)OCT_NLEVELS * nblk + blockIdx.x] - (uint32_t)M +
+                                         wave_off(OCT_NLEVELS) + rank_in_wave[OCT_NLEVELS]);
+            leaf_ids[lr] = id;
+        } else {
+            // children: one per non-empty octant of the level-L split, ids
+            // consecutive from the level-(L+1) node starting here
+            const int32_t c0 = node_id(L + 1);
+            const int sh = level_shift(L);
+            const int o0 = (int)((key >> sh) & 7ull);
+            int nc = 0;
+            int64_t pos = k;
+            for (int o = 0; o < 8; ++o) {
+                if (o < o0) { nchild[8 * (int64_t)id + o] = -1; continue; }
+                int64_t s0 = (o == o0) ? k
+                    : lower_bound_pred(pos, E, [&](int64_t j) { return (int)((sk[j] >> sh) & 7ull) >= o; });
+                bool nonempty = s0 < E && (int)((sk[s0] >> sh) & 7ull) == o;
+                if (nonempty) {
+                    const int32_t cid = c0 + nc;
+                    nchild[8 * (int64_t)id + o] = cid + index_base;
+                    if (o != o0) parent[cid] = id;  // the first child is linked by its own thread
+                    ++nc;
+                    pos = s0;
+                } else {
+                    nchild[8 * (int64_t)id + o] = -1;
+                }
+            }
+            rec.lo = make_float4(0.f, 0.f, 0.f, __int_as_float(c0));
+            rec.hi = make_float4(0.f, 0.f, 0.f, __int_as_float(-nc));
+            if ((int64_t)cnt == n) {
+                rec.lo.x = o2f(prm->omin[0]); rec.lo.y = o2f(prm->omin[1]); rec.lo.z = o2f(prm->omin[2]);
+                rec.hi.x = o2f(prm->omax[0]); rec.hi.y = o2f(prm->omax[1]); rec.hi.z = o2f(prm->omax[2]);
+                if (nc > 1) prm->branch_root = id;
+            }
+        }
+        if (L == lam && (int64_t)cnt == n) prm->branch_root = id;  // root or chain ends in a leaf
+        nodes[id] = rec;
+        if (L == lam || (int64_t)cnt == n) {
+            bmin[3 * (int64_t)id + 0] = rec.lo.x; bmin[3 * (int64_t)id + 1] = rec.lo.y; bmin[3 * (int64_t)id + 2] = rec.lo.z;
+            bmax[3 * (int64_t)id + 0] = rec.hi.x; bmax[3 * (int64_t)id + 1] = rec.hi.y; bmax[3 * (int64_t)id + 2] = rec.hi.z;
+        }
+    }
+}
+
+// Links each level-(L) node that starts a new parent range (L == m_k + 1,
+// k > 0) to its parent: that parent started earlier and wrote parent[] for
+// all its children except the first, so nothing is left to do here; the
+// first child of a node shares its start position and is linked above.
+//
+// Bottom-up bounds: one thread per leaf climbs while it is the last child to
+// finish; the union of the children's bounds is exact and order-free.
+__global__ void __launch_bounds__(256)
+k_refit(const int32_t* __restrict__ leaf_ids, int32_t n_leaves_max, const BuildParams* __restrict__ prm,
+        int64_t n, NodeRec* __restrict__ nodes, const int32_t* __restrict__ parent,
+        int32_t* __restrict__ visit, float* __restrict__ bmin, float* __restrict__ bmax,
+        const int32_t* __restrict__ nfirst, const int32_t* __restrict__ nlast) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t n_leaves = prm->n_leaves;
+    if (r >= n_leaves) return;
+    int32_t id = leaf_ids[r];
+    int32_t p = parent[id];
+    while (p >= 0) {
+        if ((int64_t)(nlast[p] - nfirst[p]) + 1 == n) break;  // holds the global bounds already
+        const int nc = -rec_b(nodes[p]);
+        __threadfence();
+        const int old = atomicAdd(&visit[p], 1);
+        if (old != nc - 1) break;
+        __threadfence();
+        const int c0 = rec_a(nodes[p]);
+        uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+        uint32_t hi3[3] = {0u, 0u, 0u};
+        for (int c = 0; c < nc; ++c) {
+            const float* cl = &nodes[c0 + c].lo.x;
+            const float* ch = &nodes[c0 + c].hi.x;
+            for (int d = 0; d < 3; ++d) {
+                const uint32_t a = f2o(load_coherent(cl + d));
+                const uint32_t b = f2o(load_coherent(ch + d));
+                lo3[d] = a < lo3[d] ? a : lo3[d];
+                hi3[d] = b > hi3[d] ? b : hi3[d];
+            }
+        }
+        float* pl = &nodes[p].lo.x;
+        float* ph = &nodes[p].hi.x;
+        for (int d = 0; d < 3; ++d) {
+            const float a = o2f(lo3[d]), b = o2f(hi3[d]);
+            pl[d] = a;
+            ph[d] = b;
+            bmin[3 * (int64_t)p + d] = a;
+            bmax[3 * (int64_t)p + d] = b;
+        }
+        id = p;
+        p = parent[id];
+    }
+}
+
+__global__ void k_set_leaf_count(BuildParams* prm, const uint32_t* total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) prm->n_leaves = (int32_t)(*total) - prm->n_nodes;
+}
+
+__global__ void k_zero_u32(uint32_t* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ============================================================== driver
+
+
+
+int bounds_blocks(int64_t n) {
+    int64_t b = (n + BND_THREADS * 8 - 1) / (BND_THREADS * 8);
+    return (int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
+int64_t sort_tiles(int64_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+int64_t topo_blocks(int64_t n) { return (n + TOPO_THREADS - 1) / TOPO_THREADS; }
+int64_t hist_words(int64_t n) {
+    int64_t a = (int64_t)RS_BUCKETS * sort_tiles(n);
+    int64_t b = (int64_t)(OCT_NLEVELS + 1) * topo_blocks(n);
+    return a > b ? a : b;
+}
+// Nodes: at most n leaves, and at most n / (T + 1) internal nodes per level.
+int64_t max_nodes(int64_t n, int T) { return n + 21 * (n / (T + 1)) + 22; }
+
+// Stage 1: bounds reduction (the caller reads prm back to learn key_bits).
+hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, const BuildBufs& b,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_bounds, dim3(bounds_blocks(n)), dim3(BND_THREADS), 0, s, coords, n, radius,
+                       b.partials, b.prm);
+    return hipGetLastError();
+}
+
+// Stage 2: keys, sort over the live bits, topology, refit.
+hipError_t launch_build_stage(const float* coords, int64_t n, int key_bits, int T, int index_base,
+                              const BuildBufs& b, hipStream_t s) {
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_keys, dim3(g), dim3(256), 0, s, coords, n, b.prm, b.keys[0], b.vals[0]);
+    int passes = (key_bits + RS_BITS - 1) / RS_BITS;
+    if (passes < 1) passes = 1;
+    const int ntiles = (int)sort_tiles(n);
+    int cur = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = p * RS_BITS;
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, s, b.keys[cur], n, shift, b.hist, ntiles);
+        hipError_t e = exclusive_scan(b.hist, b.hist, (int64_t)RS_BUCKETS * ntiles, b.scan_tmp, nullptr, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, s, b.keys[cur], b.vals[cur],
+                           b.keys[cur ^ 1], b.vals[cur ^ 1], n, shift, b.hist, ntiles, (int)(p == passes - 1),
+                           b.perm, b.scoord, coords, index_base);
+        cur ^= 1;
+    }
+    const uint64_t* sk = b.keys[cur];
+    const int nblk = (int)topo_blocks(n);
+    hipLaunchKernelGGL(k_topo_count, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, n, T, b.topo, b.hist, nblk);
+    hipError_t e = exclusive_scan(b.hist, b.hist, (int64_t)(OCT_NLEVELS + 1) * nblk, b.scan_tmp, b.total, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, b.scoord, n, T, b.topo, b.hist,
+                       nblk, b.prm, index_base, b.nodes, b.nfirst, b.nlast, b.nchild, b.bmin, b.bmax,
+                       b.parent, b.visit, b.leaf_ids);
+    hipLaunchKernelGGL(k_set_leaf_count, dim3(1), dim3(64), 0, s, b.prm, b.total);
+    hipLaunchKernelGGL(k_refit, dim3(g), dim3(256), 0, s, b.leaf_ids, (int32_t)n, b.prm, n, b.nodes,
+                       b.parent, b.visit, b.bmin, b.bmax, b.nfirst, b.nlast);
+    return hipGetLastError();
+}
+
+const uint64_t* sorted_keys(const BuildBufs& b, int key_bits) {
+    int passes = (key_bits + RS_BITS - 1) / RS_BITS;
+    if (passes < 1) passes = 1;
+    return b.keys[passes & 1];
+}
+
+}  // namespace oct

@@ -1,0 +1,586 @@
+// C-ABI of the octohip library: context, point sets, active pairs, run!,
+// build_edges, device view and the build_edges_diff pullback.  Host code that
+// sizes buffers, launches the kernels of build.hip / edges.hip / backward.hip
+// and reads back only what a host caller needs.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "octohip.h"
+#include "octohip_device.h"
+
+namespace oct {
+
+// Grow-only device buffer (reused across calls, like ensure_capacity!,
+// src/util.jl:12-17).
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max(bytes, cap + cap / 2);
+        if (want < 256) want = 256;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) { p = nullptr; return e; }
+        cap = want;
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+struct SetState {
+    const float* coords = nullptr;
+    int64_t n = 0;
+    bool dirty = true;
+    bool built = false;
+    int key_bits = 0;
+    BuildParams hp{};  // host copy after the last build
+    DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
+        nfirst, nlast, nchild, parent, visit, leaf_ids, bmin, bmax;
+    BuildBufs bufs() const {
+        BuildBufs b;
+        b.prm = prm.as<BuildParams>();
+        b.partials = partials.as<uint32_t>();
+        b.keys[0] = keys0.as<uint64_t>();
+        b.keys[1] = keys1.as<uint64_t>();
+        b.vals[0] = vals0.as<int32_t>();
+        b.vals[1] = vals1.as<int32_t>();
+        b.hist = hist.as<uint32_t>();
+        b.scan_tmp = scan_tmp.as<uint32_t>();
+        b.total = total.as<uint32_t>();
+        b.topo = topo.as<uint16_t>();
+        b.perm = perm.as<int32_t>();
+        b.scoord = scoord.as<float4>();
+        b.nodes = nodes.as<NodeRec>();
+        b.nfirst = nfirst.as<int32_t>();
+        b.nlast = nlast.as<int32_t>();
+        b.nchild = nchild.as<int32_t>();
+        b.parent = parent.as<int32_t>();
+        b.visit = visit.as<int32_t>();
+        b.leaf_ids = leaf_ids.as<int32_t>();
+        b.bmin = bmin.as<float>();
+        b.bmax = bmax.as<float>();
+        return b;
+    }
+    void release_all() {
+        DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
+                         &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &visit, &leaf_ids, &bmin, &bmax};
+        for (DevBuf* d : all) d->release();
+    }
+};
+
+struct PairState {
+    int q = 0, t = 0;
+    int cap = 64;  // candidate leaves per query leaf
+    int64_t n_edges = 0;
+    bool materialized = false;
+    DevBuf senders, receivers, rdisp, rdist, lists, lcount, qcount, qoff, scan_tmp, flags;
+    void release_all() {
+        DevBuf* all[] = {&senders, &receivers, &rdisp, &rdist, &lists, &lcount, &qcount, &qoff, &scan_tmp, &flags};
+        for (DevBuf* d : all) d->release();
+    }
+};
+
+}  // namespace oct
+
+using namespace oct;
+
+struct oct_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int index_base = 0;
+    float radius = 0.f;
+    int leaf_size = 32;
+    bool refit_mode = false;
+    std::vector<SetState> sets;
+    std::vector<PairState> pairs;
+    std::string err;
+    void* pinned = nullptr;  // small host staging area for read-backs
+    // optional per-phase kernel timing with HIP events on the context stream
+    bool profiling = false;
+    struct Mark { int phase; hipEvent_t a, b; };
+    std::vector<Mark> marks;
+    std::vector<hipEvent_t> spare;
+    double prof_ms[OCT_PROF_NPHASES] = {0};
+    int64_t prof_calls[OCT_PROF_NPHASES] = {0};
+};
+
+namespace {
+
+hipEvent_t take_event(oct_ctx* c) {
+    if (!c->spare.empty()) {
+        hipEvent_t e = c->spare.back();
+        c->spare.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Brackets the launches between begin/end with events when profiling is on.
+struct Phase {
+    oct_ctx* c;
+    int ph;
+    hipEvent_t a = nullptr;
+    Phase(oct_ctx* ctx, int phase) : c(ctx), ph(phase) {
+        if (c->profiling) {
+            a = take_event(c);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Phase() {
+        if (c->profiling && a) {
+            hipEvent_t b = take_event(c);
+            (void)hipEventRecord(b, c->stream);
+            c->marks.push_back({ph, a, b});
+        }
+    }
+};
+
+int fail(oct_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                              \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail((ctx), OCT_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int set_index(oct_ctx* c, int id) { return id - c->index_base; }
+
+bool valid_set(oct_ctx* c, int id) {
+    int k = set_index(c, id);
+    return k >= 0 && k < (int)c->sets.size();
+}
+
+TreeView view_of(const SetState& s) {
+    TreeView v;
+    v.nodes = s.nodes.as<NodeRec>();
+    v.leaf_ids = s.leaf_ids.as<int32_t>();
+    v.scoord = s.scoord.as<float4>();
+    v.params = s.prm.as<BuildParams>();
+    v.n_leaves = s.hp.n_leaves;
+    v.n_nodes = s.hp.n_nodes;
+    v.branch_root = s.hp.branch_root;
+    v.n = s.n;
+    return v;
+}
+
+int mark_all_dirty(oct_ctx* c) {
+    for (auto& s : c->sets) s.dirty = true;
+    return OCT_OK;
+}
+
+int read_params(oct_ctx* c, SetState& s) {
+    HIP_TRY(c, hipMemcpyAsync(c->pinned, s.prm.p, sizeof(BuildParams), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    memcpy(&s.hp, c->pinned, sizeof(BuildParams));
+    return OCT_OK;
+}
+
+int build_set(oct_ctx* c, SetState& s) {
+    const int64_t n = s.n;
+    s.built = false;
+    if (n == 0) {
+        memset(&s.hp, 0, sizeof(s.hp));
+        s.built = true;
+        s.dirty = false;
+        return OCT_OK;
+    }
+    if (n >= (int64_t)INT32_MAX) return fail(c, OCT_ERR_CAPACITY, "point sets are limited to 2^31 - 1 points");
+    const int T = c->leaf_size;
+    const int64_t M = max_nodes(n, T);
+    const bool fresh_prm = s.prm.p == nullptr;
+    HIP_TRY(c, s.prm.ensure(sizeof(BuildParams)));
+    if (fresh_prm)
+        HIP_TRY(c, launch_fill_u32(s.prm.as<uint32_t>(), sizeof(BuildParams) / 4, 0u, c->stream));
+    HIP_TRY(c, s.partials.ensure(6 * sizeof(uint32_t) * (size_t)bounds_blocks(n)));
+    HIP_TRY(c, s.keys0.ensure(8 * (size_t)n));
+    HIP_TRY(c, s.keys1.ensure(8 * (size_t)n));
+    HIP_TRY(c, s.vals0.ensure(4 * (size_t)n));
+    HIP_TRY(c, s.vals1.ensure(4 * (size_t)n));
+    HIP_TRY(c, s.hist.ensure(4 * (size_t)hist_words(n)));
+    HIP_TRY(c, s.scan_tmp.ensure(4 * (size_t)scan_tmp_words(hist_words(n))));
+    HIP_TRY(c, s.total.ensure(16));
+    HIP_TRY(c, s.topo.ensure(2 * (size_t)n));
+    HIP_TRY(c, s.perm.ensure(4 * (size_t)n));
+    HIP_TRY(c, s.scoord.ensure(16 * (size_t)n));
+    HIP_TRY(c, s.nodes.ensure(sizeof(NodeRec) * (size_t)M));
+    HIP_TRY(c, s.nfirst.ensure(4 * (size_t)M));
+    HIP_TRY(c, s.nlast.ensure(4 * (size_t)M));
+    HIP_TRY(c, s.nchild.ensure(32 * (size_t)M));
+    HIP_TRY(c, s.parent.ensure(4 * (size_t)M));
+    HIP_TRY(c, s.visit.ensure(4 * (size_t)M));
+    HIP_TRY(c, s.leaf_ids.ensure(4 * (size_t)n));
+    HIP_TRY(c, s.bmin.ensure(12 * (size_t)M));
+    HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
+    const BuildBufs b = s.bufs();
+    {
+        Phase ph(c, OCT_PROF_BOUNDS);
+        HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, b, c->stream));
+    }
+    int rc = read_params(c, s);
+    if (rc != OCT_OK) return rc;
+    s.key_bits = s.hp.key_bits;
+    {
+        Phase ph(c, OCT_PROF_BUILD);
+        HIP_TRY(c, launch_build_stage(s.coords, n, s.key_bits, T, c->index_base, b, c->stream));
+    }
+    rc = read_params(c, s);
+    if (rc != OCT_OK) return rc;
+    s.built = true;
+    s.dirty = false;
+    return OCT_OK;
+}
+
+PairState* find_pair(oct_ctx* c, int q, int t) {
+    for (auto& p : c->pairs)
+        if (p.q == q && p.t == t) return &p;
+    return nullptr;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+
+extern "C" {
+
+int oct_create(oct_ctx** out, int device, int ndims, int index_base) {
+    if (!out) return OCT_ERR_ARGUMENT;
+    *out = nullptr;
+    if (ndims != 3) return OCT_ERR_ARGUMENT;
+    if (index_base != 0 && index_base != 1) return OCT_ERR_ARGUMENT;
+    oct_ctx* c = new oct_ctx();
+    c->device = device;
+    c->index_base = index_base;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
+    c->own_stream = true;
+    if (hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return OCT_ERR_DEVICE;
+    }
+    *out = c;
+    return OCT_OK;
+}
+
+int oct_destroy(oct_ctx* c) {
+    if (!c) return OCT_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& s : c->sets) s.release_all();
+    for (auto& p : c->pairs) p.release_all();
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    for (auto& m : c->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
+    for (auto e : c->spare) (void)hipEventDestroy(e);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return OCT_OK;
+}
+
+const char* oct_last_error(const oct_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int oct_set_stream(oct_ctx* c, void* stream) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    if (c->own_stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        c->own_stream = false;
+    }
+    c->stream = (hipStream_t)stream;
+    return OCT_OK;
+}
+
+int oct_set_search_radius(oct_ctx* c, float r) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (!(r > 0.f)) return fail(c, OCT_ERR_ARGUMENT, "search radius must be positive");
+    c->radius = r;
+    return mark_all_dirty(c);
+}
+
+int oct_set_refit_mode(oct_ctx* c, int on) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    c->refit_mode = on != 0;
+    return mark_all_dirty(c);
+}
+
+int oct_set_leaf_size(oct_ctx* c, int leaf_size) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (leaf_size < 1 || leaf_size > 256) return fail(c, OCT_ERR_ARGUMENT, "leaf size must be in 1..256");
+    c->leaf_size = leaf_size;
+    return mark_all_dirty(c);
+}
+
+int oct_add_point_set(oct_ctx* c, const float* coords, int64_t n, int* set_id) {
+    if (!c || !set_id) return OCT_ERR_ARGUMENT;
+    if (n < 0) return fail(c, OCT_ERR_DIMENSION, "point count must be >= 0");
+    if (n > 0 && !coords) return fail(c, OCT_ERR_ARGUMENT, "null coordinates");
+    c->sets.emplace_back();
+    c->sets.back().coords = coords;
+    c->sets.back().n = n;
+    *set_id = (int)c->sets.size() - 1 + c->index_base;
+    return OCT_OK;
+}
+
+int oct_update_point_set(oct_ctx* c, int id, const float* coords, int64_t n) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (!valid_set(c, id)) return fail(c, OCT_ERR_ARGUMENT, "unknown point set");
+    SetState& s = c->sets[set_index(c, id)];
+    if (n != s.n)
+        return fail(c, OCT_ERR_DIMENSION, "update_point_set! expects same size; use resize_point_set! otherwise");
+    s.coords = coords;
+    s.dirty = true;
+    return OCT_OK;
+}
+
+int oct_resize_point_set(oct_ctx* c, int id, const float* coords, int64_t n) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (!valid_set(c, id)) return fail(c, OCT_ERR_ARGUMENT, "unknown point set");
+    if (n < 0) return fail(c, OCT_ERR_DIMENSION, "point count must be >= 0");
+    SetState& s = c->sets[set_index(c, id)];
+    s.coords = coords;
+    s.n = n;
+    s.dirty = true;
+    return OCT_OK;
+}
+
+int oct_set_active_search(oct_ctx* c, int q, int t) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (!valid_set(c, q) || !valid_set(c, t)) return fail(c, OCT_ERR_ARGUMENT, "unknown point set");
+    if (find_pair(c, q, t)) return OCT_OK;  // idempotent (src/api.jl:152)
+    c->pairs.emplace_back();
+    c->pairs.back().q = q;
+    c->pairs.back().t = t;
+    return OCT_OK;
+}
+
+int oct_run(oct_ctx* c) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (!(c->radius > 0.f)) return fail(c, OCT_ERR_ARGUMENT, "set_search_radius! before run!");
+    if (c->sets.empty()) return fail(c, OCT_ERR_ARGUMENT, "add at least one point set before run!");
+    HIP_TRY(c, hipSetDevice(c->device));
+    for (auto& s : c->sets) {
+        if (!s.dirty) continue;
+        int rc = build_set(c, s);
+        if (rc != OCT_OK) return rc;
+    }
+    for (auto& p : c->pairs) p.materialized = false;
+    return OCT_OK;
+}
+
+int oct_get_tree(oct_ctx* c, int id, oct_tree* out) {
+    if (!c || !out) return OCT_ERR_ARGUMENT;
+    if (!valid_set(c, id)) return fail(c, OCT_ERR_ARGUMENT, "unknown point set");
+    SetState& s = c->sets[set_index(c, id)];
+    if (!s.built || s.dirty) return fail(c, OCT_ERR_STATE, "run! before reading the tree");
+    memset(out, 0, sizeof(*out));
+    out->n_points = s.n;
+    if (s.n == 0) return OCT_OK;
+    out->node_bounds_min = s.bmin.as<float>();
+    out->node_bounds_max = s.bmax.as<float>();
+    out->node_first = s.nfirst.as<int32_t>();
+    out->node_last = s.nlast.as<int32_t>();
+    out->node_children = s.nchild.as<int32_t>();
+    out->permutation = s.perm.as<int32_t>();
+    out->n_nodes = s.hp.n_nodes;
+    out->n_leaves = s.hp.n_leaves;
+    for (int d = 0; d < 3; ++d) out->origin[d] = s.hp.origin[d];
+    return OCT_OK;
+}
+
+int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* out) {
+    if (!c || !out) return OCT_ERR_ARGUMENT;
+    PairState* pp = find_pair(c, q, t);
+    if (!pp) return fail(c, OCT_ERR_STATE, "build_edges!: pair not registered; call set_active_search! first");
+    if (mode != OCT_EDGES_EXACT && mode != OCT_EDGES_LEARNED) return fail(c, OCT_ERR_ARGUMENT, "unknown edge mode");
+    if (order != OCT_ORDER_LEAF && order != OCT_ORDER_RECEIVER) return fail(c, OCT_ERR_ARGUMENT, "unknown edge order");
+    if (mode == OCT_EDGES_LEARNED)
+        return fail(c, OCT_ERR_ARGUMENT, "the learned single pass needs a trained count predictor");
+    PairState& p = *pp;
+    SetState& qs = c->sets[set_index(c, q)];
+    SetState& ts = c->sets[set_index(c, t)];
+    if (!qs.built || qs.dirty || !ts.built || ts.dirty) return fail(c, OCT_ERR_STATE, "run! before build_edges");
+    HIP_TRY(c, hipSetDevice(c->device));
+    memset(out, 0, sizeof(*out));
+    p.n_edges = 0;
+    if (qs.n == 0 || ts.n == 0) {
+        p.materialized = true;
+        return OCT_OK;
+    }
+    const int self_pair = q == t;
+    const float r = c->radius;
+    const float r_sq = r * r;
+    const float inv_r = 1.0f / r;
+    const TreeView qv = view_of(qs), tv = view_of(ts);
+    const int64_t nql = qs.hp.n_leaves;
+    HIP_TRY(c, p.flags.ensure(16));
+    HIP_TRY(c, p.lcount.ensure(4 * (size_t)nql));
+    HIP_TRY(c, p.qcount.ensure(4 * (size_t)qs.n));
+    HIP_TRY(c, p.qoff.ensure(4 * (size_t)qs.n));
+    HIP_TRY(c, p.scan_tmp.ensure(4 * (size_t)scan_tmp_words(qs.n)));
+    const int by_receiver = order == OCT_ORDER_RECEIVER;
+    uint32_t* flags = p.flags.as<uint32_t>();
+    uint32_t host_flags[2] = {0, 0};
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        HIP_TRY(c, p.lists.ensure(sizeof(int2) * (size_t)nql * (size_t)p.cap));
+        HIP_TRY(c, launch_fill_u32(flags, 4, 0u, c->stream));
+        {
+            Phase ph(c, OCT_PROF_LEAF_LISTS);
+            HIP_TRY(c, launch_leaf_lists(qv, tv, r_sq, p.cap, p.lists.as<int2>(), p.lcount.as<int32_t>(), flags,
+                                         c->stream));
+        }
+        {
+            Phase ph(c, OCT_PROF_COUNT);
+            HIP_TRY(c, launch_edges(0, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r,
+                                    self_pair, by_receiver, p.qcount.as<uint32_t>(), nullptr, EdgeOut{},
+                                    c->index_base, c->stream));
+        }
+        {
+            Phase ph(c, OCT_PROF_SCAN);
+            HIP_TRY(c, exclusive_scan(p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), qs.n,
+                                      p.scan_tmp.as<uint32_t>(), flags + 1, c->stream));
+        }
+        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        memcpy(host_flags, c->pinned, 8);
+        if (host_flags[0] == 0) break;
+        p.cap = (int)std::min<uint32_t>(host_flags[0] + 16u, (uint32_t)INT32_MAX / 8u);
+    }
+    if (host_flags[0] != 0) return fail(c, OCT_ERR_DEVICE, "candidate list capacity did not settle");
+    const int64_t E = host_flags[1];
+    if (E >= (int64_t)INT32_MAX) return fail(c, OCT_ERR_CAPACITY, "edge count exceeds Int32 (the reference's limit)");
+    HIP_TRY(c, p.senders.ensure(4 * (size_t)E));
+    HIP_TRY(c, p.receivers.ensure(4 * (size_t)E));
+    HIP_TRY(c, p.rdisp.ensure(12 * (size_t)E));
+    HIP_TRY(c, p.rdist.ensure(4 * (size_t)E));
+    EdgeOut eo{p.senders.as<int32_t>(), p.receivers.as<int32_t>(), p.rdisp.as<float>(), p.rdist.as<float>()};
+    if (E > 0) {
+        Phase ph(c, OCT_PROF_FILL);
+        HIP_TRY(c, launch_edges(1, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r, self_pair,
+                                by_receiver, nullptr, p.qoff.as<uint32_t>(), eo, c->index_base, c->stream));
+    }
+    p.n_edges = E;
+    p.materialized = true;
+    out->senders = eo.senders;
+    out->receivers = eo.receivers;
+    out->rel_displacement = eo.rdisp;
+    out->rel_dist_norm = eo.rdist;
+    out->n_edges = E;
+    return OCT_OK;
+}
+
+int oct_device_view(oct_ctx* c, int q, int t, oct_view* out) {
+    if (!c || !out) return OCT_ERR_ARGUMENT;
+    if (!find_pair(c, q, t)) return fail(c, OCT_ERR_STATE, "device_view: pair not registered");
+    SetState& qs = c->sets[set_index(c, q)];
+    SetState& ts = c->sets[set_index(c, t)];
+    if (!qs.built || qs.dirty || !ts.built || ts.dirty) return fail(c, OCT_ERR_STATE, "run! before device_view");
+    memset(out, 0, sizeof(*out));
+    out->coords_q = qs.coords;
+    out->coords_t = ts.coords;
+    out->bounds_min = ts.bmin.as<float>();
+    out->bounds_max = ts.bmax.as<float>();
+    out->node_first = ts.nfirst.as<int32_t>();
+    out->node_last = ts.nlast.as<int32_t>();
+    out->node_children = ts.nchild.as<int32_t>();
+    out->perm = ts.perm.as<int32_t>();
+    out->nodes = ts.nodes.p;
+    out->sorted_t = ts.scoord.as<float>();
+    out->n_nodes = ts.n == 0 ? 0 : ts.hp.n_nodes;
+    out->root = ts.hp.branch_root;
+    out->radius = c->radius;
+    out->radius_sq = c->radius * c->radius;
+    out->n_q = qs.n;
+    out->index_base = c->index_base;
+    return OCT_OK;
+}
+
+}  // extern "C"
+
+__global__ void k_count_neighbors_device(oct_view v, int32_t* counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.n_q) return;
+    int32_t c = 0;
+    oct_for_each_neighbor_device(v, i, [&](int32_t) { ++c; });
+    counts[i] = c;
+}
+
+extern "C" {
+
+int oct_count_neighbors_device(oct_ctx* c, const oct_view* v, int32_t* counts) {
+    if (!c || !v) return OCT_ERR_ARGUMENT;
+    if (v->n_q == 0) return OCT_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_count_neighbors_device, dim3((unsigned)((v->n_q + 127) / 128)), dim3(128), 0, c->stream,
+                       *v, counts);
+    HIP_TRY(c, hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_build_edges_backward(oct_ctx* c, int64_t n_points, const oct_edges* e, const float* d_disp,
+                             const float* d_dist, float radius, float* grad) {
+    if (!c || !e) return OCT_ERR_ARGUMENT;
+    if (n_points < 0 || e->n_edges < 0) return fail(c, OCT_ERR_ARGUMENT, "negative size");
+    if (n_points > 0 && !grad) return fail(c, OCT_ERR_ARGUMENT, "null gradient buffer");
+    HIP_TRY(c, hipSetDevice(c->device));
+    Phase ph(c, OCT_PROF_BACKWARD);
+    HIP_TRY(c, launch_pullback(n_points, e->n_edges, e->senders, e->receivers, e->rel_displacement,
+                               e->rel_dist_norm, d_disp, d_dist, radius, c->index_base, grad, c->stream));
+    return OCT_OK;
+}
+
+int oct_profile(oct_ctx* c, int enable) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    c->profiling = enable != 0;
+    return OCT_OK;
+}
+
+int oct_profile_read(oct_ctx* c, int reset, double* ms, int64_t* calls) {
+    if (!c || !ms || !calls) return OCT_ERR_ARGUMENT;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (auto& m : c->marks) {
+        float t = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&t, m.a, m.b));
+        c->prof_ms[m.phase] += t;
+        c->prof_calls[m.phase] += 1;
+        c->spare.push_back(m.a);
+        c->spare.push_back(m.b);
+    }
+    c->marks.clear();
+    for (int k = 0; k < OCT_PROF_NPHASES; ++k) {
+        ms[k] = c->prof_ms[k];
+        calls[k] = c->prof_calls[k];
+        if (reset) { c->prof_ms[k] = 0; c->prof_calls[k] = 0; }
+    }
+    return OCT_OK;
+}
+
+int oct_synchronize(oct_ctx* c) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return OCT_OK;
+}
+
+}  // extern "C"

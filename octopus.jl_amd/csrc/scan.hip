@@ -1,0 +1,110 @@
+// Device-wide exclusive prefix sum over uint32 (three launches: tile sums,
+// scan of tile sums, tile scan with carry-in).  Used for the radix histograms,
+// the BFS node numbering and the per-query edge offsets.
+#include "internal.hpp"
+
+namespace oct {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 4096
+
+// block-wide exclusive scan of one value per thread; returns the block total
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* lds_waves, uint32_t& total) {
+    const int lane = lane_id();
+    const int w = threadIdx.x / OCT_WAVE;
+    const int nw = blockDim.x / OCT_WAVE;
+    uint32_t incl = (uint32_t)wave_incl_scan((int)v);
+    if (lane == OCT_WAVE - 1) lds_waves[w] = incl;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+    for (int i = 0; i < nw; ++i) {
+        uint32_t s = lds_waves[i];
+        if (i < w) wpre += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return wpre + incl - v;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tile_sums(const uint32_t* __restrict__ in,
+                                                                 int64_t n, uint32_t* __restrict__ sums) {
+    __shared__ uint32_t lw[SCAN_THREADS / OCT_WAVE];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        int64_t k = base + (int64_t)i * SCAN_THREADS + threadIdx.x;
+        if (k < n) s += in[k];
+    }
+    uint32_t tot;
+    block_excl_scan(s, lw, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// one block scans up to SCAN_TILE * 16 tile sums in place, writes the total
+__global__ void __launch_bounds__(1024) k_scan_sums(uint32_t* __restrict__ sums, int64_t m,
+                                                    uint32_t* __restrict__ total_out) {
+    __shared__ uint32_t lw[1024 / OCT_WAVE];
+    const int per = (int)((m + 1023) / 1024);
+    const int64_t lo = (int64_t)threadIdx.x * per;
+    uint32_t s = 0;
+    for (int i = 0; i < per; ++i)
+        if (lo + i < m) s += sums[lo + i];
+    uint32_t tot;
+    uint32_t pre = block_excl_scan(s, lw, tot);
+    for (int i = 0; i < per; ++i) {
+        if (lo + i < m) {
+            uint32_t v = sums[lo + i];
+            sums[lo + i] = pre;
+            pre += v;
+        }
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const uint32_t* __restrict__ in,
+                                                             uint32_t* __restrict__ out, int64_t n,
+                                                             const uint32_t* __restrict__ sums) {
+    __shared__ uint32_t lw[SCAN_THREADS / OCT_WAVE];
+    __shared__ uint32_t tile[SCAN_TILE];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        int64_t k = base + (int64_t)i * SCAN_THREADS + threadIdx.x;
+        tile[i * SCAN_THREADS + threadIdx.x] = k < n ? in[k] : 0u;
+    }
+    __syncthreads();
+    // each thread owns SCAN_ITEMS consecutive entries
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        v[i] = tile[threadIdx.x * SCAN_ITEMS + i];
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t pre = block_excl_scan(s, lw, tot) + sums[blockIdx.x];
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        tile[threadIdx.x * SCAN_ITEMS + i] = pre;
+        pre += v[i];
+    }
+    __syncthreads();
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        int64_t k = base + (int64_t)i * SCAN_THREADS + threadIdx.x;
+        if (k < n) out[k] = tile[i * SCAN_THREADS + threadIdx.x];
+    }
+}
+
+int64_t scan_tmp_words(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
+                          uint32_t* total_out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles > (int64_t)SCAN_TILE * 16) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, tmp, tiles, total_out);
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, in, out, n, tmp);
+    return hipGetLastError();
+}
+
+}  // namespace oct

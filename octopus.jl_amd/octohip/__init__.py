@@ -1,0 +1,418 @@
+"""octohip: Python front-end of the MI355X fixed-radius neighbour search engine.
+
+It mirrors the public API of una-auxme/Octopus.jl (src/Octopus.jl:31-44) —
+TNS, set_search_radius, set_refit_mode, add_point_set, update_point_set,
+resize_point_set, set_active_search, run, build_edges, build_edges_diff,
+device_view, for_each_neighbor — with the same argument meaning and error
+types, over the C ABI of liboctohip.so (hand-written HIP kernels for gfx950).
+Julia's ``!`` suffix is dropped; the TNS methods carry the same names.
+
+Coordinates are (N, 3) float32 CUDA tensors (one point per row, the bytes of
+the reference's column-major (3, N) matrix); they are referenced, never copied
+(src/types.jl:13).  torch supplies device memory and the stream; every
+computation runs in the library.  Indices are 0-based by default
+(index_base=1 reproduces the reference's 1-based values).
+"""
+from __future__ import annotations
+
+import ctypes
+from collections import namedtuple
+
+from . import _lib as L
+
+__all__ = [
+    "TNS", "ArgumentError", "DimensionMismatch", "OctopusError",
+    "set_search_radius", "set_refit_mode", "set_leaf_size", "add_point_set", "update_point_set",
+    "resize_point_set", "set_active_search", "run", "build_edges", "build_edges_diff",
+    "device_view", "for_each_neighbor", "count_neighbors_device", "get_tree",
+]
+
+
+class ArgumentError(ValueError):
+    """Julia ArgumentError (src/api.jl:25,63-65,184,190)."""
+
+
+class DimensionMismatch(ValueError):
+    """Julia DimensionMismatch (src/api.jl:97,137-138)."""
+
+
+class OctopusError(RuntimeError):
+    """Julia ErrorException (src/api.jl:306,452, src/cpu/iterator_cpu.jl:246)."""
+
+
+Edges = namedtuple("Edges", ["senders", "receivers", "rel_displacement", "rel_dist_norm"])
+Tree = namedtuple("Tree", ["node_bounds_min", "node_bounds_max", "node_first", "node_last",
+                           "node_children", "permutation", "n_nodes", "n_leaves", "origin"])
+
+_MODES = {"exact": L.OCT_EDGES_EXACT, "learned": L.OCT_EDGES_LEARNED}
+_ORDERS = {"leaf": L.OCT_ORDER_LEAF, "receiver": L.OCT_ORDER_RECEIVER}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class _DeviceArray:
+    """Zero-copy handle on library-owned device memory for torch.as_tensor."""
+
+    def __init__(self, ptr, shape, typestr):
+        self.__cuda_array_interface__ = {
+            "shape": tuple(shape), "typestr": typestr, "data": (int(ptr), False),
+            "version": 2, "strides": None,
+        }
+
+
+def _wrap(ptr, shape, dtype, device):
+    torch = _torch()
+    n = 1
+    for s in shape:
+        n *= s
+    if n == 0 or not ptr:
+        return torch.empty(shape, dtype=dtype, device=device)
+    typestr = {torch.int32: "<i4", torch.float32: "<f4"}[dtype]
+    return torch.as_tensor(_DeviceArray(ptr, shape, typestr), device=device)
+
+
+class TNS:
+    """TNS(T = Float32; ndims = 3) (src/types.jl:167-183) on one HIP device.
+
+    The C context is created on first use, so argument checks that need no
+    device behave as in the reference even without a GPU."""
+
+    def __init__(self, dtype=None, ndims: int = 3, device=None, index_base: int = 0):
+        torch = _torch()
+        dtype = torch.float32 if dtype is None else dtype
+        if dtype != torch.float32:
+            raise ArgumentError("the HIP engine computes in Float32 only")
+        if ndims != 3:
+            raise ArgumentError("the HIP engine builds octrees (ndims = 3)")
+        if index_base not in (0, 1):
+            raise ArgumentError("index_base must be 0 or 1")
+        self.dtype = dtype
+        self.ndims = ndims
+        self.index_base = index_base
+        self.device = torch.device("cuda", 0 if device is None else int(device))
+        self.radius = 0.0
+        self.refit_mode = False
+        self.target_leaf_size = 32
+        self.point_sets = []        # coords tensors, kept alive (referenced, not copied)
+        self.active_pairs = []
+        self.dirty = []
+        self._ctx = None
+        self._lib = L.load()
+
+    # ------------------------------------------------------------ context
+    def _c(self):
+        if self._ctx is None:
+            torch = _torch()
+            if not torch.cuda.is_available():
+                raise OctopusError("no HIP device: the octohip engine runs on an MI355X only")
+            h = ctypes.c_void_p()
+            rc = self._lib.oct_create(ctypes.byref(h), self.device.index, self.ndims, self.index_base)
+            if rc != L.OCT_OK:
+                raise OctopusError(f"oct_create failed ({rc})")
+            self._ctx = h
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            self._check(self._lib.oct_set_stream(h, ctypes.c_void_p(stream)))
+            if self.radius > 0:
+                self._check(self._lib.oct_set_search_radius(h, self.radius))
+            if self.target_leaf_size != 32:
+                self._check(self._lib.oct_set_leaf_size(h, self.target_leaf_size))
+            if self.refit_mode:
+                self._check(self._lib.oct_set_refit_mode(h, 1))
+        return self._ctx
+
+    def _check(self, rc):
+        if rc == L.OCT_OK:
+            return
+        msg = self._lib.oct_last_error(self._ctx).decode() if self._ctx is not None else ""
+        if rc == L.OCT_ERR_ARGUMENT:
+            raise ArgumentError(msg)
+        if rc == L.OCT_ERR_DIMENSION:
+            raise DimensionMismatch(msg)
+        if rc == L.OCT_ERR_STATE:
+            raise OctopusError(msg)
+        raise OctopusError(f"octohip error {rc}: {msg}")
+
+    def __del__(self):
+        if getattr(self, "_ctx", None) is not None:
+            try:
+                self._lib.oct_destroy(self._ctx)
+            except Exception:
+                pass
+            self._ctx = None
+
+    # method spellings of the free functions
+    def set_search_radius(self, r):
+        return set_search_radius(self, r)
+
+    def add_point_set(self, coords):
+        return add_point_set(self, coords)
+
+    def set_active_search(self, q, t):
+        return set_active_search(self, q, t)
+
+    def run(self):
+        return run(self)
+
+    def build_edges(self, q, t, **kw):
+        return build_edges(self, q, t, **kw)
+
+
+def _coords(tns: TNS, coords):
+    torch = _torch()
+    if not isinstance(coords, torch.Tensor):
+        raise ArgumentError("coords must be a torch tensor on the TNS device")
+    if coords.dim() != 2 or coords.shape[1] != tns.ndims:
+        raise DimensionMismatch(f"coords must be ({tns.ndims}, N) points, i.e. an (N, {tns.ndims}) tensor")
+    if coords.dtype != tns.dtype:
+        raise ArgumentError(f"coords must be {tns.dtype}")
+    if coords.device.type != "cuda":
+        raise ArgumentError("TNS was initialized on :hip but got coords on :cpu; "
+                            "all point sets must share a backend")
+    if not coords.is_contiguous():
+        raise ArgumentError("coords must be contiguous")
+    return coords
+
+
+def _sid(tns, set_id):
+    k = int(set_id) - tns.index_base
+    if k < 0 or k >= len(tns.point_sets):
+        raise ArgumentError(f"unknown point set {set_id}")
+    return k
+
+
+def set_search_radius(tns: TNS, r):
+    """set_search_radius!(tns, r) (src/api.jl:24-29)."""
+    if not r > 0:
+        raise ArgumentError("search radius must be positive")
+    import numpy as np
+
+    tns.radius = float(np.float32(r))
+    tns.dirty = [True] * len(tns.dirty)
+    if tns._ctx is not None:
+        tns._check(tns._lib.oct_set_search_radius(tns._ctx, tns.radius))
+    return tns
+
+
+def set_refit_mode(tns: TNS, on: bool):
+    """set_refit_mode!(tns, on) (src/api.jl:43-47)."""
+    tns.refit_mode = bool(on)
+    tns.dirty = [True] * len(tns.dirty)
+    if tns._ctx is not None:
+        tns._check(tns._lib.oct_set_refit_mode(tns._ctx, int(bool(on))))
+    return tns
+
+
+def set_leaf_size(tns: TNS, leaf_size: int):
+    """target_leaf_size (fixed to 32 in src/types.jl:177; settable here)."""
+    if not 1 <= int(leaf_size) <= 256:
+        raise ArgumentError("leaf size must be in 1..256")
+    tns.target_leaf_size = int(leaf_size)
+    tns.dirty = [True] * len(tns.dirty)
+    if tns._ctx is not None:
+        tns._check(tns._lib.oct_set_leaf_size(tns._ctx, tns.target_leaf_size))
+    return tns
+
+
+def add_point_set(tns: TNS, coords) -> int:
+    """add_point_set!(tns, coords) -> set id (src/api.jl:79-109)."""
+    c = _coords(tns, coords)
+    sid = ctypes.c_int()
+    tns._check(tns._lib.oct_add_point_set(tns._c(), ctypes.c_void_p(c.data_ptr()), c.shape[0],
+                                          ctypes.byref(sid)))
+    tns.point_sets.append(c)
+    tns.dirty.append(True)
+    return sid.value
+
+
+def update_point_set(tns: TNS, set_id: int, coords):
+    """update_point_set!(tns, id, coords), same N (src/api.jl:131-140)."""
+    c = _coords(tns, coords)
+    k = _sid(tns, set_id)
+    if c.shape[0] != tns.point_sets[k].shape[0]:
+        raise DimensionMismatch("update_point_set! expects same size; use resize_point_set! otherwise")
+    tns._check(tns._lib.oct_update_point_set(tns._c(), int(set_id), ctypes.c_void_p(c.data_ptr()), c.shape[0]))
+    tns.point_sets[k] = c
+    tns.dirty[k] = True
+    return tns
+
+
+def resize_point_set(tns: TNS, set_id: int, coords):
+    """resize_point_set!(tns, id, coords), N may change (src/api.jl:111-129)."""
+    c = _coords(tns, coords)
+    k = _sid(tns, set_id)
+    tns._check(tns._lib.oct_resize_point_set(tns._c(), int(set_id), ctypes.c_void_p(c.data_ptr()), c.shape[0]))
+    tns.point_sets[k] = c
+    tns.dirty[k] = True
+    return tns
+
+
+def set_active_search(tns: TNS, q: int, t: int):
+    """set_active_search!(tns, q, t): directed, idempotent (src/api.jl:142-165)."""
+    _sid(tns, q)
+    _sid(tns, t)
+    if (q, t) in tns.active_pairs:
+        return tns
+    tns._check(tns._lib.oct_set_active_search(tns._c(), int(q), int(t)))
+    tns.active_pairs.append((q, t))
+    return tns
+
+
+def run(tns: TNS):
+    """run!(tns) (src/api.jl:183-250): rebuild every dirty set."""
+    if not tns.radius > 0:
+        raise ArgumentError("set_search_radius! before run!")
+    if not tns.point_sets:
+        raise ArgumentError("add at least one point set before run!")
+    tns._check(tns._lib.oct_run(tns._c()))
+    tns.dirty = [False] * len(tns.dirty)
+    return tns
+
+
+def get_tree(tns: TNS, set_id: int) -> Tree:
+    """tns.trees[id] and tns.permutation[id] as device tensors (reference
+    layout, src/types.jl:23-42); valid until the next run."""
+    torch = _torch()
+    _sid(tns, set_id)
+    t = L.OctTree()
+    tns._check(tns._lib.oct_get_tree(tns._c(), int(set_id), ctypes.byref(t)))
+    m, n, dev = int(t.n_nodes), int(t.n_points), tns.device
+    return Tree(
+        node_bounds_min=_wrap(t.node_bounds_min, (m, 3), torch.float32, dev),
+        node_bounds_max=_wrap(t.node_bounds_max, (m, 3), torch.float32, dev),
+        node_first=_wrap(t.node_first, (m,), torch.int32, dev),
+        node_last=_wrap(t.node_last, (m,), torch.int32, dev),
+        node_children=_wrap(t.node_children, (m, 8), torch.int32, dev),
+        permutation=_wrap(t.permutation, (n,), torch.int32, dev),
+        n_nodes=m, n_leaves=int(t.n_leaves), origin=tuple(t.origin),
+    )
+
+
+def build_edges(tns: TNS, q: int, t: int, mode: str = "exact", order: str = "leaf") -> Edges:
+    """build_edges(tns, q, t) (src/api.jl:304-337): flat-COO
+    (senders, receivers, rel_displacement (E,3), rel_dist_norm (E,1)).
+
+    The tensors alias library-owned buffers that the next build on this pair
+    overwrites (the reference's EdgeBuffer aliasing, test/contracts.jl:223-240).
+    order="leaf" groups each receiver's edges in tree-leaf order;
+    order="receiver" writes receivers ascending like the reference."""
+    torch = _torch()
+    if (q, t) not in tns.active_pairs:
+        raise OctopusError(f"build_edges!: pair ({q}, {t}) not registered; call set_active_search! first")
+    if mode not in _MODES:
+        raise ArgumentError(f"mode must be one of {sorted(_MODES)}")
+    if order not in _ORDERS:
+        raise ArgumentError(f"order must be one of {sorted(_ORDERS)}")
+    e = L.OctEdges()
+    tns._check(tns._lib.oct_build_edges(tns._c(), int(q), int(t), _MODES[mode], _ORDERS[order], ctypes.byref(e)))
+    n, dev = int(e.n_edges), tns.device
+    return Edges(
+        senders=_wrap(e.senders, (n,), torch.int32, dev),
+        receivers=_wrap(e.receivers, (n,), torch.int32, dev),
+        rel_displacement=_wrap(e.rel_displacement, (n, 3), torch.float32, dev),
+        rel_dist_norm=_wrap(e.rel_dist_norm, (n, 1), torch.float32, dev),
+    )
+
+
+def device_view(tns: TNS, q: int, t: int) -> L.OctView:
+    """device_view(tns, q, t) (src/api.jl:451-454): a by-value handle for HIP
+    kernels (include/octohip_device.h); invalid after the next run."""
+    v = L.OctView()
+    tns._check(tns._lib.oct_device_view(tns._c(), int(q), int(t), ctypes.byref(v)))
+    return v
+
+
+def count_neighbors_device(tns: TNS, view: L.OctView):
+    """Per-query neighbour counts from a kernel that uses the device iterator
+    (self not filtered), the reference's test/query_cuda.jl:40-56 pattern."""
+    torch = _torch()
+    out = torch.empty(int(view.n_q), dtype=torch.int32, device=tns.device)
+    tns._check(tns._lib.oct_count_neighbors_device(tns._c(), ctypes.byref(view), ctypes.c_void_p(out.data_ptr())))
+    return out
+
+
+def profile(tns: TNS, enable: bool = True):
+    """Turn per-phase kernel timing (HIP events on the TNS stream) on or off."""
+    tns._check(tns._lib.oct_profile(tns._c(), int(bool(enable))))
+
+
+def profile_read(tns: TNS, reset: bool = True) -> dict:
+    """{phase: (milliseconds, launches)} since the last reset."""
+    n = len(L.PROF_PHASES)
+    ms = (ctypes.c_double * n)()
+    calls = (ctypes.c_int64 * n)()
+    tns._check(tns._lib.oct_profile_read(tns._c(), int(bool(reset)), ms, calls))
+    return {name: (ms[k], calls[k]) for k, name in enumerate(L.PROF_PHASES)}
+
+
+def synchronize(tns: TNS):
+    tns._check(tns._lib.oct_synchronize(tns._c()))
+
+
+def for_each_neighbor(f, tns: TNS, q: int, t: int, i: int):
+    """Host for_each_neighbor: the reference allows it on CPU handles only and
+    raises on device handles (src/cpu/iterator_cpu.jl:246); this engine's
+    handles are device handles, so use the device iterator instead."""
+    raise OctopusError("for_each_neighbor (host) requires a CPU TNS; use for_each_neighbor_device on GPU")
+
+
+# ----------------------------------------------------------------- autograd
+
+
+def _pullback(tns: TNS, n_points: int, senders, receivers, rdisp, rdist, d_disp, d_dist, radius):
+    torch = _torch()
+    grad = torch.empty((n_points, 3), dtype=torch.float32, device=tns.device)
+    e = L.OctEdges()
+    e.senders = senders.data_ptr() if senders.numel() else None
+    e.receivers = receivers.data_ptr() if receivers.numel() else None
+    e.rel_displacement = rdisp.data_ptr() if rdisp.numel() else None
+    e.rel_dist_norm = rdist.data_ptr() if rdist.numel() else None
+    e.n_edges = senders.numel()
+    dd = None if d_disp is None else ctypes.c_void_p(d_disp.contiguous().data_ptr())
+    dn = None if d_dist is None else ctypes.c_void_p(d_dist.contiguous().data_ptr())
+    tns._check(tns._lib.oct_build_edges_backward(tns._c(), n_points, ctypes.byref(e), dd, dn, float(radius),
+                                                 ctypes.c_void_p(grad.data_ptr())))
+    return grad
+
+
+def _make_fn():
+    torch = _torch()
+
+    class BuildEdgesDiff(torch.autograd.Function):
+        """rrule of build_edges_diff (ext/OctopusChainRulesCoreExt.jl:137-192):
+        primal = build_edges copies; pullback = the HIP adjoint kernel."""
+
+        @staticmethod
+        def forward(ctx, coords, tns, set_id, radius):
+            e = build_edges(tns, set_id, set_id)
+            s, r, d, n = (x.clone() for x in e)
+            ctx.save_for_backward(s, r, d, n)
+            ctx.tns, ctx.n_points, ctx.radius = tns, coords.shape[0], radius
+            ctx.mark_non_differentiable(s, r)
+            ctx.set_materialize_grads(False)  # an unused output passes a NULL tangent
+            return s, r, d, n
+
+        @staticmethod
+        def backward(ctx, g_s, g_r, g_disp, g_dist):
+            s, r, d, n = ctx.saved_tensors
+            grad = _pullback(ctx.tns, ctx.n_points, s, r, d, n, g_disp, g_dist, ctx.radius)
+            return grad, None, None, None
+
+    return BuildEdgesDiff
+
+
+_BuildEdgesDiff = None
+
+
+def build_edges_diff(coords, tns: TNS, set_id: int, radius) -> Edges:
+    """build_edges_diff(coords, tns, id, radius) (src/Octopus.jl:56-73):
+    differentiable copies of build_edges(tns, id, id); gradients flow to coords
+    through the HIP adjoint.  Keep coords in sync with the TNS
+    (update_point_set + run) before calling, as in the reference."""
+    global _BuildEdgesDiff
+    if _BuildEdgesDiff is None:
+        _BuildEdgesDiff = _make_fn()
+    return Edges(*_BuildEdgesDiff.apply(coords, tns, set_id, radius))

@@ -1,0 +1,90 @@
+"""CPU-only checks of the drop-in boundary: the library loads without a GPU,
+exports every entry point include/octohip.h declares, and the front-end's
+argument checks raise the reference's error types before touching a device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import octohip
+from octohip import _lib, datasets
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("octohip.h",):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        names |= set(re.findall(r"^\s*(?:int|const char\*)\s+(oct_\w+)\s*\(", src, re.M))
+    return names
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_functions()
+    for n in ("oct_create", "oct_run", "oct_build_edges", "oct_build_edges_backward", "oct_device_view"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_binding_covers_every_declared_symbol():
+    assert declared_functions() == set(_lib.SIGNATURES)
+    _lib.load()
+
+
+def test_struct_sizes_match_header():
+    # field layouts the kernels and the header agree on
+    assert ctypes.sizeof(_lib.OctEdges) == 40
+    assert ctypes.sizeof(_lib.OctTree) == 6 * 8 + 4 + 4 + 8 + 12 + 4
+    assert ctypes.sizeof(_lib.OctView) == 10 * 8 + 4 * 4 + 8 + 8
+
+
+def test_argument_errors_without_device():
+    tns = octohip.TNS()
+    with pytest.raises(octohip.ArgumentError):
+        octohip.set_search_radius(tns, 0.0)
+    with pytest.raises(octohip.ArgumentError):
+        octohip.set_search_radius(tns, -1.0)
+    with pytest.raises(octohip.ArgumentError):
+        octohip.run(tns)  # no radius
+    octohip.set_search_radius(tns, 0.1)
+    with pytest.raises(octohip.ArgumentError):
+        octohip.run(tns)  # no point sets
+    with pytest.raises(octohip.ArgumentError):
+        octohip.TNS(ndims=2)
+    with pytest.raises(octohip.OctopusError):
+        octohip.for_each_neighbor(lambda j: None, tns, 0, 0, 0)
+
+
+def test_coords_validation_without_device():
+    import torch
+
+    tns = octohip.TNS()
+    with pytest.raises(octohip.DimensionMismatch):
+        octohip.add_point_set(tns, torch.zeros(10, 2))
+    with pytest.raises(octohip.ArgumentError):
+        octohip.add_point_set(tns, torch.zeros(10, 3))  # CPU tensor on a device engine
+
+
+def test_datasets_deterministic():
+    a = datasets.uniform_cube(1000, 3)
+    b = datasets.uniform_cube(1000, 3)
+    assert a.dtype == np.float32 and np.array_equal(a, b)
+    assert a.min() >= 0 and a.max() < 1
+    assert not np.array_equal(a, datasets.uniform_cube(1000, 4))
+    z = datasets.normal(1, 0, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+
+
+def test_sph_lattice_shape():
+    pts, r = datasets.sph_lattice(side=10)
+    assert pts.shape == (1000, 3) and np.float32(r) == np.float32(0.02)
+    g = np.round(pts / 0.01)
+    assert np.all(np.abs(pts - g * 0.01) <= 0.001 + 1e-7)

@@ -1,0 +1,313 @@
+"""GPU parity: run! and build_edges through the C ABI against the CPU oracle.
+
+Bar: bit-exact for indices, tree arrays and (sorted) edge sets; the edge
+features are compared bit-exactly too (the engine builds with
+-ffp-contract=off, so no 1-ulp slack is needed).  Inputs are the reference
+tests' shapes plus the workload configurations, seeded.
+"""
+import numpy as np
+import pytest
+
+from brute import brute_edges, sort_edges
+from octohip import datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _gpu(pts):
+    return _torch().from_numpy(np.ascontiguousarray(pts, np.float32)).cuda()
+
+
+def _build(pts, r, leaf=32, base=0):
+    import octohip
+
+    tns = octohip.TNS(index_base=base)
+    octohip.set_search_radius(tns, r)
+    if leaf != 32:
+        octohip.set_leaf_size(tns, leaf)
+    c = _gpu(pts)
+    sid = octohip.add_point_set(tns, c)
+    octohip.set_active_search(tns, sid, sid)
+    octohip.run(tns)
+    return tns, sid, c
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+TREE_CASES = {
+    "uniform500": (lambda: datasets.uniform_cube(500, 1), 0.08),
+    "uniform3000": (lambda: datasets.uniform_cube(3000, 2), 0.05),
+    "blobs2000": (lambda: np.concatenate([datasets.gaussian_blob(1000, 21, 0.1, (0.25,) * 3),
+                                          datasets.gaussian_blob(1000, 22, 0.1, (-0.25,) * 3)]), 0.05),
+    "coincident64": (lambda: np.zeros((64, 3), np.float32), 0.1),
+    "single": (lambda: np.zeros((1, 3), np.float32), 1.0),
+    "plain100k": (lambda: datasets.plain_cpu()[0], 0.02),
+    "negative": (lambda: datasets.uniform_cube(2000, 3001, lo=-1.0, hi=1.0), 0.1),
+}
+
+
+@pytest.mark.parametrize("case", sorted(TREE_CASES))
+@pytest.mark.parametrize("leaf", [32, 8])
+def test_tree_parity(orc, case, leaf):
+    import octohip
+
+    make, r = TREE_CASES[case]
+    pts = make()
+    print(f"tree parity case={case} n={pts.shape[0]} leaf={leaf}", flush=True)
+    tns, sid, _ = _build(pts, r, leaf=leaf)
+    t = octohip.get_tree(tns, sid)
+    o = orc.run(pts, r, leaf_size=leaf)
+    assert t.n_nodes == o.n_nodes
+    assert np.array_equal(_np(t.permutation), o.perm)
+    assert np.array_equal(_np(t.node_first), o.node_first)
+    assert np.array_equal(_np(t.node_last), o.node_last)
+    assert np.array_equal(_np(t.node_children), o.node_children)
+    assert np.array_equal(_np(t.node_bounds_min), o.bmin)
+    assert np.array_equal(_np(t.node_bounds_max), o.bmax)
+    assert np.array_equal(np.array(t.origin, np.float32), o.origin)
+    assert t.n_leaves == int(np.sum(np.all(o.node_children == -1, axis=1)))
+
+
+def test_tree_index_base_one(orc):
+    import octohip
+
+    pts = datasets.uniform_cube(3000, 2)
+    tns, sid, _ = _build(pts, 0.05, base=1)
+    assert sid == 1
+    t = octohip.get_tree(tns, sid)
+    o = orc.run(pts, 0.05)
+    assert np.array_equal(_np(t.permutation), o.perm + 1)
+    assert np.array_equal(_np(t.node_first), o.node_first + 1)
+    kids = o.node_children.copy()
+    kids[kids >= 0] += 1
+    assert np.array_equal(_np(t.node_children), kids)
+
+
+def _edges_equal(got, want):
+    g = sort_edges(*got)
+    w = sort_edges(*want)
+    assert np.array_equal(g[0], w[0]), "senders"
+    assert np.array_equal(g[1], w[1]), "receivers"
+    assert np.array_equal(g[2], w[2]), "rel_displacement"
+    assert np.array_equal(g[3], w[3]), "rel_dist_norm"
+
+
+def _gpu_edges(e):
+    return (_np(e.senders), _np(e.receivers), _np(e.rel_displacement), _np(e.rel_dist_norm).reshape(-1))
+
+
+def _assert_receiver_runs(recv):
+    # each receiver's edges are one contiguous run
+    if recv.shape[0] == 0:
+        return
+    starts = np.concatenate([[True], recv[1:] != recv[:-1]])
+    assert np.unique(recv[starts]).shape[0] == starts.sum()
+
+
+EDGE_CASES = {
+    "n200_r015": (lambda: datasets.uniform_cube(200, 11), 0.15),
+    "n1000_r005": (lambda: datasets.uniform_cube(1000, 21), 0.05),
+    "clustered2000": (lambda: (np.float32(0.1) * datasets.normal(31, 0, 6000).astype(np.float32)).reshape(2000, 3), 0.03),
+    "coincident100": (lambda: np.zeros((100, 3), np.float32), 0.1),
+    "negative200": (lambda: datasets.uniform_cube(200, 3001, lo=-1.0, hi=1.0), 0.1),
+    "tiny_radius": (lambda: datasets.uniform_cube(200, 3002), 1e-8),
+    "huge_radius": (lambda: datasets.uniform_cube(100, 3003), 100.0),
+    "plain100k": (lambda: datasets.plain_cpu()[0], 0.02),
+    "diff262k": (lambda: datasets.differentiable()[0], float(datasets.differentiable(8)[1])),
+}
+
+
+@pytest.mark.parametrize("case", sorted(EDGE_CASES))
+def test_edges_parity(orc, case):
+    import octohip
+
+    make, r = EDGE_CASES[case]
+    pts = make()
+    print(f"edges parity case={case} n={pts.shape[0]} r={r}", flush=True)
+    tns, sid, _ = _build(pts, r)
+    e = octohip.build_edges(tns, sid, sid)
+    got = _gpu_edges(e)
+    o = orc.build_edges(orc.run(pts, r), pts)
+    _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+    _assert_receiver_runs(got[1])
+    if pts.shape[0] <= 2000:
+        _edges_equal(got, brute_edges(pts, pts, r, exclude_self=True))
+
+
+def test_edges_receiver_order(orc):
+    import octohip
+
+    pts = datasets.uniform_cube(20000, 5)
+    r = 0.04
+    tns, sid, _ = _build(pts, r)
+    e = octohip.build_edges(tns, sid, sid, order="receiver")
+    got = _gpu_edges(e)
+    o = orc.build_edges(orc.run(pts, r), pts)
+    assert np.all(np.diff(got[1]) >= 0)  # receivers ascending, as the reference writes them
+    _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+
+
+def test_edges_asymmetric(orc):
+    import octohip
+
+    a = datasets.uniform_cube(150, 41)
+    b = datasets.uniform_cube(250, 42)
+    r = 0.12
+    tns = octohip.TNS()
+    octohip.set_search_radius(tns, r)
+    ia = octohip.add_point_set(tns, _gpu(a))
+    ib = octohip.add_point_set(tns, _gpu(b))
+    octohip.set_active_search(tns, ia, ib)
+    octohip.run(tns)
+    e = octohip.build_edges(tns, ia, ib)
+    _edges_equal(_gpu_edges(e), brute_edges(a, b, r, exclude_self=False))
+    o = orc.build_edges(orc.run(b, r), a, b)
+    _edges_equal(_gpu_edges(e), (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+
+
+def test_edges_sph_lattice(orc):
+    """Config: 1M-point SPH block (100^3 lattice, h = 0.01, r = 2h)."""
+    import octohip
+
+    pts, r = datasets.sph_lattice()
+    tns, sid, _ = _build(pts, r)
+    e = octohip.build_edges(tns, sid, sid)
+    got = _gpu_edges(e)
+    o = orc.build_edges(orc.run(pts, r), pts)
+    assert got[0].shape[0] == o.n_edges
+    _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+
+
+def test_empty_and_single():
+    import octohip
+
+    tns, sid, _ = _build(np.zeros((0, 3), np.float32), 0.1)
+    e = octohip.build_edges(tns, sid, sid)
+    assert e.senders.numel() == 0 and tuple(e.rel_displacement.shape) == (0, 3)
+    tns, sid, _ = _build(np.zeros((1, 3), np.float32), 1.0)
+    assert octohip.get_tree(tns, sid).n_nodes == 1
+    e = octohip.build_edges(tns, sid, sid)
+    assert e.senders.numel() == 0
+
+
+def test_rebuild_after_update(orc):
+    import octohip
+
+    p1 = datasets.uniform_cube(150, 71)
+    p2 = datasets.uniform_cube(150, 72)
+    tns, sid, c1 = _build(p1, 0.1)
+    n1 = octohip.build_edges(tns, sid, sid).senders.numel()
+    assert n1 > 0
+    octohip.update_point_set(tns, sid, _gpu(p2))
+    octohip.run(tns)
+    e = octohip.build_edges(tns, sid, sid)
+    _edges_equal(_gpu_edges(e), brute_edges(p2, p2, 0.1, exclude_self=True))
+
+
+def test_device_iterator_counts(orc):
+    import octohip
+
+    pts = datasets.uniform_cube(5000, 9)
+    r = 0.06
+    tns, sid, _ = _build(pts, r)
+    v = octohip.device_view(tns, sid, sid)
+    cnt = _np(octohip.count_neighbors_device(tns, v))
+    o = orc.run(pts, r)
+    want = np.array([orc.for_each_neighbor(o, pts, i).shape[0] for i in range(pts.shape[0])])
+    assert np.array_equal(cnt, want)  # self included, as the device iterator does not filter
+
+
+# ----------------------------------------------------------------- backward
+
+
+def test_backward_vs_oracle_pullback(orc):
+    import octohip
+
+    pts, r = datasets.differentiable()
+    tns, sid, c = _build(pts, r)
+    e = octohip.build_edges(tns, sid, sid)
+    s, rc, d, n = _gpu_edges(e)
+    torch = _torch()
+    dd = torch.randn(d.shape, generator=torch.Generator().manual_seed(0)).float()
+    dn = torch.randn(n.shape, generator=torch.Generator().manual_seed(1)).float()
+    g = octohip._pullback(tns, pts.shape[0], e.senders, e.receivers, e.rel_displacement, e.rel_dist_norm,
+                          dd.cuda(), dn.cuda(), r)
+    want = orc.pullback(pts.shape[0], s, rc, d, n, dd.numpy(), dn.numpy(), r)
+    np.testing.assert_allclose(_np(g), want, rtol=1e-4, atol=1e-3)
+
+
+def test_backward_closed_form():
+    """loss = sum|rdisp|^2 + 0.5 sum rdist^2 -> g_i = (6/r^2) sum_j (x_i - x_j)."""
+    import octohip
+
+    torch = _torch()
+    pts, r = datasets.differentiable()
+    tns, sid, c = _build(pts, r)
+    x = c.clone().requires_grad_(True)
+    e = octohip.build_edges_diff(x, tns, sid, r)
+    loss = (e.rel_displacement ** 2).sum() + 0.5 * (e.rel_dist_norm ** 2).sum()
+    loss.backward()
+    s, rc = _np(e.senders), _np(e.receivers)
+    xd = pts.astype(np.float64)
+    want = np.zeros_like(xd)
+    np.add.at(want, rc, 6.0 / float(np.float32(r)) ** 2 * (xd[rc] - xd[s]))
+    np.testing.assert_allclose(_np(x.grad), want, rtol=1e-3, atol=5e-2)
+
+
+def test_backward_fd64_subset():
+    """Central differences in FP64 on a 2,048-point subset (radius ~0.15)."""
+    import octohip
+
+    torch = _torch()
+    pts = datasets.uniform_cube(2048, 55)
+    r = 0.15
+    tns, sid, c = _build(pts, r)
+    x = c.clone().requires_grad_(True)
+    e = octohip.build_edges_diff(x, tns, sid, r)
+    loss = (e.rel_displacement ** 2).sum() + 0.5 * (e.rel_dist_norm ** 2).sum()
+    loss.backward()
+    s, rc = _np(e.senders).astype(np.int64), _np(e.receivers).astype(np.int64)
+    xd = pts.astype(np.float64)
+    rd = float(np.float32(r))
+
+    def f(z):
+        d = (z[rc] - z[s]) / rd
+        q = np.sum(d * d, axis=1)
+        return np.sum(q) + 0.5 * np.sum(q)
+
+    rng = np.random.default_rng(0)
+    idx = rng.choice(xd.size, 64, replace=False)
+    g = _np(x.grad).reshape(-1)
+    eps = 1e-4
+    for k in idx:
+        vals = []
+        for sgn in (2, 1, -1, -2):
+            z = xd.copy().reshape(-1)
+            z[k] += sgn * eps
+            vals.append(f(z.reshape(xd.shape)))
+        fd = (-vals[0] + 8 * vals[1] - 8 * vals[2] + vals[3]) / (12 * eps)
+        assert abs(g[k] - fd) <= 1e-4 * abs(fd) + 2e-3, (k, g[k], fd)
+
+
+def test_backward_partial_tangents(orc):
+    import octohip
+
+    pts = datasets.uniform_cube(3000, 57)
+    r = 0.08
+    tns, sid, c = _build(pts, r)
+    for wd, wn in ((1.0, 0.0), (0.0, 1.0)):
+        x = c.clone().requires_grad_(True)
+        e = octohip.build_edges_diff(x, tns, sid, r)
+        loss = wd * (e.rel_displacement ** 2).sum() + wn * (e.rel_dist_norm ** 2).sum()
+        loss.backward()
+        s, rc, d, n = (_np(e.senders), _np(e.receivers), _np(e.rel_displacement), _np(e.rel_dist_norm).reshape(-1))
+        want = orc.pullback(3000, s, rc, d, n, None if wd == 0 else 2 * d, None if wn == 0 else 2 * n, r)
+        np.testing.assert_allclose(_np(x.grad), want, rtol=1e-4, atol=1e-3)
