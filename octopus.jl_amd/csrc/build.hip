@@ -507,7 +507,7 @@ k_refit(const int32_t* __restrict__ leaf_ids, int32_t n_leaves_max, const BuildP
     }
 }
 
-__global__ void k_set_leaf_count(BuildParams* prm, const uint32_t* total) {
+__global__ void k_set_leaf_count(BuildParams* prm, const uint64_t* total) {
     if (threadIdx.x == 0 && blockIdx.x == 0) prm->n_leaves = (int32_t)(*total) - prm->n_nodes;
 }
 

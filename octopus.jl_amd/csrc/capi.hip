@@ -62,7 +62,7 @@ struct SetState {
         b.vals[1] = vals1.as<int32_t>();
         b.hist = hist.as<uint32_t>();
         b.scan_tmp = scan_tmp.as<uint32_t>();
-        b.total = total.as<uint32_t>();
+        b.total = total.as<uint64_t>();
         b.topo = topo.as<uint16_t>();
         b.perm = perm.as<int32_t>();
         b.scoord = scoord.as<float4>();
@@ -221,7 +221,7 @@ int build_set(oct_ctx* c, SetState& s) {
     HIP_TRY(c, s.vals1.ensure(4 * (size_t)n));
     HIP_TRY(c, s.hist.ensure(4 * (size_t)hist_words(n)));
     HIP_TRY(c, s.scan_tmp.ensure(4 * (size_t)scan_tmp_words(hist_words(n))));
-    HIP_TRY(c, s.total.ensure(16));
+    HIP_TRY(c, s.total.ensure(16));  // uint64 total
     HIP_TRY(c, s.topo.ensure(2 * (size_t)n));
     HIP_TRY(c, s.perm.ensure(4 * (size_t)n));
     HIP_TRY(c, s.scoord.ensure(16 * (size_t)n));
@@ -435,14 +435,16 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
     const float inv_r = 1.0f / r;
     const TreeView qv = view_of(qs), tv = view_of(ts);
     const int64_t nql = qs.hp.n_leaves;
-    HIP_TRY(c, p.flags.ensure(16));
+    HIP_TRY(c, p.flags.ensure(32));
     HIP_TRY(c, p.lcount.ensure(4 * (size_t)nql));
     HIP_TRY(c, p.qcount.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.qoff.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.scan_tmp.ensure(4 * (size_t)scan_tmp_words(qs.n)));
     const int by_receiver = order == OCT_ORDER_RECEIVER;
+    // flags[0]: largest candidate count of an overflowing query leaf (0 = none);
+    // flags[2..3]: 64-bit total of the per-query counts
     uint32_t* flags = p.flags.as<uint32_t>();
-    uint32_t host_flags[2] = {0, 0};
+    uint32_t host_flags[4] = {0, 0, 0, 0};
     for (int attempt = 0; attempt < 8; ++attempt) {
         HIP_TRY(c, p.lists.ensure(sizeof(int2) * (size_t)nql * (size_t)p.cap));
         HIP_TRY(c, launch_fill_u32(flags, 4, 0u, c->stream));
@@ -460,17 +462,21 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
         {
             Phase ph(c, OCT_PROF_SCAN);
             HIP_TRY(c, exclusive_scan(p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), qs.n,
-                                      p.scan_tmp.as<uint32_t>(), flags + 1, c->stream));
+                                      p.scan_tmp.as<uint32_t>(), (uint64_t*)(flags + 2), c->stream));
         }
-        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 16, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
-        memcpy(host_flags, c->pinned, 8);
+        memcpy(host_flags, c->pinned, 16);
         if (host_flags[0] == 0) break;
         p.cap = (int)std::min<uint32_t>(host_flags[0] + 16u, (uint32_t)INT32_MAX / 8u);
     }
     if (host_flags[0] != 0) return fail(c, OCT_ERR_DEVICE, "candidate list capacity did not settle");
-    const int64_t E = host_flags[1];
-    if (E >= (int64_t)INT32_MAX) return fail(c, OCT_ERR_CAPACITY, "edge count exceeds Int32 (the reference's limit)");
+    uint64_t total = 0;
+    memcpy(&total, host_flags + 2, 8);
+    // Int32 indices and counts, as in the reference (src/types.jl:89)
+    if (total >= (uint64_t)INT32_MAX)
+        return fail(c, OCT_ERR_CAPACITY, "edge count exceeds Int32 (the reference's limit)");
+    const int64_t E = (int64_t)total;
     HIP_TRY(c, p.senders.ensure(4 * (size_t)E));
     HIP_TRY(c, p.receivers.ensure(4 * (size_t)E));
     HIP_TRY(c, p.rdisp.ensure(12 * (size_t)E));

@@ -53,7 +53,7 @@ struct BuildBufs {
     int32_t* vals[2];
     uint32_t* hist;      // radix histograms, reused for the topology table
     uint32_t* scan_tmp;
-    uint32_t* total;     // 1 word
+    uint64_t* total;     // scan total of the topology table
     uint16_t* topo;      // per sorted position: m + 1, lambda, emit flag
     int32_t* perm;       // exported, index base applied
     float4* scoord;      // Morton-sorted coords, w = original index bits
@@ -95,7 +95,9 @@ hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* sen
                            float* grad, hipStream_t s);
 
 int64_t scan_tmp_words(int64_t n);
+// Exclusive scan of n uint32 counts; the optional 64-bit total is exact even
+// when the uint32 offsets would wrap (callers check it first).
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
-                          uint32_t* total_out, hipStream_t s);
+                          uint64_t* total_out, hipStream_t s);
 
 }  // namespace oct

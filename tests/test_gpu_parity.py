@@ -121,7 +121,7 @@ EDGE_CASES = {
     "tiny_radius": (lambda: datasets.uniform_cube(200, 3002), 1e-8),
     "huge_radius": (lambda: datasets.uniform_cube(100, 3003), 100.0),
     "plain100k": (lambda: datasets.plain_cpu()[0], 0.02),
-    "diff262k": (lambda: datasets.differentiable()[0], float(datasets.differentiable(8)[1])),
+    "diff262k": (lambda: datasets.differentiable()[0], float(datasets.differentiable()[1])),
 }
 
 
@@ -184,6 +184,17 @@ def test_edges_sph_lattice(orc):
     o = orc.build_edges(orc.run(pts, r), pts)
     assert got[0].shape[0] == o.n_edges
     _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+
+
+def test_edge_count_beyond_int32_is_an_error():
+    """50,000 points with a radius covering the cube give 2.5e9 > 2^31 edges:
+    the engine must refuse before allocating (the reference's Int32 limit)."""
+    import octohip
+
+    pts = datasets.uniform_cube(50_000, 91)
+    tns, sid, _ = _build(pts, 2.0)
+    with pytest.raises(octohip.OctopusError, match="Int32"):
+        octohip.build_edges(tns, sid, sid)
 
 
 def test_empty_and_single():
