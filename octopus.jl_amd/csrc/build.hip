@@ -28,17 +28,15 @@ namespace oct {
 
 constexpr int BND_THREADS = 256;
 
-// Per-block ordered-uint min/max of the coordinates; the last block to finish
-// folds the partials and derives origin, 1/r and the live key bits.
+// Per-block ordered-uint min/max of the coordinates.
 __global__ void __launch_bounds__(BND_THREADS)
-k_bounds(const float* __restrict__ coords, int64_t n, float radius, uint32_t* __restrict__ partials,
-         BuildParams* __restrict__ prm) {
+k_bounds_partial(const float* __restrict__ coords, int64_t n, uint32_t* __restrict__ partials) {
     uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
     uint32_t mx[3] = {0u, 0u, 0u};
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         for (int d = 0; d < 3; ++d) {
-            uint32_t o = f2o(coords[3 * i + d]);
+            const uint32_t o = f2o(coords[3 * i + d]);
             mn[d] = o < mn[d] ? o : mn[d];
             mx[d] = o > mx[d] ? o : mx[d];
         }
@@ -46,35 +44,46 @@ k_bounds(const float* __restrict__ coords, int64_t n, float radius, uint32_t* __
     __shared__ uint32_t red[6][BND_THREADS / OCT_WAVE];
     const int w = threadIdx.x / OCT_WAVE;
     for (int d = 0; d < 3; ++d) {
-        uint32_t a = wave_min_u(mn[d]);
-        uint32_t b = wave_max_u(mx[d]);
+        const uint32_t a = wave_min_u(mn[d]);
+        const uint32_t b = wave_max_u(mx[d]);
         if (lane_id() == 0) { red[d][w] = a; red[3 + d][w] = b; }
     }
     __syncthreads();
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        uint32_t v[6];
-        for (int d = 0; d < 6; ++d) v[d] = red[d][0];
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        uint32_t v = red[d][0];
         for (int k = 1; k < BND_THREADS / OCT_WAVE; ++k)
-            for (int d = 0; d < 3; ++d) {
-                v[d] = red[d][k] < v[d] ? red[d][k] : v[d];
-                v[3 + d] = red[3 + d][k] > v[3 + d] ? red[3 + d][k] : v[3 + d];
-            }
-        for (int d = 0; d < 6; ++d) partials[6 * blockIdx.x + d] = v[d];
-        __threadfence();
-        uint32_t ticket = atomicAdd(&prm->done, 1u);
-        last = (ticket == gridDim.x - 1);
+            v = d < 3 ? (red[d][k] < v ? red[d][k] : v) : (red[d][k] > v ? red[d][k] : v);
+        partials[6 * blockIdx.x + d] = v;
+    }
+}
+
+// One block folds the partials and derives origin, 1/r and the live key bits.
+__global__ void __launch_bounds__(BND_THREADS)
+k_bounds_final(const uint32_t* __restrict__ partials, int nblocks, float radius, BuildParams* __restrict__ prm) {
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+    uint32_t mx[3] = {0u, 0u, 0u};
+    for (int b = threadIdx.x; b < nblocks; b += BND_THREADS)
+        for (int d = 0; d < 3; ++d) {
+            const uint32_t a = partials[6 * b + d], c = partials[6 * b + 3 + d];
+            mn[d] = a < mn[d] ? a : mn[d];
+            mx[d] = c > mx[d] ? c : mx[d];
+        }
+    __shared__ uint32_t red[6][BND_THREADS / OCT_WAVE];
+    const int w = threadIdx.x / OCT_WAVE;
+    for (int d = 0; d < 3; ++d) {
+        const uint32_t a = wave_min_u(mn[d]);
+        const uint32_t b = wave_max_u(mx[d]);
+        if (lane_id() == 0) { red[d][w] = a; red[3 + d][w] = b; }
     }
     __syncthreads();
-    if (!last || threadIdx.x != 0) return;
-    __threadfence();
-    uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
-    for (unsigned b = 0; b < gridDim.x; ++b)
+    if (threadIdx.x != 0) return;
+    uint32_t v[6];
+    for (int d = 0; d < 6; ++d) v[d] = red[d][0];
+    for (int k = 1; k < BND_THREADS / OCT_WAVE; ++k)
         for (int d = 0; d < 3; ++d) {
-            uint32_t a = (uint32_t)load_coherent((const int*)&partials[6 * b + d]);
-            uint32_t c = (uint32_t)load_coherent((const int*)&partials[6 * b + 3 + d]);
-            v[d] = a < v[d] ? a : v[d];
-            v[3 + d] = c > v[3 + d] ? c : v[3 + d];
+            v[d] = red[d][k] < v[d] ? red[d][k] : v[d];
+            v[3 + d] = red[3 + d][k] > v[3 + d] ? red[3 + d][k] : v[3 + d];
         }
     const float inv_cs = 1.0f / radius;  // one(T) / cell_size
     uint32_t cmax = 0;
@@ -89,7 +98,6 @@ k_bounds(const float* __restrict__ coords, int64_t n, float radius, uint32_t* __
     prm->inv_cs = inv_cs;
     prm->cmax = cmax;
     prm->key_bits = cmax == 0 ? 0 : 3 * (32 - __clz((int)cmax));
-    prm->done = 0;  // ready for the next build
 }
 
 // ================================================================ keys
@@ -309,24 +317,30 @@ k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, uint16_t* __rest
 template <typename P>
 __device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
     while (lo < hi) {
-        int64_t mid = lo + ((hi - lo) >> 1);
+        const int64_t mid = lo + ((hi - lo) >> 1);
         if (pred(mid)) hi = mid;
         else lo = mid + 1;
     }
     return lo;
 }
 
-// Emits every node starting at each sorted position: reference-layout
-// arrays (node_first/last/children with the index base, bounds), packed
-// records, parent links, and leaf bounds.  Nodes covering all n points get
-// the global bounds; the rest of the internal bounds come from k_refit.
+// Emits every node starting at each sorted position, with no search: the
+// BFS id of the node at (level L, position k) is the scanned (L, block)
+// offset plus the ballot rank inside the block; its parent is either the node
+// one level up at the same position or, for the first level, the last node
+// of the level above starting before k (the same ranks, minus one).
+// Writes node_first, the packed record (first child for internal nodes),
+// parent links and levels; leaves also get their range end and bounds, and
+// the nodes covering all n points (the chain down to the first split) get the
+// global bounds.  Child slots and counts come from k_link, internal bounds and
+// range ends from k_level_refit.
 __global__ void __launch_bounds__(TOPO_THREADS)
 k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, int64_t n, int T,
             const uint16_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
             BuildParams* __restrict__ prm, int index_base, NodeRec* __restrict__ nodes,
             int32_t* __restrict__ nfirst, int32_t* __restrict__ nlast, int32_t* __restrict__ nchild,
             float* __restrict__ bmin, float* __restrict__ bmax, int32_t* __restrict__ parent,
-            int32_t* __restrict__ visit, int32_t* __restrict__ leaf_ids) {
+            uint8_t* __restrict__ nlevel, int32_t* __restrict__ leaf_ids) {
     __shared__ uint32_t wc[TOPO_THREADS / OCT_WAVE][OCT_NLEVELS + 1];
     const int64_t k = (int64_t)blockIdx.x * TOPO_THREADS + threadIdx.x;
     int mk = -1, lam = -1;
@@ -347,13 +361,10 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
         if (lane == 0) wc[w][L] = (uint32_t)__popcll(b);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // total node count M = start of the leaf row in the scanned table
-        // (written by every block with the same value; also pick up leaves)
-        if (blockIdx.x == 0) {
-            const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
-            prm->n_nodes = M;
-        }
+    const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
+    if (k == 0) {
+        prm->n_nodes = M;
+        prm->chain_level = shared_levels(sk[0], sk[n - 1]);
     }
     if (!emit) return;
     auto wave_off = [&](int L) {
@@ -361,48 +372,35 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
         for (int q = 0; q < w; ++q) s += wc[q][L];
         return s;
     };
-    const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
-    auto node_id = [&](int L) -> int32_t {
+    // count of level-L node starts before position k, plus the level's offset
+    auto before = [&](int L) -> int32_t {
         return (int32_t)(hoff[(int64_t)L * nblk + blockIdx.x] + wave_off(L) + rank_in_wave[L]);
     };
+    const int L0 = shared_levels(sk[0], sk[n - 1]);  // levels 0..L0 hold all n points
     const uint64_t key = sk[k];
     const int Lfirst = mk + 1;  // 0 at k == 0
     for (int L = Lfirst; L <= lam; ++L) {
-        const int32_t id = node_id(L);
-        visit[id] = 0;
-        if (L == Lfirst && k == 0 && L == 0) parent[id] = -1;
-        if (L > Lfirst) parent[id] = node_id(L - 1);
-        int64_t E;  // exclusive end of the range
+        const int32_t id = before(L);
+        nlevel[id] = (uint8_t)L;
+        nfirst[id] = (int32_t)k + index_base;
+        for (int o = 0; o < 8; ++o) nchild[8 * (int64_t)id + o] = -1;
+        parent[id] = L == 0 ? -1 : (L > Lfirst ? before(L - 1) : before(L - 1) - 1);
+        const bool chain = (k == 0 && L <= L0);
+        NodeRec rec;
         if (L == lam) {
             // leaf: the range ends at the first j > k with m_j < lam
             int64_t e = -1;
             for (int j = 1; j <= T && k + j <= n; ++j) {
                 if (k + j == n) { e = n; break; }
-                const uint16_t tj = topo[k + j];
-                if ((int)(tj & 31) - 1 < lam) { e = k + j; break; }
+                if ((int)(topo[k + j] & 31) - 1 < lam) { e = k + j; break; }
             }
-            if (e < 0) {
-                // level-21 leaf holding more than T equal keys
+            if (e < 0)  // level-21 leaf holding more than T equal keys
                 e = lower_bound_pred(k + T + 1, n, [&](int64_t j) { return sk[j] != key; });
-            }
-            E = e;
-        } else {
-            const int sh = 63 - 3 * L;  // prefix_L(key) = key >> sh (sh = 63 at L = 0)
-            const uint64_t pre = sh >= 64 ? 0ull : (sh == 63 ? 0ull : key >> sh);
-            E = lower_bound_pred(k + 1, n, [&](int64_t j) {
-                const uint64_t pj = (sh >= 63) ? 0ull : (sk[j] >> sh);
-                return pj != pre;
-            });
-        }
-        const int32_t cnt = (int32_t)(E - k);
-        nfirst[id] = (int32_t)k + index_base;
-        nlast[id] = (int32_t)(E - 1) + index_base;
-        NodeRec rec;
-        if (L == lam) {
-            for (int o = 0; o < 8; ++o) nchild[8 * (int64_t)id + o] = -1;
+            const int32_t cnt = (int32_t)(e - k);
+            nlast[id] = (int32_t)(e - 1) + index_base;
             uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
             uint32_t hi3[3] = {0u, 0u, 0u};
-            for (int64_t j = k; j < E; ++j) {
+            for (int64_t j = k; j < e; ++j) {
                 const float4 p = scoord[j];
                 const float v[3] = {p.x, p.y, p.z};
                 for (int d = 0; d < 3; ++d) {
@@ -413,97 +411,81 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
             }
             rec.lo = make_float4(o2f(lo3[0]), o2f(lo3[1]), o2f(lo3[2]), __int_as_float((int)k));
             rec.hi = make_float4(o2f(hi3[0]), o2f(hi3[1]), o2f(hi3[2]), __int_as_float(cnt));
-            const int32_t lr = (int32_t)(hoff[(int64_t)OCT_NLEVELS * nblk + blockIdx.x] - (uint32_t)M +
-                                         wave_off(OCT_NLEVELS) + rank_in_wave[OCT_NLEVELS]);
+            const int32_t lr = before(OCT_NLEVELS) - M;
             leaf_ids[lr] = id;
+            if (chain) prm->branch_root = id;  // everything sits in one leaf
         } else {
-            // children: one per non-empty octant of the level-L split, ids
-            // consecutive from the level-(L+1) node starting here
-            const int32_t c0 = node_id(L + 1);
-            const int sh = level_shift(L);
-            const int o0 = (int)((key >> sh) & 7ull);
-            int nc = 0;
-            int64_t pos = k;
-            for (int o = 0; o < 8; ++o) {
-                if (o < o0) { nchild[8 * (int64_t)id + o] = -1; continue; }
-                int64_t s0 = (o == o0) ? k
-                    : lower_bound_pred(pos, E, [&](int64_t j) { return (int)((sk[j] >> sh) & 7ull) >= o; });
-                bool nonempty = s0 < E && (int)((sk[s0] >> sh) & 7ull) == o;
-                if (nonempty) {
-                    const int32_t cid = c0 + nc;
-                    nchild[8 * (int64_t)id + o] = cid + index_base;
-                    if (o != o0) parent[cid] = id;  // the first child is linked by its own thread
-                    ++nc;
-                    pos = s0;
-                } else {
-                    nchild[8 * (int64_t)id + o] = -1;
-                }
-            }
-            rec.lo = make_float4(0.f, 0.f, 0.f, __int_as_float(c0));
-            rec.hi = make_float4(0.f, 0.f, 0.f, __int_as_float(-nc));
-            if ((int64_t)cnt == n) {
+            rec.lo = make_float4(0.f, 0.f, 0.f, __int_as_float(before(L + 1)));  // first child
+            rec.hi = make_float4(0.f, 0.f, 0.f, __int_as_float(0));             // -children, from k_link
+            if (chain) {
                 rec.lo.x = o2f(prm->omin[0]); rec.lo.y = o2f(prm->omin[1]); rec.lo.z = o2f(prm->omin[2]);
                 rec.hi.x = o2f(prm->omax[0]); rec.hi.y = o2f(prm->omax[1]); rec.hi.z = o2f(prm->omax[2]);
-                if (nc > 1) prm->branch_root = id;
+                nlast[id] = (int32_t)(n - 1) + index_base;
+                if (L == L0) prm->branch_root = id;
             }
         }
-        if (L == lam && (int64_t)cnt == n) prm->branch_root = id;  // root or chain ends in a leaf
         nodes[id] = rec;
-        if (L == lam || (int64_t)cnt == n) {
+        if (L == lam || chain) {
             bmin[3 * (int64_t)id + 0] = rec.lo.x; bmin[3 * (int64_t)id + 1] = rec.lo.y; bmin[3 * (int64_t)id + 2] = rec.lo.z;
             bmax[3 * (int64_t)id + 0] = rec.hi.x; bmax[3 * (int64_t)id + 1] = rec.hi.y; bmax[3 * (int64_t)id + 2] = rec.hi.z;
         }
     }
 }
 
-// Links each level-(L) node that starts a new parent range (L == m_k + 1,
-// k > 0) to its parent: that parent started earlier and wrote parent[] for
-// all its children except the first, so nothing is left to do here; the
-// first child of a node shares its start position and is linked above.
-//
-// Bottom-up bounds: one thread per leaf climbs while it is the last child to
-// finish; the union of the children's bounds is exact and order-free.
+// Every non-root node writes its id into its parent's octant slot and counts
+// itself into the parent's record (b = -children).
 __global__ void __launch_bounds__(256)
-k_refit(const int32_t* __restrict__ leaf_ids, int32_t n_leaves_max, const BuildParams* __restrict__ prm,
-        int64_t n, NodeRec* __restrict__ nodes, const int32_t* __restrict__ parent,
-        int32_t* __restrict__ visit, float* __restrict__ bmin, float* __restrict__ bmax,
-        const int32_t* __restrict__ nfirst, const int32_t* __restrict__ nlast) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t n_leaves = prm->n_leaves;
-    if (r >= n_leaves) return;
-    int32_t id = leaf_ids[r];
-    int32_t p = parent[id];
-    while (p >= 0) {
-        if ((int64_t)(nlast[p] - nfirst[p]) + 1 == n) break;  // holds the global bounds already
-        const int nc = -rec_b(nodes[p]);
-        __threadfence();
-        const int old = atomicAdd(&visit[p], 1);
-        if (old != nc - 1) break;
-        __threadfence();
-        const int c0 = rec_a(nodes[p]);
+k_link(const uint64_t* __restrict__ sk, const BuildParams* __restrict__ prm, int index_base,
+       const int32_t* __restrict__ parent, const int32_t* __restrict__ nfirst, const uint8_t* __restrict__ nlevel,
+       int32_t* __restrict__ nchild, NodeRec* __restrict__ nodes) {
+    const int32_t M = prm->n_nodes;
+    for (int32_t id = 1 + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); id < M;
+         id += (int32_t)(gridDim.x * blockDim.x)) {
+        const int32_t p = parent[id];
+        const int L = nlevel[id];
+        const uint64_t key = sk[nfirst[id] - index_base];
+        const int o = (int)((key >> level_shift(L - 1)) & 7ull);
+        nchild[8 * (int64_t)p + o] = id + index_base;
+        atomicSub((int*)&nodes[p].hi.w, 1);
+    }
+}
+
+// Internal bounds and range ends for the nodes of one level, from their
+// children (finished by the previous launch: launches go deepest first).
+__global__ void __launch_bounds__(256)
+k_level_refit(int L, const uint32_t* __restrict__ hoff, int nblk, const BuildParams* __restrict__ prm,
+              int index_base, NodeRec* __restrict__ nodes, int32_t* __restrict__ nlast,
+              const int32_t* __restrict__ nfirst, float* __restrict__ bmin, float* __restrict__ bmax) {
+    const int32_t lo = (int32_t)hoff[(int64_t)L * nblk];
+    const int32_t hi = (int32_t)hoff[(int64_t)(L + 1) * nblk];
+    const int L0 = prm->chain_level;
+    for (int32_t id = lo + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); id < hi;
+         id += (int32_t)(gridDim.x * blockDim.x)) {
+        NodeRec rec = nodes[id];
+        const int nc = -rec_b(rec);
+        if (nc <= 0) continue;                                  // leaf
+        if (L <= L0 && nfirst[id] == index_base) continue;     // chain: global bounds already
+        const int c0 = rec_a(rec);
         uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
         uint32_t hi3[3] = {0u, 0u, 0u};
         for (int c = 0; c < nc; ++c) {
-            const float* cl = &nodes[c0 + c].lo.x;
-            const float* ch = &nodes[c0 + c].hi.x;
+            const NodeRec ch = nodes[c0 + c];
+            const float a[3] = {ch.lo.x, ch.lo.y, ch.lo.z};
+            const float b[3] = {ch.hi.x, ch.hi.y, ch.hi.z};
             for (int d = 0; d < 3; ++d) {
-                const uint32_t a = f2o(load_coherent(cl + d));
-                const uint32_t b = f2o(load_coherent(ch + d));
-                lo3[d] = a < lo3[d] ? a : lo3[d];
-                hi3[d] = b > hi3[d] ? b : hi3[d];
+                const uint32_t x = f2o(a[d]), y = f2o(b[d]);
+                lo3[d] = x < lo3[d] ? x : lo3[d];
+                hi3[d] = y > hi3[d] ? y : hi3[d];
             }
         }
-        float* pl = &nodes[p].lo.x;
-        float* ph = &nodes[p].hi.x;
+        rec.lo.x = o2f(lo3[0]); rec.lo.y = o2f(lo3[1]); rec.lo.z = o2f(lo3[2]);
+        rec.hi.x = o2f(hi3[0]); rec.hi.y = o2f(hi3[1]); rec.hi.z = o2f(hi3[2]);
+        nodes[id] = rec;
+        nlast[id] = nlast[c0 + nc - 1];
         for (int d = 0; d < 3; ++d) {
-            const float a = o2f(lo3[d]), b = o2f(hi3[d]);
-            pl[d] = a;
-            ph[d] = b;
-            bmin[3 * (int64_t)p + d] = a;
-            bmax[3 * (int64_t)p + d] = b;
+            bmin[3 * (int64_t)id + d] = d == 0 ? rec.lo.x : (d == 1 ? rec.lo.y : rec.lo.z);
+            bmax[3 * (int64_t)id + d] = d == 0 ? rec.hi.x : (d == 1 ? rec.hi.y : rec.hi.z);
         }
-        id = p;
-        p = parent[id];
     }
 }
 
@@ -511,14 +493,7 @@ __global__ void k_set_leaf_count(BuildParams* prm, const uint64_t* total) {
     if (threadIdx.x == 0 && blockIdx.x == 0) prm->n_leaves = (int32_t)(*total) - prm->n_nodes;
 }
 
-__global__ void k_zero_u32(uint32_t* p, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
 // ============================================================== driver
-
-
 
 int bounds_blocks(int64_t n) {
     int64_t b = (n + BND_THREADS * 8 - 1) / (BND_THREADS * 8);
@@ -537,8 +512,9 @@ int64_t max_nodes(int64_t n, int T) { return n + 21 * (n / (T + 1)) + 22; }
 // Stage 1: bounds reduction (the caller reads prm back to learn key_bits).
 hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, const BuildBufs& b,
                                hipStream_t s) {
-    hipLaunchKernelGGL(k_bounds, dim3(bounds_blocks(n)), dim3(BND_THREADS), 0, s, coords, n, radius,
-                       b.partials, b.prm);
+    const int nb = bounds_blocks(n);
+    hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(BND_THREADS), 0, s, coords, n, b.partials);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BND_THREADS), 0, s, b.partials, nb, radius, b.prm);
     return hipGetLastError();
 }
 
@@ -568,17 +544,17 @@ hipError_t launch_build_stage(const float* coords, int64_t n, int key_bits, int 
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, b.scoord, n, T, b.topo, b.hist,
                        nblk, b.prm, index_base, b.nodes, b.nfirst, b.nlast, b.nchild, b.bmin, b.bmax,
-                       b.parent, b.visit, b.leaf_ids);
+                       b.parent, b.nlevel, b.leaf_ids);
     hipLaunchKernelGGL(k_set_leaf_count, dim3(1), dim3(64), 0, s, b.prm, b.total);
-    hipLaunchKernelGGL(k_refit, dim3(g), dim3(256), 0, s, b.leaf_ids, (int32_t)n, b.prm, n, b.nodes,
-                       b.parent, b.visit, b.bmin, b.bmax, b.nfirst, b.nlast);
+    const unsigned gl = (unsigned)std::min<int64_t>((max_nodes(n, T) + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_link, dim3(gl), dim3(256), 0, s, sk, b.prm, index_base, b.parent, b.nfirst, b.nlevel,
+                       b.nchild, b.nodes);
+    // internal levels lie strictly below the chain; the chain ends at or below 21 - key_bits / 3
+    const int lmin = 21 - key_bits / 3;
+    for (int L = OCT_MAX_LEVEL - 1; L >= (lmin < 0 ? 0 : lmin); --L)
+        hipLaunchKernelGGL(k_level_refit, dim3(256), dim3(256), 0, s, L, b.hist, nblk, b.prm, index_base, b.nodes,
+                           b.nlast, b.nfirst, b.bmin, b.bmax);
     return hipGetLastError();
-}
-
-const uint64_t* sorted_keys(const BuildBufs& b, int key_bits) {
-    int passes = (key_bits + RS_BITS - 1) / RS_BITS;
-    if (passes < 1) passes = 1;
-    return b.keys[passes & 1];
 }
 
 }  // namespace oct

@@ -51,7 +51,7 @@ struct SetState {
     int key_bits = 0;
     BuildParams hp{};  // host copy after the last build
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
-        nfirst, nlast, nchild, parent, visit, leaf_ids, bmin, bmax;
+        nfirst, nlast, nchild, parent, nlevel, leaf_ids, bmin, bmax;
     BuildBufs bufs() const {
         BuildBufs b;
         b.prm = prm.as<BuildParams>();
@@ -71,7 +71,7 @@ struct SetState {
         b.nlast = nlast.as<int32_t>();
         b.nchild = nchild.as<int32_t>();
         b.parent = parent.as<int32_t>();
-        b.visit = visit.as<int32_t>();
+        b.nlevel = nlevel.as<uint8_t>();
         b.leaf_ids = leaf_ids.as<int32_t>();
         b.bmin = bmin.as<float>();
         b.bmax = bmax.as<float>();
@@ -79,7 +79,7 @@ struct SetState {
     }
     void release_all() {
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
-                         &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &visit, &leaf_ids, &bmin, &bmax};
+                         &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &bmin, &bmax};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -230,7 +230,7 @@ int build_set(oct_ctx* c, SetState& s) {
     HIP_TRY(c, s.nlast.ensure(4 * (size_t)M));
     HIP_TRY(c, s.nchild.ensure(32 * (size_t)M));
     HIP_TRY(c, s.parent.ensure(4 * (size_t)M));
-    HIP_TRY(c, s.visit.ensure(4 * (size_t)M));
+    HIP_TRY(c, s.nlevel.ensure((size_t)M));
     HIP_TRY(c, s.leaf_ids.ensure(4 * (size_t)n));
     HIP_TRY(c, s.bmin.ensure(12 * (size_t)M));
     HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
@@ -445,18 +445,20 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
     // flags[2..3]: 64-bit total of the per-query counts
     uint32_t* flags = p.flags.as<uint32_t>();
     uint32_t host_flags[4] = {0, 0, 0, 0};
+    int safe = 0;
+    bool settled = false;
     for (int attempt = 0; attempt < 8; ++attempt) {
         HIP_TRY(c, p.lists.ensure(sizeof(int2) * (size_t)nql * (size_t)p.cap));
         HIP_TRY(c, launch_fill_u32(flags, 4, 0u, c->stream));
         {
             Phase ph(c, OCT_PROF_LEAF_LISTS);
-            HIP_TRY(c, launch_leaf_lists(qv, tv, r_sq, p.cap, p.lists.as<int2>(), p.lcount.as<int32_t>(), flags,
-                                         c->stream));
+            HIP_TRY(c, launch_leaf_lists(qv, tv, r_sq, p.cap, safe, p.lists.as<int2>(), p.lcount.as<int32_t>(),
+                                         flags, c->stream));
         }
         {
             Phase ph(c, OCT_PROF_COUNT);
-            HIP_TRY(c, launch_edges(0, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r,
-                                    self_pair, by_receiver, p.qcount.as<uint32_t>(), nullptr, EdgeOut{},
+            HIP_TRY(c, launch_edges(EDGE_COUNT, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
+                                    inv_r, self_pair, by_receiver, p.qcount.as<uint32_t>(), nullptr, 0, EdgeOut{},
                                     c->index_base, c->stream));
         }
         {
@@ -467,10 +469,11 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
         HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 16, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         memcpy(host_flags, c->pinned, 16);
-        if (host_flags[0] == 0) break;
-        p.cap = (int)std::min<uint32_t>(host_flags[0] + 16u, (uint32_t)INT32_MAX / 8u);
+        if (host_flags[0] == 0 && host_flags[1] == 0) { settled = true; break; }
+        if (host_flags[0] != 0) p.cap = (int)std::min<uint32_t>(host_flags[0] + 16u, (uint32_t)INT32_MAX / 8u);
+        if (host_flags[1] != 0) safe = 1;
     }
-    if (host_flags[0] != 0) return fail(c, OCT_ERR_DEVICE, "candidate list capacity did not settle");
+    if (!settled) return fail(c, OCT_ERR_DEVICE, "candidate lists did not settle");
     uint64_t total = 0;
     memcpy(&total, host_flags + 2, 8);
     // Int32 indices and counts, as in the reference (src/types.jl:89)
@@ -484,8 +487,9 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
     EdgeOut eo{p.senders.as<int32_t>(), p.receivers.as<int32_t>(), p.rdisp.as<float>(), p.rdist.as<float>()};
     if (E > 0) {
         Phase ph(c, OCT_PROF_FILL);
-        HIP_TRY(c, launch_edges(1, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r, self_pair,
-                                by_receiver, nullptr, p.qoff.as<uint32_t>(), eo, c->index_base, c->stream));
+        HIP_TRY(c, launch_edges(EDGE_FILL, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r,
+                                self_pair, by_receiver, nullptr, p.qoff.as<uint32_t>(), E, eo, c->index_base,
+                                c->stream));
     }
     p.n_edges = E;
     p.materialized = true;
@@ -560,6 +564,12 @@ int oct_build_edges_backward(oct_ctx* c, int64_t n_points, const oct_edges* e, c
 int oct_profile(oct_ctx* c, int enable) {
     if (!c) return OCT_ERR_ARGUMENT;
     c->profiling = enable != 0;
+    // events are created up front so recording them costs no allocation
+    while (c->profiling && c->spare.size() < 256) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(c, hipEventCreate(&e));
+        c->spare.push_back(e);
+    }
     return OCT_OK;
 }
 

@@ -5,156 +5,216 @@
 // target leaves whose box lies within r of its box (FP32 box-box gap, a
 // monotone lower bound of every point distance between the two boxes, so no
 // edge is lost).  Stage 2 runs one wavefront per query leaf: lanes take
-// candidate target points (coalesced from the Morton-sorted copy), the wave
-// loops over the leaf's queries, and each (query, 64 candidates) step is one
-// ballot.  The predicate is the reference's FP32 test
+// candidate target points (coalesced from the Morton-sorted copy), the leaf's
+// queries sit one per lane and are broadcast with readlane, and each
+// (query, 64 candidates) step is one ballot.  The predicate is the
+// reference's FP32 test
 //   d2 = (dx*dx + dy*dy) + dz*dz <= r*r,
-// so the edge set is bit-identical to the CPU path.
+// so the edge set is bit-identical to the CPU path.  The emit pass stages a
+// leaf's edges in LDS and writes them out with whole-wave contiguous stores.
 #include "internal.hpp"
 
 namespace oct {
 
-constexpr int STACK_DEPTH = 1 + 7 * OCT_MAX_LEVEL;  // 148 (src/cpu/build_cpu.jl:25-43)
-
 // ---------------------------------------------------------------- stage 1
 
+static_assert(1 + 7 * OCT_MAX_LEVEL <= 192, "LIFO walk bound (src/cpu/build_cpu.jl:25-43) must fit the stack");
+constexpr int LL_STACK = 192;                              // room for multi-pops
+constexpr int LL_POP = 2;
+
 // One thread per query leaf: depth-first walk of the target tree from its
-// branch root, testing children before pushing them; leaves that pass are
-// recorded as (first sorted position, count).
+// branch root.  A stack entry is the child range of a node that passed the
+// test, so each round costs one dependent load; up to LL_POP entries are
+// expanded per round to overlap their loads.  Popping one entry at a time is
+// plain LIFO depth-first order, whose stack never exceeds 1 + 7 * 21 entries
+// (the reference's bound); multi-pop rounds are only taken while the stack
+// has room for all their pushes.  Every push is bounds-checked: a walk that
+// would still run out of room stops, and the leaf is flagged so the host
+// reruns the lists with safe = 1 (single pops, inside the proven bound).
 __global__ void __launch_bounds__(256)
 k_leaf_lists(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qleaf_ids,
              const BuildParams* __restrict__ qprm, const NodeRec* __restrict__ tnodes,
-             const BuildParams* __restrict__ tprm, float r_sq, int cap, int2* __restrict__ lists,
-             int32_t* __restrict__ lcount, uint32_t* __restrict__ overflow) {
-    const int32_t r = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (r >= qprm->n_leaves) return;
-    const NodeRec qa = qnodes[qleaf_ids[r]];
-    int2* out = lists + (int64_t)r * cap;
-    int cnt = 0;
+             const BuildParams* __restrict__ tprm, float r_sq, int cap, int safe, int2* __restrict__ lists,
+             int32_t* __restrict__ lcount, uint32_t* __restrict__ flags) {
+    const int32_t nleaves = qprm->n_leaves;
     const int32_t root = tprm->branch_root;
-    const NodeRec rr = tnodes[root];
-    if (box_box_d2(qa.lo, qa.hi, rr.lo, rr.hi) <= r_sq) {
-        if (rec_b(rr) > 0) {
-            if (cnt < cap) out[cnt] = make_int2(rec_a(rr), rec_b(rr));
-            ++cnt;
-        } else {
-            int32_t stack[STACK_DEPTH];
-            int sp = 0;
-            stack[sp++] = root;
-            while (sp > 0) {
-                const NodeRec nd = tnodes[stack[--sp]];
-                const int c0 = rec_a(nd), nc = -rec_b(nd);
-                for (int c = 0; c < nc; ++c) {
-                    const NodeRec ch = tnodes[c0 + c];
-                    if (box_box_d2(qa.lo, qa.hi, ch.lo, ch.hi) > r_sq) continue;
-                    if (rec_b(ch) > 0) {
-                        if (cnt < cap) out[cnt] = make_int2(rec_a(ch), rec_b(ch));
-                        ++cnt;
-                    } else {
-                        stack[sp++] = c0 + c;
+    for (int32_t r = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); r < nleaves;
+         r += (int32_t)(gridDim.x * blockDim.x)) {
+        const NodeRec qa = qnodes[qleaf_ids[r]];
+        int2* out = lists + (int64_t)r * cap;
+        int cnt = 0;
+        bool stack_full = false;
+        const NodeRec rr = tnodes[root];
+        if (box_box_d2(qa.lo, qa.hi, rr.lo, rr.hi) <= r_sq) {
+            if (rec_b(rr) > 0) {
+                if (cnt < cap) out[cnt] = make_int2(rec_a(rr), rec_b(rr));
+                ++cnt;
+            } else {
+                int2 stack[LL_STACK];
+                int sp = 0;
+                stack[sp++] = make_int2(rec_a(rr), -rec_b(rr));
+                while (sp > 0 && !stack_full) {
+                    int k = 1;
+                    if (!safe) {
+                        k = sp < LL_POP ? sp : LL_POP;
+                        while (k > 1 && sp - k + 8 * k > LL_STACK) --k;  // pushes must fit
                     }
+                    int2 ent[LL_POP];
+#pragma unroll
+                    for (int i = 0; i < LL_POP; ++i) ent[i] = i < k ? stack[sp - 1 - i] : make_int2(0, 0);
+                    sp -= k;
+                    NodeRec ch[LL_POP][8];
+#pragma unroll
+                    for (int i = 0; i < LL_POP; ++i)
+#pragma unroll
+                        for (int c = 0; c < 8; ++c)
+                            if (c < ent[i].y) ch[i][c] = tnodes[ent[i].x + c];
+#pragma unroll
+                    for (int i = 0; i < LL_POP; ++i)
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) {
+                            if (c >= ent[i].y) continue;
+                            if (box_box_d2(qa.lo, qa.hi, ch[i][c].lo, ch[i][c].hi) > r_sq) continue;
+                            if (rec_b(ch[i][c]) > 0) {
+                                if (cnt < cap) out[cnt] = make_int2(rec_a(ch[i][c]), rec_b(ch[i][c]));
+                                ++cnt;
+                            } else if (sp < LL_STACK) {
+                                stack[sp++] = make_int2(rec_a(ch[i][c]), -rec_b(ch[i][c]));
+                            } else {
+                                stack_full = true;
+                            }
+                        }
                 }
             }
         }
+        lcount[r] = cnt;
+        if (cnt > cap) atomicMax(&flags[0], (uint32_t)cnt);  // the host regrows cap to this
+        if (stack_full) atomicOr(&flags[1], 1u);             // the host reruns with safe = 1
     }
-    lcount[r] = cnt;
-    if (cnt > cap) atomicMax(overflow, (uint32_t)cnt);  // the host regrows cap to this
 }
 
 // ---------------------------------------------------------------- stage 2
 
-enum { EDGE_COUNT = 0, EDGE_FILL = 1 };
-
-
-
-constexpr int EW_WAVES = 4;  // waves per block, one query leaf each
+constexpr int EW_WAVES = 4;     // waves per block, each owns one query leaf at a time
+constexpr int EW_STAGE = 512;   // edges one wave stages in LDS (12 KB)
 
 template <int MODE>
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
 k_edges(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qleaf_ids,
-        const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm,
+        const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm, int64_t nq,
         const float4* __restrict__ tsc, const int2* __restrict__ lists,
         const int32_t* __restrict__ lcount, int cap, float r_sq, float inv_r, int self_pair,
         int by_receiver, uint32_t* __restrict__ qcount, const uint32_t* __restrict__ qoff,
-        EdgeOut out, int index_base) {
+        int64_t n_edges, EdgeOut out, int index_base) {
     __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
     __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
+    constexpr int STAGE = MODE == EDGE_FILL ? EW_STAGE : 1;
+    __shared__ int32_t s_sen[EW_WAVES][STAGE];
+    __shared__ int32_t s_rcv[EW_WAVES][STAGE];
+    __shared__ float s_disp[EW_WAVES][3 * STAGE];
+    __shared__ float s_dist[EW_WAVES][STAGE];
     const int wv = threadIdx.x / OCT_WAVE;
     const int lane = lane_id();
-    const int32_t r = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * EW_WAVES + wv));
-    if (r >= qprm->n_leaves) return;
-    const NodeRec qa = qnodes[qleaf_ids[r]];
-    const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
-    const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
-    const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
-    const int2* lst = lists + (int64_t)r * cap;
-
-    for (int qb = 0; qb < qc; qb += OCT_WAVE) {
-        const int nqb = min(OCT_WAVE, qc - qb);
-        // per-query state lives in lane (qi - qb)
-        uint32_t my = 0;
-        if (MODE == EDGE_FILL && lane < nqb) {
-            const float4 p = qsc[qf + qb + lane];
-            const int32_t slot = by_receiver ? __float_as_int(p.w) : qf + qb + lane;
-            my = qoff[slot];
-        }
-        for (int g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
-            int2 lf = make_int2(0, 0);
-            if (g0 + lane < ncand) lf = lst[g0 + lane];
-            const int incl = wave_incl_scan(lf.y);
-            const int total = __builtin_amdgcn_readlane(incl, OCT_WAVE - 1);
-            s_start[wv][lane + 1] = incl;
-            if (lane == 0) s_start[wv][0] = 0;
-            s_first[wv][lane] = lf.x;
-            __builtin_amdgcn_wave_barrier();
-            for (int c0 = 0; c0 < total; c0 += OCT_WAVE) {
-                const int ci = c0 + lane;
-                const bool valid = ci < total;
-                float4 tp = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (valid) {
-                    // owner leaf: last o with s_start[o] <= ci
-                    int lo = 0, hi = OCT_WAVE;
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (s_start[wv][mid] <= ci) lo = mid;
-                        else hi = mid;
+    const int32_t nleaves = qprm->n_leaves;
+    const int32_t nwaves = (int32_t)(gridDim.x * EW_WAVES);
+    for (int32_t r = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * EW_WAVES + wv)); r < nleaves; r += nwaves) {
+        const NodeRec qa = qnodes[qleaf_ids[r]];
+        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+        const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+        const int2* lst = lists + (int64_t)r * cap;
+        for (int qb = 0; qb < qc; qb += OCT_WAVE) {
+            const int nqb = min(OCT_WAVE, qc - qb);
+            // query qb + lane lives in this lane
+            float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lane < nqb) myq = qsc[qf + qb + lane];
+            const int32_t slot = by_receiver ? __float_as_int(myq.w) : qf + qb + lane;
+            uint32_t my = 0;  // COUNT: count; FILL: next output position of this lane's query
+            if (MODE == EDGE_FILL && lane < nqb) my = qoff[slot];
+            // leaf order: this block of queries owns [blk_base, blk_base + blk_tot)
+            uint32_t blk_base = 0, blk_tot = 0;
+            bool staged = false;
+            if (MODE == EDGE_FILL && !by_receiver) {
+                blk_base = (uint32_t)__builtin_amdgcn_readfirstlane((int)my);
+                const int64_t nxt = (int64_t)qf + qb + nqb;
+                const uint32_t blk_end = nxt < nq ? qoff[nxt] : (uint32_t)n_edges;
+                blk_tot = blk_end - blk_base;
+                staged = blk_tot <= (uint32_t)EW_STAGE;
+            }
+            for (int g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
+                int2 lf = make_int2(0, 0);
+                if (g0 + lane < ncand) lf = lst[g0 + lane];
+                const int incl = wave_incl_scan(lf.y);
+                const int total = __builtin_amdgcn_readlane(incl, OCT_WAVE - 1);
+                s_start[wv][lane + 1] = incl;
+                if (lane == 0) s_start[wv][0] = 0;
+                s_first[wv][lane] = lf.x;
+                __builtin_amdgcn_wave_barrier();
+                for (int c0 = 0; c0 < total; c0 += OCT_WAVE) {
+                    const int ci = c0 + lane;
+                    const bool valid = ci < total;
+                    float4 tp = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (valid) {
+                        int lo = 0, hi = OCT_WAVE;  // owner: last o with s_start[o] <= ci
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (s_start[wv][mid] <= ci) lo = mid;
+                            else hi = mid;
+                        }
+                        tp = tsc[s_first[wv][lo] + (ci - s_start[wv][lo])];
                     }
-                    tp = tsc[s_first[wv][lo] + (ci - s_start[wv][lo])];
-                }
-                const int32_t j = __float_as_int(tp.w);
-                for (int qi = 0; qi < nqb; ++qi) {
-                    const float4 qp = qsc[qf + qb + qi];
-                    const int32_t i = __float_as_int(qp.w);
-                    const float dx = qp.x - tp.x;
-                    const float dy = qp.y - tp.y;
-                    const float dz = qp.z - tp.z;
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    const bool pass = valid && d2 <= r_sq && !(self_pair && j == i);
-                    const uint64_t m = ballot(pass);
-                    const uint32_t pc = (uint32_t)__popcll(m);
-                    if (MODE == EDGE_COUNT) {
-                        if (lane == qi) my += pc;
-                    } else {
-                        const uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)my, qi);
-                        if (pass) {
-                            const int64_t e = (int64_t)base + (uint32_t)mbcnt(m);
-                            out.senders[e] = j + index_base;
-                            out.receivers[e] = i + index_base;
-                            out.rdisp[3 * e + 0] = dx * inv_r;
-                            out.rdisp[3 * e + 1] = dy * inv_r;
-                            out.rdisp[3 * e + 2] = dz * inv_r;
-                            out.rdist[e] = sqrtf(d2) * inv_r;
+                    const int32_t j = __float_as_int(tp.w);
+                    for (int qi = 0; qi < nqb; ++qi) {
+                        const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
+                        const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
+                        const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
+                        const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
+                        const float dx = qx - tp.x;
+                        const float dy = qy - tp.y;
+                        const float dz = qz - tp.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        const bool pass = valid && d2 <= r_sq && !(self_pair && j == i);
+                        const uint64_t m = ballot(pass);
+                        const uint32_t pc = (uint32_t)__popcll(m);
+                        if (MODE == EDGE_FILL && pc != 0) {
+                            const uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)my, qi);
+                            if (pass) {
+                                const uint32_t e = base + (uint32_t)mbcnt(m);
+                                if (staged) {
+                                    const uint32_t s = e - blk_base;
+                                    s_sen[wv][s] = j + index_base;
+                                    s_rcv[wv][s] = i + index_base;
+                                    s_disp[wv][3 * s + 0] = dx * inv_r;
+                                    s_disp[wv][3 * s + 1] = dy * inv_r;
+                                    s_disp[wv][3 * s + 2] = dz * inv_r;
+                                    s_dist[wv][s] = sqrtf(d2) * inv_r;
+                                } else {
+                                    out.senders[e] = j + index_base;
+                                    out.receivers[e] = i + index_base;
+                                    out.rdisp[3 * (int64_t)e + 0] = dx * inv_r;
+                                    out.rdisp[3 * (int64_t)e + 1] = dy * inv_r;
+                                    out.rdisp[3 * (int64_t)e + 2] = dz * inv_r;
+                                    out.rdist[e] = sqrtf(d2) * inv_r;
+                                }
+                            }
                         }
                         if (lane == qi) my += pc;
                     }
                 }
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
-        }
-        if (MODE == EDGE_COUNT && lane < nqb) {
-            const float4 p = qsc[qf + qb + lane];
-            const int32_t slot = by_receiver ? __float_as_int(p.w) : qf + qb + lane;
-            qcount[slot] = my;
+            if (MODE == EDGE_COUNT && lane < nqb) qcount[slot] = my;
+            if (MODE == EDGE_FILL && staged) {
+                __builtin_amdgcn_wave_barrier();
+                for (uint32_t t = lane; t < blk_tot; t += OCT_WAVE) {
+                    out.senders[blk_base + t] = s_sen[wv][t];
+                    out.receivers[blk_base + t] = s_rcv[wv][t];
+                    out.rdist[blk_base + t] = s_dist[wv][t];
+                }
+                for (uint32_t t = lane; t < 3 * blk_tot; t += OCT_WAVE)
+                    out.rdisp[3 * (int64_t)blk_base + t] = s_disp[wv][t];
+                __builtin_amdgcn_wave_barrier();
+            }
         }
     }
 }
@@ -166,29 +226,29 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
 
 // ------------------------------------------------------------- launchers
 
-hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int2* lists,
-                             int32_t* lcount, uint32_t* overflow, hipStream_t s) {
-    if (q.n_leaves <= 0) return hipSuccess;
-    const unsigned g = (unsigned)((q.n_leaves + 255) / 256);
+hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
+                             int32_t* lcount, uint32_t* flags, hipStream_t s) {
+    const int64_t want = (q.n + 255) / 256;  // n_leaves <= n
+    const unsigned g = (unsigned)(want < 1 ? 1 : (want > 4096 ? 4096 : want));
     hipLaunchKernelGGL(k_leaf_lists, dim3(g), dim3(256), 0, s, q.nodes, q.leaf_ids, q.params, t.nodes,
-                       t.params, r_sq, cap, lists, lcount, overflow);
+                       t.params, r_sq, cap, safe, lists, lcount, flags);
     return hipGetLastError();
 }
 
 hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const int2* lists,
                         const int32_t* lcount, int cap, float r_sq, float inv_r, int self_pair,
-                        int by_receiver, uint32_t* qcount, const uint32_t* qoff, EdgeOut out,
+                        int by_receiver, uint32_t* qcount, const uint32_t* qoff, int64_t n_edges, EdgeOut out,
                         int index_base, hipStream_t s) {
-    if (q.n_leaves <= 0) return hipSuccess;
-    const unsigned g = (unsigned)((q.n_leaves + EW_WAVES - 1) / EW_WAVES);
+    const int64_t want = (q.n + EW_WAVES - 1) / EW_WAVES;  // one wave per leaf, n_leaves <= n
+    const unsigned g = (unsigned)(want < 1 ? 1 : (want > 8192 ? 8192 : want));
     if (mode == EDGE_COUNT)
         hipLaunchKernelGGL(k_edges<EDGE_COUNT>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.nodes, q.leaf_ids,
-                           q.scoord, q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,
-                           by_receiver, qcount, qoff, out, index_base);
+                           q.scoord, q.params, q.n, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,
+                           by_receiver, qcount, qoff, n_edges, out, index_base);
     else
         hipLaunchKernelGGL(k_edges<EDGE_FILL>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.nodes, q.leaf_ids,
-                           q.scoord, q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,
-                           by_receiver, qcount, qoff, out, index_base);
+                           q.scoord, q.params, q.n, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,
+                           by_receiver, qcount, qoff, n_edges, out, index_base);
     return hipGetLastError();
 }
 

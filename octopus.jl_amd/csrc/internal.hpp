@@ -1,6 +1,8 @@
 // Internal structures shared by the kernels and the C-ABI host code.
 #pragma once
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace oct {
@@ -18,7 +20,8 @@ struct BuildParams {
     int32_t n_nodes;       // M
     int32_t n_leaves;
     int32_t branch_root;   // first node with count == n that is a leaf or has > 1 child
-    int32_t pad[2];
+    int32_t chain_level;   // L0: levels 0..L0 hold all n points (one node each)
+    int32_t pad;
 };
 
 // Packed traversal record of one node (32 B, two 16-B loads):
@@ -58,7 +61,8 @@ struct BuildBufs {
     int32_t* perm;       // exported, index base applied
     float4* scoord;      // Morton-sorted coords, w = original index bits
     NodeRec* nodes;
-    int32_t *nfirst, *nlast, *nchild, *parent, *visit, *leaf_ids;
+    int32_t *nfirst, *nlast, *nchild, *parent, *leaf_ids;
+    uint8_t* nlevel;
     float *bmin, *bmax;
 };
 
@@ -80,11 +84,14 @@ struct EdgeOut {
     float* rdist;
 };
 
-hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int2* lists,
-                             int32_t* lcount, uint32_t* overflow, hipStream_t s);
+enum { EDGE_COUNT = 0, EDGE_FILL = 1 };
+// flags[0]: largest candidate count of an overflowing query leaf; flags[1]:
+// a walk ran out of stack (rerun with safe = 1)
+hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
+                             int32_t* lcount, uint32_t* flags, hipStream_t s);
 hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const int2* lists,
                         const int32_t* lcount, int cap, float r_sq, float inv_r, int self_pair,
-                        int by_receiver, uint32_t* qcount, const uint32_t* qoff, EdgeOut out,
+                        int by_receiver, uint32_t* qcount, const uint32_t* qoff, int64_t n_edges, EdgeOut out,
                         int index_base, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 
