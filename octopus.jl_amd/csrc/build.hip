@@ -123,7 +123,7 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per tile
 constexpr int RS_WAVES = RS_THREADS / OCT_WAVE;
-constexpr int RS_BITS = 8;
+constexpr int RS_BITS = 9;
 constexpr int RS_BUCKETS = 1 << RS_BITS;
 
 // Tile digit histograms, stored digit-major (hist[d * ntiles + tile]) so one
@@ -194,8 +194,7 @@ k_rs_scatter(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
     }
     __syncthreads();
     // digit starts inside the tile and per-wave offsets inside each digit
-    if (threadIdx.x < RS_BUCKETS) {
-        const int d = threadIdx.x;
+    for (int d = threadIdx.x; d < RS_BUCKETS; d += RS_THREADS) {
         uint32_t run = 0;
         for (int k = 0; k < RS_WAVES; ++k) {
             const uint32_t c = wcount[k][d];
@@ -206,12 +205,13 @@ k_rs_scatter(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
     }
     __syncthreads();
     if (threadIdx.x < OCT_WAVE) {
-        // exclusive scan of the 256 digit totals by one wave (4 per lane)
-        uint32_t t4[4], s = 0;
-        for (int j = 0; j < 4; ++j) { t4[j] = dstart[4 * lane + j]; s += t4[j]; }
-        uint32_t incl = (uint32_t)wave_incl_scan((int)s);
+        // exclusive scan of the digit totals by one wave (RS_PER_LANE per lane)
+        constexpr int RS_PER_LANE = RS_BUCKETS / OCT_WAVE;
+        uint32_t tv[RS_PER_LANE], s = 0;
+        for (int j = 0; j < RS_PER_LANE; ++j) { tv[j] = dstart[RS_PER_LANE * lane + j]; s += tv[j]; }
+        const uint32_t incl = (uint32_t)wave_incl_scan((int)s);
         uint32_t pre = incl - s;
-        for (int j = 0; j < 4; ++j) { dstart[4 * lane + j] = pre; pre += t4[j]; }
+        for (int j = 0; j < RS_PER_LANE; ++j) { dstart[RS_PER_LANE * lane + j] = pre; pre += tv[j]; }
     }
     __syncthreads();
     for (int t = 0; t < RS_ITEMS; ++t) {
@@ -271,9 +271,13 @@ k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, uint16_t* __rest
     bool emit = false;
     if (k < n) {
         mk = m[li];  // -1 at k == 0
-        // lambda: first level whose range starting at k holds <= T points
+        // lambda: first level whose range starting at k holds <= T points.
+        // A repeat of the previous key (mk == 21) starts no node, so its
+        // window minimum is never needed.
         int mw = 127;
-        if (k + T >= n) {
+        if (mk >= OCT_MAX_LEVEL) {
+            mw = OCT_MAX_LEVEL;
+        } else if (k + T >= n) {
             mw = -1;
         } else {
             for (int j = 1; j <= T; ++j) mw = min(mw, (int)m[li + j]);

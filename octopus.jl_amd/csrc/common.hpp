@@ -74,6 +74,19 @@ __device__ inline int shared_levels(uint64_t a, uint64_t b) {
 // (src/cpu/build_cpu.jl:46-49): children of a level-L node differ there.
 __host__ __device__ inline int level_shift(int L) { return 63 - 3 * (L + 1); }
 
+// ------------------------------------------------------------ XCD mapping
+
+// MI355X dispatches workgroups round-robin over its 8 XCDs, each with its own
+// L2.  Remap so XCD x runs the x-th contiguous slice of logical block ids:
+// Morton-adjacent work (which shares tree nodes and candidate points) then
+// stays in one L2.  Bijective on [0, nblocks).
+#define OCT_NXCD 8u
+__device__ inline uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
+    const uint32_t x = bid % OCT_NXCD, k = bid / OCT_NXCD;
+    const uint32_t per = nblocks / OCT_NXCD, rem = nblocks % OCT_NXCD;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
 // ------------------------------------------------------------- wave utils
 
 __device__ inline int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }

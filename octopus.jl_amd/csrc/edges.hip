@@ -38,7 +38,8 @@ k_leaf_lists(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qle
              int32_t* __restrict__ lcount, uint32_t* __restrict__ flags) {
     const int32_t nleaves = qprm->n_leaves;
     const int32_t root = tprm->branch_root;
-    for (int32_t r = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); r < nleaves;
+    const int32_t lb = (int32_t)xcd_block(blockIdx.x, gridDim.x);
+    for (int32_t r = lb * (int32_t)blockDim.x + (int32_t)threadIdx.x; r < nleaves;
          r += (int32_t)(gridDim.x * blockDim.x)) {
         const NodeRec qa = qnodes[qleaf_ids[r]];
         int2* out = lists + (int64_t)r * cap;
@@ -96,8 +97,12 @@ k_leaf_lists(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qle
 // ---------------------------------------------------------------- stage 2
 
 constexpr int EW_WAVES = 4;     // waves per block, each owns one query leaf at a time
-constexpr int EW_STAGE = 512;   // edges one wave stages in LDS (12 KB)
+constexpr int EW_STAGE = 256;   // edges one wave stages in LDS (6 KB)
+constexpr int EW_TILE = 4;      // candidate chunks of 64 held in registers per tile
 
+// One wave per query leaf (grid-stride over a contiguous, XCD-local range of
+// leaves).  Candidates are loaded a tile of 4 x 64 at a time into registers;
+// the loop then runs query-outer, so each query is broadcast once per tile.
 template <int MODE>
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
 k_edges(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qleaf_ids,
@@ -117,7 +122,11 @@ k_edges(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qleaf_id
     const int lane = lane_id();
     const int32_t nleaves = qprm->n_leaves;
     const int32_t nwaves = (int32_t)(gridDim.x * EW_WAVES);
-    for (int32_t r = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * EW_WAVES + wv)); r < nleaves; r += nwaves) {
+    const int32_t w = (int32_t)xcd_block(blockIdx.x, gridDim.x) * EW_WAVES + wv;
+    const int32_t per = (nleaves + nwaves - 1) / nwaves;
+    const int32_t r_lo = __builtin_amdgcn_readfirstlane(w * per);
+    const int32_t r_hi = min(nleaves, r_lo + per);
+    for (int32_t r = r_lo; r < r_hi; ++r) {
         const NodeRec qa = qnodes[qleaf_ids[r]];
         const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
         const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
@@ -150,36 +159,42 @@ k_edges(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qleaf_id
                 if (lane == 0) s_start[wv][0] = 0;
                 s_first[wv][lane] = lf.x;
                 __builtin_amdgcn_wave_barrier();
-                for (int c0 = 0; c0 < total; c0 += OCT_WAVE) {
-                    const int ci = c0 + lane;
-                    const bool valid = ci < total;
-                    float4 tp = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (valid) {
-                        int lo = 0, hi = OCT_WAVE;  // owner: last o with s_start[o] <= ci
-                        while (hi - lo > 1) {
-                            const int mid = (lo + hi) >> 1;
-                            if (s_start[wv][mid] <= ci) lo = mid;
-                            else hi = mid;
+                for (int t0 = 0; t0 < total; t0 += EW_TILE * OCT_WAVE) {
+                    float4 tp[EW_TILE];
+#pragma unroll
+                    for (int u = 0; u < EW_TILE; ++u) {
+                        const int ci = t0 + u * OCT_WAVE + lane;
+                        tp[u] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+                        if (ci < total) {
+                            int lo = 0, hi = OCT_WAVE;  // owner: last o with s_start[o] <= ci
+                            while (hi - lo > 1) {
+                                const int mid = (lo + hi) >> 1;
+                                if (s_start[wv][mid] <= ci) lo = mid;
+                                else hi = mid;
+                            }
+                            tp[u] = tsc[s_first[wv][lo] + (ci - s_start[wv][lo])];
                         }
-                        tp = tsc[s_first[wv][lo] + (ci - s_start[wv][lo])];
                     }
-                    const int32_t j = __float_as_int(tp.w);
+                    const int nu = min(EW_TILE, (total - t0 + OCT_WAVE - 1) / OCT_WAVE);
                     for (int qi = 0; qi < nqb; ++qi) {
                         const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
                         const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
                         const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
                         const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
-                        const float dx = qx - tp.x;
-                        const float dy = qy - tp.y;
-                        const float dz = qz - tp.z;
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        const bool pass = valid && d2 <= r_sq && !(self_pair && j == i);
-                        const uint64_t m = ballot(pass);
-                        const uint32_t pc = (uint32_t)__popcll(m);
-                        if (MODE == EDGE_FILL && pc != 0) {
-                            const uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)my, qi);
-                            if (pass) {
-                                const uint32_t e = base + (uint32_t)mbcnt(m);
+                        uint32_t run = MODE == EDGE_FILL ? (uint32_t)__builtin_amdgcn_readlane((int)my, qi) : 0u;
+                        const uint32_t run0 = run;
+#pragma unroll
+                        for (int u = 0; u < EW_TILE; ++u) {
+                            if (u >= nu) break;
+                            const int32_t j = __float_as_int(tp[u].w);
+                            const float dx = qx - tp[u].x;
+                            const float dy = qy - tp[u].y;
+                            const float dz = qz - tp[u].z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            const bool pass = j >= 0 && d2 <= r_sq && !(self_pair && j == i);
+                            const uint64_t m = ballot(pass);
+                            if (MODE == EDGE_FILL && pass) {
+                                const uint32_t e = run + (uint32_t)mbcnt(m);
                                 if (staged) {
                                     const uint32_t s = e - blk_base;
                                     s_sen[wv][s] = j + index_base;
@@ -197,8 +212,9 @@ k_edges(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qleaf_id
                                     out.rdist[e] = sqrtf(d2) * inv_r;
                                 }
                             }
+                            run += (uint32_t)__popcll(m);
                         }
-                        if (lane == qi) my += pc;
+                        if (lane == qi) my += run - run0;
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -228,8 +244,8 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
 
 hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
                              int32_t* lcount, uint32_t* flags, hipStream_t s) {
-    const int64_t want = (q.n + 255) / 256;  // n_leaves <= n
-    const unsigned g = (unsigned)(want < 1 ? 1 : (want > 4096 ? 4096 : want));
+    const int64_t want = ((int64_t)q.n_leaves + 255) / 256;
+    const unsigned g = (unsigned)(want < 1 ? 1 : (want > 65535 ? 65535 : want));
     hipLaunchKernelGGL(k_leaf_lists, dim3(g), dim3(256), 0, s, q.nodes, q.leaf_ids, q.params, t.nodes,
                        t.params, r_sq, cap, safe, lists, lcount, flags);
     return hipGetLastError();
@@ -239,8 +255,9 @@ hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const in
                         const int32_t* lcount, int cap, float r_sq, float inv_r, int self_pair,
                         int by_receiver, uint32_t* qcount, const uint32_t* qoff, int64_t n_edges, EdgeOut out,
                         int index_base, hipStream_t s) {
-    const int64_t want = (q.n + EW_WAVES - 1) / EW_WAVES;  // one wave per leaf, n_leaves <= n
-    const unsigned g = (unsigned)(want < 1 ? 1 : (want > 8192 ? 8192 : want));
+    // about four leaves per wave, XCD-local contiguous ranges
+    const int64_t want = ((int64_t)q.n_leaves + 4 * EW_WAVES - 1) / (4 * EW_WAVES);
+    const unsigned g = (unsigned)(want < 1 ? 1 : (want > 16384 ? 16384 : want));
     if (mode == EDGE_COUNT)
         hipLaunchKernelGGL(k_edges<EDGE_COUNT>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.nodes, q.leaf_ids,
                            q.scoord, q.params, q.n, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,

@@ -107,12 +107,38 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const uint32_t* __r
     }
 }
 
+// Small inputs: one block of 1024 threads, each owning a contiguous run.
+constexpr int64_t SCAN_SINGLE_MAX = 131072;
+__global__ void __launch_bounds__(1024) k_scan_single(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                      int64_t n, uint64_t* __restrict__ total_out) {
+    __shared__ uint64_t lw[1024 / OCT_WAVE];
+    const int per = (int)((n + 1023) / 1024);
+    const int64_t lo = (int64_t)threadIdx.x * per;
+    uint64_t s = 0;
+    for (int i = 0; i < per; ++i)
+        if (lo + i < n) s += in[lo + i];
+    uint64_t tot;
+    uint64_t pre = block_excl_scan(s, lw, tot);
+    for (int i = 0; i < per; ++i) {
+        if (lo + i < n) {
+            const uint32_t v = in[lo + i];
+            out[lo + i] = (uint32_t)pre;
+            pre += v;
+        }
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = tot;
+}
+
 // words of uint32 scratch exclusive_scan needs for n inputs (64-bit tile sums)
 int64_t scan_tmp_words(int64_t n) { return 2 * ((n + SCAN_TILE - 1) / SCAN_TILE + 1); }
 
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
                           uint64_t* total_out, hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    if (n <= SCAN_SINGLE_MAX) {
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, in, out, n, total_out);
+        return hipGetLastError();
+    }
     const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (tiles > (int64_t)1024 * 64) return hipErrorInvalidValue;
     uint64_t* sums = (uint64_t*)tmp;
