@@ -31,6 +31,8 @@ def parse():
     ap.add_argument("--mode", default="exact", choices=["exact", "learned"])
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU oracle baseline (0 = skip)")
     ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M points)")
+    ap.add_argument("--no-profile", action="store_true", help="time without the per-phase HIP events")
+    ap.add_argument("--null-stream", action="store_true", help="run the library on torch's current stream")
     return ap.parse_args()
 
 
@@ -61,7 +63,7 @@ def main():
     snaps = snaps[1:]
     torch.cuda.synchronize()
 
-    tns = octohip.TNS()
+    tns = octohip.TNS(private_stream=not a.null_stream)
     octohip.set_search_radius(tns, float(r))
     sid = octohip.add_point_set(tns, snaps[0])
     octohip.set_active_search(tns, sid, sid)
@@ -75,8 +77,9 @@ def main():
     for k in range(a.warmup):
         step(k)
     torch.cuda.synchronize()
-    octohip.profile(tns, True)
-    octohip.profile_read(tns, reset=True)
+    if not a.no_profile:
+        octohip.profile(tns, True)
+        octohip.profile_read(tns, reset=True)
     edges = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()

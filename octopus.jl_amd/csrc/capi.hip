@@ -6,6 +6,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -123,6 +125,24 @@ struct oct_ctx {
 
 namespace {
 
+// Host wall-clock of a code section, printed to stderr when the environment
+// sets OCTOHIP_HOST_TIMING (a diagnostic of launch and wait overheads).
+struct HostTimer {
+    const char* name;
+    std::chrono::steady_clock::time_point t0;
+    static bool on() {
+        static int v = getenv("OCTOHIP_HOST_TIMING") ? 1 : 0;
+        return v != 0;
+    }
+    explicit HostTimer(const char* n) : name(n), t0(std::chrono::steady_clock::now()) {}
+    ~HostTimer() {
+        if (on()) {
+            const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+            fprintf(stderr, "[octohip host] %s %.1f us\n", name, us);
+        }
+    }
+};
+
 hipEvent_t take_event(oct_ctx* c) {
     if (!c->spare.empty()) {
         hipEvent_t e = c->spare.back();
@@ -193,6 +213,7 @@ int mark_all_dirty(oct_ctx* c) {
 
 int read_params(oct_ctx* c, SetState& s) {
     HIP_TRY(c, hipMemcpyAsync(c->pinned, s.prm.p, sizeof(BuildParams), hipMemcpyDeviceToHost, c->stream));
+    HostTimer ht("run: wait for build parameters");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     memcpy(&s.hp, c->pinned, sizeof(BuildParams));
     return OCT_OK;
@@ -467,7 +488,10 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
                                       p.scan_tmp.as<uint32_t>(), (uint64_t*)(flags + 2), c->stream));
         }
         HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 16, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        {
+            HostTimer ht("edges: wait for count total");
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+        }
         memcpy(host_flags, c->pinned, 16);
         if (host_flags[0] == 0 && host_flags[1] == 0) { settled = true; break; }
         if (host_flags[0] != 0) p.cap = (int)std::min<uint32_t>(host_flags[0] + 16u, (uint32_t)INT32_MAX / 8u);
@@ -480,6 +504,7 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
     if (total >= (uint64_t)INT32_MAX)
         return fail(c, OCT_ERR_CAPACITY, "edge count exceeds Int32 (the reference's limit)");
     const int64_t E = (int64_t)total;
+    HostTimer ht_fill("edges: allocate + launch fill");
     HIP_TRY(c, p.senders.ensure(4 * (size_t)E));
     HIP_TRY(c, p.receivers.ensure(4 * (size_t)E));
     HIP_TRY(c, p.rdisp.ensure(12 * (size_t)E));

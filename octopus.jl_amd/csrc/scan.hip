@@ -1,5 +1,5 @@
-// Device-wide exclusive prefix sum of uint32 values (three launches: tile
-// sums, scan of tile sums, tile scan with carry-in).  Used for the radix
+// Device-wide exclusive prefix sum of uint32 values: tile sums, then tile
+// scans with carry-in (two launches up to 1024 tiles, three beyond).  Used for the radix
 // histograms, the BFS node numbering and the per-query edge offsets.
 //
 // Tile sums, carries and the total are 64-bit, so the total is exact for any
@@ -107,26 +107,43 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const uint32_t* __r
     }
 }
 
-// Small inputs: one block of 1024 threads, each owning a contiguous run.
-constexpr int64_t SCAN_SINGLE_MAX = 131072;
-__global__ void __launch_bounds__(1024) k_scan_single(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                      int64_t n, uint64_t* __restrict__ total_out) {
-    __shared__ uint64_t lw[1024 / OCT_WAVE];
-    const int per = (int)((n + 1023) / 1024);
-    const int64_t lo = (int64_t)threadIdx.x * per;
-    uint64_t s = 0;
-    for (int i = 0; i < per; ++i)
-        if (lo + i < n) s += in[lo + i];
-    uint64_t tot;
-    uint64_t pre = block_excl_scan(s, lw, tot);
-    for (int i = 0; i < per; ++i) {
-        if (lo + i < n) {
-            const uint32_t v = in[lo + i];
-            out[lo + i] = (uint32_t)pre;
-            pre += v;
-        }
+// Up to SCAN_CARRY_MAX tiles: every tile block sums the totals of the tiles
+// before it itself, so the scan takes two launches.
+constexpr int64_t SCAN_CARRY_MAX = 1024;
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles_carry(const uint32_t* __restrict__ in,
+                                                                   uint32_t* __restrict__ out, int64_t n,
+                                                                   const uint64_t* __restrict__ sums,
+                                                                   uint64_t* __restrict__ total_out) {
+    __shared__ uint64_t lw[SCAN_THREADS / OCT_WAVE];
+    __shared__ uint32_t tile[SCAN_TILE];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    uint64_t c = 0;
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += SCAN_THREADS) c += sums[t];
+    uint64_t carry;
+    block_excl_scan(c, lw, carry);
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const int64_t k = base + (int64_t)i * SCAN_THREADS + threadIdx.x;
+        tile[i * SCAN_THREADS + threadIdx.x] = k < n ? in[k] : 0u;
     }
-    if (threadIdx.x == 0 && total_out) *total_out = tot;
+    __syncthreads();
+    uint32_t v[SCAN_ITEMS];
+    uint64_t s = 0;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        v[i] = tile[threadIdx.x * SCAN_ITEMS + i];
+        s += v[i];
+    }
+    uint64_t tot;
+    uint64_t pre = block_excl_scan(s, lw, tot) + carry;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        tile[threadIdx.x * SCAN_ITEMS + i] = (uint32_t)pre;
+        pre += v[i];
+    }
+    __syncthreads();
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const int64_t k = base + (int64_t)i * SCAN_THREADS + threadIdx.x;
+        if (k < n) out[k] = tile[i * SCAN_THREADS + threadIdx.x];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0 && total_out) *total_out = carry + tot;
 }
 
 // words of uint32 scratch exclusive_scan needs for n inputs (64-bit tile sums)
@@ -135,11 +152,14 @@ int64_t scan_tmp_words(int64_t n) { return 2 * ((n + SCAN_TILE - 1) / SCAN_TILE 
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
                           uint64_t* total_out, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    if (n <= SCAN_SINGLE_MAX) {
-        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, in, out, n, total_out);
+    const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles <= SCAN_CARRY_MAX) {
+        uint64_t* sums = (uint64_t*)tmp;
+        hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, in, n, sums);
+        hipLaunchKernelGGL(k_scan_tiles_carry, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, in, out, n, sums,
+                           total_out);
         return hipGetLastError();
     }
-    const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (tiles > (int64_t)1024 * 64) return hipErrorInvalidValue;
     uint64_t* sums = (uint64_t*)tmp;
     hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, in, n, sums);

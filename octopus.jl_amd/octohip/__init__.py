@@ -81,7 +81,8 @@ class TNS:
     The C context is created on first use, so argument checks that need no
     device behave as in the reference even without a GPU."""
 
-    def __init__(self, dtype=None, ndims: int = 3, device=None, index_base: int = 0):
+    def __init__(self, dtype=None, ndims: int = 3, device=None, index_base: int = 0,
+                 private_stream: bool = True):
         torch = _torch()
         dtype = torch.float32 if dtype is None else dtype
         if dtype != torch.float32:
@@ -101,6 +102,8 @@ class TNS:
         self.active_pairs = []
         self.dirty = []
         self._ctx = None
+        self._stream = None
+        self._private_stream = bool(private_stream)
         self._lib = L.load()
 
     # ------------------------------------------------------------ context
@@ -114,8 +117,11 @@ class TNS:
             if rc != L.OCT_OK:
                 raise OctopusError(f"oct_create failed ({rc})")
             self._ctx = h
-            stream = torch.cuda.current_stream(self.device).cuda_stream
-            self._check(self._lib.oct_set_stream(h, ctypes.c_void_p(stream)))
+            # The library runs on its own stream (default priority); calls
+            # order it after the caller's current stream and back (_enter/_leave).
+            self._stream = (torch.cuda.Stream(device=self.device) if self._private_stream
+                            else torch.cuda.current_stream(self.device))
+            self._check(self._lib.oct_set_stream(h, ctypes.c_void_p(self._stream.cuda_stream)))
             if self.radius > 0:
                 self._check(self._lib.oct_set_search_radius(h, self.radius))
             if self.target_leaf_size != 32:
@@ -123,6 +129,18 @@ class TNS:
             if self.refit_mode:
                 self._check(self._lib.oct_set_refit_mode(h, 1))
         return self._ctx
+
+    def _enter(self):
+        """Library work waits for everything queued on the caller's stream."""
+        ctx = self._c()
+        if self._private_stream:
+            self._stream.wait_stream(_torch().cuda.current_stream(self.device))
+        return ctx
+
+    def _leave(self):
+        """The caller's stream waits for the library's work (outputs ready)."""
+        if self._private_stream:
+            _torch().cuda.current_stream(self.device).wait_stream(self._stream)
 
     def _check(self, rc):
         if rc == L.OCT_OK:
@@ -267,7 +285,8 @@ def run(tns: TNS):
         raise ArgumentError("set_search_radius! before run!")
     if not tns.point_sets:
         raise ArgumentError("add at least one point set before run!")
-    tns._check(tns._lib.oct_run(tns._c()))
+    tns._check(tns._lib.oct_run(tns._enter()))
+    tns._leave()
     tns.dirty = [False] * len(tns.dirty)
     return tns
 
@@ -307,7 +326,8 @@ def build_edges(tns: TNS, q: int, t: int, mode: str = "exact", order: str = "lea
     if order not in _ORDERS:
         raise ArgumentError(f"order must be one of {sorted(_ORDERS)}")
     e = L.OctEdges()
-    tns._check(tns._lib.oct_build_edges(tns._c(), int(q), int(t), _MODES[mode], _ORDERS[order], ctypes.byref(e)))
+    tns._check(tns._lib.oct_build_edges(tns._enter(), int(q), int(t), _MODES[mode], _ORDERS[order], ctypes.byref(e)))
+    tns._leave()
     n, dev = int(e.n_edges), tns.device
     return Edges(
         senders=_wrap(e.senders, (n,), torch.int32, dev),
@@ -330,7 +350,9 @@ def count_neighbors_device(tns: TNS, view: L.OctView):
     (self not filtered), the reference's test/query_cuda.jl:40-56 pattern."""
     torch = _torch()
     out = torch.empty(int(view.n_q), dtype=torch.int32, device=tns.device)
-    tns._check(tns._lib.oct_count_neighbors_device(tns._c(), ctypes.byref(view), ctypes.c_void_p(out.data_ptr())))
+    tns._check(tns._lib.oct_count_neighbors_device(tns._enter(), ctypes.byref(view),
+                                                   ctypes.c_void_p(out.data_ptr())))
+    tns._leave()
     return out
 
 
@@ -373,8 +395,9 @@ def _pullback(tns: TNS, n_points: int, senders, receivers, rdisp, rdist, d_disp,
     e.n_edges = senders.numel()
     dd = None if d_disp is None else ctypes.c_void_p(d_disp.contiguous().data_ptr())
     dn = None if d_dist is None else ctypes.c_void_p(d_dist.contiguous().data_ptr())
-    tns._check(tns._lib.oct_build_edges_backward(tns._c(), n_points, ctypes.byref(e), dd, dn, float(radius),
+    tns._check(tns._lib.oct_build_edges_backward(tns._enter(), n_points, ctypes.byref(e), dd, dn, float(radius),
                                                  ctypes.c_void_p(grad.data_ptr())))
+    tns._leave()
     return grad
 
 
