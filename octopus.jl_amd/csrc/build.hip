@@ -344,7 +344,7 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
             BuildParams* __restrict__ prm, int index_base, NodeRec* __restrict__ nodes,
             int32_t* __restrict__ nfirst, int32_t* __restrict__ nlast, int32_t* __restrict__ nchild,
             float* __restrict__ bmin, float* __restrict__ bmax, int32_t* __restrict__ parent,
-            uint8_t* __restrict__ nlevel, int32_t* __restrict__ leaf_ids) {
+            uint8_t* __restrict__ nlevel, int32_t* __restrict__ leaf_ids, NodeRec* __restrict__ leaves) {
     __shared__ uint32_t wc[TOPO_THREADS / OCT_WAVE][OCT_NLEVELS + 1];
     const int64_t k = (int64_t)blockIdx.x * TOPO_THREADS + threadIdx.x;
     int mk = -1, lam = -1;
@@ -417,6 +417,7 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
             rec.hi = make_float4(o2f(hi3[0]), o2f(hi3[1]), o2f(hi3[2]), __int_as_float(cnt));
             const int32_t lr = before(OCT_NLEVELS) - M;
             leaf_ids[lr] = id;
+            leaves[lr] = rec;
             if (chain) prm->branch_root = id;  // everything sits in one leaf
         } else {
             rec.lo = make_float4(0.f, 0.f, 0.f, __int_as_float(before(L + 1)));  // first child
@@ -548,7 +549,7 @@ hipError_t launch_build_stage(const float* coords, int64_t n, int key_bits, int 
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, b.scoord, n, T, b.topo, b.hist,
                        nblk, b.prm, index_base, b.nodes, b.nfirst, b.nlast, b.nchild, b.bmin, b.bmax,
-                       b.parent, b.nlevel, b.leaf_ids);
+                       b.parent, b.nlevel, b.leaf_ids, b.leaves);
     hipLaunchKernelGGL(k_set_leaf_count, dim3(1), dim3(64), 0, s, b.prm, b.total);
     const unsigned gl = (unsigned)std::min<int64_t>((max_nodes(n, T) + 255) / 256, 2048);
     hipLaunchKernelGGL(k_link, dim3(gl), dim3(256), 0, s, sk, b.prm, index_base, b.parent, b.nfirst, b.nlevel,

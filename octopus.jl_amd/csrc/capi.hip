@@ -24,14 +24,16 @@ struct DevBuf {
     size_t cap = 0;
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap && p) return hipSuccess;
+        // grow by at least 1.5x (and 1/8 headroom, 64 KiB granules) so sizes
+        // that drift up step to step do not reallocate every call
+        size_t want = std::max(bytes + bytes / 8, cap + cap / 2);
+        want = (want + 65535) & ~(size_t)65535;
         if (p) {
             hipError_t e = hipFree(p);
             if (e != hipSuccess) return e;
             p = nullptr;
             cap = 0;
         }
-        size_t want = std::max(bytes, cap + cap / 2);
-        if (want < 256) want = 256;
         hipError_t e = hipMalloc(&p, want);
         if (e != hipSuccess) { p = nullptr; return e; }
         cap = want;
@@ -53,7 +55,7 @@ struct SetState {
     int key_bits = 0;
     BuildParams hp{};  // host copy after the last build
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
-        nfirst, nlast, nchild, parent, nlevel, leaf_ids, bmin, bmax;
+        nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax;
     BuildBufs bufs() const {
         BuildBufs b;
         b.prm = prm.as<BuildParams>();
@@ -75,13 +77,14 @@ struct SetState {
         b.parent = parent.as<int32_t>();
         b.nlevel = nlevel.as<uint8_t>();
         b.leaf_ids = leaf_ids.as<int32_t>();
+        b.leaves = leaves.as<NodeRec>();
         b.bmin = bmin.as<float>();
         b.bmax = bmax.as<float>();
         return b;
     }
     void release_all() {
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
-                         &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &bmin, &bmax};
+                         &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -197,6 +200,7 @@ TreeView view_of(const SetState& s) {
     TreeView v;
     v.nodes = s.nodes.as<NodeRec>();
     v.leaf_ids = s.leaf_ids.as<int32_t>();
+    v.leaves = s.leaves.as<NodeRec>();
     v.scoord = s.scoord.as<float4>();
     v.params = s.prm.as<BuildParams>();
     v.n_leaves = s.hp.n_leaves;
@@ -253,6 +257,7 @@ int build_set(oct_ctx* c, SetState& s) {
     HIP_TRY(c, s.parent.ensure(4 * (size_t)M));
     HIP_TRY(c, s.nlevel.ensure((size_t)M));
     HIP_TRY(c, s.leaf_ids.ensure(4 * (size_t)n));
+    HIP_TRY(c, s.leaves.ensure(sizeof(NodeRec) * (size_t)n));
     HIP_TRY(c, s.bmin.ensure(12 * (size_t)M));
     HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
     const BuildBufs b = s.bufs();

@@ -32,7 +32,7 @@ constexpr int LL_POP = 2;
 // would still run out of room stops, and the leaf is flagged so the host
 // reruns the lists with safe = 1 (single pops, inside the proven bound).
 __global__ void __launch_bounds__(256)
-k_leaf_lists(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qleaf_ids,
+k_leaf_lists(const NodeRec* __restrict__ qleaves,
              const BuildParams* __restrict__ qprm, const NodeRec* __restrict__ tnodes,
              const BuildParams* __restrict__ tprm, float r_sq, int cap, int safe, int2* __restrict__ lists,
              int32_t* __restrict__ lcount, uint32_t* __restrict__ flags) {
@@ -41,7 +41,7 @@ k_leaf_lists(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qle
     const int32_t lb = (int32_t)xcd_block(blockIdx.x, gridDim.x);
     for (int32_t r = lb * (int32_t)blockDim.x + (int32_t)threadIdx.x; r < nleaves;
          r += (int32_t)(gridDim.x * blockDim.x)) {
-        const NodeRec qa = qnodes[qleaf_ids[r]];
+        const NodeRec qa = qleaves[r];
         int2* out = lists + (int64_t)r * cap;
         int cnt = 0;
         bool stack_full = false;
@@ -105,7 +105,7 @@ constexpr int EW_TILE = 4;      // candidate chunks of 64 held in registers per 
 // the loop then runs query-outer, so each query is broadcast once per tile.
 template <int MODE>
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
-k_edges(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qleaf_ids,
+k_edges(const NodeRec* __restrict__ qleaves,
         const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm, int64_t nq,
         const float4* __restrict__ tsc, const int2* __restrict__ lists,
         const int32_t* __restrict__ lcount, int cap, float r_sq, float inv_r, int self_pair,
@@ -126,11 +126,26 @@ k_edges(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qleaf_id
     const int32_t per = (nleaves + nwaves - 1) / nwaves;
     const int32_t r_lo = __builtin_amdgcn_readfirstlane(w * per);
     const int32_t r_hi = min(nleaves, r_lo + per);
+    // leaf metadata of the next leaf is loaded while the current one runs
+    int2 nx_meta = make_int2(0, 0), nx_lf = make_int2(0, 0);
+    int32_t nx_cnt = 0;
+    if (r_lo < r_hi) {
+        const NodeRec qa = qleaves[r_lo];
+        nx_meta = make_int2(rec_a(qa), rec_b(qa));
+        nx_cnt = lcount[r_lo];
+        if (lane < cap) nx_lf = lists[(int64_t)r_lo * cap + lane];
+    }
     for (int32_t r = r_lo; r < r_hi; ++r) {
-        const NodeRec qa = qnodes[qleaf_ids[r]];
-        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
-        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
-        const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+        const int32_t qf = __builtin_amdgcn_readfirstlane(nx_meta.x);
+        const int32_t qc = __builtin_amdgcn_readfirstlane(nx_meta.y);
+        const int32_t ncand = __builtin_amdgcn_readfirstlane(min(nx_cnt, cap));
+        const int2 lf0 = nx_lf;
+        if (r + 1 < r_hi) {
+            const NodeRec qa = qleaves[r + 1];
+            nx_meta = make_int2(rec_a(qa), rec_b(qa));
+            nx_cnt = lcount[r + 1];
+            if (lane < cap) nx_lf = lists[(int64_t)(r + 1) * cap + lane];
+        }
         const int2* lst = lists + (int64_t)r * cap;
         for (int qb = 0; qb < qc; qb += OCT_WAVE) {
             const int nqb = min(OCT_WAVE, qc - qb);
@@ -152,7 +167,7 @@ k_edges(const NodeRec* __restrict__ qnodes, const int32_t* __restrict__ qleaf_id
             }
             for (int g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
                 int2 lf = make_int2(0, 0);
-                if (g0 + lane < ncand) lf = lst[g0 + lane];
+                if (g0 + lane < ncand) lf = (g0 == 0) ? lf0 : lst[g0 + lane];
                 const int incl = wave_incl_scan(lf.y);
                 const int total = __builtin_amdgcn_readlane(incl, OCT_WAVE - 1);
                 s_start[wv][lane + 1] = incl;
@@ -246,7 +261,7 @@ hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, i
                              int32_t* lcount, uint32_t* flags, hipStream_t s) {
     const int64_t want = ((int64_t)q.n_leaves + 255) / 256;
     const unsigned g = (unsigned)(want < 1 ? 1 : (want > 65535 ? 65535 : want));
-    hipLaunchKernelGGL(k_leaf_lists, dim3(g), dim3(256), 0, s, q.nodes, q.leaf_ids, q.params, t.nodes,
+    hipLaunchKernelGGL(k_leaf_lists, dim3(g), dim3(256), 0, s, q.leaves, q.params, t.nodes,
                        t.params, r_sq, cap, safe, lists, lcount, flags);
     return hipGetLastError();
 }
@@ -259,11 +274,11 @@ hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const in
     const int64_t want = ((int64_t)q.n_leaves + 4 * EW_WAVES - 1) / (4 * EW_WAVES);
     const unsigned g = (unsigned)(want < 1 ? 1 : (want > 16384 ? 16384 : want));
     if (mode == EDGE_COUNT)
-        hipLaunchKernelGGL(k_edges<EDGE_COUNT>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.nodes, q.leaf_ids,
+        hipLaunchKernelGGL(k_edges<EDGE_COUNT>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves,
                            q.scoord, q.params, q.n, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,
                            by_receiver, qcount, qoff, n_edges, out, index_base);
     else
-        hipLaunchKernelGGL(k_edges<EDGE_FILL>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.nodes, q.leaf_ids,
+        hipLaunchKernelGGL(k_edges<EDGE_FILL>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves,
                            q.scoord, q.params, q.n, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,
                            by_receiver, qcount, qoff, n_edges, out, index_base);
     return hipGetLastError();

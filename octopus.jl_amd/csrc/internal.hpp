@@ -40,6 +40,7 @@ __device__ inline int rec_b(const NodeRec& r) { return __float_as_int(r.hi.w); }
 struct TreeView {
     const NodeRec* nodes;
     const int32_t* leaf_ids;   // leaf node ids in Morton order
+    const NodeRec* leaves;     // leaf records in Morton order (a = first position, b = count)
     const float4* scoord;      // Morton-sorted coords, w = original index (int bits)
     const BuildParams* params;
     int32_t n_leaves;
@@ -62,6 +63,7 @@ struct BuildBufs {
     float4* scoord;      // Morton-sorted coords, w = original index bits
     NodeRec* nodes;
     int32_t *nfirst, *nlast, *nchild, *parent, *leaf_ids;
+    NodeRec* leaves;     // leaf records in Morton order
     uint8_t* nlevel;
     float *bmin, *bmax;
 };
