@@ -97,79 +97,27 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
 // ---------------------------------------------------------------- stage 2
 
 constexpr int EW_WAVES = 4;     // waves per block, each owns one query leaf at a time
-constexpr int EW_HITS = 1024;   // hit list entries per wave (8 KB of LDS)
+constexpr int EW_STAGE = 256;   // edges one wave stages in LDS (6 KB)
 constexpr int EW_TILE = 4;      // candidate chunks of 64 held in registers per tile
 
-// Candidate enumeration shared by the passes: the candidate leaves of a query
-// leaf, 64 at a time, prefix-summed into LDS; points are then taken 4 x 64 at
-// a time into registers (coords + sorted position).
-struct CandTile {
-    float4 p[EW_TILE];
-    int32_t pos[EW_TILE];
-};
-
-__device__ inline void load_tile(CandTile& t, int t0, int total, const int32_t* s_start, const int32_t* s_first,
-                                 const float4* __restrict__ tsc, int lane) {
-#pragma unroll
-    for (int u = 0; u < EW_TILE; ++u) {
-        const int ci = t0 + u * OCT_WAVE + lane;
-        t.p[u] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-        t.pos[u] = -1;
-        if (ci < total) {
-            int lo = 0, hi = OCT_WAVE;  // owner leaf: last o with s_start[o] <= ci
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_start[mid] <= ci) lo = mid;
-                else hi = mid;
-            }
-            t.pos[u] = s_first[lo] + (ci - s_start[lo]);
-            t.p[u] = tsc[t.pos[u]];
-        }
-    }
-}
-
-// The reference predicate for query (qx, qy, qz, i) against candidate p.
-__device__ inline bool edge_test(float qx, float qy, float qz, int32_t i, float4 p, float r_sq, int self_pair) {
-    const int32_t j = __float_as_int(p.w);
-    const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    return j >= 0 && d2 <= r_sq && !(self_pair && j == i);
-}
-
-// Writes edge e of query q against target point p (features exactly as the
-// reference: rdisp = (q - t) * inv_r, rdist = sqrt(d2) * inv_r).
-__device__ inline void emit_edge(EdgeOut out, uint32_t e, float4 q, float4 p, float inv_r, int index_base) {
-    const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    out.senders[e] = __float_as_int(p.w) + index_base;
-    out.receivers[e] = __float_as_int(q.w) + index_base;
-    out.rdisp[3 * (int64_t)e + 0] = dx * inv_r;
-    out.rdisp[3 * (int64_t)e + 1] = dy * inv_r;
-    out.rdisp[3 * (int64_t)e + 2] = dz * inv_r;
-    out.rdist[e] = sqrtf(d2) * inv_r;
-}
-
-// One wave per query leaf (contiguous, XCD-local ranges of leaves).
-//   COUNT: per-query neighbour counts (qcount).
-//   FILL:  with the exact counts and offsets, queries are split into groups
-//          whose edges fit the hit list; phase 1 records each hit (query,
-//          target position) at its final place in the list, phase 2 has all
-//          lanes compute the features of consecutive hits and write them with
-//          contiguous stores.  A lone query with more hits than the list holds
-//          writes directly.
+// One wave per query leaf (grid-stride over a contiguous, XCD-local range of
+// leaves).  Candidates are loaded a tile of 4 x 64 at a time into registers;
+// the loop then runs query-outer, so each query is broadcast once per tile.
 template <int MODE>
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
-k_edges(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm,
-        const float4* __restrict__ tsc, const int2* __restrict__ lists, const int32_t* __restrict__ lcount, int cap,
-        float r_sq, float inv_r, int self_pair, int by_receiver, uint32_t* __restrict__ qcount,
-        const uint32_t* __restrict__ qoff, EdgeOut out, int index_base) {
+k_edges(const NodeRec* __restrict__ qleaves,
+        const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm, int64_t nq,
+        const float4* __restrict__ tsc, const int2* __restrict__ lists,
+        const int32_t* __restrict__ lcount, int cap, float r_sq, float inv_r, int self_pair,
+        int by_receiver, uint32_t* __restrict__ qcount, const uint32_t* __restrict__ qoff,
+        int64_t n_edges, EdgeOut out, int index_base) {
     __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
     __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
-    constexpr int HITS = MODE == EDGE_FILL ? EW_HITS : 1;
-    constexpr int QS = MODE == EDGE_FILL ? OCT_WAVE : 1;
-    __shared__ int2 s_hit[EW_WAVES][HITS];
-    __shared__ float4 s_q[EW_WAVES][QS];
-    __shared__ uint32_t s_qoff[EW_WAVES][QS];
+    constexpr int STAGE = MODE == EDGE_FILL ? EW_STAGE : 1;
+    __shared__ int32_t s_sen[EW_WAVES][STAGE];
+    __shared__ int32_t s_rcv[EW_WAVES][STAGE];
+    __shared__ float s_disp[EW_WAVES][3 * STAGE];
+    __shared__ float s_dist[EW_WAVES][STAGE];
     const int wv = threadIdx.x / OCT_WAVE;
     const int lane = lane_id();
     const int32_t nleaves = qprm->n_leaves;
@@ -201,87 +149,102 @@ k_edges(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, con
         const int2* lst = lists + (int64_t)r * cap;
         for (int qb = 0; qb < qc; qb += OCT_WAVE) {
             const int nqb = min(OCT_WAVE, qc - qb);
-            float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);  // query qb + lane
+            // query qb + lane lives in this lane
+            float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);
             if (lane < nqb) myq = qsc[qf + qb + lane];
             const int32_t slot = by_receiver ? __float_as_int(myq.w) : qf + qb + lane;
-            // groups of consecutive queries (COUNT: all nqb at once)
-            uint32_t c_lane = 0, o_lane = 0;
-            if (MODE == EDGE_FILL && lane < nqb) {
-                c_lane = qcount[slot];
-                o_lane = qoff[slot];
-                s_q[wv][lane] = myq;
+            uint32_t my = 0;  // COUNT: count; FILL: next output position of this lane's query
+            if (MODE == EDGE_FILL && lane < nqb) my = qoff[slot];
+            // leaf order: this block of queries owns [blk_base, blk_base + blk_tot)
+            uint32_t blk_base = 0, blk_tot = 0;
+            bool staged = false;
+            if (MODE == EDGE_FILL && !by_receiver) {
+                blk_base = (uint32_t)__builtin_amdgcn_readfirstlane((int)my);
+                const int64_t nxt = (int64_t)qf + qb + nqb;
+                const uint32_t blk_end = nxt < nq ? qoff[nxt] : (uint32_t)n_edges;
+                blk_tot = blk_end - blk_base;
+                staged = blk_tot <= (uint32_t)EW_STAGE;
             }
-            const uint32_t incl = (uint32_t)wave_incl_scan((int)c_lane);
-            int g = 0;
-            while (g < nqb) {
-                int gend = nqb;
-                uint32_t gtot = 0;
-                bool direct = false;
-                uint32_t run_lane = 0;  // COUNT: count; FILL: next hit slot of this lane's query
-                if (MODE == EDGE_FILL) {
-                    const uint32_t base_incl = g > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)incl, g - 1) : 0u;
-                    const uint64_t fit = ballot(lane >= g && lane < nqb && incl - base_incl <= (uint32_t)EW_HITS);
-                    gend = fit ? 64 - __clzll((long long)fit) : g + 1;
-                    gtot = (uint32_t)__builtin_amdgcn_readlane((int)incl, gend - 1) - base_incl;
-                    direct = gtot > (uint32_t)EW_HITS;
-                    const uint32_t lstart = incl - c_lane - base_incl;
-                    run_lane = lstart;
-                    if (lane >= g && lane < gend) s_qoff[wv][lane] = o_lane - lstart;
-                }
-                for (int g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
-                    int2 lf = make_int2(0, 0);
-                    if (g0 + lane < ncand) lf = (g0 == 0) ? lf0 : lst[g0 + lane];
-                    const int cincl = wave_incl_scan(lf.y);
-                    const int total = __builtin_amdgcn_readlane(cincl, OCT_WAVE - 1);
-                    s_start[wv][lane + 1] = cincl;
-                    if (lane == 0) s_start[wv][0] = 0;
-                    s_first[wv][lane] = lf.x;
-                    __builtin_amdgcn_wave_barrier();
-                    for (int t0 = 0; t0 < total; t0 += EW_TILE * OCT_WAVE) {
-                        CandTile ct;
-                        load_tile(ct, t0, total, s_start[wv], s_first[wv], tsc, lane);
-                        const int nu = min(EW_TILE, (total - t0 + OCT_WAVE - 1) / OCT_WAVE);
-                        for (int qi = g; qi < gend; ++qi) {
-                            const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
-                            const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
-                            const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
-                            const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
-                            uint32_t run = (uint32_t)__builtin_amdgcn_readlane((int)run_lane, qi);
-                            const uint32_t run0 = run;
+            for (int g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
+                int2 lf = make_int2(0, 0);
+                if (g0 + lane < ncand) lf = (g0 == 0) ? lf0 : lst[g0 + lane];
+                const int incl = wave_incl_scan(lf.y);
+                const int total = __builtin_amdgcn_readlane(incl, OCT_WAVE - 1);
+                s_start[wv][lane + 1] = incl;
+                if (lane == 0) s_start[wv][0] = 0;
+                s_first[wv][lane] = lf.x;
+                __builtin_amdgcn_wave_barrier();
+                for (int t0 = 0; t0 < total; t0 += EW_TILE * OCT_WAVE) {
+                    float4 tp[EW_TILE];
 #pragma unroll
-                            for (int u = 0; u < EW_TILE; ++u) {
-                                if (u >= nu) break;
-                                const bool pass = edge_test(qx, qy, qz, i, ct.p[u], r_sq, self_pair);
-                                const uint64_t m = ballot(pass);
-                                if (MODE == EDGE_FILL && pass) {
-                                    const uint32_t hpos = run + (uint32_t)mbcnt(m);
-                                    if (!direct) {
-                                        s_hit[wv][hpos] = make_int2(qi, ct.pos[u]);
-                                    } else {
-                                        const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)o_lane, qi);
-                                        const float4 q4 = make_float4(qx, qy, qz, __int_as_float(i));
-                                        emit_edge(out, o + hpos, q4, ct.p[u], inv_r, index_base);
-                                    }
-                                }
-                                run += (uint32_t)__popcll(m);
+                    for (int u = 0; u < EW_TILE; ++u) {
+                        const int ci = t0 + u * OCT_WAVE + lane;
+                        tp[u] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+                        if (ci < total) {
+                            int lo = 0, hi = OCT_WAVE;  // owner: last o with s_start[o] <= ci
+                            while (hi - lo > 1) {
+                                const int mid = (lo + hi) >> 1;
+                                if (s_start[wv][mid] <= ci) lo = mid;
+                                else hi = mid;
                             }
-                            if (lane == qi) run_lane += run - run0;
+                            tp[u] = tsc[s_first[wv][lo] + (ci - s_start[wv][lo])];
                         }
                     }
-                    __builtin_amdgcn_wave_barrier();
-                }
-                if (MODE == EDGE_COUNT) {
-                    if (lane < nqb) qcount[slot] = run_lane;
-                } else if (!direct) {
-                    // phase 2: dense, position-ordered emission of the group's hits
-                    __builtin_amdgcn_wave_barrier();
-                    for (uint32_t t = lane; t < gtot; t += OCT_WAVE) {
-                        const int2 h = s_hit[wv][t];
-                        emit_edge(out, s_qoff[wv][h.x] + t, s_q[wv][h.x], tsc[h.y], inv_r, index_base);
+                    const int nu = min(EW_TILE, (total - t0 + OCT_WAVE - 1) / OCT_WAVE);
+                    for (int qi = 0; qi < nqb; ++qi) {
+                        const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
+                        const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
+                        const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
+                        const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
+                        uint32_t run = MODE == EDGE_FILL ? (uint32_t)__builtin_amdgcn_readlane((int)my, qi) : 0u;
+                        const uint32_t run0 = run;
+#pragma unroll
+                        for (int u = 0; u < EW_TILE; ++u) {
+                            if (u >= nu) break;
+                            const int32_t j = __float_as_int(tp[u].w);
+                            const float dx = qx - tp[u].x;
+                            const float dy = qy - tp[u].y;
+                            const float dz = qz - tp[u].z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            const bool pass = j >= 0 && d2 <= r_sq && !(self_pair && j == i);
+                            const uint64_t m = ballot(pass);
+                            if (MODE == EDGE_FILL && pass) {
+                                const uint32_t e = run + (uint32_t)mbcnt(m);
+                                if (staged) {
+                                    const uint32_t s = e - blk_base;
+                                    s_sen[wv][s] = j + index_base;
+                                    s_rcv[wv][s] = i + index_base;
+                                    s_disp[wv][3 * s + 0] = dx * inv_r;
+                                    s_disp[wv][3 * s + 1] = dy * inv_r;
+                                    s_disp[wv][3 * s + 2] = dz * inv_r;
+                                    s_dist[wv][s] = sqrtf(d2) * inv_r;
+                                } else {
+                                    out.senders[e] = j + index_base;
+                                    out.receivers[e] = i + index_base;
+                                    out.rdisp[3 * (int64_t)e + 0] = dx * inv_r;
+                                    out.rdisp[3 * (int64_t)e + 1] = dy * inv_r;
+                                    out.rdisp[3 * (int64_t)e + 2] = dz * inv_r;
+                                    out.rdist[e] = sqrtf(d2) * inv_r;
+                                }
+                            }
+                            run += (uint32_t)__popcll(m);
+                        }
+                        if (lane == qi) my += run - run0;
                     }
-                    __builtin_amdgcn_wave_barrier();
                 }
-                g = gend;
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (MODE == EDGE_COUNT && lane < nqb) qcount[slot] = my;
+            if (MODE == EDGE_FILL && staged) {
+                __builtin_amdgcn_wave_barrier();
+                for (uint32_t t = lane; t < blk_tot; t += OCT_WAVE) {
+                    out.senders[blk_base + t] = s_sen[wv][t];
+                    out.receivers[blk_base + t] = s_rcv[wv][t];
+                    out.rdist[blk_base + t] = s_dist[wv][t];
+                }
+                for (uint32_t t = lane; t < 3 * blk_tot; t += OCT_WAVE)
+                    out.rdisp[3 * (int64_t)blk_base + t] = s_disp[wv][t];
+                __builtin_amdgcn_wave_barrier();
             }
         }
     }
@@ -307,18 +270,17 @@ hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const in
                         const int32_t* lcount, int cap, float r_sq, float inv_r, int self_pair,
                         int by_receiver, uint32_t* qcount, const uint32_t* qoff, int64_t n_edges, EdgeOut out,
                         int index_base, hipStream_t s) {
-    (void)n_edges;
     // about four leaves per wave, XCD-local contiguous ranges
     const int64_t want = ((int64_t)q.n_leaves + 4 * EW_WAVES - 1) / (4 * EW_WAVES);
     const unsigned g = (unsigned)(want < 1 ? 1 : (want > 16384 ? 16384 : want));
     if (mode == EDGE_COUNT)
-        hipLaunchKernelGGL(k_edges<EDGE_COUNT>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
-                           q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, by_receiver, qcount, qoff,
-                           out, index_base);
+        hipLaunchKernelGGL(k_edges<EDGE_COUNT>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves,
+                           q.scoord, q.params, q.n, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,
+                           by_receiver, qcount, qoff, n_edges, out, index_base);
     else
-        hipLaunchKernelGGL(k_edges<EDGE_FILL>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
-                           q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, by_receiver, qcount, qoff,
-                           out, index_base);
+        hipLaunchKernelGGL(k_edges<EDGE_FILL>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves,
+                           q.scoord, q.params, q.n, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,
+                           by_receiver, qcount, qoff, n_edges, out, index_base);
     return hipGetLastError();
 }
 
