@@ -91,12 +91,14 @@ struct SetState {
 
 struct PairState {
     int q = 0, t = 0;
-    int cap = 64;  // candidate leaves per query leaf
+    int cap = 64;            // candidate leaves per query leaf
+    int64_t cap_edges = 0;   // edges the output buffers hold (single pass sizing)
     int64_t n_edges = 0;
     bool materialized = false;
-    DevBuf senders, receivers, rdisp, rdist, lists, lcount, qcount, qoff, scan_tmp, flags;
+    DevBuf senders, receivers, rdisp, rdist, lists, lcount, qcount, qoff, scan_tmp, flags, tiles;
     void release_all() {
-        DevBuf* all[] = {&senders, &receivers, &rdisp, &rdist, &lists, &lcount, &qcount, &qoff, &scan_tmp, &flags};
+        DevBuf* all[] = {&senders, &receivers, &rdisp, &rdist, &lists, &lcount, &qcount, &qoff, &scan_tmp, &flags,
+                         &tiles};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -442,8 +444,8 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
     if (!pp) return fail(c, OCT_ERR_STATE, "build_edges!: pair not registered; call set_active_search! first");
     if (mode != OCT_EDGES_EXACT && mode != OCT_EDGES_LEARNED) return fail(c, OCT_ERR_ARGUMENT, "unknown edge mode");
     if (order != OCT_ORDER_LEAF && order != OCT_ORDER_RECEIVER) return fail(c, OCT_ERR_ARGUMENT, "unknown edge order");
-    if (mode == OCT_EDGES_LEARNED)
-        return fail(c, OCT_ERR_ARGUMENT, "the learned single pass needs a trained count predictor");
+    if (mode == OCT_EDGES_LEARNED && order != OCT_ORDER_LEAF)
+        return fail(c, OCT_ERR_ARGUMENT, "the single pass writes leaf order; receiver order needs the exact mode");
     PairState& p = *pp;
     SetState& qs = c->sets[set_index(c, q)];
     SetState& ts = c->sets[set_index(c, t)];
@@ -461,66 +463,106 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
     const float inv_r = 1.0f / r;
     const TreeView qv = view_of(qs), tv = view_of(ts);
     const int64_t nql = qs.hp.n_leaves;
-    HIP_TRY(c, p.flags.ensure(32));
+    const int64_t ntiles = single_tiles(nql);
+    HIP_TRY(c, p.flags.ensure(64));
     HIP_TRY(c, p.lcount.ensure(4 * (size_t)nql));
     HIP_TRY(c, p.qcount.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.qoff.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.scan_tmp.ensure(4 * (size_t)scan_tmp_words(qs.n)));
+    HIP_TRY(c, p.tiles.ensure(8 * (size_t)(ntiles + 1)));
     const int by_receiver = order == OCT_ORDER_RECEIVER;
-    // flags[0]: largest candidate count of an overflowing query leaf (0 = none);
-    // flags[2..3]: 64-bit total of the per-query counts
+    // flags: [0] largest candidate count of an overflowing query leaf, [1] a
+    // list walk ran out of stack, [2..3] exact-path edge total, [4] single
+    // pass over capacity, [5] single-pass look-back gave up, [6..7] single-pass
+    // edge total, [8] single-pass tile ticket
     uint32_t* flags = p.flags.as<uint32_t>();
-    uint32_t host_flags[4] = {0, 0, 0, 0};
+    uint32_t hf[8] = {0};
     int safe = 0;
-    bool settled = false;
-    for (int attempt = 0; attempt < 8; ++attempt) {
+    bool single = mode == OCT_EDGES_LEARNED && p.cap_edges > 0;
+    bool lists_ok = false;
+    uint64_t total = 0;
+    EdgeOut eo{};
+    for (int attempt = 0; attempt < 10; ++attempt) {
         HIP_TRY(c, p.lists.ensure(sizeof(int2) * (size_t)nql * (size_t)p.cap));
-        HIP_TRY(c, launch_fill_u32(flags, 4, 0u, c->stream));
-        {
+        HIP_TRY(c, launch_fill_u32(flags, 16, 0u, c->stream));
+        if (!lists_ok) {
             Phase ph(c, OCT_PROF_LEAF_LISTS);
             HIP_TRY(c, launch_leaf_lists(qv, tv, r_sq, p.cap, safe, p.lists.as<int2>(), p.lcount.as<int32_t>(),
                                          flags, c->stream));
         }
-        {
-            Phase ph(c, OCT_PROF_COUNT);
-            HIP_TRY(c, launch_edges(EDGE_COUNT, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
-                                    inv_r, self_pair, by_receiver, p.qcount.as<uint32_t>(), nullptr, 0, EdgeOut{},
-                                    c->index_base, c->stream));
-        }
-        {
+        if (single) {
+            HIP_TRY(c, p.senders.ensure(4 * (size_t)p.cap_edges));
+            HIP_TRY(c, p.receivers.ensure(4 * (size_t)p.cap_edges));
+            HIP_TRY(c, p.rdisp.ensure(12 * (size_t)p.cap_edges));
+            HIP_TRY(c, p.rdist.ensure(4 * (size_t)p.cap_edges));
+            eo = EdgeOut{p.senders.as<int32_t>(), p.receivers.as<int32_t>(), p.rdisp.as<float>(), p.rdist.as<float>()};
+            HIP_TRY(c, launch_fill_u32((uint32_t*)p.tiles.p, 2 * (ntiles + 1), 0u, c->stream));
+            Phase ph(c, OCT_PROF_SINGLE);
+            HIP_TRY(c, launch_edges_single(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r,
+                                           self_pair, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(),
+                                           p.tiles.as<uint64_t>(), flags + 8, flags + 4, (uint64_t)p.cap_edges, eo,
+                                           c->index_base, c->stream));
+        } else {
+            {
+                Phase ph(c, OCT_PROF_COUNT);
+                HIP_TRY(c, launch_edges(EDGE_COUNT, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
+                                        inv_r, self_pair, by_receiver, p.qcount.as<uint32_t>(), nullptr, 0,
+                                        EdgeOut{}, c->index_base, c->stream));
+            }
             Phase ph(c, OCT_PROF_SCAN);
             HIP_TRY(c, exclusive_scan(p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), qs.n,
                                       p.scan_tmp.as<uint32_t>(), (uint64_t*)(flags + 2), c->stream));
         }
-        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 32, hipMemcpyDeviceToHost, c->stream));
         {
-            HostTimer ht("edges: wait for count total");
+            HostTimer ht(single ? "edges: wait for single pass" : "edges: wait for count total");
             HIP_TRY(c, hipStreamSynchronize(c->stream));
         }
-        memcpy(host_flags, c->pinned, 16);
-        if (host_flags[0] == 0 && host_flags[1] == 0) { settled = true; break; }
-        if (host_flags[0] != 0) p.cap = (int)std::min<uint32_t>(host_flags[0] + 16u, (uint32_t)INT32_MAX / 8u);
-        if (host_flags[1] != 0) safe = 1;
+        memcpy(hf, c->pinned, 32);
+        if (hf[0] != 0 || hf[1] != 0) {  // candidate lists incomplete: regrow / safe walk, redo everything
+            if (hf[0] != 0) p.cap = (int)std::min<uint32_t>(hf[0] + 16u, (uint32_t)INT32_MAX / 8u);
+            if (hf[1] != 0) safe = 1;
+            lists_ok = false;
+            continue;
+        }
+        lists_ok = true;
+        if (!single) {
+            memcpy(&total, hf + 2, 8);
+            break;
+        }
+        memcpy(&total, hf + 6, 8);
+        if (hf[5] != 0) {  // look-back did not finish: take the exact path
+            single = false;
+            continue;
+        }
+        if (hf[4] != 0) {  // predicted room exceeded: grow to the reported total and rerun
+            if (total >= (uint64_t)INT32_MAX) break;
+            p.cap_edges = (int64_t)(total + total / 8 + 1024);
+            continue;
+        }
+        break;
     }
-    if (!settled) return fail(c, OCT_ERR_DEVICE, "candidate lists did not settle");
-    uint64_t total = 0;
-    memcpy(&total, host_flags + 2, 8);
+    if (!lists_ok) return fail(c, OCT_ERR_DEVICE, "candidate lists did not settle");
     // Int32 indices and counts, as in the reference (src/types.jl:89)
     if (total >= (uint64_t)INT32_MAX)
         return fail(c, OCT_ERR_CAPACITY, "edge count exceeds Int32 (the reference's limit)");
     const int64_t E = (int64_t)total;
-    HostTimer ht_fill("edges: allocate + launch fill");
-    HIP_TRY(c, p.senders.ensure(4 * (size_t)E));
-    HIP_TRY(c, p.receivers.ensure(4 * (size_t)E));
-    HIP_TRY(c, p.rdisp.ensure(12 * (size_t)E));
-    HIP_TRY(c, p.rdist.ensure(4 * (size_t)E));
-    EdgeOut eo{p.senders.as<int32_t>(), p.receivers.as<int32_t>(), p.rdisp.as<float>(), p.rdist.as<float>()};
-    if (E > 0) {
-        Phase ph(c, OCT_PROF_FILL);
-        HIP_TRY(c, launch_edges(EDGE_FILL, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r,
-                                self_pair, by_receiver, nullptr, p.qoff.as<uint32_t>(), E, eo, c->index_base,
-                                c->stream));
+    if (!single) {
+        HostTimer ht_fill("edges: allocate + launch fill");
+        HIP_TRY(c, p.senders.ensure(4 * (size_t)E));
+        HIP_TRY(c, p.receivers.ensure(4 * (size_t)E));
+        HIP_TRY(c, p.rdisp.ensure(12 * (size_t)E));
+        HIP_TRY(c, p.rdist.ensure(4 * (size_t)E));
+        eo = EdgeOut{p.senders.as<int32_t>(), p.receivers.as<int32_t>(), p.rdisp.as<float>(), p.rdist.as<float>()};
+        if (E > 0) {
+            Phase ph(c, OCT_PROF_FILL);
+            HIP_TRY(c, launch_edges(EDGE_FILL, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
+                                    inv_r, self_pair, by_receiver, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), E,
+                                    eo, c->index_base, c->stream));
+        }
     }
+    // room for the single pass next time: the senders buffer's capacity
+    p.cap_edges = (int64_t)std::min<size_t>(p.senders.cap / 4, (size_t)INT32_MAX - 1);
     p.n_edges = E;
     p.materialized = true;
     out->senders = eo.senders;

@@ -100,33 +100,171 @@ constexpr int EW_WAVES = 4;     // waves per block, each owns one query leaf at 
 constexpr int EW_STAGE = 256;   // edges one wave stages in LDS (6 KB)
 constexpr int EW_TILE = 4;      // candidate chunks of 64 held in registers per tile
 
-// One wave per query leaf (grid-stride over a contiguous, XCD-local range of
-// leaves).  Candidates are loaded a tile of 4 x 64 at a time into registers;
-// the loop then runs query-outer, so each query is broadcast once per tile.
+// Per-wave LDS of the edge passes.
+struct WaveLds {
+    int32_t* start;  // 65: prefix of candidate-leaf sizes
+    int32_t* first;  // 64: first sorted position of each candidate leaf
+    int32_t* sen;    // EW_STAGE staged senders
+    int32_t* rcv;
+    float* disp;     // 3 * EW_STAGE
+    float* dist;
+};
+
+// One query leaf through one wave.  Lanes take candidate target points (a
+// tile of 4 x 64 in registers); the leaf's queries sit one per lane and are
+// broadcast with readlane, query-outer, so each (query, 64 candidates) step
+// is one ballot.
+//   COUNT: writes the per-query counts to qcount[slot]; returns the leaf total.
+//   FILL:  reads qcount/qoff[slot] and writes the edges; a query block whose
+//          edges fit the LDS stage is staged and written with contiguous
+//          whole-wave stores.
+// slot = sorted position (leaf order) or original index (receiver order).
+template <int MODE>
+__device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, int32_t ncand, int2 lf0, const int2* __restrict__ lst,
+                                     const float4* __restrict__ qsc, const float4* __restrict__ tsc, float r_sq,
+                                     float inv_r, int self_pair, int by_receiver, uint32_t* __restrict__ qcount,
+                                     const uint32_t* __restrict__ qoff, EdgeOut out, int index_base,
+                                     const WaveLds& L, int lane) {
+    uint32_t leaf_total = 0;
+    for (int qb = 0; qb < qc; qb += OCT_WAVE) {
+        const int nqb = min(OCT_WAVE, qc - qb);
+        // query qb + lane lives in this lane
+        float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < nqb) myq = qsc[qf + qb + lane];
+        const int32_t slot = by_receiver ? __float_as_int(myq.w) : qf + qb + lane;
+        uint32_t my = 0;  // COUNT: count; FILL: next output position of this lane's query
+        uint32_t blk_base = 0, blk_tot = 0;
+        bool staged = false;
+        if (MODE == EDGE_FILL) {
+            uint32_t c = 0;
+            if (lane < nqb) {
+                my = qoff[slot];
+                c = qcount[slot];
+            }
+            // leaf order: this block of queries owns [blk_base, blk_base + blk_tot)
+            if (!by_receiver) {
+                blk_base = (uint32_t)__builtin_amdgcn_readfirstlane((int)my);
+                blk_tot = (uint32_t)wave_sum((int)c);
+                staged = blk_tot <= (uint32_t)EW_STAGE;
+            }
+        }
+        for (int g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
+            int2 lf = make_int2(0, 0);
+            if (g0 + lane < ncand) lf = (g0 == 0) ? lf0 : lst[g0 + lane];
+            const int incl = wave_incl_scan(lf.y);
+            const int total = __builtin_amdgcn_readlane(incl, OCT_WAVE - 1);
+            L.start[lane + 1] = incl;
+            if (lane == 0) L.start[0] = 0;
+            L.first[lane] = lf.x;
+            __builtin_amdgcn_wave_barrier();
+            for (int t0 = 0; t0 < total; t0 += EW_TILE * OCT_WAVE) {
+                float4 tp[EW_TILE];
+#pragma unroll
+                for (int u = 0; u < EW_TILE; ++u) {
+                    const int ci = t0 + u * OCT_WAVE + lane;
+                    tp[u] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+                    if (ci < total) {
+                        int lo = 0, hi = OCT_WAVE;  // owner: last o with start[o] <= ci
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (L.start[mid] <= ci) lo = mid;
+                            else hi = mid;
+                        }
+                        tp[u] = tsc[L.first[lo] + (ci - L.start[lo])];
+                    }
+                }
+                const int nu = min(EW_TILE, (total - t0 + OCT_WAVE - 1) / OCT_WAVE);
+                for (int qi = 0; qi < nqb; ++qi) {
+                    const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
+                    const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
+                    const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
+                    const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
+                    uint32_t run = MODE == EDGE_FILL ? (uint32_t)__builtin_amdgcn_readlane((int)my, qi) : 0u;
+                    const uint32_t run0 = run;
+#pragma unroll
+                    for (int u = 0; u < EW_TILE; ++u) {
+                        if (u >= nu) break;
+                        const int32_t j = __float_as_int(tp[u].w);
+                        const float dx = qx - tp[u].x;
+                        const float dy = qy - tp[u].y;
+                        const float dz = qz - tp[u].z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        const bool pass = j >= 0 && d2 <= r_sq && !(self_pair && j == i);
+                        const uint64_t m = ballot(pass);
+                        if (MODE == EDGE_FILL && pass) {
+                            const uint32_t e = run + (uint32_t)mbcnt(m);
+                            if (staged) {
+                                const uint32_t s = e - blk_base;
+                                L.sen[s] = j + index_base;
+                                L.rcv[s] = i + index_base;
+                                L.disp[3 * s + 0] = dx * inv_r;
+                                L.disp[3 * s + 1] = dy * inv_r;
+                                L.disp[3 * s + 2] = dz * inv_r;
+                                L.dist[s] = sqrtf(d2) * inv_r;
+                            } else {
+                                out.senders[e] = j + index_base;
+                                out.receivers[e] = i + index_base;
+                                out.rdisp[3 * (int64_t)e + 0] = dx * inv_r;
+                                out.rdisp[3 * (int64_t)e + 1] = dy * inv_r;
+                                out.rdisp[3 * (int64_t)e + 2] = dz * inv_r;
+                                out.rdist[e] = sqrtf(d2) * inv_r;
+                            }
+                        }
+                        run += (uint32_t)__popcll(m);
+                    }
+                    if (lane == qi) my += run - run0;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (MODE == EDGE_COUNT) {
+            if (lane < nqb) qcount[slot] = my;
+            leaf_total += (uint32_t)wave_sum(lane < nqb ? (int)my : 0);
+        }
+        if (MODE == EDGE_FILL && staged) {
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t t = lane; t < blk_tot; t += OCT_WAVE) {
+                out.senders[blk_base + t] = L.sen[t];
+                out.receivers[blk_base + t] = L.rcv[t];
+                out.rdist[blk_base + t] = L.dist[t];
+            }
+            for (uint32_t t = lane; t < 3 * blk_tot; t += OCT_WAVE) out.rdisp[3 * (int64_t)blk_base + t] = L.disp[t];
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    return leaf_total;
+}
+
+template <int MODE>
+struct EdgeLds {
+    static constexpr int STAGE = MODE == EDGE_FILL ? EW_STAGE : 1;
+    int32_t start[EW_WAVES][OCT_WAVE + 1];
+    int32_t first[EW_WAVES][OCT_WAVE];
+    int32_t sen[EW_WAVES][STAGE];
+    int32_t rcv[EW_WAVES][STAGE];
+    float disp[EW_WAVES][3 * STAGE];
+    float dist[EW_WAVES][STAGE];
+    __device__ WaveLds wave(int w) { return WaveLds{start[w], first[w], sen[w], rcv[w], disp[w], dist[w]}; }
+};
+
+// Two-pass driver: one wave per query leaf over a contiguous, XCD-local range
+// of leaves, the next leaf's metadata loaded while the current one runs.
 template <int MODE>
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
-k_edges(const NodeRec* __restrict__ qleaves,
-        const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm, int64_t nq,
-        const float4* __restrict__ tsc, const int2* __restrict__ lists,
-        const int32_t* __restrict__ lcount, int cap, float r_sq, float inv_r, int self_pair,
-        int by_receiver, uint32_t* __restrict__ qcount, const uint32_t* __restrict__ qoff,
-        int64_t n_edges, EdgeOut out, int index_base) {
-    __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
-    __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
-    constexpr int STAGE = MODE == EDGE_FILL ? EW_STAGE : 1;
-    __shared__ int32_t s_sen[EW_WAVES][STAGE];
-    __shared__ int32_t s_rcv[EW_WAVES][STAGE];
-    __shared__ float s_disp[EW_WAVES][3 * STAGE];
-    __shared__ float s_dist[EW_WAVES][STAGE];
+k_edges(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm,
+        const float4* __restrict__ tsc, const int2* __restrict__ lists, const int32_t* __restrict__ lcount, int cap,
+        float r_sq, float inv_r, int self_pair, int by_receiver, uint32_t* __restrict__ qcount,
+        const uint32_t* __restrict__ qoff, EdgeOut out, int index_base) {
+    __shared__ EdgeLds<MODE> lds;
     const int wv = threadIdx.x / OCT_WAVE;
     const int lane = lane_id();
+    const WaveLds L = lds.wave(wv);
     const int32_t nleaves = qprm->n_leaves;
     const int32_t nwaves = (int32_t)(gridDim.x * EW_WAVES);
     const int32_t w = (int32_t)xcd_block(blockIdx.x, gridDim.x) * EW_WAVES + wv;
     const int32_t per = (nleaves + nwaves - 1) / nwaves;
     const int32_t r_lo = __builtin_amdgcn_readfirstlane(w * per);
     const int32_t r_hi = min(nleaves, r_lo + per);
-    // leaf metadata of the next leaf is loaded while the current one runs
     int2 nx_meta = make_int2(0, 0), nx_lf = make_int2(0, 0);
     int32_t nx_cnt = 0;
     if (r_lo < r_hi) {
@@ -146,107 +284,121 @@ k_edges(const NodeRec* __restrict__ qleaves,
             nx_cnt = lcount[r + 1];
             if (lane < cap) nx_lf = lists[(int64_t)(r + 1) * cap + lane];
         }
+        leaf_pass<MODE>(qf, qc, ncand, lf0, lists + (int64_t)r * cap, qsc, tsc, r_sq, inv_r, self_pair, by_receiver,
+                        qcount, qoff, out, index_base, L, lane);
+    }
+}
+
+// ------------------------------------------------------------ single pass
+
+constexpr int SP_LEAVES_PER_WAVE = 4;
+constexpr int SP_TILE_LEAVES = EW_WAVES * SP_LEAVES_PER_WAVE;
+constexpr uint64_t SP_FLAG_AGG = 1ull << 62;   // tile aggregate published
+constexpr uint64_t SP_FLAG_PRE = 2ull << 62;   // tile inclusive prefix published
+constexpr uint64_t SP_VALUE = (1ull << 62) - 1;
+constexpr uint32_t SP_SPIN_LIMIT = 1u << 22;   // bounded wait: a stuck look-back reports an error
+
+// Single traversal in leaf order: a block takes the next tile ticket (16
+// consecutive query leaves, 4 per wave), counts its edges exactly (the COUNT
+// body), gets its dense output offset by decoupled look-back over the tile
+// states of earlier tickets, then emits (the FILL body) at that offset.  No
+// count kernel, no scan, no host round trip before the emit.  A tile whose
+// range would pass cap_edges writes nothing and raises flags[0]; the host
+// then reruns with room for the total the last tile reports in flags[2..3].
+// Tickets are taken in order by running blocks, so every tile a look-back
+// waits on is held by a block that is already running and never waits on a
+// later tile: the wait always ends (and is bounded anyway).
+__global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
+k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
+               const BuildParams* __restrict__ qprm, const float4* __restrict__ tsc, const int2* __restrict__ lists,
+               const int32_t* __restrict__ lcount, int cap, float r_sq, float inv_r, int self_pair,
+               uint32_t* __restrict__ qcount, uint32_t* __restrict__ qoff, uint64_t* __restrict__ tile_state,
+               uint32_t* __restrict__ ticket, uint32_t* __restrict__ flags, uint64_t cap_edges, EdgeOut out,
+               int index_base) {
+    __shared__ EdgeLds<EDGE_FILL> lds;
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wtot[EW_WAVES];
+    __shared__ uint64_t s_prefix;
+    __shared__ uint32_t s_ok;
+    const int wv = threadIdx.x / OCT_WAVE;
+    const int lane = lane_id();
+    const WaveLds L = lds.wave(wv);
+    const int32_t nleaves = qprm->n_leaves;
+    const int32_t ntiles = (nleaves + SP_TILE_LEAVES - 1) / SP_TILE_LEAVES;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int32_t t = (int32_t)s_tile;
+    if (t >= ntiles) return;
+    const int32_t r_lo = t * SP_TILE_LEAVES + wv * SP_LEAVES_PER_WAVE;
+    const int32_t r_hi = min(nleaves, r_lo + SP_LEAVES_PER_WAVE);
+    // phase A: exact counts
+    uint32_t wtot = 0;
+    for (int32_t r = r_lo; r < r_hi; ++r) {
+        const NodeRec qa = qleaves[r];
+        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+        const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
         const int2* lst = lists + (int64_t)r * cap;
+        const int2 lf0 = lst[lane < ncand ? lane : 0];
+        wtot += leaf_pass<EDGE_COUNT>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount, nullptr,
+                                      out, index_base, L, lane);
+    }
+    if (lane == 0) s_wtot[wv] = wtot;
+    __syncthreads();
+    // decoupled look-back (one thread)
+    if (threadIdx.x == 0) {
+        uint64_t agg = 0;
+        for (int k = 0; k < EW_WAVES; ++k) agg += s_wtot[k];
+        __hip_atomic_store(&tile_state[t], SP_FLAG_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t excl = 0;
+        uint32_t ok = 1, spins = 0;
+        for (int32_t k = t - 1; k >= 0;) {
+            const uint64_t v = __hip_atomic_load(&tile_state[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((v & ~SP_VALUE) == 0) {
+                if (++spins > SP_SPIN_LIMIT) { ok = 0; break; }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += v & SP_VALUE;
+            if ((v & ~SP_VALUE) == SP_FLAG_PRE) break;
+            --k;
+        }
+        __hip_atomic_store(&tile_state[t], SP_FLAG_PRE | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!ok) atomicOr(&flags[1], 1u);
+        if (excl + agg > cap_edges) { ok = 0; atomicOr(&flags[0], 1u); }
+        if (t == ntiles - 1) {
+            __hip_atomic_store((uint64_t*)(flags + 2), excl + agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = excl;
+        s_ok = ok;
+    }
+    __syncthreads();
+    if (!s_ok) return;
+    // phase B: offsets of this wave's queries, then the emit
+    uint64_t base = s_prefix;
+    for (int k = 0; k < wv; ++k) base += s_wtot[k];
+    uint32_t run = (uint32_t)base;
+    for (int32_t r = r_lo; r < r_hi; ++r) {
+        const NodeRec qa = qleaves[r];
+        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
         for (int qb = 0; qb < qc; qb += OCT_WAVE) {
             const int nqb = min(OCT_WAVE, qc - qb);
-            // query qb + lane lives in this lane
-            float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (lane < nqb) myq = qsc[qf + qb + lane];
-            const int32_t slot = by_receiver ? __float_as_int(myq.w) : qf + qb + lane;
-            uint32_t my = 0;  // COUNT: count; FILL: next output position of this lane's query
-            if (MODE == EDGE_FILL && lane < nqb) my = qoff[slot];
-            // leaf order: this block of queries owns [blk_base, blk_base + blk_tot)
-            uint32_t blk_base = 0, blk_tot = 0;
-            bool staged = false;
-            if (MODE == EDGE_FILL && !by_receiver) {
-                blk_base = (uint32_t)__builtin_amdgcn_readfirstlane((int)my);
-                const int64_t nxt = (int64_t)qf + qb + nqb;
-                const uint32_t blk_end = nxt < nq ? qoff[nxt] : (uint32_t)n_edges;
-                blk_tot = blk_end - blk_base;
-                staged = blk_tot <= (uint32_t)EW_STAGE;
-            }
-            for (int g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
-                int2 lf = make_int2(0, 0);
-                if (g0 + lane < ncand) lf = (g0 == 0) ? lf0 : lst[g0 + lane];
-                const int incl = wave_incl_scan(lf.y);
-                const int total = __builtin_amdgcn_readlane(incl, OCT_WAVE - 1);
-                s_start[wv][lane + 1] = incl;
-                if (lane == 0) s_start[wv][0] = 0;
-                s_first[wv][lane] = lf.x;
-                __builtin_amdgcn_wave_barrier();
-                for (int t0 = 0; t0 < total; t0 += EW_TILE * OCT_WAVE) {
-                    float4 tp[EW_TILE];
-#pragma unroll
-                    for (int u = 0; u < EW_TILE; ++u) {
-                        const int ci = t0 + u * OCT_WAVE + lane;
-                        tp[u] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-                        if (ci < total) {
-                            int lo = 0, hi = OCT_WAVE;  // owner: last o with s_start[o] <= ci
-                            while (hi - lo > 1) {
-                                const int mid = (lo + hi) >> 1;
-                                if (s_start[wv][mid] <= ci) lo = mid;
-                                else hi = mid;
-                            }
-                            tp[u] = tsc[s_first[wv][lo] + (ci - s_start[wv][lo])];
-                        }
-                    }
-                    const int nu = min(EW_TILE, (total - t0 + OCT_WAVE - 1) / OCT_WAVE);
-                    for (int qi = 0; qi < nqb; ++qi) {
-                        const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
-                        const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
-                        const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
-                        const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
-                        uint32_t run = MODE == EDGE_FILL ? (uint32_t)__builtin_amdgcn_readlane((int)my, qi) : 0u;
-                        const uint32_t run0 = run;
-#pragma unroll
-                        for (int u = 0; u < EW_TILE; ++u) {
-                            if (u >= nu) break;
-                            const int32_t j = __float_as_int(tp[u].w);
-                            const float dx = qx - tp[u].x;
-                            const float dy = qy - tp[u].y;
-                            const float dz = qz - tp[u].z;
-                            const float d2 = dx * dx + dy * dy + dz * dz;
-                            const bool pass = j >= 0 && d2 <= r_sq && !(self_pair && j == i);
-                            const uint64_t m = ballot(pass);
-                            if (MODE == EDGE_FILL && pass) {
-                                const uint32_t e = run + (uint32_t)mbcnt(m);
-                                if (staged) {
-                                    const uint32_t s = e - blk_base;
-                                    s_sen[wv][s] = j + index_base;
-                                    s_rcv[wv][s] = i + index_base;
-                                    s_disp[wv][3 * s + 0] = dx * inv_r;
-                                    s_disp[wv][3 * s + 1] = dy * inv_r;
-                                    s_disp[wv][3 * s + 2] = dz * inv_r;
-                                    s_dist[wv][s] = sqrtf(d2) * inv_r;
-                                } else {
-                                    out.senders[e] = j + index_base;
-                                    out.receivers[e] = i + index_base;
-                                    out.rdisp[3 * (int64_t)e + 0] = dx * inv_r;
-                                    out.rdisp[3 * (int64_t)e + 1] = dy * inv_r;
-                                    out.rdisp[3 * (int64_t)e + 2] = dz * inv_r;
-                                    out.rdist[e] = sqrtf(d2) * inv_r;
-                                }
-                            }
-                            run += (uint32_t)__popcll(m);
-                        }
-                        if (lane == qi) my += run - run0;
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
-            if (MODE == EDGE_COUNT && lane < nqb) qcount[slot] = my;
-            if (MODE == EDGE_FILL && staged) {
-                __builtin_amdgcn_wave_barrier();
-                for (uint32_t t = lane; t < blk_tot; t += OCT_WAVE) {
-                    out.senders[blk_base + t] = s_sen[wv][t];
-                    out.receivers[blk_base + t] = s_rcv[wv][t];
-                    out.rdist[blk_base + t] = s_dist[wv][t];
-                }
-                for (uint32_t t = lane; t < 3 * blk_tot; t += OCT_WAVE)
-                    out.rdisp[3 * (int64_t)blk_base + t] = s_disp[wv][t];
-                __builtin_amdgcn_wave_barrier();
-            }
+            const uint32_t c = lane < nqb ? qcount[qf + qb + lane] : 0u;  // written by this lane in phase A
+            const uint32_t incl = (uint32_t)wave_incl_scan((int)c);
+            if (lane < nqb) qoff[qf + qb + lane] = run + incl - c;
+            run += (uint32_t)__builtin_amdgcn_readlane((int)incl, OCT_WAVE - 1);
         }
+    }
+    for (int32_t r = r_lo; r < r_hi; ++r) {
+        const NodeRec qa = qleaves[r];
+        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+        const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+        const int2* lst = lists + (int64_t)r * cap;
+        const int2 lf0 = lst[lane < ncand ? lane : 0];
+        leaf_pass<EDGE_FILL>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount, qoff, out,
+                             index_base, L, lane);
     }
 }
 
@@ -270,17 +422,33 @@ hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const in
                         const int32_t* lcount, int cap, float r_sq, float inv_r, int self_pair,
                         int by_receiver, uint32_t* qcount, const uint32_t* qoff, int64_t n_edges, EdgeOut out,
                         int index_base, hipStream_t s) {
+    (void)n_edges;
     // about four leaves per wave, XCD-local contiguous ranges
     const int64_t want = ((int64_t)q.n_leaves + 4 * EW_WAVES - 1) / (4 * EW_WAVES);
     const unsigned g = (unsigned)(want < 1 ? 1 : (want > 16384 ? 16384 : want));
     if (mode == EDGE_COUNT)
-        hipLaunchKernelGGL(k_edges<EDGE_COUNT>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves,
-                           q.scoord, q.params, q.n, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,
-                           by_receiver, qcount, qoff, n_edges, out, index_base);
+        hipLaunchKernelGGL(k_edges<EDGE_COUNT>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
+                           q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, by_receiver, qcount, qoff,
+                           out, index_base);
     else
-        hipLaunchKernelGGL(k_edges<EDGE_FILL>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves,
-                           q.scoord, q.params, q.n, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair,
-                           by_receiver, qcount, qoff, n_edges, out, index_base);
+        hipLaunchKernelGGL(k_edges<EDGE_FILL>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
+                           q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, by_receiver, qcount, qoff,
+                           out, index_base);
+    return hipGetLastError();
+}
+
+int64_t single_tiles(int64_t n_leaves) { return (n_leaves + SP_TILE_LEAVES - 1) / SP_TILE_LEAVES; }
+
+// tile_state (single_tiles words of 8 bytes) and ticket must be zero.
+hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
+                               int cap, float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
+                               uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint64_t cap_edges,
+                               EdgeOut out, int index_base, hipStream_t s) {
+    const int64_t tiles = single_tiles(q.n_leaves);
+    if (tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_edges_single, dim3((unsigned)tiles), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
+                       q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, tile_state, ticket,
+                       flags, cap_edges, out, index_base);
     return hipGetLastError();
 }
 

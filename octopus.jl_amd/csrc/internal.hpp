@@ -96,6 +96,13 @@ hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const in
                         int by_receiver, uint32_t* qcount, const uint32_t* qoff, int64_t n_edges, EdgeOut out,
                         int index_base, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+int64_t single_tiles(int64_t n_leaves);
+// Single pass: flags[0] capacity exceeded, flags[1] look-back gave up,
+// flags[2..3] 64-bit edge total (written by the last tile).
+hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
+                               int cap, float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
+                               uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint64_t cap_edges,
+                               EdgeOut out, int index_base, hipStream_t s);
 
 // Adjoint (see backward.hip).
 hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* senders,

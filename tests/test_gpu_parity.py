@@ -322,3 +322,47 @@ def test_backward_partial_tangents(orc):
         s, rc, d, n = (_np(e.senders), _np(e.receivers), _np(e.rel_displacement), _np(e.rel_dist_norm).reshape(-1))
         want = orc.pullback(3000, s, rc, d, n, None if wd == 0 else 2 * d, None if wn == 0 else 2 * n, r)
         np.testing.assert_allclose(_np(x.grad), want, rtol=1e-4, atol=1e-3)
+
+
+# ------------------------------------------------------------- single pass
+
+SINGLE_CASES = ["clustered2000", "coincident100", "huge_radius", "plain100k", "diff262k", "n200_r015"]
+
+
+@pytest.mark.parametrize("case", SINGLE_CASES)
+def test_single_pass_parity(orc, case):
+    """mode="learned": one traversal with in-kernel exact counts and a
+    look-back for the offsets; the first call sizes the output buffers, the
+    second runs the single pass.  Edge set and features bit-exact."""
+    import octohip
+
+    make, r = EDGE_CASES[case]
+    pts = make()
+    print(f"single pass case={case} n={pts.shape[0]}", flush=True)
+    tns, sid, _ = _build(pts, r)
+    o = orc.build_edges(orc.run(pts, r), pts)
+    want = (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm)
+    for call in range(2):
+        e = octohip.build_edges(tns, sid, sid, mode="learned")
+        got = _gpu_edges(e)
+        assert got[0].shape[0] == o.n_edges, (call, got[0].shape[0], o.n_edges)
+        _edges_equal(got, want)
+        _assert_receiver_runs(got[1])
+
+
+def test_single_pass_capacity_regrow(orc):
+    """A larger point set after a small one: the single pass runs out of room,
+    reports its total and is rerun with a larger buffer."""
+    import octohip
+
+    small = datasets.uniform_cube(2000, 81)
+    big = datasets.uniform_cube(30000, 82)
+    r = 0.05
+    tns, sid, _ = _build(small, r)
+    octohip.build_edges(tns, sid, sid, mode="learned")
+    octohip.build_edges(tns, sid, sid, mode="learned")
+    octohip.resize_point_set(tns, sid, _gpu(big))
+    octohip.run(tns)
+    e = octohip.build_edges(tns, sid, sid, mode="learned")
+    o = orc.build_edges(orc.run(big, r), big)
+    _edges_equal(_gpu_edges(e), (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
