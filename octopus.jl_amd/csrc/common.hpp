@@ -118,6 +118,10 @@ __device__ inline int wave_sum(int v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, OCT_WAVE);
     return v;
 }
+__device__ inline uint64_t wave_sum_u64(uint64_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, OCT_WAVE);
+    return v;
+}
 // inclusive prefix sum across the wave
 __device__ inline int wave_incl_scan(int v) {
     int lane = lane_id();

@@ -18,79 +18,101 @@ namespace oct {
 
 // ---------------------------------------------------------------- stage 1
 
-static_assert(1 + 7 * OCT_MAX_LEVEL <= 192, "LIFO walk bound (src/cpu/build_cpu.jl:25-43) must fit the stack");
-constexpr int LL_STACK = 192;                              // room for multi-pops
-constexpr int LL_POP = 2;
+static_assert(1 + 7 * OCT_MAX_LEVEL <= 160, "LIFO walk bound (src/cpu/build_cpu.jl:25-43) must fit the stack");
+constexpr int LL_SAFE_STACK = 160;  // entries of the safe walk (scratch)
+constexpr int LL_FAST_STACK = 32;   // entries of the fast walk (LDS, 4 bytes each)
+constexpr int LL_THREADS = 256;
 
-// One thread per query leaf: depth-first walk of the target tree from its
-// branch root.  A stack entry is the child range of a node that passed the
-// test, so each round costs one dependent load; up to LL_POP entries are
-// expanded per round to overlap their loads.  Popping one entry at a time is
-// plain LIFO depth-first order, whose stack never exceeds 1 + 7 * 21 entries
-// (the reference's bound); multi-pop rounds are only taken while the stack
-// has room for all their pushes.  Every push is bounds-checked: a walk that
-// would still run out of room stops, and the leaf is flagged so the host
-// reruns the lists with safe = 1 (single pops, inside the proven bound).
-__global__ void __launch_bounds__(256)
+// Appends a passing child: a leaf goes to the list, an internal node's child
+// range to the stack.  Returns false when the stack has no room.
+template <typename Push>
+__device__ inline bool ll_visit(const float4& qlo, const float4& qhi, const NodeRec& ch, float r_sq, int2* out,
+                                int cap, int& cnt, Push push) {
+    if (box_box_d2(qlo, qhi, ch.lo, ch.hi) > r_sq) return true;
+    if (rec_b(ch) > 0) {
+        if (cnt < cap) out[cnt] = make_int2(rec_a(ch), rec_b(ch));
+        ++cnt;
+        return true;
+    }
+    return push(rec_a(ch), -rec_b(ch));
+}
+
+// One thread per query leaf: LIFO depth-first walk of the target tree from
+// its branch root.  A stack entry is the child range of a node that passed
+// the test, so each round costs one dependent load of up to 8 records.
+//   FAST: the stack lives in LDS (column per thread, entries packed as
+//         first_child * 8 + nchild - 1, which needs fewer than 2^29 nodes) and
+//         holds LL_FAST_STACK entries; a walk that needs more stops and the
+//         leaf is flagged (flags[1]) so the host reruns the lists in safe mode.
+//   safe: the stack holds 1 + 7 * 21 entries, the bound of a LIFO walk over a
+//         21-level tree (src/cpu/build_cpu.jl:25-43), so it never runs out.
+// A list longer than cap is counted in full and flagged through flags[0] (the
+// host regrows cap to that count and reruns).
+template <bool FAST>
+__global__ void __launch_bounds__(LL_THREADS)
 k_leaf_lists(const NodeRec* __restrict__ qleaves,
              const BuildParams* __restrict__ qprm, const NodeRec* __restrict__ tnodes,
-             const BuildParams* __restrict__ tprm, float r_sq, int cap, int safe, int2* __restrict__ lists,
+             const BuildParams* __restrict__ tprm, float r_sq, int cap, int2* __restrict__ lists,
              int32_t* __restrict__ lcount, uint32_t* __restrict__ flags) {
+    __shared__ uint32_t fstk[FAST ? LL_FAST_STACK : 1][LL_THREADS];
     const int32_t nleaves = qprm->n_leaves;
     const int32_t root = tprm->branch_root;
     const int32_t lb = (int32_t)xcd_block(blockIdx.x, gridDim.x);
-    for (int32_t r = lb * (int32_t)blockDim.x + (int32_t)threadIdx.x; r < nleaves;
-         r += (int32_t)(gridDim.x * blockDim.x)) {
+    const int tid = threadIdx.x;
+    for (int32_t r = lb * (int32_t)blockDim.x + tid; r < nleaves; r += (int32_t)(gridDim.x * blockDim.x)) {
         const NodeRec qa = qleaves[r];
         int2* out = lists + (int64_t)r * cap;
         int cnt = 0;
-        bool stack_full = false;
+        bool full = false;
         const NodeRec rr = tnodes[root];
         if (box_box_d2(qa.lo, qa.hi, rr.lo, rr.hi) <= r_sq) {
             if (rec_b(rr) > 0) {
                 if (cnt < cap) out[cnt] = make_int2(rec_a(rr), rec_b(rr));
                 ++cnt;
-            } else {
-                int2 stack[LL_STACK];
+            } else if (FAST) {
                 int sp = 0;
-                stack[sp++] = make_int2(rec_a(rr), -rec_b(rr));
-                while (sp > 0 && !stack_full) {
-                    int k = 1;
-                    if (!safe) {
-                        k = sp < LL_POP ? sp : LL_POP;
-                        while (k > 1 && sp - k + 8 * k > LL_STACK) --k;  // pushes must fit
-                    }
-                    int2 ent[LL_POP];
+                auto push = [&](int32_t first, int32_t nch) {
+                    if (sp >= LL_FAST_STACK) return false;
+                    fstk[sp++][tid] = ((uint32_t)first << 3) | (uint32_t)(nch - 1);
+                    return true;
+                };
+                push(rec_a(rr), -rec_b(rr));
+                while (sp > 0 && !full) {
+                    const uint32_t ent = fstk[--sp][tid];
+                    const int32_t first = (int32_t)(ent >> 3);
+                    const int32_t nch = (int32_t)(ent & 7u) + 1;
+                    NodeRec ch[8];
 #pragma unroll
-                    for (int i = 0; i < LL_POP; ++i) ent[i] = i < k ? stack[sp - 1 - i] : make_int2(0, 0);
-                    sp -= k;
-                    NodeRec ch[LL_POP][8];
+                    for (int c = 0; c < 8; ++c)
+                        if (c < nch) ch[c] = tnodes[first + c];
 #pragma unroll
-                    for (int i = 0; i < LL_POP; ++i)
+                    for (int c = 0; c < 8; ++c)
+                        if (c < nch && !ll_visit(qa.lo, qa.hi, ch[c], r_sq, out, cap, cnt, push)) full = true;
+                }
+            } else {
+                int2 stack[LL_SAFE_STACK];
+                int sp = 0;
+                auto push = [&](int32_t first, int32_t nch) {
+                    if (sp >= LL_SAFE_STACK) return false;
+                    stack[sp++] = make_int2(first, nch);
+                    return true;
+                };
+                push(rec_a(rr), -rec_b(rr));
+                while (sp > 0 && !full) {
+                    const int2 ent = stack[--sp];
+                    NodeRec ch[8];
 #pragma unroll
-                        for (int c = 0; c < 8; ++c)
-                            if (c < ent[i].y) ch[i][c] = tnodes[ent[i].x + c];
+                    for (int c = 0; c < 8; ++c)
+                        if (c < ent.y) ch[c] = tnodes[ent.x + c];
 #pragma unroll
-                    for (int i = 0; i < LL_POP; ++i)
-#pragma unroll
-                        for (int c = 0; c < 8; ++c) {
-                            if (c >= ent[i].y) continue;
-                            if (box_box_d2(qa.lo, qa.hi, ch[i][c].lo, ch[i][c].hi) > r_sq) continue;
-                            if (rec_b(ch[i][c]) > 0) {
-                                if (cnt < cap) out[cnt] = make_int2(rec_a(ch[i][c]), rec_b(ch[i][c]));
-                                ++cnt;
-                            } else if (sp < LL_STACK) {
-                                stack[sp++] = make_int2(rec_a(ch[i][c]), -rec_b(ch[i][c]));
-                            } else {
-                                stack_full = true;
-                            }
-                        }
+                    for (int c = 0; c < 8; ++c)
+                        if (c < ent.y && !ll_visit(qa.lo, qa.hi, ch[c], r_sq, out, cap, cnt, push)) full = true;
                 }
             }
         }
         lcount[r] = cnt;
         if (cnt > cap) atomicMax(&flags[0], (uint32_t)cnt);  // the host regrows cap to this
-        if (stack_full) atomicOr(&flags[1], 1u);             // the host reruns with safe = 1
+        if (full) atomicOr(&flags[1], 1u);                   // the host reruns in safe mode
     }
 }
 
@@ -235,6 +257,100 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, int32_t ncand, int2
     return leaf_total;
 }
 
+// A leaf whose candidates fit one register tile (at most 64 queries, 64
+// candidate leaves and 4 x 64 candidate points) is handled in one sweep: the
+// loop is query-outer, so the hits come out grouped by query, queries in
+// sorted order and each query's hits in candidate order, which is exactly the
+// output order.  Hits are appended to the wave's LDS hit list as (query
+// sorted position, target sorted position) while they fit; the return value
+// is the exact hit count either way (a count above `room` means the list
+// could not hold them all).
+__device__ inline bool one_tile_leaf(int32_t qc, int32_t ncand, int total) {
+    return qc <= OCT_WAVE && ncand <= OCT_WAVE && total <= EW_TILE * OCT_WAVE;
+}
+
+__device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, int32_t ncand, int2 lf0,
+                                             const float4* __restrict__ qsc, const float4* __restrict__ tsc,
+                                             float r_sq, int self_pair, const WaveLds& L, int lane, int2* hits,
+                                             uint32_t room) {
+    const int nqb = qc;
+    float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane < nqb) myq = qsc[qf + lane];
+    int2 lf = make_int2(0, 0);
+    if (lane < ncand) lf = lf0;
+    const int incl = wave_incl_scan(lf.y);
+    const int total = __builtin_amdgcn_readlane(incl, OCT_WAVE - 1);
+    L.start[lane + 1] = incl;
+    if (lane == 0) L.start[0] = 0;
+    L.first[lane] = lf.x;
+    __builtin_amdgcn_wave_barrier();
+    float4 tp[EW_TILE];
+    int32_t tpos[EW_TILE];
+#pragma unroll
+    for (int u = 0; u < EW_TILE; ++u) {
+        const int ci = u * OCT_WAVE + lane;
+        tp[u] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+        tpos[u] = 0;
+        if (ci < total) {
+            int lo = 0, hi = OCT_WAVE;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (L.start[mid] <= ci) lo = mid;
+                else hi = mid;
+            }
+            tpos[u] = L.first[lo] + (ci - L.start[lo]);
+            tp[u] = tsc[tpos[u]];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int nu = min(EW_TILE, (total + OCT_WAVE - 1) / OCT_WAVE);
+    uint32_t run = 0;
+    for (int qi = 0; qi < nqb; ++qi) {
+        const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
+        const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
+        const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
+        const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
+#pragma unroll
+        for (int u = 0; u < EW_TILE; ++u) {
+            if (u >= nu) break;
+            const int32_t j = __float_as_int(tp[u].w);
+            const float dx = qx - tp[u].x;
+            const float dy = qy - tp[u].y;
+            const float dz = qz - tp[u].z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            const bool pass = j >= 0 && d2 <= r_sq && !(self_pair && j == i);
+            const uint64_t m = ballot(pass);
+            if (pass) {
+                const uint32_t h = run + (uint32_t)mbcnt(m);
+                if (h < room) hits[h] = make_int2(qf + qi, tpos[u]);
+            }
+            run += (uint32_t)__popcll(m);
+        }
+    }
+    return run;
+}
+
+// Writes n recorded hits at out[base ..): every lane takes consecutive hits,
+// so the stores of one instruction are contiguous.
+__device__ inline void emit_hits(const int2* hits, uint32_t n, uint32_t base, const float4* __restrict__ qsc,
+                                 const float4* __restrict__ tsc, float inv_r, EdgeOut out, int index_base,
+                                 int lane) {
+    for (uint32_t t = lane; t < n; t += OCT_WAVE) {
+        const int2 h = hits[t];
+        const float4 q = qsc[h.x];
+        const float4 p = tsc[h.y];
+        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        const uint32_t e = base + t;
+        out.senders[e] = __float_as_int(p.w) + index_base;
+        out.receivers[e] = __float_as_int(q.w) + index_base;
+        out.rdisp[3 * (int64_t)e + 0] = dx * inv_r;
+        out.rdisp[3 * (int64_t)e + 1] = dy * inv_r;
+        out.rdisp[3 * (int64_t)e + 2] = dz * inv_r;
+        out.rdist[e] = sqrtf(d2) * inv_r;
+    }
+}
+
 template <int MODE>
 struct EdgeLds {
     static constexpr int STAGE = MODE == EDGE_FILL ? EW_STAGE : 1;
@@ -298,16 +414,59 @@ constexpr uint64_t SP_FLAG_PRE = 2ull << 62;   // tile inclusive prefix publishe
 constexpr uint64_t SP_VALUE = (1ull << 62) - 1;
 constexpr uint32_t SP_SPIN_LIMIT = 1u << 22;   // bounded wait: a stuck look-back reports an error
 
+constexpr int SP_HITS = 1024;  // hit list entries per wave (8 KB of LDS)
+static_assert(6 * EW_STAGE * 4 <= SP_HITS * 8, "the FILL stage is laid over the hit list");
+
+// Decoupled look-back by one whole wave: lanes read the states of the 64
+// tiles before `hi` at once, the nearest tile with a published inclusive
+// prefix ends the walk, and every tile between must have published at least
+// its aggregate (else the wave waits and reads the window again).  Tiles
+// before 0 count as an inclusive prefix of 0.  Returns the exclusive prefix of
+// tile t; *ok = 0 when the bounded wait ran out.
+__device__ inline uint64_t wave_look_back(const uint64_t* tile_state, int32_t t, int lane, uint32_t* ok) {
+    uint64_t excl = 0;
+    uint32_t spins = 0;
+    int32_t hi = t - 1;
+    while (hi >= 0) {
+        const int32_t k = hi - lane;  // lane 0 is the nearest tile
+        const uint64_t v = k >= 0 ? __hip_atomic_load(&tile_state[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : SP_FLAG_PRE;
+        const uint64_t flag = v & ~SP_VALUE;
+        const uint64_t pre = ballot(flag == SP_FLAG_PRE);
+        const uint64_t none = ballot(flag == 0);
+        // lanes up to and including the nearest inclusive prefix (or all 64)
+        const uint64_t need = pre ? (pre ^ (pre - 1)) : ~0ull;
+        if (none & need) {
+            if (++spins > SP_SPIN_LIMIT) {
+                *ok = 0;
+                return excl;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += wave_sum_u64(((need >> lane) & 1ull) ? (v & SP_VALUE) : 0ull);
+        if (pre) break;
+        hi -= OCT_WAVE;
+    }
+    return excl;
+}
+
 // Single traversal in leaf order: a block takes the next tile ticket (16
-// consecutive query leaves, 4 per wave), counts its edges exactly (the COUNT
-// body), gets its dense output offset by decoupled look-back over the tile
-// states of earlier tickets, then emits (the FILL body) at that offset.  No
-// count kernel, no scan, no host round trip before the emit.  A tile whose
-// range would pass cap_edges writes nothing and raises flags[0]; the host
-// then reruns with room for the total the last tile reports in flags[2..3].
-// Tickets are taken in order by running blocks, so every tile a look-back
-// waits on is held by a block that is already running and never waits on a
-// later tile: the wait always ends (and is bounded anyway).
+// consecutive query leaves, 4 per wave).  Phase A sweeps each leaf once: a
+// leaf that fits one register tile records its hits in the wave's LDS hit
+// list (query-grouped, already in output order) and counts them; any other
+// leaf, or one whose hits no longer fit the list, is counted with the COUNT
+// body.  The tile's dense output offset then comes from a decoupled
+// look-back over the tile states of earlier tickets.  Phase B writes the
+// recorded hits with contiguous stores, then re-sweeps only the leaves that
+// were not recorded (the exact recount, with the FILL body, whose LDS stage
+// lies over the hit list once it is written out).  No count kernel, no scan
+// and no host round trip precede the emit.  A tile whose range would pass
+// cap_edges writes nothing and raises flags[0]; the host then reruns with
+// room for the total the last tile reports in flags[2..3].  Tickets are
+// taken in order by running blocks, so every tile a look-back waits on is
+// held by a block that is already running and never waits on a later tile:
+// the wait always ends (and is bounded anyway).
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
 k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
                const BuildParams* __restrict__ qprm, const float4* __restrict__ tsc, const int2* __restrict__ lists,
@@ -315,14 +474,19 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
                uint32_t* __restrict__ qcount, uint32_t* __restrict__ qoff, uint64_t* __restrict__ tile_state,
                uint32_t* __restrict__ ticket, uint32_t* __restrict__ flags, uint64_t cap_edges, EdgeOut out,
                int index_base) {
-    __shared__ EdgeLds<EDGE_FILL> lds;
+    __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
+    __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
+    __shared__ int2 s_hits[EW_WAVES][SP_HITS];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wtot[EW_WAVES];
     __shared__ uint64_t s_prefix;
     __shared__ uint32_t s_ok;
     const int wv = threadIdx.x / OCT_WAVE;
     const int lane = lane_id();
-    const WaveLds L = lds.wave(wv);
+    int2* hits = s_hits[wv];
+    int32_t* stage = reinterpret_cast<int32_t*>(hits);
+    const WaveLds L{s_start[wv], s_first[wv], stage, stage + EW_STAGE, reinterpret_cast<float*>(stage + 2 * EW_STAGE),
+                    reinterpret_cast<float*>(stage + 5 * EW_STAGE)};
     const int32_t nleaves = qprm->n_leaves;
     const int32_t ntiles = (nleaves + SP_TILE_LEAVES - 1) / SP_TILE_LEAVES;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
@@ -331,74 +495,105 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
     if (t >= ntiles) return;
     const int32_t r_lo = t * SP_TILE_LEAVES + wv * SP_LEAVES_PER_WAVE;
     const int32_t r_hi = min(nleaves, r_lo + SP_LEAVES_PER_WAVE);
-    // phase A: exact counts
+    // phase A: one sweep per leaf, exact counts
+    uint32_t cnt[SP_LEAVES_PER_WAVE];
+    uint32_t recorded = 0;  // bit k: leaf r_lo + k sits in the hit list
+    uint32_t hfill = 0;
     uint32_t wtot = 0;
-    for (int32_t r = r_lo; r < r_hi; ++r) {
+#pragma unroll
+    for (int k = 0; k < SP_LEAVES_PER_WAVE; ++k) {
+        cnt[k] = 0;
+        const int32_t r = r_lo + k;
+        if (r >= r_hi) continue;
         const NodeRec qa = qleaves[r];
         const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
         const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
         const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
         const int2* lst = lists + (int64_t)r * cap;
         const int2 lf0 = lst[lane < ncand ? lane : 0];
-        wtot += leaf_pass<EDGE_COUNT>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount, nullptr,
+        const int total = __builtin_amdgcn_readfirstlane(wave_sum(lane < ncand ? lf0.y : 0));
+        uint32_t c;
+        if (one_tile_leaf(qc, ncand, total)) {
+            c = leaf_sweep_record(qf, qc, ncand, lf0, qsc, tsc, r_sq, self_pair, L, lane, hits + hfill,
+                                  (uint32_t)SP_HITS - hfill);
+            if (c <= (uint32_t)SP_HITS - hfill) {
+                recorded |= 1u << k;
+                hfill += c;
+            } else {
+                // the list is full: count per query for the recount in phase B
+                leaf_pass<EDGE_COUNT>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount,
+                                      nullptr, out, index_base, L, lane);
+            }
+        } else {
+            c = leaf_pass<EDGE_COUNT>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount, nullptr,
                                       out, index_base, L, lane);
+        }
+        cnt[k] = c;
+        wtot += c;
     }
     if (lane == 0) s_wtot[wv] = wtot;
     __syncthreads();
-    // decoupled look-back (one thread)
-    if (threadIdx.x == 0) {
+    if (wv == 0) {
         uint64_t agg = 0;
         for (int k = 0; k < EW_WAVES; ++k) agg += s_wtot[k];
-        __hip_atomic_store(&tile_state[t], SP_FLAG_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint64_t excl = 0;
-        uint32_t ok = 1, spins = 0;
-        for (int32_t k = t - 1; k >= 0;) {
-            const uint64_t v = __hip_atomic_load(&tile_state[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((v & ~SP_VALUE) == 0) {
-                if (++spins > SP_SPIN_LIMIT) { ok = 0; break; }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+        if (lane == 0)
+            __hip_atomic_store(&tile_state[t], SP_FLAG_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t ok = 1;
+        const uint64_t excl = wave_look_back(tile_state, t, lane, &ok);
+        if (lane == 0) {
+            __hip_atomic_store(&tile_state[t], SP_FLAG_PRE | (excl + agg), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            if (!ok) atomicOr(&flags[1], 1u);
+            if (excl + agg > cap_edges) {
+                ok = 0;
+                atomicOr(&flags[0], 1u);
             }
-            excl += v & SP_VALUE;
-            if ((v & ~SP_VALUE) == SP_FLAG_PRE) break;
-            --k;
+            if (t == ntiles - 1)
+                __hip_atomic_store((uint64_t*)(flags + 2), excl + agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_prefix = excl;
+            s_ok = ok;
         }
-        __hip_atomic_store(&tile_state[t], SP_FLAG_PRE | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!ok) atomicOr(&flags[1], 1u);
-        if (excl + agg > cap_edges) { ok = 0; atomicOr(&flags[0], 1u); }
-        if (t == ntiles - 1) {
-            __hip_atomic_store((uint64_t*)(flags + 2), excl + agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_prefix = excl;
-        s_ok = ok;
     }
     __syncthreads();
     if (!s_ok) return;
-    // phase B: offsets of this wave's queries, then the emit
+    // phase B: recorded leaves straight from the hit list ...
     uint64_t base = s_prefix;
     for (int k = 0; k < wv; ++k) base += s_wtot[k];
-    uint32_t run = (uint32_t)base;
-    for (int32_t r = r_lo; r < r_hi; ++r) {
-        const NodeRec qa = qleaves[r];
-        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
-        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
-        for (int qb = 0; qb < qc; qb += OCT_WAVE) {
-            const int nqb = min(OCT_WAVE, qc - qb);
-            const uint32_t c = lane < nqb ? qcount[qf + qb + lane] : 0u;  // written by this lane in phase A
-            const uint32_t incl = (uint32_t)wave_incl_scan((int)c);
-            if (lane < nqb) qoff[qf + qb + lane] = run + incl - c;
-            run += (uint32_t)__builtin_amdgcn_readlane((int)incl, OCT_WAVE - 1);
+    uint32_t pos = (uint32_t)base;
+    uint32_t hread = 0;
+#pragma unroll
+    for (int k = 0; k < SP_LEAVES_PER_WAVE; ++k) {
+        if (recorded & (1u << k)) {
+            emit_hits(hits + hread, cnt[k], pos, qsc, tsc, inv_r, out, index_base, lane);
+            hread += cnt[k];
         }
+        pos += cnt[k];
     }
-    for (int32_t r = r_lo; r < r_hi; ++r) {
-        const NodeRec qa = qleaves[r];
-        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
-        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
-        const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
-        const int2* lst = lists + (int64_t)r * cap;
-        const int2 lf0 = lst[lane < ncand ? lane : 0];
-        leaf_pass<EDGE_FILL>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount, qoff, out,
-                             index_base, L, lane);
+    __builtin_amdgcn_wave_barrier();
+    // ... then the others re-swept; their stage lies over the hit list
+    pos = (uint32_t)base;
+#pragma unroll
+    for (int k = 0; k < SP_LEAVES_PER_WAVE; ++k) {
+        const int32_t r = r_lo + k;
+        if (r < r_hi && !(recorded & (1u << k))) {
+            const NodeRec qa = qleaves[r];
+            const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+            const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+            uint32_t run = pos;
+            for (int qb = 0; qb < qc; qb += OCT_WAVE) {
+                const int nqb = min(OCT_WAVE, qc - qb);
+                const uint32_t c = lane < nqb ? qcount[qf + qb + lane] : 0u;  // written by this lane in phase A
+                const uint32_t incl = (uint32_t)wave_incl_scan((int)c);
+                if (lane < nqb) qoff[qf + qb + lane] = run + incl - c;
+                run += (uint32_t)__builtin_amdgcn_readlane((int)incl, OCT_WAVE - 1);
+            }
+            const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+            const int2* lst = lists + (int64_t)r * cap;
+            const int2 lf0 = lst[lane < ncand ? lane : 0];
+            leaf_pass<EDGE_FILL>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount, qoff, out,
+                                 index_base, L, lane);
+        }
+        pos += cnt[k];
     }
 }
 
@@ -411,10 +606,14 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
 
 hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
                              int32_t* lcount, uint32_t* flags, hipStream_t s) {
-    const int64_t want = ((int64_t)q.n_leaves + 255) / 256;
+    const int64_t want = ((int64_t)q.n_leaves + LL_THREADS - 1) / LL_THREADS;
     const unsigned g = (unsigned)(want < 1 ? 1 : (want > 65535 ? 65535 : want));
-    hipLaunchKernelGGL(k_leaf_lists, dim3(g), dim3(256), 0, s, q.leaves, q.params, t.nodes,
-                       t.params, r_sq, cap, safe, lists, lcount, flags);
+    if (safe || t.n_nodes >= (1 << 29))
+        hipLaunchKernelGGL(k_leaf_lists<false>, dim3(g), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes,
+                           t.params, r_sq, cap, lists, lcount, flags);
+    else
+        hipLaunchKernelGGL(k_leaf_lists<true>, dim3(g), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes,
+                           t.params, r_sq, cap, lists, lcount, flags);
     return hipGetLastError();
 }
 
