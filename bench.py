@@ -33,6 +33,10 @@ def parse():
     ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M points)")
     ap.add_argument("--no-profile", action="store_true", help="time without the per-phase HIP events")
     ap.add_argument("--null-stream", action="store_true", help="run the library on torch's current stream")
+    ap.add_argument("--predictor", default="sph_lattice",
+                    help="shipped predictor for the learned mode ('none': plan-free single pass)")
+    ap.add_argument("--predict-always", action="store_true",
+                    help="run the predictor on every step (default: first call and after overflows / recounts)")
     return ap.parse_args()
 
 
@@ -67,15 +71,28 @@ def main():
     octohip.set_search_radius(tns, float(r))
     sid = octohip.add_point_set(tns, snaps[0])
     octohip.set_active_search(tns, sid, sid)
+    predictor = None
+    if a.mode == "learned" and a.predictor != "none":
+        path = os.path.join(octohip.predictor.SHIPPED, a.predictor + ".f32")
+        if os.path.exists(path):
+            octohip.predictor.load(tns, a.predictor)
+            predictor = a.predictor
+
+    if predictor and a.predict_always:
+        octohip.predictor.set_policy(tns, True)
+    used = [0]
 
     def step(k):
         octohip.update_point_set(tns, sid, snaps[k])
         octohip.run(tns)
         e = octohip.build_edges(tns, sid, sid, mode=a.mode)
+        if predictor:
+            used[0] += int(octohip.predictor.edge_stats(tns, sid, sid)["used_predictor"])
         return e.senders.shape[0]
 
     for k in range(a.warmup):
         step(k)
+    used[0] = 0
     torch.cuda.synchronize()
     if not a.no_profile:
         octohip.profile(tns, True)
@@ -89,6 +106,7 @@ def main():
     t1 = time.perf_counter()
     prof = octohip.profile_read(tns, reset=True)
     octohip.profile(tns, False)
+    st = octohip.predictor.edge_stats(tns, sid, sid)
     sec = t1 - t0
     ms_step = 1e3 * sec / a.steps
     e_avg = edges / a.steps
@@ -100,7 +118,7 @@ def main():
     alg_bytes = 24.0 * e_avg + 12.0 * n  # 24 B per emitted edge + 12 B per query point read
     achieved = alg_bytes / k_avg_s / 1e9 if k_avg_s > 0 else 0.0
     run_ms = (prof["bounds"][0] + prof["build"][0]) / a.steps
-    edges_ms = sum(prof[p][0] for p in ("leaf_lists", "count", "scan", "fill", "single")) / a.steps
+    edges_ms = sum(prof[p][0] for p in ("leaf_lists", "count", "scan", "fill", "single", "predict")) / a.steps
     step_bytes = 16.0 * n + 24.0 * e_avg + 12.0 * n  # coords + perm write, edge output, query reads (tree bytes excluded)
 
     cpu = None
@@ -135,7 +153,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"SPH block: {n} FP32 points on a {a.side}^3 lattice, h=0.01, jitter 0.1h, "
                                f"radius 2h, N(0,(0.01h)^2) step displacement; update + run! + build_edges per step",
-                   "mode": a.mode, "points": n, "edges_per_step": e_avg, "radius": float(r)},
+                   "mode": a.mode, "points": n, "edges_per_step": e_avg, "radius": float(r),
+                   "predictor": predictor, "predict_always": bool(a.predict_always),
+                   "timed_steps_using_predictor": used[0]},
+        "last_step": {k: st[k] for k in ("single_pass", "used_predictor", "attempts", "recounted_leaves",
+                                         "recorded_leaves", "predicted_total", "n_edges", "n_leaves")},
         "breakdown_ms_per_step": {"run": run_ms, "build_edges": edges_ms,
                                   **{k: v[0] / a.steps for k, v in prof.items() if v[1]}},
         "points_per_s": n * a.steps / sec,

@@ -154,12 +154,78 @@ int oct_build_edges_backward(oct_ctx* ctx, int64_t n_points, const oct_edges* ed
 #define OCT_PROF_FILL 5        /* edge emission */
 #define OCT_PROF_BACKWARD 6    /* build_edges_diff adjoint */
 #define OCT_PROF_SINGLE 7      /* learned single pass */
-#define OCT_PROF_NPHASES 8
+#define OCT_PROF_PREDICT 8     /* predictor features, inference and plan */
+#define OCT_PROF_NPHASES 9
 int oct_profile(oct_ctx* ctx, int enable);
 int oct_profile_read(oct_ctx* ctx, int reset, double* ms, int64_t* calls);
 
 /* Block until all work queued on the context's stream is done. */
 int oct_synchronize(oct_ctx* ctx);
+
+/* ---- Learned per-leaf hit-count predictor ------------------------------
+ * No counterpart in the reference (whose GPU build_edges always runs the
+ * exact count pass, ext/OctopusCUDAExt.jl:373-387): a 29 -> 64 -> 64 -> 1
+ * ReLU MLP over per-leaf features predicts log1p(hits) of every query leaf.
+ * OCT_EDGES_LEARNED uses the predictions to size the edge buffers on a
+ * pair's first call and to plan which leaves each wave emits in the single
+ * pass (see oct_set_predictor_policy); the edges never depend on it (every leaf is counted exactly before
+ * anything is written, a misjudged leaf is recounted).  Parameter block
+ * (OCT_MLP_PARAMS floats): feature mean[29], feature std[29], W1[29*64]
+ * (row = input), b1[64], W2[64*64] (row = input), b2[64], W3[64], b3[1]. */
+#define OCT_MLP_FEATURES 29
+#define OCT_MLP_HIDDEN 64
+#define OCT_MLP_PARAMS 6203
+#define OCT_MLP_TRAINED (OCT_MLP_PARAMS - 2 * OCT_MLP_FEATURES)
+
+/* When OCT_EDGES_LEARNED runs the predictor: by default on a pair's first
+ * call and again after a call that needed a second attempt, fell back to the
+ * exact path or recounted more than 1/64 of its leaves (otherwise the call
+ * reuses the buffers and fixed runs of 4 leaves per wave); always != 0: on
+ * every call. */
+int oct_set_predictor_policy(oct_ctx* ctx, int always);
+/* Install a parameter block from host memory (NULL removes the predictor). */
+int oct_predictor_load(oct_ctx* ctx, const float* host_params);
+/* Copy the installed parameter block to host memory (OCT_ERR_STATE if none). */
+int oct_predictor_save(oct_ctx* ctx, float* host_params);
+/* Count the exact per-leaf hits of the active pair (q, t) (the count pass
+ * only, no edges are written) and append its rows (features, log1p of the
+ * hits) to the training set; writes the number of rows added. */
+int oct_predictor_collect(oct_ctx* ctx, int q, int t, int64_t* rows);
+/* Drop the training set. */
+int oct_predictor_clear_data(oct_ctx* ctx);
+/* Adam on the training set, `steps` steps of `batch` hashed rows (seed
+ * picks them).  With init != 0, or no predictor installed, the network
+ * starts from seeded He-normal weights and the feature statistics of the
+ * training set.  Writes the mean squared error of the last batch. */
+int oct_predictor_train(oct_ctx* ctx, int steps, int batch, float lr, uint64_t seed, int init, float* loss);
+
+/* What the last build of the pair did, with device arrays (valid until the
+ * next build on the pair) of per-leaf features (n_leaves * 29, row-major),
+ * predicted hits and exact hits; predicted is NULL when the predictor was not
+ * used, features NULL when they were not computed. */
+typedef struct oct_edge_stats {
+    const float* features;
+    const float* predicted;
+    const uint32_t* exact;
+    int64_t n_leaves;
+    int64_t n_edges;
+    int64_t recounted_leaves;  /* single pass: leaves swept twice */
+    int64_t recorded_leaves;   /* single pass: leaves swept once */
+    double predicted_total;    /* sum of the per-leaf predictions */
+    int32_t single_pass;       /* the edges came from the single pass */
+    int32_t used_predictor;
+    int32_t attempts;          /* passes the call needed (1 when nothing overflowed) */
+    int32_t pad;
+} oct_edge_stats;
+int oct_edge_stats_get(oct_ctx* ctx, int q, int t, oct_edge_stats* out);
+
+/* The network on n rows of features (device arrays): y = log1p(hits). */
+int oct_mlp_forward(oct_ctx* ctx, const float* params, const float* features, int64_t n, float* y);
+/* Gradient of mean((f(x) - target)^2) over n rows with respect to the
+ * OCT_MLP_TRAINED trained parameters (the block without the feature
+ * statistics, same order); loss (device, 1 float) may be NULL. */
+int oct_mlp_gradient(oct_ctx* ctx, const float* params, const float* features, const float* targets, int n,
+                     float* grad, float* loss);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -93,12 +94,29 @@ struct PairState {
     int q = 0, t = 0;
     int cap = 64;            // candidate leaves per query leaf
     int64_t cap_edges = 0;   // edges the output buffers hold (single pass sizing)
+    bool replan = true;      // run the predictor on the next learned call
     int64_t n_edges = 0;
     bool materialized = false;
     DevBuf senders, receivers, rdisp, rdist, lists, lcount, qcount, qoff, scan_tmp, flags, tiles;
+    // predictor inputs and plan: direction codes of the list entries,
+    // features, predicted and exact per-leaf hits, plan work / scan / runs
+    DevBuf ldir, feat, pred, lhits, pwork, pscan, wave_first;
+    oct_edge_stats st{};
     void release_all() {
         DevBuf* all[] = {&senders, &receivers, &rdisp, &rdist, &lists, &lcount, &qcount, &qoff, &scan_tmp, &flags,
-                         &tiles};
+                         &tiles, &ldir, &feat, &pred, &lhits, &pwork, &pscan, &wave_first};
+        for (DevBuf* d : all) d->release();
+    }
+};
+
+// Predictor parameters, Adam state and training set of a context.
+struct Predictor {
+    bool loaded = false;
+    int adam_step = 0;
+    int64_t rows = 0;
+    DevBuf params, m, v, scratch, loss, X, T;
+    void release_all() {
+        DevBuf* all[] = {&params, &m, &v, &scratch, &loss, &X, &T};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -115,8 +133,10 @@ struct oct_ctx {
     float radius = 0.f;
     int leaf_size = 32;
     bool refit_mode = false;
+    bool predict_always = false;
     std::vector<SetState> sets;
     std::vector<PairState> pairs;
+    Predictor mlp;
     std::string err;
     void* pinned = nullptr;  // small host staging area for read-backs
     // optional per-phase kernel timing with HIP events on the context stream
@@ -205,6 +225,7 @@ TreeView view_of(const SetState& s) {
     v.leaves = s.leaves.as<NodeRec>();
     v.scoord = s.scoord.as<float4>();
     v.params = s.prm.as<BuildParams>();
+    v.nlevel = s.nlevel.as<uint8_t>();
     v.n_leaves = s.hp.n_leaves;
     v.n_nodes = s.hp.n_nodes;
     v.branch_root = s.hp.branch_root;
@@ -319,6 +340,7 @@ int oct_destroy(oct_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto& s : c->sets) s.release_all();
     for (auto& p : c->pairs) p.release_all();
+    c->mlp.release_all();
     if (c->pinned) (void)hipHostFree(c->pinned);
     for (auto& m : c->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
     for (auto e : c->spare) (void)hipEventDestroy(e);
@@ -438,7 +460,9 @@ int oct_get_tree(oct_ctx* c, int id, oct_tree* out) {
     return OCT_OK;
 }
 
-int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* out) {
+// Flat-COO edges of pair (q, t).  collect: exact path that also keeps the
+// predictor's features and the exact per-leaf hits (training rows).
+static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collect, oct_edges* out) {
     if (!c || !out) return OCT_ERR_ARGUMENT;
     PairState* pp = find_pair(c, q, t);
     if (!pp) return fail(c, OCT_ERR_STATE, "build_edges!: pair not registered; call set_active_search! first");
@@ -452,6 +476,7 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
     if (!qs.built || qs.dirty || !ts.built || ts.dirty) return fail(c, OCT_ERR_STATE, "run! before build_edges");
     HIP_TRY(c, hipSetDevice(c->device));
     memset(out, 0, sizeof(*out));
+    memset(&p.st, 0, sizeof(p.st));
     p.n_edges = 0;
     if (qs.n == 0 || ts.n == 0) {
         p.materialized = true;
@@ -464,31 +489,73 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
     const TreeView qv = view_of(qs), tv = view_of(ts);
     const int64_t nql = qs.hp.n_leaves;
     const int64_t ntiles = single_tiles(nql);
+    // the predictor runs on the pair's first call and after a call that
+    // overflowed or recounted many leaves (or on every call, if asked)
+    const bool predict = mode == OCT_EDGES_LEARNED && !collect && c->mlp.loaded &&
+                         (c->predict_always || p.cap_edges <= 0 || p.replan);
+    const bool features = predict || collect;
     HIP_TRY(c, p.flags.ensure(64));
     HIP_TRY(c, p.lcount.ensure(4 * (size_t)nql));
     HIP_TRY(c, p.qcount.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.qoff.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.scan_tmp.ensure(4 * (size_t)scan_tmp_words(qs.n)));
     HIP_TRY(c, p.tiles.ensure(8 * (size_t)(ntiles + 1)));
+    HIP_TRY(c, p.lhits.ensure(4 * (size_t)nql));
+    if (features) HIP_TRY(c, p.feat.ensure(4 * (size_t)nql * MLP_IN));
+    if (predict) {
+        HIP_TRY(c, p.pred.ensure(4 * (size_t)nql));
+        HIP_TRY(c, p.pwork.ensure(16 * (size_t)plan_chunks(nql)));
+        HIP_TRY(c, p.pscan.ensure(4 * (size_t)scan_tmp_words(plan_chunks(nql))));
+        HIP_TRY(c, p.wave_first.ensure(4 * (size_t)(nql + 2)));
+    }
     const int by_receiver = order == OCT_ORDER_RECEIVER;
-    // flags: [0] largest candidate count of an overflowing query leaf, [1] a
-    // list walk ran out of stack, [2..3] exact-path edge total, [4] single
-    // pass over capacity, [5] single-pass look-back gave up, [6..7] single-pass
-    // edge total, [8] single-pass tile ticket
+    // flags (uint32 words): [0] largest candidate count of an overflowing
+    // query leaf, [1] a list walk ran out of stack, [2..3] exact-path edge
+    // total, [4] single pass over capacity, [5] single-pass look-back gave up
+    // or malformed plan, [6..7] single-pass edge total, [8] single-pass tile
+    // ticket, [9] recounted leaves, [10] recorded leaves, [12] plan runs,
+    // [14..15] predicted total
     uint32_t* flags = p.flags.as<uint32_t>();
-    uint32_t hf[8] = {0};
+    uint32_t hf[16] = {0};
     int safe = 0;
-    bool single = mode == OCT_EDGES_LEARNED && p.cap_edges > 0;
+    bool single = mode == OCT_EDGES_LEARNED && !collect && (p.cap_edges > 0 || predict);
     bool lists_ok = false;
     uint64_t total = 0;
+    int attempts = 0;
     EdgeOut eo{};
     for (int attempt = 0; attempt < 10; ++attempt) {
+        ++attempts;
         HIP_TRY(c, p.lists.ensure(sizeof(int2) * (size_t)nql * (size_t)p.cap));
+        if (features) HIP_TRY(c, p.ldir.ensure((size_t)nql * (size_t)p.cap));
         HIP_TRY(c, launch_fill_u32(flags, 16, 0u, c->stream));
         if (!lists_ok) {
             Phase ph(c, OCT_PROF_LEAF_LISTS);
-            HIP_TRY(c, launch_leaf_lists(qv, tv, r_sq, p.cap, safe, p.lists.as<int2>(), p.lcount.as<int32_t>(),
-                                         flags, c->stream));
+            HIP_TRY(c, launch_leaf_lists(qv, tv, r_sq, p.cap, safe, p.lists.as<int2>(),
+                                         features ? p.ldir.as<uint8_t>() : nullptr, p.lcount.as<int32_t>(), flags,
+                                         c->stream));
+        }
+        if (features) {
+            Phase ph(c, OCT_PROF_PREDICT);
+            HIP_TRY(c, launch_leaf_features(qv, p.lists.as<int2>(), p.ldir.as<uint8_t>(), p.lcount.as<int32_t>(),
+                                            p.cap, r, p.feat.as<float>(), c->stream));
+            if (predict && single)
+                HIP_TRY(c, launch_predict_plan(p.feat.as<float>(), nql, c->mlp.params.as<float>(),
+                                               p.pred.as<float>(), SP_PLAN_BUDGET, SP_PLAN_MAX_LEAVES,
+                                               p.pwork.as<uint32_t>(), p.pscan.as<uint32_t>(),
+                                               p.wave_first.as<int32_t>(), (int32_t*)(flags + 12),
+                                               (uint64_t*)(flags + 14), c->stream));
+        }
+        if (single && p.cap_edges <= 0) {
+            // first call on the pair: size the outputs from the predicted total
+            HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 64, hipMemcpyDeviceToHost, c->stream));
+            {
+                HostTimer ht("edges: wait for predicted total");
+                HIP_TRY(c, hipStreamSynchronize(c->stream));
+            }
+            memcpy(hf, c->pinned, 64);
+            uint64_t pt = 0;
+            memcpy(&pt, hf + 14, 8);
+            p.cap_edges = (int64_t)std::min<uint64_t>(pt + pt / 16 + 4096, (uint64_t)INT32_MAX - 1);
         }
         if (single) {
             HIP_TRY(c, p.senders.ensure(4 * (size_t)p.cap_edges));
@@ -500,25 +567,27 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
             Phase ph(c, OCT_PROF_SINGLE);
             HIP_TRY(c, launch_edges_single(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r,
                                            self_pair, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(),
-                                           p.tiles.as<uint64_t>(), flags + 8, flags + 4, (uint64_t)p.cap_edges, eo,
-                                           c->index_base, c->stream));
+                                           predict ? p.wave_first.as<int32_t>() : nullptr,
+                                           predict ? (const int32_t*)(flags + 12) : nullptr, p.lhits.as<uint32_t>(),
+                                           p.tiles.as<uint64_t>(), flags + 8, flags + 4, flags + 9,
+                                           (uint64_t)p.cap_edges, eo, c->index_base, c->stream));
         } else {
             {
                 Phase ph(c, OCT_PROF_COUNT);
                 HIP_TRY(c, launch_edges(EDGE_COUNT, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
-                                        inv_r, self_pair, by_receiver, p.qcount.as<uint32_t>(), nullptr, 0,
-                                        EdgeOut{}, c->index_base, c->stream));
+                                        inv_r, self_pair, by_receiver, p.qcount.as<uint32_t>(), nullptr,
+                                        p.lhits.as<uint32_t>(), 0, EdgeOut{}, c->index_base, c->stream));
             }
             Phase ph(c, OCT_PROF_SCAN);
             HIP_TRY(c, exclusive_scan(p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), qs.n,
                                       p.scan_tmp.as<uint32_t>(), (uint64_t*)(flags + 2), c->stream));
         }
-        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 32, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 64, hipMemcpyDeviceToHost, c->stream));
         {
             HostTimer ht(single ? "edges: wait for single pass" : "edges: wait for count total");
             HIP_TRY(c, hipStreamSynchronize(c->stream));
         }
-        memcpy(hf, c->pinned, 32);
+        memcpy(hf, c->pinned, 64);
         if (hf[0] != 0 || hf[1] != 0) {  // candidate lists incomplete: regrow / safe walk, redo everything
             if (hf[0] != 0) p.cap = (int)std::min<uint32_t>(hf[0] + 16u, (uint32_t)INT32_MAX / 8u);
             if (hf[1] != 0) safe = 1;
@@ -531,11 +600,11 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
             break;
         }
         memcpy(&total, hf + 6, 8);
-        if (hf[5] != 0) {  // look-back did not finish: take the exact path
+        if (hf[5] != 0) {  // look-back did not finish or the plan was malformed: take the exact path
             single = false;
             continue;
         }
-        if (hf[4] != 0) {  // predicted room exceeded: grow to the reported total and rerun
+        if (hf[4] != 0) {  // room exceeded: grow to the reported total and rerun
             if (total >= (uint64_t)INT32_MAX) break;
             p.cap_edges = (int64_t)(total + total / 8 + 1024);
             continue;
@@ -547,6 +616,17 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
     if (total >= (uint64_t)INT32_MAX)
         return fail(c, OCT_ERR_CAPACITY, "edge count exceeds Int32 (the reference's limit)");
     const int64_t E = (int64_t)total;
+    if (collect) {  // counts only: the training rows need no edges
+        p.n_edges = 0;
+        p.materialized = false;
+        p.st.features = p.feat.as<float>();
+        p.st.exact = p.lhits.as<uint32_t>();
+        p.st.n_leaves = nql;
+        p.st.n_edges = E;
+        p.st.attempts = attempts;
+        out->n_edges = E;
+        return OCT_OK;
+    }
     if (!single) {
         HostTimer ht_fill("edges: allocate + launch fill");
         HIP_TRY(c, p.senders.ensure(4 * (size_t)E));
@@ -557,20 +637,39 @@ int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* ou
         if (E > 0) {
             Phase ph(c, OCT_PROF_FILL);
             HIP_TRY(c, launch_edges(EDGE_FILL, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
-                                    inv_r, self_pair, by_receiver, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), E,
-                                    eo, c->index_base, c->stream));
+                                    inv_r, self_pair, by_receiver, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(),
+                                    nullptr, E, eo, c->index_base, c->stream));
         }
     }
     // room for the single pass next time: the senders buffer's capacity
     p.cap_edges = (int64_t)std::min<size_t>(p.senders.cap / 4, (size_t)INT32_MAX - 1);
     p.n_edges = E;
     p.materialized = true;
+    uint64_t pt = 0;
+    memcpy(&pt, hf + 14, 8);
+    p.st.features = features ? p.feat.as<float>() : nullptr;
+    p.st.predicted = predict && single ? p.pred.as<float>() : nullptr;
+    p.st.exact = p.lhits.as<uint32_t>();
+    p.st.n_leaves = nql;
+    p.st.n_edges = E;
+    p.st.recounted_leaves = single ? hf[9] : 0;
+    p.st.recorded_leaves = single ? hf[10] : 0;
+    p.st.predicted_total = predict && single ? (double)pt : 0.0;
+    p.st.single_pass = single ? 1 : 0;
+    p.st.used_predictor = predict && single ? 1 : 0;
+    p.st.attempts = attempts;
+    if (mode == OCT_EDGES_LEARNED)
+        p.replan = !single || attempts > 1 || (int64_t)hf[9] * 64 > nql;
     out->senders = eo.senders;
     out->receivers = eo.receivers;
     out->rel_displacement = eo.rdisp;
     out->rel_dist_norm = eo.rdist;
     out->n_edges = E;
     return OCT_OK;
+}
+
+int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* out) {
+    return edges_impl(c, q, t, mode, order, false, out);
 }
 
 int oct_device_view(oct_ctx* c, int q, int t, oct_view* out) {
@@ -667,6 +766,179 @@ int oct_profile_read(oct_ctx* c, int reset, double* ms, int64_t* calls) {
 
 int oct_synchronize(oct_ctx* c) {
     if (!c) return OCT_ERR_ARGUMENT;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return OCT_OK;
+}
+
+// ------------------------------------------------------------ predictor
+
+namespace {
+
+// Grows a device buffer keeping its first `keep` bytes.
+int grow_keep(oct_ctx* c, DevBuf& b, size_t bytes, size_t keep) {
+    if (bytes <= b.cap && b.p) return OCT_OK;
+    DevBuf nb;
+    HIP_TRY(c, nb.ensure(bytes));
+    if (keep > 0) {
+        HIP_TRY(c, hipMemcpyAsync(nb.p, b.p, keep, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    b.release();
+    b = nb;
+    return OCT_OK;
+}
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// He-normal weights (Box-Muller over splitmix64), zero biases, identity
+// feature statistics (the caller replaces them).
+void he_init(std::vector<float>& P, uint64_t seed) {
+    P.assign(MLP_PARAMS, 0.f);
+    uint64_t s = seed ^ 0x6F63746F70757321ull;
+    auto normal = [&]() {
+        const double u1 = ((splitmix64(s) >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+        const double u2 = (splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
+        return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    };
+    for (int i = 0; i < MLP_IN; ++i) P[MLP_SD + i] = 1.f;
+    for (int i = 0; i < MLP_IN * MLP_H; ++i) P[MLP_W1 + i] = (float)(normal() * sqrt(2.0 / MLP_IN));
+    for (int i = 0; i < MLP_H * MLP_H; ++i) P[MLP_W2 + i] = (float)(normal() * sqrt(2.0 / MLP_H));
+    for (int i = 0; i < MLP_H; ++i) P[MLP_W3 + i] = (float)(normal() * sqrt(1.0 / MLP_H));
+}
+
+}  // namespace
+
+int oct_predictor_load(oct_ctx* c, const float* host_params) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!host_params) {
+        c->mlp.loaded = false;
+        return OCT_OK;
+    }
+    for (int i = 0; i < MLP_PARAMS; ++i)
+        if (!std::isfinite(host_params[i])) return fail(c, OCT_ERR_ARGUMENT, "predictor parameters must be finite");
+    for (int i = 0; i < MLP_IN; ++i)
+        if (!(host_params[MLP_SD + i] > 0.f)) return fail(c, OCT_ERR_ARGUMENT, "feature std must be positive");
+    HIP_TRY(c, c->mlp.params.ensure(4 * MLP_PARAMS));
+    HIP_TRY(c, hipMemcpyAsync(c->mlp.params.p, host_params, 4 * MLP_PARAMS, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->mlp.loaded = true;
+    c->mlp.adam_step = 0;
+    return OCT_OK;
+}
+
+int oct_predictor_save(oct_ctx* c, float* host_params) {
+    if (!c || !host_params) return OCT_ERR_ARGUMENT;
+    if (!c->mlp.loaded) return fail(c, OCT_ERR_STATE, "no predictor installed");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(host_params, c->mlp.params.p, 4 * MLP_PARAMS, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return OCT_OK;
+}
+
+int oct_predictor_collect(oct_ctx* c, int q, int t, int64_t* rows) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (rows) *rows = 0;
+    oct_edges e{};
+    int rc = edges_impl(c, q, t, OCT_EDGES_EXACT, OCT_ORDER_LEAF, true, &e);
+    if (rc != OCT_OK) return rc;
+    PairState& p = *find_pair(c, q, t);
+    const int64_t n = p.st.features ? p.st.n_leaves : 0;
+    if (n == 0) return OCT_OK;
+    Predictor& m = c->mlp;
+    rc = grow_keep(c, m.X, 4 * (size_t)(m.rows + n) * MLP_IN, 4 * (size_t)m.rows * MLP_IN);
+    if (rc != OCT_OK) return rc;
+    rc = grow_keep(c, m.T, 4 * (size_t)(m.rows + n), 4 * (size_t)m.rows);
+    if (rc != OCT_OK) return rc;
+    HIP_TRY(c, hipMemcpyAsync(m.X.as<float>() + m.rows * MLP_IN, p.st.features, 4 * (size_t)n * MLP_IN,
+                              hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, launch_log1p_u32(p.st.exact, n, m.T.as<float>() + m.rows, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    m.rows += n;
+    if (rows) *rows = n;
+    return OCT_OK;
+}
+
+int oct_predictor_clear_data(oct_ctx* c) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    c->mlp.rows = 0;
+    return OCT_OK;
+}
+
+int oct_predictor_train(oct_ctx* c, int steps, int batch, float lr, uint64_t seed, int init, float* loss) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    Predictor& m = c->mlp;
+    if (m.rows <= 0) return fail(c, OCT_ERR_STATE, "no training rows; call oct_predictor_collect first");
+    if (steps < 0 || batch < 1 || batch > (1 << 20) || !(lr > 0.f))
+        return fail(c, OCT_ERR_ARGUMENT, "train: steps >= 0, batch in 1..2^20, lr > 0");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, m.params.ensure(4 * MLP_PARAMS));
+    HIP_TRY(c, m.m.ensure(4 * MLP_TRAINED));
+    HIP_TRY(c, m.v.ensure(4 * MLP_TRAINED));
+    HIP_TRY(c, m.loss.ensure(16));
+    HIP_TRY(c, m.scratch.ensure(4 * (size_t)mlp_scratch_floats(batch)));
+    if (init || !m.loaded) {
+        std::vector<float> P;
+        he_init(P, seed);
+        HIP_TRY(c, hipMemcpyAsync(m.params.p, P.data(), 4 * MLP_PARAMS, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, launch_feature_stats(m.X.as<float>(), m.rows, m.params.as<float>(), c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        m.adam_step = 0;
+    }
+    if (m.adam_step == 0) {
+        HIP_TRY(c, launch_fill_u32(m.m.as<uint32_t>(), MLP_TRAINED, 0u, c->stream));
+        HIP_TRY(c, launch_fill_u32(m.v.as<uint32_t>(), MLP_TRAINED, 0u, c->stream));
+    }
+    for (int s = 0; s < steps; ++s) {
+        ++m.adam_step;
+        const uint64_t sseed = (seed * 0x9E3779B97F4A7C15ull + (uint64_t)m.adam_step) | 1ull;
+        HIP_TRY(c, launch_mlp_step(m.X.as<float>(), m.T.as<float>(), m.rows, batch, sseed, m.params.as<float>(),
+                                   m.m.as<float>(), m.v.as<float>(), m.adam_step, lr, m.scratch.as<float>(), nullptr,
+                                   s == steps - 1 ? m.loss.as<float>() : nullptr, c->stream));
+    }
+    float hl = 0.f;
+    if (steps > 0) HIP_TRY(c, hipMemcpyAsync(&hl, m.loss.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = hl;
+    m.loaded = true;
+    return OCT_OK;
+}
+
+int oct_set_predictor_policy(oct_ctx* c, int always) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    c->predict_always = always != 0;
+    return OCT_OK;
+}
+
+int oct_edge_stats_get(oct_ctx* c, int q, int t, oct_edge_stats* out) {
+    if (!c || !out) return OCT_ERR_ARGUMENT;
+    PairState* p = find_pair(c, q, t);
+    if (!p) return fail(c, OCT_ERR_STATE, "edge stats: pair not registered");
+    *out = p->st;
+    return OCT_OK;
+}
+
+int oct_mlp_forward(oct_ctx* c, const float* params, const float* X, int64_t n, float* y) {
+    if (!c || !params || !X || !y || n < 0) return OCT_ERR_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, launch_mlp_infer(X, n, params, y, nullptr, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return OCT_OK;
+}
+
+int oct_mlp_gradient(oct_ctx* c, const float* params, const float* X, const float* T, int n, float* grad,
+                     float* loss) {
+    if (!c || !params || !X || !T || !grad || n < 1 || n > (1 << 20)) return OCT_ERR_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    Predictor& m = c->mlp;
+    HIP_TRY(c, m.scratch.ensure(4 * (size_t)mlp_scratch_floats(n)));
+    HIP_TRY(c, launch_mlp_step(X, T, n, n, 0, const_cast<float*>(params), nullptr, nullptr, 1, 0.f,
+                               m.scratch.as<float>(), grad, loss, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return OCT_OK;
 }

@@ -23,14 +23,38 @@ constexpr int LL_SAFE_STACK = 160;  // entries of the safe walk (scratch)
 constexpr int LL_FAST_STACK = 32;   // entries of the fast walk (LDS, 4 bytes each)
 constexpr int LL_THREADS = 256;
 
-// Appends a passing child: a leaf goes to the list, an internal node's child
-// range to the stack.  Returns false when the stack has no room.
-template <typename Push>
+// Direction of box c seen from box q, one of 27 codes sx + 3 sy + 9 sz with
+// s = 0 (below), 1 (level), 2 (above) per axis: the centres must differ by
+// more than a quarter of the two extents summed to count as off-level.  Code
+// 13 is the same position.  Feeds the per-leaf count predictor's
+// adjacent-leaf features.
+__device__ inline uint8_t dir_code(const float4& qlo, const float4& qhi, const float4& clo, const float4& chi) {
+    const float d[3] = {0.5f * (clo.x + chi.x) - 0.5f * (qlo.x + qhi.x), 0.5f * (clo.y + chi.y) - 0.5f * (qlo.y + qhi.y),
+                        0.5f * (clo.z + chi.z) - 0.5f * (qlo.z + qhi.z)};
+    const float tau[3] = {0.25f * ((qhi.x - qlo.x) + (chi.x - clo.x)), 0.25f * ((qhi.y - qlo.y) + (chi.y - clo.y)),
+                          0.25f * ((qhi.z - qlo.z) + (chi.z - clo.z))};
+    int code = 0, mul = 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int s = d[k] > tau[k] ? 2 : (d[k] < -tau[k] ? 0 : 1);
+        code += s * mul;
+        mul *= 3;
+    }
+    return (uint8_t)code;
+}
+
+// Appends a passing child: a leaf goes to the list (and its direction code to
+// dout when asked), an internal node's child range to the stack.  Returns
+// false when the stack has no room.
+template <bool DIRS, typename Push>
 __device__ inline bool ll_visit(const float4& qlo, const float4& qhi, const NodeRec& ch, float r_sq, int2* out,
-                                int cap, int& cnt, Push push) {
+                                uint8_t* dout, int cap, int& cnt, Push push) {
     if (box_box_d2(qlo, qhi, ch.lo, ch.hi) > r_sq) return true;
     if (rec_b(ch) > 0) {
-        if (cnt < cap) out[cnt] = make_int2(rec_a(ch), rec_b(ch));
+        if (cnt < cap) {
+            out[cnt] = make_int2(rec_a(ch), rec_b(ch));
+            if (DIRS) dout[cnt] = dir_code(qlo, qhi, ch.lo, ch.hi);
+        }
         ++cnt;
         return true;
     }
@@ -48,12 +72,12 @@ __device__ inline bool ll_visit(const float4& qlo, const float4& qhi, const Node
 //         21-level tree (src/cpu/build_cpu.jl:25-43), so it never runs out.
 // A list longer than cap is counted in full and flagged through flags[0] (the
 // host regrows cap to that count and reruns).
-template <bool FAST>
+template <bool FAST, bool DIRS>
 __global__ void __launch_bounds__(LL_THREADS)
 k_leaf_lists(const NodeRec* __restrict__ qleaves,
              const BuildParams* __restrict__ qprm, const NodeRec* __restrict__ tnodes,
              const BuildParams* __restrict__ tprm, float r_sq, int cap, int2* __restrict__ lists,
-             int32_t* __restrict__ lcount, uint32_t* __restrict__ flags) {
+             uint8_t* __restrict__ ldir, int32_t* __restrict__ lcount, uint32_t* __restrict__ flags) {
     __shared__ uint32_t fstk[FAST ? LL_FAST_STACK : 1][LL_THREADS];
     const int32_t nleaves = qprm->n_leaves;
     const int32_t root = tprm->branch_root;
@@ -62,12 +86,16 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
     for (int32_t r = lb * (int32_t)blockDim.x + tid; r < nleaves; r += (int32_t)(gridDim.x * blockDim.x)) {
         const NodeRec qa = qleaves[r];
         int2* out = lists + (int64_t)r * cap;
+        uint8_t* dout = DIRS ? ldir + (int64_t)r * cap : nullptr;
         int cnt = 0;
         bool full = false;
         const NodeRec rr = tnodes[root];
         if (box_box_d2(qa.lo, qa.hi, rr.lo, rr.hi) <= r_sq) {
             if (rec_b(rr) > 0) {
-                if (cnt < cap) out[cnt] = make_int2(rec_a(rr), rec_b(rr));
+                if (cnt < cap) {
+                    out[cnt] = make_int2(rec_a(rr), rec_b(rr));
+                    if (DIRS) dout[cnt] = dir_code(qa.lo, qa.hi, rr.lo, rr.hi);
+                }
                 ++cnt;
             } else if (FAST) {
                 int sp = 0;
@@ -87,7 +115,7 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
                         if (c < nch) ch[c] = tnodes[first + c];
 #pragma unroll
                     for (int c = 0; c < 8; ++c)
-                        if (c < nch && !ll_visit(qa.lo, qa.hi, ch[c], r_sq, out, cap, cnt, push)) full = true;
+                        if (c < nch && !ll_visit<DIRS>(qa.lo, qa.hi, ch[c], r_sq, out, dout, cap, cnt, push)) full = true;
                 }
             } else {
                 int2 stack[LL_SAFE_STACK];
@@ -106,7 +134,7 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
                         if (c < ent.y) ch[c] = tnodes[ent.x + c];
 #pragma unroll
                     for (int c = 0; c < 8; ++c)
-                        if (c < ent.y && !ll_visit(qa.lo, qa.hi, ch[c], r_sq, out, cap, cnt, push)) full = true;
+                        if (c < ent.y && !ll_visit<DIRS>(qa.lo, qa.hi, ch[c], r_sq, out, dout, cap, cnt, push)) full = true;
                 }
             }
         }
@@ -370,7 +398,7 @@ __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
 k_edges(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm,
         const float4* __restrict__ tsc, const int2* __restrict__ lists, const int32_t* __restrict__ lcount, int cap,
         float r_sq, float inv_r, int self_pair, int by_receiver, uint32_t* __restrict__ qcount,
-        const uint32_t* __restrict__ qoff, EdgeOut out, int index_base) {
+        const uint32_t* __restrict__ qoff, uint32_t* __restrict__ lhits, EdgeOut out, int index_base) {
     __shared__ EdgeLds<MODE> lds;
     const int wv = threadIdx.x / OCT_WAVE;
     const int lane = lane_id();
@@ -400,8 +428,9 @@ k_edges(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, con
             nx_cnt = lcount[r + 1];
             if (lane < cap) nx_lf = lists[(int64_t)(r + 1) * cap + lane];
         }
-        leaf_pass<MODE>(qf, qc, ncand, lf0, lists + (int64_t)r * cap, qsc, tsc, r_sq, inv_r, self_pair, by_receiver,
-                        qcount, qoff, out, index_base, L, lane);
+        const uint32_t lt = leaf_pass<MODE>(qf, qc, ncand, lf0, lists + (int64_t)r * cap, qsc, tsc, r_sq, inv_r,
+                                            self_pair, by_receiver, qcount, qoff, out, index_base, L, lane);
+        if (MODE == EDGE_COUNT && lhits && lane == 0) lhits[r] = lt;
     }
 }
 
@@ -414,7 +443,7 @@ constexpr uint64_t SP_FLAG_PRE = 2ull << 62;   // tile inclusive prefix publishe
 constexpr uint64_t SP_VALUE = (1ull << 62) - 1;
 constexpr uint32_t SP_SPIN_LIMIT = 1u << 22;   // bounded wait: a stuck look-back reports an error
 
-constexpr int SP_HITS = 1024;  // hit list entries per wave (8 KB of LDS)
+constexpr int SP_HITS = SP_PLAN_BUDGET;  // hit list entries per wave (8 KB of LDS)
 static_assert(6 * EW_STAGE * 4 <= SP_HITS * 8, "the FILL stage is laid over the hit list");
 
 // Decoupled look-back by one whole wave: lanes read the states of the 64
@@ -451,32 +480,42 @@ __device__ inline uint64_t wave_look_back(const uint64_t* tile_state, int32_t t,
     return excl;
 }
 
-// Single traversal in leaf order: a block takes the next tile ticket (16
-// consecutive query leaves, 4 per wave).  Phase A sweeps each leaf once: a
-// leaf that fits one register tile records its hits in the wave's LDS hit
-// list (query-grouped, already in output order) and counts them; any other
-// leaf, or one whose hits no longer fit the list, is counted with the COUNT
-// body.  The tile's dense output offset then comes from a decoupled
-// look-back over the tile states of earlier tickets.  Phase B writes the
-// recorded hits with contiguous stores, then re-sweeps only the leaves that
-// were not recorded (the exact recount, with the FILL body, whose LDS stage
-// lies over the hit list once it is written out).  No count kernel, no scan
-// and no host round trip precede the emit.  A tile whose range would pass
-// cap_edges writes nothing and raises flags[0]; the host then reruns with
-// room for the total the last tile reports in flags[2..3].  Tickets are
-// taken in order by running blocks, so every tile a look-back waits on is
-// held by a block that is already running and never waits on a later tile:
-// the wait always ends (and is bounded anyway).
+constexpr int SP_MAX_LEAVES = SP_PLAN_MAX_LEAVES;  // leaves one wave may own under a plan
+
+// Single traversal in leaf order.  Waves own runs of consecutive query
+// leaves: SP_LEAVES_PER_WAVE each, or the runs of a plan (wave_first[w] ..
+// wave_first[w + 1], at most SP_MAX_LEAVES, *plan_waves runs) that packs the
+// predicted hits of a run into one hit list.  A tile is EW_WAVES consecutive
+// runs; blocks take tiles by ticket until none is left.  Phase A sweeps each
+// leaf once: a leaf that fits one register tile records its hits in the
+// wave's LDS hit list (query-grouped, already in output order) and counts
+// them; any other leaf, or one whose hits no longer fit the list, is counted
+// with the COUNT body and recounted in phase B (stats[0] counts those, stats[1]
+// the recorded ones).  lhits (optional) receives every leaf's exact hit count.
+// The tile's dense output offset comes from a decoupled look-back over the
+// tile states of earlier tickets.  Phase B writes the recorded hits with
+// contiguous stores, then re-sweeps the other leaves (the exact recount, with
+// the FILL body, whose LDS stage lies over the hit list once it is written
+// out).  No count kernel, no scan and no host round trip precede the emit.
+// A tile whose range would pass cap_edges writes nothing and raises flags[0];
+// the host then reruns with room for the total the last tile reports in
+// flags[2..3].  flags[1] reports a look-back that gave up or a run longer than
+// SP_MAX_LEAVES (the host then takes the exact path).  Tickets are taken in
+// order, and a block finishes its tile before taking another, so every tile a
+// look-back waits on is held by a running block that never waits on a later
+// tile: the wait always ends (and is bounded anyway).
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
 k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
                const BuildParams* __restrict__ qprm, const float4* __restrict__ tsc, const int2* __restrict__ lists,
                const int32_t* __restrict__ lcount, int cap, float r_sq, float inv_r, int self_pair,
-               uint32_t* __restrict__ qcount, uint32_t* __restrict__ qoff, uint64_t* __restrict__ tile_state,
-               uint32_t* __restrict__ ticket, uint32_t* __restrict__ flags, uint64_t cap_edges, EdgeOut out,
-               int index_base) {
+               uint32_t* __restrict__ qcount, uint32_t* __restrict__ qoff, const int32_t* __restrict__ wave_first,
+               const int32_t* __restrict__ plan_waves, uint32_t* __restrict__ lhits,
+               uint64_t* __restrict__ tile_state, uint32_t* __restrict__ ticket, uint32_t* __restrict__ flags,
+               uint32_t* __restrict__ stats, uint64_t cap_edges, EdgeOut out, int index_base) {
     __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
     __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
     __shared__ int2 s_hits[EW_WAVES][SP_HITS];
+    __shared__ uint32_t s_cnt[EW_WAVES][SP_MAX_LEAVES];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wtot[EW_WAVES];
     __shared__ uint64_t s_prefix;
@@ -484,116 +523,156 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
     const int wv = threadIdx.x / OCT_WAVE;
     const int lane = lane_id();
     int2* hits = s_hits[wv];
+    uint32_t* cntw = s_cnt[wv];
     int32_t* stage = reinterpret_cast<int32_t*>(hits);
     const WaveLds L{s_start[wv], s_first[wv], stage, stage + EW_STAGE, reinterpret_cast<float*>(stage + 2 * EW_STAGE),
                     reinterpret_cast<float*>(stage + 5 * EW_STAGE)};
     const int32_t nleaves = qprm->n_leaves;
-    const int32_t ntiles = (nleaves + SP_TILE_LEAVES - 1) / SP_TILE_LEAVES;
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const int32_t t = (int32_t)s_tile;
-    if (t >= ntiles) return;
-    const int32_t r_lo = t * SP_TILE_LEAVES + wv * SP_LEAVES_PER_WAVE;
-    const int32_t r_hi = min(nleaves, r_lo + SP_LEAVES_PER_WAVE);
-    // phase A: one sweep per leaf, exact counts
-    uint32_t cnt[SP_LEAVES_PER_WAVE];
-    uint32_t recorded = 0;  // bit k: leaf r_lo + k sits in the hit list
-    uint32_t hfill = 0;
-    uint32_t wtot = 0;
-#pragma unroll
-    for (int k = 0; k < SP_LEAVES_PER_WAVE; ++k) {
-        cnt[k] = 0;
-        const int32_t r = r_lo + k;
-        if (r >= r_hi) continue;
-        const NodeRec qa = qleaves[r];
-        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
-        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
-        const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
-        const int2* lst = lists + (int64_t)r * cap;
-        const int2 lf0 = lst[lane < ncand ? lane : 0];
-        const int total = __builtin_amdgcn_readfirstlane(wave_sum(lane < ncand ? lf0.y : 0));
-        uint32_t c;
-        if (one_tile_leaf(qc, ncand, total)) {
-            c = leaf_sweep_record(qf, qc, ncand, lf0, qsc, tsc, r_sq, self_pair, L, lane, hits + hfill,
-                                  (uint32_t)SP_HITS - hfill);
-            if (c <= (uint32_t)SP_HITS - hfill) {
-                recorded |= 1u << k;
-                hfill += c;
+    const int32_t nwaves = wave_first ? *plan_waves : (nleaves + SP_LEAVES_PER_WAVE - 1) / SP_LEAVES_PER_WAVE;
+    const int32_t ntiles = (nwaves + EW_WAVES - 1) / EW_WAVES;
+    uint32_t n_recount = 0, n_record = 0;  // this wave's, over all its tiles
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const int32_t t = (int32_t)s_tile;
+        if (t >= ntiles) break;
+        const int32_t w = t * EW_WAVES + wv;
+        int32_t r_lo = 0, r_hi = 0;
+        bool bad = false;
+        if (w < nwaves) {
+            if (wave_first) {
+                r_lo = wave_first[w];
+                r_hi = wave_first[w + 1];
             } else {
-                // the list is full: count per query for the recount in phase B
-                leaf_pass<EDGE_COUNT>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount,
-                                      nullptr, out, index_base, L, lane);
+                r_lo = w * SP_LEAVES_PER_WAVE;
+                r_hi = min(nleaves, r_lo + SP_LEAVES_PER_WAVE);
             }
-        } else {
-            c = leaf_pass<EDGE_COUNT>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount, nullptr,
-                                      out, index_base, L, lane);
         }
-        cnt[k] = c;
-        wtot += c;
-    }
-    if (lane == 0) s_wtot[wv] = wtot;
-    __syncthreads();
-    if (wv == 0) {
-        uint64_t agg = 0;
-        for (int k = 0; k < EW_WAVES; ++k) agg += s_wtot[k];
-        if (lane == 0)
-            __hip_atomic_store(&tile_state[t], SP_FLAG_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t ok = 1;
-        const uint64_t excl = wave_look_back(tile_state, t, lane, &ok);
-        if (lane == 0) {
-            __hip_atomic_store(&tile_state[t], SP_FLAG_PRE | (excl + agg), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            if (!ok) atomicOr(&flags[1], 1u);
-            if (excl + agg > cap_edges) {
-                ok = 0;
-                atomicOr(&flags[0], 1u);
-            }
-            if (t == ntiles - 1)
-                __hip_atomic_store((uint64_t*)(flags + 2), excl + agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_prefix = excl;
-            s_ok = ok;
+        r_lo = __builtin_amdgcn_readfirstlane(r_lo);
+        r_hi = __builtin_amdgcn_readfirstlane(r_hi);
+        if (r_lo < 0 || r_hi > nleaves || r_hi - r_lo > SP_MAX_LEAVES) {
+            bad = true;  // a malformed plan: nothing is swept, the host takes the exact path
+            r_hi = r_lo;
         }
-    }
-    __syncthreads();
-    if (!s_ok) return;
-    // phase B: recorded leaves straight from the hit list ...
-    uint64_t base = s_prefix;
-    for (int k = 0; k < wv; ++k) base += s_wtot[k];
-    uint32_t pos = (uint32_t)base;
-    uint32_t hread = 0;
-#pragma unroll
-    for (int k = 0; k < SP_LEAVES_PER_WAVE; ++k) {
-        if (recorded & (1u << k)) {
-            emit_hits(hits + hread, cnt[k], pos, qsc, tsc, inv_r, out, index_base, lane);
-            hread += cnt[k];
-        }
-        pos += cnt[k];
-    }
-    __builtin_amdgcn_wave_barrier();
-    // ... then the others re-swept; their stage lies over the hit list
-    pos = (uint32_t)base;
-#pragma unroll
-    for (int k = 0; k < SP_LEAVES_PER_WAVE; ++k) {
-        const int32_t r = r_lo + k;
-        if (r < r_hi && !(recorded & (1u << k))) {
+        // phase A: one sweep per leaf, exact counts
+        uint64_t recorded = 0;  // bit k: leaf r_lo + k sits in the hit list
+        uint32_t hfill = 0, wtot = 0, nrec = 0;
+        for (int32_t r = r_lo; r < r_hi; ++r) {
+            const int k = r - r_lo;
             const NodeRec qa = qleaves[r];
             const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
             const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
-            uint32_t run = pos;
-            for (int qb = 0; qb < qc; qb += OCT_WAVE) {
-                const int nqb = min(OCT_WAVE, qc - qb);
-                const uint32_t c = lane < nqb ? qcount[qf + qb + lane] : 0u;  // written by this lane in phase A
-                const uint32_t incl = (uint32_t)wave_incl_scan((int)c);
-                if (lane < nqb) qoff[qf + qb + lane] = run + incl - c;
-                run += (uint32_t)__builtin_amdgcn_readlane((int)incl, OCT_WAVE - 1);
-            }
             const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
             const int2* lst = lists + (int64_t)r * cap;
             const int2 lf0 = lst[lane < ncand ? lane : 0];
-            leaf_pass<EDGE_FILL>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount, qoff, out,
-                                 index_base, L, lane);
+            const int total = __builtin_amdgcn_readfirstlane(wave_sum(lane < ncand ? lf0.y : 0));
+            uint32_t c;
+            if (one_tile_leaf(qc, ncand, total)) {
+                c = leaf_sweep_record(qf, qc, ncand, lf0, qsc, tsc, r_sq, self_pair, L, lane, hits + hfill,
+                                      (uint32_t)SP_HITS - hfill);
+                if (c <= (uint32_t)SP_HITS - hfill) {
+                    recorded |= 1ull << k;
+                    hfill += c;
+                    ++nrec;
+                } else {
+                    // the list is full: count per query for the recount in phase B
+                    leaf_pass<EDGE_COUNT>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount,
+                                          nullptr, out, index_base, L, lane);
+                }
+            } else {
+                c = leaf_pass<EDGE_COUNT>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount,
+                                          nullptr, out, index_base, L, lane);
+            }
+            if (lane == 0) {
+                cntw[k] = c;
+                if (lhits) lhits[r] = c;
+            }
+            wtot += c;
         }
-        pos += cnt[k];
+        if (lane == 0) {
+            s_wtot[wv] = wtot;
+            if (bad) atomicOr(&flags[1], 2u);
+        }
+        n_recount += (uint32_t)(r_hi - r_lo) - nrec;
+        n_record += nrec;
+        __syncthreads();
+        if (wv == 0) {
+            uint64_t agg = 0;
+            for (int k = 0; k < EW_WAVES; ++k) agg += s_wtot[k];
+            if (lane == 0)
+                __hip_atomic_store(&tile_state[t], SP_FLAG_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t ok = 1;
+            const uint64_t excl = wave_look_back(tile_state, t, lane, &ok);
+            if (lane == 0) {
+                __hip_atomic_store(&tile_state[t], SP_FLAG_PRE | (excl + agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (!ok) atomicOr(&flags[1], 1u);
+                if (excl + agg > cap_edges) {
+                    ok = 0;
+                    atomicOr(&flags[0], 1u);
+                }
+                if (t == ntiles - 1)
+                    __hip_atomic_store((uint64_t*)(flags + 2), excl + agg, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                s_prefix = excl;
+                s_ok = ok;
+            }
+        }
+        __syncthreads();
+        if (s_ok) {
+            // phase B: recorded leaves straight from the hit list ...
+            uint64_t base = s_prefix;
+            for (int k = 0; k < wv; ++k) base += s_wtot[k];
+            uint32_t pos = (uint32_t)base;
+            uint32_t hread = 0;
+            for (int32_t r = r_lo; r < r_hi; ++r) {
+                const int k = r - r_lo;
+                const uint32_t c = cntw[k];
+                if (recorded & (1ull << k)) {
+                    emit_hits(hits + hread, c, pos, qsc, tsc, inv_r, out, index_base, lane);
+                    hread += c;
+                }
+                pos += c;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // ... then the others re-swept; their stage lies over the hit list
+            pos = (uint32_t)base;
+            for (int32_t r = r_lo; r < r_hi; ++r) {
+                const int k = r - r_lo;
+                if (!(recorded & (1ull << k))) {
+                    const NodeRec qa = qleaves[r];
+                    const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+                    const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+                    uint32_t run = pos;
+                    for (int qb = 0; qb < qc; qb += OCT_WAVE) {
+                        const int nqb = min(OCT_WAVE, qc - qb);
+                        const uint32_t c = lane < nqb ? qcount[qf + qb + lane] : 0u;  // this lane's, phase A
+                        const uint32_t incl = (uint32_t)wave_incl_scan((int)c);
+                        if (lane < nqb) qoff[qf + qb + lane] = run + incl - c;
+                        run += (uint32_t)__builtin_amdgcn_readlane((int)incl, OCT_WAVE - 1);
+                    }
+                    const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+                    const int2* lst = lists + (int64_t)r * cap;
+                    const int2 lf0 = lst[lane < ncand ? lane : 0];
+                    leaf_pass<EDGE_FILL>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount, qoff,
+                                         out, index_base, L, lane);
+                }
+                pos += cntw[k];
+            }
+        }
+        __syncthreads();  // the next ticket reuses the block's LDS
+    }
+    // one pair of global atomics per block
+    if (threadIdx.x == 0) s_wtot[0] = s_wtot[1] = 0;
+    __syncthreads();
+    if (lane == 0) {
+        atomicAdd(&s_wtot[0], n_recount);
+        atomicAdd(&s_wtot[1], n_record);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && (s_wtot[0] | s_wtot[1])) {
+        atomicAdd(&stats[0], s_wtot[0]);
+        atomicAdd(&stats[1], s_wtot[1]);
     }
 }
 
@@ -605,22 +684,25 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
 // ------------------------------------------------------------- launchers
 
 hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
-                             int32_t* lcount, uint32_t* flags, hipStream_t s) {
+                             uint8_t* ldir, int32_t* lcount, uint32_t* flags, hipStream_t s) {
     const int64_t want = ((int64_t)q.n_leaves + LL_THREADS - 1) / LL_THREADS;
     const unsigned g = (unsigned)(want < 1 ? 1 : (want > 65535 ? 65535 : want));
-    if (safe || t.n_nodes >= (1 << 29))
-        hipLaunchKernelGGL(k_leaf_lists<false>, dim3(g), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes,
-                           t.params, r_sq, cap, lists, lcount, flags);
-    else
-        hipLaunchKernelGGL(k_leaf_lists<true>, dim3(g), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes,
-                           t.params, r_sq, cap, lists, lcount, flags);
+#define OCT_LL_LAUNCH(F, D)                                                                                 \
+    hipLaunchKernelGGL((k_leaf_lists<F, D>), dim3(g), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes, t.params, \
+                       r_sq, cap, lists, ldir, lcount, flags)
+    const bool fast = !safe && t.n_nodes < (1 << 29);
+    if (fast && ldir) OCT_LL_LAUNCH(true, true);
+    else if (fast) OCT_LL_LAUNCH(true, false);
+    else if (ldir) OCT_LL_LAUNCH(false, true);
+    else OCT_LL_LAUNCH(false, false);
+#undef OCT_LL_LAUNCH
     return hipGetLastError();
 }
 
 hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const int2* lists,
                         const int32_t* lcount, int cap, float r_sq, float inv_r, int self_pair,
-                        int by_receiver, uint32_t* qcount, const uint32_t* qoff, int64_t n_edges, EdgeOut out,
-                        int index_base, hipStream_t s) {
+                        int by_receiver, uint32_t* qcount, const uint32_t* qoff, uint32_t* lhits, int64_t n_edges,
+                        EdgeOut out, int index_base, hipStream_t s) {
     (void)n_edges;
     // about four leaves per wave, XCD-local contiguous ranges
     const int64_t want = ((int64_t)q.n_leaves + 4 * EW_WAVES - 1) / (4 * EW_WAVES);
@@ -628,26 +710,30 @@ hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const in
     if (mode == EDGE_COUNT)
         hipLaunchKernelGGL(k_edges<EDGE_COUNT>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, by_receiver, qcount, qoff,
-                           out, index_base);
+                           lhits, out, index_base);
     else
         hipLaunchKernelGGL(k_edges<EDGE_FILL>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, by_receiver, qcount, qoff,
-                           out, index_base);
+                           lhits, out, index_base);
     return hipGetLastError();
 }
 
-int64_t single_tiles(int64_t n_leaves) { return (n_leaves + SP_TILE_LEAVES - 1) / SP_TILE_LEAVES; }
+// Upper bound of the tiles of any plan (every run holds at least one leaf).
+int64_t single_tiles(int64_t n_leaves) { return (n_leaves + EW_WAVES - 1) / EW_WAVES; }
 
 // tile_state (single_tiles words of 8 bytes) and ticket must be zero.
 hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
                                int cap, float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
-                               uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint64_t cap_edges,
-                               EdgeOut out, int index_base, hipStream_t s) {
-    const int64_t tiles = single_tiles(q.n_leaves);
+                               const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
+                               uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint32_t* stats,
+                               uint64_t cap_edges, EdgeOut out, int index_base, hipStream_t s) {
+    const int64_t tiles = wave_first ? single_tiles(q.n_leaves)
+                                     : (q.n_leaves + SP_TILE_LEAVES - 1) / SP_TILE_LEAVES;
     if (tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_edges_single, dim3((unsigned)tiles), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
-                       q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, tile_state, ticket,
-                       flags, cap_edges, out, index_base);
+    const unsigned g = (unsigned)std::min<int64_t>(tiles, 2048);  // blocks loop over tickets
+    hipLaunchKernelGGL(k_edges_single, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
+                       t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first, plan_waves,
+                       lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base);
     return hipGetLastError();
 }
 

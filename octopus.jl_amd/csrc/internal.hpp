@@ -4,6 +4,7 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "octohip.h"
 
 namespace oct {
 
@@ -43,6 +44,7 @@ struct TreeView {
     const NodeRec* leaves;     // leaf records in Morton order (a = first position, b = count)
     const float4* scoord;      // Morton-sorted coords, w = original index (int bits)
     const BuildParams* params;
+    const uint8_t* nlevel;     // tree level per node
     int32_t n_leaves;
     int32_t n_nodes;
     int32_t branch_root;
@@ -89,20 +91,67 @@ struct EdgeOut {
 enum { EDGE_COUNT = 0, EDGE_FILL = 1 };
 // flags[0]: largest candidate count of an overflowing query leaf; flags[1]:
 // a walk ran out of stack (rerun with safe = 1)
+// ldir (optional): one direction code per list entry, for the predictor.
 hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
-                             int32_t* lcount, uint32_t* flags, hipStream_t s);
+                             uint8_t* ldir, int32_t* lcount, uint32_t* flags, hipStream_t s);
 hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const int2* lists,
                         const int32_t* lcount, int cap, float r_sq, float inv_r, int self_pair,
-                        int by_receiver, uint32_t* qcount, const uint32_t* qoff, int64_t n_edges, EdgeOut out,
-                        int index_base, hipStream_t s);
+                        int by_receiver, uint32_t* qcount, const uint32_t* qoff, uint32_t* lhits, int64_t n_edges,
+                        EdgeOut out, int index_base, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 int64_t single_tiles(int64_t n_leaves);
-// Single pass: flags[0] capacity exceeded, flags[1] look-back gave up,
-// flags[2..3] 64-bit edge total (written by the last tile).
+// Single pass: flags[0] capacity exceeded, flags[1] look-back gave up (1) or
+// malformed plan (2), flags[2..3] 64-bit edge total (written by the last
+// tile); stats[0] recounted leaves, stats[1] recorded leaves.  wave_first /
+// plan_waves: optional plan of leaf runs per wave (null: fixed runs).
 hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
                                int cap, float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
-                               uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint64_t cap_edges,
-                               EdgeOut out, int index_base, hipStream_t s);
+                               const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
+                               uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint32_t* stats,
+                               uint64_t cap_edges, EdgeOut out, int index_base, hipStream_t s);
+
+// Per-leaf count predictor (see predictor.hip).  Parameter block layout in
+// floats: feature mean and std, then W1 (29 x 64, row = input), b1, W2
+// (64 x 64, row = input), b2, W3 (64), b3.
+constexpr int MLP_IN = 29;
+constexpr int MLP_ADJ = 26;
+constexpr int MLP_H = 64;
+constexpr int MLP_MU = 0;
+constexpr int MLP_SD = MLP_MU + MLP_IN;
+constexpr int MLP_W1 = MLP_SD + MLP_IN;
+constexpr int MLP_B1 = MLP_W1 + MLP_IN * MLP_H;
+constexpr int MLP_W2 = MLP_B1 + MLP_H;
+constexpr int MLP_B2 = MLP_W2 + MLP_H * MLP_H;
+constexpr int MLP_W3 = MLP_B2 + MLP_H;
+constexpr int MLP_B3 = MLP_W3 + MLP_H;
+constexpr int MLP_PARAMS = MLP_B3 + 1;
+constexpr int MLP_TRAINED = MLP_PARAMS - MLP_W1;  // everything but the feature statistics
+constexpr int MLP_ROWS_PER_CHUNK = 256;           // rows per partial gradient sum
+static_assert(MLP_PARAMS == OCT_MLP_PARAMS, "parameter block size is part of the C-ABI");
+
+hipError_t launch_leaf_features(const TreeView& q, const int2* lists, const uint8_t* ldir, const int32_t* lcount,
+                                int cap, float r, float* feat, hipStream_t s);
+hipError_t launch_mlp_infer(const float* X, int64_t n, const float* P, float* y, float* hits, hipStream_t s);
+int64_t mlp_scratch_floats(int batch);
+// One training step on `batch` rows (seed 0: rows 0..batch-1, else hashed
+// rows); lr 0 computes the gradient (grad_out) without updating.
+hipError_t launch_mlp_step(const float* X, const float* T, int64_t n_data, int batch, uint64_t seed, float* P,
+                           float* m, float* v, int step, float lr, float* scratch, float* grad_out, float* loss_out,
+                           hipStream_t s);
+hipError_t launch_feature_stats(const float* X, int64_t n, float* P, hipStream_t s);
+hipError_t launch_log1p_u32(const uint32_t* in, int64_t n, float* out, hipStream_t s);
+// Predictions and emission plan from per-leaf features: pred = predicted
+// hits; runs of consecutive leaves packed greedily within chunks of 64 leaves
+// so that a run's predicted hits fit `budget` hit-list entries and it holds at
+// most max_leaves leaves; wave_first[n_runs + 1]; pred_total = sum of the
+// rounded-up predictions.  work: 4 * plan_chunks(n) words; scan_tmp:
+// scan_tmp_words(plan_chunks(n)) words.
+int64_t plan_chunks(int64_t n_leaves);
+hipError_t launch_predict_plan(const float* feat, int64_t n, const float* P, float* pred, uint32_t budget,
+                               int max_leaves, uint32_t* work, uint32_t* scan_tmp, int32_t* wave_first,
+                               int32_t* n_runs, uint64_t* pred_total, hipStream_t s);
+constexpr uint32_t SP_PLAN_BUDGET = 1024;  // hit-list entries of one wave (edges.hip)
+constexpr int SP_PLAN_MAX_LEAVES = 64;     // leaves one wave may own (edges.hip)
 
 // Adjoint (see backward.hip).
 hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* senders,

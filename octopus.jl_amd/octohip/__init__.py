@@ -439,3 +439,6 @@ def build_edges_diff(coords, tns: TNS, set_id: int, radius) -> Edges:
     if _BuildEdgesDiff is None:
         _BuildEdgesDiff = _make_fn()
     return Edges(*_BuildEdgesDiff.apply(coords, tns, set_id, radius))
+
+
+from . import predictor  # noqa: E402  (learned per-leaf count predictor)

@@ -74,6 +74,28 @@ class OctView(ctypes.Structure):
     ]
 
 
+class OctEdgeStats(ctypes.Structure):
+    _fields_ = [
+        ("features", ctypes.c_void_p),
+        ("predicted", ctypes.c_void_p),
+        ("exact", ctypes.c_void_p),
+        ("n_leaves", ctypes.c_int64),
+        ("n_edges", ctypes.c_int64),
+        ("recounted_leaves", ctypes.c_int64),
+        ("recorded_leaves", ctypes.c_int64),
+        ("predicted_total", ctypes.c_double),
+        ("single_pass", ctypes.c_int32),
+        ("used_predictor", ctypes.c_int32),
+        ("attempts", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
+MLP_FEATURES = 29
+MLP_HIDDEN = 64
+MLP_PARAMS = 6203
+MLP_TRAINED = MLP_PARAMS - 2 * MLP_FEATURES
+
 _P = ctypes.c_void_p
 # name -> (restype, argtypes); every function declared in include/octohip.h
 SIGNATURES = {
@@ -100,9 +122,19 @@ SIGNATURES = {
     "oct_profile_read": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int64)]),
     "oct_synchronize": (ctypes.c_int, [_P]),
+    "oct_set_predictor_policy": (ctypes.c_int, [_P, ctypes.c_int]),
+    "oct_predictor_load": (ctypes.c_int, [_P, _P]),
+    "oct_predictor_save": (ctypes.c_int, [_P, _P]),
+    "oct_predictor_collect": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
+    "oct_predictor_clear_data": (ctypes.c_int, [_P]),
+    "oct_predictor_train": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
+                                           ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+    "oct_edge_stats_get": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OctEdgeStats)]),
+    "oct_mlp_forward": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
+    "oct_mlp_gradient": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int, _P, _P]),
 }
 
-PROF_PHASES = ["bounds", "build", "leaf_lists", "count", "scan", "fill", "backward", "single"]
+PROF_PHASES = ["bounds", "build", "leaf_lists", "count", "scan", "fill", "backward", "single", "predict"]
 
 _lib = None
 
