@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--null-stream", action="store_true", help="run the library on torch's current stream")
     ap.add_argument("--predictor", default="sph_lattice",
                     help="shipped predictor for the learned mode ('none': plan-free single pass)")
+    ap.add_argument("--no-refit", action="store_true",
+                    help="full sort every step (default: refit mode, the almost-sorted path of time stepping)")
     ap.add_argument("--predict-always", action="store_true",
                     help="run the predictor on every step (default: first call and after overflows / recounts)")
     return ap.parse_args()
@@ -69,6 +71,7 @@ def main():
 
     tns = octohip.TNS(private_stream=not a.null_stream)
     octohip.set_search_radius(tns, float(r))
+    octohip.set_refit_mode(tns, not a.no_refit)
     sid = octohip.add_point_set(tns, snaps[0])
     octohip.set_active_search(tns, sid, sid)
     predictor = None
@@ -81,10 +84,12 @@ def main():
     if predictor and a.predict_always:
         octohip.predictor.set_policy(tns, True)
     used = [0]
+    refits = [0]
 
     def step(k):
         octohip.update_point_set(tns, sid, snaps[k])
         octohip.run(tns)
+        refits[0] += int(octohip.build_info(tns, sid)["refit_used"])
         e = octohip.build_edges(tns, sid, sid, mode=a.mode)
         if predictor:
             used[0] += int(octohip.predictor.edge_stats(tns, sid, sid)["used_predictor"])
@@ -93,6 +98,7 @@ def main():
     for k in range(a.warmup):
         step(k)
     used[0] = 0
+    refits[0] = 0
     torch.cuda.synchronize()
     if not a.no_profile:
         octohip.profile(tns, True)
@@ -155,7 +161,8 @@ def main():
                                f"radius 2h, N(0,(0.01h)^2) step displacement; update + run! + build_edges per step",
                    "mode": a.mode, "points": n, "edges_per_step": e_avg, "radius": float(r),
                    "predictor": predictor, "predict_always": bool(a.predict_always),
-                   "timed_steps_using_predictor": used[0]},
+                   "timed_steps_using_predictor": used[0], "refit_mode": not a.no_refit,
+                   "timed_steps_refit": refits[0]},
         "last_step": {k: st[k] for k in ("single_pass", "used_predictor", "attempts", "recounted_leaves",
                                          "recorded_leaves", "predicted_total", "n_edges", "n_leaves")},
         "breakdown_ms_per_step": {"run": run_ms, "build_edges": edges_ms,

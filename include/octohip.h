@@ -122,6 +122,19 @@ int oct_run(oct_ctx* ctx);
 /* The built tree of a set (tns.trees[id], tns.permutation[id]). */
 int oct_get_tree(oct_ctx* ctx, int set_id, oct_tree* out);
 
+/* How the last oct_run built a set.  In refit mode a set rebuilt with the
+ * same point count first tries the almost-sorted path (no counterpart in the
+ * reference, which only stores the flag, src/api.jl:43-47,234-238): the
+ * previous sorted order is kept and only the points whose key changed are
+ * placed, which gives exactly the full sort's order; with more than
+ * min(32768, n / 8) changed points the full sort runs instead. */
+typedef struct oct_build_info {
+    int64_t changed_points;    /* points whose key changed (-1: no previous sort tried) */
+    int32_t refit_used;        /* 1: the almost-sorted path built the order */
+    int32_t pad;
+} oct_build_info;
+int oct_build_info_get(oct_ctx* ctx, int set_id, oct_build_info* out);
+
 /* build_edges / build_edges! (src/api.jl:304-337,
  * ext/OctopusCUDAExt.jl:345-415): flat-COO edges of the active pair (q, t);
  * the self pair (q == t) excludes j == i.  mode: OCT_EDGES_*; order:

@@ -241,6 +241,130 @@ k_rs_scatter(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
     }
 }
 
+// ================================================== almost-sorted refit
+//
+// Time stepping moves few points across a cell boundary, so the previous
+// sorted order (keys PK, point indices PI, sorted by (key, index)) is almost
+// the new one.  Points whose key did not change keep their relative order;
+// only the changed points C are placed: each one's rank within C (by key,
+// then index) plus the number of unchanged points before it, found by binary
+// search in the previous order.  The result is exactly the stable sort's
+// order, so the tree is the one the full sort would build.
+
+constexpr int RF_THREADS = 256;
+constexpr int RF_TILE = 1024;  // C entries staged in LDS per round of the rank kernel
+
+__device__ inline bool key_idx_less(uint64_t ka, int32_t ia, uint64_t kb, int32_t ib) {
+    return ka < kb || (ka == kb && ia < ib);
+}
+
+// K1: new keys in the previous order; vals0 = PI, vals1 = changed flag.
+__global__ void __launch_bounds__(RF_THREADS)
+k_refit_keys(const float* __restrict__ coords, int64_t n, const BuildParams* __restrict__ prm,
+             const uint64_t* __restrict__ pk, const float4* __restrict__ prev_sc, uint64_t* __restrict__ nk,
+             int32_t* __restrict__ pi, uint32_t* __restrict__ chg) {
+    const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const int32_t idx = __float_as_int(prev_sc[i].w);
+    const float ox = prm->origin[0], oy = prm->origin[1], oz = prm->origin[2];
+    const float inv = prm->inv_cs;
+    const float px = coords[3 * (int64_t)idx], py = coords[3 * (int64_t)idx + 1], pz = coords[3 * (int64_t)idx + 2];
+    const uint64_t k = morton3(floor_clamp((px - ox) * inv, 0x1fffffu), floor_clamp((py - oy) * inv, 0x1fffffu),
+                               floor_clamp((pz - oz) * inv, 0x1fffffu));
+    nk[i] = k;
+    pi[i] = idx;
+    chg[i] = k != pk[i] ? 1u : 0u;
+}
+
+// K3: compact the changed points (their new key and index).
+__global__ void __launch_bounds__(RF_THREADS)
+k_refit_compact(int64_t n, const uint32_t* __restrict__ cpos, const uint64_t* __restrict__ total,
+                const uint64_t* __restrict__ nk, const int32_t* __restrict__ pi, uint64_t* __restrict__ ck,
+                int32_t* __restrict__ ci) {
+    const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = cpos[i];
+    const uint32_t next = i + 1 < n ? cpos[i + 1] : (uint32_t)*total;
+    if (next != c) {
+        ck[c] = nk[i];
+        ci[c] = pi[i];
+    }
+}
+
+// K4: rank of each changed point within C (all-pairs count, C staged in LDS
+// tiles) and its number of unchanged predecessors (binary search of the
+// previous order); writes C sorted (csk, csi) with final positions cspos.
+__global__ void __launch_bounds__(RF_THREADS)
+k_refit_rank(int64_t n, const uint64_t* __restrict__ total, const uint64_t* __restrict__ ck,
+             const int32_t* __restrict__ ci, const uint64_t* __restrict__ pk, const int32_t* __restrict__ pi,
+             const uint32_t* __restrict__ cpos, uint64_t* __restrict__ csk, int32_t* __restrict__ csi,
+             uint32_t* __restrict__ cspos) {
+    __shared__ uint64_t tk[RF_TILE];
+    __shared__ int32_t ti[RF_TILE];
+    const int64_t nc = (int64_t)*total;
+    const int64_t c = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
+    const bool mine = c < nc;
+    const uint64_t key = mine ? ck[c] : 0ull;
+    const int32_t idx = mine ? ci[c] : 0;
+    uint32_t rank = 0;
+    for (int64_t t0 = 0; t0 < nc; t0 += RF_TILE) {
+        const int tn = (int)(nc - t0 < RF_TILE ? nc - t0 : RF_TILE);
+        __syncthreads();
+        for (int j = threadIdx.x; j < tn; j += RF_THREADS) {
+            tk[j] = ck[t0 + j];
+            ti[j] = ci[t0 + j];
+        }
+        __syncthreads();
+        if (mine)
+            for (int j = 0; j < tn; ++j) rank += key_idx_less(tk[j], ti[j], key, idx) ? 1u : 0u;
+    }
+    if (!mine) return;
+    int64_t lo = 0, hi = n;  // first position of the previous order not below (key, idx)
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (key_idx_less(pk[mid], pi[mid], key, idx)) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint32_t before_changed = lo < n ? cpos[lo] : (uint32_t)nc;
+    const uint32_t unchanged_before = (uint32_t)lo - before_changed;
+    csk[rank] = key;
+    csi[rank] = idx;
+    cspos[rank] = unchanged_before + rank;
+}
+
+__device__ inline void refit_write(int64_t pos, uint64_t key, int32_t idx, const float* __restrict__ coords,
+                                   uint64_t* __restrict__ kout, int32_t* __restrict__ perm, float4* __restrict__ sc,
+                                   int index_base) {
+    kout[pos] = key;
+    perm[pos] = idx + index_base;
+    const float* p = coords + 3 * (int64_t)idx;
+    sc[pos] = make_float4(p[0], p[1], p[2], __int_as_float(idx));
+}
+
+// K5: unchanged points to their new positions; K6 (c < |C|): changed points.
+__global__ void __launch_bounds__(RF_THREADS)
+k_refit_place(const float* __restrict__ coords, int64_t n, const uint64_t* __restrict__ total,
+              const uint64_t* __restrict__ pk, const int32_t* __restrict__ pi, const uint32_t* __restrict__ cpos,
+              const uint64_t* __restrict__ csk, const int32_t* __restrict__ csi, const uint32_t* __restrict__ cspos,
+              uint64_t* __restrict__ kout, int32_t* __restrict__ perm, float4* __restrict__ sc, int index_base) {
+    const int64_t nc = (int64_t)*total;
+    const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
+    if (i < nc) refit_write(cspos[i], csk[i], csi[i], coords, kout, perm, sc, index_base);
+    if (i >= n) return;
+    const uint32_t c = cpos[i];
+    const uint32_t next = i + 1 < n ? cpos[i + 1] : (uint32_t)nc;
+    if (next != c) return;  // changed: placed above
+    const uint64_t key = pk[i];
+    const int32_t idx = pi[i];
+    int64_t lo = 0, hi = nc;  // changed points below (key, idx)
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (key_idx_less(csk[mid], csi[mid], key, idx)) lo = mid + 1;
+        else hi = mid;
+    }
+    refit_write((int64_t)(i - c) + lo, key, idx, coords, kout, perm, sc, index_base);
+}
+
 // ============================================================ topology
 
 constexpr int TOPO_THREADS = 256;
@@ -523,9 +647,10 @@ hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, con
     return hipGetLastError();
 }
 
-// Stage 2: keys, sort over the live bits, topology, refit.
-hipError_t launch_build_stage(const float* coords, int64_t n, int key_bits, int T, int index_base,
-                              const BuildBufs& b, hipStream_t s) {
+// Stage 2a: keys and the full stable sort over the live bits; *cur = the
+// keys buffer that ends up holding the sorted keys.
+hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int index_base, const BuildBufs& b,
+                             int* cur_out, hipStream_t s) {
     const unsigned g = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_keys, dim3(g), dim3(256), 0, s, coords, n, b.prm, b.keys[0], b.vals[0]);
     int passes = (key_bits + RS_BITS - 1) / RS_BITS;
@@ -542,6 +667,41 @@ hipError_t launch_build_stage(const float* coords, int64_t n, int key_bits, int 
                            b.perm, b.scoord, coords, index_base);
         cur ^= 1;
     }
+    *cur_out = cur;
+    return hipGetLastError();
+}
+
+// Stage 2a' (almost-sorted path), part 1: new keys in the previous order
+// (keys[cur] and scoord hold the previous sort) and the changed count in
+// *b.total.  The caller reads the count and either finishes the refit or
+// runs the full sort.
+hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, const BuildBufs& b, hipStream_t s) {
+    const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
+    hipLaunchKernelGGL(k_refit_keys, dim3(g), dim3(RF_THREADS), 0, s, coords, n, b.prm, b.keys[cur], b.scoord,
+                       b.keys[cur ^ 1], b.vals[0], (uint32_t*)b.vals[1]);
+    return exclusive_scan((const uint32_t*)b.vals[1], (uint32_t*)b.vals[1], n, b.scan_tmp, b.total, s);
+}
+
+// Part 2, for n_changed <= REFIT_MAX_CHANGED: the sorted keys end in keys[cur ^ 1].
+hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t n_changed, int cur, int index_base,
+                               const BuildBufs& b, const RefitBufs& r, hipStream_t s) {
+    const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
+    const uint32_t* cpos = (const uint32_t*)b.vals[1];
+    hipLaunchKernelGGL(k_refit_compact, dim3(g), dim3(RF_THREADS), 0, s, n, cpos, b.total, b.keys[cur ^ 1], b.vals[0],
+                       r.ck, r.ci);
+    if (n_changed > 0) {
+        const unsigned gc = (unsigned)((n_changed + RF_THREADS - 1) / RF_THREADS);
+        hipLaunchKernelGGL(k_refit_rank, dim3(gc), dim3(RF_THREADS), 0, s, n, b.total, r.ck, r.ci, b.keys[cur],
+                           b.vals[0], cpos, r.csk, r.csi, r.cspos);
+    }
+    hipLaunchKernelGGL(k_refit_place, dim3(g), dim3(RF_THREADS), 0, s, coords, n, b.total, b.keys[cur], b.vals[0],
+                       cpos, r.csk, r.csi, r.cspos, b.keys[cur ^ 1], b.perm, b.scoord, index_base);
+    return hipGetLastError();
+}
+
+// Stage 2b: topology and bounds from the sorted keys in keys[cur].
+hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base, int cur, const BuildBufs& b,
+                                 hipStream_t s) {
     const uint64_t* sk = b.keys[cur];
     const int nblk = (int)topo_blocks(n);
     hipLaunchKernelGGL(k_topo_count, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, n, T, b.topo, b.hist, nblk);

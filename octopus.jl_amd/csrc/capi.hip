@@ -55,6 +55,12 @@ struct SetState {
     bool built = false;
     int key_bits = 0;
     BuildParams hp{};  // host copy after the last build
+    // the previous sort (keys[cur], scoord) for the almost-sorted refit
+    bool sorted_ok = false;
+    int64_t sorted_n = 0;
+    int cur = 0;
+    oct_build_info info{};
+    DevBuf rk, ri, rsk, rsi, rpos;  // refit: changed points
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
         nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax;
     BuildBufs bufs() const {
@@ -85,7 +91,8 @@ struct SetState {
     }
     void release_all() {
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
-                         &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax};
+                         &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
+                         &rk, &ri, &rsk, &rsi, &rpos};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -238,11 +245,16 @@ int mark_all_dirty(oct_ctx* c) {
     return OCT_OK;
 }
 
-int read_params(oct_ctx* c, SetState& s) {
+// Waits for the stream and copies the build parameters (and, when asked, the
+// 64-bit scan total) to the host.
+int read_params(oct_ctx* c, SetState& s, uint64_t* total = nullptr) {
     HIP_TRY(c, hipMemcpyAsync(c->pinned, s.prm.p, sizeof(BuildParams), hipMemcpyDeviceToHost, c->stream));
+    if (total)
+        HIP_TRY(c, hipMemcpyAsync((char*)c->pinned + 1024, s.total.p, 8, hipMemcpyDeviceToHost, c->stream));
     HostTimer ht("run: wait for build parameters");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     memcpy(&s.hp, c->pinned, sizeof(BuildParams));
+    if (total) memcpy(total, (char*)c->pinned + 1024, 8);
     return OCT_OK;
 }
 
@@ -284,19 +296,47 @@ int build_set(oct_ctx* c, SetState& s) {
     HIP_TRY(c, s.bmin.ensure(12 * (size_t)M));
     HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
     const BuildBufs b = s.bufs();
+    // almost-sorted refit: the previous sort of the same n points is reused
+    const bool try_refit = c->refit_mode && s.sorted_ok && s.sorted_n == n;
+    s.sorted_ok = false;
+    memset(&s.info, 0, sizeof(s.info));
     {
         Phase ph(c, OCT_PROF_BOUNDS);
         HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, b, c->stream));
     }
-    int rc = read_params(c, s);
+    if (try_refit) {
+        Phase ph(c, OCT_PROF_BUILD);
+        HIP_TRY(c, launch_refit_prepare(s.coords, n, s.cur, b, c->stream));
+    }
+    uint64_t changed = 0;
+    int rc = read_params(c, s, try_refit ? &changed : nullptr);
     if (rc != OCT_OK) return rc;
     s.key_bits = s.hp.key_bits;
+    int cur = 0;
     {
         Phase ph(c, OCT_PROF_BUILD);
-        HIP_TRY(c, launch_build_stage(s.coords, n, s.key_bits, T, c->index_base, b, c->stream));
+        if (try_refit && (int64_t)changed <= std::min<int64_t>(REFIT_MAX_CHANGED, n / 8)) {
+            HIP_TRY(c, s.rk.ensure(8 * (size_t)REFIT_MAX_CHANGED));
+            HIP_TRY(c, s.ri.ensure(4 * (size_t)REFIT_MAX_CHANGED));
+            HIP_TRY(c, s.rsk.ensure(8 * (size_t)REFIT_MAX_CHANGED));
+            HIP_TRY(c, s.rsi.ensure(4 * (size_t)REFIT_MAX_CHANGED));
+            HIP_TRY(c, s.rpos.ensure(4 * (size_t)REFIT_MAX_CHANGED));
+            const RefitBufs r{s.rk.as<uint64_t>(), s.ri.as<int32_t>(), s.rsk.as<uint64_t>(), s.rsi.as<int32_t>(),
+                              s.rpos.as<uint32_t>()};
+            HIP_TRY(c, launch_refit_finish(s.coords, n, (int64_t)changed, s.cur, c->index_base, b, r, c->stream));
+            cur = s.cur ^ 1;
+            s.info.refit_used = 1;
+        } else {
+            HIP_TRY(c, launch_sort_stage(s.coords, n, s.key_bits, c->index_base, b, &cur, c->stream));
+        }
+        s.info.changed_points = try_refit ? (int64_t)changed : -1;
+        HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, cur, b, c->stream));
     }
     rc = read_params(c, s);
     if (rc != OCT_OK) return rc;
+    s.cur = cur;
+    s.sorted_ok = true;
+    s.sorted_n = n;
     s.built = true;
     s.dirty = false;
     return OCT_OK;
@@ -906,6 +946,13 @@ int oct_predictor_train(oct_ctx* c, int steps, int batch, float lr, uint64_t see
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (loss) *loss = hl;
     m.loaded = true;
+    return OCT_OK;
+}
+
+int oct_build_info_get(oct_ctx* c, int id, oct_build_info* out) {
+    if (!c || !out) return OCT_ERR_ARGUMENT;
+    if (!valid_set(c, id)) return fail(c, OCT_ERR_ARGUMENT, "unknown point set");
+    *out = c->sets[set_index(c, id)].info;
     return OCT_OK;
 }
 

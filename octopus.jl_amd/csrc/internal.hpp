@@ -77,8 +77,26 @@ int64_t hist_words(int64_t n);
 int64_t max_nodes(int64_t n, int T);
 hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, const BuildBufs& b,
                                hipStream_t s);
-hipError_t launch_build_stage(const float* coords, int64_t n, int key_bits, int T, int index_base,
-                              const BuildBufs& b, hipStream_t s);
+hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int index_base, const BuildBufs& b,
+                             int* cur_out, hipStream_t s);
+hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base, int cur, const BuildBufs& b,
+                                 hipStream_t s);
+
+// Almost-sorted refit (time stepping): the previous sort is keys[cur] and
+// scoord.  prepare: new keys in that order and the changed count in *b.total;
+// finish (changed count <= REFIT_MAX_CHANGED): sorted keys in keys[cur ^ 1],
+// perm and scoord rewritten, identical to the full sort's.
+constexpr int64_t REFIT_MAX_CHANGED = 32768;
+struct RefitBufs {
+    uint64_t* ck;    // changed keys (compacted)
+    int32_t* ci;     // changed point indices
+    uint64_t* csk;   // changed, sorted by (key, index)
+    int32_t* csi;
+    uint32_t* cspos; // their final positions
+};
+hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, const BuildBufs& b, hipStream_t s);
+hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t n_changed, int cur, int index_base,
+                               const BuildBufs& b, const RefitBufs& r, hipStream_t s);
 
 // Edge outputs (see edges.hip).
 struct EdgeOut {

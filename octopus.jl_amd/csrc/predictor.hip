@@ -19,42 +19,37 @@ namespace oct {
 
 // ------------------------------------------------------------ features
 
-constexpr int FT_WAVES = 4;  // leaves per block, one wave each
+constexpr int FT_THREADS = 128;
 
-__global__ void __launch_bounds__(FT_WAVES * OCT_WAVE)
+// One thread per query leaf (many leaves per wave keep the chain of
+// dependent loads overlapped); the 26 direction bins live in LDS.
+__global__ void __launch_bounds__(FT_THREADS)
 k_leaf_features(const NodeRec* __restrict__ qleaves, const int32_t* __restrict__ leaf_ids,
                 const uint8_t* __restrict__ nlevel, int64_t n_leaves, const int2* __restrict__ lists,
                 const uint8_t* __restrict__ ldir, const int32_t* __restrict__ lcount, int cap, float r,
                 float* __restrict__ feat) {
-    __shared__ uint32_t bins[FT_WAVES][32];
-    const int wv = threadIdx.x / OCT_WAVE;
-    const int lane = lane_id();
-    const int64_t l = (int64_t)blockIdx.x * FT_WAVES + wv;
-    if (l >= n_leaves) return;  // wave-uniform
-    if (lane < 32) bins[wv][lane] = 0;
-    __builtin_amdgcn_wave_barrier();
+    __shared__ uint32_t acc[MLP_ADJ][FT_THREADS];
+    const int tid = threadIdx.x;
+    const int64_t l = (int64_t)blockIdx.x * FT_THREADS + tid;
+    if (l >= n_leaves) return;
+#pragma unroll
+    for (int b = 0; b < MLP_ADJ; ++b) acc[b][tid] = 0u;
     const int n = min(lcount[l], cap);
     const int2* lst = lists + l * cap;
     const uint8_t* dir = ldir + l * cap;
-    // lanes take consecutive list entries; integer bin sums are order-free
-    for (int e = lane; e < n; e += OCT_WAVE) {
+    for (int e = 0; e < n; ++e) {
         const int d = dir[e];
-        if (d != 13 && d <= 26) atomicAdd(&bins[wv][d < 13 ? d : d - 1], (uint32_t)lst[e].y);
+        if (d == 13 || d > 26) continue;
+        acc[d < 13 ? d : d - 1][tid] += (uint32_t)lst[e].y;
     }
-    __builtin_amdgcn_wave_barrier();
     const NodeRec qa = qleaves[l];
-    float v = 0.f;
-    if (lane == 0) {
-        v = log1pf((float)rec_b(qa));
-    } else if (lane == 1) {
-        v = (float)nlevel[leaf_ids[l]] / 21.f;
-    } else if (lane == 2) {
-        const float ext = fmaxf(fmaxf(qa.hi.x - qa.lo.x, qa.hi.y - qa.lo.y), qa.hi.z - qa.lo.z);
-        v = fmaxf(log2f(fmaxf(ext / r, 1e-30f)), -20.f);
-    } else if (lane < MLP_IN) {
-        v = log1pf((float)bins[wv][lane - 3]);
-    }
-    if (lane < MLP_IN) feat[l * MLP_IN + lane] = v;
+    const float ext = fmaxf(fmaxf(qa.hi.x - qa.lo.x, qa.hi.y - qa.lo.y), qa.hi.z - qa.lo.z);
+    float* f = feat + l * MLP_IN;
+    f[0] = log1pf((float)rec_b(qa));
+    f[1] = (float)nlevel[leaf_ids[l]] / 21.f;
+    f[2] = fmaxf(log2f(fmaxf(ext / r, 1e-30f)), -20.f);
+#pragma unroll
+    for (int b = 0; b < MLP_ADJ; ++b) f[3 + b] = log1pf((float)acc[b][tid]);
 }
 
 // ------------------------------------------------------------ inference
@@ -373,8 +368,8 @@ k_plan_write(const float* __restrict__ pred, int64_t n, uint32_t budget, int max
 hipError_t launch_leaf_features(const TreeView& q, const int2* lists, const uint8_t* ldir, const int32_t* lcount,
                                 int cap, float r, float* feat, hipStream_t s) {
     if (q.n_leaves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_leaf_features, dim3((unsigned)((q.n_leaves + FT_WAVES - 1) / FT_WAVES)),
-                       dim3(FT_WAVES * OCT_WAVE), 0, s, q.leaves, q.leaf_ids, q.nlevel, (int64_t)q.n_leaves, lists, ldir,
+    hipLaunchKernelGGL(k_leaf_features, dim3((unsigned)((q.n_leaves + FT_THREADS - 1) / FT_THREADS)),
+                       dim3(FT_THREADS), 0, s, q.leaves, q.leaf_ids, q.nlevel, (int64_t)q.n_leaves, lists, ldir,
                        lcount, cap, r, feat);
     return hipGetLastError();
 }

@@ -310,6 +310,15 @@ def get_tree(tns: TNS, set_id: int) -> Tree:
     )
 
 
+def build_info(tns: TNS, set_id: int) -> dict:
+    """How the last run built a set: {"refit_used": bool, "changed_points": int}
+    (changed_points -1 when no previous sort was tried)."""
+    _sid(tns, set_id)
+    b = L.OctBuildInfo()
+    tns._check(tns._lib.oct_build_info_get(tns._c(), int(set_id), ctypes.byref(b)))
+    return {"refit_used": bool(b.refit_used), "changed_points": int(b.changed_points)}
+
+
 def build_edges(tns: TNS, q: int, t: int, mode: str = "exact", order: str = "leaf") -> Edges:
     """build_edges(tns, q, t) (src/api.jl:304-337): flat-COO
     (senders, receivers, rel_displacement (E,3), rel_dist_norm (E,1)).

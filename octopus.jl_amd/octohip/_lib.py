@@ -74,6 +74,14 @@ class OctView(ctypes.Structure):
     ]
 
 
+class OctBuildInfo(ctypes.Structure):
+    _fields_ = [
+        ("changed_points", ctypes.c_int64),
+        ("refit_used", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
 class OctEdgeStats(ctypes.Structure):
     _fields_ = [
         ("features", ctypes.c_void_p),
@@ -122,6 +130,7 @@ SIGNATURES = {
     "oct_profile_read": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int64)]),
     "oct_synchronize": (ctypes.c_int, [_P]),
+    "oct_build_info_get": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(OctBuildInfo)]),
     "oct_set_predictor_policy": (ctypes.c_int, [_P, ctypes.c_int]),
     "oct_predictor_load": (ctypes.c_int, [_P, _P]),
     "oct_predictor_save": (ctypes.c_int, [_P, _P]),

@@ -44,6 +44,8 @@ def test_struct_sizes_match_header():
     assert ctypes.sizeof(_lib.OctEdges) == 40
     assert ctypes.sizeof(_lib.OctTree) == 6 * 8 + 4 + 4 + 8 + 12 + 4
     assert ctypes.sizeof(_lib.OctView) == 10 * 8 + 4 * 4 + 8 + 8
+    assert ctypes.sizeof(_lib.OctEdgeStats) == 3 * 8 + 4 * 8 + 8 + 4 * 4
+    assert ctypes.sizeof(_lib.OctBuildInfo) == 16
 
 
 def test_argument_errors_without_device():
