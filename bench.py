@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--null-stream", action="store_true", help="run the library on torch's current stream")
     ap.add_argument("--predictor", default="sph_lattice",
                     help="shipped predictor for the learned mode ('none': plan-free single pass)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="before the workload, copy a MIB-MiB device buffer 3 times (a known-size kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--no-refit", action="store_true",
                     help="full sort every step (default: refit mode, the almost-sorted path of time stepping)")
     ap.add_argument("--predict-always", action="store_true",
@@ -53,6 +56,14 @@ def main():
 
     import octohip
     from octohip import datasets
+
+    if a.calibrate_copy > 0:
+        src = torch.ones(a.calibrate_copy * 262144, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     h = 0.01
     pts, r = datasets.sph_lattice(side=a.side, h=h)

@@ -406,6 +406,8 @@ int oct_set_stream(oct_ctx* c, void* stream) {
 int oct_set_search_radius(oct_ctx* c, float r) {
     if (!c) return OCT_ERR_ARGUMENT;
     if (!(r > 0.f)) return fail(c, OCT_ERR_ARGUMENT, "search radius must be positive");
+    // the kernels rely on a finite r * r (empty candidate slots sit at infinity)
+    if (!std::isfinite(r * r)) return fail(c, OCT_ERR_ARGUMENT, "search radius squared must be finite in Float32");
     c->radius = r;
     return mark_all_dirty(c);
 }

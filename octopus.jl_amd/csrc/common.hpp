@@ -96,7 +96,7 @@ __device__ inline int mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+__device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 template <typename T>
 __device__ inline T wave_min_u(T v) {

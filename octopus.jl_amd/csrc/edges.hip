@@ -154,11 +154,63 @@ constexpr int EW_TILE = 4;      // candidate chunks of 64 held in registers per 
 struct WaveLds {
     int32_t* start;  // 65: prefix of candidate-leaf sizes
     int32_t* first;  // 64: first sorted position of each candidate leaf
-    int32_t* sen;    // EW_STAGE staged senders
+    int32_t* sen;    // `stage` staged senders
     int32_t* rcv;
-    float* disp;     // 3 * EW_STAGE
+    float* disp;     // 3 * stage
     float* dist;
+    uint32_t stage;  // edges the stage holds
+    int32_t* cand;   // EW_TILE * 64: sorted positions of the candidates that pass the query-box filter
 };
+
+// Loads candidate points t0 .. t0 + 255 of the sequence that L.start /
+// L.first describe into tp / tpos (w = point index; -1 marks an empty slot).
+// With `filter`, only points whose FP32 distance to the query box is within r
+// are kept, compacted in sequence order: that distance is a lower bound of
+// the distance to every query in the box (each step is a monotone IEEE
+// operation), so no edge is lost.  Returns the number of points loaded.
+__device__ inline int load_tile(int t0, int total, const float4* __restrict__ tsc, const WaveLds& L, int lane,
+                                bool filter, float4 qlo, float4 qhi, float r_sq, float4 (&tp)[EW_TILE],
+                                int32_t (&tpos)[EW_TILE]) {
+#pragma unroll
+    for (int u = 0; u < EW_TILE; ++u) {
+        const int ci = t0 + u * OCT_WAVE + lane;
+        tp[u] = make_float4(INFINITY, INFINITY, INFINITY, __int_as_float(-1));  // never within r
+        tpos[u] = 0;
+        if (ci < total) {
+            int lo = 0, hi = OCT_WAVE;  // owner: last o with start[o] <= ci
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (L.start[mid] <= ci) lo = mid;
+                else hi = mid;
+            }
+            tpos[u] = L.first[lo] + (ci - L.start[lo]);
+            tp[u] = tsc[tpos[u]];
+        }
+    }
+    const int n = min(total - t0, EW_TILE * OCT_WAVE);
+    if (!filter) return n;
+    int kept = 0;
+#pragma unroll
+    for (int u = 0; u < EW_TILE; ++u) {
+        const bool keep = t0 + u * OCT_WAVE + lane < total &&
+                          point_box_d2(tp[u].x, tp[u].y, tp[u].z, qlo, qhi) <= r_sq;
+        const uint64_t m = ballot(keep);
+        if (keep) L.cand[kept + mbcnt(m)] = tpos[u];
+        kept += __popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < EW_TILE; ++u) {
+        const int k = u * OCT_WAVE + lane;
+        tp[u] = make_float4(INFINITY, INFINITY, INFINITY, __int_as_float(-1));  // never within r
+        if (k < kept) {
+            tpos[u] = L.cand[k];
+            tp[u] = tsc[tpos[u]];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    return kept;
+}
 
 // One query leaf through one wave.  Lanes take candidate target points (a
 // tile of 4 x 64 in registers); the leaf's queries sit one per lane and are
@@ -170,12 +222,14 @@ struct WaveLds {
 //          whole-wave stores.
 // slot = sorted position (leaf order) or original index (receiver order).
 template <int MODE>
-__device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, int32_t ncand, int2 lf0, const int2* __restrict__ lst,
-                                     const float4* __restrict__ qsc, const float4* __restrict__ tsc, float r_sq,
-                                     float inv_r, int self_pair, int by_receiver, uint32_t* __restrict__ qcount,
+__device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 qhi, int32_t ncand, int2 lf0,
+                                     const int2* __restrict__ lst, const float4* __restrict__ qsc,
+                                     const float4* __restrict__ tsc, float r_sq, float inv_r, int self_pair,
+                                     int by_receiver, uint32_t* __restrict__ qcount,
                                      const uint32_t* __restrict__ qoff, EdgeOut out, int index_base,
                                      const WaveLds& L, int lane) {
     uint32_t leaf_total = 0;
+    const bool filter = qc <= OCT_WAVE;  // the leaf box bounds the one query block
     for (int qb = 0; qb < qc; qb += OCT_WAVE) {
         const int nqb = min(OCT_WAVE, qc - qb);
         // query qb + lane lives in this lane
@@ -195,7 +249,7 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, int32_t ncand, int2
             if (!by_receiver) {
                 blk_base = (uint32_t)__builtin_amdgcn_readfirstlane((int)my);
                 blk_tot = (uint32_t)wave_sum((int)c);
-                staged = blk_tot <= (uint32_t)EW_STAGE;
+                staged = blk_tot <= L.stage;
             }
         }
         for (int g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
@@ -209,26 +263,15 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, int32_t ncand, int2
             __builtin_amdgcn_wave_barrier();
             for (int t0 = 0; t0 < total; t0 += EW_TILE * OCT_WAVE) {
                 float4 tp[EW_TILE];
-#pragma unroll
-                for (int u = 0; u < EW_TILE; ++u) {
-                    const int ci = t0 + u * OCT_WAVE + lane;
-                    tp[u] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-                    if (ci < total) {
-                        int lo = 0, hi = OCT_WAVE;  // owner: last o with start[o] <= ci
-                        while (hi - lo > 1) {
-                            const int mid = (lo + hi) >> 1;
-                            if (L.start[mid] <= ci) lo = mid;
-                            else hi = mid;
-                        }
-                        tp[u] = tsc[L.first[lo] + (ci - L.start[lo])];
-                    }
-                }
-                const int nu = min(EW_TILE, (total - t0 + OCT_WAVE - 1) / OCT_WAVE);
+                int32_t tpos[EW_TILE];
+                const int got = load_tile(t0, total, tsc, L, lane, filter, qlo, qhi, r_sq, tp, tpos);
+                const int nu = (got + OCT_WAVE - 1) / OCT_WAVE;
                 for (int qi = 0; qi < nqb; ++qi) {
                     const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
                     const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
                     const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
                     const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
+        const int32_t ix = self_pair ? i : -2;  // -2 matches no point: only the self pair drops j == i
                     uint32_t run = MODE == EDGE_FILL ? (uint32_t)__builtin_amdgcn_readlane((int)my, qi) : 0u;
                     const uint32_t run0 = run;
 #pragma unroll
@@ -239,9 +282,10 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, int32_t ncand, int2
                         const float dy = qy - tp[u].y;
                         const float dz = qz - tp[u].z;
                         const float d2 = dx * dx + dy * dy + dz * dz;
-                        const bool pass = j >= 0 && d2 <= r_sq && !(self_pair && j == i);
-                        const uint64_t m = ballot(pass);
-                        if (MODE == EDGE_FILL && pass) {
+                                                const bool in_r = d2 <= r_sq;
+                        const bool other = j != ix;
+                        const uint64_t m = ballot(in_r) & ballot(other);  // compare masks ANDed, no branch
+                        if (MODE == EDGE_FILL && (in_r & other)) {
                             const uint32_t e = run + (uint32_t)mbcnt(m);
                             if (staged) {
                                 const uint32_t s = e - blk_base;
@@ -297,9 +341,16 @@ __device__ inline bool one_tile_leaf(int32_t qc, int32_t ncand, int total) {
     return qc <= OCT_WAVE && ncand <= OCT_WAVE && total <= EW_TILE * OCT_WAVE;
 }
 
-__device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, int32_t ncand, int2 lf0,
+// Hit-list entry: compact = query slot in the leaf (6 bits) << 26 | target
+// sorted position (needs fewer than 2^26 target points); wide = (query
+// sorted position, target sorted position).
+constexpr uint32_t HIT_POS_BITS = 26;
+constexpr uint32_t HIT_POS_MASK = (1u << HIT_POS_BITS) - 1u;
+
+template <bool WIDE>
+__device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, float4 qlo, float4 qhi, int32_t ncand, int2 lf0,
                                              const float4* __restrict__ qsc, const float4* __restrict__ tsc,
-                                             float r_sq, int self_pair, const WaveLds& L, int lane, int2* hits,
+                                             float r_sq, int self_pair, const WaveLds& L, int lane, uint32_t* hits,
                                              uint32_t room) {
     const int nqb = qc;
     float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -314,30 +365,15 @@ __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, int32_t nca
     __builtin_amdgcn_wave_barrier();
     float4 tp[EW_TILE];
     int32_t tpos[EW_TILE];
-#pragma unroll
-    for (int u = 0; u < EW_TILE; ++u) {
-        const int ci = u * OCT_WAVE + lane;
-        tp[u] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-        tpos[u] = 0;
-        if (ci < total) {
-            int lo = 0, hi = OCT_WAVE;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (L.start[mid] <= ci) lo = mid;
-                else hi = mid;
-            }
-            tpos[u] = L.first[lo] + (ci - L.start[lo]);
-            tp[u] = tsc[tpos[u]];
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    const int nu = min(EW_TILE, (total + OCT_WAVE - 1) / OCT_WAVE);
+    const int got = load_tile(0, total, tsc, L, lane, true, qlo, qhi, r_sq, tp, tpos);
+    const int nu = (got + OCT_WAVE - 1) / OCT_WAVE;
     uint32_t run = 0;
     for (int qi = 0; qi < nqb; ++qi) {
         const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
         const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
         const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
         const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
+        const int32_t ix = self_pair ? i : -2;  // -2 matches no point: only the self pair drops j == i
 #pragma unroll
         for (int u = 0; u < EW_TILE; ++u) {
             if (u >= nu) break;
@@ -346,11 +382,15 @@ __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, int32_t nca
             const float dy = qy - tp[u].y;
             const float dz = qz - tp[u].z;
             const float d2 = dx * dx + dy * dy + dz * dz;
-            const bool pass = j >= 0 && d2 <= r_sq && !(self_pair && j == i);
-            const uint64_t m = ballot(pass);
-            if (pass) {
+                        const bool in_r = d2 <= r_sq;
+            const bool other = j != ix;
+            const uint64_t m = ballot(in_r) & ballot(other);  // compare masks ANDed, no branch
+            if (in_r & other) {
                 const uint32_t h = run + (uint32_t)mbcnt(m);
-                if (h < room) hits[h] = make_int2(qf + qi, tpos[u]);
+                if (h < room) {
+                    if (WIDE) reinterpret_cast<int2*>(hits)[h] = make_int2(qf + qi, tpos[u]);
+                    else hits[h] = ((uint32_t)qi << HIT_POS_BITS) | (uint32_t)tpos[u];
+                }
             }
             run += (uint32_t)__popcll(m);
         }
@@ -358,23 +398,32 @@ __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, int32_t nca
     return run;
 }
 
-// Writes n recorded hits at out[base ..): every lane takes consecutive hits,
-// so the stores of one instruction are contiguous.
-__device__ inline void emit_hits(const int2* hits, uint32_t n, uint32_t base, const float4* __restrict__ qsc,
-                                 const float4* __restrict__ tsc, float inv_r, EdgeOut out, int index_base,
-                                 int lane) {
+// Writes n recorded hits of the leaf whose first query is qf at out[base ..):
+// every lane takes consecutive hits, so the stores of one instruction are
+// contiguous (the displacement as one 12-byte store per edge).
+template <bool WIDE>
+__device__ inline void emit_hits(const uint32_t* hits, uint32_t n, uint32_t base, int32_t qf,
+                                 const float4* __restrict__ qsc, const float4* __restrict__ tsc, float inv_r,
+                                 EdgeOut out, int index_base, int lane) {
     for (uint32_t t = lane; t < n; t += OCT_WAVE) {
-        const int2 h = hits[t];
-        const float4 q = qsc[h.x];
-        const float4 p = tsc[h.y];
+        int32_t qp, tp;
+        if (WIDE) {
+            const int2 h = reinterpret_cast<const int2*>(hits)[t];
+            qp = h.x;
+            tp = h.y;
+        } else {
+            const uint32_t h = hits[t];
+            qp = qf + (int32_t)(h >> HIT_POS_BITS);
+            tp = (int32_t)(h & HIT_POS_MASK);
+        }
+        const float4 q = qsc[qp];
+        const float4 p = tsc[tp];
         const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
         const float d2 = dx * dx + dy * dy + dz * dz;
         const uint32_t e = base + t;
         out.senders[e] = __float_as_int(p.w) + index_base;
         out.receivers[e] = __float_as_int(q.w) + index_base;
-        out.rdisp[3 * (int64_t)e + 0] = dx * inv_r;
-        out.rdisp[3 * (int64_t)e + 1] = dy * inv_r;
-        out.rdisp[3 * (int64_t)e + 2] = dz * inv_r;
+        *reinterpret_cast<float3*>(out.rdisp + 3 * (int64_t)e) = make_float3(dx * inv_r, dy * inv_r, dz * inv_r);
         out.rdist[e] = sqrtf(d2) * inv_r;
     }
 }
@@ -388,7 +437,10 @@ struct EdgeLds {
     int32_t rcv[EW_WAVES][STAGE];
     float disp[EW_WAVES][3 * STAGE];
     float dist[EW_WAVES][STAGE];
-    __device__ WaveLds wave(int w) { return WaveLds{start[w], first[w], sen[w], rcv[w], disp[w], dist[w]}; }
+    int32_t cand[EW_WAVES][EW_TILE * OCT_WAVE];
+    __device__ WaveLds wave(int w) {
+        return WaveLds{start[w], first[w], sen[w], rcv[w], disp[w], dist[w], (uint32_t)STAGE, cand[w]};
+    }
 };
 
 // Two-pass driver: one wave per query leaf over a contiguous, XCD-local range
@@ -409,27 +461,28 @@ k_edges(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, con
     const int32_t per = (nleaves + nwaves - 1) / nwaves;
     const int32_t r_lo = __builtin_amdgcn_readfirstlane(w * per);
     const int32_t r_hi = min(nleaves, r_lo + per);
-    int2 nx_meta = make_int2(0, 0), nx_lf = make_int2(0, 0);
+    NodeRec nx_rec{};
+    int2 nx_lf = make_int2(0, 0);
     int32_t nx_cnt = 0;
     if (r_lo < r_hi) {
-        const NodeRec qa = qleaves[r_lo];
-        nx_meta = make_int2(rec_a(qa), rec_b(qa));
+        nx_rec = qleaves[r_lo];
         nx_cnt = lcount[r_lo];
         if (lane < cap) nx_lf = lists[(int64_t)r_lo * cap + lane];
     }
     for (int32_t r = r_lo; r < r_hi; ++r) {
-        const int32_t qf = __builtin_amdgcn_readfirstlane(nx_meta.x);
-        const int32_t qc = __builtin_amdgcn_readfirstlane(nx_meta.y);
+        const NodeRec qa = nx_rec;
+        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
         const int32_t ncand = __builtin_amdgcn_readfirstlane(min(nx_cnt, cap));
         const int2 lf0 = nx_lf;
         if (r + 1 < r_hi) {
-            const NodeRec qa = qleaves[r + 1];
-            nx_meta = make_int2(rec_a(qa), rec_b(qa));
+            nx_rec = qleaves[r + 1];
             nx_cnt = lcount[r + 1];
             if (lane < cap) nx_lf = lists[(int64_t)(r + 1) * cap + lane];
         }
-        const uint32_t lt = leaf_pass<MODE>(qf, qc, ncand, lf0, lists + (int64_t)r * cap, qsc, tsc, r_sq, inv_r,
-                                            self_pair, by_receiver, qcount, qoff, out, index_base, L, lane);
+        const uint32_t lt = leaf_pass<MODE>(qf, qc, qa.lo, qa.hi, ncand, lf0, lists + (int64_t)r * cap, qsc, tsc,
+                                            r_sq, inv_r, self_pair, by_receiver, qcount, qoff, out, index_base, L,
+                                            lane);
         if (MODE == EDGE_COUNT && lhits && lane == 0) lhits[r] = lt;
     }
 }
@@ -437,14 +490,15 @@ k_edges(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, con
 // ------------------------------------------------------------ single pass
 
 constexpr int SP_LEAVES_PER_WAVE = 4;
-constexpr int SP_TILE_LEAVES = EW_WAVES * SP_LEAVES_PER_WAVE;
+constexpr int SP_MAX_LEAVES = SP_PLAN_MAX_LEAVES;  // leaves one wave may own under a plan
 constexpr uint64_t SP_FLAG_AGG = 1ull << 62;   // tile aggregate published
 constexpr uint64_t SP_FLAG_PRE = 2ull << 62;   // tile inclusive prefix published
 constexpr uint64_t SP_VALUE = (1ull << 62) - 1;
 constexpr uint32_t SP_SPIN_LIMIT = 1u << 22;   // bounded wait: a stuck look-back reports an error
 
-constexpr int SP_HITS = SP_PLAN_BUDGET;  // hit list entries per wave (8 KB of LDS)
-static_assert(6 * EW_STAGE * 4 <= SP_HITS * 8, "the FILL stage is laid over the hit list");
+constexpr int SP_HIT_WORDS = 1024;                 // hit list of one wave: 4 KB
+constexpr uint32_t SP_STAGE = SP_HIT_WORDS * 4 / 24;  // FILL stage laid over the hit list: 170 edges
+static_assert(SP_PLAN_BUDGET == SP_HIT_WORDS, "the plan packs one wave's compact hit list");
 
 // Decoupled look-back by one whole wave: lanes read the states of the 64
 // tiles before `hi` at once, the nearest tile with a published inclusive
@@ -480,30 +534,30 @@ __device__ inline uint64_t wave_look_back(const uint64_t* tile_state, int32_t t,
     return excl;
 }
 
-constexpr int SP_MAX_LEAVES = SP_PLAN_MAX_LEAVES;  // leaves one wave may own under a plan
-
-// Single traversal in leaf order.  Waves own runs of consecutive query
-// leaves: SP_LEAVES_PER_WAVE each, or the runs of a plan (wave_first[w] ..
-// wave_first[w + 1], at most SP_MAX_LEAVES, *plan_waves runs) that packs the
-// predicted hits of a run into one hit list.  A tile is EW_WAVES consecutive
-// runs; blocks take tiles by ticket until none is left.  Phase A sweeps each
+// Single traversal in leaf order.  Every wave owns a run of consecutive query
+// leaves, which is also its output tile: SP_LEAVES_PER_WAVE leaves per run, or
+// the runs of a plan (wave_first[w] .. wave_first[w + 1], at most
+// SP_MAX_LEAVES, *plan_waves runs) that packs the predicted hits of a run into
+// the wave's hit list.  Blocks take EW_WAVES consecutive runs per ticket
+// until none is left; the waves then work independently.  Phase A sweeps each
 // leaf once: a leaf that fits one register tile records its hits in the
 // wave's LDS hit list (query-grouped, already in output order) and counts
 // them; any other leaf, or one whose hits no longer fit the list, is counted
-// with the COUNT body and recounted in phase B (stats[0] counts those, stats[1]
-// the recorded ones).  lhits (optional) receives every leaf's exact hit count.
-// The tile's dense output offset comes from a decoupled look-back over the
-// tile states of earlier tickets.  Phase B writes the recorded hits with
+// with the COUNT body and recounted in phase B (stats[0] counts those,
+// stats[1] the recorded ones).  lhits (optional) receives every leaf's exact
+// hit count.  The run's dense output offset comes from a decoupled look-back
+// over the earlier runs' states.  Phase B writes the recorded hits with
 // contiguous stores, then re-sweeps the other leaves (the exact recount, with
 // the FILL body, whose LDS stage lies over the hit list once it is written
 // out).  No count kernel, no scan and no host round trip precede the emit.
-// A tile whose range would pass cap_edges writes nothing and raises flags[0];
-// the host then reruns with room for the total the last tile reports in
-// flags[2..3].  flags[1] reports a look-back that gave up or a run longer than
-// SP_MAX_LEAVES (the host then takes the exact path).  Tickets are taken in
-// order, and a block finishes its tile before taking another, so every tile a
-// look-back waits on is held by a running block that never waits on a later
-// tile: the wait always ends (and is bounded anyway).
+// A run whose range would pass cap_edges writes nothing and raises flags[0];
+// the host then reruns with room for the total the last run reports in
+// flags[2..3].  flags[1] reports a look-back that gave up (1) or a run longer
+// than SP_MAX_LEAVES (2); the host then takes the exact path.  A look-back
+// only waits on lower runs, which belong to this block's lower waves or to
+// tickets taken earlier by running blocks, so every wait ends (and is bounded
+// anyway).  WIDE: 8-byte hits (any point count; half the list).
+template <bool WIDE>
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
 k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
                const BuildParams* __restrict__ qprm, const float4* __restrict__ tsc, const int2* __restrict__ lists,
@@ -512,47 +566,49 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
                const int32_t* __restrict__ plan_waves, uint32_t* __restrict__ lhits,
                uint64_t* __restrict__ tile_state, uint32_t* __restrict__ ticket, uint32_t* __restrict__ flags,
                uint32_t* __restrict__ stats, uint64_t cap_edges, EdgeOut out, int index_base) {
+    constexpr uint32_t HITS = WIDE ? SP_HIT_WORDS / 2 : SP_HIT_WORDS;
     __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
     __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
-    __shared__ int2 s_hits[EW_WAVES][SP_HITS];
+    __shared__ uint32_t s_hits[EW_WAVES][SP_HIT_WORDS];
     __shared__ uint32_t s_cnt[EW_WAVES][SP_MAX_LEAVES];
+    __shared__ int32_t s_qf[EW_WAVES][SP_MAX_LEAVES];
+    __shared__ int32_t s_cand[EW_WAVES][EW_TILE * OCT_WAVE];
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wtot[EW_WAVES];
-    __shared__ uint64_t s_prefix;
-    __shared__ uint32_t s_ok;
+    __shared__ uint32_t s_stat[2];
     const int wv = threadIdx.x / OCT_WAVE;
     const int lane = lane_id();
-    int2* hits = s_hits[wv];
+    uint32_t* hits = s_hits[wv];
     uint32_t* cntw = s_cnt[wv];
+    int32_t* qfw = s_qf[wv];
     int32_t* stage = reinterpret_cast<int32_t*>(hits);
-    const WaveLds L{s_start[wv], s_first[wv], stage, stage + EW_STAGE, reinterpret_cast<float*>(stage + 2 * EW_STAGE),
-                    reinterpret_cast<float*>(stage + 5 * EW_STAGE)};
+    const WaveLds L{s_start[wv], s_first[wv], stage, stage + SP_STAGE, reinterpret_cast<float*>(stage + 2 * SP_STAGE),
+                    reinterpret_cast<float*>(stage + 5 * SP_STAGE), SP_STAGE, s_cand[wv]};
     const int32_t nleaves = qprm->n_leaves;
-    const int32_t nwaves = wave_first ? *plan_waves : (nleaves + SP_LEAVES_PER_WAVE - 1) / SP_LEAVES_PER_WAVE;
-    const int32_t ntiles = (nwaves + EW_WAVES - 1) / EW_WAVES;
-    uint32_t n_recount = 0, n_record = 0;  // this wave's, over all its tiles
+    const int32_t nruns = wave_first ? *plan_waves : (nleaves + SP_LEAVES_PER_WAVE - 1) / SP_LEAVES_PER_WAVE;
+    const int32_t ntickets = (nruns + EW_WAVES - 1) / EW_WAVES;
+    uint32_t n_recount = 0, n_record = 0;  // this wave's, over all its runs
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
         __syncthreads();
-        const int32_t t = (int32_t)s_tile;
-        if (t >= ntiles) break;
-        const int32_t w = t * EW_WAVES + wv;
-        int32_t r_lo = 0, r_hi = 0;
-        bool bad = false;
-        if (w < nwaves) {
-            if (wave_first) {
-                r_lo = wave_first[w];
-                r_hi = wave_first[w + 1];
-            } else {
-                r_lo = w * SP_LEAVES_PER_WAVE;
-                r_hi = min(nleaves, r_lo + SP_LEAVES_PER_WAVE);
-            }
+        const int32_t tk = (int32_t)s_tile;
+        __syncthreads();  // everyone has the ticket before thread 0 takes the next one
+        if (tk >= ntickets) break;
+        const int32_t w = tk * EW_WAVES + wv;  // this wave's run and output tile
+        if (w >= nruns) continue;               // wave-uniform
+        int32_t r_lo, r_hi;
+        if (wave_first) {
+            r_lo = wave_first[w];
+            r_hi = wave_first[w + 1];
+        } else {
+            r_lo = w * SP_LEAVES_PER_WAVE;
+            r_hi = min(nleaves, r_lo + SP_LEAVES_PER_WAVE);
         }
         r_lo = __builtin_amdgcn_readfirstlane(r_lo);
         r_hi = __builtin_amdgcn_readfirstlane(r_hi);
-        if (r_lo < 0 || r_hi > nleaves || r_hi - r_lo > SP_MAX_LEAVES) {
+        bool bad = false;
+        if (r_lo < 0 || r_hi > nleaves || r_hi < r_lo || r_hi - r_lo > SP_MAX_LEAVES) {
             bad = true;  // a malformed plan: nothing is swept, the host takes the exact path
-            r_hi = r_lo;
+            r_hi = r_lo = 0;
         }
         // phase A: one sweep per leaf, exact counts
         uint64_t recorded = 0;  // bit k: leaf r_lo + k sits in the hit list
@@ -568,111 +624,102 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
             const int total = __builtin_amdgcn_readfirstlane(wave_sum(lane < ncand ? lf0.y : 0));
             uint32_t c;
             if (one_tile_leaf(qc, ncand, total)) {
-                c = leaf_sweep_record(qf, qc, ncand, lf0, qsc, tsc, r_sq, self_pair, L, lane, hits + hfill,
-                                      (uint32_t)SP_HITS - hfill);
-                if (c <= (uint32_t)SP_HITS - hfill) {
+                c = leaf_sweep_record<WIDE>(qf, qc, qa.lo, qa.hi, ncand, lf0, qsc, tsc, r_sq, self_pair, L, lane,
+                                            WIDE ? hits + 2 * hfill : hits + hfill, HITS - hfill);
+                if (c <= HITS - hfill) {
                     recorded |= 1ull << k;
                     hfill += c;
                     ++nrec;
                 } else {
                     // the list is full: count per query for the recount in phase B
-                    leaf_pass<EDGE_COUNT>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount,
+                    leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0,
+                                          qcount,
                                           nullptr, out, index_base, L, lane);
                 }
             } else {
-                c = leaf_pass<EDGE_COUNT>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount,
+                c = leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0,
+                                          qcount,
                                           nullptr, out, index_base, L, lane);
             }
             if (lane == 0) {
                 cntw[k] = c;
+                qfw[k] = qf;
                 if (lhits) lhits[r] = c;
             }
             wtot += c;
         }
-        if (lane == 0) {
-            s_wtot[wv] = wtot;
-            if (bad) atomicOr(&flags[1], 2u);
-        }
         n_recount += (uint32_t)(r_hi - r_lo) - nrec;
         n_record += nrec;
-        __syncthreads();
-        if (wv == 0) {
-            uint64_t agg = 0;
-            for (int k = 0; k < EW_WAVES; ++k) agg += s_wtot[k];
-            if (lane == 0)
-                __hip_atomic_store(&tile_state[t], SP_FLAG_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t ok = 1;
-            const uint64_t excl = wave_look_back(tile_state, t, lane, &ok);
-            if (lane == 0) {
-                __hip_atomic_store(&tile_state[t], SP_FLAG_PRE | (excl + agg), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                if (!ok) atomicOr(&flags[1], 1u);
-                if (excl + agg > cap_edges) {
-                    ok = 0;
-                    atomicOr(&flags[0], 1u);
-                }
-                if (t == ntiles - 1)
-                    __hip_atomic_store((uint64_t*)(flags + 2), excl + agg, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                s_prefix = excl;
-                s_ok = ok;
-            }
+        // this run's output offset
+        if (lane == 0) {
+            if (bad) atomicOr(&flags[1], 2u);
+            __hip_atomic_store(&tile_state[w], SP_FLAG_AGG | (uint64_t)wtot, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
-        __syncthreads();
-        if (s_ok) {
-            // phase B: recorded leaves straight from the hit list ...
-            uint64_t base = s_prefix;
-            for (int k = 0; k < wv; ++k) base += s_wtot[k];
-            uint32_t pos = (uint32_t)base;
-            uint32_t hread = 0;
-            for (int32_t r = r_lo; r < r_hi; ++r) {
-                const int k = r - r_lo;
-                const uint32_t c = cntw[k];
-                if (recorded & (1ull << k)) {
-                    emit_hits(hits + hread, c, pos, qsc, tsc, inv_r, out, index_base, lane);
-                    hread += c;
-                }
-                pos += c;
-            }
-            __builtin_amdgcn_wave_barrier();
-            // ... then the others re-swept; their stage lies over the hit list
-            pos = (uint32_t)base;
-            for (int32_t r = r_lo; r < r_hi; ++r) {
-                const int k = r - r_lo;
-                if (!(recorded & (1ull << k))) {
-                    const NodeRec qa = qleaves[r];
-                    const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
-                    const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
-                    uint32_t run = pos;
-                    for (int qb = 0; qb < qc; qb += OCT_WAVE) {
-                        const int nqb = min(OCT_WAVE, qc - qb);
-                        const uint32_t c = lane < nqb ? qcount[qf + qb + lane] : 0u;  // this lane's, phase A
-                        const uint32_t incl = (uint32_t)wave_incl_scan((int)c);
-                        if (lane < nqb) qoff[qf + qb + lane] = run + incl - c;
-                        run += (uint32_t)__builtin_amdgcn_readlane((int)incl, OCT_WAVE - 1);
-                    }
-                    const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
-                    const int2* lst = lists + (int64_t)r * cap;
-                    const int2 lf0 = lst[lane < ncand ? lane : 0];
-                    leaf_pass<EDGE_FILL>(qf, qc, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0, qcount, qoff,
-                                         out, index_base, L, lane);
-                }
-                pos += cntw[k];
-            }
+        uint32_t ok = 1;
+        const uint64_t excl = wave_look_back(tile_state, w, lane, &ok);
+        const uint64_t incl = excl + wtot;
+        if (lane == 0) {
+            __hip_atomic_store(&tile_state[w], SP_FLAG_PRE | incl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!ok) atomicOr(&flags[1], 1u);
+            if (incl > cap_edges) atomicOr(&flags[0], 1u);
+            if (w == nruns - 1)
+                __hip_atomic_store((uint64_t*)(flags + 2), incl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __syncthreads();  // the next ticket reuses the block's LDS
+        if (!ok || incl > cap_edges) continue;  // wave-uniform
+        __builtin_amdgcn_wave_barrier();
+        // phase B: recorded leaves straight from the hit list ...
+        uint32_t pos = (uint32_t)excl;
+        uint32_t hread = 0;
+        for (int32_t r = r_lo; r < r_hi; ++r) {
+            const int k = r - r_lo;
+            const uint32_t c = cntw[k];
+            if (recorded & (1ull << k)) {
+                emit_hits<WIDE>(WIDE ? hits + 2 * hread : hits + hread, c, pos, qfw[k], qsc, tsc, inv_r, out,
+                                index_base, lane);
+                hread += c;
+            }
+            pos += c;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // ... then the others re-swept; their stage lies over the hit list
+        pos = (uint32_t)excl;
+        for (int32_t r = r_lo; r < r_hi; ++r) {
+            const int k = r - r_lo;
+            if (!(recorded & (1ull << k))) {
+                const NodeRec qa = qleaves[r];
+                const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+                const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+                uint32_t run = pos;
+                for (int qb = 0; qb < qc; qb += OCT_WAVE) {
+                    const int nqb = min(OCT_WAVE, qc - qb);
+                    const uint32_t c = lane < nqb ? qcount[qf + qb + lane] : 0u;  // this lane's, phase A
+                    const uint32_t inc = (uint32_t)wave_incl_scan((int)c);
+                    if (lane < nqb) qoff[qf + qb + lane] = run + inc - c;
+                    run += (uint32_t)__builtin_amdgcn_readlane((int)inc, OCT_WAVE - 1);
+                }
+                const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+                const int2* lst = lists + (int64_t)r * cap;
+                const int2 lf0 = lst[lane < ncand ? lane : 0];
+                leaf_pass<EDGE_FILL>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0,
+                                     qcount, qoff,
+                                     out, index_base, L, lane);
+            }
+            pos += cntw[k];
+        }
+        __builtin_amdgcn_wave_barrier();  // the next run reuses the wave's LDS
     }
     // one pair of global atomics per block
-    if (threadIdx.x == 0) s_wtot[0] = s_wtot[1] = 0;
+    if (threadIdx.x == 0) s_stat[0] = s_stat[1] = 0;
     __syncthreads();
     if (lane == 0) {
-        atomicAdd(&s_wtot[0], n_recount);
-        atomicAdd(&s_wtot[1], n_record);
+        atomicAdd(&s_stat[0], n_recount);
+        atomicAdd(&s_stat[1], n_record);
     }
     __syncthreads();
-    if (threadIdx.x == 0 && (s_wtot[0] | s_wtot[1])) {
-        atomicAdd(&stats[0], s_wtot[0]);
-        atomicAdd(&stats[1], s_wtot[1]);
+    if (threadIdx.x == 0 && (s_stat[0] | s_stat[1])) {
+        atomicAdd(&stats[0], s_stat[0]);
+        atomicAdd(&stats[1], s_stat[1]);
     }
 }
 
@@ -718,8 +765,8 @@ hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const in
     return hipGetLastError();
 }
 
-// Upper bound of the tiles of any plan (every run holds at least one leaf).
-int64_t single_tiles(int64_t n_leaves) { return (n_leaves + EW_WAVES - 1) / EW_WAVES; }
+// Upper bound of the runs (output tiles) of any plan: every run holds a leaf.
+int64_t single_tiles(int64_t n_leaves) { return n_leaves + EW_WAVES; }
 
 // tile_state (single_tiles words of 8 bytes) and ticket must be zero.
 hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
@@ -727,13 +774,18 @@ hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2*
                                const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
                                uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint32_t* stats,
                                uint64_t cap_edges, EdgeOut out, int index_base, hipStream_t s) {
-    const int64_t tiles = wave_first ? single_tiles(q.n_leaves)
-                                     : (q.n_leaves + SP_TILE_LEAVES - 1) / SP_TILE_LEAVES;
-    if (tiles <= 0) return hipSuccess;
-    const unsigned g = (unsigned)std::min<int64_t>(tiles, 2048);  // blocks loop over tickets
-    hipLaunchKernelGGL(k_edges_single, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
-                       t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first, plan_waves,
-                       lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base);
+    const int64_t runs = wave_first ? q.n_leaves : (q.n_leaves + SP_LEAVES_PER_WAVE - 1) / SP_LEAVES_PER_WAVE;
+    const int64_t tickets = (runs + EW_WAVES - 1) / EW_WAVES;
+    if (tickets <= 0) return hipSuccess;
+    const unsigned g = (unsigned)std::min<int64_t>(tickets, 4096);  // blocks loop over tickets
+    if (t.n < ((int64_t)1 << HIT_POS_BITS))
+        hipLaunchKernelGGL(k_edges_single<false>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
+                           q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first,
+                           plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base);
+    else
+        hipLaunchKernelGGL(k_edges_single<true>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
+                           q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first,
+                           plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base);
     return hipGetLastError();
 }
 
