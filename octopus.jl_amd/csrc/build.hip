@@ -404,22 +404,44 @@ k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, uint16_t* __rest
         } else if (k + T >= n) {
             mw = -1;
         } else {
-            for (int j = 1; j <= T; ++j) mw = min(mw, (int)m[li + j]);
+            // eight independent LDS reads per step (the window never passes
+            // the staged span)
+            int j = 1;
+            for (; j + 7 <= T; j += 8) {
+                int v = m[li + j];
+#pragma unroll
+                for (int q = 1; q < 8; ++q) v = min(v, (int)m[li + j + q]);
+                mw = min(mw, v);
+            }
+            for (; j <= T; ++j) mw = min(mw, (int)m[li + j]);
         }
         lam = min(OCT_MAX_LEVEL, max(mk, mw) + 1);
         if (k == 0) {
             emit = true;
         } else if (lam > mk) {
-            // parent range (level mk, contains k) must hold more than T points
+            // parent range (level mk, contains k) must hold more than T points:
+            // a = distance back to its start, b = distance on to its end,
+            // both searched eight positions per step
             const int P = mk;
             int a = T + 1, b = T + 1;
-            for (int j = 1; j <= T; ++j) {
-                if (k - j < 0) break;
-                if ((int)m[li - j] < P || k - j == 0) { a = j; break; }
+            for (int j0 = 1; j0 <= T; j0 += 8) {
+                uint32_t hit = 0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int j = j0 + q;
+                    if (j <= T && k - j >= 0 && ((int)m[li - j] < P || k - j == 0)) hit |= 1u << q;
+                }
+                if (hit) { a = j0 + __builtin_ctz(hit); break; }
+                if (k - (j0 + 7) <= 0) break;  // reached position 0 without a start: a stays T + 1
             }
-            for (int j = 1; j <= T; ++j) {
-                if (k + j >= n) { b = j; break; }
-                if ((int)m[li + j] < P) { b = j; break; }
+            for (int j0 = 1; j0 <= T; j0 += 8) {
+                uint32_t hit = 0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int j = j0 + q;
+                    if (j <= T && (k + j >= n || (int)m[li + j] < P)) hit |= 1u << q;
+                }
+                if (hit) { b = j0 + __builtin_ctz(hit); break; }
             }
             emit = (a + b) > T;
         }
@@ -470,7 +492,13 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
             float* __restrict__ bmin, float* __restrict__ bmax, int32_t* __restrict__ parent,
             uint8_t* __restrict__ nlevel, int32_t* __restrict__ leaf_ids, NodeRec* __restrict__ leaves) {
     __shared__ uint32_t wc[TOPO_THREADS / OCT_WAVE][OCT_NLEVELS + 1];
-    const int64_t k = (int64_t)blockIdx.x * TOPO_THREADS + threadIdx.x;
+    __shared__ int8_t mm[TOPO_THREADS + TOPO_MAX_T + 1];  // m of positions base .. base + 256 + T
+    const int64_t base = (int64_t)blockIdx.x * TOPO_THREADS;
+    for (int i = threadIdx.x; i < TOPO_THREADS + T + 1; i += TOPO_THREADS) {
+        const int64_t j = base + i;
+        mm[i] = j < n ? (int8_t)((int)(topo[j] & 31) - 1) : (int8_t)-2;
+    }
+    const int64_t k = base + threadIdx.x;
     int mk = -1, lam = -1;
     bool emit = false;
     if (k < n) {
@@ -518,9 +546,19 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
         if (L == lam) {
             // leaf: the range ends at the first j > k with m_j < lam
             int64_t e = -1;
-            for (int j = 1; j <= T && k + j <= n; ++j) {
-                if (k + j == n) { e = n; break; }
-                if ((int)(topo[k + j] & 31) - 1 < lam) { e = k + j; break; }
+            const int li = threadIdx.x;
+            for (int j0 = 1; j0 <= T; j0 += 8) {
+                uint32_t hit = 0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int j = j0 + q;
+                    if (j <= T && (k + j >= n || (int)mm[li + j] < lam)) hit |= 1u << q;
+                }
+                if (hit) {
+                    e = k + j0 + __builtin_ctz(hit);
+                    if (e > n) e = n;
+                    break;
+                }
             }
             if (e < 0)  // level-21 leaf holding more than T equal keys
                 e = lower_bound_pred(k + T + 1, n, [&](int64_t j) { return sk[j] != key; });
@@ -528,13 +566,18 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
             nlast[id] = (int32_t)(e - 1) + index_base;
             uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
             uint32_t hi3[3] = {0u, 0u, 0u};
-            for (int64_t j = k; j < e; ++j) {
-                const float4 p = scoord[j];
-                const float v[3] = {p.x, p.y, p.z};
-                for (int d = 0; d < 3; ++d) {
-                    const uint32_t o = f2o(v[d]);
-                    lo3[d] = o < lo3[d] ? o : lo3[d];
-                    hi3[d] = o > hi3[d] ? o : hi3[d];
+            for (int64_t j0 = k; j0 < e; j0 += 4) {
+                float4 pp[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) pp[q] = scoord[j0 + q < e ? j0 + q : k];  // four loads in flight
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float v[3] = {pp[q].x, pp[q].y, pp[q].z};
+                    for (int d = 0; d < 3; ++d) {
+                        const uint32_t o = f2o(v[d]);
+                        lo3[d] = o < lo3[d] ? o : lo3[d];
+                        hi3[d] = o > hi3[d] ? o : hi3[d];
+                    }
                 }
             }
             rec.lo = make_float4(o2f(lo3[0]), o2f(lo3[1]), o2f(lo3[2]), __int_as_float((int)k));

@@ -400,31 +400,46 @@ __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, float4 qlo,
 
 // Writes n recorded hits of the leaf whose first query is qf at out[base ..):
 // every lane takes consecutive hits, so the stores of one instruction are
-// contiguous (the displacement as one 12-byte store per edge).
+// contiguous (the displacement as one 12-byte store per edge).  EMIT_BATCH
+// hits per lane are loaded before any is written, so their gathers overlap.
+constexpr int EMIT_BATCH = 4;
+
 template <bool WIDE>
 __device__ inline void emit_hits(const uint32_t* hits, uint32_t n, uint32_t base, int32_t qf,
                                  const float4* __restrict__ qsc, const float4* __restrict__ tsc, float inv_r,
                                  EdgeOut out, int index_base, int lane) {
-    for (uint32_t t = lane; t < n; t += OCT_WAVE) {
-        int32_t qp, tp;
-        if (WIDE) {
-            const int2 h = reinterpret_cast<const int2*>(hits)[t];
-            qp = h.x;
-            tp = h.y;
-        } else {
-            const uint32_t h = hits[t];
-            qp = qf + (int32_t)(h >> HIT_POS_BITS);
-            tp = (int32_t)(h & HIT_POS_MASK);
+    for (uint32_t t0 = 0; t0 < n; t0 += EMIT_BATCH * OCT_WAVE) {
+        float4 q[EMIT_BATCH], p[EMIT_BATCH];
+#pragma unroll
+        for (int k = 0; k < EMIT_BATCH; ++k) {
+            const uint32_t t = t0 + (uint32_t)(k * OCT_WAVE + lane);
+            int32_t qp = qf, tp = 0;
+            if (t < n) {
+                if (WIDE) {
+                    const int2 h = reinterpret_cast<const int2*>(hits)[t];
+                    qp = h.x;
+                    tp = h.y;
+                } else {
+                    const uint32_t h = hits[t];
+                    qp = qf + (int32_t)(h >> HIT_POS_BITS);
+                    tp = (int32_t)(h & HIT_POS_MASK);
+                }
+            }
+            q[k] = qsc[qp];
+            p[k] = tsc[tp];
         }
-        const float4 q = qsc[qp];
-        const float4 p = tsc[tp];
-        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        const uint32_t e = base + t;
-        out.senders[e] = __float_as_int(p.w) + index_base;
-        out.receivers[e] = __float_as_int(q.w) + index_base;
-        *reinterpret_cast<float3*>(out.rdisp + 3 * (int64_t)e) = make_float3(dx * inv_r, dy * inv_r, dz * inv_r);
-        out.rdist[e] = sqrtf(d2) * inv_r;
+#pragma unroll
+        for (int k = 0; k < EMIT_BATCH; ++k) {
+            const uint32_t t = t0 + (uint32_t)(k * OCT_WAVE + lane);
+            if (t >= n) break;
+            const float dx = q[k].x - p[k].x, dy = q[k].y - p[k].y, dz = q[k].z - p[k].z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            const uint32_t e = base + t;
+            out.senders[e] = __float_as_int(p[k].w) + index_base;
+            out.receivers[e] = __float_as_int(q[k].w) + index_base;
+            *reinterpret_cast<float3*>(out.rdisp + 3 * (int64_t)e) = make_float3(dx * inv_r, dy * inv_r, dz * inv_r);
+            out.rdist[e] = sqrtf(d2) * inv_r;
+        }
     }
 }
 
