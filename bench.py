@@ -4,9 +4,10 @@ Workload (one step = one pass of the hot path over one batch): the 3D SPH
 block, 1,000,000 FP32 points on a 100^3 lattice (h = 0.01, U(+-0.1h) jitter),
 radius 2h; every step adds N(0, (0.01h)^2) per coordinate (the snapshots are
 generated on the device before the timed region), then update_point_set +
-run! + build_edges on the self pair.  The metric is edges/s of build_edges
-with a bit-exact edge set; the line reports the exact two-pass path and, when
-available, the learned single pass.
+run! (refit mode: the almost-sorted path of time stepping) + build_edges on
+the self pair.  The metric is edges/s of build_edges with a bit-exact edge
+set, learned single pass (the headline `value`) against the exact two-pass
+path, both measured in the same process on the same snapshots.
 
 Prints ONE JSON line.
 """
@@ -21,6 +22,10 @@ sys.path.insert(0, os.path.join(ROOT, "octopus.jl_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "edges/s, learned vs exact-count build_edges, 1M pts 3D, bit-exact edges"
+# HBM bytes per launch of the dominant kernels, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a 512 MiB copy (the file records the commands)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_sph1m.json")
 
 
 def parse():
@@ -28,8 +33,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="exact", choices=["exact", "learned"])
-    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU oracle baseline (0 = skip)")
+    ap.add_argument("--mode", default="both", choices=["both", "exact", "learned"],
+                    help="both: learned (headline value) and exact, in that order")
+    ap.add_argument("--cpu-steps", type=int, default=30, help="steps of the CPU oracle baseline (0 = skip)")
     ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M points)")
     ap.add_argument("--no-profile", action="store_true", help="time without the per-phase HIP events")
     ap.add_argument("--null-stream", action="store_true", help="run the library on torch's current stream")
@@ -45,16 +51,83 @@ def parse():
     return ap.parse_args()
 
 
+def run_mode(a, mode, snaps, r):
+    """Times a.steps steps of one build_edges mode on a fresh TNS."""
+    import torch
+
+    import octohip
+
+    tns = octohip.TNS(private_stream=not a.null_stream)
+    octohip.set_search_radius(tns, float(r))
+    octohip.set_refit_mode(tns, not a.no_refit)
+    sid = octohip.add_point_set(tns, snaps[0])
+    octohip.set_active_search(tns, sid, sid)
+    predictor = None
+    if mode == "learned" and a.predictor != "none":
+        if os.path.exists(os.path.join(octohip.predictor.SHIPPED, a.predictor + ".f32")):
+            octohip.predictor.load(tns, a.predictor)
+            predictor = a.predictor
+            if a.predict_always:
+                octohip.predictor.set_policy(tns, True)
+    used = [0]
+    refits = [0]
+
+    def step(k):
+        octohip.update_point_set(tns, sid, snaps[k])
+        octohip.run(tns)
+        refits[0] += int(octohip.build_info(tns, sid)["refit_used"])
+        e = octohip.build_edges(tns, sid, sid, mode=mode)
+        if predictor:
+            used[0] += int(octohip.predictor.edge_stats(tns, sid, sid)["used_predictor"])
+        return e.senders.shape[0]
+
+    step(0)
+    first_call = octohip.predictor.edge_stats(tns, sid, sid)
+    for k in range(1, a.warmup):
+        step(k)
+    used[0] = 0
+    refits[0] = 0
+    torch.cuda.synchronize()
+    if not a.no_profile:
+        octohip.profile(tns, True)
+        octohip.profile_read(tns, reset=True)
+    edges = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.warmup, a.warmup + a.steps):
+        edges += step(k)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = octohip.profile_read(tns, reset=True)
+    octohip.profile(tns, False)
+    st = octohip.predictor.edge_stats(tns, sid, sid)
+    sec = t1 - t0
+    res = {
+        "mode": mode,
+        "value": edges / sec,
+        "ms_per_step": 1e3 * sec / a.steps,
+        "edges_per_step": edges / a.steps,
+        "breakdown_ms_per_step": {k: v[0] / a.steps for k, v in prof.items() if v[1]},
+        "kernel_calls": {k: v[1] for k, v in prof.items() if v[1]},
+        "predictor": predictor,
+        "first_call": {k: first_call[k] for k in ("single_pass", "used_predictor", "attempts")},
+        "timed_steps_using_predictor": used[0],
+        "timed_steps_refit": refits[0],
+        "last_step": {k: st[k] for k in ("single_pass", "used_predictor", "attempts", "recounted_leaves",
+                                         "recorded_leaves", "predicted_total", "n_edges", "n_leaves")},
+    }
+    del tns
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": a.gpus}))
         sys.exit(2)
-    import numpy as np
     import torch
 
-    import octohip
     from octohip import datasets
 
     if a.calibrate_copy > 0:
@@ -80,63 +153,25 @@ def main():
     snaps = snaps[1:]
     torch.cuda.synchronize()
 
-    tns = octohip.TNS(private_stream=not a.null_stream)
-    octohip.set_search_radius(tns, float(r))
-    octohip.set_refit_mode(tns, not a.no_refit)
-    sid = octohip.add_point_set(tns, snaps[0])
-    octohip.set_active_search(tns, sid, sid)
-    predictor = None
-    if a.mode == "learned" and a.predictor != "none":
-        path = os.path.join(octohip.predictor.SHIPPED, a.predictor + ".f32")
-        if os.path.exists(path):
-            octohip.predictor.load(tns, a.predictor)
-            predictor = a.predictor
+    modes = ["learned", "exact"] if a.mode == "both" else [a.mode]
+    results = {m: run_mode(a, m, snaps, r) for m in modes}
+    head = results[modes[0]]
 
-    if predictor and a.predict_always:
-        octohip.predictor.set_policy(tns, True)
-    used = [0]
-    refits = [0]
-
-    def step(k):
-        octohip.update_point_set(tns, sid, snaps[k])
-        octohip.run(tns)
-        refits[0] += int(octohip.build_info(tns, sid)["refit_used"])
-        e = octohip.build_edges(tns, sid, sid, mode=a.mode)
-        if predictor:
-            used[0] += int(octohip.predictor.edge_stats(tns, sid, sid)["used_predictor"])
-        return e.senders.shape[0]
-
-    for k in range(a.warmup):
-        step(k)
-    used[0] = 0
-    refits[0] = 0
-    torch.cuda.synchronize()
-    if not a.no_profile:
-        octohip.profile(tns, True)
-        octohip.profile_read(tns, reset=True)
-    edges = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(a.warmup, total_steps):
-        edges += step(k)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = octohip.profile_read(tns, reset=True)
-    octohip.profile(tns, False)
-    st = octohip.predictor.edge_stats(tns, sid, sid)
-    sec = t1 - t0
-    ms_step = 1e3 * sec / a.steps
-    e_avg = edges / a.steps
-
-    # dominant kernel: edge emission (fill) in exact mode, the single pass in learned mode
-    kname = "fill" if a.mode == "exact" else "single"
-    kms, kcalls = prof[kname]
+    # dominant kernel: the single pass (learned) or the fill (exact)
+    kname = "single" if head["mode"] == "learned" else "fill"
+    kms = head["breakdown_ms_per_step"].get(kname, 0.0) * a.steps
+    kcalls = head["kernel_calls"].get(kname, 0)
     k_avg_s = (kms / max(kcalls, 1)) / 1e3
+    e_avg = head["edges_per_step"]
     alg_bytes = 24.0 * e_avg + 12.0 * n  # 24 B per emitted edge + 12 B per query point read
     achieved = alg_bytes / k_avg_s / 1e9 if k_avg_s > 0 else 0.0
-    run_ms = (prof["bounds"][0] + prof["build"][0]) / a.steps
-    edges_ms = sum(prof[p][0] for p in ("leaf_lists", "count", "scan", "fill", "single", "predict")) / a.steps
-    step_bytes = 16.0 * n + 24.0 * e_avg + 12.0 * n  # coords + perm write, edge output, query reads (tree bytes excluded)
+    kernel_symbol = "k_edges_single" if kname == "single" else "k_edges<EDGE_FILL>"
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        tr = json.load(open(TRAFFIC_FILE))
+        if kernel_symbol in tr.get("kernels", {}):
+            traffic = tr["kernels"][kernel_symbol]["bytes_per_launch"]
+    step_bytes = 16.0 * n + 24.0 * e_avg + 12.0 * n  # coords + perm write, edge output, query reads
 
     cpu = None
     if a.cpu_steps > 0:
@@ -144,48 +179,45 @@ def main():
 
         oracle.build()
         threads = oracle.max_threads()
-        host = [s.cpu().numpy() for s in snaps[a.warmup:a.warmup + a.cpu_steps]]
+        host = [s.cpu().numpy() for s in snaps]
         ce = 0
         c0 = time.perf_counter()
-        for hp in host:
+        for k in range(a.cpu_steps):
+            hp = host[k % len(host)]
             t = oracle.run(hp, float(r))
             ce += oracle.build_edges(t, hp).n_edges
         c1 = time.perf_counter()
         cpu = {"value": ce / (c1 - c0), "unit": "edges/s", "cores": threads, "kind": "port",
-               "sample": f"{len(host)} steps of the same 1M-point SPH workload (run! + build_edges, "
-                         f"C oracle, OpenMP {threads} threads), {c1 - c0:.1f} s"}
+               "sample": f"{a.cpu_steps} steps of the same 1M-point SPH workload (run! + build_edges, C oracle, "
+                         f"OpenMP {threads} threads), {c1 - c0:.1f} s"}
 
     out = {
-        "metric": "edges/s, learned vs exact-count build_edges, 1M pts 3D, bit-exact edges",
-        "value": edges / sec,
+        "metric": METRIC,
+        "value": head["value"],
         "unit": "edges/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"SPH block: {n} FP32 points on a {a.side}^3 lattice, h=0.01, jitter 0.1h, "
-                               f"radius 2h, N(0,(0.01h)^2) step displacement; update + run! + build_edges per step",
-                   "mode": a.mode, "points": n, "edges_per_step": e_avg, "radius": float(r),
-                   "predictor": predictor, "predict_always": bool(a.predict_always),
-                   "timed_steps_using_predictor": used[0], "refit_mode": not a.no_refit,
-                   "timed_steps_refit": refits[0]},
-        "last_step": {k: st[k] for k in ("single_pass", "used_predictor", "attempts", "recounted_leaves",
-                                         "recorded_leaves", "predicted_total", "n_edges", "n_leaves")},
-        "breakdown_ms_per_step": {"run": run_ms, "build_edges": edges_ms,
-                                  **{k: v[0] / a.steps for k, v in prof.items() if v[1]}},
-        "points_per_s": n * a.steps / sec,
-        "hbm_fraction_step": (step_bytes / (ms_step / 1e3)) / 1e9 / HBM_PEAK_GBS,
+                               f"radius 2h, N(0,(0.01h)^2) step displacement; update + run! (refit mode) + "
+                               f"build_edges per step",
+                   "mode": head["mode"], "points": n, "edges_per_step": e_avg, "radius": float(r)},
+        "modes": results,
+        "points_per_s": n / (head["ms_per_step"] / 1e3),
+        "hbm_fraction_step": (step_bytes / (head["ms_per_step"] / 1e3)) / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_edges<EDGE_FILL>" if a.mode == "exact" else "k_edges_single",
-                     "kernel_avg_ms": k_avg_s * 1e3, "alg_bytes_per_launch": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kernel_symbol, "kernel_avg_ms": k_avg_s * 1e3, "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
+    if "learned" in results and "exact" in results:
+        out["learned_over_exact"] = results["learned"]["value"] / results["exact"]["value"]
     print(json.dumps(out))
 
 
