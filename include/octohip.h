@@ -232,6 +232,36 @@ typedef struct oct_edge_stats {
 } oct_edge_stats;
 int oct_edge_stats_get(oct_ctx* ctx, int q, int t, oct_edge_stats* out);
 
+/* ---- Batched sets ---------------------------------------------------------
+ * No counterpart in the reference, which runs the sets of a TNS one after
+ * the other (src/api.jl:196).  A batch holds many point sets, each searched
+ * against itself with its own radius: coords is one device array with the
+ * sets concatenated (set s = points offsets[s] .. offsets[s+1], host array of
+ * n_sets + 1), radii a host array of n_sets.  Every stage of run! and the
+ * single-pass build_edges is one launch for all sets (the sets form one
+ * forest); the first oct_batch_run of a configuration runs eagerly and
+ * captures the step as a HIP graph, later calls replay it (after updating the
+ * coordinates in place; a replay that would overflow a buffer falls back to
+ * an eager step).  Each set needs at most 2^14 cells per axis (extent / r).
+ * Edge indices are local to their set (plus index_base). */
+typedef struct oct_batch_edges {
+    int32_t* senders;
+    int32_t* receivers;
+    float* rel_displacement;      /* 3 * n_edges */
+    float* rel_dist_norm;
+    int64_t n_edges;
+    const int64_t* set_offsets;   /* device, n_sets + 1: set s owns edges [off[s], off[s+1]) */
+    int32_t n_sets;
+    int32_t replayed;             /* 1: this call replayed the captured graph */
+} oct_batch_edges;
+int oct_batch_create(oct_ctx* ctx, int n_sets, const float* coords, const int64_t* offsets, const float* radii,
+                     int* batch_id);
+int oct_batch_destroy(oct_ctx* ctx, int batch_id);
+/* Leaf size (1..256), leaves per single-pass run (1..64) and graph replay on
+ * or off; a change drops the captured graph. */
+int oct_batch_config(oct_ctx* ctx, int batch_id, int leaf_size, int run_leaves, int use_graph);
+int oct_batch_run(oct_ctx* ctx, int batch_id, oct_batch_edges* out);
+
 /* The network on n rows of features (device arrays): y = log1p(hits). */
 int oct_mlp_forward(oct_ctx* ctx, const float* params, const float* features, int64_t n, float* y);
 /* Gradient of mean((f(x) - target)^2) over n rows with respect to the

@@ -26,78 +26,15 @@ namespace oct {
 
 // =============================================================== bounds
 
-constexpr int BND_THREADS = 256;
-
-// Per-block ordered-uint min/max of the coordinates.
+// Per-block partial min/max of one set's coordinates.
 __global__ void __launch_bounds__(BND_THREADS)
 k_bounds_partial(const float* __restrict__ coords, int64_t n, uint32_t* __restrict__ partials) {
-    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
-    uint32_t mx[3] = {0u, 0u, 0u};
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        for (int d = 0; d < 3; ++d) {
-            const uint32_t o = f2o(coords[3 * i + d]);
-            mn[d] = o < mn[d] ? o : mn[d];
-            mx[d] = o > mx[d] ? o : mx[d];
-        }
-    }
-    __shared__ uint32_t red[6][BND_THREADS / OCT_WAVE];
-    const int w = threadIdx.x / OCT_WAVE;
-    for (int d = 0; d < 3; ++d) {
-        const uint32_t a = wave_min_u(mn[d]);
-        const uint32_t b = wave_max_u(mx[d]);
-        if (lane_id() == 0) { red[d][w] = a; red[3 + d][w] = b; }
-    }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        const int d = threadIdx.x;
-        uint32_t v = red[d][0];
-        for (int k = 1; k < BND_THREADS / OCT_WAVE; ++k)
-            v = d < 3 ? (red[d][k] < v ? red[d][k] : v) : (red[d][k] > v ? red[d][k] : v);
-        partials[6 * blockIdx.x + d] = v;
-    }
+    bounds_part(coords, 0, n, blockIdx.x, gridDim.x, partials + 6 * blockIdx.x);
 }
 
-// One block folds the partials and derives origin, 1/r and the live key bits.
 __global__ void __launch_bounds__(BND_THREADS)
 k_bounds_final(const uint32_t* __restrict__ partials, int nblocks, float radius, BuildParams* __restrict__ prm) {
-    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
-    uint32_t mx[3] = {0u, 0u, 0u};
-    for (int b = threadIdx.x; b < nblocks; b += BND_THREADS)
-        for (int d = 0; d < 3; ++d) {
-            const uint32_t a = partials[6 * b + d], c = partials[6 * b + 3 + d];
-            mn[d] = a < mn[d] ? a : mn[d];
-            mx[d] = c > mx[d] ? c : mx[d];
-        }
-    __shared__ uint32_t red[6][BND_THREADS / OCT_WAVE];
-    const int w = threadIdx.x / OCT_WAVE;
-    for (int d = 0; d < 3; ++d) {
-        const uint32_t a = wave_min_u(mn[d]);
-        const uint32_t b = wave_max_u(mx[d]);
-        if (lane_id() == 0) { red[d][w] = a; red[3 + d][w] = b; }
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    uint32_t v[6];
-    for (int d = 0; d < 6; ++d) v[d] = red[d][0];
-    for (int k = 1; k < BND_THREADS / OCT_WAVE; ++k)
-        for (int d = 0; d < 3; ++d) {
-            v[d] = red[d][k] < v[d] ? red[d][k] : v[d];
-            v[3 + d] = red[3 + d][k] > v[3 + d] ? red[3 + d][k] : v[3 + d];
-        }
-    const float inv_cs = 1.0f / radius;  // one(T) / cell_size
-    uint32_t cmax = 0;
-    for (int d = 0; d < 3; ++d) {
-        prm->omin[d] = v[d];
-        prm->omax[d] = v[3 + d];
-        const float o = o2f(v[d]) - 1e-6f;  // ox - T(1e-6)
-        prm->origin[d] = o;
-        const uint32_t c = floor_clamp((o2f(v[3 + d]) - o) * inv_cs, 0x1fffffu);
-        cmax = c > cmax ? c : cmax;
-    }
-    prm->inv_cs = inv_cs;
-    prm->cmax = cmax;
-    prm->key_bits = cmax == 0 ? 0 : 3 * (32 - __clz((int)cmax));
+    bounds_fold(partials, 0, nblocks, radius, prm);
 }
 
 // ================================================================ keys
@@ -374,7 +311,7 @@ constexpr int TOPO_MAX_T = 256;  // largest leaf size the window supports
 // whether nodes start here; per-block counts of nodes per level (rows
 // 0..21) and of leaves (row 22).
 __global__ void __launch_bounds__(TOPO_THREADS)
-k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, uint16_t* __restrict__ topo,
+k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, int batched, uint16_t* __restrict__ topo,
              uint32_t* __restrict__ hist, int nblk) {
     __shared__ int8_t m[TOPO_THREADS + 2 * TOPO_MAX_T + 1];
     __shared__ uint32_t wc[TOPO_THREADS / OCT_WAVE][OCT_NLEVELS + 1];
@@ -384,7 +321,7 @@ k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, uint16_t* __rest
     for (int i = threadIdx.x; i < span; i += TOPO_THREADS) {
         const int64_t j = lo + i;
         int8_t v = -1;
-        if (j > 0 && j < n) v = (int8_t)shared_levels(sk[j - 1], sk[j]);
+        if (j > 0 && j < n) v = (int8_t)key_shared(sk[j - 1], sk[j], batched);  // -1 where a set starts
         else if (j >= n) v = -2;  // past the end: never read as a real value
         m[i] = v;
     }
@@ -416,7 +353,7 @@ k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, uint16_t* __rest
             for (; j <= T; ++j) mw = min(mw, (int)m[li + j]);
         }
         lam = min(OCT_MAX_LEVEL, max(mk, mw) + 1);
-        if (k == 0) {
+        if (mk < 0) {  // position 0 or the first point of a set: its root chain starts here
             emit = true;
         } else if (lam > mk) {
             // parent range (level mk, contains k) must hold more than T points:
@@ -487,7 +424,7 @@ __device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
 __global__ void __launch_bounds__(TOPO_THREADS)
 k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, int64_t n, int T,
             const uint16_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
-            BuildParams* __restrict__ prm, int index_base, NodeRec* __restrict__ nodes,
+            BuildParams* __restrict__ prm, Forest forest, int index_base, NodeRec* __restrict__ nodes,
             int32_t* __restrict__ nfirst, int32_t* __restrict__ nlast, int32_t* __restrict__ nchild,
             float* __restrict__ bmin, float* __restrict__ bmax, int32_t* __restrict__ parent,
             uint8_t* __restrict__ nlevel, int32_t* __restrict__ leaf_ids, NodeRec* __restrict__ leaves) {
@@ -518,11 +455,16 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
     }
     __syncthreads();
     const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
-    if (k == 0) {
-        prm->n_nodes = M;
-        prm->chain_level = shared_levels(sk[0], sk[n - 1]);
-    }
+    if (k == 0) prm->n_nodes = M;
     if (!emit) return;
+    // the set of this position, its point range and chain depth (levels
+    // 0..L0 hold all of the set's points)
+    const uint64_t key = sk[k];
+    const int set = key_set(key, forest.batched);
+    const int64_t s_lo = forest.set_off[set], s_hi = forest.set_off[set + 1];
+    BuildParams* sp = forest.sprm + set;
+    const int L0 = key_shared(sk[s_lo], sk[s_hi - 1], forest.batched);
+    if (k == s_lo) sp->chain_level = L0;
     auto wave_off = [&](int L) {
         uint32_t s = 0;
         for (int q = 0; q < w; ++q) s += wc[q][L];
@@ -532,16 +474,14 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
     auto before = [&](int L) -> int32_t {
         return (int32_t)(hoff[(int64_t)L * nblk + blockIdx.x] + wave_off(L) + rank_in_wave[L]);
     };
-    const int L0 = shared_levels(sk[0], sk[n - 1]);  // levels 0..L0 hold all n points
-    const uint64_t key = sk[k];
-    const int Lfirst = mk + 1;  // 0 at k == 0
+    const int Lfirst = mk + 1;  // 0 where a set starts
     for (int L = Lfirst; L <= lam; ++L) {
         const int32_t id = before(L);
         nlevel[id] = (uint8_t)L;
         nfirst[id] = (int32_t)k + index_base;
         for (int o = 0; o < 8; ++o) nchild[8 * (int64_t)id + o] = -1;
         parent[id] = L == 0 ? -1 : (L > Lfirst ? before(L - 1) : before(L - 1) - 1);
-        const bool chain = (k == 0 && L <= L0);
+        const bool chain = (k == s_lo && L <= L0);
         NodeRec rec;
         if (L == lam) {
             // leaf: the range ends at the first j > k with m_j < lam
@@ -585,15 +525,15 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
             const int32_t lr = before(OCT_NLEVELS) - M;
             leaf_ids[lr] = id;
             leaves[lr] = rec;
-            if (chain) prm->branch_root = id;  // everything sits in one leaf
+            if (chain) sp->branch_root = id;  // the whole set sits in one leaf
         } else {
             rec.lo = make_float4(0.f, 0.f, 0.f, __int_as_float(before(L + 1)));  // first child
             rec.hi = make_float4(0.f, 0.f, 0.f, __int_as_float(0));             // -children, from k_link
             if (chain) {
-                rec.lo.x = o2f(prm->omin[0]); rec.lo.y = o2f(prm->omin[1]); rec.lo.z = o2f(prm->omin[2]);
-                rec.hi.x = o2f(prm->omax[0]); rec.hi.y = o2f(prm->omax[1]); rec.hi.z = o2f(prm->omax[2]);
-                nlast[id] = (int32_t)(n - 1) + index_base;
-                if (L == L0) prm->branch_root = id;
+                rec.lo.x = o2f(sp->omin[0]); rec.lo.y = o2f(sp->omin[1]); rec.lo.z = o2f(sp->omin[2]);
+                rec.hi.x = o2f(sp->omax[0]); rec.hi.y = o2f(sp->omax[1]); rec.hi.z = o2f(sp->omax[2]);
+                nlast[id] = (int32_t)(s_hi - 1) + index_base;
+                if (L == L0) sp->branch_root = id;
             }
         }
         nodes[id] = rec;
@@ -607,15 +547,16 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
 // Every non-root node writes its id into its parent's octant slot and counts
 // itself into the parent's record (b = -children).
 __global__ void __launch_bounds__(256)
-k_link(const uint64_t* __restrict__ sk, const BuildParams* __restrict__ prm, int index_base,
+k_link(const uint64_t* __restrict__ sk, const BuildParams* __restrict__ prm, int batched, int index_base,
        const int32_t* __restrict__ parent, const int32_t* __restrict__ nfirst, const uint8_t* __restrict__ nlevel,
        int32_t* __restrict__ nchild, NodeRec* __restrict__ nodes) {
     const int32_t M = prm->n_nodes;
     for (int32_t id = 1 + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); id < M;
          id += (int32_t)(gridDim.x * blockDim.x)) {
         const int32_t p = parent[id];
+        if (p < 0) continue;  // the root of another set
         const int L = nlevel[id];
-        const uint64_t key = sk[nfirst[id] - index_base];
+        const uint64_t key = key_morton(sk[nfirst[id] - index_base], batched);
         const int o = (int)((key >> level_shift(L - 1)) & 7ull);
         nchild[8 * (int64_t)p + o] = id + index_base;
         atomicSub((int*)&nodes[p].hi.w, 1);
@@ -626,7 +567,7 @@ k_link(const uint64_t* __restrict__ sk, const BuildParams* __restrict__ prm, int
 // children (finished by the previous launch: launches go deepest first).
 __global__ void __launch_bounds__(256)
 k_level_refit(int L, const uint32_t* __restrict__ hoff, int nblk, const BuildParams* __restrict__ prm,
-              int index_base, NodeRec* __restrict__ nodes, int32_t* __restrict__ nlast,
+              int skip_chain, int index_base, NodeRec* __restrict__ nodes, int32_t* __restrict__ nlast,
               const int32_t* __restrict__ nfirst, float* __restrict__ bmin, float* __restrict__ bmax) {
     const int32_t lo = (int32_t)hoff[(int64_t)L * nblk];
     const int32_t hi = (int32_t)hoff[(int64_t)(L + 1) * nblk];
@@ -636,7 +577,7 @@ k_level_refit(int L, const uint32_t* __restrict__ hoff, int nblk, const BuildPar
         NodeRec rec = nodes[id];
         const int nc = -rec_b(rec);
         if (nc <= 0) continue;                                  // leaf
-        if (L <= L0 && nfirst[id] == index_base) continue;     // chain: global bounds already
+        if (skip_chain && L <= L0 && nfirst[id] == index_base) continue;  // chain: set bounds already
         const int c0 = rec_a(rec);
         uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
         uint32_t hi3[3] = {0u, 0u, 0u};
@@ -690,6 +631,27 @@ hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, con
     return hipGetLastError();
 }
 
+// Radix passes over keys[0] / vals[0] at the given digit shifts (9 bits
+// each); *cur = the keys buffer holding the sorted keys.  The last pass
+// writes perm (+ index base) and the sorted float4 coordinates.
+hipError_t launch_sort_passes(const float* coords, int64_t n, const int* shifts, int npasses, int index_base,
+                              const BuildBufs& b, int* cur_out, hipStream_t s) {
+    const int ntiles = (int)sort_tiles(n);
+    int cur = 0;
+    for (int p = 0; p < npasses; ++p) {
+        const int shift = shifts[p];
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, s, b.keys[cur], n, shift, b.hist, ntiles);
+        hipError_t e = exclusive_scan(b.hist, b.hist, (int64_t)RS_BUCKETS * ntiles, b.scan_tmp, nullptr, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, s, b.keys[cur], b.vals[cur],
+                           b.keys[cur ^ 1], b.vals[cur ^ 1], n, shift, b.hist, ntiles, (int)(p == npasses - 1),
+                           b.perm, b.scoord, coords, index_base);
+        cur ^= 1;
+    }
+    *cur_out = cur;
+    return hipGetLastError();
+}
+
 // Stage 2a: keys and the full stable sort over the live bits; *cur = the
 // keys buffer that ends up holding the sorted keys.
 hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int index_base, const BuildBufs& b,
@@ -698,20 +660,9 @@ hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int i
     hipLaunchKernelGGL(k_keys, dim3(g), dim3(256), 0, s, coords, n, b.prm, b.keys[0], b.vals[0]);
     int passes = (key_bits + RS_BITS - 1) / RS_BITS;
     if (passes < 1) passes = 1;
-    const int ntiles = (int)sort_tiles(n);
-    int cur = 0;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = p * RS_BITS;
-        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, s, b.keys[cur], n, shift, b.hist, ntiles);
-        hipError_t e = exclusive_scan(b.hist, b.hist, (int64_t)RS_BUCKETS * ntiles, b.scan_tmp, nullptr, s);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, s, b.keys[cur], b.vals[cur],
-                           b.keys[cur ^ 1], b.vals[cur ^ 1], n, shift, b.hist, ntiles, (int)(p == passes - 1),
-                           b.perm, b.scoord, coords, index_base);
-        cur ^= 1;
-    }
-    *cur_out = cur;
-    return hipGetLastError();
+    int shifts[8];
+    for (int p = 0; p < passes; ++p) shifts[p] = p * RS_BITS;
+    return launch_sort_passes(coords, n, shifts, passes, index_base, b, cur_out, s);
 }
 
 // Stage 2a' (almost-sorted path), part 1: new keys in the previous order
@@ -742,26 +693,41 @@ hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t n_changed
     return hipGetLastError();
 }
 
-// Stage 2b: topology and bounds from the sorted keys in keys[cur].
+// Stage 2b: topology and bounds from the sorted keys in keys[cur].  A forest
+// (batched sets) refits every level; a single set skips its top chain, whose
+// nodes already hold the set's bounds.
 hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base, int cur, const BuildBufs& b,
-                                 hipStream_t s) {
+                                 const Forest& f, hipStream_t s) {
     const uint64_t* sk = b.keys[cur];
     const int nblk = (int)topo_blocks(n);
-    hipLaunchKernelGGL(k_topo_count, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, n, T, b.topo, b.hist, nblk);
+    hipLaunchKernelGGL(k_topo_count, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, n, T, f.batched, b.topo, b.hist,
+                       nblk);
     hipError_t e = exclusive_scan(b.hist, b.hist, (int64_t)(OCT_NLEVELS + 1) * nblk, b.scan_tmp, b.total, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, b.scoord, n, T, b.topo, b.hist,
-                       nblk, b.prm, index_base, b.nodes, b.nfirst, b.nlast, b.nchild, b.bmin, b.bmax,
+                       nblk, b.prm, f, index_base, b.nodes, b.nfirst, b.nlast, b.nchild, b.bmin, b.bmax,
                        b.parent, b.nlevel, b.leaf_ids, b.leaves);
     hipLaunchKernelGGL(k_set_leaf_count, dim3(1), dim3(64), 0, s, b.prm, b.total);
     const unsigned gl = (unsigned)std::min<int64_t>((max_nodes(n, T) + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_link, dim3(gl), dim3(256), 0, s, sk, b.prm, index_base, b.parent, b.nfirst, b.nlevel,
-                       b.nchild, b.nodes);
-    // internal levels lie strictly below the chain; the chain ends at or below 21 - key_bits / 3
-    const int lmin = 21 - key_bits / 3;
+    hipLaunchKernelGGL(k_link, dim3(gl), dim3(256), 0, s, sk, b.prm, f.batched, index_base, b.parent, b.nfirst,
+                       b.nlevel, b.nchild, b.nodes);
+    // internal levels lie strictly below a set's chain; the chain ends at or below 21 - key_bits / 3
+    const int lmin = f.batched ? 0 : 21 - key_bits / 3;
     for (int L = OCT_MAX_LEVEL - 1; L >= (lmin < 0 ? 0 : lmin); --L)
-        hipLaunchKernelGGL(k_level_refit, dim3(256), dim3(256), 0, s, L, b.hist, nblk, b.prm, index_base, b.nodes,
-                           b.nlast, b.nfirst, b.bmin, b.bmax);
+        hipLaunchKernelGGL(k_level_refit, dim3(256), dim3(256), 0, s, L, b.hist, nblk, b.prm, f.batched ? 0 : 1,
+                           index_base, b.nodes, b.nlast, b.nfirst, b.bmin, b.bmax);
+    return hipGetLastError();
+}
+
+__global__ void k_single_set_offsets(int64_t* off, int64_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        off[0] = 0;
+        off[1] = n;
+    }
+}
+
+hipError_t launch_single_set_offsets(int64_t* off, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_single_set_offsets, dim3(1), dim3(64), 0, s, off, n);
     return hipGetLastError();
 }
 

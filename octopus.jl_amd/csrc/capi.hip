@@ -61,6 +61,7 @@ struct SetState {
     int cur = 0;
     oct_build_info info{};
     DevBuf rk, ri, rsk, rsi, rpos;  // refit: changed points
+    DevBuf soff;                    // {0, n}: the one-set forest descriptor
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
         nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax;
     BuildBufs bufs() const {
@@ -92,7 +93,7 @@ struct SetState {
     void release_all() {
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
                          &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
-                         &rk, &ri, &rsk, &rsi, &rpos};
+                         &rk, &ri, &rsk, &rsi, &rpos, &soff};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -128,6 +129,49 @@ struct Predictor {
     }
 };
 
+// A batch: many point sets searched against themselves as one forest (see
+// batch.hip).  Every step is enqueued with host parameters fixed after the
+// first eager call (sort shifts, capacities, grids sized by the point count),
+// so it can be captured once and replayed as a HIP graph.
+struct BatchState {
+    bool live = false;
+    int S = 0;
+    int64_t N = 0;
+    const float* coords = nullptr;
+    std::vector<int64_t> off_h;
+    std::vector<float> radii_h;
+    int leaf_size = 32;
+    int run_leaves = SP_DEFAULT_RUN;
+    bool use_graph = true;
+    int nblk_bounds = 0;
+    int max_bits = -1;         // Morton bits the captured sort covers (-1: not set up)
+    std::vector<int> shifts;   // radix digit shifts
+    int cap = 64;              // candidate leaves per query leaf
+    int64_t cap_edges = 0;     // edge buffer capacity (edges)
+    int64_t n_edges = 0;
+    int replayed = 0;
+    int steps = 0;
+    SetState set;              // forest build buffers; set.prm holds the forest's totals
+    PairState pair;            // edge buffers
+    DevBuf off, radii, bprefix, sprm, eoff, loff, bflags, bscan;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    void release_all() {
+        drop_graph();
+        set.release_all();
+        pair.release_all();
+        DevBuf* all[] = {&off, &radii, &bprefix, &sprm, &eoff, &loff, &bflags, &bscan};
+        for (DevBuf* d : all) d->release();
+        live = false;
+    }
+};
+
 }  // namespace oct
 
 using namespace oct;
@@ -141,8 +185,10 @@ struct oct_ctx {
     int leaf_size = 32;
     bool refit_mode = false;
     bool predict_always = false;
+    int run_leaves = SP_DEFAULT_RUN;  // leaves per single-pass run without a plan
     std::vector<SetState> sets;
     std::vector<PairState> pairs;
+    std::vector<BatchState> batches;
     Predictor mlp;
     std::string err;
     void* pinned = nullptr;  // small host staging area for read-backs
@@ -330,7 +376,10 @@ int build_set(oct_ctx* c, SetState& s) {
             HIP_TRY(c, launch_sort_stage(s.coords, n, s.key_bits, c->index_base, b, &cur, c->stream));
         }
         s.info.changed_points = try_refit ? (int64_t)changed : -1;
-        HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, cur, b, c->stream));
+        HIP_TRY(c, s.soff.ensure(16));
+        HIP_TRY(c, launch_single_set_offsets(s.soff.as<int64_t>(), n, c->stream));
+        const Forest f{0, s.soff.as<int64_t>(), s.prm.as<BuildParams>()};
+        HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, cur, b, f, c->stream));
     }
     rc = read_params(c, s);
     if (rc != OCT_OK) return rc;
@@ -380,6 +429,7 @@ int oct_destroy(oct_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto& s : c->sets) s.release_all();
     for (auto& p : c->pairs) p.release_all();
+    for (auto& b : c->batches) b.release_all();
     c->mlp.release_all();
     if (c->pinned) (void)hipHostFree(c->pinned);
     for (auto& m : c->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
@@ -574,7 +624,7 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
             Phase ph(c, OCT_PROF_LEAF_LISTS);
             HIP_TRY(c, launch_leaf_lists(qv, tv, r_sq, p.cap, safe, p.lists.as<int2>(),
                                          features ? p.ldir.as<uint8_t>() : nullptr, p.lcount.as<int32_t>(), flags,
-                                         c->stream));
+                                         SetLookup{}, c->stream));
         }
         if (features) {
             Phase ph(c, OCT_PROF_PREDICT);
@@ -612,7 +662,8 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
                                            predict ? p.wave_first.as<int32_t>() : nullptr,
                                            predict ? (const int32_t*)(flags + 12) : nullptr, p.lhits.as<uint32_t>(),
                                            p.tiles.as<uint64_t>(), flags + 8, flags + 4, flags + 9,
-                                           (uint64_t)p.cap_edges, eo, c->index_base, c->stream));
+                                           (uint64_t)p.cap_edges, eo, c->index_base, c->run_leaves, SetLookup{},
+                                           c->stream));
         } else {
             {
                 Phase ph(c, OCT_PROF_COUNT);
@@ -977,6 +1028,303 @@ int oct_mlp_forward(oct_ctx* c, const float* params, const float* X, int64_t n, 
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, launch_mlp_infer(X, n, params, y, nullptr, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return OCT_OK;
+}
+
+// ------------------------------------------------------------ batches
+
+namespace {
+
+BatchState* batch_of(oct_ctx* c, int id) {
+    const int k = id - c->index_base;
+    if (k < 0 || k >= (int)c->batches.size() || !c->batches[k].live) return nullptr;
+    return &c->batches[k];
+}
+
+// flags of a batch step (uint32 words): [0] a set needs more key bits than
+// the sort covers, [1] largest overflowing candidate count, [2] a list walk
+// ran out of stack, [4] single pass over capacity, [5] look-back gave up,
+// [6..7] edge total, [8] ticket, [9] recounted leaves, [10] recorded leaves
+constexpr int BF_WORDS = 16;
+
+int batch_ensure(oct_ctx* c, BatchState& B) {
+    const int64_t N = B.N;
+    SetState& s = B.set;
+    const int64_t M = max_nodes(N, B.leaf_size) + 22 * (int64_t)B.S;
+    HIP_TRY(c, s.prm.ensure(sizeof(BuildParams)));
+    HIP_TRY(c, s.partials.ensure(6 * sizeof(uint32_t) * (size_t)std::max(B.nblk_bounds, 1)));
+    HIP_TRY(c, s.keys0.ensure(8 * (size_t)N));
+    HIP_TRY(c, s.keys1.ensure(8 * (size_t)N));
+    HIP_TRY(c, s.vals0.ensure(4 * (size_t)N));
+    HIP_TRY(c, s.vals1.ensure(4 * (size_t)N));
+    HIP_TRY(c, s.hist.ensure(4 * (size_t)hist_words(N)));
+    HIP_TRY(c, s.scan_tmp.ensure(4 * (size_t)scan_tmp_words(hist_words(N))));
+    HIP_TRY(c, s.total.ensure(16));
+    HIP_TRY(c, s.topo.ensure(2 * (size_t)N));
+    HIP_TRY(c, s.perm.ensure(4 * (size_t)N));
+    HIP_TRY(c, s.scoord.ensure(16 * (size_t)N));
+    HIP_TRY(c, s.nodes.ensure(sizeof(NodeRec) * (size_t)M));
+    HIP_TRY(c, s.nfirst.ensure(4 * (size_t)M));
+    HIP_TRY(c, s.nlast.ensure(4 * (size_t)M));
+    HIP_TRY(c, s.nchild.ensure(32 * (size_t)M));
+    HIP_TRY(c, s.parent.ensure(4 * (size_t)M));
+    HIP_TRY(c, s.nlevel.ensure((size_t)M));
+    HIP_TRY(c, s.leaf_ids.ensure(4 * (size_t)N));
+    HIP_TRY(c, s.leaves.ensure(sizeof(NodeRec) * (size_t)N));
+    HIP_TRY(c, s.bmin.ensure(12 * (size_t)M));
+    HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
+    PairState& p = B.pair;
+    HIP_TRY(c, p.flags.ensure(4 * BF_WORDS));
+    HIP_TRY(c, p.lcount.ensure(4 * (size_t)N));
+    HIP_TRY(c, p.lists.ensure(sizeof(int2) * (size_t)N * (size_t)B.cap));
+    HIP_TRY(c, p.qcount.ensure(4 * (size_t)N));
+    HIP_TRY(c, p.qoff.ensure(4 * (size_t)N));
+    HIP_TRY(c, p.lhits.ensure(4 * (size_t)N));
+    HIP_TRY(c, p.tiles.ensure(8 * (size_t)(single_tiles(N) + 1)));
+    HIP_TRY(c, B.loff.ensure(4 * (size_t)N));
+    HIP_TRY(c, B.bscan.ensure(4 * (size_t)scan_tmp_words(N)));
+    HIP_TRY(c, B.eoff.ensure(8 * (size_t)(B.S + 1)));
+    const int64_t ce = std::max<int64_t>(B.cap_edges, 1);
+    HIP_TRY(c, p.senders.ensure(4 * (size_t)ce));
+    HIP_TRY(c, p.receivers.ensure(4 * (size_t)ce));
+    HIP_TRY(c, p.rdisp.ensure(12 * (size_t)ce));
+    HIP_TRY(c, p.rdist.ensure(4 * (size_t)ce));
+    return OCT_OK;
+}
+
+// Enqueues one whole batch step with the fixed host parameters (no host
+// wait, no allocation: capturable).
+int batch_enqueue(oct_ctx* c, BatchState& B) {
+    const hipStream_t st = c->stream;
+    const int64_t N = B.N;
+    SetState& s = B.set;
+    PairState& p = B.pair;
+    const BuildBufs b = s.bufs();
+    uint32_t* flags = p.flags.as<uint32_t>();
+    BuildParams* sprm = B.sprm.as<BuildParams>();
+    const int64_t* off = B.off.as<int64_t>();
+    HIP_TRY(c, launch_fill_u32(flags, BF_WORDS, 0u, st));
+    HIP_TRY(c, launch_bounds_batch(B.coords, off, B.bprefix.as<int32_t>(), B.S, B.nblk_bounds, B.radii.as<float>(),
+                                   b.partials, sprm, st));
+    HIP_TRY(c, launch_batch_check_bits(sprm, off, B.S, B.max_bits, flags, st));
+    HIP_TRY(c, launch_keys_batch(B.coords, N, off, B.S, sprm, b.keys[0], b.vals[0], st));
+    int cur = 0;
+    HIP_TRY(c, launch_sort_passes(B.coords, N, B.shifts.data(), (int)B.shifts.size(), 0, b, &cur, st));
+    const Forest f{1, off, sprm};
+    HIP_TRY(c, launch_topology_stage(N, 0, B.leaf_size, 0, cur, b, f, st));
+    // edges of every set against itself; grids are sized by N (an upper
+    // bound of the leaves) and the kernels read the real counts on the device
+    TreeView v;
+    v.nodes = s.nodes.as<NodeRec>();
+    v.leaf_ids = s.leaf_ids.as<int32_t>();
+    v.leaves = s.leaves.as<NodeRec>();
+    v.scoord = s.scoord.as<float4>();
+    v.params = s.prm.as<BuildParams>();
+    v.nlevel = s.nlevel.as<uint8_t>();
+    v.n_leaves = (int32_t)N;
+    // the fast walk packs node ids in 29 bits: a larger forest takes the safe walk
+    v.n_nodes = (int32_t)std::min<int64_t>(max_nodes(N, B.leaf_size) + 22 * (int64_t)B.S, INT32_MAX);
+    v.branch_root = 0;
+    v.n = N;
+    const SetLookup sets{1, b.keys[cur], off, sprm};
+    HIP_TRY(c, launch_leaf_lists(v, v, 0.f, B.cap, 0, p.lists.as<int2>(), nullptr, p.lcount.as<int32_t>(), flags + 1,
+                                 sets, st));
+    HIP_TRY(c, launch_fill_u32(p.lhits.as<uint32_t>(), N, 0u, st));
+    HIP_TRY(c, launch_fill_u32((uint32_t*)p.tiles.p, 2 * (single_tiles(N) + 1), 0u, st));
+    const EdgeOut eo{p.senders.as<int32_t>(), p.receivers.as<int32_t>(), p.rdisp.as<float>(), p.rdist.as<float>()};
+    HIP_TRY(c, launch_edges_single(v, v, p.lists.as<int2>(), p.lcount.as<int32_t>(), B.cap, 0.f, 0.f, 1,
+                                   p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), nullptr, nullptr,
+                                   p.lhits.as<uint32_t>(), p.tiles.as<uint64_t>(), flags + 8, flags + 4, flags + 9,
+                                   (uint64_t)B.cap_edges, eo, c->index_base, B.run_leaves, sets, st));
+    HIP_TRY(c, exclusive_scan(p.lhits.as<uint32_t>(), B.loff.as<uint32_t>(), N, B.bscan.as<uint32_t>(),
+                              (uint64_t*)(flags + 6), st));
+    HIP_TRY(c, launch_batch_edge_offsets(B.loff.as<uint32_t>(), (const uint64_t*)(flags + 6), v.leaves, v.params,
+                                         off, B.S, B.eoff.as<int64_t>(), st));
+    return OCT_OK;
+}
+
+// The sort's digit shifts for the sets' Morton bits and the set id bits.
+void batch_shifts(BatchState& B) {
+    B.shifts.clear();
+    const int mb = std::max(B.max_bits, 1);
+    for (int sh = 0; sh < mb; sh += RADIX_BITS) B.shifts.push_back(sh);
+    int set_bits = 1;
+    while ((1 << set_bits) < B.S) ++set_bits;
+    if (B.S > 1)
+        for (int sh = BATCH_SHIFT; sh < BATCH_SHIFT + set_bits; sh += RADIX_BITS) B.shifts.push_back(sh);
+}
+
+int batch_read_flags(oct_ctx* c, BatchState& B, uint32_t* hf) {
+    HIP_TRY(c, hipMemcpyAsync(c->pinned, B.pair.flags.p, 4 * BF_WORDS, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    memcpy(hf, c->pinned, 4 * BF_WORDS);
+    return OCT_OK;
+}
+
+// Eager step: learns the sort bits, grows what overflowed, reruns until the
+// step completes; then captures the step for replays.
+int batch_eager(oct_ctx* c, BatchState& B) {
+    B.drop_graph();
+    HIP_TRY(c, B.sprm.ensure(sizeof(BuildParams) * (size_t)B.S));
+    HIP_TRY(c, B.set.partials.ensure(6 * sizeof(uint32_t) * (size_t)B.nblk_bounds));
+    HIP_TRY(c, launch_bounds_batch(B.coords, B.off.as<int64_t>(), B.bprefix.as<int32_t>(), B.S, B.nblk_bounds,
+                                   B.radii.as<float>(), B.set.partials.as<uint32_t>(), B.sprm.as<BuildParams>(),
+                                   c->stream));
+    std::vector<BuildParams> hp((size_t)B.S);
+    HIP_TRY(c, hipMemcpyAsync(hp.data(), B.sprm.p, sizeof(BuildParams) * (size_t)B.S, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int mb = 0;
+    for (int s = 0; s < B.S; ++s)
+        if (B.off_h[s + 1] > B.off_h[s]) mb = std::max(mb, hp[s].key_bits);
+    if (mb > BATCH_SHIFT)
+        return fail(c, OCT_ERR_CAPACITY, "batched sets are limited to 2^14 cells per axis (extent / radius)");
+    B.max_bits = mb;
+    batch_shifts(B);
+    uint32_t hf[BF_WORDS] = {0};
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        int rc = batch_ensure(c, B);
+        if (rc != OCT_OK) return rc;
+        rc = batch_enqueue(c, B);
+        if (rc != OCT_OK) return rc;
+        rc = batch_read_flags(c, B, hf);
+        if (rc != OCT_OK) return rc;
+        uint64_t total = 0;
+        memcpy(&total, hf + 6, 8);
+        if (hf[0] != 0) return fail(c, OCT_ERR_DEVICE, "batch: key bits changed between bounds and keys");
+        if (hf[2] != 0) return fail(c, OCT_ERR_CAPACITY, "batch: a candidate walk ran out of stack");
+        if (hf[1] != 0) {
+            B.cap = (int)std::min<uint32_t>(hf[1] + 16u, 4096u);
+            continue;
+        }
+        if (hf[5] != 0) return fail(c, OCT_ERR_DEVICE, "batch: single-pass look-back did not finish");
+        if (hf[4] != 0) {
+            if (total >= (uint64_t)INT32_MAX) return fail(c, OCT_ERR_CAPACITY, "edge count exceeds Int32");
+            B.cap_edges = (int64_t)(total + total / 8 + 4096);
+            continue;
+        }
+        B.n_edges = (int64_t)total;
+        B.replayed = 0;
+        if (B.use_graph) {
+            // capture the same step for the next calls (a graph replays kernels only)
+            HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int erc = batch_enqueue(c, B);
+            hipGraph_t g = nullptr;
+            const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+            if (erc != OCT_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return erc;
+            }
+            HIP_TRY(c, ee);
+            B.graph = g;
+            HIP_TRY(c, hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+        }
+        return OCT_OK;
+    }
+    return fail(c, OCT_ERR_DEVICE, "batch: buffers did not settle");
+}
+
+}  // namespace
+
+int oct_batch_create(oct_ctx* c, int n_sets, const float* coords, const int64_t* offsets, const float* radii,
+                     int* batch_id) {
+    if (!c || !batch_id || !offsets || !radii) return OCT_ERR_ARGUMENT;
+    if (n_sets < 1 || n_sets > (1 << 20)) return fail(c, OCT_ERR_ARGUMENT, "batch: n_sets must be in 1..2^20");
+    if (offsets[0] != 0) return fail(c, OCT_ERR_ARGUMENT, "batch: offsets[0] must be 0");
+    for (int s = 0; s < n_sets; ++s) {
+        if (offsets[s + 1] < offsets[s]) return fail(c, OCT_ERR_ARGUMENT, "batch: offsets must be nondecreasing");
+        if (!(radii[s] > 0.f) || !std::isfinite(radii[s] * radii[s]))
+            return fail(c, OCT_ERR_ARGUMENT, "batch: radii must be positive with a finite square");
+    }
+    const int64_t N = offsets[n_sets];
+    if (N <= 0) return fail(c, OCT_ERR_ARGUMENT, "batch: no points");
+    if (N >= (int64_t)INT32_MAX) return fail(c, OCT_ERR_CAPACITY, "batch: at most 2^31 - 1 points");
+    if (!coords) return fail(c, OCT_ERR_ARGUMENT, "batch: coords is NULL");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->batches.emplace_back();
+    BatchState& B = c->batches.back();
+    B.live = true;
+    B.S = n_sets;
+    B.N = N;
+    B.coords = coords;
+    B.off_h.assign(offsets, offsets + n_sets + 1);
+    B.radii_h.assign(radii, radii + n_sets);
+    B.leaf_size = c->leaf_size;
+    B.run_leaves = c->run_leaves;
+    std::vector<int32_t> bp((size_t)n_sets + 1, 0);
+    for (int s = 0; s < n_sets; ++s) bp[s + 1] = bp[s] + bounds_blocks(std::max<int64_t>(offsets[s + 1] - offsets[s], 1));
+    B.nblk_bounds = bp[n_sets];
+    HIP_TRY(c, B.off.ensure(8 * (size_t)(n_sets + 1)));
+    HIP_TRY(c, B.radii.ensure(4 * (size_t)n_sets));
+    HIP_TRY(c, B.bprefix.ensure(4 * (size_t)(n_sets + 1)));
+    HIP_TRY(c, hipMemcpy(B.off.p, offsets, 8 * (size_t)(n_sets + 1), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(B.radii.p, radii, 4 * (size_t)n_sets, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(B.bprefix.p, bp.data(), 4 * (size_t)(n_sets + 1), hipMemcpyHostToDevice));
+    *batch_id = (int)c->batches.size() - 1 + c->index_base;
+    return OCT_OK;
+}
+
+int oct_batch_destroy(oct_ctx* c, int batch_id) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    BatchState* B = batch_of(c, batch_id);
+    if (!B) return fail(c, OCT_ERR_ARGUMENT, "unknown batch");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    B->release_all();
+    return OCT_OK;
+}
+
+int oct_batch_config(oct_ctx* c, int batch_id, int leaf_size, int run_leaves, int use_graph) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    BatchState* B = batch_of(c, batch_id);
+    if (!B) return fail(c, OCT_ERR_ARGUMENT, "unknown batch");
+    if (leaf_size < 1 || leaf_size > 256) return fail(c, OCT_ERR_ARGUMENT, "leaf size must be in 1..256");
+    if (run_leaves < 1 || run_leaves > SP_PLAN_MAX_LEAVES)
+        return fail(c, OCT_ERR_ARGUMENT, "leaves per run must be in 1..64");
+    if (leaf_size != B->leaf_size || run_leaves != B->run_leaves || (use_graph != 0) != B->use_graph) {
+        B->leaf_size = leaf_size;
+        B->run_leaves = run_leaves;
+        B->use_graph = use_graph != 0;
+        B->drop_graph();
+        B->max_bits = -1;
+    }
+    return OCT_OK;
+}
+
+int oct_batch_run(oct_ctx* c, int batch_id, oct_batch_edges* out) {
+    if (!c || !out) return OCT_ERR_ARGUMENT;
+    BatchState* Bp = batch_of(c, batch_id);
+    if (!Bp) return fail(c, OCT_ERR_ARGUMENT, "unknown batch");
+    BatchState& B = *Bp;
+    HIP_TRY(c, hipSetDevice(c->device));
+    memset(out, 0, sizeof(*out));
+    bool done = false;
+    if (B.exec && B.max_bits >= 0) {
+        HIP_TRY(c, hipGraphLaunch(B.exec, c->stream));
+        uint32_t hf[BF_WORDS] = {0};
+        int rc = batch_read_flags(c, B, hf);
+        if (rc != OCT_OK) return rc;
+        uint64_t total = 0;
+        memcpy(&total, hf + 6, 8);
+        if (hf[0] == 0 && hf[1] == 0 && hf[2] == 0 && hf[4] == 0 && hf[5] == 0) {
+            B.n_edges = (int64_t)total;
+            B.replayed = 1;
+            done = true;
+        }
+    }
+    if (!done) {
+        const int rc = batch_eager(c, B);
+        if (rc != OCT_OK) return rc;
+    }
+    ++B.steps;
+    PairState& p = B.pair;
+    out->senders = p.senders.as<int32_t>();
+    out->receivers = p.receivers.as<int32_t>();
+    out->rel_displacement = p.rdisp.as<float>();
+    out->rel_dist_norm = p.rdist.as<float>();
+    out->n_edges = B.n_edges;
+    out->set_offsets = B.eoff.as<int64_t>();
+    out->n_sets = B.S;
+    out->replayed = B.replayed;
     return OCT_OK;
 }
 

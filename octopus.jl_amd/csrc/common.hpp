@@ -74,6 +74,15 @@ __device__ inline int shared_levels(uint64_t a, uint64_t b) {
 // (src/cpu/build_cpu.jl:46-49): children of a level-L node differ there.
 __host__ __device__ inline int level_shift(int L) { return 63 - 3 * (L + 1); }
 
+// Set id and Morton part of a (possibly batched) key; levels two keys share,
+// -1 across a set boundary (a new tree starts there).
+__device__ inline int key_set(uint64_t k, int batched) { return batched ? (int)(k >> 42) : 0; }
+__device__ inline uint64_t key_morton(uint64_t k, int batched) { return batched ? (k & ((1ull << 42) - 1ull)) : k; }
+__device__ inline int key_shared(uint64_t a, uint64_t b, int batched) {
+    if (batched && ((a ^ b) >> 42)) return -1;
+    return shared_levels(key_morton(a, batched), key_morton(b, batched));
+}
+
 // ------------------------------------------------------------ XCD mapping
 
 // MI355X dispatches workgroups round-robin over its 8 XCDs, each with its own

@@ -76,15 +76,23 @@ template <bool FAST, bool DIRS>
 __global__ void __launch_bounds__(LL_THREADS)
 k_leaf_lists(const NodeRec* __restrict__ qleaves,
              const BuildParams* __restrict__ qprm, const NodeRec* __restrict__ tnodes,
-             const BuildParams* __restrict__ tprm, float r_sq, int cap, int2* __restrict__ lists,
-             uint8_t* __restrict__ ldir, int32_t* __restrict__ lcount, uint32_t* __restrict__ flags) {
+             const BuildParams* __restrict__ tprm, float r_sq0, int cap, int2* __restrict__ lists,
+             uint8_t* __restrict__ ldir, int32_t* __restrict__ lcount, uint32_t* __restrict__ flags,
+             SetLookup sets) {
     __shared__ uint32_t fstk[FAST ? LL_FAST_STACK : 1][LL_THREADS];
     const int32_t nleaves = qprm->n_leaves;
-    const int32_t root = tprm->branch_root;
+    const int32_t root0 = tprm->branch_root;
     const int32_t lb = (int32_t)xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     for (int32_t r = lb * (int32_t)blockDim.x + tid; r < nleaves; r += (int32_t)(gridDim.x * blockDim.x)) {
         const NodeRec qa = qleaves[r];
+        int32_t root = root0;
+        float r_sq = r_sq0;
+        if (sets.batched) {  // a forest: the leaf's own set's tree and radius
+            const BuildParams& sp = sets.sprm[key_set(sets.qsk[rec_a(qa)], 1)];
+            root = sp.branch_root;
+            r_sq = sp.radius * sp.radius;
+        }
         int2* out = lists + (int64_t)r * cap;
         uint8_t* dout = DIRS ? ldir + (int64_t)r * cap : nullptr;
         int cnt = 0;
@@ -504,7 +512,6 @@ k_edges(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, con
 
 // ------------------------------------------------------------ single pass
 
-constexpr int SP_LEAVES_PER_WAVE = 4;
 constexpr int SP_MAX_LEAVES = SP_PLAN_MAX_LEAVES;  // leaves one wave may own under a plan
 constexpr uint64_t SP_FLAG_AGG = 1ull << 62;   // tile aggregate published
 constexpr uint64_t SP_FLAG_PRE = 2ull << 62;   // tile inclusive prefix published
@@ -550,7 +557,7 @@ __device__ inline uint64_t wave_look_back(const uint64_t* tile_state, int32_t t,
 }
 
 // Single traversal in leaf order.  Every wave owns a run of consecutive query
-// leaves, which is also its output tile: SP_LEAVES_PER_WAVE leaves per run, or
+// leaves, which is also its output tile: leaves_per_run leaves per run, or
 // the runs of a plan (wave_first[w] .. wave_first[w + 1], at most
 // SP_MAX_LEAVES, *plan_waves runs) that packs the predicted hits of a run into
 // the wave's hit list.  Blocks take EW_WAVES consecutive runs per ticket
@@ -580,8 +587,23 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
                uint32_t* __restrict__ qcount, uint32_t* __restrict__ qoff, const int32_t* __restrict__ wave_first,
                const int32_t* __restrict__ plan_waves, uint32_t* __restrict__ lhits,
                uint64_t* __restrict__ tile_state, uint32_t* __restrict__ ticket, uint32_t* __restrict__ flags,
-               uint32_t* __restrict__ stats, uint64_t cap_edges, EdgeOut out, int index_base) {
+               uint32_t* __restrict__ stats, uint64_t cap_edges, EdgeOut out, int index_base0,
+               int leaves_per_run, SetLookup sets) {
     constexpr uint32_t HITS = WIDE ? SP_HIT_WORDS / 2 : SP_HIT_WORDS;
+    // per-leaf radius and output index base: the kernel's, or the leaf's set's
+    // in a forest (indices stay local to each set)
+    auto leaf_params = [&](int32_t qf, float& lr_sq, float& linv_r, int& libase) {
+        lr_sq = r_sq;
+        linv_r = inv_r;
+        libase = index_base0;
+        if (sets.batched) {
+            const int s = key_set(sets.qsk[qf], 1);
+            const float rr = sets.sprm[s].radius;
+            lr_sq = rr * rr;
+            linv_r = 1.0f / rr;
+            libase = index_base0 - (int)sets.set_off[s];
+        }
+    };
     __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
     __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
     __shared__ uint32_t s_hits[EW_WAVES][SP_HIT_WORDS];
@@ -599,7 +621,7 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
     const WaveLds L{s_start[wv], s_first[wv], stage, stage + SP_STAGE, reinterpret_cast<float*>(stage + 2 * SP_STAGE),
                     reinterpret_cast<float*>(stage + 5 * SP_STAGE), SP_STAGE, s_cand[wv]};
     const int32_t nleaves = qprm->n_leaves;
-    const int32_t nruns = wave_first ? *plan_waves : (nleaves + SP_LEAVES_PER_WAVE - 1) / SP_LEAVES_PER_WAVE;
+    const int32_t nruns = wave_first ? *plan_waves : (nleaves + leaves_per_run - 1) / leaves_per_run;
     const int32_t ntickets = (nruns + EW_WAVES - 1) / EW_WAVES;
     uint32_t n_recount = 0, n_record = 0;  // this wave's, over all its runs
     for (;;) {
@@ -615,8 +637,8 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
             r_lo = wave_first[w];
             r_hi = wave_first[w + 1];
         } else {
-            r_lo = w * SP_LEAVES_PER_WAVE;
-            r_hi = min(nleaves, r_lo + SP_LEAVES_PER_WAVE);
+            r_lo = w * leaves_per_run;
+            r_hi = min(nleaves, r_lo + leaves_per_run);
         }
         r_lo = __builtin_amdgcn_readfirstlane(r_lo);
         r_hi = __builtin_amdgcn_readfirstlane(r_hi);
@@ -637,9 +659,12 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
             const int2* lst = lists + (int64_t)r * cap;
             const int2 lf0 = lst[lane < ncand ? lane : 0];
             const int total = __builtin_amdgcn_readfirstlane(wave_sum(lane < ncand ? lf0.y : 0));
+            float lr_sq, linv_r;
+            int libase;
+            leaf_params(qf, lr_sq, linv_r, libase);
             uint32_t c;
             if (one_tile_leaf(qc, ncand, total)) {
-                c = leaf_sweep_record<WIDE>(qf, qc, qa.lo, qa.hi, ncand, lf0, qsc, tsc, r_sq, self_pair, L, lane,
+                c = leaf_sweep_record<WIDE>(qf, qc, qa.lo, qa.hi, ncand, lf0, qsc, tsc, lr_sq, self_pair, L, lane,
                                             WIDE ? hits + 2 * hfill : hits + hfill, HITS - hfill);
                 if (c <= HITS - hfill) {
                     recorded |= 1ull << k;
@@ -647,14 +672,12 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
                     ++nrec;
                 } else {
                     // the list is full: count per query for the recount in phase B
-                    leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0,
-                                          qcount,
-                                          nullptr, out, index_base, L, lane);
+                    leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r, self_pair,
+                                          0, qcount, nullptr, out, libase, L, lane);
                 }
             } else {
-                c = leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0,
-                                          qcount,
-                                          nullptr, out, index_base, L, lane);
+                c = leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r, self_pair,
+                                          0, qcount, nullptr, out, libase, L, lane);
             }
             if (lane == 0) {
                 cntw[k] = c;
@@ -690,8 +713,11 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
             const int k = r - r_lo;
             const uint32_t c = cntw[k];
             if (recorded & (1ull << k)) {
-                emit_hits<WIDE>(WIDE ? hits + 2 * hread : hits + hread, c, pos, qfw[k], qsc, tsc, inv_r, out,
-                                index_base, lane);
+                float lr_sq, linv_r;
+                int libase;
+                leaf_params(qfw[k], lr_sq, linv_r, libase);
+                emit_hits<WIDE>(WIDE ? hits + 2 * hread : hits + hread, c, pos, qfw[k], qsc, tsc, linv_r, out,
+                                libase, lane);
                 hread += c;
             }
             pos += c;
@@ -716,9 +742,11 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
                 const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
                 const int2* lst = lists + (int64_t)r * cap;
                 const int2 lf0 = lst[lane < ncand ? lane : 0];
-                leaf_pass<EDGE_FILL>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, r_sq, inv_r, self_pair, 0,
-                                     qcount, qoff,
-                                     out, index_base, L, lane);
+                float lr_sq, linv_r;
+                int libase;
+                leaf_params(qf, lr_sq, linv_r, libase);
+                leaf_pass<EDGE_FILL>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r, self_pair, 0,
+                                     qcount, qoff, out, libase, L, lane);
             }
             pos += cntw[k];
         }
@@ -746,12 +774,13 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
 // ------------------------------------------------------------- launchers
 
 hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
-                             uint8_t* ldir, int32_t* lcount, uint32_t* flags, hipStream_t s) {
+                             uint8_t* ldir, int32_t* lcount, uint32_t* flags, const SetLookup& sets,
+                             hipStream_t s) {
     const int64_t want = ((int64_t)q.n_leaves + LL_THREADS - 1) / LL_THREADS;
     const unsigned g = (unsigned)(want < 1 ? 1 : (want > 65535 ? 65535 : want));
 #define OCT_LL_LAUNCH(F, D)                                                                                 \
     hipLaunchKernelGGL((k_leaf_lists<F, D>), dim3(g), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes, t.params, \
-                       r_sq, cap, lists, ldir, lcount, flags)
+                       r_sq, cap, lists, ldir, lcount, flags, sets)
     const bool fast = !safe && t.n_nodes < (1 << 29);
     if (fast && ldir) OCT_LL_LAUNCH(true, true);
     else if (fast) OCT_LL_LAUNCH(true, false);
@@ -788,19 +817,23 @@ hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2*
                                int cap, float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
                                const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
                                uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint32_t* stats,
-                               uint64_t cap_edges, EdgeOut out, int index_base, hipStream_t s) {
-    const int64_t runs = wave_first ? q.n_leaves : (q.n_leaves + SP_LEAVES_PER_WAVE - 1) / SP_LEAVES_PER_WAVE;
+                               uint64_t cap_edges, EdgeOut out, int index_base, int leaves_per_run,
+                               const SetLookup& sets, hipStream_t s) {
+    if (leaves_per_run < 1 || leaves_per_run > SP_MAX_LEAVES) return hipErrorInvalidValue;
+    const int64_t runs = wave_first ? q.n_leaves : (q.n_leaves + leaves_per_run - 1) / leaves_per_run;
     const int64_t tickets = (runs + EW_WAVES - 1) / EW_WAVES;
     if (tickets <= 0) return hipSuccess;
     const unsigned g = (unsigned)std::min<int64_t>(tickets, 4096);  // blocks loop over tickets
     if (t.n < ((int64_t)1 << HIT_POS_BITS))
         hipLaunchKernelGGL(k_edges_single<false>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first,
-                           plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base);
+                           plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base,
+                           leaves_per_run, sets);
     else
         hipLaunchKernelGGL(k_edges_single<true>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first,
-                           plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base);
+                           plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base,
+                           leaves_per_run, sets);
     return hipGetLastError();
 }
 

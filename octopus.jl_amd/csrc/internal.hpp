@@ -22,7 +22,19 @@ struct BuildParams {
     int32_t n_leaves;
     int32_t branch_root;   // first node with count == n that is a leaf or has > 1 child
     int32_t chain_level;   // L0: levels 0..L0 hold all n points (one node each)
-    int32_t pad;
+    float radius;          // search radius of the set (batched builds: per set)
+};
+
+// Batched builds (many sets in one forest): keys carry the set id above bit
+// BATCH_SHIFT and the set's own Morton key below it (at most 14 bits per
+// axis), so one sort orders all sets and a change of set id is a tree
+// boundary.  A single set has batched = 0 and plain 63-bit keys.
+constexpr int BATCH_SHIFT = 42;
+constexpr uint64_t BATCH_MORTON = (1ull << BATCH_SHIFT) - 1ull;
+struct Forest {
+    int batched;
+    const int64_t* set_off;  // S + 1 offsets of the sets in sorted order (positions)
+    BuildParams* sprm;       // per-set parameters (the single set: the set's own)
 };
 
 // Packed traversal record of one node (32 B, two 16-B loads):
@@ -50,6 +62,85 @@ struct TreeView {
     int32_t branch_root;
     int64_t n;
 };
+
+// ----- bounds reduction pieces (build.hip, batch.hip)
+
+constexpr int BND_THREADS = 256;
+
+// Ordered-uint min/max of the coordinates of points i0 .. i1, the share of
+// block lb of nb (grid-stride).
+__device__ inline void bounds_part(const float* __restrict__ coords, int64_t i0, int64_t i1, int lb, int nb,
+                                   uint32_t* __restrict__ out) {
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+    uint32_t mx[3] = {0u, 0u, 0u};
+    const int64_t stride = (int64_t)nb * blockDim.x;
+    for (int64_t i = i0 + (int64_t)lb * blockDim.x + threadIdx.x; i < i1; i += stride) {
+        for (int d = 0; d < 3; ++d) {
+            const uint32_t o = f2o(coords[3 * i + d]);
+            mn[d] = o < mn[d] ? o : mn[d];
+            mx[d] = o > mx[d] ? o : mx[d];
+        }
+    }
+    __shared__ uint32_t red[6][BND_THREADS / OCT_WAVE];
+    const int w = threadIdx.x / OCT_WAVE;
+    for (int d = 0; d < 3; ++d) {
+        const uint32_t a = wave_min_u(mn[d]);
+        const uint32_t b = wave_max_u(mx[d]);
+        if (lane_id() == 0) { red[d][w] = a; red[3 + d][w] = b; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        uint32_t v = red[d][0];
+        for (int k = 1; k < BND_THREADS / OCT_WAVE; ++k)
+            v = d < 3 ? (red[d][k] < v ? red[d][k] : v) : (red[d][k] > v ? red[d][k] : v);
+        out[d] = v;
+    }
+}
+
+// One block folds the partials b0 .. b1 and derives origin, 1/r and the live
+// key bits of one set.
+__device__ inline void bounds_fold(const uint32_t* __restrict__ partials, int b0, int b1, float radius,
+                                   BuildParams* __restrict__ prm) {
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+    uint32_t mx[3] = {0u, 0u, 0u};
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += BND_THREADS)
+        for (int d = 0; d < 3; ++d) {
+            const uint32_t a = partials[6 * b + d], c = partials[6 * b + 3 + d];
+            mn[d] = a < mn[d] ? a : mn[d];
+            mx[d] = c > mx[d] ? c : mx[d];
+        }
+    __shared__ uint32_t red[6][BND_THREADS / OCT_WAVE];
+    const int w = threadIdx.x / OCT_WAVE;
+    for (int d = 0; d < 3; ++d) {
+        const uint32_t a = wave_min_u(mn[d]);
+        const uint32_t b = wave_max_u(mx[d]);
+        if (lane_id() == 0) { red[d][w] = a; red[3 + d][w] = b; }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    uint32_t v[6];
+    for (int d = 0; d < 6; ++d) v[d] = red[d][0];
+    for (int k = 1; k < BND_THREADS / OCT_WAVE; ++k)
+        for (int d = 0; d < 3; ++d) {
+            v[d] = red[d][k] < v[d] ? red[d][k] : v[d];
+            v[3 + d] = red[3 + d][k] > v[3 + d] ? red[3 + d][k] : v[3 + d];
+        }
+    const float inv_cs = 1.0f / radius;  // one(T) / cell_size
+    uint32_t cmax = 0;
+    for (int d = 0; d < 3; ++d) {
+        prm->omin[d] = v[d];
+        prm->omax[d] = v[3 + d];
+        const float o = o2f(v[d]) - 1e-6f;  // ox - T(1e-6)
+        prm->origin[d] = o;
+        const uint32_t c = floor_clamp((o2f(v[3 + d]) - o) * inv_cs, 0x1fffffu);
+        cmax = c > cmax ? c : cmax;
+    }
+    prm->inv_cs = inv_cs;
+    prm->radius = radius;
+    prm->cmax = cmax;
+    prm->key_bits = cmax == 0 ? 0 : 3 * (32 - __clz((int)cmax));
+}
 
 // Device buffers of one point set's build (see build.hip).
 struct BuildBufs {
@@ -79,8 +170,12 @@ hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, con
                                hipStream_t s);
 hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int index_base, const BuildBufs& b,
                              int* cur_out, hipStream_t s);
+constexpr int RADIX_BITS = 9;
+hipError_t launch_sort_passes(const float* coords, int64_t n, const int* shifts, int npasses, int index_base,
+                              const BuildBufs& b, int* cur_out, hipStream_t s);
 hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base, int cur, const BuildBufs& b,
-                                 hipStream_t s);
+                                 const Forest& f, hipStream_t s);
+hipError_t launch_single_set_offsets(int64_t* off, int64_t n, hipStream_t s);
 
 // Almost-sorted refit (time stepping): the previous sort is keys[cur] and
 // scoord.  prepare: new keys in that order and the changed count in *b.total;
@@ -109,9 +204,19 @@ struct EdgeOut {
 enum { EDGE_COUNT = 0, EDGE_FILL = 1 };
 // flags[0]: largest candidate count of an overflowing query leaf; flags[1]:
 // a walk ran out of stack (rerun with safe = 1)
+// Per-leaf set parameters of a forest (batched sets, self pairs): the set of
+// a query leaf comes from the key of its first point; batched = 0: one set,
+// the kernel arguments apply.
+struct SetLookup {
+    int batched;
+    const uint64_t* qsk;      // sorted keys of the query forest
+    const int64_t* set_off;   // S + 1 positions
+    const BuildParams* sprm;  // per-set parameters
+};
 // ldir (optional): one direction code per list entry, for the predictor.
 hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
-                             uint8_t* ldir, int32_t* lcount, uint32_t* flags, hipStream_t s);
+                             uint8_t* ldir, int32_t* lcount, uint32_t* flags, const SetLookup& sets,
+                             hipStream_t s);
 hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const int2* lists,
                         const int32_t* lcount, int cap, float r_sq, float inv_r, int self_pair,
                         int by_receiver, uint32_t* qcount, const uint32_t* qoff, uint32_t* lhits, int64_t n_edges,
@@ -126,7 +231,9 @@ hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2*
                                int cap, float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
                                const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
                                uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint32_t* stats,
-                               uint64_t cap_edges, EdgeOut out, int index_base, hipStream_t s);
+                               uint64_t cap_edges, EdgeOut out, int index_base, int leaves_per_run,
+                               const SetLookup& sets, hipStream_t s);
+constexpr int SP_DEFAULT_RUN = 4;  // leaves per run of the single pass without a plan
 
 // Per-leaf count predictor (see predictor.hip).  Parameter block layout in
 // floats: feature mean and std, then W1 (29 x 64, row = input), b1, W2
@@ -170,6 +277,18 @@ hipError_t launch_predict_plan(const float* feat, int64_t n, const float* P, flo
                                int32_t* n_runs, uint64_t* pred_total, hipStream_t s);
 constexpr uint32_t SP_PLAN_BUDGET = 1024;  // hit-list entries of one wave (edges.hip)
 constexpr int SP_PLAN_MAX_LEAVES = 64;     // leaves one wave may own (edges.hip)
+
+// Batched forests (see batch.hip).
+hipError_t launch_bounds_batch(const float* coords, const int64_t* off, const int32_t* bprefix, int S,
+                               int nblocks, const float* radii, uint32_t* partials, BuildParams* sprm,
+                               hipStream_t s);
+hipError_t launch_keys_batch(const float* coords, int64_t n, const int64_t* off, int S, const BuildParams* sprm,
+                             uint64_t* keys, int32_t* vals, hipStream_t s);
+hipError_t launch_batch_check_bits(const BuildParams* sprm, const int64_t* off, int S, int max_bits,
+                                   uint32_t* flags, hipStream_t s);
+hipError_t launch_batch_edge_offsets(const uint32_t* loff, const uint64_t* total, const NodeRec* leaves,
+                                     const BuildParams* gprm, const int64_t* off, int S, int64_t* eoff,
+                                     hipStream_t s);
 
 // Adjoint (see backward.hip).
 hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* senders,

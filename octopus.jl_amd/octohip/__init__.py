@@ -71,7 +71,7 @@ def _wrap(ptr, shape, dtype, device):
         n *= s
     if n == 0 or not ptr:
         return torch.empty(shape, dtype=dtype, device=device)
-    typestr = {torch.int32: "<i4", torch.float32: "<f4"}[dtype]
+    typestr = {torch.int32: "<i4", torch.float32: "<f4", torch.int64: "<i8"}[dtype]
     return torch.as_tensor(_DeviceArray(ptr, shape, typestr), device=device)
 
 
@@ -451,3 +451,4 @@ def build_edges_diff(coords, tns: TNS, set_id: int, radius) -> Edges:
 
 
 from . import predictor  # noqa: E402  (learned per-leaf count predictor)
+from .batch import Batch  # noqa: E402  (batched sets)

@@ -74,6 +74,19 @@ class OctView(ctypes.Structure):
     ]
 
 
+class OctBatchEdges(ctypes.Structure):
+    _fields_ = [
+        ("senders", ctypes.c_void_p),
+        ("receivers", ctypes.c_void_p),
+        ("rel_displacement", ctypes.c_void_p),
+        ("rel_dist_norm", ctypes.c_void_p),
+        ("n_edges", ctypes.c_int64),
+        ("set_offsets", ctypes.c_void_p),
+        ("n_sets", ctypes.c_int32),
+        ("replayed", ctypes.c_int32),
+    ]
+
+
 class OctBuildInfo(ctypes.Structure):
     _fields_ = [
         ("changed_points", ctypes.c_int64),
@@ -139,6 +152,10 @@ SIGNATURES = {
     "oct_predictor_train": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
                                            ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
     "oct_edge_stats_get": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OctEdgeStats)]),
+    "oct_batch_create": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.POINTER(ctypes.c_int)]),
+    "oct_batch_destroy": (ctypes.c_int, [_P, ctypes.c_int]),
+    "oct_batch_config": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "oct_batch_run": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(OctBatchEdges)]),
     "oct_mlp_forward": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
     "oct_mlp_gradient": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int, _P, _P]),
 }
