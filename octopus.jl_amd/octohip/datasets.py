@@ -109,12 +109,35 @@ def mean_neighbors_sample(pts: np.ndarray, r: float, sample: int = 512, seed: in
     return tot / q.shape[0]
 
 
-def calibrate_radius(pts: np.ndarray, target: float, seed: int = 0) -> np.float32:
-    """Bisection (in log r) for the radius whose sample mean neighbour count is target."""
+def calibrate_radius(pts: np.ndarray, target: float, seed: int = 0, sample: int = 128,
+                     keep: int = 256) -> np.float32:
+    """Bisection (in log r) for the radius whose sample mean neighbour count is
+    target.  The FP32 squared distances of a fixed sample of query points are
+    computed once; each sample keeps its `keep` smallest (counts saturate
+    there, far above the targets used here)."""
+    n = pts.shape[0]
+    if n < 2:
+        return np.float32(0.1)
+    idx = (splitmix64(seed, 9, min(sample, n)) % np.uint64(n)).astype(np.int64)
+    q = pts[idx].astype(np.float32)
+    p32 = pts.astype(np.float32)
+    kk = min(keep, n)
+    rows = []
+    for a in range(0, q.shape[0], 64):
+        d = p32[None, :, :] - q[a:a + 64, None, :]
+        d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+        part = np.partition(d2, kk - 1, axis=1)[:, :kk]
+        rows.append(np.sort(part, axis=1))
+    near = np.concatenate(rows)
+
+    def mean_count(r):
+        r2 = np.float32(r) * np.float32(r)
+        return float(np.mean(np.sum(near <= r2, axis=1) - 1))
+
     lo, hi = 1e-4, 0.5
     for _ in range(30):
         mid = math.sqrt(lo * hi)
-        if mean_neighbors_sample(pts, mid, seed=seed) < target:
+        if mean_count(mid) < target:
             lo = mid
         else:
             hi = mid
