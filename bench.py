@@ -50,6 +50,7 @@ def parse():
                     help="run the predictor on every step (default: first call and after overflows / recounts)")
     ap.add_argument("--workload", default="sph", choices=["sph", "batched"],
                     help="sph: the headline 1M-point SPH block; batched: 256 uneven sets in one replayed graph")
+    ap.add_argument("--no-autotune", action="store_true", help="batched: keep --leaf-size / --run-leaves")
     ap.add_argument("--leaf-size", type=int, default=32)
     ap.add_argument("--run-leaves", type=int, default=4)
     return ap.parse_args()
@@ -150,6 +151,9 @@ def bench_batched(a):
     tns = octohip.TNS()
     b = octohip.Batch(tns, coords, off, radii)
     b.config(a.leaf_size, a.run_leaves, True)
+    tuned = None
+    if not a.no_autotune:
+        tuned = b.autotune(reps=3)
 
     def step(k):
         coords.copy_(snaps[k])
@@ -209,7 +213,9 @@ def bench_batched(a):
         "config": {"workload": "256 sets, sizes log-uniform in [1000, 65536], 4-component Gaussian mixtures, "
                                "radius per set for ~20 neighbours; in-place update + run! + build_edges per step",
                    "sets": len(sets), "points": n, "edges_per_step": edges / a.steps,
-                   "leaf_size": a.leaf_size, "run_leaves": a.run_leaves, "timed_replays": replays},
+                   "leaf_size": tuned["best"][0] if tuned else a.leaf_size,
+                   "run_leaves": tuned["best"][1] if tuned else a.run_leaves, "timed_replays": replays},
+        "autotune_ms_per_step": {f"leaf{k[0]}_run{k[1]}": v for k, v in tuned["ms"].items()} if tuned else None,
         "sets_per_s": len(sets) * a.steps / sec,
         "sequential_single_set_api": {"ms_per_step": 1e3 * seq_sec, "edges_per_s": seq_edges / seq_sec},
         "cpu_baseline": cpu,

@@ -262,6 +262,23 @@ int oct_batch_destroy(oct_ctx* ctx, int batch_id);
 int oct_batch_config(oct_ctx* ctx, int batch_id, int leaf_size, int run_leaves, int use_graph);
 int oct_batch_run(oct_ctx* ctx, int batch_id, oct_batch_edges* out);
 
+/* ---- Autotuner -------------------------------------------------------------
+ * No counterpart in the reference (leaf size 32 and the launch shapes are
+ * fixed there, src/types.jl:177).  Each candidate (leaf size, leaves per
+ * single-pass run) is timed with events over `reps` steps after one untimed
+ * step; the fastest is kept and its index (leaf index * n_runs + run index)
+ * written to *best, the times (ms per step) to ms_out[n_leaf_sizes *
+ * n_runs] when not NULL.  Results never depend on the choice. */
+int oct_batch_autotune(oct_ctx* ctx, int batch_id, const int* leaf_sizes, int n_leaf_sizes, const int* runs,
+                       int n_runs, int reps, double* ms_out, int* best);
+/* The same for the active pair (q, t) of a context: a step is run! of every
+ * set plus build_edges(q, t, mode); sets the context's leaf size and run
+ * length. */
+int oct_autotune(oct_ctx* ctx, int q, int t, int mode, const int* leaf_sizes, int n_leaf_sizes, const int* runs,
+                 int n_runs, int reps, double* ms_out, int* best);
+/* Leaves per single-pass run without a plan (1..64, default 4). */
+int oct_set_run_leaves(oct_ctx* ctx, int run_leaves);
+
 /* The network on n rows of features (device arrays): y = log1p(hits). */
 int oct_mlp_forward(oct_ctx* ctx, const float* params, const float* features, int64_t n, float* y);
 /* Gradient of mean((f(x) - target)^2) over n rows with respect to the

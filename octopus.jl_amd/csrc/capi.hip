@@ -1328,6 +1328,101 @@ int oct_batch_run(oct_ctx* c, int batch_id, oct_batch_edges* out) {
     return OCT_OK;
 }
 
+// ------------------------------------------------------------ autotuner
+
+extern "C++" {
+namespace {
+
+// Milliseconds of device time of `reps` calls of fn, measured with events on
+// the context stream after one untimed call.
+template <typename F>
+int time_device(oct_ctx* c, int reps, F fn, double* ms) {
+    int rc = fn();
+    if (rc != OCT_OK) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(c, hipEventCreate(&a));
+    HIP_TRY(c, hipEventCreate(&b));
+    HIP_TRY(c, hipEventRecord(a, c->stream));
+    for (int k = 0; k < reps && rc == OCT_OK; ++k) rc = fn();
+    HIP_TRY(c, hipEventRecord(b, c->stream));
+    HIP_TRY(c, hipEventSynchronize(b));
+    float t = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = (double)t / std::max(reps, 1);
+    return rc;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int oct_batch_autotune(oct_ctx* c, int batch_id, const int* leaf_sizes, int n_leaf_sizes, const int* runs,
+                       int n_runs, int reps, double* ms_out, int* best) {
+    if (!c || !leaf_sizes || !runs || n_leaf_sizes < 1 || n_runs < 1 || reps < 1) return OCT_ERR_ARGUMENT;
+    BatchState* B = batch_of(c, batch_id);
+    if (!B) return fail(c, OCT_ERR_ARGUMENT, "unknown batch");
+    const bool graph = B->use_graph;
+    double best_ms = 1e300;
+    int best_k = -1;
+    for (int i = 0; i < n_leaf_sizes; ++i)
+        for (int j = 0; j < n_runs; ++j) {
+            const int k = i * n_runs + j;
+            int rc = oct_batch_config(c, batch_id, leaf_sizes[i], runs[j], graph ? 1 : 0);
+            if (rc != OCT_OK) return rc;
+            oct_batch_edges e;
+            double ms = 0.0;
+            rc = time_device(c, reps, [&]() { return oct_batch_run(c, batch_id, &e); }, &ms);
+            if (rc != OCT_OK) return rc;
+            if (ms_out) ms_out[k] = ms;
+            if (ms < best_ms) { best_ms = ms; best_k = k; }
+        }
+    int rc = oct_batch_config(c, batch_id, leaf_sizes[best_k / n_runs], runs[best_k % n_runs], graph ? 1 : 0);
+    if (rc != OCT_OK) return rc;
+    if (best) *best = best_k;
+    return OCT_OK;
+}
+
+int oct_set_run_leaves(oct_ctx* c, int run_leaves) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (run_leaves < 1 || run_leaves > SP_PLAN_MAX_LEAVES)
+        return fail(c, OCT_ERR_ARGUMENT, "leaves per run must be in 1..64");
+    c->run_leaves = run_leaves;
+    return OCT_OK;
+}
+
+int oct_autotune(oct_ctx* c, int q, int t, int mode, const int* leaf_sizes, int n_leaf_sizes, const int* runs,
+                 int n_runs, int reps, double* ms_out, int* best) {
+    if (!c || !leaf_sizes || !runs || n_leaf_sizes < 1 || n_runs < 1 || reps < 1) return OCT_ERR_ARGUMENT;
+    if (!find_pair(c, q, t)) return fail(c, OCT_ERR_STATE, "autotune: pair not registered");
+    double best_ms = 1e300;
+    int best_k = -1;
+    for (int i = 0; i < n_leaf_sizes; ++i)
+        for (int j = 0; j < n_runs; ++j) {
+            const int k = i * n_runs + j;
+            int rc = oct_set_leaf_size(c, leaf_sizes[i]);
+            if (rc != OCT_OK) return rc;
+            rc = oct_set_run_leaves(c, runs[j]);
+            if (rc != OCT_OK) return rc;
+            oct_edges e;
+            double ms = 0.0;
+            // one step: rebuild the dirty sets and the edges of the pair
+            rc = time_device(c, reps, [&]() {
+                for (auto& s : c->sets) s.dirty = true;
+                int r = oct_run(c);
+                return r == OCT_OK ? oct_build_edges(c, q, t, mode, OCT_ORDER_LEAF, &e) : r;
+            }, &ms);
+            if (rc != OCT_OK) return rc;
+            if (ms_out) ms_out[k] = ms;
+            if (ms < best_ms) { best_ms = ms; best_k = k; }
+        }
+    int rc = oct_set_leaf_size(c, leaf_sizes[best_k / n_runs]);
+    if (rc == OCT_OK) rc = oct_set_run_leaves(c, runs[best_k % n_runs]);
+    if (rc != OCT_OK) return rc;
+    if (best) *best = best_k;
+    return OCT_OK;
+}
+
 int oct_mlp_gradient(oct_ctx* c, const float* params, const float* X, const float* T, int n, float* grad,
                      float* loss) {
     if (!c || !params || !X || !T || !grad || n < 1 || n > (1 << 20)) return OCT_ERR_ARGUMENT;

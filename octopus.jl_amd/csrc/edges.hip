@@ -345,8 +345,63 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
 // sorted position, target sorted position) while they fit; the return value
 // is the exact hit count either way (a count above `room` means the list
 // could not hold them all).
-__device__ inline bool one_tile_leaf(int32_t qc, int32_t ncand, int total) {
-    return qc <= OCT_WAVE && ncand <= OCT_WAVE && total <= EW_TILE * OCT_WAVE;
+constexpr int REC_MAX_CAND = 4 * EW_TILE * OCT_WAVE;  // candidate points a recorded leaf may have before filtering
+constexpr uint32_t REC_NOT_TILE = 0xffffffffu;        // the filtered candidates do not fit one register tile
+
+__device__ inline bool recordable_leaf(int32_t qc, int32_t ncand, int total) {
+    return qc <= OCT_WAVE && ncand <= OCT_WAVE && total <= REC_MAX_CAND;
+}
+
+// All candidates of the sequence L.start / L.first (at most REC_MAX_CAND),
+// filtered by their distance to the query box and compacted into one register
+// tile.  Returns the number kept, or -1 when more than one tile survives.
+__device__ inline int load_filtered_tile(int total, const float4* __restrict__ tsc, const WaveLds& L, int lane,
+                                         float4 qlo, float4 qhi, float r_sq, float4 (&tp)[EW_TILE],
+                                         int32_t (&tpos)[EW_TILE]) {
+    int kept = 0;
+    for (int t0 = 0; t0 < total; t0 += EW_TILE * OCT_WAVE) {
+        bool keep[EW_TILE];
+#pragma unroll
+        for (int u = 0; u < EW_TILE; ++u) {
+            const int ci = t0 + u * OCT_WAVE + lane;
+            keep[u] = false;
+            tpos[u] = 0;
+            if (ci < total) {
+                int lo = 0, hi = OCT_WAVE;  // owner: last o with start[o] <= ci
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L.start[mid] <= ci) lo = mid;
+                    else hi = mid;
+                }
+                tpos[u] = L.first[lo] + (ci - L.start[lo]);
+                const float4 p = tsc[tpos[u]];
+                keep[u] = point_box_d2(p.x, p.y, p.z, qlo, qhi) <= r_sq;
+            }
+        }
+        int add = 0;
+#pragma unroll
+        for (int u = 0; u < EW_TILE; ++u) add += __popcll(ballot(keep[u]));
+        if (kept + add > EW_TILE * OCT_WAVE) return -1;  // wave-uniform
+#pragma unroll
+        for (int u = 0; u < EW_TILE; ++u) {
+            const uint64_t m = ballot(keep[u]);
+            if (keep[u]) L.cand[kept + mbcnt(m)] = tpos[u];
+            kept += __popcll(m);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < EW_TILE; ++u) {
+        const int k = u * OCT_WAVE + lane;
+        tp[u] = make_float4(INFINITY, INFINITY, INFINITY, __int_as_float(-1));  // never within r
+        tpos[u] = 0;
+        if (k < kept) {
+            tpos[u] = L.cand[k];
+            tp[u] = tsc[tpos[u]];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    return kept;
 }
 
 // Hit-list entry: compact = query slot in the leaf (6 bits) << 26 | target
@@ -373,7 +428,8 @@ __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, float4 qlo,
     __builtin_amdgcn_wave_barrier();
     float4 tp[EW_TILE];
     int32_t tpos[EW_TILE];
-    const int got = load_tile(0, total, tsc, L, lane, true, qlo, qhi, r_sq, tp, tpos);
+    const int got = load_filtered_tile(total, tsc, L, lane, qlo, qhi, r_sq, tp, tpos);
+    if (got < 0) return REC_NOT_TILE;
     const int nu = (got + OCT_WAVE - 1) / OCT_WAVE;
     uint32_t run = 0;
     for (int qi = 0; qi < nqb; ++qi) {
@@ -662,10 +718,11 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
             float lr_sq, linv_r;
             int libase;
             leaf_params(qf, lr_sq, linv_r, libase);
-            uint32_t c;
-            if (one_tile_leaf(qc, ncand, total)) {
+            uint32_t c = REC_NOT_TILE;
+            if (recordable_leaf(qc, ncand, total))
                 c = leaf_sweep_record<WIDE>(qf, qc, qa.lo, qa.hi, ncand, lf0, qsc, tsc, lr_sq, self_pair, L, lane,
                                             WIDE ? hits + 2 * hfill : hits + hfill, HITS - hfill);
+            if (c != REC_NOT_TILE) {
                 if (c <= HITS - hfill) {
                     recorded |= 1ull << k;
                     hfill += c;

@@ -365,6 +365,36 @@ def count_neighbors_device(tns: TNS, view: L.OctView):
     return out
 
 
+def set_run_leaves(tns: TNS, run_leaves: int):
+    """Leaves per single-pass run without a plan (1..64, default 4)."""
+    tns._check(tns._lib.oct_set_run_leaves(tns._c(), int(run_leaves)))
+
+
+def autotune(tns: TNS, q: int, t: int, mode: str = "learned", leaf_sizes=(8, 16, 32, 64), runs=(1, 2, 4, 8),
+             reps: int = 3) -> dict:
+    """Times run! + build_edges(q, t, mode) for every (leaf size, leaves per
+    run) candidate and keeps the fastest (the reference fixes the leaf size at
+    32, src/types.jl:177).  Returns {"best": (leaf, run), "ms": {...}}; the
+    edges never depend on the choice."""
+    import numpy as np
+
+    if mode not in _MODES:
+        raise ArgumentError(f"mode must be one of {sorted(_MODES)}")
+    ls = np.ascontiguousarray(np.asarray(leaf_sizes, dtype=np.int32))
+    rs = np.ascontiguousarray(np.asarray(runs, dtype=np.int32))
+    ms = np.zeros(ls.shape[0] * rs.shape[0], dtype=np.float64)
+    best = ctypes.c_int()
+    tns._check(tns._lib.oct_autotune(tns._enter(), int(q), int(t), _MODES[mode], ls.ctypes.data_as(ctypes.c_void_p),
+                                     int(ls.shape[0]), rs.ctypes.data_as(ctypes.c_void_p), int(rs.shape[0]), int(reps),
+                                     ms.ctypes.data_as(ctypes.c_void_p), ctypes.byref(best)))
+    tns._leave()
+    k = best.value
+    tns.target_leaf_size = int(ls[k // rs.shape[0]])
+    tns.dirty = [True] * len(tns.dirty)
+    table = {(int(ls[i // rs.shape[0]]), int(rs[i % rs.shape[0]])): float(ms[i]) for i in range(ms.shape[0])}
+    return {"best": (int(ls[k // rs.shape[0]]), int(rs[k % rs.shape[0]])), "ms": table}
+
+
 def profile(tns: TNS, enable: bool = True):
     """Turn per-phase kernel timing (HIP events on the TNS stream) on or off."""
     tns._check(tns._lib.oct_profile(tns._c(), int(bool(enable))))

@@ -156,6 +156,11 @@ SIGNATURES = {
     "oct_batch_destroy": (ctypes.c_int, [_P, ctypes.c_int]),
     "oct_batch_config": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "oct_batch_run": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(OctBatchEdges)]),
+    "oct_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P,
+                                          ctypes.POINTER(ctypes.c_int)]),
+    "oct_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, _P, ctypes.c_int,
+                                    ctypes.c_int, _P, ctypes.POINTER(ctypes.c_int)]),
+    "oct_set_run_leaves": (ctypes.c_int, [_P, ctypes.c_int]),
     "oct_mlp_forward": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
     "oct_mlp_gradient": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int, _P, _P]),
 }

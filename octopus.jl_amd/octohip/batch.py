@@ -68,6 +68,23 @@ class Batch:
             replayed=bool(out.replayed),
         )
 
+    def autotune(self, leaf_sizes=(8, 12, 16, 24, 32), runs=(1, 2, 4, 8), reps: int = 3) -> dict:
+        """Times every (leaf size, leaves per run) candidate on this batch's
+        current coordinates and keeps the fastest; returns {"best": (leaf,
+        run), "ms": {(leaf, run): ms per step}}."""
+        ls = np.ascontiguousarray(np.asarray(leaf_sizes, dtype=np.int32))
+        rs = np.ascontiguousarray(np.asarray(runs, dtype=np.int32))
+        ms = np.zeros(ls.shape[0] * rs.shape[0], dtype=np.float64)
+        best = ctypes.c_int()
+        self.tns._check(self.tns._lib.oct_batch_autotune(
+            self.tns._enter(), self.id, ls.ctypes.data_as(ctypes.c_void_p), int(ls.shape[0]),
+            rs.ctypes.data_as(ctypes.c_void_p), int(rs.shape[0]), int(reps), ms.ctypes.data_as(ctypes.c_void_p),
+            ctypes.byref(best)))
+        self.tns._leave()
+        table = {(int(ls[i // rs.shape[0]]), int(rs[i % rs.shape[0]])): float(ms[i]) for i in range(ms.shape[0])}
+        k = best.value
+        return {"best": (int(ls[k // rs.shape[0]]), int(rs[k % rs.shape[0]])), "ms": table}
+
     def close(self):
         if self.id is not None:
             self.tns._check(self.tns._lib.oct_batch_destroy(self.tns._c(), self.id))

@@ -1,0 +1,74 @@
+"""Autotuner: it keeps the fastest measured candidate, and the edges it
+produces are exactly those of any other candidate (the choice of leaf size
+and run length never changes a result)."""
+import numpy as np
+import pytest
+
+from brute import sort_edges
+from octohip import datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _np_edges(e):
+    return sort_edges(e.senders.cpu().numpy(), e.receivers.cpu().numpy(), e.rel_displacement.cpu().numpy(),
+                      e.rel_dist_norm.cpu().numpy().reshape(-1))
+
+
+def test_pair_autotune_keeps_fastest_and_same_edges(orc):
+    import octohip
+
+    pts = datasets.uniform_cube(60000, 12)
+    r = 0.03
+    tns = octohip.TNS()
+    octohip.set_search_radius(tns, r)
+    c = _torch().from_numpy(pts).cuda()
+    sid = octohip.add_point_set(tns, c)
+    octohip.set_active_search(tns, sid, sid)
+    octohip.run(tns)
+    base = _np_edges(octohip.build_edges(tns, sid, sid, mode="learned"))
+    res = octohip.autotune(tns, sid, sid, mode="learned", leaf_sizes=(8, 32), runs=(1, 4), reps=2)
+    print(f"pair autotune {res}", flush=True)
+    assert res["best"] == min(res["ms"], key=res["ms"].get)
+    octohip.run(tns)
+    tuned = _np_edges(octohip.build_edges(tns, sid, sid, mode="learned"))
+    for a, b in zip(base, tuned):
+        assert np.array_equal(a, b)
+    o = orc.build_edges(orc.run(pts, r), pts)
+    for a, b in zip(tuned, sort_edges(o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm)):
+        assert np.array_equal(a, b)
+
+
+def test_batch_autotune_keeps_fastest_and_same_edges():
+    import octohip
+
+    sets, radii = datasets.batched_sets(n_sets=16, seed=21, lo=500, hi=8000)
+    off = np.concatenate([[0], np.cumsum([s.shape[0] for s in sets])]).astype(np.int64)
+    tns = octohip.TNS()
+    c = _torch().from_numpy(np.concatenate(sets)).cuda()
+    b = octohip.Batch(tns, c, off, np.array(radii, np.float32))
+    before = b.run()
+    eoff0 = before.set_offsets.cpu().numpy()
+    base = [sort_edges(*(x[eoff0[k]:eoff0[k + 1]] for x in (before.senders.cpu().numpy(),
+                                                                before.receivers.cpu().numpy(),
+                                                                before.rel_displacement.cpu().numpy(),
+                                                                before.rel_dist_norm.cpu().numpy().reshape(-1))))
+            for k in range(len(sets))]
+    res = b.autotune(leaf_sizes=(8, 16, 32), runs=(1, 4, 8), reps=2)
+    print(f"batch autotune {res}", flush=True)
+    assert res["best"] == min(res["ms"], key=res["ms"].get)
+    after = b.run()
+    eoff = after.set_offsets.cpu().numpy()
+    for k in range(len(sets)):
+        got = sort_edges(*(x[eoff[k]:eoff[k + 1]] for x in (after.senders.cpu().numpy(),
+                                                             after.receivers.cpu().numpy(),
+                                                             after.rel_displacement.cpu().numpy(),
+                                                             after.rel_dist_norm.cpu().numpy().reshape(-1))))
+        for x, y in zip(got, base[k]):
+            assert np.array_equal(x, y)
