@@ -155,9 +155,13 @@ def bench_batched(a):
     if not a.no_autotune:
         tuned = b.autotune(reps=3)
 
+    last = {}
+
     def step(k):
         coords.copy_(snaps[k])
         res = b.run()
+        last["recounted_leaves"] = res.recounted_leaves
+        last["recorded_leaves"] = res.recorded_leaves
         return res.senders.shape[0], res.replayed
 
     for k in range(a.warmup):
@@ -217,6 +221,7 @@ def bench_batched(a):
                    "run_leaves": tuned["best"][1] if tuned else a.run_leaves, "timed_replays": replays},
         "autotune_ms_per_step": {f"leaf{k[0]}_run{k[1]}": v for k, v in tuned["ms"].items()} if tuned else None,
         "sets_per_s": len(sets) * a.steps / sec,
+        "last_step": dict(last),
         "sequential_single_set_api": {"ms_per_step": 1e3 * seq_sec, "edges_per_s": seq_edges / seq_sec},
         "cpu_baseline": cpu,
     }))

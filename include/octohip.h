@@ -253,6 +253,8 @@ typedef struct oct_batch_edges {
     const int64_t* set_offsets;   /* device, n_sets + 1: set s owns edges [off[s], off[s+1]) */
     int32_t n_sets;
     int32_t replayed;             /* 1: this call replayed the captured graph */
+    int64_t recounted_leaves;     /* single pass: leaves swept twice */
+    int64_t recorded_leaves;      /* single pass: leaves swept once */
 } oct_batch_edges;
 int oct_batch_create(oct_ctx* ctx, int n_sets, const float* coords, const int64_t* offsets, const float* radii,
                      int* batch_id);

@@ -151,6 +151,7 @@ struct BatchState {
     int64_t n_edges = 0;
     int replayed = 0;
     int steps = 0;
+    int64_t recounted = 0, recorded = 0;  // leaves the last step swept twice / once
     SetState set;              // forest build buffers; set.prm holds the forest's totals
     PairState pair;            // edge buffers
     DevBuf off, radii, bprefix, sprm, eoff, loff, bflags, bscan;
@@ -1204,6 +1205,8 @@ int batch_eager(oct_ctx* c, BatchState& B) {
             continue;
         }
         B.n_edges = (int64_t)total;
+        B.recounted = hf[9];
+        B.recorded = hf[10];
         B.replayed = 0;
         if (B.use_graph) {
             // capture the same step for the next calls (a graph replays kernels only)
@@ -1307,6 +1310,8 @@ int oct_batch_run(oct_ctx* c, int batch_id, oct_batch_edges* out) {
         memcpy(&total, hf + 6, 8);
         if (hf[0] == 0 && hf[1] == 0 && hf[2] == 0 && hf[4] == 0 && hf[5] == 0) {
             B.n_edges = (int64_t)total;
+            B.recounted = hf[9];
+            B.recorded = hf[10];
             B.replayed = 1;
             done = true;
         }
@@ -1325,6 +1330,8 @@ int oct_batch_run(oct_ctx* c, int batch_id, oct_batch_edges* out) {
     out->set_offsets = B.eoff.as<int64_t>();
     out->n_sets = B.S;
     out->replayed = B.replayed;
+    out->recounted_leaves = B.recounted;
+    out->recorded_leaves = B.recorded;
     return OCT_OK;
 }
 

@@ -84,6 +84,8 @@ class OctBatchEdges(ctypes.Structure):
         ("set_offsets", ctypes.c_void_p),
         ("n_sets", ctypes.c_int32),
         ("replayed", ctypes.c_int32),
+        ("recounted_leaves", ctypes.c_int64),
+        ("recorded_leaves", ctypes.c_int64),
     ]
 
 

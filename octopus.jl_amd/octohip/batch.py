@@ -15,7 +15,7 @@ import numpy as np
 from . import _lib as L
 
 BatchEdges = namedtuple("BatchEdges", ["senders", "receivers", "rel_displacement", "rel_dist_norm",
-                                       "set_offsets", "replayed"])
+                                       "set_offsets", "replayed", "recounted_leaves", "recorded_leaves"])
 
 
 class Batch:
@@ -66,6 +66,8 @@ class Batch:
             rel_dist_norm=_wrap(out.rel_dist_norm, (n, 1), torch.float32, dev),
             set_offsets=_wrap(out.set_offsets, (self.n_sets + 1,), torch.int64, dev),
             replayed=bool(out.replayed),
+            recounted_leaves=int(out.recounted_leaves),
+            recorded_leaves=int(out.recorded_leaves),
         )
 
     def autotune(self, leaf_sizes=(8, 12, 16, 24, 32), runs=(1, 2, 4, 8), reps: int = 3) -> dict:
