@@ -50,9 +50,11 @@ def parse():
                     help="run the predictor on every step (default: first call and after overflows / recounts)")
     ap.add_argument("--workload", default="sph", choices=["sph", "batched"],
                     help="sph: the headline 1M-point SPH block; batched: 256 uneven sets in one replayed graph")
-    ap.add_argument("--no-autotune", action="store_true", help="batched: keep --leaf-size / --run-leaves")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched: keep --leaf-size / --run-leaves / --run-chunk")
     ap.add_argument("--leaf-size", type=int, default=32)
     ap.add_argument("--run-leaves", type=int, default=4)
+    ap.add_argument("--run-chunk", type=int, default=4, help="single-pass runs per block ticket (1..4)")
     return ap.parse_args()
 
 
@@ -150,7 +152,7 @@ def bench_batched(a):
     coords = snaps[0].clone()
     tns = octohip.TNS()
     b = octohip.Batch(tns, coords, off, radii)
-    b.config(a.leaf_size, a.run_leaves, True)
+    b.config(a.leaf_size, a.run_leaves, True, a.run_chunk)
     tuned = None
     if not a.no_autotune:
         tuned = b.autotune(reps=3)
@@ -218,8 +220,10 @@ def bench_batched(a):
                                "radius per set for ~20 neighbours; in-place update + run! + build_edges per step",
                    "sets": len(sets), "points": n, "edges_per_step": edges / a.steps,
                    "leaf_size": tuned["best"][0] if tuned else a.leaf_size,
-                   "run_leaves": tuned["best"][1] if tuned else a.run_leaves, "timed_replays": replays},
-        "autotune_ms_per_step": {f"leaf{k[0]}_run{k[1]}": v for k, v in tuned["ms"].items()} if tuned else None,
+                   "run_leaves": tuned["best"][1] if tuned else a.run_leaves,
+                   "run_chunk": tuned["best"][2] if tuned else a.run_chunk, "timed_replays": replays},
+        "autotune_ms_per_step": ({f"leaf{k[0]}_run{k[1]}_chunk{k[2]}": v for k, v in tuned["ms"].items()}
+                                 if tuned else None),
         "sets_per_s": len(sets) * a.steps / sec,
         "last_step": dict(last),
         "sequential_single_set_api": {"ms_per_step": 1e3 * seq_sec, "edges_per_s": seq_edges / seq_sec},

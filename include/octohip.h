@@ -228,7 +228,7 @@ typedef struct oct_edge_stats {
     int32_t single_pass;       /* the edges came from the single pass */
     int32_t used_predictor;
     int32_t attempts;          /* passes the call needed (1 when nothing overflowed) */
-    int32_t pad;
+    int32_t heavy_leaves;      /* single pass: leaves counted ahead (too large to record in one sweep) */
 } oct_edge_stats;
 int oct_edge_stats_get(oct_ctx* ctx, int q, int t, oct_edge_stats* out);
 
@@ -255,31 +255,38 @@ typedef struct oct_batch_edges {
     int32_t replayed;             /* 1: this call replayed the captured graph */
     int64_t recounted_leaves;     /* single pass: leaves swept twice */
     int64_t recorded_leaves;      /* single pass: leaves swept once */
+    int64_t heavy_leaves;         /* single pass: leaves counted ahead (too large to record in one sweep) */
 } oct_batch_edges;
 int oct_batch_create(oct_ctx* ctx, int n_sets, const float* coords, const int64_t* offsets, const float* radii,
                      int* batch_id);
 int oct_batch_destroy(oct_ctx* ctx, int batch_id);
-/* Leaf size (1..256), leaves per single-pass run (1..64) and graph replay on
- * or off; a change drops the captured graph. */
-int oct_batch_config(oct_ctx* ctx, int batch_id, int leaf_size, int run_leaves, int use_graph);
+/* Leaf size (1..256), leaves per single-pass run (1..64), single-pass runs
+ * per block ticket (1..4) and graph replay on or off; a change drops the
+ * captured graph. */
+int oct_batch_config(oct_ctx* ctx, int batch_id, int leaf_size, int run_leaves, int run_chunk, int use_graph);
 int oct_batch_run(oct_ctx* ctx, int batch_id, oct_batch_edges* out);
 
 /* ---- Autotuner -------------------------------------------------------------
  * No counterpart in the reference (leaf size 32 and the launch shapes are
  * fixed there, src/types.jl:177).  Each candidate (leaf size, leaves per
- * single-pass run) is timed with events over `reps` steps after one untimed
- * step; the fastest is kept and its index (leaf index * n_runs + run index)
- * written to *best, the times (ms per step) to ms_out[n_leaf_sizes *
- * n_runs] when not NULL.  Results never depend on the choice. */
+ * single-pass run, runs per block ticket) is timed with events over `reps`
+ * steps after one untimed step; the fastest is kept and its index
+ * ((leaf index * n_runs + run index) * n_chunks + chunk index) written to
+ * *best, the times (ms per step) to ms_out[n_leaf_sizes * n_runs * n_chunks]
+ * when not NULL.  Results never depend on the choice. */
 int oct_batch_autotune(oct_ctx* ctx, int batch_id, const int* leaf_sizes, int n_leaf_sizes, const int* runs,
-                       int n_runs, int reps, double* ms_out, int* best);
+                       int n_runs, const int* chunks, int n_chunks, int reps, double* ms_out, int* best);
 /* The same for the active pair (q, t) of a context: a step is run! of every
- * set plus build_edges(q, t, mode); sets the context's leaf size and run
- * length. */
+ * set plus build_edges(q, t, mode); sets the context's leaf size, run length
+ * and runs per ticket. */
 int oct_autotune(oct_ctx* ctx, int q, int t, int mode, const int* leaf_sizes, int n_leaf_sizes, const int* runs,
-                 int n_runs, int reps, double* ms_out, int* best);
+                 int n_runs, const int* chunks, int n_chunks, int reps, double* ms_out, int* best);
 /* Leaves per single-pass run without a plan (1..64, default 4). */
 int oct_set_run_leaves(oct_ctx* ctx, int run_leaves);
+/* Single-pass runs a block takes per ticket (1..4, default 4): larger shares
+ * candidate points in a CU's caches, smaller starts every run at once (better
+ * for uneven runs). */
+int oct_set_run_chunk(oct_ctx* ctx, int run_chunk);
 
 /* The network on n rows of features (device arrays): y = log1p(hits). */
 int oct_mlp_forward(oct_ctx* ctx, const float* params, const float* features, int64_t n, float* y);

@@ -109,10 +109,16 @@ struct PairState {
     // predictor inputs and plan: direction codes of the list entries,
     // features, predicted and exact per-leaf hits, plan work / scan / runs
     DevBuf ldir, feat, pred, lhits, pwork, pscan, wave_first;
+    // heavy query leaves, counted ahead of the single pass: their list
+    // (int32 per leaf) followed by a flag byte per leaf
+    DevBuf heavy;
+    bool heavy_on = true;  // count heavy leaves ahead: first call, and while many leaves are not recorded
+    int32_t* heavy_list() { return heavy.as<int32_t>(); }
+    uint8_t* heavy_flags(int64_t n_leaves) { return (uint8_t*)(heavy.as<int32_t>() + n_leaves); }
     oct_edge_stats st{};
     void release_all() {
         DevBuf* all[] = {&senders, &receivers, &rdisp, &rdist, &lists, &lcount, &qcount, &qoff, &scan_tmp, &flags,
-                         &tiles, &ldir, &feat, &pred, &lhits, &pwork, &pscan, &wave_first};
+                         &tiles, &ldir, &feat, &pred, &lhits, &pwork, &pscan, &wave_first, &heavy};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -142,6 +148,7 @@ struct BatchState {
     std::vector<float> radii_h;
     int leaf_size = 32;
     int run_leaves = SP_DEFAULT_RUN;
+    int run_chunk = SP_DEFAULT_CHUNK;
     bool use_graph = true;
     int nblk_bounds = 0;
     int max_bits = -1;         // Morton bits the captured sort covers (-1: not set up)
@@ -152,6 +159,7 @@ struct BatchState {
     int replayed = 0;
     int steps = 0;
     int64_t recounted = 0, recorded = 0;  // leaves the last step swept twice / once
+    int64_t heavy = 0;                    // leaves the last step counted ahead
     SetState set;              // forest build buffers; set.prm holds the forest's totals
     PairState pair;            // edge buffers
     DevBuf off, radii, bprefix, sprm, eoff, loff, bflags, bscan;
@@ -187,6 +195,7 @@ struct oct_ctx {
     bool refit_mode = false;
     bool predict_always = false;
     int run_leaves = SP_DEFAULT_RUN;  // leaves per single-pass run without a plan
+    int run_chunk = SP_DEFAULT_CHUNK;  // single-pass runs per block ticket
     std::vector<SetState> sets;
     std::vector<PairState> pairs;
     std::vector<BatchState> batches;
@@ -587,7 +596,7 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     const bool predict = mode == OCT_EDGES_LEARNED && !collect && c->mlp.loaded &&
                          (c->predict_always || p.cap_edges <= 0 || p.replan);
     const bool features = predict || collect;
-    HIP_TRY(c, p.flags.ensure(64));
+    HIP_TRY(c, p.flags.ensure(128));
     HIP_TRY(c, p.lcount.ensure(4 * (size_t)nql));
     HIP_TRY(c, p.qcount.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.qoff.ensure(4 * (size_t)qs.n));
@@ -607,11 +616,13 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     // total, [4] single pass over capacity, [5] single-pass look-back gave up
     // or malformed plan, [6..7] single-pass edge total, [8] single-pass tile
     // ticket, [9] recounted leaves, [10] recorded leaves, [12] plan runs,
-    // [14..15] predicted total
+    // [14..15] predicted total; [16] heavy leaves (kept while the lists are)
     uint32_t* flags = p.flags.as<uint32_t>();
-    uint32_t hf[16] = {0};
+    uint32_t hf[17] = {0};
     int safe = 0;
     bool single = mode == OCT_EDGES_LEARNED && !collect && (p.cap_edges > 0 || predict);
+    const bool heavy = single && p.heavy_on;
+    if (heavy) HIP_TRY(c, p.heavy.ensure(5 * (size_t)nql));
     bool lists_ok = false;
     uint64_t total = 0;
     int attempts = 0;
@@ -620,12 +631,13 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
         ++attempts;
         HIP_TRY(c, p.lists.ensure(sizeof(int2) * (size_t)nql * (size_t)p.cap));
         if (features) HIP_TRY(c, p.ldir.ensure((size_t)nql * (size_t)p.cap));
-        HIP_TRY(c, launch_fill_u32(flags, 16, 0u, c->stream));
+        HIP_TRY(c, launch_fill_u32(flags, lists_ok ? 16 : 17, 0u, c->stream));
         if (!lists_ok) {
             Phase ph(c, OCT_PROF_LEAF_LISTS);
             HIP_TRY(c, launch_leaf_lists(qv, tv, r_sq, p.cap, safe, p.lists.as<int2>(),
                                          features ? p.ldir.as<uint8_t>() : nullptr, p.lcount.as<int32_t>(), flags,
-                                         SetLookup{}, c->stream));
+                                         SetLookup{}, heavy ? p.heavy_flags(nql) : nullptr,
+                                         heavy ? p.heavy_list() : nullptr, flags + 16, c->stream));
         }
         if (features) {
             Phase ph(c, OCT_PROF_PREDICT);
@@ -658,13 +670,18 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
             eo = EdgeOut{p.senders.as<int32_t>(), p.receivers.as<int32_t>(), p.rdisp.as<float>(), p.rdist.as<float>()};
             HIP_TRY(c, launch_fill_u32((uint32_t*)p.tiles.p, 2 * (ntiles + 1), 0u, c->stream));
             Phase ph(c, OCT_PROF_SINGLE);
+            if (heavy)
+                HIP_TRY(c, launch_edges_heavy(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
+                                              inv_r, self_pair, p.heavy_list(), flags + 16, p.qcount.as<uint32_t>(),
+                                              p.lhits.as<uint32_t>(), SetLookup{}, c->stream));
             HIP_TRY(c, launch_edges_single(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r,
                                            self_pair, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(),
                                            predict ? p.wave_first.as<int32_t>() : nullptr,
                                            predict ? (const int32_t*)(flags + 12) : nullptr, p.lhits.as<uint32_t>(),
                                            p.tiles.as<uint64_t>(), flags + 8, flags + 4, flags + 9,
-                                           (uint64_t)p.cap_edges, eo, c->index_base, c->run_leaves, SetLookup{},
-                                           c->stream));
+                                           (uint64_t)p.cap_edges, eo, c->index_base, c->run_leaves, c->run_chunk,
+                                           SetLookup{},
+                                           heavy ? p.heavy_flags(nql) : nullptr, c->stream));
         } else {
             {
                 Phase ph(c, OCT_PROF_COUNT);
@@ -676,12 +693,12 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
             HIP_TRY(c, exclusive_scan(p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), qs.n,
                                       p.scan_tmp.as<uint32_t>(), (uint64_t*)(flags + 2), c->stream));
         }
-        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 64, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 68, hipMemcpyDeviceToHost, c->stream));
         {
             HostTimer ht(single ? "edges: wait for single pass" : "edges: wait for count total");
             HIP_TRY(c, hipStreamSynchronize(c->stream));
         }
-        memcpy(hf, c->pinned, 64);
+        memcpy(hf, c->pinned, 68);
         if (hf[0] != 0 || hf[1] != 0) {  // candidate lists incomplete: regrow / safe walk, redo everything
             if (hf[0] != 0) p.cap = (int)std::min<uint32_t>(hf[0] + 16u, (uint32_t)INT32_MAX / 8u);
             if (hf[1] != 0) safe = 1;
@@ -748,12 +765,17 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     p.st.n_edges = E;
     p.st.recounted_leaves = single ? hf[9] : 0;
     p.st.recorded_leaves = single ? hf[10] : 0;
+    p.st.heavy_leaves = heavy && single ? (int32_t)hf[16] : 0;
     p.st.predicted_total = predict && single ? (double)pt : 0.0;
     p.st.single_pass = single ? 1 : 0;
     p.st.used_predictor = predict && single ? 1 : 0;
     p.st.attempts = attempts;
-    if (mode == OCT_EDGES_LEARNED)
+    if (mode == OCT_EDGES_LEARNED) {
         p.replan = !single || attempts > 1 || (int64_t)hf[9] * 64 > nql;
+        // uneven leaves (many not recorded in one sweep): count the heavy ones
+        // ahead next time; even ones skip the estimate and the extra launch
+        if (single) p.heavy_on = (int64_t)hf[9] * 64 > nql;
+    }
     out->senders = eo.senders;
     out->receivers = eo.receivers;
     out->rel_displacement = eo.rdisp;
@@ -1045,7 +1067,8 @@ BatchState* batch_of(oct_ctx* c, int id) {
 // flags of a batch step (uint32 words): [0] a set needs more key bits than
 // the sort covers, [1] largest overflowing candidate count, [2] a list walk
 // ran out of stack, [4] single pass over capacity, [5] look-back gave up,
-// [6..7] edge total, [8] ticket, [9] recounted leaves, [10] recorded leaves
+// [6..7] edge total, [8] ticket, [9] recounted leaves, [10] recorded leaves,
+// [11] heavy leaves
 constexpr int BF_WORDS = 16;
 
 int batch_ensure(oct_ctx* c, BatchState& B) {
@@ -1081,6 +1104,7 @@ int batch_ensure(oct_ctx* c, BatchState& B) {
     HIP_TRY(c, p.qcount.ensure(4 * (size_t)N));
     HIP_TRY(c, p.qoff.ensure(4 * (size_t)N));
     HIP_TRY(c, p.lhits.ensure(4 * (size_t)N));
+    HIP_TRY(c, p.heavy.ensure(5 * (size_t)N));
     HIP_TRY(c, p.tiles.ensure(8 * (size_t)(single_tiles(N) + 1)));
     HIP_TRY(c, B.loff.ensure(4 * (size_t)N));
     HIP_TRY(c, B.bscan.ensure(4 * (size_t)scan_tmp_words(N)));
@@ -1129,14 +1153,19 @@ int batch_enqueue(oct_ctx* c, BatchState& B) {
     v.n = N;
     const SetLookup sets{1, b.keys[cur], off, sprm};
     HIP_TRY(c, launch_leaf_lists(v, v, 0.f, B.cap, 0, p.lists.as<int2>(), nullptr, p.lcount.as<int32_t>(), flags + 1,
-                                 sets, st));
+                                 sets, p.heavy_flags(N), p.heavy_list(), flags + 11, st));
     HIP_TRY(c, launch_fill_u32(p.lhits.as<uint32_t>(), N, 0u, st));
     HIP_TRY(c, launch_fill_u32((uint32_t*)p.tiles.p, 2 * (single_tiles(N) + 1), 0u, st));
     const EdgeOut eo{p.senders.as<int32_t>(), p.receivers.as<int32_t>(), p.rdisp.as<float>(), p.rdist.as<float>()};
+    HIP_TRY(c, launch_edges_heavy(v, v, p.lists.as<int2>(), p.lcount.as<int32_t>(), B.cap, 0.f, 0.f, 1,
+                                  p.heavy_list(), flags + 11, p.qcount.as<uint32_t>(), p.lhits.as<uint32_t>(), sets,
+                                  st));
     HIP_TRY(c, launch_edges_single(v, v, p.lists.as<int2>(), p.lcount.as<int32_t>(), B.cap, 0.f, 0.f, 1,
                                    p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), nullptr, nullptr,
                                    p.lhits.as<uint32_t>(), p.tiles.as<uint64_t>(), flags + 8, flags + 4, flags + 9,
-                                   (uint64_t)B.cap_edges, eo, c->index_base, B.run_leaves, sets, st));
+                                   (uint64_t)B.cap_edges, eo, c->index_base, B.run_leaves, B.run_chunk, sets,
+                                   p.heavy_flags(N),
+                                   st));
     HIP_TRY(c, exclusive_scan(p.lhits.as<uint32_t>(), B.loff.as<uint32_t>(), N, B.bscan.as<uint32_t>(),
                               (uint64_t*)(flags + 6), st));
     HIP_TRY(c, launch_batch_edge_offsets(B.loff.as<uint32_t>(), (const uint64_t*)(flags + 6), v.leaves, v.params,
@@ -1207,6 +1236,7 @@ int batch_eager(oct_ctx* c, BatchState& B) {
         B.n_edges = (int64_t)total;
         B.recounted = hf[9];
         B.recorded = hf[10];
+        B.heavy = hf[11];
         B.replayed = 0;
         if (B.use_graph) {
             // capture the same step for the next calls (a graph replays kernels only)
@@ -1254,6 +1284,7 @@ int oct_batch_create(oct_ctx* c, int n_sets, const float* coords, const int64_t*
     B.radii_h.assign(radii, radii + n_sets);
     B.leaf_size = c->leaf_size;
     B.run_leaves = c->run_leaves;
+    B.run_chunk = c->run_chunk;
     std::vector<int32_t> bp((size_t)n_sets + 1, 0);
     for (int s = 0; s < n_sets; ++s) bp[s + 1] = bp[s] + bounds_blocks(std::max<int64_t>(offsets[s + 1] - offsets[s], 1));
     B.nblk_bounds = bp[n_sets];
@@ -1276,16 +1307,19 @@ int oct_batch_destroy(oct_ctx* c, int batch_id) {
     return OCT_OK;
 }
 
-int oct_batch_config(oct_ctx* c, int batch_id, int leaf_size, int run_leaves, int use_graph) {
+int oct_batch_config(oct_ctx* c, int batch_id, int leaf_size, int run_leaves, int run_chunk, int use_graph) {
     if (!c) return OCT_ERR_ARGUMENT;
     BatchState* B = batch_of(c, batch_id);
     if (!B) return fail(c, OCT_ERR_ARGUMENT, "unknown batch");
     if (leaf_size < 1 || leaf_size > 256) return fail(c, OCT_ERR_ARGUMENT, "leaf size must be in 1..256");
     if (run_leaves < 1 || run_leaves > SP_PLAN_MAX_LEAVES)
         return fail(c, OCT_ERR_ARGUMENT, "leaves per run must be in 1..64");
-    if (leaf_size != B->leaf_size || run_leaves != B->run_leaves || (use_graph != 0) != B->use_graph) {
+    if (run_chunk < 1 || run_chunk > SP_MAX_CHUNK) return fail(c, OCT_ERR_ARGUMENT, "runs per ticket must be in 1..4");
+    if (leaf_size != B->leaf_size || run_leaves != B->run_leaves || run_chunk != B->run_chunk ||
+        (use_graph != 0) != B->use_graph) {
         B->leaf_size = leaf_size;
         B->run_leaves = run_leaves;
+        B->run_chunk = run_chunk;
         B->use_graph = use_graph != 0;
         B->drop_graph();
         B->max_bits = -1;
@@ -1312,6 +1346,7 @@ int oct_batch_run(oct_ctx* c, int batch_id, oct_batch_edges* out) {
             B.n_edges = (int64_t)total;
             B.recounted = hf[9];
             B.recorded = hf[10];
+            B.heavy = hf[11];
             B.replayed = 1;
             done = true;
         }
@@ -1332,6 +1367,7 @@ int oct_batch_run(oct_ctx* c, int batch_id, oct_batch_edges* out) {
     out->replayed = B.replayed;
     out->recounted_leaves = B.recounted;
     out->recorded_leaves = B.recorded;
+    out->heavy_leaves = B.heavy;
     return OCT_OK;
 }
 
@@ -1365,26 +1401,29 @@ int time_device(oct_ctx* c, int reps, F fn, double* ms) {
 }  // extern "C++"
 
 int oct_batch_autotune(oct_ctx* c, int batch_id, const int* leaf_sizes, int n_leaf_sizes, const int* runs,
-                       int n_runs, int reps, double* ms_out, int* best) {
-    if (!c || !leaf_sizes || !runs || n_leaf_sizes < 1 || n_runs < 1 || reps < 1) return OCT_ERR_ARGUMENT;
+                       int n_runs, const int* chunks, int n_chunks, int reps, double* ms_out, int* best) {
+    if (!c || !leaf_sizes || !runs || !chunks || n_leaf_sizes < 1 || n_runs < 1 || n_chunks < 1 || reps < 1)
+        return OCT_ERR_ARGUMENT;
     BatchState* B = batch_of(c, batch_id);
     if (!B) return fail(c, OCT_ERR_ARGUMENT, "unknown batch");
-    const bool graph = B->use_graph;
+    const int graph = B->use_graph ? 1 : 0;
     double best_ms = 1e300;
     int best_k = -1;
     for (int i = 0; i < n_leaf_sizes; ++i)
-        for (int j = 0; j < n_runs; ++j) {
-            const int k = i * n_runs + j;
-            int rc = oct_batch_config(c, batch_id, leaf_sizes[i], runs[j], graph ? 1 : 0);
-            if (rc != OCT_OK) return rc;
-            oct_batch_edges e;
-            double ms = 0.0;
-            rc = time_device(c, reps, [&]() { return oct_batch_run(c, batch_id, &e); }, &ms);
-            if (rc != OCT_OK) return rc;
-            if (ms_out) ms_out[k] = ms;
-            if (ms < best_ms) { best_ms = ms; best_k = k; }
-        }
-    int rc = oct_batch_config(c, batch_id, leaf_sizes[best_k / n_runs], runs[best_k % n_runs], graph ? 1 : 0);
+        for (int j = 0; j < n_runs; ++j)
+            for (int m = 0; m < n_chunks; ++m) {
+                const int k = (i * n_runs + j) * n_chunks + m;
+                int rc = oct_batch_config(c, batch_id, leaf_sizes[i], runs[j], chunks[m], graph);
+                if (rc != OCT_OK) return rc;
+                oct_batch_edges e;
+                double ms = 0.0;
+                rc = time_device(c, reps, [&]() { return oct_batch_run(c, batch_id, &e); }, &ms);
+                if (rc != OCT_OK) return rc;
+                if (ms_out) ms_out[k] = ms;
+                if (ms < best_ms) { best_ms = ms; best_k = k; }
+            }
+    const int bi = best_k / (n_runs * n_chunks), bj = (best_k / n_chunks) % n_runs, bm = best_k % n_chunks;
+    int rc = oct_batch_config(c, batch_id, leaf_sizes[bi], runs[bj], chunks[bm], graph);
     if (rc != OCT_OK) return rc;
     if (best) *best = best_k;
     return OCT_OK;
@@ -1398,33 +1437,44 @@ int oct_set_run_leaves(oct_ctx* c, int run_leaves) {
     return OCT_OK;
 }
 
+int oct_set_run_chunk(oct_ctx* c, int run_chunk) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (run_chunk < 1 || run_chunk > SP_MAX_CHUNK) return fail(c, OCT_ERR_ARGUMENT, "runs per ticket must be in 1..4");
+    c->run_chunk = run_chunk;
+    return OCT_OK;
+}
+
 int oct_autotune(oct_ctx* c, int q, int t, int mode, const int* leaf_sizes, int n_leaf_sizes, const int* runs,
-                 int n_runs, int reps, double* ms_out, int* best) {
-    if (!c || !leaf_sizes || !runs || n_leaf_sizes < 1 || n_runs < 1 || reps < 1) return OCT_ERR_ARGUMENT;
+                 int n_runs, const int* chunks, int n_chunks, int reps, double* ms_out, int* best) {
+    if (!c || !leaf_sizes || !runs || !chunks || n_leaf_sizes < 1 || n_runs < 1 || n_chunks < 1 || reps < 1)
+        return OCT_ERR_ARGUMENT;
     if (!find_pair(c, q, t)) return fail(c, OCT_ERR_STATE, "autotune: pair not registered");
     double best_ms = 1e300;
     int best_k = -1;
     for (int i = 0; i < n_leaf_sizes; ++i)
-        for (int j = 0; j < n_runs; ++j) {
-            const int k = i * n_runs + j;
-            int rc = oct_set_leaf_size(c, leaf_sizes[i]);
-            if (rc != OCT_OK) return rc;
-            rc = oct_set_run_leaves(c, runs[j]);
-            if (rc != OCT_OK) return rc;
-            oct_edges e;
-            double ms = 0.0;
-            // one step: rebuild the dirty sets and the edges of the pair
-            rc = time_device(c, reps, [&]() {
-                for (auto& s : c->sets) s.dirty = true;
-                int r = oct_run(c);
-                return r == OCT_OK ? oct_build_edges(c, q, t, mode, OCT_ORDER_LEAF, &e) : r;
-            }, &ms);
-            if (rc != OCT_OK) return rc;
-            if (ms_out) ms_out[k] = ms;
-            if (ms < best_ms) { best_ms = ms; best_k = k; }
-        }
-    int rc = oct_set_leaf_size(c, leaf_sizes[best_k / n_runs]);
-    if (rc == OCT_OK) rc = oct_set_run_leaves(c, runs[best_k % n_runs]);
+        for (int j = 0; j < n_runs; ++j)
+            for (int m = 0; m < n_chunks; ++m) {
+                const int k = (i * n_runs + j) * n_chunks + m;
+                int rc = oct_set_leaf_size(c, leaf_sizes[i]);
+                if (rc == OCT_OK) rc = oct_set_run_leaves(c, runs[j]);
+                if (rc == OCT_OK) rc = oct_set_run_chunk(c, chunks[m]);
+                if (rc != OCT_OK) return rc;
+                oct_edges e;
+                double ms = 0.0;
+                // one step: rebuild the dirty sets and the edges of the pair
+                rc = time_device(c, reps, [&]() {
+                    for (auto& s : c->sets) s.dirty = true;
+                    int r = oct_run(c);
+                    return r == OCT_OK ? oct_build_edges(c, q, t, mode, OCT_ORDER_LEAF, &e) : r;
+                }, &ms);
+                if (rc != OCT_OK) return rc;
+                if (ms_out) ms_out[k] = ms;
+                if (ms < best_ms) { best_ms = ms; best_k = k; }
+            }
+    const int bi = best_k / (n_runs * n_chunks), bj = (best_k / n_chunks) % n_runs, bm = best_k % n_chunks;
+    int rc = oct_set_leaf_size(c, leaf_sizes[bi]);
+    if (rc == OCT_OK) rc = oct_set_run_leaves(c, runs[bj]);
+    if (rc == OCT_OK) rc = oct_set_run_chunk(c, chunks[bm]);
     if (rc != OCT_OK) return rc;
     if (best) *best = best_k;
     return OCT_OK;

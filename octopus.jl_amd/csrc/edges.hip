@@ -22,6 +22,20 @@ static_assert(1 + 7 * OCT_MAX_LEVEL <= 160, "LIFO walk bound (src/cpu/build_cpu.
 constexpr int LL_SAFE_STACK = 160;  // entries of the safe walk (scratch)
 constexpr int LL_FAST_STACK = 32;   // entries of the fast walk (LDS, 4 bytes each)
 constexpr int LL_THREADS = 256;
+constexpr int EW_WAVES = 4;     // waves per block, each owns one query leaf at a time
+constexpr int EW_STAGE = 256;   // edges one wave stages in LDS (6 KB)
+constexpr int EW_TILE = 4;      // candidate chunks of 64 held in registers per tile
+// A query leaf the single pass records in one sweep has at most 64 queries,
+// 64 candidate leaves and REC_MAX_CAND candidate points, of which at most
+// one register tile (256) passes the point-box filter.
+constexpr int REC_MAX_CAND = 1024;
+// Heavy leaf: one that will not be recorded in one sweep (more than 64
+// queries or candidate leaves, an estimated filtered candidate count above
+// one tile) or whose estimated query x candidate pairs exceed HEAVY_PAIRS
+// (about four hit lists of hits at the usual hit rate of a quarter).  The
+// single pass takes their counts from a parallel count pass run before it,
+// so no run's aggregate waits on them.
+constexpr float HEAVY_PAIRS = 4096.f;
 
 // Direction of box c seen from box q, one of 27 codes sx + 3 sy + 9 sz with
 // s = 0 (below), 1 (level), 2 (above) per axis: the centres must differ by
@@ -46,11 +60,26 @@ __device__ inline uint8_t dir_code(const float4& qlo, const float4& qhi, const f
 // Appends a passing child: a leaf goes to the list (and its direction code to
 // dout when asked), an internal node's child range to the stack.  Returns
 // false when the stack has no room.
-template <bool DIRS, typename Push>
-__device__ inline bool ll_visit(const float4& qlo, const float4& qhi, const NodeRec& ch, float r_sq, int2* out,
-                                uint8_t* dout, int cap, int& cnt, Push push) {
+// Fraction of [lo, hi] inside [a, b] (1 for a degenerate extent).
+__device__ inline float span_frac(float lo, float hi, float a, float b) {
+    const float ext = hi - lo;
+    const float ov = fminf(hi, b) - fmaxf(lo, a);
+    return ext > 0.f ? fminf(fmaxf(__fdividef(ov, ext), 0.f), 1.f) : 1.f;
+}
+
+// Estimated candidates of a leaf that pass the point-box filter: the part of
+// its box inside the query box grown by r, times its point count.
+__device__ inline float filtered_estimate(const float4& qlo, const float4& qhi, const NodeRec& ch, float r) {
+    return (float)rec_b(ch) * span_frac(ch.lo.x, ch.hi.x, qlo.x - r, qhi.x + r) *
+           span_frac(ch.lo.y, ch.hi.y, qlo.y - r, qhi.y + r) * span_frac(ch.lo.z, ch.hi.z, qlo.z - r, qhi.z + r);
+}
+
+template <bool DIRS, bool HEAVY, typename Push>
+__device__ inline bool ll_visit(const float4& qlo, const float4& qhi, const NodeRec& ch, float r_sq, float r,
+                                int2* out, uint8_t* dout, int cap, int& cnt, float& est, Push push) {
     if (box_box_d2(qlo, qhi, ch.lo, ch.hi) > r_sq) return true;
     if (rec_b(ch) > 0) {
+        if (HEAVY) est += filtered_estimate(qlo, qhi, ch, r);
         if (cnt < cap) {
             out[cnt] = make_int2(rec_a(ch), rec_b(ch));
             if (DIRS) dout[cnt] = dir_code(qlo, qhi, ch.lo, ch.hi);
@@ -72,13 +101,14 @@ __device__ inline bool ll_visit(const float4& qlo, const float4& qhi, const Node
 //         21-level tree (src/cpu/build_cpu.jl:25-43), so it never runs out.
 // A list longer than cap is counted in full and flagged through flags[0] (the
 // host regrows cap to that count and reruns).
-template <bool FAST, bool DIRS>
+template <bool FAST, bool DIRS, bool HEAVY>
 __global__ void __launch_bounds__(LL_THREADS)
 k_leaf_lists(const NodeRec* __restrict__ qleaves,
              const BuildParams* __restrict__ qprm, const NodeRec* __restrict__ tnodes,
              const BuildParams* __restrict__ tprm, float r_sq0, int cap, int2* __restrict__ lists,
              uint8_t* __restrict__ ldir, int32_t* __restrict__ lcount, uint32_t* __restrict__ flags,
-             SetLookup sets) {
+             SetLookup sets, uint8_t* __restrict__ lheavy, int32_t* __restrict__ heavy_list,
+             uint32_t* __restrict__ n_heavy) {
     __shared__ uint32_t fstk[FAST ? LL_FAST_STACK : 1][LL_THREADS];
     const int32_t nleaves = qprm->n_leaves;
     const int32_t root0 = tprm->branch_root;
@@ -96,10 +126,13 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
         int2* out = lists + (int64_t)r * cap;
         uint8_t* dout = DIRS ? ldir + (int64_t)r * cap : nullptr;
         int cnt = 0;
+        float est = 0.f;  // HEAVY: estimated filtered candidate points
+        const float rad = HEAVY ? sqrtf(r_sq) : 0.f;
         bool full = false;
         const NodeRec rr = tnodes[root];
         if (box_box_d2(qa.lo, qa.hi, rr.lo, rr.hi) <= r_sq) {
             if (rec_b(rr) > 0) {
+                if (HEAVY) est += filtered_estimate(qa.lo, qa.hi, rr, rad);
                 if (cnt < cap) {
                     out[cnt] = make_int2(rec_a(rr), rec_b(rr));
                     if (DIRS) dout[cnt] = dir_code(qa.lo, qa.hi, rr.lo, rr.hi);
@@ -123,7 +156,7 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
                         if (c < nch) ch[c] = tnodes[first + c];
 #pragma unroll
                     for (int c = 0; c < 8; ++c)
-                        if (c < nch && !ll_visit<DIRS>(qa.lo, qa.hi, ch[c], r_sq, out, dout, cap, cnt, push)) full = true;
+                        if (c < nch && !ll_visit<DIRS, HEAVY>(qa.lo, qa.hi, ch[c], r_sq, rad, out, dout, cap, cnt, est, push)) full = true;
                 }
             } else {
                 int2 stack[LL_SAFE_STACK];
@@ -142,21 +175,33 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
                         if (c < ent.y) ch[c] = tnodes[ent.x + c];
 #pragma unroll
                     for (int c = 0; c < 8; ++c)
-                        if (c < ent.y && !ll_visit<DIRS>(qa.lo, qa.hi, ch[c], r_sq, out, dout, cap, cnt, push)) full = true;
+                        if (c < ent.y && !ll_visit<DIRS, HEAVY>(qa.lo, qa.hi, ch[c], r_sq, rad, out, dout, cap, cnt, est, push)) full = true;
                 }
             }
         }
         lcount[r] = cnt;
         if (cnt > cap) atomicMax(&flags[0], (uint32_t)cnt);  // the host regrows cap to this
         if (full) atomicOr(&flags[1], 1u);                   // the host reruns in safe mode
+        if (HEAVY) {  // heavy leaves: flagged, and listed compactly (one atomic per wave)
+            const int qc = rec_b(qa);
+            const bool heavy = qc > OCT_WAVE || cnt > OCT_WAVE || est > (float)(EW_TILE * OCT_WAVE) ||
+                               (float)qc * est > HEAVY_PAIRS;
+            lheavy[r] = heavy ? 1 : 0;
+            const uint64_t hm = ballot(heavy);
+            if (hm) {
+                const int lead = __ffsll((long long)hm) - 1;
+                uint32_t base = 0;
+                if (lane_id() == lead) base = atomicAdd(n_heavy, (uint32_t)__popcll(hm));
+                base = (uint32_t)__shfl((int)base, lead, OCT_WAVE);
+                const uint32_t idx = base + (uint32_t)mbcnt(hm);
+                if (heavy && idx < (uint32_t)nleaves) heavy_list[idx] = r;
+            }
+        }
     }
 }
 
 // ---------------------------------------------------------------- stage 2
 
-constexpr int EW_WAVES = 4;     // waves per block, each owns one query leaf at a time
-constexpr int EW_STAGE = 256;   // edges one wave stages in LDS (6 KB)
-constexpr int EW_TILE = 4;      // candidate chunks of 64 held in registers per tile
 
 // Per-wave LDS of the edge passes.
 struct WaveLds {
@@ -279,7 +324,7 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
                     const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
                     const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
                     const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
-        const int32_t ix = self_pair ? i : -2;  // -2 matches no point: only the self pair drops j == i
+                    const int32_t ix = self_pair ? i : -2;  // -2 matches no point: only the self pair drops j == i
                     uint32_t run = MODE == EDGE_FILL ? (uint32_t)__builtin_amdgcn_readlane((int)my, qi) : 0u;
                     const uint32_t run0 = run;
 #pragma unroll
@@ -290,7 +335,7 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
                         const float dy = qy - tp[u].y;
                         const float dz = qz - tp[u].z;
                         const float d2 = dx * dx + dy * dy + dz * dz;
-                                                const bool in_r = d2 <= r_sq;
+                        const bool in_r = d2 <= r_sq;
                         const bool other = j != ix;
                         const uint64_t m = ballot(in_r) & ballot(other);  // compare masks ANDed, no branch
                         if (MODE == EDGE_FILL && (in_r & other)) {
@@ -345,7 +390,6 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
 // sorted position, target sorted position) while they fit; the return value
 // is the exact hit count either way (a count above `room` means the list
 // could not hold them all).
-constexpr int REC_MAX_CAND = 4 * EW_TILE * OCT_WAVE;  // candidate points a recorded leaf may have before filtering
 constexpr uint32_t REC_NOT_TILE = 0xffffffffu;        // the filtered candidates do not fit one register tile
 
 __device__ inline bool recordable_leaf(int32_t qc, int32_t ncand, int total) {
@@ -414,7 +458,7 @@ template <bool WIDE>
 __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, float4 qlo, float4 qhi, int32_t ncand, int2 lf0,
                                              const float4* __restrict__ qsc, const float4* __restrict__ tsc,
                                              float r_sq, int self_pair, const WaveLds& L, int lane, uint32_t* hits,
-                                             uint32_t room) {
+                                             uint32_t room, uint32_t* __restrict__ qcount) {
     const int nqb = qc;
     float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);
     if (lane < nqb) myq = qsc[qf + lane];
@@ -432,7 +476,9 @@ __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, float4 qlo,
     if (got < 0) return REC_NOT_TILE;
     const int nu = (got + OCT_WAVE - 1) / OCT_WAVE;
     uint32_t run = 0;
+    uint32_t mine = 0;  // hits of this lane's query
     for (int qi = 0; qi < nqb; ++qi) {
+        const uint32_t run0 = run;
         const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
         const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
         const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
@@ -446,7 +492,7 @@ __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, float4 qlo,
             const float dy = qy - tp[u].y;
             const float dz = qz - tp[u].z;
             const float d2 = dx * dx + dy * dy + dz * dz;
-                        const bool in_r = d2 <= r_sq;
+            const bool in_r = d2 <= r_sq;
             const bool other = j != ix;
             const uint64_t m = ballot(in_r) & ballot(other);  // compare masks ANDed, no branch
             if (in_r & other) {
@@ -458,7 +504,11 @@ __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, float4 qlo,
             }
             run += (uint32_t)__popcll(m);
         }
+        if (lane == qi) mine = run - run0;
     }
+    // a list that could not hold every hit: the per-query counts let the
+    // caller write this leaf with the FILL body without another count sweep
+    if (run > room && lane < nqb) qcount[qf + lane] = mine;
     return run;
 }
 
@@ -566,6 +616,59 @@ k_edges(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, con
     }
 }
 
+// Radius and output index base of the query leaf whose first sorted position
+// is qf: the call's own, or in a forest those of the leaf's set (indices stay
+// local to each set).
+__device__ inline void leaf_set_params(const SetLookup& sets, int32_t qf, float r_sq, float inv_r, int index_base0,
+                                       float& lr_sq, float& linv_r, int& libase) {
+    lr_sq = r_sq;
+    linv_r = inv_r;
+    libase = index_base0;
+    if (sets.batched) {
+        const int s = key_set(sets.qsk[qf], 1);
+        const float rr = sets.sprm[s].radius;
+        lr_sq = rr * rr;
+        linv_r = 1.0f / rr;
+        libase = index_base0 - (int)sets.set_off[s];
+    }
+}
+
+// ------------------------------------------------------------ heavy leaves
+
+// Exact counts of the heavy query leaves that k_leaf_lists listed, one wave
+// per leaf, before the single pass: per-query counts into qcount, the leaf's
+// total into lhits[r].  Same sweep as the two-pass COUNT kernel.
+__global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
+k_edges_heavy(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
+              const BuildParams* __restrict__ qprm, const float4* __restrict__ tsc, const int2* __restrict__ lists,
+              const int32_t* __restrict__ lcount, int cap, float r_sq, float inv_r, int self_pair,
+              const int32_t* __restrict__ heavy_list, const uint32_t* __restrict__ n_heavy,
+              uint32_t* __restrict__ qcount, uint32_t* __restrict__ lhits, SetLookup sets) {
+    __shared__ EdgeLds<EDGE_COUNT> lds;
+    const int wv = threadIdx.x / OCT_WAVE;
+    const int lane = lane_id();
+    const WaveLds L = lds.wave(wv);
+    const int32_t nleaves = qprm->n_leaves;
+    const int32_t nh = (int32_t)min(*n_heavy, (uint32_t)max(nleaves, 0));
+    const int32_t nwaves = (int32_t)(gridDim.x * EW_WAVES);
+    for (int32_t i = (int32_t)blockIdx.x * EW_WAVES + wv; i < nh; i += nwaves) {
+        const int32_t r = __builtin_amdgcn_readfirstlane(heavy_list[i]);
+        if (r < 0 || r >= nleaves) continue;  // wave-uniform; k_leaf_lists writes only leaves
+        const NodeRec qa = qleaves[r];
+        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+        const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+        const int2* lst = lists + (int64_t)r * cap;
+        const int2 lf0 = lst[lane < ncand ? lane : 0];
+        float lr_sq, linv_r;
+        int libase;
+        leaf_set_params(sets, qf, r_sq, inv_r, 0, lr_sq, linv_r, libase);
+        const uint32_t c = leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r,
+                                                 self_pair, 0, qcount, nullptr, EdgeOut{}, libase, L, lane);
+        if (lane == 0) lhits[r] = c;
+    }
+}
+
 // ------------------------------------------------------------ single pass
 
 constexpr int SP_MAX_LEAVES = SP_PLAN_MAX_LEAVES;  // leaves one wave may own under a plan
@@ -577,6 +680,60 @@ constexpr uint32_t SP_SPIN_LIMIT = 1u << 22;   // bounded wait: a stuck look-bac
 constexpr int SP_HIT_WORDS = 1024;                 // hit list of one wave: 4 KB
 constexpr uint32_t SP_STAGE = SP_HIT_WORDS * 4 / 24;  // FILL stage laid over the hit list: 170 edges
 static_assert(SP_PLAN_BUDGET == SP_HIT_WORDS, "the plan packs one wave's compact hit list");
+
+// Run hand-out of the single pass.  A block takes `chunk` (1..EW_WAVES)
+// consecutive runs per global ticket (neighbouring runs share candidate
+// points in the CU's caches, and one atomic serves several runs); its waves
+// take those runs one at a time as each becomes free, with no block barrier,
+// so a slow run holds back no other wave.  A larger chunk gives more cache
+// reuse; a smaller one starts every run as soon as it is handed out (a run
+// that waits for a wave of its block delays the look-back of all later runs),
+// which suits uneven runs.  The LDS word q holds (ticket << SP_Q_SHIFT) | runs
+// handed out of it.  The wave that finds the count at `chunk` takes the next
+// ticket, keeps its first run and installs it with count 1; a wave that finds
+// a larger count (at most EW_WAVES - 1 of them, each adding once) waits for
+// the ticket to change.  Returns the run (ticket * chunk + k); runs at or
+// past the end mean that no work is left.  chunk 1 takes the global ticket
+// directly.
+constexpr int SP_Q_SHIFT = 8;
+constexpr unsigned long long SP_Q_NONE = 0xffffffffull;  // no ticket yet: never a real one
+
+__device__ inline int64_t take_run(unsigned long long* q, uint32_t* ticket, int lane, int chunk) {
+    if (chunk == 1) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(ticket, 1u);
+        return (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+    }
+    for (;;) {
+        unsigned long long old = 0;
+        if (lane == 0) old = atomicAdd(q, 1ull);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)old);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(old >> 32));
+        old = ((unsigned long long)hi << 32) | lo;
+        const unsigned long long tk = old >> SP_Q_SHIFT;
+        const uint32_t k = (uint32_t)(old & ((1u << SP_Q_SHIFT) - 1u));
+        if (k < (uint32_t)chunk) return (int64_t)tk * chunk + k;
+        if (k == (uint32_t)chunk) {  // used up: this wave fetches the next ticket
+            uint32_t t = 0;
+            if (lane == 0) {
+                t = atomicAdd(ticket, 1u);
+                __hip_atomic_store(q, ((unsigned long long)t << SP_Q_SHIFT) | 1ull, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+            return (int64_t)t * chunk;
+        }
+        // another wave is fetching the next ticket
+        for (;;) {
+            unsigned long long cur = 0;
+            if (lane == 0) cur = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t chi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(cur >> 32));
+            const uint32_t clo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)cur);
+            if (((((unsigned long long)chi << 32) | clo) >> SP_Q_SHIFT) != tk) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
 
 // Decoupled look-back by one whole wave: lanes read the states of the 64
 // tiles before `hi` at once, the nearest tile with a published inclusive
@@ -616,13 +773,14 @@ __device__ inline uint64_t wave_look_back(const uint64_t* tile_state, int32_t t,
 // leaves, which is also its output tile: leaves_per_run leaves per run, or
 // the runs of a plan (wave_first[w] .. wave_first[w + 1], at most
 // SP_MAX_LEAVES, *plan_waves runs) that packs the predicted hits of a run into
-// the wave's hit list.  Blocks take EW_WAVES consecutive runs per ticket
-// until none is left; the waves then work independently.  Phase A sweeps each
+// the wave's hit list.  Waves take runs through take_run (run_chunk runs per
+// block ticket) until none is left.  Phase A sweeps each
 // leaf once: a leaf that fits one register tile records its hits in the
 // wave's LDS hit list (query-grouped, already in output order) and counts
-// them; any other leaf, or one whose hits no longer fit the list, is counted
-// with the COUNT body and recounted in phase B (stats[0] counts those,
-// stats[1] the recorded ones).  lhits (optional) receives every leaf's exact
+// them; a leaf whose hits no longer fit the list keeps its per-query counts
+// from that sweep, any other leaf is counted with the COUNT body, and both
+// are swept again in phase B (stats[0] counts those, stats[1] the recorded
+// ones).  lhits (optional) receives every leaf's exact
 // hit count.  The run's dense output offset comes from a decoupled look-back
 // over the earlier runs' states.  Phase B writes the recorded hits with
 // contiguous stores, then re-sweeps the other leaves (the exact recount, with
@@ -632,9 +790,15 @@ __device__ inline uint64_t wave_look_back(const uint64_t* tile_state, int32_t t,
 // the host then reruns with room for the total the last run reports in
 // flags[2..3].  flags[1] reports a look-back that gave up (1) or a run longer
 // than SP_MAX_LEAVES (2); the host then takes the exact path.  A look-back
-// only waits on lower runs, which belong to this block's lower waves or to
-// tickets taken earlier by running blocks, so every wait ends (and is bounded
-// anyway).  WIDE: 8-byte hits (any point count; half the list).
+// only waits on lower runs.  Runs are handed out in increasing order, and a
+// run a block holds but has not started is above every run its waves hold, so
+// the lowest run without an aggregate is either in a phase A (which waits on
+// nothing) or next in line at a block whose waves only wait on lower, already
+// published runs: every wait ends (and is bounded anyway).  With lheavy, the leaves it flags were counted ahead by
+// k_edges_heavy (totals in lhits, per-query counts in qcount): phase A takes
+// their totals as they are, so a run publishes its aggregate without waiting
+// on its heaviest leaves, and phase B sweeps them once, with the FILL body.
+// WIDE: 8-byte hits (any point count; half the list).
 template <bool WIDE>
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
 k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
@@ -644,29 +808,15 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
                const int32_t* __restrict__ plan_waves, uint32_t* __restrict__ lhits,
                uint64_t* __restrict__ tile_state, uint32_t* __restrict__ ticket, uint32_t* __restrict__ flags,
                uint32_t* __restrict__ stats, uint64_t cap_edges, EdgeOut out, int index_base0,
-               int leaves_per_run, SetLookup sets) {
+               int leaves_per_run, int run_chunk, SetLookup sets, const uint8_t* __restrict__ lheavy) {
     constexpr uint32_t HITS = WIDE ? SP_HIT_WORDS / 2 : SP_HIT_WORDS;
-    // per-leaf radius and output index base: the kernel's, or the leaf's set's
-    // in a forest (indices stay local to each set)
-    auto leaf_params = [&](int32_t qf, float& lr_sq, float& linv_r, int& libase) {
-        lr_sq = r_sq;
-        linv_r = inv_r;
-        libase = index_base0;
-        if (sets.batched) {
-            const int s = key_set(sets.qsk[qf], 1);
-            const float rr = sets.sprm[s].radius;
-            lr_sq = rr * rr;
-            linv_r = 1.0f / rr;
-            libase = index_base0 - (int)sets.set_off[s];
-        }
-    };
     __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
     __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
     __shared__ uint32_t s_hits[EW_WAVES][SP_HIT_WORDS];
     __shared__ uint32_t s_cnt[EW_WAVES][SP_MAX_LEAVES];
     __shared__ int32_t s_qf[EW_WAVES][SP_MAX_LEAVES];
     __shared__ int32_t s_cand[EW_WAVES][EW_TILE * OCT_WAVE];
-    __shared__ uint32_t s_tile;
+    __shared__ unsigned long long s_q;  // run hand-out: ticket << SP_Q_SHIFT | runs taken from it
     __shared__ uint32_t s_stat[2];
     const int wv = threadIdx.x / OCT_WAVE;
     const int lane = lane_id();
@@ -678,16 +828,13 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
                     reinterpret_cast<float*>(stage + 5 * SP_STAGE), SP_STAGE, s_cand[wv]};
     const int32_t nleaves = qprm->n_leaves;
     const int32_t nruns = wave_first ? *plan_waves : (nleaves + leaves_per_run - 1) / leaves_per_run;
-    const int32_t ntickets = (nruns + EW_WAVES - 1) / EW_WAVES;
+    if (threadIdx.x == 0) s_q = (SP_Q_NONE << SP_Q_SHIFT) | (unsigned long long)run_chunk;
+    __syncthreads();
     uint32_t n_recount = 0, n_record = 0;  // this wave's, over all its runs
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-        __syncthreads();
-        const int32_t tk = (int32_t)s_tile;
-        __syncthreads();  // everyone has the ticket before thread 0 takes the next one
-        if (tk >= ntickets) break;
-        const int32_t w = tk * EW_WAVES + wv;  // this wave's run and output tile
-        if (w >= nruns) continue;               // wave-uniform
+        const int64_t run = take_run(&s_q, ticket, lane, run_chunk);
+        if (run >= nruns) break;
+        const int32_t w = (int32_t)run;  // this wave's run and output tile
         int32_t r_lo, r_hi;
         if (wave_first) {
             r_lo = wave_first[w];
@@ -708,39 +855,41 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
         uint32_t hfill = 0, wtot = 0, nrec = 0;
         for (int32_t r = r_lo; r < r_hi; ++r) {
             const int k = r - r_lo;
-            const NodeRec qa = qleaves[r];
-            const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
-            const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
-            const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
-            const int2* lst = lists + (int64_t)r * cap;
-            const int2 lf0 = lst[lane < ncand ? lane : 0];
-            const int total = __builtin_amdgcn_readfirstlane(wave_sum(lane < ncand ? lf0.y : 0));
-            float lr_sq, linv_r;
-            int libase;
-            leaf_params(qf, lr_sq, linv_r, libase);
-            uint32_t c = REC_NOT_TILE;
-            if (recordable_leaf(qc, ncand, total))
-                c = leaf_sweep_record<WIDE>(qf, qc, qa.lo, qa.hi, ncand, lf0, qsc, tsc, lr_sq, self_pair, L, lane,
-                                            WIDE ? hits + 2 * hfill : hits + hfill, HITS - hfill);
-            if (c != REC_NOT_TILE) {
-                if (c <= HITS - hfill) {
-                    recorded |= 1ull << k;
-                    hfill += c;
-                    ++nrec;
-                } else {
-                    // the list is full: count per query for the recount in phase B
-                    leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r, self_pair,
-                                          0, qcount, nullptr, out, libase, L, lane);
-                }
+            const bool pre = lheavy && __builtin_amdgcn_readfirstlane(lheavy[r]) != 0;
+            uint32_t c;
+            if (pre) {
+                c = __builtin_amdgcn_readfirstlane(lhits[r]);  // counted by the heavy-leaf pass
             } else {
-                c = leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r, self_pair,
-                                          0, qcount, nullptr, out, libase, L, lane);
+                const NodeRec qa = qleaves[r];
+                const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+                const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+                const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+                const int2* lst = lists + (int64_t)r * cap;
+                const int2 lf0 = lst[lane < ncand ? lane : 0];
+                const int total = __builtin_amdgcn_readfirstlane(wave_sum(lane < ncand ? lf0.y : 0));
+                float lr_sq, linv_r;
+                int libase;
+                leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
+                c = REC_NOT_TILE;
+                if (recordable_leaf(qc, ncand, total))
+                    c = leaf_sweep_record<WIDE>(qf, qc, qa.lo, qa.hi, ncand, lf0, qsc, tsc, lr_sq, self_pair, L,
+                                                lane, WIDE ? hits + 2 * hfill : hits + hfill, HITS - hfill, qcount);
+                if (c != REC_NOT_TILE) {
+                    if (c <= HITS - hfill) {  // else the list is full: qcount holds the per-query counts
+                        recorded |= 1ull << k;
+                        hfill += c;
+                        ++nrec;
+                    }
+                } else {
+                    c = leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r,
+                                              self_pair, 0, qcount, nullptr, out, libase, L, lane);
+                }
+                if (lane == 0) {
+                    qfw[k] = qf;
+                    if (lhits) lhits[r] = c;
+                }
             }
-            if (lane == 0) {
-                cntw[k] = c;
-                qfw[k] = qf;
-                if (lhits) lhits[r] = c;
-            }
+            if (lane == 0) cntw[k] = c;
             wtot += c;
         }
         n_recount += (uint32_t)(r_hi - r_lo) - nrec;
@@ -772,7 +921,7 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
             if (recorded & (1ull << k)) {
                 float lr_sq, linv_r;
                 int libase;
-                leaf_params(qfw[k], lr_sq, linv_r, libase);
+                leaf_set_params(sets, qfw[k], r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
                 emit_hits<WIDE>(WIDE ? hits + 2 * hread : hits + hread, c, pos, qfw[k], qsc, tsc, linv_r, out,
                                 libase, lane);
                 hread += c;
@@ -801,7 +950,7 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
                 const int2 lf0 = lst[lane < ncand ? lane : 0];
                 float lr_sq, linv_r;
                 int libase;
-                leaf_params(qf, lr_sq, linv_r, libase);
+                leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
                 leaf_pass<EDGE_FILL>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r, self_pair, 0,
                                      qcount, qoff, out, libase, L, lane);
             }
@@ -832,17 +981,22 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
 
 hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
                              uint8_t* ldir, int32_t* lcount, uint32_t* flags, const SetLookup& sets,
-                             hipStream_t s) {
+                             uint8_t* lheavy, int32_t* heavy_list, uint32_t* n_heavy, hipStream_t s) {
     const int64_t want = ((int64_t)q.n_leaves + LL_THREADS - 1) / LL_THREADS;
     const unsigned g = (unsigned)(want < 1 ? 1 : (want > 65535 ? 65535 : want));
-#define OCT_LL_LAUNCH(F, D)                                                                                 \
-    hipLaunchKernelGGL((k_leaf_lists<F, D>), dim3(g), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes, t.params, \
-                       r_sq, cap, lists, ldir, lcount, flags, sets)
+    if ((lheavy != nullptr) != (heavy_list != nullptr) || (lheavy && !n_heavy)) return hipErrorInvalidValue;
+#define OCT_LL_LAUNCH(F, D, H)                                                                              \
+    hipLaunchKernelGGL((k_leaf_lists<F, D, H>), dim3(g), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes,    \
+                       t.params, r_sq, cap, lists, ldir, lcount, flags, sets, lheavy, heavy_list, n_heavy)
+#define OCT_LL_HEAVY(F, D)               \
+    if (lheavy) OCT_LL_LAUNCH(F, D, true); \
+    else OCT_LL_LAUNCH(F, D, false)
     const bool fast = !safe && t.n_nodes < (1 << 29);
-    if (fast && ldir) OCT_LL_LAUNCH(true, true);
-    else if (fast) OCT_LL_LAUNCH(true, false);
-    else if (ldir) OCT_LL_LAUNCH(false, true);
-    else OCT_LL_LAUNCH(false, false);
+    if (fast && ldir) OCT_LL_HEAVY(true, true);
+    else if (fast) OCT_LL_HEAVY(true, false);
+    else if (ldir) OCT_LL_HEAVY(false, true);
+    else OCT_LL_HEAVY(false, false);
+#undef OCT_LL_HEAVY
 #undef OCT_LL_LAUNCH
     return hipGetLastError();
 }
@@ -875,8 +1029,10 @@ hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2*
                                const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
                                uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint32_t* stats,
                                uint64_t cap_edges, EdgeOut out, int index_base, int leaves_per_run,
-                               const SetLookup& sets, hipStream_t s) {
+                               int run_chunk, const SetLookup& sets, const uint8_t* lheavy, hipStream_t s) {
     if (leaves_per_run < 1 || leaves_per_run > SP_MAX_LEAVES) return hipErrorInvalidValue;
+    if (run_chunk < 1 || run_chunk > EW_WAVES) return hipErrorInvalidValue;
+    if (lheavy && !lhits) return hipErrorInvalidValue;  // the heavy totals live in lhits
     const int64_t runs = wave_first ? q.n_leaves : (q.n_leaves + leaves_per_run - 1) / leaves_per_run;
     const int64_t tickets = (runs + EW_WAVES - 1) / EW_WAVES;
     if (tickets <= 0) return hipSuccess;
@@ -885,12 +1041,25 @@ hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2*
         hipLaunchKernelGGL(k_edges_single<false>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first,
                            plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base,
-                           leaves_per_run, sets);
+                           leaves_per_run, run_chunk, sets, lheavy);
     else
         hipLaunchKernelGGL(k_edges_single<true>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first,
                            plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base,
-                           leaves_per_run, sets);
+                           leaves_per_run, run_chunk, sets, lheavy);
+    return hipGetLastError();
+}
+
+hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
+                              int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
+                              const uint32_t* n_heavy, uint32_t* qcount, uint32_t* lhits, const SetLookup& sets,
+                              hipStream_t s) {
+    // a fixed grid (the count is on the device); idle waves exit at once
+    const int64_t want = ((int64_t)q.n_leaves + EW_WAVES - 1) / EW_WAVES;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, 2048));
+    hipLaunchKernelGGL(k_edges_heavy, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
+                       t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, heavy_list, n_heavy, qcount, lhits,
+                       sets);
     return hipGetLastError();
 }
 

@@ -214,9 +214,19 @@ struct SetLookup {
     const BuildParams* sprm;  // per-set parameters
 };
 // ldir (optional): one direction code per list entry, for the predictor.
+// lheavy (optional): per query leaf, 1 when the single pass cannot record it
+// in one sweep ("heavy": more than 64 queries or candidate leaves, or more
+// than 1024 candidate points); heavy_list[0 .. *n_heavy) lists those leaves
+// in no particular order (*n_heavy must be zero before the call).
 hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
                              uint8_t* ldir, int32_t* lcount, uint32_t* flags, const SetLookup& sets,
-                             hipStream_t s);
+                             uint8_t* lheavy, int32_t* heavy_list, uint32_t* n_heavy, hipStream_t s);
+// Exact counts of the listed heavy leaves ahead of the single pass: per-query
+// counts into qcount, leaf totals into lhits.
+hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
+                              int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
+                              const uint32_t* n_heavy, uint32_t* qcount, uint32_t* lhits, const SetLookup& sets,
+                              hipStream_t s);
 hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const int2* lists,
                         const int32_t* lcount, int cap, float r_sq, float inv_r, int self_pair,
                         int by_receiver, uint32_t* qcount, const uint32_t* qoff, uint32_t* lhits, int64_t n_edges,
@@ -227,13 +237,17 @@ int64_t single_tiles(int64_t n_leaves);
 // malformed plan (2), flags[2..3] 64-bit edge total (written by the last
 // tile); stats[0] recounted leaves, stats[1] recorded leaves.  wave_first /
 // plan_waves: optional plan of leaf runs per wave (null: fixed runs).
+// lheavy (optional, needs lhits): leaves launch_edges_heavy counted already.
+// run_chunk: runs a block takes per ticket (1..SP_MAX_CHUNK).
 hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
                                int cap, float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
                                const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
                                uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint32_t* stats,
                                uint64_t cap_edges, EdgeOut out, int index_base, int leaves_per_run,
-                               const SetLookup& sets, hipStream_t s);
-constexpr int SP_DEFAULT_RUN = 4;  // leaves per run of the single pass without a plan
+                               int run_chunk, const SetLookup& sets, const uint8_t* lheavy, hipStream_t s);
+constexpr int SP_DEFAULT_RUN = 4;    // leaves per run of the single pass without a plan
+constexpr int SP_DEFAULT_CHUNK = 4;  // runs per block ticket of the single pass
+constexpr int SP_MAX_CHUNK = 4;      // at most one run per wave of a block
 
 // Per-leaf count predictor (see predictor.hip).  Parameter block layout in
 // floats: feature mean and std, then W1 (29 x 64, row = input), b1, W2

@@ -370,29 +370,45 @@ def set_run_leaves(tns: TNS, run_leaves: int):
     tns._check(tns._lib.oct_set_run_leaves(tns._c(), int(run_leaves)))
 
 
+def set_run_chunk(tns: TNS, run_chunk: int):
+    """Single-pass runs a block takes per ticket (1..4, default 4)."""
+    tns._check(tns._lib.oct_set_run_chunk(tns._c(), int(run_chunk)))
+
+
+def _tune_table(ls, rs, cs, ms, k):
+    """(leaf, run, chunk) of candidate index k and the table of all times."""
+    def key(i):
+        return (int(ls[i // (rs.shape[0] * cs.shape[0])]), int(rs[(i // cs.shape[0]) % rs.shape[0]]),
+                int(cs[i % cs.shape[0]]))
+
+    return key(k), {key(i): float(ms[i]) for i in range(ms.shape[0])}
+
+
 def autotune(tns: TNS, q: int, t: int, mode: str = "learned", leaf_sizes=(8, 16, 32, 64), runs=(1, 2, 4, 8),
-             reps: int = 3) -> dict:
+             chunks=(1, 4), reps: int = 3) -> dict:
     """Times run! + build_edges(q, t, mode) for every (leaf size, leaves per
-    run) candidate and keeps the fastest (the reference fixes the leaf size at
-    32, src/types.jl:177).  Returns {"best": (leaf, run), "ms": {...}}; the
-    edges never depend on the choice."""
+    run, runs per block ticket) candidate and keeps the fastest (the
+    reference fixes the leaf size at 32, src/types.jl:177).  Returns
+    {"best": (leaf, run, chunk), "ms": {...}}; the edges never depend on the
+    choice."""
     import numpy as np
 
     if mode not in _MODES:
         raise ArgumentError(f"mode must be one of {sorted(_MODES)}")
     ls = np.ascontiguousarray(np.asarray(leaf_sizes, dtype=np.int32))
     rs = np.ascontiguousarray(np.asarray(runs, dtype=np.int32))
-    ms = np.zeros(ls.shape[0] * rs.shape[0], dtype=np.float64)
+    cs = np.ascontiguousarray(np.asarray(chunks, dtype=np.int32))
+    ms = np.zeros(ls.shape[0] * rs.shape[0] * cs.shape[0], dtype=np.float64)
     best = ctypes.c_int()
     tns._check(tns._lib.oct_autotune(tns._enter(), int(q), int(t), _MODES[mode], ls.ctypes.data_as(ctypes.c_void_p),
-                                     int(ls.shape[0]), rs.ctypes.data_as(ctypes.c_void_p), int(rs.shape[0]), int(reps),
+                                     int(ls.shape[0]), rs.ctypes.data_as(ctypes.c_void_p), int(rs.shape[0]),
+                                     cs.ctypes.data_as(ctypes.c_void_p), int(cs.shape[0]), int(reps),
                                      ms.ctypes.data_as(ctypes.c_void_p), ctypes.byref(best)))
     tns._leave()
-    k = best.value
-    tns.target_leaf_size = int(ls[k // rs.shape[0]])
+    top, table = _tune_table(ls, rs, cs, ms, best.value)
+    tns.target_leaf_size = top[0]
     tns.dirty = [True] * len(tns.dirty)
-    table = {(int(ls[i // rs.shape[0]]), int(rs[i % rs.shape[0]])): float(ms[i]) for i in range(ms.shape[0])}
-    return {"best": (int(ls[k // rs.shape[0]]), int(rs[k % rs.shape[0]])), "ms": table}
+    return {"best": top, "ms": table}
 
 
 def profile(tns: TNS, enable: bool = True):

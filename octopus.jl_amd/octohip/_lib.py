@@ -86,6 +86,7 @@ class OctBatchEdges(ctypes.Structure):
         ("replayed", ctypes.c_int32),
         ("recounted_leaves", ctypes.c_int64),
         ("recorded_leaves", ctypes.c_int64),
+        ("heavy_leaves", ctypes.c_int64),
     ]
 
 
@@ -110,7 +111,7 @@ class OctEdgeStats(ctypes.Structure):
         ("single_pass", ctypes.c_int32),
         ("used_predictor", ctypes.c_int32),
         ("attempts", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("heavy_leaves", ctypes.c_int32),
     ]
 
 
@@ -156,13 +157,14 @@ SIGNATURES = {
     "oct_edge_stats_get": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OctEdgeStats)]),
     "oct_batch_create": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.POINTER(ctypes.c_int)]),
     "oct_batch_destroy": (ctypes.c_int, [_P, ctypes.c_int]),
-    "oct_batch_config": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "oct_batch_config": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "oct_batch_run": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(OctBatchEdges)]),
-    "oct_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P,
-                                          ctypes.POINTER(ctypes.c_int)]),
+    "oct_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, _P, ctypes.c_int, _P, ctypes.c_int,
+                                          ctypes.c_int, _P, ctypes.POINTER(ctypes.c_int)]),
     "oct_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, _P, ctypes.c_int,
-                                    ctypes.c_int, _P, ctypes.POINTER(ctypes.c_int)]),
+                                    _P, ctypes.c_int, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_int)]),
     "oct_set_run_leaves": (ctypes.c_int, [_P, ctypes.c_int]),
+    "oct_set_run_chunk": (ctypes.c_int, [_P, ctypes.c_int]),
     "oct_mlp_forward": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
     "oct_mlp_gradient": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int, _P, _P]),
 }

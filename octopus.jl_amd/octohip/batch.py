@@ -15,7 +15,8 @@ import numpy as np
 from . import _lib as L
 
 BatchEdges = namedtuple("BatchEdges", ["senders", "receivers", "rel_displacement", "rel_dist_norm",
-                                       "set_offsets", "replayed", "recounted_leaves", "recorded_leaves"])
+                                       "set_offsets", "replayed", "recounted_leaves", "recorded_leaves",
+                                       "heavy_leaves"])
 
 
 class Batch:
@@ -45,9 +46,9 @@ class Batch:
         self.id = bid.value
         self.n_sets = int(self.radii.shape[0])
 
-    def config(self, leaf_size: int = 32, run_leaves: int = 4, use_graph: bool = True):
+    def config(self, leaf_size: int = 32, run_leaves: int = 4, use_graph: bool = True, run_chunk: int = 4):
         self.tns._check(self.tns._lib.oct_batch_config(self.tns._c(), self.id, int(leaf_size), int(run_leaves),
-                                                       int(bool(use_graph))))
+                                                       int(run_chunk), int(bool(use_graph))))
 
     def run(self) -> BatchEdges:
         """run! + build_edges of every set; tensors alias library buffers
@@ -68,24 +69,28 @@ class Batch:
             replayed=bool(out.replayed),
             recounted_leaves=int(out.recounted_leaves),
             recorded_leaves=int(out.recorded_leaves),
+            heavy_leaves=int(out.heavy_leaves),
         )
 
-    def autotune(self, leaf_sizes=(8, 12, 16, 24, 32), runs=(1, 2, 4, 8), reps: int = 3) -> dict:
-        """Times every (leaf size, leaves per run) candidate on this batch's
-        current coordinates and keeps the fastest; returns {"best": (leaf,
-        run), "ms": {(leaf, run): ms per step}}."""
+    def autotune(self, leaf_sizes=(8, 12, 16, 24, 32), runs=(1, 2, 4, 8), chunks=(1, 4), reps: int = 3) -> dict:
+        """Times every (leaf size, leaves per run, runs per block ticket)
+        candidate on this batch's current coordinates and keeps the fastest;
+        returns {"best": (leaf, run, chunk), "ms": {(leaf, run, chunk): ms per
+        step}}."""
+        from . import _tune_table
+
         ls = np.ascontiguousarray(np.asarray(leaf_sizes, dtype=np.int32))
         rs = np.ascontiguousarray(np.asarray(runs, dtype=np.int32))
-        ms = np.zeros(ls.shape[0] * rs.shape[0], dtype=np.float64)
+        cs = np.ascontiguousarray(np.asarray(chunks, dtype=np.int32))
+        ms = np.zeros(ls.shape[0] * rs.shape[0] * cs.shape[0], dtype=np.float64)
         best = ctypes.c_int()
         self.tns._check(self.tns._lib.oct_batch_autotune(
             self.tns._enter(), self.id, ls.ctypes.data_as(ctypes.c_void_p), int(ls.shape[0]),
-            rs.ctypes.data_as(ctypes.c_void_p), int(rs.shape[0]), int(reps), ms.ctypes.data_as(ctypes.c_void_p),
-            ctypes.byref(best)))
+            rs.ctypes.data_as(ctypes.c_void_p), int(rs.shape[0]), cs.ctypes.data_as(ctypes.c_void_p),
+            int(cs.shape[0]), int(reps), ms.ctypes.data_as(ctypes.c_void_p), ctypes.byref(best)))
         self.tns._leave()
-        table = {(int(ls[i // rs.shape[0]]), int(rs[i % rs.shape[0]])): float(ms[i]) for i in range(ms.shape[0])}
-        k = best.value
-        return {"best": (int(ls[k // rs.shape[0]]), int(rs[k % rs.shape[0]])), "ms": table}
+        top, table = _tune_table(ls, rs, cs, ms, best.value)
+        return {"best": top, "ms": table}
 
     def close(self):
         if self.id is not None:

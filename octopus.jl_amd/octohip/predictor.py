@@ -102,6 +102,7 @@ def edge_stats(tns, q: int, t: int) -> dict:
         "n_edges": int(s.n_edges),
         "recounted_leaves": int(s.recounted_leaves),
         "recorded_leaves": int(s.recorded_leaves),
+        "heavy_leaves": int(s.heavy_leaves),
         "predicted_total": float(s.predicted_total),
         "single_pass": bool(s.single_pass),
         "used_predictor": bool(s.used_predictor),

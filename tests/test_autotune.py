@@ -33,9 +33,10 @@ def test_pair_autotune_keeps_fastest_and_same_edges(orc):
     octohip.set_active_search(tns, sid, sid)
     octohip.run(tns)
     base = _np_edges(octohip.build_edges(tns, sid, sid, mode="learned"))
-    res = octohip.autotune(tns, sid, sid, mode="learned", leaf_sizes=(8, 32), runs=(1, 4), reps=2)
+    res = octohip.autotune(tns, sid, sid, mode="learned", leaf_sizes=(8, 32), runs=(1, 4), chunks=(1, 2, 4), reps=2)
     print(f"pair autotune {res}", flush=True)
     assert res["best"] == min(res["ms"], key=res["ms"].get)
+    assert len(res["ms"]) == len({k[2] for k in res["ms"]}) * len({k[:2] for k in res["ms"]})
     octohip.run(tns)
     tuned = _np_edges(octohip.build_edges(tns, sid, sid, mode="learned"))
     for a, b in zip(base, tuned):
@@ -60,9 +61,10 @@ def test_batch_autotune_keeps_fastest_and_same_edges():
                                                                 before.rel_displacement.cpu().numpy(),
                                                                 before.rel_dist_norm.cpu().numpy().reshape(-1))))
             for k in range(len(sets))]
-    res = b.autotune(leaf_sizes=(8, 16, 32), runs=(1, 4, 8), reps=2)
+    res = b.autotune(leaf_sizes=(8, 16, 32), runs=(1, 4, 8), chunks=(1, 3, 4), reps=2)
     print(f"batch autotune {res}", flush=True)
     assert res["best"] == min(res["ms"], key=res["ms"].get)
+    assert len(res["ms"]) == len({k[2] for k in res["ms"]}) * len({k[:2] for k in res["ms"]})
     after = b.run()
     eoff = after.set_offsets.cpu().numpy()
     for k in range(len(sets)):

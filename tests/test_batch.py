@@ -27,13 +27,13 @@ def _sets(n_sets, seed, lo, hi):
     return sets, np.array(radii, np.float32), off
 
 
-def _batch(sets, radii, off, leaf=32, run=4, graph=True):
+def _batch(sets, radii, off, leaf=32, run=4, graph=True, chunk=4):
     import octohip
 
     tns = octohip.TNS()
     c = _torch().from_numpy(np.concatenate(sets).astype(np.float32)).cuda()
     b = octohip.Batch(tns, c, off, radii)
-    b.config(leaf, run, graph)
+    b.config(leaf, run, graph, chunk)
     return tns, b, c
 
 
@@ -93,9 +93,9 @@ def test_batch_graph_replay_equals_eager(orc):
 def test_batch_config_does_not_change_edges():
     sets, radii, off = _sets(10, 9, 500, 5000)
     results = []
-    for leaf, run in ((8, 1), (32, 4), (64, 16)):
-        print(f"config leaf={leaf} run={run}", flush=True)
-        tns, b, _ = _batch(sets, radii, off, leaf=leaf, run=run)
+    for leaf, run, chunk in ((8, 1, 4), (32, 4, 1), (64, 16, 2), (16, 2, 3)):
+        print(f"config leaf={leaf} run={run} chunk={chunk}", flush=True)
+        tns, b, _ = _batch(sets, radii, off, leaf=leaf, run=run, chunk=chunk)
         results.append(_per_set(b.run(), len(sets)))
     for other in results[1:]:
         for k in range(len(sets)):

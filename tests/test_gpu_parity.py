@@ -366,3 +366,42 @@ def test_single_pass_capacity_regrow(orc):
     e = octohip.build_edges(tns, sid, sid, mode="learned")
     o = orc.build_edges(orc.run(big, r), big)
     _edges_equal(_gpu_edges(e), (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+
+
+def test_single_pass_heavy_leaves(orc):
+    """Uneven leaves (a Gaussian mixture at ~20 neighbours on average, so
+    cores hold leaves with thousands of hits): the heavy ones are counted by
+    the parallel count pass before the single pass.  The learned edges must be
+    exactly the exact path's, in the same order, and the oracle's set; on
+    even data the pre-count switches off after the first call."""
+    import octohip
+    from octohip import predictor
+
+    pts = datasets.gaussian_mixture(30000, 41, 4, 0.02, 0.1)
+    r = float(datasets.calibrate_radius(pts, 20.0, seed=1))
+    tns, sid, _ = _build(pts, r)
+    ex = _gpu_edges(octohip.build_edges(tns, sid, sid, mode="exact"))
+    o = orc.build_edges(orc.run(pts, r), pts)
+    octohip.build_edges(tns, sid, sid, mode="learned")  # sizes the single pass's buffers
+    for call in range(3):
+        got = _gpu_edges(octohip.build_edges(tns, sid, sid, mode="learned"))
+        st = predictor.edge_stats(tns, sid, sid)
+        print(f"mixture call {call}: heavy={st['heavy_leaves']} recounted={st['recounted_leaves']} "
+              f"recorded={st['recorded_leaves']} single={st['single_pass']}", flush=True)
+        assert st["single_pass"]
+        assert st["heavy_leaves"] > 0
+        for a, b in zip(got, ex):  # same edges in the same order
+            assert np.array_equal(a, b)
+        _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+
+    even = datasets.uniform_cube(20000, 42)
+    tns, sid, _ = _build(even, 0.03)
+    o = orc.build_edges(orc.run(even, 0.03), even)
+    heavy = []
+    for call in range(3):
+        got = _gpu_edges(octohip.build_edges(tns, sid, sid, mode="learned"))
+        st = predictor.edge_stats(tns, sid, sid)
+        heavy.append(st["heavy_leaves"])
+        _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+    print(f"uniform heavy per call {heavy}", flush=True)
+    assert heavy[2] == 0
