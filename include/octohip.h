@@ -96,6 +96,11 @@ int oct_destroy(oct_ctx* ctx);
 const char* oct_last_error(const oct_ctx* ctx);
 /* Order all work of the context after/before other work on this stream. */
 int oct_set_stream(oct_ctx* ctx, void* hip_stream);
+/* No counterpart in the reference (Julia tasks order work on one stream).
+ * direction 0: the context's stream waits for the work queued so far on
+ * hip_stream; 1: hip_stream waits for the work queued so far on the
+ * context's stream.  One reused event per direction, no host wait. */
+int oct_stream_join(oct_ctx* ctx, void* hip_stream, int direction);
 
 /* set_search_radius! (src/api.jl:24-29): r > 0, marks every set dirty. */
 int oct_set_search_radius(oct_ctx* ctx, float r);

@@ -54,7 +54,10 @@ struct SetState {
     bool dirty = true;
     bool built = false;
     int key_bits = 0;
-    BuildParams hp{};  // host copy after the last build
+    BuildParams hp{};       // host copy of the build parameters
+    bool hp_fresh = false;  // hp holds the last build's tree counts (read at a later wait, not by run!)
+    int64_t leaves_known = -1;  // last leaf count read back (sizes the per-leaf arrays of the next build)
+    int built_leaf = 32;        // leaf size of the last build
     // the previous sort (keys[cur], scoord) for the almost-sorted refit
     bool sorted_ok = false;
     int64_t sorted_n = 0;
@@ -112,13 +115,14 @@ struct PairState {
     // heavy query leaves, counted ahead of the single pass: their list
     // (int32 per leaf) followed by a flag byte per leaf
     DevBuf heavy;
+    DevBuf eprm;  // the query tree's build parameters with the leaf count clamped to the rows
     bool heavy_on = true;  // count heavy leaves ahead: first call, and while many leaves are not recorded
     int32_t* heavy_list() { return heavy.as<int32_t>(); }
     uint8_t* heavy_flags(int64_t n_leaves) { return (uint8_t*)(heavy.as<int32_t>() + n_leaves); }
     oct_edge_stats st{};
     void release_all() {
         DevBuf* all[] = {&senders, &receivers, &rdisp, &rdist, &lists, &lcount, &qcount, &qoff, &scan_tmp, &flags,
-                         &tiles, &ldir, &feat, &pred, &lhits, &pwork, &pscan, &wave_first, &heavy};
+                         &tiles, &ldir, &feat, &pred, &lhits, &pwork, &pscan, &wave_first, &heavy, &eprm};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -189,6 +193,7 @@ struct oct_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    hipEvent_t join[2] = {nullptr, nullptr};  // oct_stream_join: caller -> context, context -> caller
     int index_base = 0;
     float radius = 0.f;
     int leaf_size = 32;
@@ -274,6 +279,12 @@ int fail(oct_ctx* c, int code, const std::string& msg) {
             return fail((ctx), OCT_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+#define HIP_TRY_RC(call)              \
+    do {                              \
+        const int rc_ = (call);       \
+        if (rc_ != OCT_OK) return rc_; \
+    } while (0)
+
 int set_index(oct_ctx* c, int id) { return id - c->index_base; }
 
 bool valid_set(oct_ctx* c, int id) {
@@ -281,17 +292,20 @@ bool valid_set(oct_ctx* c, int id) {
     return k >= 0 && k < (int)c->sets.size();
 }
 
-TreeView view_of(const SetState& s) {
+// Device view of a built set.  Launch shapes use rows (an upper bound of the
+// leaves when the counts are not back on the host yet; kernels read the real
+// counts from params) and the node bound of the build.
+TreeView view_of(const SetState& s, int64_t rows, const BuildParams* params = nullptr) {
     TreeView v;
     v.nodes = s.nodes.as<NodeRec>();
     v.leaf_ids = s.leaf_ids.as<int32_t>();
     v.leaves = s.leaves.as<NodeRec>();
     v.scoord = s.scoord.as<float4>();
-    v.params = s.prm.as<BuildParams>();
+    v.params = params ? params : s.prm.as<BuildParams>();
     v.nlevel = s.nlevel.as<uint8_t>();
-    v.n_leaves = s.hp.n_leaves;
-    v.n_nodes = s.hp.n_nodes;
-    v.branch_root = s.hp.branch_root;
+    v.n_leaves = (int32_t)rows;
+    v.n_nodes = s.hp_fresh ? s.hp.n_nodes : (int32_t)std::min<int64_t>(max_nodes(s.n, s.built_leaf), INT32_MAX);
+    v.branch_root = s.hp_fresh ? s.hp.branch_root : 0;
     v.n = s.n;
     return v;
 }
@@ -314,11 +328,30 @@ int read_params(oct_ctx* c, SetState& s, uint64_t* total = nullptr) {
     return OCT_OK;
 }
 
+// The tree counts of the last build on the host (waits for the stream when
+// they are not back yet).
+int ensure_hp(oct_ctx* c, SetState& s) {
+    if (s.hp_fresh || s.n == 0) return OCT_OK;
+    int rc = read_params(c, s);
+    if (rc != OCT_OK) return rc;
+    s.hp_fresh = true;
+    s.leaves_known = s.hp.n_leaves;
+    return OCT_OK;
+}
+
+// Takes a copy of a set's build parameters read back at a later wait.
+void take_hp(SetState& s, const void* src) {
+    memcpy(&s.hp, src, sizeof(BuildParams));
+    s.hp_fresh = true;
+    s.leaves_known = s.hp.n_leaves;
+}
+
 int build_set(oct_ctx* c, SetState& s) {
     const int64_t n = s.n;
     s.built = false;
     if (n == 0) {
         memset(&s.hp, 0, sizeof(s.hp));
+        s.hp_fresh = true;
         s.built = true;
         s.dirty = false;
         return OCT_OK;
@@ -391,8 +424,9 @@ int build_set(oct_ctx* c, SetState& s) {
         const Forest f{0, s.soff.as<int64_t>(), s.prm.as<BuildParams>()};
         HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, cur, b, f, c->stream));
     }
-    rc = read_params(c, s);
-    if (rc != OCT_OK) return rc;
+    // no wait here: the tree counts come back with the next wait that needs them
+    s.hp_fresh = false;
+    s.built_leaf = T;
     s.cur = cur;
     s.sorted_ok = true;
     s.sorted_n = n;
@@ -444,12 +478,27 @@ int oct_destroy(oct_ctx* c) {
     if (c->pinned) (void)hipHostFree(c->pinned);
     for (auto& m : c->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
     for (auto e : c->spare) (void)hipEventDestroy(e);
+    for (auto e : c->join)
+        if (e) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OCT_OK;
 }
 
 const char* oct_last_error(const oct_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int oct_stream_join(oct_ctx* c, void* other, int direction) {
+    if (!c || (direction != 0 && direction != 1)) return OCT_ERR_ARGUMENT;
+    hipStream_t o = (hipStream_t)other;
+    if (o == c->stream) return OCT_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipEvent_t& e = c->join[direction];
+    if (!e) HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // the waiting stream takes the event's state now, so the event is reusable at once
+    HIP_TRY(c, hipEventRecord(e, direction == 0 ? o : c->stream));
+    HIP_TRY(c, hipStreamWaitEvent(direction == 0 ? c->stream : o, e, 0));
+    return OCT_OK;
+}
 
 int oct_set_stream(oct_ctx* c, void* stream) {
     if (!c) return OCT_ERR_ARGUMENT;
@@ -550,6 +599,8 @@ int oct_get_tree(oct_ctx* c, int id, oct_tree* out) {
     memset(out, 0, sizeof(*out));
     out->n_points = s.n;
     if (s.n == 0) return OCT_OK;
+    int rc = ensure_hp(c, s);
+    if (rc != OCT_OK) return rc;
     out->node_bounds_min = s.bmin.as<float>();
     out->node_bounds_max = s.bmax.as<float>();
     out->node_first = s.nfirst.as<int32_t>();
@@ -588,28 +639,28 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     const float r = c->radius;
     const float r_sq = r * r;
     const float inv_r = 1.0f / r;
-    const TreeView qv = view_of(qs), tv = view_of(ts);
-    const int64_t nql = qs.hp.n_leaves;
-    const int64_t ntiles = single_tiles(nql);
     // the predictor runs on the pair's first call and after a call that
     // overflowed or recounted many leaves (or on every call, if asked)
     const bool predict = mode == OCT_EDGES_LEARNED && !collect && c->mlp.loaded &&
                          (c->predict_always || p.cap_edges <= 0 || p.replan);
     const bool features = predict || collect;
+    // Rows of the per-leaf arrays: the exact leaf count when it is on the
+    // host, else the last one read back plus a margin.  The kernels see the
+    // query tree's parameters through a copy whose leaf count is clamped to
+    // the rows (flags[18] = 1 when it was), so a tree that grew past the
+    // margin is redone with the count read back at this call's wait.
+    // Features and the plan need the exact count: they wait for it.
+    if (features || qs.leaves_known < 0) HIP_TRY_RC(ensure_hp(c, qs));
+    auto rows_for = [&]() -> int64_t {
+        if (qs.hp_fresh) return std::max<int64_t>(qs.hp.n_leaves, 1);
+        return std::min<int64_t>(qs.n, qs.leaves_known + qs.leaves_known / 4 + 1024);
+    };
+    int64_t nql = rows_for();
     HIP_TRY(c, p.flags.ensure(128));
-    HIP_TRY(c, p.lcount.ensure(4 * (size_t)nql));
+    HIP_TRY(c, p.eprm.ensure(sizeof(BuildParams)));
     HIP_TRY(c, p.qcount.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.qoff.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.scan_tmp.ensure(4 * (size_t)scan_tmp_words(qs.n)));
-    HIP_TRY(c, p.tiles.ensure(8 * (size_t)(ntiles + 1)));
-    HIP_TRY(c, p.lhits.ensure(4 * (size_t)nql));
-    if (features) HIP_TRY(c, p.feat.ensure(4 * (size_t)nql * MLP_IN));
-    if (predict) {
-        HIP_TRY(c, p.pred.ensure(4 * (size_t)nql));
-        HIP_TRY(c, p.pwork.ensure(16 * (size_t)plan_chunks(nql)));
-        HIP_TRY(c, p.pscan.ensure(4 * (size_t)scan_tmp_words(plan_chunks(nql))));
-        HIP_TRY(c, p.wave_first.ensure(4 * (size_t)(nql + 2)));
-    }
     const int by_receiver = order == OCT_ORDER_RECEIVER;
     // flags (uint32 words): [0] largest candidate count of an overflowing
     // query leaf, [1] a list walk ran out of stack, [2..3] exact-path edge
@@ -617,18 +668,36 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     // or malformed plan, [6..7] single-pass edge total, [8] single-pass tile
     // ticket, [9] recounted leaves, [10] recorded leaves, [12] plan runs,
     // [14..15] predicted total; [16] heavy leaves (kept while the lists are)
+    // [18] more query leaves than rows (launch_clamp_leaves)
     uint32_t* flags = p.flags.as<uint32_t>();
-    uint32_t hf[17] = {0};
+    constexpr int HF_WORDS = 19;
+    uint32_t hf[HF_WORDS] = {0};
     int safe = 0;
     bool single = mode == OCT_EDGES_LEARNED && !collect && (p.cap_edges > 0 || predict);
     const bool heavy = single && p.heavy_on;
-    if (heavy) HIP_TRY(c, p.heavy.ensure(5 * (size_t)nql));
     bool lists_ok = false;
     uint64_t total = 0;
     int attempts = 0;
     EdgeOut eo{};
+    TreeView qv{}, tv{};
     for (int attempt = 0; attempt < 10; ++attempt) {
         ++attempts;
+        const int64_t ntiles = single_tiles(nql);
+        HIP_TRY(c, p.lcount.ensure(4 * (size_t)nql));
+        HIP_TRY(c, p.tiles.ensure(8 * (size_t)(ntiles + 1)));
+        HIP_TRY(c, p.lhits.ensure(4 * (size_t)nql));
+        if (heavy) HIP_TRY(c, p.heavy.ensure(5 * (size_t)nql));
+        if (features) HIP_TRY(c, p.feat.ensure(4 * (size_t)nql * MLP_IN));
+        if (predict) {
+            HIP_TRY(c, p.pred.ensure(4 * (size_t)nql));
+            HIP_TRY(c, p.pwork.ensure(16 * (size_t)plan_chunks(nql)));
+            HIP_TRY(c, p.pscan.ensure(4 * (size_t)scan_tmp_words(plan_chunks(nql))));
+            HIP_TRY(c, p.wave_first.ensure(4 * (size_t)(nql + 2)));
+        }
+        HIP_TRY(c, launch_clamp_leaves(qs.prm.as<BuildParams>(), p.eprm.as<BuildParams>(), nql, flags + 18,
+                                       c->stream));
+        qv = view_of(qs, nql, p.eprm.as<BuildParams>());
+        tv = view_of(ts, ts.hp_fresh ? ts.hp.n_leaves : ts.n);
         HIP_TRY(c, p.lists.ensure(sizeof(int2) * (size_t)nql * (size_t)p.cap));
         if (features) HIP_TRY(c, p.ldir.ensure((size_t)nql * (size_t)p.cap));
         HIP_TRY(c, launch_fill_u32(flags, lists_ok ? 16 : 17, 0u, c->stream));
@@ -657,7 +726,7 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
                 HostTimer ht("edges: wait for predicted total");
                 HIP_TRY(c, hipStreamSynchronize(c->stream));
             }
-            memcpy(hf, c->pinned, 64);
+            memcpy(hf, c->pinned, 64);  // (features: the leaf count was exact already)
             uint64_t pt = 0;
             memcpy(&pt, hf + 14, 8);
             p.cap_edges = (int64_t)std::min<uint64_t>(pt + pt / 16 + 4096, (uint64_t)INT32_MAX - 1);
@@ -693,12 +762,24 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
             HIP_TRY(c, exclusive_scan(p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), qs.n,
                                       p.scan_tmp.as<uint32_t>(), (uint64_t*)(flags + 2), c->stream));
         }
-        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 68, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 4 * HF_WORDS, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync((char*)c->pinned + 1024, qs.prm.p, sizeof(BuildParams), hipMemcpyDeviceToHost,
+                                  c->stream));
+        if (&ts != &qs)
+            HIP_TRY(c, hipMemcpyAsync((char*)c->pinned + 2048, ts.prm.p, sizeof(BuildParams),
+                                      hipMemcpyDeviceToHost, c->stream));
         {
             HostTimer ht(single ? "edges: wait for single pass" : "edges: wait for count total");
             HIP_TRY(c, hipStreamSynchronize(c->stream));
         }
-        memcpy(hf, c->pinned, 68);
+        memcpy(hf, c->pinned, 4 * HF_WORDS);
+        take_hp(qs, (char*)c->pinned + 1024);
+        if (&ts != &qs) take_hp(ts, (char*)c->pinned + 2048);
+        if (hf[18] != 0) {  // more leaves than rows: redo everything with the count now known
+            nql = rows_for();
+            lists_ok = false;
+            continue;
+        }
         if (hf[0] != 0 || hf[1] != 0) {  // candidate lists incomplete: regrow / safe walk, redo everything
             if (hf[0] != 0) p.cap = (int)std::min<uint32_t>(hf[0] + 16u, (uint32_t)INT32_MAX / 8u);
             if (hf[1] != 0) safe = 1;
@@ -732,7 +813,7 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
         p.materialized = false;
         p.st.features = p.feat.as<float>();
         p.st.exact = p.lhits.as<uint32_t>();
-        p.st.n_leaves = nql;
+        p.st.n_leaves = qs.hp.n_leaves;
         p.st.n_edges = E;
         p.st.attempts = attempts;
         out->n_edges = E;
@@ -761,7 +842,8 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     p.st.features = features ? p.feat.as<float>() : nullptr;
     p.st.predicted = predict && single ? p.pred.as<float>() : nullptr;
     p.st.exact = p.lhits.as<uint32_t>();
-    p.st.n_leaves = nql;
+    const int64_t n_leaves = qs.hp.n_leaves;  // read back at the last wait
+    p.st.n_leaves = n_leaves;
     p.st.n_edges = E;
     p.st.recounted_leaves = single ? hf[9] : 0;
     p.st.recorded_leaves = single ? hf[10] : 0;
@@ -771,10 +853,10 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     p.st.used_predictor = predict && single ? 1 : 0;
     p.st.attempts = attempts;
     if (mode == OCT_EDGES_LEARNED) {
-        p.replan = !single || attempts > 1 || (int64_t)hf[9] * 64 > nql;
+        p.replan = !single || attempts > 1 || (int64_t)hf[9] * 64 > n_leaves;
         // uneven leaves (many not recorded in one sweep): count the heavy ones
         // ahead next time; even ones skip the estimate and the extra launch
-        if (single) p.heavy_on = (int64_t)hf[9] * 64 > nql;
+        if (single) p.heavy_on = (int64_t)hf[9] * 64 > n_leaves;
     }
     out->senders = eo.senders;
     out->receivers = eo.receivers;
@@ -794,6 +876,8 @@ int oct_device_view(oct_ctx* c, int q, int t, oct_view* out) {
     SetState& qs = c->sets[set_index(c, q)];
     SetState& ts = c->sets[set_index(c, t)];
     if (!qs.built || qs.dirty || !ts.built || ts.dirty) return fail(c, OCT_ERR_STATE, "run! before device_view");
+    int rc = ensure_hp(c, ts);
+    if (rc != OCT_OK) return rc;
     memset(out, 0, sizeof(*out));
     out->coords_q = qs.coords;
     out->coords_t = ts.coords;

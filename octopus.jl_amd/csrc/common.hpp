@@ -86,14 +86,26 @@ __device__ inline int key_shared(uint64_t a, uint64_t b, int batched) {
 // ------------------------------------------------------------ XCD mapping
 
 // MI355X dispatches workgroups round-robin over its 8 XCDs, each with its own
-// L2.  Remap so XCD x runs the x-th contiguous slice of logical block ids:
-// Morton-adjacent work (which shares tree nodes and candidate points) then
-// stays in one L2.  Bijective on [0, nblocks).
+// L2.  For work counted on the device, with a grid sized by an upper bound:
+// the XCD of this block (of the first min(grid, 8)) takes an equal contiguous
+// share [lo, hi) of [0, n), and this block is number k of the nb blocks on
+// that XCD.  Morton-adjacent work (which shares tree nodes and candidate
+// points) then stays in one L2, and every XCD gets its share whatever the
+// grid's excess.
 #define OCT_NXCD 8u
-__device__ inline uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
-    const uint32_t x = bid % OCT_NXCD, k = bid / OCT_NXCD;
-    const uint32_t per = nblocks / OCT_NXCD, rem = nblocks % OCT_NXCD;
-    return x * per + (x < rem ? x : rem) + k;
+struct XcdShare {
+    int64_t lo, hi;
+    uint32_t k, nb;
+};
+__device__ inline XcdShare xcd_share(int64_t n) {
+    const uint32_t nx = gridDim.x < OCT_NXCD ? gridDim.x : OCT_NXCD;
+    const uint32_t x = blockIdx.x % nx;
+    XcdShare s;
+    s.k = blockIdx.x / nx;
+    s.nb = (gridDim.x - x + nx - 1) / nx;
+    s.lo = n * (int64_t)x / (int64_t)nx;
+    s.hi = n * (int64_t)(x + 1) / (int64_t)nx;
+    return s;
 }
 
 // ------------------------------------------------------------- wave utils

@@ -112,9 +112,10 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
     __shared__ uint32_t fstk[FAST ? LL_FAST_STACK : 1][LL_THREADS];
     const int32_t nleaves = qprm->n_leaves;
     const int32_t root0 = tprm->branch_root;
-    const int32_t lb = (int32_t)xcd_block(blockIdx.x, gridDim.x);
+    const XcdShare sh = xcd_share(nleaves);
     const int tid = threadIdx.x;
-    for (int32_t r = lb * (int32_t)blockDim.x + tid; r < nleaves; r += (int32_t)(gridDim.x * blockDim.x)) {
+    for (int32_t r = (int32_t)(sh.lo + (int64_t)sh.k * LL_THREADS) + tid; r < (int32_t)sh.hi;
+         r += (int32_t)(sh.nb * LL_THREADS)) {
         const NodeRec qa = qleaves[r];
         int32_t root = root0;
         float r_sq = r_sq0;
@@ -585,11 +586,14 @@ k_edges(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, con
     const int lane = lane_id();
     const WaveLds L = lds.wave(wv);
     const int32_t nleaves = qprm->n_leaves;
-    const int32_t nwaves = (int32_t)(gridDim.x * EW_WAVES);
-    const int32_t w = (int32_t)xcd_block(blockIdx.x, gridDim.x) * EW_WAVES + wv;
-    const int32_t per = (nleaves + nwaves - 1) / nwaves;
-    const int32_t r_lo = __builtin_amdgcn_readfirstlane(w * per);
-    const int32_t r_hi = min(nleaves, r_lo + per);
+    // this XCD's share of the leaves, split into contiguous runs per wave
+    const XcdShare sh = xcd_share(nleaves);
+    const int32_t nwaves = (int32_t)(sh.nb * EW_WAVES);
+    const int32_t w = (int32_t)sh.k * EW_WAVES + wv;
+    const int32_t len = (int32_t)(sh.hi - sh.lo);
+    const int32_t per = (len + nwaves - 1) / nwaves;
+    const int32_t r_lo = __builtin_amdgcn_readfirstlane((int32_t)sh.lo + min(w * per, len));
+    const int32_t r_hi = min((int32_t)sh.hi, r_lo + per);
     NodeRec nx_rec{};
     int2 nx_lf = make_int2(0, 0);
     int32_t nx_cnt = 0;
@@ -1060,6 +1064,22 @@ hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* 
     hipLaunchKernelGGL(k_edges_heavy, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
                        t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, heavy_list, n_heavy, qcount, lhits,
                        sets);
+    return hipGetLastError();
+}
+
+// dst = *src with n_leaves clamped to rows; *over = 1 when it was (else 0).
+__global__ void k_clamp_leaves(const BuildParams* __restrict__ src, BuildParams* __restrict__ dst, int64_t rows,
+                               uint32_t* __restrict__ over) {
+    BuildParams p = *src;
+    const bool big = (int64_t)p.n_leaves > rows;
+    if (big) p.n_leaves = (int32_t)rows;
+    *dst = p;
+    *over = big ? 1u : 0u;
+}
+
+hipError_t launch_clamp_leaves(const BuildParams* src, BuildParams* dst, int64_t rows, uint32_t* over,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_clamp_leaves, dim3(1), dim3(1), 0, s, src, dst, rows, over);
     return hipGetLastError();
 }
 

@@ -221,6 +221,11 @@ struct SetLookup {
 hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
                              uint8_t* ldir, int32_t* lcount, uint32_t* flags, const SetLookup& sets,
                              uint8_t* lheavy, int32_t* heavy_list, uint32_t* n_heavy, hipStream_t s);
+// *dst = *src with n_leaves clamped to rows (*over = 1 if it was, else 0):
+// the edge kernels of a tree whose leaf count is not on the host yet read the
+// clamped copy, so they never index past per-leaf arrays of `rows` entries.
+hipError_t launch_clamp_leaves(const BuildParams* src, BuildParams* dst, int64_t rows, uint32_t* over,
+                               hipStream_t s);
 // Exact counts of the listed heavy leaves ahead of the single pass: per-query
 // counts into qcount, leaf totals into lhits.
 hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,

@@ -134,13 +134,16 @@ class TNS:
         """Library work waits for everything queued on the caller's stream."""
         ctx = self._c()
         if self._private_stream:
-            self._stream.wait_stream(_torch().cuda.current_stream(self.device))
+            self._check(self._lib.oct_stream_join(ctx, self._caller_stream(), 0))
         return ctx
 
     def _leave(self):
         """The caller's stream waits for the library's work (outputs ready)."""
         if self._private_stream:
-            _torch().cuda.current_stream(self.device).wait_stream(self._stream)
+            self._check(self._lib.oct_stream_join(self._ctx, self._caller_stream(), 1))
+
+    def _caller_stream(self):
+        return ctypes.c_void_p(_torch().cuda.current_stream(self.device).cuda_stream)
 
     def _check(self, rc):
         if rc == L.OCT_OK:

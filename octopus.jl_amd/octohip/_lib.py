@@ -165,6 +165,7 @@ SIGNATURES = {
                                     _P, ctypes.c_int, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_int)]),
     "oct_set_run_leaves": (ctypes.c_int, [_P, ctypes.c_int]),
     "oct_set_run_chunk": (ctypes.c_int, [_P, ctypes.c_int]),
+    "oct_stream_join": (ctypes.c_int, [_P, _P, ctypes.c_int]),
     "oct_mlp_forward": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
     "oct_mlp_gradient": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int, _P, _P]),
 }

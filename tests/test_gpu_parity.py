@@ -405,3 +405,32 @@ def test_single_pass_heavy_leaves(orc):
         _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
     print(f"uniform heavy per call {heavy}", flush=True)
     assert heavy[2] == 0
+
+
+@pytest.mark.parametrize("mode", ["exact", "learned"])
+def test_leaf_count_growth_between_builds(orc, mode):
+    """run! does not wait for the tree counts; build_edges sizes its per-leaf
+    arrays from the last count read back plus a margin.  A rebuild with four
+    times as many leaves (leaf size 32 -> 8) overflows that margin: the
+    kernels see a clamped leaf count, the call notices at its wait and redoes
+    the edges with the real count.  Edges must still match the oracle."""
+    import octohip
+    from octohip import predictor
+
+    pts = datasets.uniform_cube(120000, 61)
+    r = 0.02
+    tns, sid, c = _build(pts, r)
+    octohip.build_edges(tns, sid, sid, mode=mode)
+    octohip.build_edges(tns, sid, sid, mode=mode)
+    n32 = octohip.get_tree(tns, sid).node_first.shape[0]
+    octohip.set_leaf_size(tns, 8)
+    octohip.run(tns)
+    e = octohip.build_edges(tns, sid, sid, mode=mode)
+    st = predictor.edge_stats(tns, sid, sid)
+    print(f"mode={mode} nodes@32={n32} leaves@8={st['n_leaves']} attempts={st['attempts']}", flush=True)
+    assert st["attempts"] >= 2
+    o = orc.build_edges(orc.run(pts, r, 8), pts)
+    _edges_equal(_gpu_edges(e), (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+    e = octohip.build_edges(tns, sid, sid, mode=mode)  # the count is known now: one attempt
+    assert predictor.edge_stats(tns, sid, sid)["attempts"] == 1
+    _edges_equal(_gpu_edges(e), (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
