@@ -82,10 +82,11 @@ def run_mode(a, mode, snaps, r):
     def step(k):
         octohip.update_point_set(tns, sid, snaps[k])
         octohip.run(tns)
-        refits[0] += int(octohip.build_info(tns, sid)["refit_used"])
         e = octohip.build_edges(tns, sid, sid, mode=mode)
+        # (after build_edges: the refit's outcome is back on the host by then)
+        refits[0] += int(octohip.build_info(tns, sid)["refit_used"])
         if predictor:
-            used[0] += int(octohip.predictor.edge_stats(tns, sid, sid)["used_predictor"])
+            used[0] += int(octohip.predictor.edge_stats(tns, sid, sid, tensors=False)["used_predictor"])
         return e.senders.shape[0]
 
     step(0)

@@ -213,13 +213,21 @@ k_refit_keys(const float* __restrict__ coords, int64_t n, const BuildParams* __r
     chg[i] = k != pk[i] ? 1u : 0u;
 }
 
+// K2b: the changed count, and whether the refit can run (prm->refit_fail).
+__global__ void k_refit_check(const uint64_t* __restrict__ total, int64_t limit, int key_bits,
+                              BuildParams* __restrict__ prm) {
+    const uint64_t nc = *total;
+    prm->changed = (uint32_t)(nc < 0xffffffffull ? nc : 0xffffffffull);
+    prm->refit_fail = (nc > (uint64_t)limit || prm->key_bits != key_bits) ? 1u : 0u;
+}
+
 // K3: compact the changed points (their new key and index).
 __global__ void __launch_bounds__(RF_THREADS)
 k_refit_compact(int64_t n, const uint32_t* __restrict__ cpos, const uint64_t* __restrict__ total,
                 const uint64_t* __restrict__ nk, const int32_t* __restrict__ pi, uint64_t* __restrict__ ck,
-                int32_t* __restrict__ ci) {
+                int32_t* __restrict__ ci, const BuildParams* __restrict__ prm) {
     const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || prm->refit_fail) return;
     const uint32_t c = cpos[i];
     const uint32_t next = i + 1 < n ? cpos[i + 1] : (uint32_t)*total;
     if (next != c) {
@@ -235,10 +243,12 @@ __global__ void __launch_bounds__(RF_THREADS)
 k_refit_rank(int64_t n, const uint64_t* __restrict__ total, const uint64_t* __restrict__ ck,
              const int32_t* __restrict__ ci, const uint64_t* __restrict__ pk, const int32_t* __restrict__ pi,
              const uint32_t* __restrict__ cpos, uint64_t* __restrict__ csk, int32_t* __restrict__ csi,
-             uint32_t* __restrict__ cspos) {
+             uint32_t* __restrict__ cspos, const BuildParams* __restrict__ prm) {
     __shared__ uint64_t tk[RF_TILE];
     __shared__ int32_t ti[RF_TILE];
     const int64_t nc = (int64_t)*total;
+    // block-uniform exits (before any barrier): a failed refit, or a block past the changed points
+    if (prm->refit_fail || (int64_t)blockIdx.x * RF_THREADS >= nc) return;
     const int64_t c = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
     const bool mine = c < nc;
     const uint64_t key = mine ? ck[c] : 0ull;
@@ -283,9 +293,14 @@ __global__ void __launch_bounds__(RF_THREADS)
 k_refit_place(const float* __restrict__ coords, int64_t n, const uint64_t* __restrict__ total,
               const uint64_t* __restrict__ pk, const int32_t* __restrict__ pi, const uint32_t* __restrict__ cpos,
               const uint64_t* __restrict__ csk, const int32_t* __restrict__ csi, const uint32_t* __restrict__ cspos,
-              uint64_t* __restrict__ kout, int32_t* __restrict__ perm, float4* __restrict__ sc, int index_base) {
+              uint64_t* __restrict__ kout, int32_t* __restrict__ perm, float4* __restrict__ sc, int index_base,
+              const BuildParams* __restrict__ prm) {
     const int64_t nc = (int64_t)*total;
     const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
+    if (prm->refit_fail) {  // keep the previous order (perm and scoord stay as they are)
+        if (i < n) kout[i] = pk[i];
+        return;
+    }
     if (i < nc) refit_write(cspos[i], csk[i], csi[i], coords, kout, perm, sc, index_base);
     if (i >= n) return;
     const uint32_t c = cpos[i];
@@ -669,27 +684,33 @@ hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int i
 // (keys[cur] and scoord hold the previous sort) and the changed count in
 // *b.total.  The caller reads the count and either finishes the refit or
 // runs the full sort.
-hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, const BuildBufs& b, hipStream_t s) {
+hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, int64_t limit, int key_bits,
+                                const BuildBufs& b, hipStream_t s) {
+    if (limit > REFIT_MAX_CHANGED) return hipErrorInvalidValue;  // the changed-point buffers hold that many
     const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
     hipLaunchKernelGGL(k_refit_keys, dim3(g), dim3(RF_THREADS), 0, s, coords, n, b.prm, b.keys[cur], b.scoord,
                        b.keys[cur ^ 1], b.vals[0], (uint32_t*)b.vals[1]);
-    return exclusive_scan((const uint32_t*)b.vals[1], (uint32_t*)b.vals[1], n, b.scan_tmp, b.total, s);
+    hipError_t e = exclusive_scan((const uint32_t*)b.vals[1], (uint32_t*)b.vals[1], n, b.scan_tmp, b.total, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_refit_check, dim3(1), dim3(1), 0, s, b.total, limit, key_bits, b.prm);
+    return hipGetLastError();
 }
 
-// Part 2, for n_changed <= REFIT_MAX_CHANGED: the sorted keys end in keys[cur ^ 1].
-hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t n_changed, int cur, int index_base,
+// Part 2: the sorted keys end in keys[cur ^ 1].  Grids cover `limit` changed
+// points; the kernels read the real count (and refit_fail) on the device.
+hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t limit, int cur, int index_base,
                                const BuildBufs& b, const RefitBufs& r, hipStream_t s) {
     const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
     const uint32_t* cpos = (const uint32_t*)b.vals[1];
     hipLaunchKernelGGL(k_refit_compact, dim3(g), dim3(RF_THREADS), 0, s, n, cpos, b.total, b.keys[cur ^ 1], b.vals[0],
-                       r.ck, r.ci);
-    if (n_changed > 0) {
-        const unsigned gc = (unsigned)((n_changed + RF_THREADS - 1) / RF_THREADS);
+                       r.ck, r.ci, b.prm);
+    if (limit > 0) {
+        const unsigned gc = (unsigned)((limit + RF_THREADS - 1) / RF_THREADS);
         hipLaunchKernelGGL(k_refit_rank, dim3(gc), dim3(RF_THREADS), 0, s, n, b.total, r.ck, r.ci, b.keys[cur],
-                           b.vals[0], cpos, r.csk, r.csi, r.cspos);
+                           b.vals[0], cpos, r.csk, r.csi, r.cspos, b.prm);
     }
     hipLaunchKernelGGL(k_refit_place, dim3(g), dim3(RF_THREADS), 0, s, coords, n, b.total, b.keys[cur], b.vals[0],
-                       cpos, r.csk, r.csi, r.cspos, b.keys[cur ^ 1], b.perm, b.scoord, index_base);
+                       cpos, r.csk, r.csi, r.cspos, b.keys[cur ^ 1], b.perm, b.scoord, index_base, b.prm);
     return hipGetLastError();
 }
 

@@ -58,6 +58,7 @@ struct SetState {
     bool hp_fresh = false;  // hp holds the last build's tree counts (read at a later wait, not by run!)
     int64_t leaves_known = -1;  // last leaf count read back (sizes the per-leaf arrays of the next build)
     int built_leaf = 32;        // leaf size of the last build
+    bool refit_pending = false;  // the last build refitted without a wait: its outcome comes with hp
     // the previous sort (keys[cur], scoord) for the almost-sorted refit
     bool sorted_ok = false;
     int64_t sorted_n = 0;
@@ -328,17 +329,6 @@ int read_params(oct_ctx* c, SetState& s, uint64_t* total = nullptr) {
     return OCT_OK;
 }
 
-// The tree counts of the last build on the host (waits for the stream when
-// they are not back yet).
-int ensure_hp(oct_ctx* c, SetState& s) {
-    if (s.hp_fresh || s.n == 0) return OCT_OK;
-    int rc = read_params(c, s);
-    if (rc != OCT_OK) return rc;
-    s.hp_fresh = true;
-    s.leaves_known = s.hp.n_leaves;
-    return OCT_OK;
-}
-
 // Takes a copy of a set's build parameters read back at a later wait.
 void take_hp(SetState& s, const void* src) {
     memcpy(&s.hp, src, sizeof(BuildParams));
@@ -346,8 +336,47 @@ void take_hp(SetState& s, const void* src) {
     s.leaves_known = s.hp.n_leaves;
 }
 
-int build_set(oct_ctx* c, SetState& s) {
+int build_set(oct_ctx* c, SetState& s, bool force_full = false);
+
+// With fresh build parameters: records how a refit without a wait went, and
+// when it could not run (too many changed points, or a new key width)
+// rebuilds the set with the full sort.  *rebuilt tells the caller that the
+// tree (and hp) changed.
+int settle_refit(oct_ctx* c, SetState& s, bool* rebuilt) {
+    *rebuilt = false;
+    if (!s.refit_pending || !s.hp_fresh) return OCT_OK;
+    s.refit_pending = false;
+    const int64_t changed = s.hp.changed;
+    if (!s.hp.refit_fail) {
+        s.info.refit_used = 1;
+        s.info.changed_points = changed;
+        return OCT_OK;
+    }
+    HIP_TRY_RC(build_set(c, s, true));
+    s.info.refit_used = 0;
+    s.info.changed_points = changed;
+    *rebuilt = true;
+    return OCT_OK;
+}
+
+// The tree counts of the last build on the host (waits for the stream when
+// they are not back yet; settles a pending refit).
+int ensure_hp(oct_ctx* c, SetState& s) {
+    for (int k = 0; k < 3 && !(s.hp_fresh && !s.refit_pending) && s.n > 0; ++k) {
+        if (!s.hp_fresh) {
+            HIP_TRY_RC(read_params(c, s));
+            s.hp_fresh = true;
+            s.leaves_known = s.hp.n_leaves;
+        }
+        bool rebuilt = false;
+        HIP_TRY_RC(settle_refit(c, s, &rebuilt));
+    }
+    return OCT_OK;
+}
+
+int build_set(oct_ctx* c, SetState& s, bool force_full) {
     const int64_t n = s.n;
+    if (s.refit_pending) HIP_TRY_RC(ensure_hp(c, s));  // the previous build's outcome first
     s.built = false;
     if (n == 0) {
         memset(&s.hp, 0, sizeof(s.hp));
@@ -385,26 +414,23 @@ int build_set(oct_ctx* c, SetState& s) {
     HIP_TRY(c, s.bmin.ensure(12 * (size_t)M));
     HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
     const BuildBufs b = s.bufs();
-    // almost-sorted refit: the previous sort of the same n points is reused
-    const bool try_refit = c->refit_mode && s.sorted_ok && s.sorted_n == n;
+    // almost-sorted refit: the previous sort of the same n points is reused.
+    // It runs without a host wait: the device decides whether it can
+    // (prm->refit_fail), and whoever reads the build parameters back settles
+    // the outcome (settle_refit), rebuilding with the full sort if it could not.
+    const bool try_refit = c->refit_mode && s.sorted_ok && s.sorted_n == n && !force_full;
     s.sorted_ok = false;
     memset(&s.info, 0, sizeof(s.info));
+    s.info.changed_points = -1;
     {
         Phase ph(c, OCT_PROF_BOUNDS);
         HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, b, c->stream));
     }
-    if (try_refit) {
-        Phase ph(c, OCT_PROF_BUILD);
-        HIP_TRY(c, launch_refit_prepare(s.coords, n, s.cur, b, c->stream));
-    }
-    uint64_t changed = 0;
-    int rc = read_params(c, s, try_refit ? &changed : nullptr);
-    if (rc != OCT_OK) return rc;
-    s.key_bits = s.hp.key_bits;
     int cur = 0;
     {
         Phase ph(c, OCT_PROF_BUILD);
-        if (try_refit && (int64_t)changed <= std::min<int64_t>(REFIT_MAX_CHANGED, n / 8)) {
+        if (try_refit) {
+            const int64_t limit = std::min<int64_t>(REFIT_MAX_CHANGED, n / 8);
             HIP_TRY(c, s.rk.ensure(8 * (size_t)REFIT_MAX_CHANGED));
             HIP_TRY(c, s.ri.ensure(4 * (size_t)REFIT_MAX_CHANGED));
             HIP_TRY(c, s.rsk.ensure(8 * (size_t)REFIT_MAX_CHANGED));
@@ -412,13 +438,16 @@ int build_set(oct_ctx* c, SetState& s) {
             HIP_TRY(c, s.rpos.ensure(4 * (size_t)REFIT_MAX_CHANGED));
             const RefitBufs r{s.rk.as<uint64_t>(), s.ri.as<int32_t>(), s.rsk.as<uint64_t>(), s.rsi.as<int32_t>(),
                               s.rpos.as<uint32_t>()};
-            HIP_TRY(c, launch_refit_finish(s.coords, n, (int64_t)changed, s.cur, c->index_base, b, r, c->stream));
+            // the key width must stay s.key_bits (the levels the topology refits depend on it)
+            HIP_TRY(c, launch_refit_prepare(s.coords, n, s.cur, limit, s.key_bits, b, c->stream));
+            HIP_TRY(c, launch_refit_finish(s.coords, n, limit, s.cur, c->index_base, b, r, c->stream));
             cur = s.cur ^ 1;
-            s.info.refit_used = 1;
+            s.refit_pending = true;
         } else {
+            HIP_TRY_RC(read_params(c, s));  // the key width sizes the radix passes
+            s.key_bits = s.hp.key_bits;
             HIP_TRY(c, launch_sort_stage(s.coords, n, s.key_bits, c->index_base, b, &cur, c->stream));
         }
-        s.info.changed_points = try_refit ? (int64_t)changed : -1;
         HIP_TRY(c, s.soff.ensure(16));
         HIP_TRY(c, launch_single_set_offsets(s.soff.as<int64_t>(), n, c->stream));
         const Forest f{0, s.soff.as<int64_t>(), s.prm.as<BuildParams>()};
@@ -613,6 +642,8 @@ int oct_get_tree(oct_ctx* c, int id, oct_tree* out) {
     return OCT_OK;
 }
 
+constexpr size_t HF_BYTES = 128;  // flag words of a pair's edge call; the build parameters follow
+
 // Flat-COO edges of pair (q, t).  collect: exact path that also keeps the
 // predictor's features and the exact per-leaf hits (training rows).
 static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collect, oct_edges* out) {
@@ -656,7 +687,7 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
         return std::min<int64_t>(qs.n, qs.leaves_known + qs.leaves_known / 4 + 1024);
     };
     int64_t nql = rows_for();
-    HIP_TRY(c, p.flags.ensure(128));
+    HIP_TRY(c, p.flags.ensure(HF_BYTES + 2 * sizeof(BuildParams)));
     HIP_TRY(c, p.eprm.ensure(sizeof(BuildParams)));
     HIP_TRY(c, p.qcount.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.qoff.ensure(4 * (size_t)qs.n));
@@ -668,10 +699,12 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     // or malformed plan, [6..7] single-pass edge total, [8] single-pass tile
     // ticket, [9] recounted leaves, [10] recorded leaves, [12] plan runs,
     // [14..15] predicted total; [16] heavy leaves (kept while the lists are)
-    // [18] more query leaves than rows (launch_clamp_leaves)
+    // [18] more query leaves than rows (launch_clamp_leaves); the query and
+    // target build parameters follow at byte HF_BYTES
     uint32_t* flags = p.flags.as<uint32_t>();
     constexpr int HF_WORDS = 19;
     uint32_t hf[HF_WORDS] = {0};
+    BuildParams* raw_prm = (BuildParams*)((char*)p.flags.p + HF_BYTES);
     int safe = 0;
     bool single = mode == OCT_EDGES_LEARNED && !collect && (p.cap_edges > 0 || predict);
     const bool heavy = single && p.heavy_on;
@@ -694,8 +727,8 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
             HIP_TRY(c, p.pscan.ensure(4 * (size_t)scan_tmp_words(plan_chunks(nql))));
             HIP_TRY(c, p.wave_first.ensure(4 * (size_t)(nql + 2)));
         }
-        HIP_TRY(c, launch_clamp_leaves(qs.prm.as<BuildParams>(), p.eprm.as<BuildParams>(), nql, flags + 18,
-                                       c->stream));
+        HIP_TRY(c, launch_clamp_leaves(qs.prm.as<BuildParams>(), ts.prm.as<BuildParams>(), p.eprm.as<BuildParams>(),
+                                       nql, flags + 18, raw_prm, c->stream));
         qv = view_of(qs, nql, p.eprm.as<BuildParams>());
         tv = view_of(ts, ts.hp_fresh ? ts.hp.n_leaves : ts.n);
         HIP_TRY(c, p.lists.ensure(sizeof(int2) * (size_t)nql * (size_t)p.cap));
@@ -762,19 +795,24 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
             HIP_TRY(c, exclusive_scan(p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), qs.n,
                                       p.scan_tmp.as<uint32_t>(), (uint64_t*)(flags + 2), c->stream));
         }
-        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, 4 * HF_WORDS, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipMemcpyAsync((char*)c->pinned + 1024, qs.prm.p, sizeof(BuildParams), hipMemcpyDeviceToHost,
+        // one read-back: flags and both sets' build parameters
+        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, HF_BYTES + 2 * sizeof(BuildParams), hipMemcpyDeviceToHost,
                                   c->stream));
-        if (&ts != &qs)
-            HIP_TRY(c, hipMemcpyAsync((char*)c->pinned + 2048, ts.prm.p, sizeof(BuildParams),
-                                      hipMemcpyDeviceToHost, c->stream));
         {
             HostTimer ht(single ? "edges: wait for single pass" : "edges: wait for count total");
             HIP_TRY(c, hipStreamSynchronize(c->stream));
         }
         memcpy(hf, c->pinned, 4 * HF_WORDS);
-        take_hp(qs, (char*)c->pinned + 1024);
-        if (&ts != &qs) take_hp(ts, (char*)c->pinned + 2048);
+        take_hp(qs, (char*)c->pinned + HF_BYTES);
+        if (&ts != &qs) take_hp(ts, (char*)c->pinned + HF_BYTES + sizeof(BuildParams));
+        bool rq = false, rt = false;
+        HIP_TRY_RC(settle_refit(c, qs, &rq));
+        if (&ts != &qs) HIP_TRY_RC(settle_refit(c, ts, &rt));
+        if (rq || rt) {  // a refit could not run: the sets were rebuilt, redo the edges
+            nql = rows_for();
+            lists_ok = false;
+            continue;
+        }
         if (hf[18] != 0) {  // more leaves than rows: redo everything with the count now known
             nql = rows_for();
             lists_ok = false;
@@ -1112,7 +1150,9 @@ int oct_predictor_train(oct_ctx* c, int steps, int batch, float lr, uint64_t see
 int oct_build_info_get(oct_ctx* c, int id, oct_build_info* out) {
     if (!c || !out) return OCT_ERR_ARGUMENT;
     if (!valid_set(c, id)) return fail(c, OCT_ERR_ARGUMENT, "unknown point set");
-    *out = c->sets[set_index(c, id)].info;
+    SetState& s = c->sets[set_index(c, id)];
+    if (s.refit_pending) HIP_TRY_RC(ensure_hp(c, s));  // the refit's outcome comes with the build parameters
+    *out = s.info;
     return OCT_OK;
 }
 

@@ -1067,19 +1067,23 @@ hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* 
     return hipGetLastError();
 }
 
-// dst = *src with n_leaves clamped to rows; *over = 1 when it was (else 0).
-__global__ void k_clamp_leaves(const BuildParams* __restrict__ src, BuildParams* __restrict__ dst, int64_t rows,
-                               uint32_t* __restrict__ over) {
+// dst = *src with n_leaves clamped to rows; *over = 1 when it was (else 0);
+// raw[0] = *src and raw[1] = *tsrc, for the host to read back with the flags.
+__global__ void k_clamp_leaves(const BuildParams* __restrict__ src, const BuildParams* __restrict__ tsrc,
+                               BuildParams* __restrict__ dst, int64_t rows, uint32_t* __restrict__ over,
+                               BuildParams* __restrict__ raw) {
     BuildParams p = *src;
+    raw[0] = p;
+    raw[1] = *tsrc;
     const bool big = (int64_t)p.n_leaves > rows;
     if (big) p.n_leaves = (int32_t)rows;
     *dst = p;
     *over = big ? 1u : 0u;
 }
 
-hipError_t launch_clamp_leaves(const BuildParams* src, BuildParams* dst, int64_t rows, uint32_t* over,
-                               hipStream_t s) {
-    hipLaunchKernelGGL(k_clamp_leaves, dim3(1), dim3(1), 0, s, src, dst, rows, over);
+hipError_t launch_clamp_leaves(const BuildParams* src, const BuildParams* tsrc, BuildParams* dst, int64_t rows,
+                               uint32_t* over, BuildParams* raw, hipStream_t s) {
+    hipLaunchKernelGGL(k_clamp_leaves, dim3(1), dim3(1), 0, s, src, tsrc, dst, rows, over, raw);
     return hipGetLastError();
 }
 

@@ -23,6 +23,8 @@ struct BuildParams {
     int32_t branch_root;   // first node with count == n that is a leaf or has > 1 child
     int32_t chain_level;   // L0: levels 0..L0 hold all n points (one node each)
     float radius;          // search radius of the set (batched builds: per set)
+    uint32_t changed;      // refit: points whose key changed
+    uint32_t refit_fail;   // refit: 1 when it could not run (too many changed points, or a new key width)
 };
 
 // Batched builds (many sets in one forest): keys carry the set id above bit
@@ -138,6 +140,8 @@ __device__ inline void bounds_fold(const uint32_t* __restrict__ partials, int b0
     }
     prm->inv_cs = inv_cs;
     prm->radius = radius;
+    prm->changed = 0;
+    prm->refit_fail = 0;
     prm->cmax = cmax;
     prm->key_bits = cmax == 0 ? 0 : 3 * (32 - __clz((int)cmax));
 }
@@ -178,9 +182,13 @@ hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base,
 hipError_t launch_single_set_offsets(int64_t* off, int64_t n, hipStream_t s);
 
 // Almost-sorted refit (time stepping): the previous sort is keys[cur] and
-// scoord.  prepare: new keys in that order and the changed count in *b.total;
-// finish (changed count <= REFIT_MAX_CHANGED): sorted keys in keys[cur ^ 1],
-// perm and scoord rewritten, identical to the full sort's.
+// scoord.  prepare: new keys in that order, the changed count in *b.total and
+// prm->changed, and prm->refit_fail = 1 when that count passes `limit` (at
+// most REFIT_MAX_CHANGED) or the key width is no longer key_bits; finish:
+// sorted keys in keys[cur ^ 1], perm and scoord rewritten, identical to the
+// full sort's, or (refit_fail) the previous sort copied to keys[cur ^ 1] with
+// perm and scoord left as they were, a valid tree of the previous order that
+// the host replaces by a full rebuild.  No host wait in between.
 constexpr int64_t REFIT_MAX_CHANGED = 32768;
 struct RefitBufs {
     uint64_t* ck;    // changed keys (compacted)
@@ -189,8 +197,9 @@ struct RefitBufs {
     int32_t* csi;
     uint32_t* cspos; // their final positions
 };
-hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, const BuildBufs& b, hipStream_t s);
-hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t n_changed, int cur, int index_base,
+hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, int64_t limit, int key_bits,
+                                const BuildBufs& b, hipStream_t s);
+hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t limit, int cur, int index_base,
                                const BuildBufs& b, const RefitBufs& r, hipStream_t s);
 
 // Edge outputs (see edges.hip).
@@ -224,8 +233,9 @@ hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, i
 // *dst = *src with n_leaves clamped to rows (*over = 1 if it was, else 0):
 // the edge kernels of a tree whose leaf count is not on the host yet read the
 // clamped copy, so they never index past per-leaf arrays of `rows` entries.
-hipError_t launch_clamp_leaves(const BuildParams* src, BuildParams* dst, int64_t rows, uint32_t* over,
-                               hipStream_t s);
+// raw[0..1] receive *src and *tsrc unchanged (read back with the flags).
+hipError_t launch_clamp_leaves(const BuildParams* src, const BuildParams* tsrc, BuildParams* dst, int64_t rows,
+                               uint32_t* over, BuildParams* raw, hipStream_t s);
 // Exact counts of the listed heavy leaves ahead of the single pass: per-query
 // counts into qcount, leaf totals into lhits.
 hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,

@@ -85,9 +85,10 @@ def train(tns, steps: int = 2000, batch: int = 4096, lr: float = 1e-3, seed: int
     return float(loss.value)
 
 
-def edge_stats(tns, q: int, t: int) -> dict:
+def edge_stats(tns, q: int, t: int, tensors: bool = True) -> dict:
     """What the last build of (q, t) did, with device tensors of the per-leaf
-    features, predicted hits (None unless the predictor ran) and exact hits."""
+    features, predicted hits (None unless the predictor ran) and exact hits
+    (all None with tensors=False)."""
     from . import _wrap, _torch
 
     torch = _torch()
@@ -95,9 +96,9 @@ def edge_stats(tns, q: int, t: int) -> dict:
     tns._check(tns._lib.oct_edge_stats_get(tns._c(), int(q), int(t), ctypes.byref(s)))
     n, dev = int(s.n_leaves), tns.device
     return {
-        "features": _wrap(s.features, (n, FEATURES), torch.float32, dev) if s.features else None,
-        "predicted": _wrap(s.predicted, (n,), torch.float32, dev) if s.predicted else None,
-        "exact": _wrap(s.exact, (n,), torch.int32, dev) if s.exact and n else None,
+        "features": _wrap(s.features, (n, FEATURES), torch.float32, dev) if tensors and s.features else None,
+        "predicted": _wrap(s.predicted, (n,), torch.float32, dev) if tensors and s.predicted else None,
+        "exact": _wrap(s.exact, (n,), torch.int32, dev) if tensors and s.exact and n else None,
         "n_leaves": n,
         "n_edges": int(s.n_edges),
         "recounted_leaves": int(s.recounted_leaves),

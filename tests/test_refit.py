@@ -100,3 +100,30 @@ def test_refit_off_reports_no_previous_sort():
 
     tns, sid, _ = _pair(datasets.uniform_cube(1000, 5), 0.05, False)
     assert octohip.build_info(tns, sid) == {"refit_used": False, "changed_points": -1}
+
+
+@pytest.mark.parametrize("mode", ["exact", "learned"])
+def test_refit_failure_found_at_build_edges(orc, mode):
+    """run! refits without waiting; the device finds too many changed points
+    and keeps a valid tree of the previous order, and build_edges (the first
+    read-back) rebuilds with the full sort and redoes the edges."""
+    import octohip
+
+    pts = datasets.uniform_cube(20000, 14)
+    r = 0.03
+    tns, sid, _ = _pair(pts, r, True)
+    octohip.build_edges(tns, sid, sid, mode=mode)
+    rng = np.random.default_rng(5)
+    moved = (pts.astype(np.float64) + rng.normal(0, 0.05, pts.shape)).astype(np.float32)
+    octohip.update_point_set(tns, sid, _torch().from_numpy(moved).cuda())
+    octohip.run(tns)
+    e = octohip.build_edges(tns, sid, sid, mode=mode)
+    info = octohip.build_info(tns, sid)
+    print(f"{mode}: {info}", flush=True)
+    assert not info["refit_used"] and info["changed_points"] > 20000 // 8
+    ot = orc.run(moved, r)
+    o = orc.build_edges(ot, moved)
+    got = sort_edges(_np(e.senders), _np(e.receivers), _np(e.rel_displacement), _np(e.rel_dist_norm).reshape(-1))
+    for x, y in zip(got, sort_edges(o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm)):
+        assert np.array_equal(x, y)
+    assert np.array_equal(_np(octohip.get_tree(tns, sid).permutation), ot.perm)
