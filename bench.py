@@ -38,7 +38,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=30, help="steps of the CPU oracle baseline (0 = skip)")
     ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M points)")
     ap.add_argument("--no-profile", action="store_true", help="time without the per-phase HIP events")
-    ap.add_argument("--null-stream", action="store_true", help="run the library on torch's current stream")
+    ap.add_argument("--private-stream", action="store_true",
+                    help="run the library on its own stream, joined to torch's current stream per call "
+                         "(default: on torch's current stream)")
     ap.add_argument("--predictor", default="sph_lattice",
                     help="shipped predictor for the learned mode ('none': plan-free single pass)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -64,7 +66,7 @@ def run_mode(a, mode, snaps, r):
 
     import octohip
 
-    tns = octohip.TNS(private_stream=not a.null_stream)
+    tns = octohip.TNS(private_stream=a.private_stream)
     octohip.set_search_radius(tns, float(r))
     octohip.set_refit_mode(tns, not a.no_refit)
     sid = octohip.add_point_set(tns, snaps[0])

@@ -94,7 +94,10 @@ typedef struct oct_view {
 int oct_create(oct_ctx** out, int device, int ndims, int index_base);
 int oct_destroy(oct_ctx* ctx);
 const char* oct_last_error(const oct_ctx* ctx);
-/* Order all work of the context after/before other work on this stream. */
+/* Run the context's work on this stream from now on (work already queued on
+ * the previous stream stays ahead of it; no host wait unless the previous
+ * stream was the context's own).  Setting the current stream again is free,
+ * so a caller may pass its current stream before every call. */
 int oct_set_stream(oct_ctx* ctx, void* hip_stream);
 /* No counterpart in the reference (Julia tasks order work on one stream).
  * direction 0: the context's stream waits for the work queued so far on

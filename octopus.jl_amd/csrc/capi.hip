@@ -94,10 +94,28 @@ struct SetState {
         b.bmax = bmax.as<float>();
         return b;
     }
+    // The refit step (bounds, refit, topology: some 25 launches) as a HIP
+    // graph per parity of cur, replayed while its key (sizes, buffers,
+    // radius) holds.  It reads a copy of the caller's coordinates (cin), so
+    // the caller may pass a new array every step.
+    DevBuf cin;
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        uint64_t key = 0;
+    };
+    StepGraph rg[2];
+    void drop_graphs() {
+        for (auto& g : rg) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+            g.key = 0;
+        }
+    }
     void release_all() {
+        drop_graphs();
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
                          &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
-                         &rk, &ri, &rsk, &rsi, &rpos, &soff};
+                         &rk, &ri, &rsk, &rsi, &rpos, &soff, &cin};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -168,13 +186,10 @@ struct BatchState {
     SetState set;              // forest build buffers; set.prm holds the forest's totals
     PairState pair;            // edge buffers
     DevBuf off, radii, bprefix, sprm, eoff, loff, bflags, bscan;
-    hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
-        graph = nullptr;
     }
     void release_all() {
         drop_graph();
@@ -195,6 +210,7 @@ struct oct_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     hipEvent_t join[2] = {nullptr, nullptr};  // oct_stream_join: caller -> context, context -> caller
+    hipStream_t cap_stream = nullptr;          // graph captures (the caller's stream may be the legacy default)
     int index_base = 0;
     float radius = 0.f;
     int leaf_size = 32;
@@ -285,6 +301,36 @@ int fail(oct_ctx* c, int code, const std::string& msg) {
         const int rc_ = (call);       \
         if (rc_ != OCT_OK) return rc_; \
     } while (0)
+
+// Captures what `enqueue` launches on c->stream into an executable graph.
+// The capture runs on the context's own stream: the caller's stream may be
+// the legacy default stream, which cannot capture; the graph then launches
+// on any stream.
+template <typename F>
+int capture_graph(oct_ctx* c, F enqueue, hipGraphExec_t* exec) {
+    *exec = nullptr;
+    if (!c->cap_stream) HIP_TRY(c, hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+    const hipStream_t saved = c->stream;
+    c->stream = c->cap_stream;
+    const hipError_t be = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (be != hipSuccess) {
+        c->stream = saved;
+        HIP_TRY(c, be);
+    }
+    const int erc = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+    c->stream = saved;
+    if (erc != OCT_OK || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (erc != OCT_OK) return erc;
+        HIP_TRY(c, ee);
+    }
+    const hipError_t ie = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_TRY(c, ie);
+    return OCT_OK;
+}
 
 int set_index(oct_ctx* c, int id) { return id - c->index_base; }
 
@@ -422,35 +468,73 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
     s.sorted_ok = false;
     memset(&s.info, 0, sizeof(s.info));
     s.info.changed_points = -1;
-    {
-        Phase ph(c, OCT_PROF_BOUNDS);
-        HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, b, c->stream));
-    }
+    HIP_TRY(c, s.soff.ensure(16));
+    const Forest f{0, s.soff.as<int64_t>(), s.prm.as<BuildParams>()};
     int cur = 0;
-    {
-        Phase ph(c, OCT_PROF_BUILD);
-        if (try_refit) {
-            const int64_t limit = std::min<int64_t>(REFIT_MAX_CHANGED, n / 8);
-            HIP_TRY(c, s.rk.ensure(8 * (size_t)REFIT_MAX_CHANGED));
-            HIP_TRY(c, s.ri.ensure(4 * (size_t)REFIT_MAX_CHANGED));
-            HIP_TRY(c, s.rsk.ensure(8 * (size_t)REFIT_MAX_CHANGED));
-            HIP_TRY(c, s.rsi.ensure(4 * (size_t)REFIT_MAX_CHANGED));
-            HIP_TRY(c, s.rpos.ensure(4 * (size_t)REFIT_MAX_CHANGED));
-            const RefitBufs r{s.rk.as<uint64_t>(), s.ri.as<int32_t>(), s.rsk.as<uint64_t>(), s.rsi.as<int32_t>(),
-                              s.rpos.as<uint32_t>()};
-            // the key width must stay s.key_bits (the levels the topology refits depend on it)
-            HIP_TRY(c, launch_refit_prepare(s.coords, n, s.cur, limit, s.key_bits, b, c->stream));
-            HIP_TRY(c, launch_refit_finish(s.coords, n, limit, s.cur, c->index_base, b, r, c->stream));
-            cur = s.cur ^ 1;
-            s.refit_pending = true;
+    if (try_refit) {
+        const int64_t limit = std::min<int64_t>(REFIT_MAX_CHANGED, n / 8);
+        HIP_TRY(c, s.rk.ensure(8 * (size_t)REFIT_MAX_CHANGED));
+        HIP_TRY(c, s.ri.ensure(4 * (size_t)REFIT_MAX_CHANGED));
+        HIP_TRY(c, s.rsk.ensure(8 * (size_t)REFIT_MAX_CHANGED));
+        HIP_TRY(c, s.rsi.ensure(4 * (size_t)REFIT_MAX_CHANGED));
+        HIP_TRY(c, s.rpos.ensure(4 * (size_t)REFIT_MAX_CHANGED));
+        const RefitBufs r{s.rk.as<uint64_t>(), s.ri.as<int32_t>(), s.rsk.as<uint64_t>(), s.rsi.as<int32_t>(),
+                          s.rpos.as<uint32_t>()};
+        // the key width must stay s.key_bits (the levels the topology refits depend on it)
+        auto enqueue = [&](const float* coords) -> int {
+            {
+                Phase ph(c, OCT_PROF_BOUNDS);
+                HIP_TRY(c, launch_bounds_stage(coords, n, c->radius, b, c->stream));
+            }
+            Phase ph(c, OCT_PROF_BUILD);
+            HIP_TRY(c, launch_refit_prepare(coords, n, s.cur, limit, s.key_bits, b, c->stream));
+            HIP_TRY(c, launch_refit_finish(coords, n, limit, s.cur, c->index_base, b, r, c->stream));
+            HIP_TRY(c, launch_single_set_offsets(s.soff.as<int64_t>(), n, c->stream));
+            HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, s.cur ^ 1, b, f, c->stream));
+            return OCT_OK;
+        };
+        if (c->profiling) {  // phase events are not captured: launch one by one
+            HIP_TRY_RC(enqueue(s.coords));
         } else {
-            HIP_TRY_RC(read_params(c, s));  // the key width sizes the radix passes
-            s.key_bits = s.hp.key_bits;
-            HIP_TRY(c, launch_sort_stage(s.coords, n, s.key_bits, c->index_base, b, &cur, c->stream));
+            HIP_TRY(c, s.cin.ensure(12 * (size_t)n));
+            HIP_TRY(c, hipMemcpyAsync(s.cin.p, s.coords, 12 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+            // the graph's key: everything its launches bake in
+            uint64_t key = 1469598103934665603ull;
+            auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
+            const DevBuf* bufs[] = {&s.prm, &s.partials, &s.keys0, &s.keys1, &s.vals0, &s.vals1, &s.hist, &s.scan_tmp,
+                                    &s.total, &s.topo, &s.perm, &s.scoord, &s.nodes, &s.nfirst, &s.nlast, &s.nchild,
+                                    &s.parent, &s.nlevel, &s.leaf_ids, &s.leaves, &s.bmin, &s.bmax, &s.rk, &s.ri,
+                                    &s.rsk, &s.rsi, &s.rpos, &s.soff, &s.cin};
+            for (const DevBuf* d : bufs) mix((uint64_t)(uintptr_t)d->p);
+            uint32_t rbits = 0;
+            memcpy(&rbits, &c->radius, 4);
+            mix((uint64_t)n);
+            mix((uint64_t)T);
+            mix((uint64_t)c->index_base);
+            mix((uint64_t)s.key_bits);
+            mix((uint64_t)rbits);
+            SetState::StepGraph& g = s.rg[s.cur & 1];
+            if (!g.exec || g.key != key) {
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+                g.exec = nullptr;
+                g.key = 0;
+                HIP_TRY_RC(capture_graph(c, [&]() { return enqueue(s.cin.as<float>()); }, &g.exec));
+                g.key = key;
+            }
+            HIP_TRY(c, hipGraphLaunch(g.exec, c->stream));
         }
-        HIP_TRY(c, s.soff.ensure(16));
+        cur = s.cur ^ 1;
+        s.refit_pending = true;
+    } else {
+        {
+            Phase ph(c, OCT_PROF_BOUNDS);
+            HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, b, c->stream));
+        }
+        Phase ph(c, OCT_PROF_BUILD);
+        HIP_TRY_RC(read_params(c, s));  // the key width sizes the radix passes
+        s.key_bits = s.hp.key_bits;
+        HIP_TRY(c, launch_sort_stage(s.coords, n, s.key_bits, c->index_base, b, &cur, c->stream));
         HIP_TRY(c, launch_single_set_offsets(s.soff.as<int64_t>(), n, c->stream));
-        const Forest f{0, s.soff.as<int64_t>(), s.prm.as<BuildParams>()};
         HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, cur, b, f, c->stream));
     }
     // no wait here: the tree counts come back with the next wait that needs them
@@ -509,6 +593,7 @@ int oct_destroy(oct_ctx* c) {
     for (auto e : c->spare) (void)hipEventDestroy(e);
     for (auto e : c->join)
         if (e) (void)hipEventDestroy(e);
+    if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OCT_OK;
@@ -531,13 +616,20 @@ int oct_stream_join(oct_ctx* c, void* other, int direction) {
 
 int oct_set_stream(oct_ctx* c, void* stream) {
     if (!c) return OCT_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
+    const hipStream_t s = (hipStream_t)stream;
+    if (s == c->stream) return OCT_OK;  // the common case of a caller that follows its current stream
+    HIP_TRY(c, hipSetDevice(c->device));
     if (c->own_stream) {
-        (void)hipStreamSynchronize(c->stream);
-        (void)hipStreamDestroy(c->stream);
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipStreamDestroy(c->stream));
         c->own_stream = false;
+    } else if (c->stream) {  // work queued on the previous stream stays ahead of the new one's
+        hipEvent_t& e = c->join[0];
+        if (!e) HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIP_TRY(c, hipEventRecord(e, c->stream));
+        HIP_TRY(c, hipStreamWaitEvent(s, e, 0));
     }
-    c->stream = (hipStream_t)stream;
+    c->stream = s;
     return OCT_OK;
 }
 
@@ -1364,17 +1456,7 @@ int batch_eager(oct_ctx* c, BatchState& B) {
         B.replayed = 0;
         if (B.use_graph) {
             // capture the same step for the next calls (a graph replays kernels only)
-            HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int erc = batch_enqueue(c, B);
-            hipGraph_t g = nullptr;
-            const hipError_t ee = hipStreamEndCapture(c->stream, &g);
-            if (erc != OCT_OK) {
-                if (g) (void)hipGraphDestroy(g);
-                return erc;
-            }
-            HIP_TRY(c, ee);
-            B.graph = g;
-            HIP_TRY(c, hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+            HIP_TRY_RC(capture_graph(c, [&]() { return batch_enqueue(c, B); }, &B.exec));
         }
         return OCT_OK;
     }

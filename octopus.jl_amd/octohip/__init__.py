@@ -82,7 +82,7 @@ class TNS:
     device behave as in the reference even without a GPU."""
 
     def __init__(self, dtype=None, ndims: int = 3, device=None, index_base: int = 0,
-                 private_stream: bool = True):
+                 private_stream: bool = False):
         torch = _torch()
         dtype = torch.float32 if dtype is None else dtype
         if dtype != torch.float32:
@@ -117,11 +117,15 @@ class TNS:
             if rc != L.OCT_OK:
                 raise OctopusError(f"oct_create failed ({rc})")
             self._ctx = h
-            # The library runs on its own stream (default priority); calls
-            # order it after the caller's current stream and back (_enter/_leave).
-            self._stream = (torch.cuda.Stream(device=self.device) if self._private_stream
-                            else torch.cuda.current_stream(self.device))
-            self._check(self._lib.oct_set_stream(h, ctypes.c_void_p(self._stream.cuda_stream)))
+            # By default the library runs on the caller's current stream, like
+            # any torch op (set again on every call: free while it is the same
+            # stream).  private_stream: its own stream (default priority),
+            # ordered after the caller's current stream and back (_enter/_leave).
+            if self._private_stream:
+                self._stream = torch.cuda.Stream(device=self.device)
+                self._check(self._lib.oct_set_stream(h, ctypes.c_void_p(self._stream.cuda_stream)))
+            else:
+                self._check(self._lib.oct_set_stream(h, self._caller_stream()))
             if self.radius > 0:
                 self._check(self._lib.oct_set_search_radius(h, self.radius))
             if self.target_leaf_size != 32:
@@ -131,10 +135,12 @@ class TNS:
         return self._ctx
 
     def _enter(self):
-        """Library work waits for everything queued on the caller's stream."""
+        """Library work is ordered after everything queued on the caller's stream."""
         ctx = self._c()
         if self._private_stream:
             self._check(self._lib.oct_stream_join(ctx, self._caller_stream(), 0))
+        else:
+            self._check(self._lib.oct_set_stream(ctx, self._caller_stream()))
         return ctx
 
     def _leave(self):
@@ -143,7 +149,11 @@ class TNS:
             self._check(self._lib.oct_stream_join(self._ctx, self._caller_stream(), 1))
 
     def _caller_stream(self):
-        return ctypes.c_void_p(_torch().cuda.current_stream(self.device).cuda_stream)
+        torch = _torch()
+        raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+        if raw is not None:  # the handle without building a Stream object
+            return ctypes.c_void_p(raw(self.device.index))
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def _check(self, rc):
         if rc == L.OCT_OK:

@@ -434,3 +434,33 @@ def test_leaf_count_growth_between_builds(orc, mode):
     e = octohip.build_edges(tns, sid, sid, mode=mode)  # the count is known now: one attempt
     assert predictor.edge_stats(tns, sid, sid)["attempts"] == 1
     _edges_equal(_gpu_edges(e), (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+
+
+@pytest.mark.parametrize("private", [False, True])
+def test_caller_streams(orc, private):
+    """The library follows torch's current stream (or, private_stream, joins
+    its own stream to it): results produced under a side stream and then on
+    the default stream are the oracle's, with no explicit synchronization."""
+    import octohip
+
+    torch = _torch()
+    pts = datasets.uniform_cube(30000, 71)
+    r = 0.03
+    o = orc.build_edges(orc.run(pts, r), pts)
+    want = (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm)
+    tns = octohip.TNS(private_stream=private)
+    octohip.set_search_radius(tns, r)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        c = torch.from_numpy(pts).cuda()
+        sid = octohip.add_point_set(tns, c)
+        octohip.set_active_search(tns, sid, sid)
+        octohip.run(tns)
+        for mode in ("exact", "learned"):
+            _edges_equal(_gpu_edges(octohip.build_edges(tns, sid, sid, mode=mode)), want)
+    c2 = c.clone()  # on the default stream, after the side stream's work
+    torch.cuda.current_stream().wait_stream(side)
+    octohip.update_point_set(tns, sid, c2)
+    octohip.run(tns)
+    for mode in ("exact", "learned"):
+        _edges_equal(_gpu_edges(octohip.build_edges(tns, sid, sid, mode=mode)), want)
