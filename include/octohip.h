@@ -168,7 +168,7 @@ int oct_build_edges_backward(oct_ctx* ctx, int64_t n_points, const oct_edges* ed
  * default).  oct_profile_read waits for the stream, then writes the summed
  * milliseconds and launch counts of the OCT_PROF_NPHASES phases below. */
 #define OCT_PROF_BOUNDS 0      /* origin / extent reduction */
-#define OCT_PROF_BUILD 1       /* keys, radix sort, topology, refit */
+#define OCT_PROF_BUILD 1       /* keys, radix sort, topology, refit (a refit step: with its bounds) */
 #define OCT_PROF_LEAF_LISTS 2  /* candidate target leaves per query leaf */
 #define OCT_PROF_COUNT 3       /* per-query edge counts */
 #define OCT_PROF_SCAN 4        /* exclusive scan of the counts */
@@ -178,6 +178,9 @@ int oct_build_edges_backward(oct_ctx* ctx, int64_t n_points, const oct_edges* ed
 #define OCT_PROF_PREDICT 8     /* predictor features, inference and plan */
 #define OCT_PROF_NPHASES 9
 int oct_profile(oct_ctx* ctx, int enable);
+/* The same for the phases whose bits (1 << OCT_PROF_*) are set: timing one
+ * phase costs two events per launch of it. */
+int oct_profile_phases(oct_ctx* ctx, uint32_t mask);
 int oct_profile_read(oct_ctx* ctx, int reset, double* ms, int64_t* calls);
 
 /* Block until all work queued on the context's stream is done. */

@@ -225,7 +225,8 @@ struct oct_ctx {
     std::string err;
     void* pinned = nullptr;  // small host staging area for read-backs
     // optional per-phase kernel timing with HIP events on the context stream
-    bool profiling = false;
+    uint32_t prof_mask = 0;  // phases timed with events (bit per OCT_PROF_*)
+    bool capturing = false;  // a graph capture is open: no phase events
     struct Mark { int phase; hipEvent_t a, b; };
     std::vector<Mark> marks;
     std::vector<hipEvent_t> spare;
@@ -270,13 +271,13 @@ struct Phase {
     int ph;
     hipEvent_t a = nullptr;
     Phase(oct_ctx* ctx, int phase) : c(ctx), ph(phase) {
-        if (c->profiling) {
+        if (((c->prof_mask >> ph) & 1u) && !c->capturing) {
             a = take_event(c);
             (void)hipEventRecord(a, c->stream);
         }
     }
     ~Phase() {
-        if (c->profiling && a) {
+        if (a) {
             hipEvent_t b = take_event(c);
             (void)hipEventRecord(b, c->stream);
             c->marks.push_back({ph, a, b});
@@ -317,7 +318,9 @@ int capture_graph(oct_ctx* c, F enqueue, hipGraphExec_t* exec) {
         c->stream = saved;
         HIP_TRY(c, be);
     }
+    c->capturing = true;
     const int erc = enqueue();
+    c->capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(c->stream, &g);
     c->stream = saved;
@@ -364,12 +367,23 @@ int mark_all_dirty(oct_ctx* c) {
 
 // Waits for the stream and copies the build parameters (and, when asked, the
 // 64-bit scan total) to the host.
+// Waits for the context's stream by polling it: a read-back in the middle of
+// a step costs the wake-up latency of a blocking wait every time, which
+// polling avoids (the host thread spins for the duration instead).
+hipError_t wait_stream(hipStream_t s) {
+    for (;;) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e != hipErrorNotReady) return e;
+        __builtin_ia32_pause();
+    }
+}
+
 int read_params(oct_ctx* c, SetState& s, uint64_t* total = nullptr) {
     HIP_TRY(c, hipMemcpyAsync(c->pinned, s.prm.p, sizeof(BuildParams), hipMemcpyDeviceToHost, c->stream));
     if (total)
         HIP_TRY(c, hipMemcpyAsync((char*)c->pinned + 1024, s.total.p, 8, hipMemcpyDeviceToHost, c->stream));
     HostTimer ht("run: wait for build parameters");
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, wait_stream(c->stream));
     memcpy(&s.hp, c->pinned, sizeof(BuildParams));
     if (total) memcpy(total, (char*)c->pinned + 1024, 8);
     return OCT_OK;
@@ -493,9 +507,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
             HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, s.cur ^ 1, b, f, c->stream));
             return OCT_OK;
         };
-        if (c->profiling) {  // phase events are not captured: launch one by one
-            HIP_TRY_RC(enqueue(s.coords));
-        } else {
+        {
             HIP_TRY(c, s.cin.ensure(12 * (size_t)n));
             HIP_TRY(c, hipMemcpyAsync(s.cin.p, s.coords, 12 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
             // the graph's key: everything its launches bake in
@@ -521,6 +533,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
                 HIP_TRY_RC(capture_graph(c, [&]() { return enqueue(s.cin.as<float>()); }, &g.exec));
                 g.key = key;
             }
+            Phase ph(c, OCT_PROF_BUILD);  // the whole step: bounds, refit and topology
             HIP_TRY(c, hipGraphLaunch(g.exec, c->stream));
         }
         cur = s.cur ^ 1;
@@ -892,7 +905,7 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
                                   c->stream));
         {
             HostTimer ht(single ? "edges: wait for single pass" : "edges: wait for count total");
-            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            HIP_TRY(c, wait_stream(c->stream));
         }
         memcpy(hf, c->pinned, 4 * HF_WORDS);
         take_hp(qs, (char*)c->pinned + HF_BYTES);
@@ -1062,17 +1075,19 @@ int oct_build_edges_backward(oct_ctx* c, int64_t n_points, const oct_edges* e, c
     return OCT_OK;
 }
 
-int oct_profile(oct_ctx* c, int enable) {
+int oct_profile_phases(oct_ctx* c, uint32_t mask) {
     if (!c) return OCT_ERR_ARGUMENT;
-    c->profiling = enable != 0;
+    c->prof_mask = mask & ((1u << OCT_PROF_NPHASES) - 1u);
     // events are created up front so recording them costs no allocation
-    while (c->profiling && c->spare.size() < 256) {
+    while (c->prof_mask && c->spare.size() < 256) {
         hipEvent_t e = nullptr;
         HIP_TRY(c, hipEventCreate(&e));
         c->spare.push_back(e);
     }
     return OCT_OK;
 }
+
+int oct_profile(oct_ctx* c, int enable) { return oct_profile_phases(c, enable ? ~0u : 0u); }
 
 int oct_profile_read(oct_ctx* c, int reset, double* ms, int64_t* calls) {
     if (!c || !ms || !calls) return OCT_ERR_ARGUMENT;
@@ -1402,7 +1417,7 @@ void batch_shifts(BatchState& B) {
 
 int batch_read_flags(oct_ctx* c, BatchState& B, uint32_t* hf) {
     HIP_TRY(c, hipMemcpyAsync(c->pinned, B.pair.flags.p, 4 * BF_WORDS, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, wait_stream(c->stream));
     memcpy(hf, c->pinned, 4 * BF_WORDS);
     return OCT_OK;
 }

@@ -424,9 +424,18 @@ def autotune(tns: TNS, q: int, t: int, mode: str = "learned", leaf_sizes=(8, 16,
     return {"best": top, "ms": table}
 
 
-def profile(tns: TNS, enable: bool = True):
-    """Turn per-phase kernel timing (HIP events on the TNS stream) on or off."""
-    tns._check(tns._lib.oct_profile(tns._c(), int(bool(enable))))
+def profile(tns: TNS, enable: bool = True, phases=None):
+    """Turn per-phase kernel timing (HIP events on the TNS stream) on or off;
+    phases: names from PROF_PHASES to time (default: all)."""
+    if phases is None or not enable:
+        tns._check(tns._lib.oct_profile(tns._c(), int(bool(enable))))
+        return
+    mask = 0
+    for name in phases:
+        if name not in L.PROF_PHASES:
+            raise ArgumentError(f"unknown phase {name!r}; one of {L.PROF_PHASES}")
+        mask |= 1 << L.PROF_PHASES.index(name)
+    tns._check(tns._lib.oct_profile_phases(tns._c(), mask))
 
 
 def profile_read(tns: TNS, reset: bool = True) -> dict:

@@ -143,6 +143,7 @@ SIGNATURES = {
     "oct_build_edges_backward": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(OctEdges), _P, _P,
                                                 ctypes.c_float, _P]),
     "oct_profile": (ctypes.c_int, [_P, ctypes.c_int]),
+    "oct_profile_phases": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "oct_profile_read": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int64)]),
     "oct_synchronize": (ctypes.c_int, [_P]),
