@@ -31,7 +31,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_sph1m.json")
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100, help="timed steps (the SPH configuration's 100 time steps)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", default="both", choices=["both", "exact", "learned"],
                     help="both: learned (headline value) and exact, in that order")
@@ -50,8 +50,9 @@ def parse():
                     help="full sort every step (default: refit mode, the almost-sorted path of time stepping)")
     ap.add_argument("--predict-always", action="store_true",
                     help="run the predictor on every step (default: first call and after overflows / recounts)")
-    ap.add_argument("--workload", default="sph", choices=["sph", "batched"],
-                    help="sph: the headline 1M-point SPH block; batched: 256 uneven sets in one replayed graph")
+    ap.add_argument("--workload", default="sph", choices=["sph", "batched", "clustered"],
+                    help="sph: the headline 1M-point SPH block; batched: 256 uneven sets in one replayed graph; "
+                         "clustered: 4M clustered points (the learned predictor's own distribution)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched: keep --leaf-size / --run-leaves / --run-chunk")
     ap.add_argument("--leaf-size", type=int, default=32)
@@ -244,6 +245,81 @@ def bench_batched(a):
     }))
 
 
+def bench_clustered(a):
+    """4M points, 90% from a 16-cluster Gaussian mixture (sigma 0.05) and 10%
+    uniform, radius 0.01 (about 155 neighbours per point, 6e8 edges): the
+    learned predictor's own distribution.  A step is run! (refit of unmoved
+    points) + build_edges; learned (with the shipped clustered predictor) and
+    exact on the same points."""
+    import torch
+
+    import octohip
+    from octohip import datasets
+
+    pts, r = datasets.clustered(seed=4)
+    n = pts.shape[0]
+    coords = torch.from_numpy(pts).cuda()
+    steps = min(a.steps, 5)
+    res = {}
+    for mode in ("learned", "exact"):
+        tns = octohip.TNS()
+        octohip.set_search_radius(tns, float(r))
+        octohip.set_refit_mode(tns, True)
+        sid = octohip.add_point_set(tns, coords)
+        octohip.set_active_search(tns, sid, sid)
+        pred = None
+        if mode == "learned" and os.path.exists(os.path.join(octohip.predictor.SHIPPED, "clustered4m.f32")):
+            octohip.predictor.load(tns, "clustered4m")
+            pred = "clustered4m"
+        first = None
+        for k in range(2):
+            octohip.update_point_set(tns, sid, coords)
+            octohip.run(tns)
+            octohip.build_edges(tns, sid, sid, mode=mode)
+            if k == 0:
+                first = octohip.predictor.edge_stats(tns, sid, sid, tensors=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        edges = 0
+        for k in range(steps):
+            octohip.update_point_set(tns, sid, coords)
+            octohip.run(tns)
+            edges += octohip.build_edges(tns, sid, sid, mode=mode).senders.shape[0]
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        st = octohip.predictor.edge_stats(tns, sid, sid, tensors=False)
+        octohip.profile(tns, True)
+        octohip.profile_read(tns, reset=True)
+        octohip.update_point_set(tns, sid, coords)
+        octohip.run(tns)
+        octohip.build_edges(tns, sid, sid, mode=mode)
+        torch.cuda.synchronize()
+        prof = octohip.profile_read(tns, reset=True)
+        octohip.profile(tns, False)
+        res[mode] = {"value": edges / sec, "ms_per_step": 1e3 * sec / steps, "edges_per_step": edges / steps,
+                     "breakdown_ms_per_step": {k: v[0] for k, v in prof.items() if v[1]},
+                     "predictor": pred,
+                     "first_call": {k: first[k] for k in ("single_pass", "used_predictor", "attempts",
+                                                          "predicted_total", "n_edges")},
+                     "last_step": {k: st[k] for k in ("single_pass", "used_predictor", "attempts",
+                                                      "recounted_leaves", "recorded_leaves", "heavy_leaves",
+                                                      "n_leaves")}}
+        del tns
+        torch.cuda.synchronize()
+    head = res["learned"]
+    print(json.dumps({
+        "metric": "edges/s, learned vs exact build_edges, 4M clustered points",
+        "value": head["value"], "unit": "edges/s", "n_gpus": 1, "steps": steps, "warmup": 2,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "4,000,000 points: 90% from a 16-cluster Gaussian mixture (sigma 0.05), 10% uniform, "
+                               "radius 0.01; run! + build_edges per step on unmoved points",
+                   "points": n, "edges_per_step": head["edges_per_step"]},
+        "learned_over_exact": head["value"] / res["exact"]["value"],
+        "modes": res,
+    }))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -252,6 +328,9 @@ def main():
         sys.exit(2)
     if a.workload == "batched":
         bench_batched(a)
+        return
+    if a.workload == "clustered":
+        bench_clustered(a)
         return
     import torch
 
@@ -306,7 +385,7 @@ def main():
 
         oracle.build()
         threads = oracle.max_threads()
-        host = [s.cpu().numpy() for s in snaps]
+        host = [snaps[k].cpu().numpy() for k in range(min(a.cpu_steps, len(snaps)))]
         ce = 0
         c0 = time.perf_counter()
         for k in range(a.cpu_steps):

@@ -889,6 +889,11 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
                                            (uint64_t)p.cap_edges, eo, c->index_base, c->run_leaves, c->run_chunk,
                                            SetLookup{},
                                            heavy ? p.heavy_flags(nql) : nullptr, c->stream));
+            if (heavy)
+                HIP_TRY(c, launch_edges_heavy_fill(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
+                                                   inv_r, self_pair, p.heavy_list(), flags + 16,
+                                                   p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), flags + 4, eo,
+                                                   c->index_base, SetLookup{}, c->stream));
         } else {
             {
                 Phase ph(c, OCT_PROF_COUNT);
@@ -996,7 +1001,10 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     p.st.used_predictor = predict && single ? 1 : 0;
     p.st.attempts = attempts;
     if (mode == OCT_EDGES_LEARNED) {
-        p.replan = !single || attempts > 1 || (int64_t)hf[9] * 64 > n_leaves;
+        // a new plan helps leaves that were recounted because a run's list
+        // filled up, not heavy leaves (no plan records those)
+        const int64_t heavy_n = heavy && single ? (int64_t)hf[16] : 0;
+        p.replan = !single || attempts > 1 || ((int64_t)hf[9] - heavy_n) * 64 > n_leaves;
         // uneven leaves (many not recorded in one sweep): count the heavy ones
         // ahead next time; even ones skip the estimate and the extra launch
         if (single) p.heavy_on = (int64_t)hf[9] * 64 > n_leaves;
@@ -1397,6 +1405,9 @@ int batch_enqueue(oct_ctx* c, BatchState& B) {
                                    (uint64_t)B.cap_edges, eo, c->index_base, B.run_leaves, B.run_chunk, sets,
                                    p.heavy_flags(N),
                                    st));
+    HIP_TRY(c, launch_edges_heavy_fill(v, v, p.lists.as<int2>(), p.lcount.as<int32_t>(), B.cap, 0.f, 0.f, 1,
+                                       p.heavy_list(), flags + 11, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(),
+                                       flags + 4, eo, c->index_base, sets, st));
     HIP_TRY(c, exclusive_scan(p.lhits.as<uint32_t>(), B.loff.as<uint32_t>(), N, B.bscan.as<uint32_t>(),
                               (uint64_t*)(flags + 6), st));
     HIP_TRY(c, launch_batch_edge_offsets(B.loff.as<uint32_t>(), (const uint64_t*)(flags + 6), v.leaves, v.params,

@@ -673,6 +673,52 @@ k_edges_heavy(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qs
     }
 }
 
+// Writes the edges of the heavy leaves after the single pass, one wave per
+// leaf: the single pass left each one's output position in qoff at its first
+// query, and its per-query counts in qcount came from k_edges_heavy.  Nothing
+// runs when the single pass raised flags[0] (capacity) or flags[1]
+// (look-back or plan): the host then redoes the call.
+__global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
+k_edges_heavy_fill(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
+                   const BuildParams* __restrict__ qprm, const float4* __restrict__ tsc,
+                   const int2* __restrict__ lists, const int32_t* __restrict__ lcount, int cap, float r_sq,
+                   float inv_r, int self_pair, const int32_t* __restrict__ heavy_list,
+                   const uint32_t* __restrict__ n_heavy, uint32_t* __restrict__ qcount, uint32_t* __restrict__ qoff,
+                   const uint32_t* __restrict__ flags, EdgeOut out, int index_base0, SetLookup sets) {
+    __shared__ EdgeLds<EDGE_FILL> lds;
+    if (flags[0] != 0 || flags[1] != 0) return;
+    const int wv = threadIdx.x / OCT_WAVE;
+    const int lane = lane_id();
+    const WaveLds L = lds.wave(wv);
+    const int32_t nleaves = qprm->n_leaves;
+    const int32_t nh = (int32_t)min(*n_heavy, (uint32_t)max(nleaves, 0));
+    const int32_t nwaves = (int32_t)(gridDim.x * EW_WAVES);
+    for (int32_t i = (int32_t)blockIdx.x * EW_WAVES + wv; i < nh; i += nwaves) {
+        const int32_t r = __builtin_amdgcn_readfirstlane(heavy_list[i]);
+        if (r < 0 || r >= nleaves) continue;  // wave-uniform; k_leaf_lists writes only leaves
+        const NodeRec qa = qleaves[r];
+        const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+        const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+        uint32_t run = (uint32_t)__builtin_amdgcn_readfirstlane((int)qoff[qf]);  // before the scan rewrites it
+        __builtin_amdgcn_wave_barrier();
+        for (int qb = 0; qb < qc; qb += OCT_WAVE) {
+            const int nqb = min(OCT_WAVE, qc - qb);
+            const uint32_t c = lane < nqb ? qcount[qf + qb + lane] : 0u;
+            const uint32_t inc = (uint32_t)wave_incl_scan((int)c);
+            if (lane < nqb) qoff[qf + qb + lane] = run + inc - c;
+            run += (uint32_t)__builtin_amdgcn_readlane((int)inc, OCT_WAVE - 1);
+        }
+        const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+        const int2* lst = lists + (int64_t)r * cap;
+        const int2 lf0 = lst[lane < ncand ? lane : 0];
+        float lr_sq, linv_r;
+        int libase;
+        leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
+        leaf_pass<EDGE_FILL>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r, self_pair, 0, qcount,
+                             qoff, out, libase, L, lane);
+    }
+}
+
 // ------------------------------------------------------------ single pass
 
 constexpr int SP_MAX_LEAVES = SP_PLAN_MAX_LEAVES;  // leaves one wave may own under a plan
@@ -798,10 +844,12 @@ __device__ inline uint64_t wave_look_back(const uint64_t* tile_state, int32_t t,
 // run a block holds but has not started is above every run its waves hold, so
 // the lowest run without an aggregate is either in a phase A (which waits on
 // nothing) or next in line at a block whose waves only wait on lower, already
-// published runs: every wait ends (and is bounded anyway).  With lheavy, the leaves it flags were counted ahead by
-// k_edges_heavy (totals in lhits, per-query counts in qcount): phase A takes
-// their totals as they are, so a run publishes its aggregate without waiting
-// on its heaviest leaves, and phase B sweeps them once, with the FILL body.
+// published runs: every wait ends (and is bounded anyway).  With lheavy, the
+// leaves it flags were counted ahead by k_edges_heavy (totals in lhits,
+// per-query counts in qcount): phase A takes their totals as they are, so a
+// run publishes its aggregate without waiting on its heaviest leaves, and
+// phase B only leaves their output positions for k_edges_heavy_fill: the
+// two-pass path, for exactly the leaves the single pass cannot record.
 // WIDE: 8-byte hits (any point count; half the list).
 template <bool WIDE>
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
@@ -856,6 +904,7 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
         }
         // phase A: one sweep per leaf, exact counts
         uint64_t recorded = 0;  // bit k: leaf r_lo + k sits in the hit list
+        uint64_t counted = 0;   // bit k: leaf r_lo + k was counted by the heavy-leaf pass
         uint32_t hfill = 0, wtot = 0, nrec = 0;
         for (int32_t r = r_lo; r < r_hi; ++r) {
             const int k = r - r_lo;
@@ -863,6 +912,7 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
             uint32_t c;
             if (pre) {
                 c = __builtin_amdgcn_readfirstlane(lhits[r]);  // counted by the heavy-leaf pass
+                counted |= 1ull << k;
             } else {
                 const NodeRec qa = qleaves[r];
                 const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
@@ -933,11 +983,16 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
             pos += c;
         }
         __builtin_amdgcn_wave_barrier();
-        // ... then the others re-swept; their stage lies over the hit list
+        // ... then the others re-swept; their stage lies over the hit list.
+        // A leaf the heavy-leaf pass counted only leaves its output position
+        // (in qoff at its first query) for k_edges_heavy_fill, so no run
+        // holds its wave for a long fill.
         pos = (uint32_t)excl;
         for (int32_t r = r_lo; r < r_hi; ++r) {
             const int k = r - r_lo;
-            if (!(recorded & (1ull << k))) {
+            if (counted & (1ull << k)) {
+                if (lane == 0) qoff[rec_a(qleaves[r])] = pos;
+            } else if (!(recorded & (1ull << k))) {
                 const NodeRec qa = qleaves[r];
                 const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
                 const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
@@ -1084,6 +1139,18 @@ __global__ void k_clamp_leaves(const BuildParams* __restrict__ src, const BuildP
 hipError_t launch_clamp_leaves(const BuildParams* src, const BuildParams* tsrc, BuildParams* dst, int64_t rows,
                                uint32_t* over, BuildParams* raw, hipStream_t s) {
     hipLaunchKernelGGL(k_clamp_leaves, dim3(1), dim3(1), 0, s, src, tsrc, dst, rows, over, raw);
+    return hipGetLastError();
+}
+
+hipError_t launch_edges_heavy_fill(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
+                                   int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
+                                   const uint32_t* n_heavy, uint32_t* qcount, uint32_t* qoff, const uint32_t* flags,
+                                   EdgeOut out, int index_base, const SetLookup& sets, hipStream_t s) {
+    const int64_t want = ((int64_t)q.n_leaves + EW_WAVES - 1) / EW_WAVES;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, 2048));
+    hipLaunchKernelGGL(k_edges_heavy_fill, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
+                       t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, heavy_list, n_heavy, qcount, qoff,
+                       flags, out, index_base, sets);
     return hipGetLastError();
 }
 

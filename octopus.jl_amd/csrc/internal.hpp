@@ -236,6 +236,13 @@ hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, i
 // raw[0..1] receive *src and *tsrc unchanged (read back with the flags).
 hipError_t launch_clamp_leaves(const BuildParams* src, const BuildParams* tsrc, BuildParams* dst, int64_t rows,
                                uint32_t* over, BuildParams* raw, hipStream_t s);
+// Edges of the listed heavy leaves after the single pass (which leaves each
+// one's output position in qoff at its first query); flags = the single
+// pass's (nothing runs when it raised flags[0] or flags[1]).
+hipError_t launch_edges_heavy_fill(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
+                                   int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
+                                   const uint32_t* n_heavy, uint32_t* qcount, uint32_t* qoff, const uint32_t* flags,
+                                   EdgeOut out, int index_base, const SetLookup& sets, hipStream_t s);
 // Exact counts of the listed heavy leaves ahead of the single pass: per-query
 // counts into qcount, leaf totals into lhits.
 hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
