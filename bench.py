@@ -113,6 +113,7 @@ def run_mode(a, mode, snaps, r):
     t1 = time.perf_counter()
     st = octohip.predictor.edge_stats(tns, sid, sid)
     kprof = octohip.profile_read(tns, reset=True)[kphase] if not a.no_profile else (0.0, 0)
+    used_timed, refits_timed = used[0], refits[0]
     prof = {}
     if not a.no_profile:
         octohip.profile(tns, True)
@@ -132,8 +133,8 @@ def run_mode(a, mode, snaps, r):
         "timed_kernel": {"phase": kphase, "ms_total": kprof[0], "launches": kprof[1]},
         "predictor": predictor,
         "first_call": {k: first_call[k] for k in ("single_pass", "used_predictor", "attempts")},
-        "timed_steps_using_predictor": used[0],
-        "timed_steps_refit": refits[0],
+        "timed_steps_using_predictor": used_timed,
+        "timed_steps_refit": refits_timed,
         "last_step": {k: st[k] for k in ("single_pass", "used_predictor", "attempts", "recounted_leaves",
                                          "recorded_leaves", "predicted_total", "n_edges", "n_leaves")},
     }
