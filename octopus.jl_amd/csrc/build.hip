@@ -213,21 +213,21 @@ k_refit_keys(const float* __restrict__ coords, int64_t n, const BuildParams* __r
     chg[i] = k != pk[i] ? 1u : 0u;
 }
 
-// K2b: the changed count, and whether the refit can run (prm->refit_fail).
-__global__ void k_refit_check(const uint64_t* __restrict__ total, int64_t limit, int key_bits,
-                              BuildParams* __restrict__ prm) {
-    const uint64_t nc = *total;
-    prm->changed = (uint32_t)(nc < 0xffffffffull ? nc : 0xffffffffull);
-    prm->refit_fail = (nc > (uint64_t)limit || prm->key_bits != key_bits) ? 1u : 0u;
-}
-
-// K3: compact the changed points (their new key and index).
+// K3: whether the refit can run (the changed count within limit and the key
+// width unchanged; thread 0 records both in prm for the later kernels and the
+// host), then compact the changed points (their new key and index).
 __global__ void __launch_bounds__(RF_THREADS)
 k_refit_compact(int64_t n, const uint32_t* __restrict__ cpos, const uint64_t* __restrict__ total,
                 const uint64_t* __restrict__ nk, const int32_t* __restrict__ pi, uint64_t* __restrict__ ck,
-                int32_t* __restrict__ ci, const BuildParams* __restrict__ prm) {
+                int32_t* __restrict__ ci, int64_t limit, int key_bits, BuildParams* __restrict__ prm) {
     const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
-    if (i >= n || prm->refit_fail) return;
+    const uint64_t nc = *total;
+    const bool fail = nc > (uint64_t)limit || prm->key_bits != key_bits;
+    if (i == 0) {
+        prm->changed = (uint32_t)(nc < 0xffffffffull ? nc : 0xffffffffull);
+        prm->refit_fail = fail ? 1u : 0u;
+    }
+    if (i >= n || fail) return;
     const uint32_t c = cpos[i];
     const uint32_t next = i + 1 < n ? cpos[i + 1] : (uint32_t)*total;
     if (next != c) {
@@ -439,10 +439,11 @@ __device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
 __global__ void __launch_bounds__(TOPO_THREADS)
 k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, int64_t n, int T,
             const uint16_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
-            BuildParams* __restrict__ prm, Forest forest, int index_base, NodeRec* __restrict__ nodes,
-            int32_t* __restrict__ nfirst, int32_t* __restrict__ nlast, int32_t* __restrict__ nchild,
-            float* __restrict__ bmin, float* __restrict__ bmax, int32_t* __restrict__ parent,
-            uint8_t* __restrict__ nlevel, int32_t* __restrict__ leaf_ids, NodeRec* __restrict__ leaves) {
+            const uint64_t* __restrict__ total, BuildParams* __restrict__ prm, Forest forest, int index_base,
+            NodeRec* __restrict__ nodes, int32_t* __restrict__ nfirst, int32_t* __restrict__ nlast,
+            int32_t* __restrict__ nchild, float* __restrict__ bmin, float* __restrict__ bmax,
+            int32_t* __restrict__ parent, uint8_t* __restrict__ nlevel, int32_t* __restrict__ leaf_ids,
+            NodeRec* __restrict__ leaves) {
     __shared__ uint32_t wc[TOPO_THREADS / OCT_WAVE][OCT_NLEVELS + 1];
     __shared__ int8_t mm[TOPO_THREADS + TOPO_MAX_T + 1];  // m of positions base .. base + 256 + T
     const int64_t base = (int64_t)blockIdx.x * TOPO_THREADS;
@@ -470,7 +471,10 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
     }
     __syncthreads();
     const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
-    if (k == 0) prm->n_nodes = M;
+    if (k == 0) {
+        prm->n_nodes = M;
+        prm->n_leaves = (int32_t)(*total) - M;  // the scan's last row counts the leaves
+    }
     if (!emit) return;
     // the set of this position, its point range and chain depth (levels
     // 0..L0 hold all of the set's points)
@@ -617,10 +621,6 @@ k_level_refit(int L, const uint32_t* __restrict__ hoff, int nblk, const BuildPar
     }
 }
 
-__global__ void k_set_leaf_count(BuildParams* prm, const uint64_t* total) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) prm->n_leaves = (int32_t)(*total) - prm->n_nodes;
-}
-
 // ============================================================== driver
 
 int bounds_blocks(int64_t n) {
@@ -684,26 +684,22 @@ hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int i
 // (keys[cur] and scoord hold the previous sort) and the changed count in
 // *b.total.  The caller reads the count and either finishes the refit or
 // runs the full sort.
-hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, int64_t limit, int key_bits,
-                                const BuildBufs& b, hipStream_t s) {
-    if (limit > REFIT_MAX_CHANGED) return hipErrorInvalidValue;  // the changed-point buffers hold that many
+hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, const BuildBufs& b, hipStream_t s) {
     const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
     hipLaunchKernelGGL(k_refit_keys, dim3(g), dim3(RF_THREADS), 0, s, coords, n, b.prm, b.keys[cur], b.scoord,
                        b.keys[cur ^ 1], b.vals[0], (uint32_t*)b.vals[1]);
-    hipError_t e = exclusive_scan((const uint32_t*)b.vals[1], (uint32_t*)b.vals[1], n, b.scan_tmp, b.total, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_refit_check, dim3(1), dim3(1), 0, s, b.total, limit, key_bits, b.prm);
-    return hipGetLastError();
+    return exclusive_scan((const uint32_t*)b.vals[1], (uint32_t*)b.vals[1], n, b.scan_tmp, b.total, s);
 }
 
 // Part 2: the sorted keys end in keys[cur ^ 1].  Grids cover `limit` changed
 // points; the kernels read the real count (and refit_fail) on the device.
-hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t limit, int cur, int index_base,
-                               const BuildBufs& b, const RefitBufs& r, hipStream_t s) {
+hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t limit, int key_bits, int cur,
+                               int index_base, const BuildBufs& b, const RefitBufs& r, hipStream_t s) {
+    if (limit > REFIT_MAX_CHANGED) return hipErrorInvalidValue;  // the changed-point buffers hold that many
     const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
     const uint32_t* cpos = (const uint32_t*)b.vals[1];
     hipLaunchKernelGGL(k_refit_compact, dim3(g), dim3(RF_THREADS), 0, s, n, cpos, b.total, b.keys[cur ^ 1], b.vals[0],
-                       r.ck, r.ci, b.prm);
+                       r.ck, r.ci, limit, key_bits, b.prm);
     if (limit > 0) {
         const unsigned gc = (unsigned)((limit + RF_THREADS - 1) / RF_THREADS);
         hipLaunchKernelGGL(k_refit_rank, dim3(gc), dim3(RF_THREADS), 0, s, n, b.total, r.ck, r.ci, b.keys[cur],
@@ -726,9 +722,8 @@ hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base,
     hipError_t e = exclusive_scan(b.hist, b.hist, (int64_t)(OCT_NLEVELS + 1) * nblk, b.scan_tmp, b.total, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, b.scoord, n, T, b.topo, b.hist,
-                       nblk, b.prm, f, index_base, b.nodes, b.nfirst, b.nlast, b.nchild, b.bmin, b.bmax,
+                       nblk, b.total, b.prm, f, index_base, b.nodes, b.nfirst, b.nlast, b.nchild, b.bmin, b.bmax,
                        b.parent, b.nlevel, b.leaf_ids, b.leaves);
-    hipLaunchKernelGGL(k_set_leaf_count, dim3(1), dim3(64), 0, s, b.prm, b.total);
     const unsigned gl = (unsigned)std::min<int64_t>((max_nodes(n, T) + 255) / 256, 2048);
     hipLaunchKernelGGL(k_link, dim3(gl), dim3(256), 0, s, sk, b.prm, f.batched, index_base, b.parent, b.nfirst,
                        b.nlevel, b.nchild, b.nodes);

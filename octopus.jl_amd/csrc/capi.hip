@@ -66,6 +66,7 @@ struct SetState {
     oct_build_info info{};
     DevBuf rk, ri, rsk, rsi, rpos;  // refit: changed points
     DevBuf soff;                    // {0, n}: the one-set forest descriptor
+    int64_t soff_n = -1;            // the n soff holds
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
         nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax;
     BuildBufs bufs() const {
@@ -117,6 +118,7 @@ struct SetState {
                          &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
                          &rk, &ri, &rsk, &rsi, &rpos, &soff, &cin};
         for (DevBuf* d : all) d->release();
+        soff_n = -1;
     }
 };
 
@@ -483,6 +485,10 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
     memset(&s.info, 0, sizeof(s.info));
     s.info.changed_points = -1;
     HIP_TRY(c, s.soff.ensure(16));
+    if (s.soff_n != n) {  // {0, n}: the one-set forest descriptor, written when n changes
+        HIP_TRY(c, launch_single_set_offsets(s.soff.as<int64_t>(), n, c->stream));
+        s.soff_n = n;
+    }
     const Forest f{0, s.soff.as<int64_t>(), s.prm.as<BuildParams>()};
     int cur = 0;
     if (try_refit) {
@@ -501,9 +507,8 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
                 HIP_TRY(c, launch_bounds_stage(coords, n, c->radius, b, c->stream));
             }
             Phase ph(c, OCT_PROF_BUILD);
-            HIP_TRY(c, launch_refit_prepare(coords, n, s.cur, limit, s.key_bits, b, c->stream));
-            HIP_TRY(c, launch_refit_finish(coords, n, limit, s.cur, c->index_base, b, r, c->stream));
-            HIP_TRY(c, launch_single_set_offsets(s.soff.as<int64_t>(), n, c->stream));
+            HIP_TRY(c, launch_refit_prepare(coords, n, s.cur, b, c->stream));
+            HIP_TRY(c, launch_refit_finish(coords, n, limit, s.key_bits, s.cur, c->index_base, b, r, c->stream));
             HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, s.cur ^ 1, b, f, c->stream));
             return OCT_OK;
         };
@@ -547,7 +552,6 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
         HIP_TRY_RC(read_params(c, s));  // the key width sizes the radix passes
         s.key_bits = s.hp.key_bits;
         HIP_TRY(c, launch_sort_stage(s.coords, n, s.key_bits, c->index_base, b, &cur, c->stream));
-        HIP_TRY(c, launch_single_set_offsets(s.soff.as<int64_t>(), n, c->stream));
         HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, cur, b, f, c->stream));
     }
     // no wait here: the tree counts come back with the next wait that needs them

@@ -182,11 +182,11 @@ hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base,
 hipError_t launch_single_set_offsets(int64_t* off, int64_t n, hipStream_t s);
 
 // Almost-sorted refit (time stepping): the previous sort is keys[cur] and
-// scoord.  prepare: new keys in that order, the changed count in *b.total and
-// prm->changed, and prm->refit_fail = 1 when that count passes `limit` (at
-// most REFIT_MAX_CHANGED) or the key width is no longer key_bits; finish:
+// scoord.  prepare: new keys in that order and the changed count in *b.total;
+// finish: prm->changed, and prm->refit_fail = 1 when that count passes
+// `limit` (at most REFIT_MAX_CHANGED) or the key width is no longer key_bits;
 // sorted keys in keys[cur ^ 1], perm and scoord rewritten, identical to the
-// full sort's, or (refit_fail) the previous sort copied to keys[cur ^ 1] with
+// then the full sort's order, or (refit_fail) the previous sort copied to keys[cur ^ 1] with
 // perm and scoord left as they were, a valid tree of the previous order that
 // the host replaces by a full rebuild.  No host wait in between.
 constexpr int64_t REFIT_MAX_CHANGED = 32768;
@@ -197,10 +197,9 @@ struct RefitBufs {
     int32_t* csi;
     uint32_t* cspos; // their final positions
 };
-hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, int64_t limit, int key_bits,
-                                const BuildBufs& b, hipStream_t s);
-hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t limit, int cur, int index_base,
-                               const BuildBufs& b, const RefitBufs& r, hipStream_t s);
+hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, const BuildBufs& b, hipStream_t s);
+hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t limit, int key_bits, int cur,
+                               int index_base, const BuildBufs& b, const RefitBufs& r, hipStream_t s);
 
 // Edge outputs (see edges.hip).
 struct EdgeOut {
