@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched: keep --leaf-size / --run-leaves / --run-chunk")
     ap.add_argument("--leaf-size", type=int, default=32)
-    ap.add_argument("--run-leaves", type=int, default=4)
+    ap.add_argument("--run-leaves", type=int, default=6)
     ap.add_argument("--run-chunk", type=int, default=4, help="single-pass runs per block ticket (1..4)")
     return ap.parse_args()
 

@@ -292,7 +292,7 @@ int oct_batch_autotune(oct_ctx* ctx, int batch_id, const int* leaf_sizes, int n_
  * and runs per ticket. */
 int oct_autotune(oct_ctx* ctx, int q, int t, int mode, const int* leaf_sizes, int n_leaf_sizes, const int* runs,
                  int n_runs, const int* chunks, int n_chunks, int reps, double* ms_out, int* best);
-/* Leaves per single-pass run without a plan (1..64, default 4). */
+/* Leaves per single-pass run without a plan (1..64, default 6). */
 int oct_set_run_leaves(oct_ctx* ctx, int run_leaves);
 /* Single-pass runs a block takes per ticket (1..4, default 4): larger shares
  * candidate points in a CU's caches, smaller starts every run at once (better

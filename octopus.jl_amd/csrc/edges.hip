@@ -732,7 +732,7 @@ constexpr uint64_t SP_FLAG_PRE = 2ull << 62;   // tile inclusive prefix publishe
 constexpr uint64_t SP_VALUE = (1ull << 62) - 1;
 constexpr uint32_t SP_SPIN_LIMIT = 1u << 22;   // bounded wait: a stuck look-back reports an error
 
-constexpr int SP_HIT_WORDS = 1024;                 // hit list of one wave: 4 KB
+constexpr int SP_HIT_WORDS = 1536;                 // hit list of one wave: 6 KB
 constexpr uint32_t SP_STAGE = SP_HIT_WORDS * 4 / 24;  // FILL stage laid over the hit list: 170 edges
 static_assert(SP_PLAN_BUDGET == SP_HIT_WORDS, "the plan packs one wave's compact hit list");
 

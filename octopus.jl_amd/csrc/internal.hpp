@@ -266,7 +266,7 @@ hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2*
                                uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint32_t* stats,
                                uint64_t cap_edges, EdgeOut out, int index_base, int leaves_per_run,
                                int run_chunk, const SetLookup& sets, const uint8_t* lheavy, hipStream_t s);
-constexpr int SP_DEFAULT_RUN = 4;    // leaves per run of the single pass without a plan
+constexpr int SP_DEFAULT_RUN = 6;    // leaves per run of the single pass without a plan
 constexpr int SP_DEFAULT_CHUNK = 4;  // runs per block ticket of the single pass
 constexpr int SP_MAX_CHUNK = 4;      // at most one run per wave of a block
 
@@ -310,7 +310,7 @@ int64_t plan_chunks(int64_t n_leaves);
 hipError_t launch_predict_plan(const float* feat, int64_t n, const float* P, float* pred, uint32_t budget,
                                int max_leaves, uint32_t* work, uint32_t* scan_tmp, int32_t* wave_first,
                                int32_t* n_runs, uint64_t* pred_total, hipStream_t s);
-constexpr uint32_t SP_PLAN_BUDGET = 1024;  // hit-list entries of one wave (edges.hip)
+constexpr uint32_t SP_PLAN_BUDGET = 1536;  // hit-list entries of one wave (edges.hip)
 constexpr int SP_PLAN_MAX_LEAVES = 64;     // leaves one wave may own (edges.hip)
 
 // Batched forests (see batch.hip).

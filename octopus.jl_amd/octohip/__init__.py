@@ -379,7 +379,7 @@ def count_neighbors_device(tns: TNS, view: L.OctView):
 
 
 def set_run_leaves(tns: TNS, run_leaves: int):
-    """Leaves per single-pass run without a plan (1..64, default 4)."""
+    """Leaves per single-pass run without a plan (1..64, default 6)."""
     tns._check(tns._lib.oct_set_run_leaves(tns._c(), int(run_leaves)))
 
 

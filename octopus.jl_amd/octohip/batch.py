@@ -46,7 +46,7 @@ class Batch:
         self.id = bid.value
         self.n_sets = int(self.radii.shape[0])
 
-    def config(self, leaf_size: int = 32, run_leaves: int = 4, use_graph: bool = True, run_chunk: int = 4):
+    def config(self, leaf_size: int = 32, run_leaves: int = 6, use_graph: bool = True, run_chunk: int = 4):
         self.tns._check(self.tns._lib.oct_batch_config(self.tns._c(), self.id, int(leaf_size), int(run_leaves),
                                                        int(run_chunk), int(bool(use_graph))))
 

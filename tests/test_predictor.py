@@ -287,7 +287,7 @@ def test_plan_records_more_leaves_than_fixed_runs(orc):
     want = sort_edges(o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm)
     for a, b in zip(got_plan, want):
         assert np.array_equal(a, b)
-    assert fixed["recounted_leaves"] > fixed["n_leaves"] // 10
+    assert fixed["recounted_leaves"] > fixed["n_leaves"] // 20  # fixed runs do overflow their lists here
     # the leaves left are those whose candidates do not fit one register tile
     assert plan["recounted_leaves"] < 0.8 * fixed["recounted_leaves"]
 
