@@ -138,6 +138,10 @@ struct PairState {
     DevBuf heavy;
     DevBuf eprm;  // the query tree's build parameters with the leaf count clamped to the rows
     bool heavy_on = true;  // count heavy leaves ahead: first call, and while many leaves are not recorded
+    // a call that returned on the single pass's early total: its recount
+    // statistics are final only when the kernel ends (read later)
+    bool stats_pending = false;
+    int64_t pending_heavy = 0, pending_leaves = 0;
     int32_t* heavy_list() { return heavy.as<int32_t>(); }
     uint8_t* heavy_flags(int64_t n_leaves) { return (uint8_t*)(heavy.as<int32_t>() + n_leaves); }
     oct_edge_stats st{};
@@ -226,6 +230,8 @@ struct oct_ctx {
     Predictor mlp;
     std::string err;
     void* pinned = nullptr;  // small host staging area for read-backs
+    uint32_t* sig = nullptr;      // host-coherent block the single pass hands its total through (SP_SIG_MARK)
+    uint32_t* sig_dev = nullptr;  // its device address
     // optional per-phase kernel timing with HIP events on the context stream
     uint32_t prof_mask = 0;  // phases timed with events (bit per OCT_PROF_*)
     bool capturing = false;  // a graph capture is open: no phase events
@@ -593,6 +599,20 @@ int oct_create(oct_ctx** out, int device, int ndims, int index_base) {
         delete c;
         return OCT_ERR_DEVICE;
     }
+    if (hipHostMalloc((void**)&c->sig, 4096, hipHostMallocCoherent) != hipSuccess) {
+        (void)hipHostFree(c->pinned);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return OCT_ERR_DEVICE;
+    }
+    memset(c->sig, 0, 4096);
+    if (hipHostGetDevicePointer((void**)&c->sig_dev, c->sig, 0) != hipSuccess || !c->sig_dev) {
+        (void)hipHostFree(c->sig);
+        (void)hipHostFree(c->pinned);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return OCT_ERR_DEVICE;
+    }
     *out = c;
     return OCT_OK;
 }
@@ -606,6 +626,7 @@ int oct_destroy(oct_ctx* c) {
     for (auto& b : c->batches) b.release_all();
     c->mlp.release_all();
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->sig) (void)hipHostFree(c->sig);
     for (auto& m : c->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
     for (auto e : c->spare) (void)hipEventDestroy(e);
     for (auto e : c->join)
@@ -752,6 +773,16 @@ int oct_get_tree(oct_ctx* c, int id, oct_tree* out) {
 }
 
 constexpr size_t HF_BYTES = 128;  // flag words of a pair's edge call; the build parameters follow
+constexpr int SIG_STATS = SP_SIG_MARK + 4;  // deferred recount statistics in the signal block
+
+// A single-pass call's recount statistics set the policy of the calls after
+// it: a new plan helps leaves that were recounted because a run's list filled
+// up, not heavy leaves (no plan records those); uneven leaves (many not
+// recorded in one sweep) keep the heavy-leaf count on, even ones switch it off.
+void recount_policy(PairState& p, uint32_t recounted, int64_t heavy_n, int64_t n_leaves) {
+    if (((int64_t)recounted - heavy_n) * 64 > n_leaves) p.replan = true;
+    p.heavy_on = (int64_t)recounted * 64 > n_leaves;
+}
 
 // Flat-COO edges of pair (q, t).  collect: exact path that also keeps the
 // predictor's features and the exact per-leaf hits (training rows).
@@ -769,6 +800,13 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     if (!qs.built || qs.dirty || !ts.built || ts.dirty) return fail(c, OCT_ERR_STATE, "run! before build_edges");
     HIP_TRY(c, hipSetDevice(c->device));
     memset(out, 0, sizeof(*out));
+    // the previous call's recount statistics (final once its kernels ended)
+    // come back with this call's read-back, queued ahead of everything else
+    const bool prev_pending = p.stats_pending;
+    if (prev_pending)
+        HIP_TRY(c, hipMemcpyAsync(c->sig + SIG_STATS, p.flags.as<uint32_t>() + 9, 8, hipMemcpyDeviceToHost,
+                                  c->stream));
+    p.stats_pending = false;
     memset(&p.st, 0, sizeof(p.st));
     p.n_edges = 0;
     if (qs.n == 0 || ts.n == 0) {
@@ -821,6 +859,10 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     uint64_t total = 0;
     int attempts = 0;
     EdgeOut eo{};
+    // the single pass hands its total over early with fixed runs (a plan's
+    // runs are checked by each run, so a plan waits for the kernels to end)
+    const bool early = !predict && c->sig != nullptr;
+    bool early_done = false;  // the last read-back came from the single pass, not after the kernels
     TreeView qv{}, tv{};
     for (int attempt = 0; attempt < 10; ++attempt) {
         ++attempts;
@@ -885,14 +927,18 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
                 HIP_TRY(c, launch_edges_heavy(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
                                               inv_r, self_pair, p.heavy_list(), flags + 16, p.qcount.as<uint32_t>(),
                                               p.lhits.as<uint32_t>(), SetLookup{}, c->stream));
+            if (early) {  // cleared before the launch; the single pass's last run sets it
+                __atomic_store_n((uint64_t*)(c->sig + SP_SIG_MARK), 0ull, __ATOMIC_RELEASE);
+            }
             HIP_TRY(c, launch_edges_single(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r,
                                            self_pair, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(),
                                            predict ? p.wave_first.as<int32_t>() : nullptr,
                                            predict ? (const int32_t*)(flags + 12) : nullptr, p.lhits.as<uint32_t>(),
                                            p.tiles.as<uint64_t>(), flags + 8, flags + 4, flags + 9,
                                            (uint64_t)p.cap_edges, eo, c->index_base, c->run_leaves, c->run_chunk,
-                                           SetLookup{},
-                                           heavy ? p.heavy_flags(nql) : nullptr, c->stream));
+                                           SetLookup{}, heavy ? p.heavy_flags(nql) : nullptr,
+                                           early ? c->sig_dev : nullptr, flags,
+                                           (int)((HF_BYTES + 2 * sizeof(BuildParams)) / 4), c->stream));
             if (heavy)
                 HIP_TRY(c, launch_edges_heavy_fill(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
                                                    inv_r, self_pair, p.heavy_list(), flags + 16,
@@ -909,16 +955,39 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
             HIP_TRY(c, exclusive_scan(p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), qs.n,
                                       p.scan_tmp.as<uint32_t>(), (uint64_t*)(flags + 2), c->stream));
         }
-        // one read-back: flags and both sets' build parameters
-        HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, HF_BYTES + 2 * sizeof(BuildParams), hipMemcpyDeviceToHost,
-                                  c->stream));
-        {
+        // one read-back: flags and both sets' build parameters, either handed
+        // over by the single pass as soon as its total is known (the call then
+        // returns while the rest of the pass runs) or copied after the kernels
+        const uint8_t* rb = (const uint8_t*)c->pinned;
+        bool got = false;
+        if (single && early) {
+            HostTimer ht("edges: wait for the single pass's total");
+            const uint64_t* mark = (const uint64_t*)(c->sig + SP_SIG_MARK);
+            for (;;) {
+                if (__atomic_load_n(mark, __ATOMIC_ACQUIRE) != 0) {
+                    got = true;
+                    break;
+                }
+                const hipError_t qe = hipStreamQuery(c->stream);
+                if (qe == hipSuccess) {  // the kernels ended: the marker is there or was never written
+                    got = __atomic_load_n(mark, __ATOMIC_ACQUIRE) != 0;
+                    break;
+                }
+                if (qe != hipErrorNotReady) HIP_TRY(c, qe);
+                __builtin_ia32_pause();
+            }
+            if (got) rb = (const uint8_t*)c->sig;
+        }
+        if (!got) {
+            HIP_TRY(c, hipMemcpyAsync(c->pinned, flags, HF_BYTES + 2 * sizeof(BuildParams), hipMemcpyDeviceToHost,
+                                      c->stream));
             HostTimer ht(single ? "edges: wait for single pass" : "edges: wait for count total");
             HIP_TRY(c, wait_stream(c->stream));
         }
-        memcpy(hf, c->pinned, 4 * HF_WORDS);
-        take_hp(qs, (char*)c->pinned + HF_BYTES);
-        if (&ts != &qs) take_hp(ts, (char*)c->pinned + HF_BYTES + sizeof(BuildParams));
+        early_done = got;
+        memcpy(hf, rb, 4 * HF_WORDS);
+        take_hp(qs, rb + HF_BYTES);
+        if (&ts != &qs) take_hp(ts, rb + HF_BYTES + sizeof(BuildParams));
         bool rq = false, rt = false;
         HIP_TRY_RC(settle_refit(c, qs, &rq));
         if (&ts != &qs) HIP_TRY_RC(settle_refit(c, ts, &rt));
@@ -997,21 +1066,28 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     const int64_t n_leaves = qs.hp.n_leaves;  // read back at the last wait
     p.st.n_leaves = n_leaves;
     p.st.n_edges = E;
-    p.st.recounted_leaves = single ? hf[9] : 0;
-    p.st.recorded_leaves = single ? hf[10] : 0;
+    // an early total leaves the recount statistics for later (edge_stats or
+    // the next call), when the kernels have ended
+    p.st.recounted_leaves = single && !early_done ? hf[9] : 0;
+    p.st.recorded_leaves = single && !early_done ? hf[10] : 0;
     p.st.heavy_leaves = heavy && single ? (int32_t)hf[16] : 0;
     p.st.predicted_total = predict && single ? (double)pt : 0.0;
     p.st.single_pass = single ? 1 : 0;
     p.st.used_predictor = predict && single ? 1 : 0;
     p.st.attempts = attempts;
-    if (mode == OCT_EDGES_LEARNED) {
-        // a new plan helps leaves that were recounted because a run's list
-        // filled up, not heavy leaves (no plan records those)
-        const int64_t heavy_n = heavy && single ? (int64_t)hf[16] : 0;
-        p.replan = !single || attempts > 1 || ((int64_t)hf[9] - heavy_n) * 64 > n_leaves;
-        // uneven leaves (many not recorded in one sweep): count the heavy ones
-        // ahead next time; even ones skip the estimate and the extra launch
-        if (single) p.heavy_on = (int64_t)hf[9] * 64 > n_leaves;
+    if (mode == OCT_EDGES_LEARNED) p.replan = !single || attempts > 1;
+    // the previous call's statistics came back with this read-back (older,
+    // so applied first), then this call's when they are final
+    if (prev_pending) recount_policy(p, c->sig[SIG_STATS], p.pending_heavy, p.pending_leaves);
+    if (mode == OCT_EDGES_LEARNED && single) {
+        const int64_t heavy_n = heavy ? (int64_t)hf[16] : 0;
+        if (!early_done) {
+            recount_policy(p, hf[9], heavy_n, n_leaves);
+        } else {
+            p.stats_pending = true;
+            p.pending_heavy = heavy_n;
+            p.pending_leaves = n_leaves;
+        }
     }
     out->senders = eo.senders;
     out->receivers = eo.receivers;
@@ -1285,6 +1361,16 @@ int oct_edge_stats_get(oct_ctx* c, int q, int t, oct_edge_stats* out) {
     if (!c || !out) return OCT_ERR_ARGUMENT;
     PairState* p = find_pair(c, q, t);
     if (!p) return fail(c, OCT_ERR_STATE, "edge stats: pair not registered");
+    if (p->stats_pending) {  // the call returned on the early total: its statistics are final at the kernels' end
+        HIP_TRY(c, hipSetDevice(c->device));
+        HIP_TRY(c, wait_stream(c->stream));
+        uint32_t st2[2] = {0, 0};
+        HIP_TRY(c, hipMemcpy(st2, p->flags.as<uint32_t>() + 9, 8, hipMemcpyDeviceToHost));
+        p->st.recounted_leaves = st2[0];
+        p->st.recorded_leaves = st2[1];
+        recount_policy(*p, st2[0], p->pending_heavy, p->pending_leaves);
+        p->stats_pending = false;
+    }
     *out = p->st;
     return OCT_OK;
 }
@@ -1407,8 +1493,7 @@ int batch_enqueue(oct_ctx* c, BatchState& B) {
                                    p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), nullptr, nullptr,
                                    p.lhits.as<uint32_t>(), p.tiles.as<uint64_t>(), flags + 8, flags + 4, flags + 9,
                                    (uint64_t)B.cap_edges, eo, c->index_base, B.run_leaves, B.run_chunk, sets,
-                                   p.heavy_flags(N),
-                                   st));
+                                   p.heavy_flags(N), nullptr, nullptr, 0, st));
     HIP_TRY(c, launch_edges_heavy_fill(v, v, p.lists.as<int2>(), p.lcount.as<int32_t>(), B.cap, 0.f, 0.f, 1,
                                        p.heavy_list(), flags + 11, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(),
                                        flags + 4, eo, c->index_base, sets, st));

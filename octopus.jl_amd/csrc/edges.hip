@@ -733,6 +733,7 @@ constexpr uint64_t SP_VALUE = (1ull << 62) - 1;
 constexpr uint32_t SP_SPIN_LIMIT = 1u << 22;   // bounded wait: a stuck look-back reports an error
 
 constexpr int SP_HIT_WORDS = 1536;                 // hit list of one wave: 6 KB
+static_assert(SP_SIG_MARK % 2 == 0, "the marker is a 64-bit word");
 constexpr uint32_t SP_STAGE = SP_HIT_WORDS * 4 / 24;  // FILL stage laid over the hit list: 170 edges
 static_assert(SP_PLAN_BUDGET == SP_HIT_WORDS, "the plan packs one wave's compact hit list");
 
@@ -865,7 +866,8 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
                const int32_t* __restrict__ plan_waves, uint32_t* __restrict__ lhits,
                uint64_t* __restrict__ tile_state, uint32_t* __restrict__ ticket, uint32_t* __restrict__ flags,
                uint32_t* __restrict__ stats, uint64_t cap_edges, EdgeOut out, int index_base0,
-               int leaves_per_run, int run_chunk, SetLookup sets, const uint8_t* __restrict__ lheavy) {
+               int leaves_per_run, int run_chunk, SetLookup sets, const uint8_t* __restrict__ lheavy,
+               uint32_t* __restrict__ sig, const uint32_t* __restrict__ sig_src, int sig_words) {
     constexpr uint32_t HITS = WIDE ? SP_HIT_WORDS / 2 : SP_HIT_WORDS;
     __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
     __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
@@ -968,6 +970,20 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
             if (incl > cap_edges) atomicOr(&flags[0], 1u);
             if (w == nruns - 1)
                 __hip_atomic_store((uint64_t*)(flags + 2), incl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (sig && w == nruns - 1) {  // wave-uniform
+            // The total is known: hand it to the host now, with the call's flag
+            // words (sig_src), so it can return while phase B still runs.  By
+            // now every run has published its aggregate (this look-back read
+            // them back to an inclusive prefix, which needed the same of the
+            // runs below it), so no later look-back can wait or give up.
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+            for (int i = lane; i < sig_words; i += OCT_WAVE)
+                sig[i] = __hip_atomic_load(&sig_src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            if (lane == 0)
+                __hip_atomic_store(reinterpret_cast<unsigned long long*>(sig + SP_SIG_MARK), 1ull + incl,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         if (!ok || incl > cap_edges) continue;  // wave-uniform
         __builtin_amdgcn_wave_barrier();
@@ -1093,7 +1109,9 @@ hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2*
                                const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
                                uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint32_t* stats,
                                uint64_t cap_edges, EdgeOut out, int index_base, int leaves_per_run,
-                               int run_chunk, const SetLookup& sets, const uint8_t* lheavy, hipStream_t s) {
+                               int run_chunk, const SetLookup& sets, const uint8_t* lheavy, uint32_t* sig,
+                               const uint32_t* sig_src, int sig_words, hipStream_t s) {
+    if (sig && (wave_first || sig_words < 0 || sig_words > SP_SIG_MARK)) return hipErrorInvalidValue;
     if (leaves_per_run < 1 || leaves_per_run > SP_MAX_LEAVES) return hipErrorInvalidValue;
     if (run_chunk < 1 || run_chunk > EW_WAVES) return hipErrorInvalidValue;
     if (lheavy && !lhits) return hipErrorInvalidValue;  // the heavy totals live in lhits
@@ -1105,12 +1123,12 @@ hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2*
         hipLaunchKernelGGL(k_edges_single<false>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first,
                            plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base,
-                           leaves_per_run, run_chunk, sets, lheavy);
+                           leaves_per_run, run_chunk, sets, lheavy, sig, sig_src, sig_words);
     else
         hipLaunchKernelGGL(k_edges_single<true>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first,
                            plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base,
-                           leaves_per_run, run_chunk, sets, lheavy);
+                           leaves_per_run, run_chunk, sets, lheavy, sig, sig_src, sig_words);
     return hipGetLastError();
 }
 

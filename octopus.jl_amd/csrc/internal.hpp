@@ -260,12 +260,17 @@ int64_t single_tiles(int64_t n_leaves);
 // plan_waves: optional plan of leaf runs per wave (null: fixed runs).
 // lheavy (optional, needs lhits): leaves launch_edges_heavy counted already.
 // run_chunk: runs a block takes per ticket (1..SP_MAX_CHUNK).
+// sig (optional, fixed runs only): host-visible memory; the last run copies
+// sig_words words of sig_src there and then writes 1 + the edge total as the
+// 64-bit word at sig + SP_SIG_MARK, while the other runs' phase B goes on.
 hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
                                int cap, float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
                                const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
                                uint64_t* tile_state, uint32_t* ticket, uint32_t* flags, uint32_t* stats,
                                uint64_t cap_edges, EdgeOut out, int index_base, int leaves_per_run,
-                               int run_chunk, const SetLookup& sets, const uint8_t* lheavy, hipStream_t s);
+                               int run_chunk, const SetLookup& sets, const uint8_t* lheavy, uint32_t* sig,
+                               const uint32_t* sig_src, int sig_words, hipStream_t s);
+constexpr int SP_SIG_MARK = 128;      // 32-bit word of the early-total marker in the signal block
 constexpr int SP_DEFAULT_RUN = 6;    // leaves per run of the single pass without a plan
 constexpr int SP_DEFAULT_CHUNK = 4;  // runs per block ticket of the single pass
 constexpr int SP_MAX_CHUNK = 4;      // at most one run per wave of a block

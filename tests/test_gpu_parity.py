@@ -464,3 +464,42 @@ def test_caller_streams(orc, private):
     octohip.run(tns)
     for mode in ("exact", "learned"):
         _edges_equal(_gpu_edges(octohip.build_edges(tns, sid, sid, mode=mode)), want)
+
+
+def test_single_pass_returns_on_its_total(orc):
+    """With fixed runs the learned call returns as soon as the single pass
+    knows its total, while the pass's emit still runs: work the caller then
+    queues on its stream (here: sums of the outputs, then the next step) must
+    see complete edges, step after step, and the statistics must be final."""
+    import octohip
+    from octohip import predictor
+
+    torch = _torch()
+    rng = np.random.default_rng(17)
+    pts = datasets.uniform_cube(40000, 77)
+    r = 0.03
+    tns, sid, c = _build(pts, r)
+    octohip.set_refit_mode(tns, True)
+    octohip.build_edges(tns, sid, sid, mode="learned")  # sizes the buffers
+    sums = []
+    steps = []
+    cur = pts
+    for k in range(4):
+        cur = (cur.astype(np.float64) + rng.normal(0, 2e-4, cur.shape)).astype(np.float32)
+        octohip.update_point_set(tns, sid, _gpu(cur))
+        octohip.run(tns)
+        e = octohip.build_edges(tns, sid, sid, mode="learned")
+        # queued on the caller's stream behind the emit: no explicit wait
+        sums.append(torch.stack([e.senders.double().sum(), e.receivers.double().sum(),
+                                 e.rel_dist_norm.double().sum(), e.rel_displacement.double().abs().sum()]))
+        steps.append(cur)
+    st = predictor.edge_stats(tns, sid, sid)
+    assert st["single_pass"] and st["recorded_leaves"] + st["recounted_leaves"] == st["n_leaves"]
+    for k, cur in enumerate(steps):
+        o = orc.build_edges(orc.run(cur, r), cur)
+        want = np.array([o.senders.astype(np.float64).sum(), o.receivers.astype(np.float64).sum(),
+                         o.rel_dist_norm.astype(np.float64).sum(), np.abs(o.rel_displacement.astype(np.float64)).sum()])
+        got = sums[k].cpu().numpy()
+        print(f"step {k}: edges {o.n_edges}", flush=True)
+        np.testing.assert_array_equal(got[:2], want[:2])
+        np.testing.assert_allclose(got[2:], want[2:], rtol=1e-12)
