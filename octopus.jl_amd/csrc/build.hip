@@ -444,7 +444,10 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
             int32_t* __restrict__ nchild, float* __restrict__ bmin, float* __restrict__ bmax,
             int32_t* __restrict__ parent, uint8_t* __restrict__ nlevel, int32_t* __restrict__ leaf_ids,
             NodeRec* __restrict__ leaves) {
-    __shared__ uint32_t wc[TOPO_THREADS / OCT_WAVE][OCT_NLEVELS + 1];
+    constexpr int NW = TOPO_THREADS / OCT_WAVE;
+    __shared__ uint32_t wc[NW][OCT_NLEVELS + 1];
+    __shared__ uint64_t wmask[NW][OCT_NLEVELS + 1];  // per wave and level: lanes starting a node
+    __shared__ uint32_t woff[NW][OCT_NLEVELS + 1];   // ids before the wave's first such lane
     __shared__ int8_t mm[TOPO_THREADS + TOPO_MAX_T + 1];  // m of positions base .. base + 256 + T
     const int64_t base = (int64_t)blockIdx.x * TOPO_THREADS;
     for (int i = threadIdx.x; i < TOPO_THREADS + T + 1; i += TOPO_THREADS) {
@@ -462,12 +465,22 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
     }
     const int w = threadIdx.x / OCT_WAVE;
     const int lane = lane_id();
-    uint32_t rank_in_wave[OCT_NLEVELS + 1];
     for (int L = 0; L <= OCT_NLEVELS; ++L) {
         const bool ind = L < OCT_NLEVELS ? (emit && mk < L && L <= lam) : emit;
         const uint64_t b = ballot(ind);
-        rank_in_wave[L] = (uint32_t)mbcnt(b);
-        if (lane == 0) wc[w][L] = (uint32_t)__popcll(b);
+        if (lane == 0) {
+            wmask[w][L] = b;
+            wc[w][L] = (uint32_t)__popcll(b);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x <= OCT_NLEVELS) {  // the block's offset per level, then each wave's
+        const int L = threadIdx.x;
+        uint32_t s = hoff[(int64_t)L * nblk + blockIdx.x];
+        for (int q = 0; q < NW; ++q) {
+            woff[q][L] = s;
+            s += wc[q][L];
+        }
     }
     __syncthreads();
     const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
@@ -484,15 +497,9 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
     BuildParams* sp = forest.sprm + set;
     const int L0 = key_shared(sk[s_lo], sk[s_hi - 1], forest.batched);
     if (k == s_lo) sp->chain_level = L0;
-    auto wave_off = [&](int L) {
-        uint32_t s = 0;
-        for (int q = 0; q < w; ++q) s += wc[q][L];
-        return s;
-    };
     // count of level-L node starts before position k, plus the level's offset
-    auto before = [&](int L) -> int32_t {
-        return (int32_t)(hoff[(int64_t)L * nblk + blockIdx.x] + wave_off(L) + rank_in_wave[L]);
-    };
+    // (LDS reads: a per-lane level cannot index registers without a select chain)
+    auto before = [&](int L) -> int32_t { return (int32_t)(woff[w][L] + (uint32_t)mbcnt(wmask[w][L])); };
     const int Lfirst = mk + 1;  // 0 where a set starts
     for (int L = Lfirst; L <= lam; ++L) {
         const int32_t id = before(L);

@@ -480,6 +480,7 @@ def test_single_pass_returns_on_its_total(orc):
     r = 0.03
     tns, sid, c = _build(pts, r)
     octohip.set_refit_mode(tns, True)
+    octohip.run(tns)
     octohip.build_edges(tns, sid, sid, mode="learned")  # sizes the buffers
     sums = []
     steps = []
