@@ -450,11 +450,23 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
     __shared__ uint32_t woff[NW][OCT_NLEVELS + 1];   // ids before the wave's first such lane
     __shared__ int8_t mm[TOPO_THREADS + TOPO_MAX_T + 1];  // m of positions base .. base + 256 + T
     const int64_t base = (int64_t)blockIdx.x * TOPO_THREADS;
+    // loads that do not depend on the ranks are issued first, so their
+    // latencies overlap the ballots and barriers below
+    const uint32_t hrow = threadIdx.x <= OCT_NLEVELS ? hoff[(int64_t)threadIdx.x * nblk + blockIdx.x] : 0u;
+    const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
+    const int64_t k = base + threadIdx.x;
+    const uint64_t key = k < n ? sk[k] : 0ull;
+    int64_t s_lo1 = 0, s_hi1 = 0;  // one set: its range and chain depth
+    int L01 = 0;
+    if (!forest.batched && n > 0) {
+        s_lo1 = forest.set_off[0];
+        s_hi1 = forest.set_off[1];
+        L01 = key_shared(sk[s_lo1], sk[s_hi1 - 1], 0);
+    }
     for (int i = threadIdx.x; i < TOPO_THREADS + T + 1; i += TOPO_THREADS) {
         const int64_t j = base + i;
         mm[i] = j < n ? (int8_t)((int)(topo[j] & 31) - 1) : (int8_t)-2;
     }
-    const int64_t k = base + threadIdx.x;
     int mk = -1, lam = -1;
     bool emit = false;
     if (k < n) {
@@ -476,14 +488,13 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
     __syncthreads();
     if (threadIdx.x <= OCT_NLEVELS) {  // the block's offset per level, then each wave's
         const int L = threadIdx.x;
-        uint32_t s = hoff[(int64_t)L * nblk + blockIdx.x];
+        uint32_t s = hrow;
         for (int q = 0; q < NW; ++q) {
             woff[q][L] = s;
             s += wc[q][L];
         }
     }
     __syncthreads();
-    const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
     if (k == 0) {
         prm->n_nodes = M;
         prm->n_leaves = (int32_t)(*total) - M;  // the scan's last row counts the leaves
@@ -491,11 +502,15 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
     if (!emit) return;
     // the set of this position, its point range and chain depth (levels
     // 0..L0 hold all of the set's points)
-    const uint64_t key = sk[k];
     const int set = key_set(key, forest.batched);
-    const int64_t s_lo = forest.set_off[set], s_hi = forest.set_off[set + 1];
+    int64_t s_lo = s_lo1, s_hi = s_hi1;
+    int L0 = L01;
+    if (forest.batched) {
+        s_lo = forest.set_off[set];
+        s_hi = forest.set_off[set + 1];
+        L0 = key_shared(sk[s_lo], sk[s_hi - 1], 1);
+    }
     BuildParams* sp = forest.sprm + set;
-    const int L0 = key_shared(sk[s_lo], sk[s_hi - 1], forest.batched);
     if (k == s_lo) sp->chain_level = L0;
     // count of level-L node starts before position k, plus the level's offset
     // (LDS reads: a per-lane level cannot index registers without a select chain)

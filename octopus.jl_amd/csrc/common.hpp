@@ -135,21 +135,24 @@ __device__ inline T wave_max_u(T v) {
     }
     return v;
 }
-__device__ inline int wave_sum(int v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, OCT_WAVE);
+// inclusive prefix sum across the wave in DPP moves (no LDS round trips):
+// row_shr 1, 2, 4, 8 scan each row of 16 lanes (lanes shifted in from
+// outside the row read 0), then row_bcast:15 adds row 0's total to row 1
+// and row 2's to row 3, and row_bcast:31 adds the total of rows 0-1 to
+// rows 2-3.  Call with every lane of the wave active.
+__device__ inline int wave_incl_scan(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
     return v;
 }
+// sum across the wave (the same in every lane)
+__device__ inline int wave_sum(int v) { return __builtin_amdgcn_readlane(wave_incl_scan(v), OCT_WAVE - 1); }
 __device__ inline uint64_t wave_sum_u64(uint64_t v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, OCT_WAVE);
-    return v;
-}
-// inclusive prefix sum across the wave
-__device__ inline int wave_incl_scan(int v) {
-    int lane = lane_id();
-    for (int o = 1; o < OCT_WAVE; o <<= 1) {
-        int w = __shfl_up(v, o, OCT_WAVE);
-        if (lane >= o) v += w;
-    }
     return v;
 }
 
