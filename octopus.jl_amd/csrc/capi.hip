@@ -507,11 +507,9 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
         const RefitBufs r{s.rk.as<uint64_t>(), s.ri.as<int32_t>(), s.rsk.as<uint64_t>(), s.rsi.as<int32_t>(),
                           s.rpos.as<uint32_t>()};
         // the key width must stay s.key_bits (the levels the topology refits depend on it)
+        // the graph starts after the bounds, which run eagerly on the caller's
+        // coordinates and copy them into cin (the graph reads that copy)
         auto enqueue = [&](const float* coords) -> int {
-            {
-                Phase ph(c, OCT_PROF_BOUNDS);
-                HIP_TRY(c, launch_bounds_stage(coords, n, c->radius, b, c->stream));
-            }
             Phase ph(c, OCT_PROF_BUILD);
             HIP_TRY(c, launch_refit_prepare(coords, n, s.cur, b, c->stream));
             HIP_TRY(c, launch_refit_finish(coords, n, limit, s.key_bits, s.cur, c->index_base, b, r, c->stream));
@@ -520,7 +518,10 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
         };
         {
             HIP_TRY(c, s.cin.ensure(12 * (size_t)n));
-            HIP_TRY(c, hipMemcpyAsync(s.cin.p, s.coords, 12 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+            {
+                Phase ph(c, OCT_PROF_BOUNDS);
+                HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, s.cin.as<float>(), b, c->stream));
+            }
             // the graph's key: everything its launches bake in
             uint64_t key = 1469598103934665603ull;
             auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
@@ -544,7 +545,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
                 HIP_TRY_RC(capture_graph(c, [&]() { return enqueue(s.cin.as<float>()); }, &g.exec));
                 g.key = key;
             }
-            Phase ph(c, OCT_PROF_BUILD);  // the whole step: bounds, refit and topology
+            Phase ph(c, OCT_PROF_BUILD);  // refit and topology
             HIP_TRY(c, hipGraphLaunch(g.exec, c->stream));
         }
         cur = s.cur ^ 1;
@@ -552,7 +553,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
     } else {
         {
             Phase ph(c, OCT_PROF_BOUNDS);
-            HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, b, c->stream));
+            HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, nullptr, b, c->stream));
         }
         Phase ph(c, OCT_PROF_BUILD);
         HIP_TRY_RC(read_params(c, s));  // the key width sizes the radix passes
@@ -878,13 +879,16 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
             HIP_TRY(c, p.pscan.ensure(4 * (size_t)scan_tmp_words(plan_chunks(nql))));
             HIP_TRY(c, p.wave_first.ensure(4 * (size_t)(nql + 2)));
         }
+        // the same launch clears the flag words (the heavy-leaf count at [16]
+        // too when the lists are rebuilt) and the single pass's run states
         HIP_TRY(c, launch_clamp_leaves(qs.prm.as<BuildParams>(), ts.prm.as<BuildParams>(), p.eprm.as<BuildParams>(),
-                                       nql, flags + 18, raw_prm, c->stream));
+                                       nql, flags + 18, raw_prm, flags, lists_ok ? 16 : 17,
+                                       single ? (uint32_t*)p.tiles.p : nullptr, single ? 2 * (ntiles + 1) : 0,
+                                       c->stream));
         qv = view_of(qs, nql, p.eprm.as<BuildParams>());
         tv = view_of(ts, ts.hp_fresh ? ts.hp.n_leaves : ts.n);
         HIP_TRY(c, p.lists.ensure(sizeof(int2) * (size_t)nql * (size_t)p.cap));
         if (features) HIP_TRY(c, p.ldir.ensure((size_t)nql * (size_t)p.cap));
-        HIP_TRY(c, launch_fill_u32(flags, lists_ok ? 16 : 17, 0u, c->stream));
         if (!lists_ok) {
             Phase ph(c, OCT_PROF_LEAF_LISTS);
             HIP_TRY(c, launch_leaf_lists(qv, tv, r_sq, p.cap, safe, p.lists.as<int2>(),
@@ -921,7 +925,6 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
             HIP_TRY(c, p.rdisp.ensure(12 * (size_t)p.cap_edges));
             HIP_TRY(c, p.rdist.ensure(4 * (size_t)p.cap_edges));
             eo = EdgeOut{p.senders.as<int32_t>(), p.receivers.as<int32_t>(), p.rdisp.as<float>(), p.rdist.as<float>()};
-            HIP_TRY(c, launch_fill_u32((uint32_t*)p.tiles.p, 2 * (ntiles + 1), 0u, c->stream));
             Phase ph(c, OCT_PROF_SINGLE);
             if (heavy)
                 HIP_TRY(c, launch_edges_heavy(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,

@@ -1147,9 +1147,15 @@ hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* 
 
 // dst = *src with n_leaves clamped to rows; *over = 1 when it was (else 0);
 // raw[0] = *src and raw[1] = *tsrc, for the host to read back with the flags.
-__global__ void k_clamp_leaves(const BuildParams* __restrict__ src, const BuildParams* __restrict__ tsrc,
-                               BuildParams* __restrict__ dst, int64_t rows, uint32_t* __restrict__ over,
-                               BuildParams* __restrict__ raw) {
+__global__ void __launch_bounds__(256)
+k_clamp_leaves(const BuildParams* __restrict__ src, const BuildParams* __restrict__ tsrc,
+               BuildParams* __restrict__ dst, int64_t rows, uint32_t* __restrict__ over,
+               BuildParams* __restrict__ raw, uint32_t* __restrict__ z0, int64_t n0, uint32_t* __restrict__ z1,
+               int64_t n1) {
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = i0; i < n0; i += step) z0[i] = 0u;
+    for (int64_t i = i0; i < n1; i += step) z1[i] = 0u;
+    if (i0 != 0) return;
     BuildParams p = *src;
     raw[0] = p;
     raw[1] = *tsrc;
@@ -1160,8 +1166,11 @@ __global__ void k_clamp_leaves(const BuildParams* __restrict__ src, const BuildP
 }
 
 hipError_t launch_clamp_leaves(const BuildParams* src, const BuildParams* tsrc, BuildParams* dst, int64_t rows,
-                               uint32_t* over, BuildParams* raw, hipStream_t s) {
-    hipLaunchKernelGGL(k_clamp_leaves, dim3(1), dim3(1), 0, s, src, tsrc, dst, rows, over, raw);
+                               uint32_t* over, BuildParams* raw, uint32_t* z0, int64_t n0, uint32_t* z1, int64_t n1,
+                               hipStream_t s) {
+    const int64_t n = std::max<int64_t>(std::max<int64_t>(n0, n1), 1);
+    const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 512);
+    hipLaunchKernelGGL(k_clamp_leaves, dim3(g), dim3(256), 0, s, src, tsrc, dst, rows, over, raw, z0, n0, z1, n1);
     return hipGetLastError();
 }
 

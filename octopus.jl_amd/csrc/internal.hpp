@@ -170,7 +170,9 @@ int64_t sort_tiles(int64_t n);
 int64_t topo_blocks(int64_t n);
 int64_t hist_words(int64_t n);
 int64_t max_nodes(int64_t n, int T);
-hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, const BuildBufs& b,
+// Bounds, origin and key width of one set (per-block partials, then a fold);
+// with `copy`, the coordinates are copied there on the way.
+hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, float* copy, const BuildBufs& b,
                                hipStream_t s);
 hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int index_base, const BuildBufs& b,
                              int* cur_out, hipStream_t s);
@@ -233,8 +235,11 @@ hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, i
 // the edge kernels of a tree whose leaf count is not on the host yet read the
 // clamped copy, so they never index past per-leaf arrays of `rows` entries.
 // raw[0..1] receive *src and *tsrc unchanged (read back with the flags).
+// The same launch zeroes z0[0 .. n0) and z1[0 .. n1) (the call's flag words
+// and the single pass's run states), which saves two fill launches.
 hipError_t launch_clamp_leaves(const BuildParams* src, const BuildParams* tsrc, BuildParams* dst, int64_t rows,
-                               uint32_t* over, BuildParams* raw, hipStream_t s);
+                               uint32_t* over, BuildParams* raw, uint32_t* z0, int64_t n0, uint32_t* z1, int64_t n1,
+                               hipStream_t s);
 // Edges of the listed heavy leaves after the single pass (which leaves each
 // one's output position in qoff at its first query); flags = the single
 // pass's (nothing runs when it raised flags[0] or flags[1]).

@@ -504,3 +504,37 @@ def test_single_pass_returns_on_its_total(orc):
         print(f"step {k}: edges {o.n_edges}", flush=True)
         np.testing.assert_array_equal(got[:2], want[:2])
         np.testing.assert_allclose(got[2:], want[2:], rtol=1e-12)
+
+
+def test_unaligned_coordinates(orc):
+    """Coordinates that start 4 bytes past a 16-byte boundary (a view into a
+    larger buffer) take the bounds kernel's 4-byte load path; tree and edges
+    must be the oracle's, in refit steps too (the copy the step graph reads)."""
+    import octohip
+
+    torch = _torch()
+    pts = datasets.uniform_cube(20011, 83)  # not a multiple of four: the partial group
+    r = 0.03
+    buf = torch.zeros(3 * pts.shape[0] + 1, dtype=torch.float32, device="cuda")
+    c = buf[1:].view(pts.shape[0], 3)
+    c.copy_(torch.from_numpy(pts).cuda())
+    assert c.data_ptr() % 16 == 4
+    tns = octohip.TNS()
+    octohip.set_search_radius(tns, r)
+    octohip.set_refit_mode(tns, True)
+    sid = octohip.add_point_set(tns, c)
+    octohip.set_active_search(tns, sid, sid)
+    rng = np.random.default_rng(3)
+    cur = pts
+    for step in range(3):
+        octohip.run(tns)
+        t = octohip.get_tree(tns, sid)
+        o = orc.run(cur, r)
+        assert np.array_equal(_np(t.node_bounds_min), o.bmin) and np.array_equal(_np(t.node_bounds_max), o.bmax)
+        assert np.array_equal(np.array(t.origin, np.float32), o.origin)
+        e = octohip.build_edges(tns, sid, sid, mode="learned")
+        ob = orc.build_edges(o, cur)
+        _edges_equal(_gpu_edges(e), (ob.senders, ob.receivers, ob.rel_displacement, ob.rel_dist_norm))
+        cur = (cur.astype(np.float64) + rng.normal(0, 1e-4, cur.shape)).astype(np.float32)
+        c.copy_(torch.from_numpy(cur).cuda())
+        octohip.update_point_set(tns, sid, c)
