@@ -201,6 +201,124 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
     }
 }
 
+// Cooperative form of the fast walk: eight lanes per query leaf, lane c
+// testing child c of the popped node.  The one-thread walk loaded a node's
+// children with sixteen 16-byte loads per lane, every lane at its own node,
+// so each wave-wide load touched 64 cache lines and the L1 request rate
+// bounded the kernel; here a load instruction covers eight nodes' contiguous
+// children.  The visit order is the one-thread walk's: the passing leaves of
+// a node are appended in child order (ballot ranks) and its passing internal
+// children pushed in child order, so the last one is popped first.  The
+// stack (LL_FAST_STACK entries per leaf, in LDS) overflowing flags the leaf
+// for the safe walk, as before.
+constexpr int LL_COOP = 8;                              // lanes per query leaf
+constexpr int LL_COOP_GROUPS = LL_THREADS / LL_COOP;    // query leaves per block
+
+template <bool DIRS, bool HEAVY>
+__global__ void __launch_bounds__(LL_THREADS)
+k_leaf_lists_coop(const NodeRec* __restrict__ qleaves, const BuildParams* __restrict__ qprm,
+                  const NodeRec* __restrict__ tnodes, const BuildParams* __restrict__ tprm, float r_sq0, int cap,
+                  int2* __restrict__ lists, uint8_t* __restrict__ ldir, int32_t* __restrict__ lcount,
+                  uint32_t* __restrict__ flags, SetLookup sets, uint8_t* __restrict__ lheavy,
+                  int32_t* __restrict__ heavy_list, uint32_t* __restrict__ n_heavy) {
+    __shared__ uint32_t stk[LL_COOP_GROUPS][LL_FAST_STACK];
+    const int32_t nleaves = qprm->n_leaves;
+    const int32_t root0 = tprm->branch_root;
+    const XcdShare sh = xcd_share(nleaves);
+    const int lane = lane_id();
+    const int c = lane & (LL_COOP - 1);        // this lane's child slot
+    const int gsh = lane & ~(LL_COOP - 1);     // the group's first lane in the wave
+    const int g = threadIdx.x / LL_COOP;       // the group in the block
+    const uint32_t below = (1u << c) - 1u;
+    for (int32_t r = (int32_t)(sh.lo + (int64_t)sh.k * LL_COOP_GROUPS) + g; r < (int32_t)sh.hi;
+         r += (int32_t)(sh.nb * LL_COOP_GROUPS)) {  // uniform in the group
+        const NodeRec qa = qleaves[r];
+        int32_t root = root0;
+        float r_sq = r_sq0;
+        if (sets.batched) {  // a forest: the leaf's own set's tree and radius
+            const BuildParams& sp = sets.sprm[key_set(sets.qsk[rec_a(qa)], 1)];
+            root = sp.branch_root;
+            r_sq = sp.radius * sp.radius;
+        }
+        int2* out = lists + (int64_t)r * cap;
+        uint8_t* dout = DIRS ? ldir + (int64_t)r * cap : nullptr;
+        int cnt = 0;
+        float est = 0.f;  // HEAVY: estimated filtered candidate points
+        const float rad = HEAVY ? sqrtf(r_sq) : 0.f;
+        bool full = false;
+        const NodeRec rr = tnodes[root];
+        if (box_box_d2(qa.lo, qa.hi, rr.lo, rr.hi) <= r_sq) {
+            if (rec_b(rr) > 0) {
+                if (HEAVY) est += filtered_estimate(qa.lo, qa.hi, rr, rad);
+                if (c == 0 && cap > 0) {
+                    out[0] = make_int2(rec_a(rr), rec_b(rr));
+                    if (DIRS) dout[0] = dir_code(qa.lo, qa.hi, rr.lo, rr.hi);
+                }
+                cnt = 1;
+            } else {
+                if (c == 0) stk[g][0] = ((uint32_t)rec_a(rr) << 3) | (uint32_t)(-rec_b(rr) - 1);
+                int sp = 1;
+                while (sp > 0 && !full) {  // uniform in the group
+                    const uint32_t ent = stk[g][--sp];
+                    const int32_t first = (int32_t)(ent >> 3);
+                    const int32_t nch = (int32_t)(ent & 7u) + 1;
+                    NodeRec ch{};
+                    bool pass = false;
+                    if (c < nch) {
+                        ch = tnodes[first + c];
+                        pass = box_box_d2(qa.lo, qa.hi, ch.lo, ch.hi) <= r_sq;
+                    }
+                    const bool leaf = pass && rec_b(ch) > 0;
+                    const bool inner = pass && rec_b(ch) <= 0;
+                    const uint32_t lm = (uint32_t)(ballot(leaf) >> gsh) & 0xffu;
+                    const uint32_t im = (uint32_t)(ballot(inner) >> gsh) & 0xffu;
+                    if (leaf) {
+                        const int pos = cnt + __popc(lm & below);
+                        if (pos < cap) {
+                            out[pos] = make_int2(rec_a(ch), rec_b(ch));
+                            if (DIRS) dout[pos] = dir_code(qa.lo, qa.hi, ch.lo, ch.hi);
+                        }
+                    }
+                    if (HEAVY) {  // added in child order, as the one-thread walk does
+                        const float e = leaf ? filtered_estimate(qa.lo, qa.hi, ch, rad) : 0.f;
+#pragma unroll
+                        for (int j = 0; j < LL_COOP; ++j)
+                            if (lm & (1u << j)) est += __shfl(e, gsh + j, OCT_WAVE);
+                    }
+                    cnt += __popc(lm);
+                    const int np = __popc(im);
+                    if (sp + np > LL_FAST_STACK) {
+                        full = true;
+                    } else {
+                        if (inner) stk[g][sp + __popc(im & below)] = ((uint32_t)rec_a(ch) << 3) | (uint32_t)(-rec_b(ch) - 1);
+                        sp += np;
+                    }
+                }
+            }
+        }
+        if (c == 0) {
+            lcount[r] = cnt;
+            if (cnt > cap) atomicMax(&flags[0], (uint32_t)cnt);  // the host regrows cap to this
+            if (full) atomicOr(&flags[1], 1u);                   // the host reruns in safe mode
+        }
+        if (HEAVY) {  // heavy leaves: flagged, and listed compactly (one atomic per wave)
+            const int qc = rec_b(qa);
+            const bool heavy = qc > OCT_WAVE || cnt > OCT_WAVE || est > (float)(EW_TILE * OCT_WAVE) ||
+                               (float)qc * est > HEAVY_PAIRS;
+            if (c == 0) lheavy[r] = heavy ? 1 : 0;
+            const uint64_t hm = ballot(heavy && c == 0);
+            if (hm) {
+                const int lead = __ffsll((long long)hm) - 1;
+                uint32_t base = 0;
+                if (lane == lead) base = atomicAdd(n_heavy, (uint32_t)__popcll(hm));
+                base = (uint32_t)__shfl((int)base, lead, OCT_WAVE);
+                const uint32_t idx = base + (uint32_t)mbcnt(hm);
+                if (heavy && c == 0 && idx < (uint32_t)nleaves) heavy_list[idx] = r;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- stage 2
 
 
@@ -1072,9 +1190,18 @@ hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, i
     if (lheavy) OCT_LL_LAUNCH(F, D, true); \
     else OCT_LL_LAUNCH(F, D, false)
     const bool fast = !safe && t.n_nodes < (1 << 29);
-    if (fast && ldir) OCT_LL_HEAVY(true, true);
-    else if (fast) OCT_LL_HEAVY(true, false);
-    else if (ldir) OCT_LL_HEAVY(false, true);
+    if (fast) {  // eight lanes per query leaf
+        const int64_t wantc = ((int64_t)q.n_leaves + LL_COOP_GROUPS - 1) / LL_COOP_GROUPS;
+        const unsigned gc = (unsigned)(wantc < 1 ? 1 : (wantc > 65535 ? 65535 : wantc));
+#define OCT_LLC_LAUNCH(D, H)                                                                                       \
+    hipLaunchKernelGGL((k_leaf_lists_coop<D, H>), dim3(gc), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes,   \
+                       t.params, r_sq, cap, lists, ldir, lcount, flags, sets, lheavy, heavy_list, n_heavy)
+        if (ldir && lheavy) OCT_LLC_LAUNCH(true, true);
+        else if (ldir) OCT_LLC_LAUNCH(true, false);
+        else if (lheavy) OCT_LLC_LAUNCH(false, true);
+        else OCT_LLC_LAUNCH(false, false);
+#undef OCT_LLC_LAUNCH
+    } else if (ldir) OCT_LL_HEAVY(false, true);
     else OCT_LL_HEAVY(false, false);
 #undef OCT_LL_HEAVY
 #undef OCT_LL_LAUNCH
