@@ -20,7 +20,7 @@ namespace oct {
 
 static_assert(1 + 7 * OCT_MAX_LEVEL <= 160, "LIFO walk bound (src/cpu/build_cpu.jl:25-43) must fit the stack");
 constexpr int LL_SAFE_STACK = 160;  // entries of the safe walk (scratch)
-constexpr int LL_FAST_STACK = 32;   // entries of the fast walk (LDS, 4 bytes each)
+constexpr int LL_FAST_STACK = 32;   // stack entries per query leaf of the cooperative walk (LDS, 4 bytes each)
 constexpr int LL_THREADS = 256;
 constexpr int EW_WAVES = 4;     // waves per block, each owns one query leaf at a time
 constexpr int EW_STAGE = 256;   // edges one wave stages in LDS (6 KB)
@@ -90,18 +90,14 @@ __device__ inline bool ll_visit(const float4& qlo, const float4& qhi, const Node
     return push(rec_a(ch), -rec_b(ch));
 }
 
-// One thread per query leaf: LIFO depth-first walk of the target tree from
-// its branch root.  A stack entry is the child range of a node that passed
-// the test, so each round costs one dependent load of up to 8 records.
-//   FAST: the stack lives in LDS (column per thread, entries packed as
-//         first_child * 8 + nchild - 1, which needs fewer than 2^29 nodes) and
-//         holds LL_FAST_STACK entries; a walk that needs more stops and the
-//         leaf is flagged (flags[1]) so the host reruns the lists in safe mode.
-//   safe: the stack holds 1 + 7 * 21 entries, the bound of a LIFO walk over a
-//         21-level tree (src/cpu/build_cpu.jl:25-43), so it never runs out.
-// A list longer than cap is counted in full and flagged through flags[0] (the
-// host regrows cap to that count and reruns).
-template <bool FAST, bool DIRS, bool HEAVY>
+// Safe walk: one thread per query leaf, LIFO depth-first walk of the target
+// tree from its branch root.  A stack entry is the child range of a node that
+// passed the test; the stack holds 1 + 7 * 21 entries, the bound of a LIFO
+// walk over a 21-level tree (src/cpu/build_cpu.jl:25-43), so it never runs
+// out.  The host runs it when the cooperative walk below (the usual one)
+// outgrew its LDS stack.  A list longer than cap is counted in full and
+// flagged through flags[0] (the host regrows cap to that count and reruns).
+template <bool DIRS, bool HEAVY>
 __global__ void __launch_bounds__(LL_THREADS)
 k_leaf_lists(const NodeRec* __restrict__ qleaves,
              const BuildParams* __restrict__ qprm, const NodeRec* __restrict__ tnodes,
@@ -109,7 +105,6 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
              uint8_t* __restrict__ ldir, int32_t* __restrict__ lcount, uint32_t* __restrict__ flags,
              SetLookup sets, uint8_t* __restrict__ lheavy, int32_t* __restrict__ heavy_list,
              uint32_t* __restrict__ n_heavy) {
-    __shared__ uint32_t fstk[FAST ? LL_FAST_STACK : 1][LL_THREADS];
     const int32_t nleaves = qprm->n_leaves;
     const int32_t root0 = tprm->branch_root;
     const XcdShare sh = xcd_share(nleaves);
@@ -139,26 +134,6 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
                     if (DIRS) dout[cnt] = dir_code(qa.lo, qa.hi, rr.lo, rr.hi);
                 }
                 ++cnt;
-            } else if (FAST) {
-                int sp = 0;
-                auto push = [&](int32_t first, int32_t nch) {
-                    if (sp >= LL_FAST_STACK) return false;
-                    fstk[sp++][tid] = ((uint32_t)first << 3) | (uint32_t)(nch - 1);
-                    return true;
-                };
-                push(rec_a(rr), -rec_b(rr));
-                while (sp > 0 && !full) {
-                    const uint32_t ent = fstk[--sp][tid];
-                    const int32_t first = (int32_t)(ent >> 3);
-                    const int32_t nch = (int32_t)(ent & 7u) + 1;
-                    NodeRec ch[8];
-#pragma unroll
-                    for (int c = 0; c < 8; ++c)
-                        if (c < nch) ch[c] = tnodes[first + c];
-#pragma unroll
-                    for (int c = 0; c < 8; ++c)
-                        if (c < nch && !ll_visit<DIRS, HEAVY>(qa.lo, qa.hi, ch[c], r_sq, rad, out, dout, cap, cnt, est, push)) full = true;
-                }
             } else {
                 int2 stack[LL_SAFE_STACK];
                 int sp = 0;
@@ -279,11 +254,12 @@ k_leaf_lists_coop(const NodeRec* __restrict__ qleaves, const BuildParams* __rest
                             if (DIRS) dout[pos] = dir_code(qa.lo, qa.hi, ch.lo, ch.hi);
                         }
                     }
-                    if (HEAVY) {  // added in child order, as the one-thread walk does
-                        const float e = leaf ? filtered_estimate(qa.lo, qa.hi, ch, rad) : 0.f;
-#pragma unroll
-                        for (int j = 0; j < LL_COOP; ++j)
-                            if (lm & (1u << j)) est += __shfl(e, gsh + j, OCT_WAVE);
+                    if (HEAVY && lm) {  // the group's sum (a fixed order: the estimate is a heuristic)
+                        float e = leaf ? filtered_estimate(qa.lo, qa.hi, ch, rad) : 0.f;
+                        e += __shfl_xor(e, 1, OCT_WAVE);
+                        e += __shfl_xor(e, 2, OCT_WAVE);
+                        e += __shfl_xor(e, 4, OCT_WAVE);
+                        est += e;
                     }
                     cnt += __popc(lm);
                     const int np = __popc(im);
@@ -301,20 +277,41 @@ k_leaf_lists_coop(const NodeRec* __restrict__ qleaves, const BuildParams* __rest
             if (cnt > cap) atomicMax(&flags[0], (uint32_t)cnt);  // the host regrows cap to this
             if (full) atomicOr(&flags[1], 1u);                   // the host reruns in safe mode
         }
-        if (HEAVY) {  // heavy leaves: flagged, and listed compactly (one atomic per wave)
+        if (HEAVY && c == 0) {  // heavy leaves: flagged here, listed below
             const int qc = rec_b(qa);
             const bool heavy = qc > OCT_WAVE || cnt > OCT_WAVE || est > (float)(EW_TILE * OCT_WAVE) ||
                                (float)qc * est > HEAVY_PAIRS;
-            if (c == 0) lheavy[r] = heavy ? 1 : 0;
-            const uint64_t hm = ballot(heavy && c == 0);
-            if (hm) {
-                const int lead = __ffsll((long long)hm) - 1;
-                uint32_t base = 0;
-                if (lane == lead) base = atomicAdd(n_heavy, (uint32_t)__popcll(hm));
-                base = (uint32_t)__shfl((int)base, lead, OCT_WAVE);
-                const uint32_t idx = base + (uint32_t)mbcnt(hm);
-                if (heavy && c == 0 && idx < (uint32_t)nleaves) heavy_list[idx] = r;
-            }
+            lheavy[r] = heavy ? 1 : 0;
+        }
+    }
+    if (HEAVY) {
+        // The block's heavy leaves go to heavy_list with one atomic per block
+        // (one per wave, with eight leaves a wave, made the shared counter the
+        // kernel's bottleneck): count, reserve, then write, re-reading the
+        // flags this block's threads wrote.
+        __shared__ uint32_t hcount, hbase, hnext;
+        if (threadIdx.x == 0) {
+            hcount = 0;
+            hnext = 0;
+        }
+        __syncthreads();
+        uint32_t mine = 0;
+        if (c == 0)
+            for (int32_t r = (int32_t)(sh.lo + (int64_t)sh.k * LL_COOP_GROUPS) + g; r < (int32_t)sh.hi;
+                 r += (int32_t)(sh.nb * LL_COOP_GROUPS))
+                mine += lheavy[r];
+        if (mine) atomicAdd(&hcount, mine);
+        __syncthreads();
+        if (threadIdx.x == 0) hbase = hcount ? atomicAdd(n_heavy, hcount) : 0u;
+        __syncthreads();
+        if (mine) {
+            uint32_t idx = hbase + atomicAdd(&hnext, mine);
+            for (int32_t r = (int32_t)(sh.lo + (int64_t)sh.k * LL_COOP_GROUPS) + g; r < (int32_t)sh.hi;
+                 r += (int32_t)(sh.nb * LL_COOP_GROUPS))
+                if (lheavy[r]) {
+                    if (idx < (uint32_t)nleaves) heavy_list[idx] = r;
+                    ++idx;
+                }
         }
     }
 }
@@ -1180,30 +1177,25 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
 hipError_t launch_leaf_lists(const TreeView& q, const TreeView& t, float r_sq, int cap, int safe, int2* lists,
                              uint8_t* ldir, int32_t* lcount, uint32_t* flags, const SetLookup& sets,
                              uint8_t* lheavy, int32_t* heavy_list, uint32_t* n_heavy, hipStream_t s) {
-    const int64_t want = ((int64_t)q.n_leaves + LL_THREADS - 1) / LL_THREADS;
-    const unsigned g = (unsigned)(want < 1 ? 1 : (want > 65535 ? 65535 : want));
     if ((lheavy != nullptr) != (heavy_list != nullptr) || (lheavy && !n_heavy)) return hipErrorInvalidValue;
-#define OCT_LL_LAUNCH(F, D, H)                                                                              \
-    hipLaunchKernelGGL((k_leaf_lists<F, D, H>), dim3(g), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes,    \
-                       t.params, r_sq, cap, lists, ldir, lcount, flags, sets, lheavy, heavy_list, n_heavy)
-#define OCT_LL_HEAVY(F, D)               \
-    if (lheavy) OCT_LL_LAUNCH(F, D, true); \
-    else OCT_LL_LAUNCH(F, D, false)
-    const bool fast = !safe && t.n_nodes < (1 << 29);
-    if (fast) {  // eight lanes per query leaf
-        const int64_t wantc = ((int64_t)q.n_leaves + LL_COOP_GROUPS - 1) / LL_COOP_GROUPS;
-        const unsigned gc = (unsigned)(wantc < 1 ? 1 : (wantc > 65535 ? 65535 : wantc));
-#define OCT_LLC_LAUNCH(D, H)                                                                                       \
-    hipLaunchKernelGGL((k_leaf_lists_coop<D, H>), dim3(gc), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes,   \
-                       t.params, r_sq, cap, lists, ldir, lcount, flags, sets, lheavy, heavy_list, n_heavy)
-        if (ldir && lheavy) OCT_LLC_LAUNCH(true, true);
-        else if (ldir) OCT_LLC_LAUNCH(true, false);
-        else if (lheavy) OCT_LLC_LAUNCH(false, true);
-        else OCT_LLC_LAUNCH(false, false);
-#undef OCT_LLC_LAUNCH
-    } else if (ldir) OCT_LL_HEAVY(false, true);
-    else OCT_LL_HEAVY(false, false);
-#undef OCT_LL_HEAVY
+    // the cooperative walk (eight lanes per query leaf), or the one-thread
+    // safe walk when asked or when the packed stack entries cannot hold the
+    // node ids (2^29 nodes)
+    const bool coop = !safe && t.n_nodes < (1 << 29);
+    const int64_t per = coop ? LL_COOP_GROUPS : LL_THREADS;  // query leaves per block
+    const int64_t want = ((int64_t)q.n_leaves + per - 1) / per;
+    const unsigned g = (unsigned)(want < 1 ? 1 : (want > 65535 ? 65535 : want));
+#define OCT_LL_LAUNCH(K, D, H)                                                                                   \
+    hipLaunchKernelGGL((K<D, H>), dim3(g), dim3(LL_THREADS), 0, s, q.leaves, q.params, t.nodes, t.params, r_sq, cap, \
+                       lists, ldir, lcount, flags, sets, lheavy, heavy_list, n_heavy)
+#define OCT_LL_PICK(K)                                    \
+    if (ldir && lheavy) OCT_LL_LAUNCH(K, true, true);     \
+    else if (ldir) OCT_LL_LAUNCH(K, true, false);         \
+    else if (lheavy) OCT_LL_LAUNCH(K, false, true);       \
+    else OCT_LL_LAUNCH(K, false, false)
+    if (coop) OCT_LL_PICK(k_leaf_lists_coop);
+    else OCT_LL_PICK(k_leaf_lists);
+#undef OCT_LL_PICK
 #undef OCT_LL_LAUNCH
     return hipGetLastError();
 }
