@@ -89,7 +89,8 @@ def run_mode(a, mode, snaps, r):
         # (after build_edges: the refit's outcome is back on the host by then)
         refits[0] += int(octohip.build_info(tns, sid)["refit_used"])
         if predictor:
-            used[0] += int(octohip.predictor.edge_stats(tns, sid, sid, tensors=False)["used_predictor"])
+            # (known when the call returns: no wait for the stream)
+            used[0] += int(octohip.predictor.edge_stats(tns, sid, sid, tensors=False, wait=False)["used_predictor"])
         return e.senders.shape[0]
 
     step(0)

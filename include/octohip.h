@@ -241,7 +241,14 @@ typedef struct oct_edge_stats {
     int32_t attempts;          /* passes the call needed (1 when nothing overflowed) */
     int32_t heavy_leaves;      /* single pass: leaves counted ahead (too large to record in one sweep) */
 } oct_edge_stats;
+/* A learned call that returned on the single pass's early total knows its
+ * recount counters (recounted / recorded leaves) only when its kernels end:
+ * oct_edge_stats_get waits for them; oct_edge_stats_peek never waits and
+ * leaves those two counters at the previous call's values until they are in
+ * (everything else is the last call's).  For callers that read statistics
+ * every step without holding up the stream. */
 int oct_edge_stats_get(oct_ctx* ctx, int q, int t, oct_edge_stats* out);
+int oct_edge_stats_peek(oct_ctx* ctx, int q, int t, oct_edge_stats* out);
 
 /* ---- Batched sets ---------------------------------------------------------
  * No counterpart in the reference, which runs the sets of a TNS one after

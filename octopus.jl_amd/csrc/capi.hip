@@ -1378,6 +1378,14 @@ int oct_edge_stats_get(oct_ctx* c, int q, int t, oct_edge_stats* out) {
     return OCT_OK;
 }
 
+int oct_edge_stats_peek(oct_ctx* c, int q, int t, oct_edge_stats* out) {
+    if (!c || !out) return OCT_ERR_ARGUMENT;
+    PairState* p = find_pair(c, q, t);
+    if (!p) return fail(c, OCT_ERR_STATE, "edge stats: pair not registered");
+    *out = p->st;
+    return OCT_OK;
+}
+
 int oct_mlp_forward(oct_ctx* c, const float* params, const float* X, int64_t n, float* y) {
     if (!c || !params || !X || !y || n < 0) return OCT_ERR_ARGUMENT;
     HIP_TRY(c, hipSetDevice(c->device));

@@ -156,6 +156,7 @@ SIGNATURES = {
     "oct_predictor_train": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
                                            ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
     "oct_edge_stats_get": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OctEdgeStats)]),
+    "oct_edge_stats_peek": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OctEdgeStats)]),
     "oct_batch_create": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.POINTER(ctypes.c_int)]),
     "oct_batch_destroy": (ctypes.c_int, [_P, ctypes.c_int]),
     "oct_batch_config": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),

@@ -85,15 +85,18 @@ def train(tns, steps: int = 2000, batch: int = 4096, lr: float = 1e-3, seed: int
     return float(loss.value)
 
 
-def edge_stats(tns, q: int, t: int, tensors: bool = True) -> dict:
+def edge_stats(tns, q: int, t: int, tensors: bool = True, wait: bool = True) -> dict:
     """What the last build of (q, t) did, with device tensors of the per-leaf
     features, predicted hits (None unless the predictor ran) and exact hits
-    (all None with tensors=False)."""
+    (all None with tensors=False).  wait=False never waits for the stream:
+    the recount counters of a call that returned on the single pass's early
+    total are then the previous call's until its kernels end."""
     from . import _wrap, _torch
 
     torch = _torch()
     s = L.OctEdgeStats()
-    tns._check(tns._lib.oct_edge_stats_get(tns._c(), int(q), int(t), ctypes.byref(s)))
+    get = tns._lib.oct_edge_stats_get if wait else tns._lib.oct_edge_stats_peek
+    tns._check(get(tns._c(), int(q), int(t), ctypes.byref(s)))
     n, dev = int(s.n_leaves), tns.device
     return {
         "features": _wrap(s.features, (n, FEATURES), torch.float32, dev) if tensors and s.features else None,

@@ -494,8 +494,11 @@ def test_single_pass_returns_on_its_total(orc):
         sums.append(torch.stack([e.senders.double().sum(), e.receivers.double().sum(),
                                  e.rel_dist_norm.double().sum(), e.rel_displacement.double().abs().sum()]))
         steps.append(cur)
+    pk = predictor.edge_stats(tns, sid, sid, tensors=False, wait=False)  # no wait for the stream
     st = predictor.edge_stats(tns, sid, sid)
     assert st["single_pass"] and st["recorded_leaves"] + st["recounted_leaves"] == st["n_leaves"]
+    for k in ("n_edges", "n_leaves", "single_pass", "used_predictor", "attempts", "heavy_leaves"):
+        assert pk[k] == st[k], k
     for k, cur in enumerate(steps):
         o = orc.build_edges(orc.run(cur, r), cur)
         want = np.array([o.senders.astype(np.float64).sum(), o.receivers.astype(np.float64).sum(),
