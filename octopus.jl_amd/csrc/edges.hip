@@ -759,7 +759,7 @@ __device__ inline void leaf_set_params(const SetLookup& sets, int32_t qf, float 
 
 // ------------------------------------------------------------ heavy leaves
 
-// Exact counts of the heavy query leaves that k_leaf_lists listed, one wave
+// Exact counts of the heavy query leaves that the candidate walk listed, one wave
 // per leaf, before the single pass: per-query counts into qcount, the leaf's
 // total into lhits[r].  Same sweep as the two-pass COUNT kernel.
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
@@ -777,7 +777,7 @@ k_edges_heavy(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qs
     const int32_t nwaves = (int32_t)(gridDim.x * EW_WAVES);
     for (int32_t i = (int32_t)blockIdx.x * EW_WAVES + wv; i < nh; i += nwaves) {
         const int32_t r = __builtin_amdgcn_readfirstlane(heavy_list[i]);
-        if (r < 0 || r >= nleaves) continue;  // wave-uniform; k_leaf_lists writes only leaves
+        if (r < 0 || r >= nleaves) continue;  // wave-uniform; the candidate walk lists only leaves
         const NodeRec qa = qleaves[r];
         const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
         const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
@@ -815,7 +815,7 @@ k_edges_heavy_fill(const NodeRec* __restrict__ qleaves, const float4* __restrict
     const int32_t nwaves = (int32_t)(gridDim.x * EW_WAVES);
     for (int32_t i = (int32_t)blockIdx.x * EW_WAVES + wv; i < nh; i += nwaves) {
         const int32_t r = __builtin_amdgcn_readfirstlane(heavy_list[i]);
-        if (r < 0 || r >= nleaves) continue;  // wave-uniform; k_leaf_lists writes only leaves
+        if (r < 0 || r >= nleaves) continue;  // wave-uniform; the candidate walk lists only leaves
         const NodeRec qa = qleaves[r];
         const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
         const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
