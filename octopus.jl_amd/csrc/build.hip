@@ -397,7 +397,11 @@ __global__ void __launch_bounds__(TOPO_THREADS)
 k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, int batched, uint16_t* __restrict__ topo,
              uint32_t* __restrict__ hist, int nblk) {
     __shared__ int8_t m[TOPO_THREADS + 2 * TOPO_MAX_T + 1];
-    __shared__ uint32_t wc[TOPO_THREADS / OCT_WAVE][OCT_NLEVELS + 1];
+    // the block's node starts per level as differences: a position starting
+    // nodes at levels mk+1 .. lam adds 1 at mk+1 and takes 1 at lam+1; [23]
+    // counts the leaves (one per starting position)
+    __shared__ int32_t dl[OCT_NLEVELS + 2];
+    if (threadIdx.x < OCT_NLEVELS + 2) dl[threadIdx.x] = 0;
     const int64_t base = (int64_t)blockIdx.x * TOPO_THREADS;
     const int64_t lo = base - T;  // m[i] holds m_{lo + i}
     const int span = TOPO_THREADS + 2 * T + 1;
@@ -468,18 +472,19 @@ k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, int batched, uin
     }
     // m is stored +1 so k == 0 (m = -1) fits the low 5 bits
     if (k < n) topo[k] = (uint16_t)((mk + 1) | (lam << 5) | (emit ? (1 << 10) : 0));
-    const int w = threadIdx.x / OCT_WAVE;
-    for (int L = 0; L < OCT_NLEVELS; ++L) {
-        const uint64_t b = ballot(emit && mk < L && L <= lam);
-        if (lane_id() == 0) wc[w][L] = (uint32_t)__popcll(b);
+    if (emit) {  // mk + 1 >= 0 and lam + 1 <= OCT_NLEVELS
+        atomicAdd(&dl[mk + 1], 1);
+        atomicSub(&dl[lam + 1], 1);
+        atomicAdd(&dl[OCT_NLEVELS + 1], 1);
     }
-    const uint64_t bl = ballot(emit);
-    if (lane_id() == 0) wc[w][OCT_NLEVELS] = (uint32_t)__popcll(bl);
     __syncthreads();
     if (threadIdx.x <= OCT_NLEVELS) {
-        uint32_t s = 0;
-        for (int q = 0; q < TOPO_THREADS / OCT_WAVE; ++q) s += wc[q][threadIdx.x];
-        hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = s;
+        int32_t s = 0;
+        if (threadIdx.x < OCT_NLEVELS)
+            for (int l = 0; l <= (int)threadIdx.x; ++l) s += dl[l];
+        else
+            s = dl[OCT_NLEVELS + 1];
+        hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = (uint32_t)s;
     }
 }
 
