@@ -100,8 +100,11 @@ k_bounds_partial(const float* __restrict__ coords, int64_t n, float* __restrict_
     }
 }
 
+// (also zeroes `zero[0 .. nz)`: the tile sums k_refit_keys adds up next)
 __global__ void __launch_bounds__(BND_THREADS)
-k_bounds_final(const uint32_t* __restrict__ partials, int nblocks, float radius, BuildParams* __restrict__ prm) {
+k_bounds_final(const uint32_t* __restrict__ partials, int nblocks, float radius, BuildParams* __restrict__ prm,
+               uint64_t* __restrict__ zero, int64_t nz) {
+    for (int64_t i = threadIdx.x; i < nz; i += BND_THREADS) zero[i] = 0ull;
     bounds_fold(partials, 0, nblocks, radius, prm);
 }
 
@@ -267,18 +270,34 @@ __device__ inline bool key_idx_less(uint64_t ka, int32_t ia, uint64_t kb, int32_
 __global__ void __launch_bounds__(RF_THREADS)
 k_refit_keys(const float* __restrict__ coords, int64_t n, const BuildParams* __restrict__ prm,
              const uint64_t* __restrict__ pk, const float4* __restrict__ prev_sc, uint64_t* __restrict__ nk,
-             int32_t* __restrict__ pi, uint32_t* __restrict__ chg) {
+             int32_t* __restrict__ pi, uint32_t* __restrict__ chg, uint64_t* __restrict__ tile_sums) {
     const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
-    if (i >= n) return;
-    const int32_t idx = __float_as_int(prev_sc[i].w);
-    const float ox = prm->origin[0], oy = prm->origin[1], oz = prm->origin[2];
-    const float inv = prm->inv_cs;
-    const float px = coords[3 * (int64_t)idx], py = coords[3 * (int64_t)idx + 1], pz = coords[3 * (int64_t)idx + 2];
-    const uint64_t k = morton3(floor_clamp((px - ox) * inv, 0x1fffffu), floor_clamp((py - oy) * inv, 0x1fffffu),
-                               floor_clamp((pz - oz) * inv, 0x1fffffu));
-    nk[i] = k;
-    pi[i] = idx;
-    chg[i] = k != pk[i] ? 1u : 0u;
+    bool changed = false;
+    if (i < n) {
+        const int32_t idx = __float_as_int(prev_sc[i].w);
+        const float ox = prm->origin[0], oy = prm->origin[1], oz = prm->origin[2];
+        const float inv = prm->inv_cs;
+        const float px = coords[3 * (int64_t)idx], py = coords[3 * (int64_t)idx + 1],
+                    pz = coords[3 * (int64_t)idx + 2];
+        const uint64_t k = morton3(floor_clamp((px - ox) * inv, 0x1fffffu),
+                                   floor_clamp((py - oy) * inv, 0x1fffffu), floor_clamp((pz - oz) * inv, 0x1fffffu));
+        nk[i] = k;
+        pi[i] = idx;
+        changed = k != pk[i];
+        chg[i] = changed ? 1u : 0u;
+    }
+    // the changed count of this block into its scan tile's sum (the scan of
+    // chg then needs no pass of its own to sum the tiles)
+    __shared__ uint32_t bc;
+    if (threadIdx.x == 0) bc = 0;
+    __syncthreads();
+    const uint64_t m = ballot(changed);
+    if (lane_id() == 0 && m) atomicAdd(&bc, (uint32_t)__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0 && bc) {
+        const int64_t tile = ((int64_t)blockIdx.x * RF_THREADS) / SCAN_TILE_INPUTS;
+        atomicAdd(reinterpret_cast<unsigned long long*>(tile_sums + tile), (unsigned long long)bc);
+    }
 }
 
 // K3: whether the refit can run (the changed count within limit and the key
@@ -741,7 +760,8 @@ hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, flo
         hipLaunchKernelGGL(k_bounds_partial<true>, dim3(nb), dim3(BND_THREADS), 0, s, coords, n, copy, b.partials);
     else
         hipLaunchKernelGGL(k_bounds_partial<false>, dim3(nb), dim3(BND_THREADS), 0, s, coords, n, copy, b.partials);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BND_THREADS), 0, s, b.partials, nb, radius, b.prm);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BND_THREADS), 0, s, b.partials, nb, radius, b.prm,
+                       reinterpret_cast<uint64_t*>(b.scan_tmp), scan_tiles(n));
     return hipGetLastError();
 }
 
@@ -785,9 +805,10 @@ hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int i
 // runs the full sort.
 hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, const BuildBufs& b, hipStream_t s) {
     const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
+    // the tile sums in scan_tmp were zeroed by the bounds fold just before
     hipLaunchKernelGGL(k_refit_keys, dim3(g), dim3(RF_THREADS), 0, s, coords, n, b.prm, b.keys[cur], b.scoord,
-                       b.keys[cur ^ 1], b.vals[0], (uint32_t*)b.vals[1]);
-    return exclusive_scan((const uint32_t*)b.vals[1], (uint32_t*)b.vals[1], n, b.scan_tmp, b.total, s);
+                       b.keys[cur ^ 1], b.vals[0], (uint32_t*)b.vals[1], reinterpret_cast<uint64_t*>(b.scan_tmp));
+    return exclusive_scan_summed((const uint32_t*)b.vals[1], (uint32_t*)b.vals[1], n, b.scan_tmp, b.total, s);
 }
 
 // Part 2: the sorted keys end in keys[cur ^ 1].  Grids cover `limit` changed

@@ -466,7 +466,8 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
     HIP_TRY(c, s.vals0.ensure(4 * (size_t)n));
     HIP_TRY(c, s.vals1.ensure(4 * (size_t)n));
     HIP_TRY(c, s.hist.ensure(4 * (size_t)hist_words(n)));
-    HIP_TRY(c, s.scan_tmp.ensure(4 * (size_t)scan_tmp_words(hist_words(n))));
+    // the scans: radix and topology histograms, and (refit) the n changed flags
+    HIP_TRY(c, s.scan_tmp.ensure(4 * (size_t)scan_tmp_words(std::max<int64_t>(n, hist_words(n)))));
     HIP_TRY(c, s.total.ensure(16));  // uint64 total
     HIP_TRY(c, s.topo.ensure(2 * (size_t)n));
     HIP_TRY(c, s.perm.ensure(4 * (size_t)n));

@@ -342,9 +342,16 @@ hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* sen
                            float* grad, hipStream_t s);
 
 int64_t scan_tmp_words(int64_t n);
+// Tiles of the device scan (SCAN_TILE_INPUTS inputs each).
+constexpr int SCAN_TILE_INPUTS = 4096;
+int64_t scan_tiles(int64_t n);
 // Exclusive scan of n uint32 counts; the optional 64-bit total is exact even
 // when the uint32 offsets would wrap (callers check it first).
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
                           uint64_t* total_out, hipStream_t s);
+// The same scan when tmp already holds the 64-bit sums of the input's tiles
+// (its producer added them up): one launch instead of two.
+hipError_t exclusive_scan_summed(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
+                                 uint64_t* total_out, hipStream_t s);
 
 }  // namespace oct
