@@ -526,8 +526,9 @@ __device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
 // Writes node_first, the packed record (first child for internal nodes),
 // parent links and levels; leaves also get their range end and bounds, and
 // the nodes covering all n points (the chain down to the first split) get the
-// global bounds.  Child slots and counts come from k_link, internal bounds and
-// range ends from k_level_refit.
+// global bounds.  Child slots and counts come from k_link_refit, internal
+// bounds and range ends from k_link_refit and k_refit_merge, which find each
+// leaf's ancestors in ranc (per level, in leaf order).
 __global__ void __launch_bounds__(TOPO_THREADS)
 k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, int64_t n, int T,
             const uint16_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
@@ -535,7 +536,7 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
             NodeRec* __restrict__ nodes, int32_t* __restrict__ nfirst, int32_t* __restrict__ nlast,
             int32_t* __restrict__ nchild, float* __restrict__ bmin, float* __restrict__ bmax,
             int32_t* __restrict__ parent, uint8_t* __restrict__ nlevel, int32_t* __restrict__ leaf_ids,
-            NodeRec* __restrict__ leaves) {
+            NodeRec* __restrict__ leaves, int32_t* __restrict__ ranc, int64_t astride) {
     constexpr int NW = TOPO_THREADS / OCT_WAVE;
     __shared__ uint32_t wc[NW][OCT_NLEVELS + 1];
     __shared__ uint64_t wmask[NW][OCT_NLEVELS + 1];  // per wave and level: lanes starting a node
@@ -656,12 +657,16 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
             rec.lo = make_float4(o2f(lo3[0]), o2f(lo3[1]), o2f(lo3[2]), __int_as_float((int)k));
             rec.hi = make_float4(o2f(hi3[0]), o2f(hi3[1]), o2f(hi3[2]), __int_as_float(cnt));
             const int32_t lr = before(OCT_NLEVELS) - M;
+            // the leaf's ancestor at each level the refit visits (-1 from its
+            // own level on): the last node of that level starting at or before k
+            for (int La = forest.batched ? 0 : L0 + 1; La < OCT_MAX_LEVEL; ++La)
+                ranc[(int64_t)La * astride + lr] = La >= lam ? -1 : before(La) - (La >= Lfirst ? 0 : 1);
             leaf_ids[lr] = id;
             leaves[lr] = rec;
             if (chain) sp->branch_root = id;  // the whole set sits in one leaf
         } else {
             rec.lo = make_float4(0.f, 0.f, 0.f, __int_as_float(before(L + 1)));  // first child
-            rec.hi = make_float4(0.f, 0.f, 0.f, __int_as_float(0));             // -children, from k_link
+            rec.hi = make_float4(0.f, 0.f, 0.f, __int_as_float(0));             // -children, from k_link_refit
             if (chain) {
                 rec.lo.x = o2f(sp->omin[0]); rec.lo.y = o2f(sp->omin[1]); rec.lo.z = o2f(sp->omin[2]);
                 rec.hi.x = o2f(sp->omax[0]); rec.hi.y = o2f(sp->omax[1]); rec.hi.z = o2f(sp->omax[2]);
@@ -677,61 +682,281 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
     }
 }
 
-// Every non-root node writes its id into its parent's octant slot and counts
-// itself into the parent's record (b = -children).
-__global__ void __launch_bounds__(256)
-k_link(const uint64_t* __restrict__ sk, const BuildParams* __restrict__ prm, int batched, int index_base,
-       const int32_t* __restrict__ parent, const int32_t* __restrict__ nfirst, const uint8_t* __restrict__ nlevel,
-       int32_t* __restrict__ nchild, NodeRec* __restrict__ nodes) {
-    const int32_t M = prm->n_nodes;
-    for (int32_t id = 1 + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); id < M;
-         id += (int32_t)(gridDim.x * blockDim.x)) {
-        const int32_t p = parent[id];
-        if (p < 0) continue;  // the root of another set
-        const int L = nlevel[id];
-        const uint64_t key = key_morton(sk[nfirst[id] - index_base], batched);
-        const int o = (int)((key >> level_shift(L - 1)) & 7ull);
-        nchild[8 * (int64_t)p + o] = id + index_base;
-        atomicSub((int*)&nodes[p].hi.w, 1);
+// Internal bounds and range ends travel as seven maxima: ~f2o of the lower
+// bounds, f2o of the upper ones, the range end.  A node's results, stored.
+__device__ inline void store_refit(int32_t id, const uint32_t* v, NodeRec* __restrict__ nodes,
+                                   int32_t* __restrict__ nlast, float* __restrict__ bmin, float* __restrict__ bmax) {
+    float* nr = reinterpret_cast<float*>(nodes + id);  // x, y, z of lo and hi: the counts in w stay
+    for (int d = 0; d < 3; ++d) {
+        const float lo = o2f(~v[d]), hi = o2f(v[3 + d]);
+        nr[d] = lo;
+        nr[4 + d] = hi;
+        bmin[3 * (int64_t)id + d] = lo;
+        bmax[3 * (int64_t)id + d] = hi;
+    }
+    nlast[id] = (int32_t)v[6];
+}
+
+// An unfinished run: the node (-1: none) and its seven maxima.
+constexpr int REFIT_SLOT_WORDS = 8;
+constexpr int LR_THREADS = 512;  // k_link_refit's blocks; the refit takes LR_THREADS leaves per block
+constexpr int LR_WAVES = LR_THREADS / OCT_WAVE;
+int64_t refit_blocks(int64_t n) { return (n + LR_THREADS - 1) / LR_THREADS; }
+int64_t refit_slot_words(int64_t n) { return (int64_t)OCT_MAX_LEVEL * 2 * refit_blocks(n) * REFIT_SLOT_WORDS; }
+__device__ inline void put_slot(uint32_t* p, int32_t id, const uint32_t* v) {
+    reinterpret_cast<uint4*>(p)[0] = make_uint4((uint32_t)id, v[0], v[1], v[2]);
+    reinterpret_cast<uint4*>(p)[1] = make_uint4(v[3], v[4], v[5], v[6]);
+}
+
+// Two jobs in one launch, both on k_topo_emit's output and independent of
+// each other.
+// The last link_blocks blocks: every non-root node writes its id into its
+// parent's octant slot and counts itself into the parent's record
+// (b = -children).
+// The blocks before them: internal bounds and range ends from the leaves.  A block
+// takes LR_THREADS consecutive leaves (Morton order) and their ancestors per
+// level (ranc), then goes down the levels: at each level the lanes under one
+// node form a run of consecutive lanes; a segmented wave scan leaves the
+// maxima of each wave's part of a run in its last lane, and the parts that
+// cross into later waves are carried over in LDS.  A run that holds all of
+// its node's leaves (the leaves either side of the block tell) stores the
+// results; a run cut by an edge of the block goes to the block's slot for
+// that level and edge (slots[L][block][edge]; a run over the whole block
+// fills both), and k_refit_merge finishes those nodes.  The chain of a single
+// set keeps its set's bounds.
+__global__ void __launch_bounds__(LR_THREADS)
+k_link_refit(const uint64_t* __restrict__ sk, const BuildParams* __restrict__ prm, int batched, int index_base,
+             int link_blocks, const int32_t* __restrict__ parent, const int32_t* __restrict__ nfirst,
+             const uint8_t* __restrict__ nlevel, const int32_t* __restrict__ ranc, int64_t astride,
+             const NodeRec* __restrict__ leaves, int32_t* __restrict__ nchild, NodeRec* __restrict__ nodes,
+             int32_t* __restrict__ nlast, float* __restrict__ bmin, float* __restrict__ bmax,
+             uint32_t* __restrict__ slots, int64_t bstride) {
+    __shared__ int32_t s_anc[LR_WAVES][OCT_MAX_LEVEL][OCT_WAVE];  // the lanes' ancestors per level
+    __shared__ int32_t s_edge[OCT_MAX_LEVEL][2];                   // those of the leaves before / after the block
+    __shared__ int32_t s_fid[LR_WAVES], s_lid[LR_WAVES], s_one[LR_WAVES];  // per wave: ids at lanes 0 / 63, one run
+    __shared__ uint32_t s_lv[LR_WAVES][7];                         // the maxima at lane 63
+    __shared__ int32_t s_cut[2];                                   // the level's edge slots were written
+    const int nrb = (int)gridDim.x - link_blocks;  // the refit's blocks come first
+    if ((int)blockIdx.x >= nrb) {
+        const int32_t M = prm->n_nodes;
+        for (int32_t id = 1 + (int32_t)((blockIdx.x - nrb) * blockDim.x + threadIdx.x); id < M;
+             id += (int32_t)(link_blocks * blockDim.x)) {
+            const int32_t p = parent[id];
+            if (p < 0) continue;  // the root of another set
+            const int L = nlevel[id];
+            const uint64_t key = key_morton(sk[nfirst[id] - index_base], batched);
+            const int o = (int)((key >> level_shift(L - 1)) & 7ull);
+            nchild[8 * (int64_t)p + o] = id + index_base;
+            atomicSub((int*)&nodes[p].hi.w, 1);
+        }
+        return;
+    }
+    const int32_t nl = prm->n_leaves;
+    const int L0 = batched ? -1 : prm->chain_level;  // levels 0..L0: the chain, not refit
+    const int lane = lane_id();
+    const int wv = threadIdx.x / OCT_WAVE;
+    const uint64_t upto = lane == OCT_WAVE - 1 ? ~0ull : ((2ull << lane) - 1ull);
+    for (int64_t blk = blockIdx.x; blk * LR_THREADS < nl; blk += nrb) {  // block-uniform
+        const int32_t bbase = (int32_t)(blk * LR_THREADS);
+        const int32_t lr = bbase + (int32_t)threadIdx.x;
+        const bool valid = lr < nl;
+        // every level's ancestor load in flight at once; the first thread also
+        // reads the leaf before the block's, the last one the leaf after
+        const bool edge = (threadIdx.x == 0 && bbase > 0) || (threadIdx.x == LR_THREADS - 1 && lr + 1 < nl);
+        const int64_t ej = threadIdx.x == 0 ? (int64_t)bbase - 1 : (int64_t)lr + 1;
+        int32_t a[OCT_MAX_LEVEL], x[OCT_MAX_LEVEL];
+#pragma unroll
+        for (int L = 0; L < OCT_MAX_LEVEL; ++L) {
+            a[L] = -1;
+            x[L] = -1;
+            if (L > L0) {
+                if (valid) a[L] = ranc[(int64_t)L * astride + lr];
+                if (edge) x[L] = ranc[(int64_t)L * astride + ej];
+            }
+        }
+        uint32_t v[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
+        if (valid) {
+            const NodeRec r = leaves[lr];
+            const int32_t first = __float_as_int(r.lo.w);
+            v[0] = ~f2o(r.lo.x); v[1] = ~f2o(r.lo.y); v[2] = ~f2o(r.lo.z);
+            v[3] = f2o(r.hi.x);  v[4] = f2o(r.hi.y);  v[5] = f2o(r.hi.z);
+            v[6] = (uint32_t)(first + __float_as_int(r.hi.w) - 1 + index_base);
+        }
+        __syncthreads();  // the last block's LDS reads are done
+#pragma unroll
+        for (int L = 0; L < OCT_MAX_LEVEL; ++L) {
+            s_anc[wv][L][lane] = a[L];
+            if (threadIdx.x == 0) s_edge[L][0] = x[L];
+            if (threadIdx.x == LR_THREADS - 1) s_edge[L][1] = x[L];
+        }
+        for (int L = OCT_MAX_LEVEL - 1; L > L0; --L) {
+            uint32_t* sl = slots + ((int64_t)L * bstride + blk) * 2 * REFIT_SLOT_WORDS;
+            if (threadIdx.x == 0) {
+                s_cut[0] = 0;
+                s_cut[1] = 0;
+            }
+            __syncthreads();  // s_anc written; the last level's carries read
+            const int32_t cur = s_anc[wv][L][lane];
+            const bool active = cur >= 0;
+            const uint64_t am = ballot(active);
+            const int32_t prev = __shfl_up(cur, 1, OCT_WAVE), next = __shfl_down(cur, 1, OCT_WAVE);
+            const bool head = active && (lane == 0 || !((am >> (lane - 1)) & 1ull) || prev != cur);
+            const bool tail = active && (lane == OCT_WAVE - 1 || !((am >> (lane + 1)) & 1ull) || next != cur);
+            const uint64_t hm = ballot(head) & upto;
+            // an inactive lane keeps its values: its segment starts past itself
+            const int seg = active ? 63 - __clzll((long long)hm) : lane + 1;
+#pragma unroll
+            for (int q = 0; q < 7; ++q) v[q] = seg_max_scan(v[q], lane, seg);
+            if (lane == 0) s_fid[wv] = cur;
+            if (lane == OCT_WAVE - 1) {
+                s_lid[wv] = cur;
+                s_one[wv] = active && seg == 0;
+                for (int q = 0; q < 7; ++q) s_lv[wv][q] = v[q];
+            }
+            __syncthreads();
+            // the run open into this wave from the block's earlier ones, and
+            // whether it reaches back to the block's first lane
+            int32_t cid = -1;
+            bool creach = false;
+            uint32_t cv[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
+            for (int y = 0; y < wv; ++y) {
+                if (s_one[y] && s_lid[y] == cid) {
+                    for (int q = 0; q < 7; ++q) cv[q] = s_lv[y][q] > cv[q] ? s_lv[y][q] : cv[q];
+                } else {
+                    cid = s_lid[y];
+                    creach = y == 0 && s_one[y];
+                    for (int q = 0; q < 7; ++q) cv[q] = s_lv[y][q];
+                }
+            }
+            bool reach0 = false;  // this lane's run reaches the block's first lane
+            if (active && seg == 0) {
+                if (cid == cur) {
+                    for (int q = 0; q < 7; ++q) v[q] = cv[q] > v[q] ? cv[q] : v[q];
+                    reach0 = creach;
+                } else {
+                    reach0 = wv == 0;
+                }
+            }
+            const bool cont = tail && lane == OCT_WAVE - 1 && wv < LR_WAVES - 1 && s_fid[wv + 1] == cur;
+            const bool reachE = tail && lane == OCT_WAVE - 1 && wv == LR_WAVES - 1;  // to the block's last lane
+            // all of the node's leaves lie in this run: the leaves either side
+            // of the block are not the node's where the run reaches that edge
+            const bool whole = (!reach0 || s_edge[L][0] != cur) && (!reachE || s_edge[L][1] != cur);
+            if (tail && !cont) {
+                if (whole) {
+                    store_refit(cur, v, nodes, nlast, bmin, bmax);
+                } else {
+                    if (reach0) {
+                        put_slot(sl, cur, v);
+                        s_cut[0] = 1;
+                    }
+                    if (reachE) {
+                        put_slot(sl + REFIT_SLOT_WORDS, cur, v);
+                        s_cut[1] = 1;
+                    }
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (!s_cut[0]) sl[0] = ~0u;
+                if (!s_cut[1]) sl[REFIT_SLOT_WORDS] = ~0u;
+            }
+        }
     }
 }
 
-// Internal bounds and range ends for the nodes of one level, from their
-// children (finished by the previous launch: launches go deepest first).
-__global__ void __launch_bounds__(256)
-k_level_refit(int L, const uint32_t* __restrict__ hoff, int nblk, const BuildParams* __restrict__ prm,
-              int skip_chain, int index_base, NodeRec* __restrict__ nodes, int32_t* __restrict__ nlast,
-              const int32_t* __restrict__ nfirst, float* __restrict__ bmin, float* __restrict__ bmax) {
-    const int32_t lo = (int32_t)hoff[(int64_t)L * nblk];
-    const int32_t hi = (int32_t)hoff[(int64_t)(L + 1) * nblk];
-    const int L0 = prm->chain_level;
-    for (int32_t id = lo + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); id < hi;
-         id += (int32_t)(gridDim.x * blockDim.x)) {
-        NodeRec rec = nodes[id];
-        const int nc = -rec_b(rec);
-        if (nc <= 0) continue;                                  // leaf
-        if (skip_chain && L <= L0 && nfirst[id] == index_base) continue;  // chain: set bounds already
-        const int c0 = rec_a(rec);
-        uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
-        uint32_t hi3[3] = {0u, 0u, 0u};
-        for (int c = 0; c < nc; ++c) {
-            const NodeRec ch = nodes[c0 + c];
-            const float a[3] = {ch.lo.x, ch.lo.y, ch.lo.z};
-            const float b[3] = {ch.hi.x, ch.hi.y, ch.hi.z};
-            for (int d = 0; d < 3; ++d) {
-                const uint32_t x = f2o(a[d]), y = f2o(b[d]);
-                lo3[d] = x < lo3[d] ? x : lo3[d];
-                hi3[d] = y > hi3[d] ? y : hi3[d];
+// The runs k_link_refit left unfinished, one block per level.  A level's
+// slots (two per k_link_refit block) lie in leaf order and the slots of one node are consecutive (its
+// leaves are), so a segmented max over them, in chunks of 1024 slots with
+// the open segment carried from chunk to chunk, finishes every such node.
+constexpr int RM_THREADS = 1024;
+__global__ void __launch_bounds__(RM_THREADS)
+k_refit_merge(const BuildParams* __restrict__ prm, int batched, const uint32_t* __restrict__ slots,
+              int64_t bstride, NodeRec* __restrict__ nodes, int32_t* __restrict__ nlast, float* __restrict__ bmin,
+              float* __restrict__ bmax) {
+    constexpr int NW = RM_THREADS / OCT_WAVE;
+    __shared__ int32_t s_last[NW], s_one[NW];  // per wave: the id at lane 63, one segment over the wave
+    __shared__ uint32_t s_lastv[NW][7];        // the maxima at lane 63
+    __shared__ int32_t s_cid[NW + 1];          // the open segment into each wave (NW: into the next chunk)
+    __shared__ uint32_t s_cv[NW + 1][7];
+    const int L = blockIdx.x;
+    const int L0 = batched ? -1 : prm->chain_level;
+    if (L <= L0) return;
+    const int64_t nslots = 2 * (((int64_t)prm->n_leaves + LR_THREADS - 1) / LR_THREADS);
+    const uint32_t* ls = slots + (int64_t)L * bstride * 2 * REFIT_SLOT_WORDS;
+    const int w = threadIdx.x / OCT_WAVE, lane = lane_id();
+    if (threadIdx.x == 0) {
+        s_cid[NW] = -1;
+        for (int q = 0; q < 7; ++q) s_cv[NW][q] = 0u;
+    }
+    auto load = [&](int64_t e, int32_t& id, uint32_t* v) {
+        id = -1;
+        for (int q = 0; q < 7; ++q) v[q] = 0u;
+        if (e < nslots) {
+            const uint4 a = reinterpret_cast<const uint4*>(ls + e * REFIT_SLOT_WORDS)[0];
+            const uint4 b = reinterpret_cast<const uint4*>(ls + e * REFIT_SLOT_WORDS)[1];
+            id = (int32_t)a.x;
+            v[0] = a.y; v[1] = a.z; v[2] = a.w; v[3] = b.x; v[4] = b.y; v[5] = b.z; v[6] = b.w;
+        }
+    };
+    int32_t idn;
+    uint32_t vn[7];
+    load(threadIdx.x, idn, vn);
+    const uint64_t upto = lane == OCT_WAVE - 1 ? ~0ull : ((2ull << lane) - 1ull);
+    for (int64_t c0 = 0; c0 < nslots; c0 += RM_THREADS) {  // block-uniform
+        const int64_t e = c0 + threadIdx.x;
+        int32_t id = idn;
+        uint32_t v[7];
+        for (int q = 0; q < 7; ++q) v[q] = vn[q];
+        load(e + RM_THREADS, idn, vn);  // the next chunk, in flight meanwhile
+        int32_t nid = -1;
+        if (lane == OCT_WAVE - 1 && e + 1 < nslots) nid = (int32_t)ls[(e + 1) * REFIT_SLOT_WORDS];
+        const int32_t pid = __shfl_up(id, 1, OCT_WAVE), dn = __shfl_down(id, 1, OCT_WAVE);
+        if (lane < OCT_WAVE - 1) nid = dn;
+        const bool act = id >= 0;
+        const bool head = act && (lane == 0 || pid != id);
+        const uint64_t hm = ballot(head) & upto;
+        const int seg = act ? 63 - __clzll((long long)hm) : lane + 1;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) v[q] = seg_max_scan(v[q], lane, seg);
+        __syncthreads();  // the last chunk's reads of s_cid / s_cv are done
+        if (lane == OCT_WAVE - 1) {
+            s_last[w] = id;
+            s_one[w] = act && seg == 0;
+            for (int q = 0; q < 7; ++q) s_lastv[w][q] = v[q];
+        }
+        __syncthreads();
+        if (w == 0) {
+            // lane 0: the segment open into the chunk; lane j: the one open
+            // after wave j - 1, which continues lane j - 1's when wave j - 1 is
+            // one segment of that node
+            int32_t cid = -1;
+            uint32_t cv[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
+            bool one = false;
+            if (lane == 0) {
+                cid = s_cid[NW];
+                for (int q = 0; q < 7; ++q) cv[q] = s_cv[NW][q];
+            } else if (lane <= NW) {
+                cid = s_last[lane - 1];
+                one = s_one[lane - 1] != 0;
+                for (int q = 0; q < 7; ++q) cv[q] = s_lastv[lane - 1][q];
+            }
+            const int32_t cprev = __shfl_up(cid, 1, OCT_WAVE);
+            const bool cact = lane <= NW;
+            const bool chead = cact && (lane == 0 || !(one && cid == cprev));
+            const uint64_t chm = ballot(chead) & upto;
+            const int cseg = cact ? 63 - __clzll((long long)chm) : lane + 1;
+#pragma unroll
+            for (int q = 0; q < 7; ++q) cv[q] = seg_max_scan(cv[q], lane, cseg);
+            if (cact) {
+                s_cid[lane] = cid;
+                for (int q = 0; q < 7; ++q) s_cv[lane][q] = cv[q];
             }
         }
-        rec.lo.x = o2f(lo3[0]); rec.lo.y = o2f(lo3[1]); rec.lo.z = o2f(lo3[2]);
-        rec.hi.x = o2f(hi3[0]); rec.hi.y = o2f(hi3[1]); rec.hi.z = o2f(hi3[2]);
-        nodes[id] = rec;
-        nlast[id] = nlast[c0 + nc - 1];
-        for (int d = 0; d < 3; ++d) {
-            bmin[3 * (int64_t)id + d] = d == 0 ? rec.lo.x : (d == 1 ? rec.lo.y : rec.lo.z);
-            bmax[3 * (int64_t)id + d] = d == 0 ? rec.hi.x : (d == 1 ? rec.hi.y : rec.hi.z);
-        }
+        __syncthreads();
+        if (act && seg == 0 && s_cid[w] == id)
+            for (int q = 0; q < 7; ++q) v[q] = s_cv[w][q] > v[q] ? s_cv[w][q] : v[q];
+        if (act && nid != id) store_refit(id, v, nodes, nlast, bmin, bmax);
     }
 }
 
@@ -843,15 +1068,17 @@ hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base,
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, b.scoord, n, T, b.topo, b.hist,
                        nblk, b.total, b.prm, f, index_base, b.nodes, b.nfirst, b.nlast, b.nchild, b.bmin, b.bmax,
-                       b.parent, b.nlevel, b.leaf_ids, b.leaves);
-    const unsigned gl = (unsigned)std::min<int64_t>((max_nodes(n, T) + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_link, dim3(gl), dim3(256), 0, s, sk, b.prm, f.batched, index_base, b.parent, b.nfirst,
-                       b.nlevel, b.nchild, b.nodes);
-    // internal levels lie strictly below a set's chain; the chain ends at or below 21 - key_bits / 3
-    const int lmin = f.batched ? 0 : 21 - key_bits / 3;
-    for (int L = OCT_MAX_LEVEL - 1; L >= (lmin < 0 ? 0 : lmin); --L)
-        hipLaunchKernelGGL(k_level_refit, dim3(256), dim3(256), 0, s, L, b.hist, nblk, b.prm, f.batched ? 0 : 1,
-                           index_base, b.nodes, b.nlast, b.nfirst, b.bmin, b.bmax);
+                       b.parent, b.nlevel, b.leaf_ids, b.leaves, b.ranc, n);
+    // child links over the nodes, and the refit over the leaves (at most n of
+    // each; both loops stride over the real counts)
+    const int gl = (int)std::min<int64_t>((max_nodes(n, T) + LR_THREADS - 1) / LR_THREADS, 1024);
+    const int gr = (int)std::min<int64_t>(refit_blocks(n), 512);
+    const int64_t bstride = refit_blocks(n);  // refit blocks of leaves, at most
+    hipLaunchKernelGGL(k_link_refit, dim3(gl + gr), dim3(LR_THREADS), 0, s, sk, b.prm, f.batched, index_base, gl,
+                       b.parent, b.nfirst, b.nlevel, b.ranc, n, b.leaves, b.nchild, b.nodes, b.nlast, b.bmin, b.bmax,
+                       b.rslots, bstride);
+    hipLaunchKernelGGL(k_refit_merge, dim3(OCT_MAX_LEVEL), dim3(RM_THREADS), 0, s, b.prm, f.batched, b.rslots,
+                       bstride, b.nodes, b.nlast, b.bmin, b.bmax);
     return hipGetLastError();
 }
 

@@ -68,7 +68,7 @@ struct SetState {
     DevBuf soff;                    // {0, n}: the one-set forest descriptor
     int64_t soff_n = -1;            // the n soff holds
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
-        nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax;
+        nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax, rslots, ranc;
     BuildBufs bufs() const {
         BuildBufs b;
         b.prm = prm.as<BuildParams>();
@@ -93,6 +93,8 @@ struct SetState {
         b.leaves = leaves.as<NodeRec>();
         b.bmin = bmin.as<float>();
         b.bmax = bmax.as<float>();
+        b.rslots = rslots.as<uint32_t>();
+        b.ranc = ranc.as<int32_t>();
         return b;
     }
     // The refit step (bounds, refit, topology: some 25 launches) as a HIP
@@ -116,7 +118,7 @@ struct SetState {
         drop_graphs();
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
                          &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
-                         &rk, &ri, &rsk, &rsi, &rpos, &soff, &cin};
+                         &rslots, &ranc, &rk, &ri, &rsk, &rsi, &rpos, &soff, &cin};
         for (DevBuf* d : all) d->release();
         soff_n = -1;
     }
@@ -482,6 +484,8 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
     HIP_TRY(c, s.leaves.ensure(sizeof(NodeRec) * (size_t)n));
     HIP_TRY(c, s.bmin.ensure(12 * (size_t)M));
     HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
+    HIP_TRY(c, s.rslots.ensure(4 * (size_t)refit_slot_words(n)));
+    HIP_TRY(c, s.ranc.ensure(4 * (size_t)OCT_MAX_LEVEL * (size_t)n));
     const BuildBufs b = s.bufs();
     // almost-sorted refit: the previous sort of the same n points is reused.
     // It runs without a host wait: the device decides whether it can
@@ -528,7 +532,8 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
             auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
             const DevBuf* bufs[] = {&s.prm, &s.partials, &s.keys0, &s.keys1, &s.vals0, &s.vals1, &s.hist, &s.scan_tmp,
                                     &s.total, &s.topo, &s.perm, &s.scoord, &s.nodes, &s.nfirst, &s.nlast, &s.nchild,
-                                    &s.parent, &s.nlevel, &s.leaf_ids, &s.leaves, &s.bmin, &s.bmax, &s.rk, &s.ri,
+                                    &s.parent, &s.nlevel, &s.leaf_ids, &s.leaves, &s.bmin, &s.bmax, &s.rslots,
+                                    &s.ranc, &s.rk, &s.ri,
                                     &s.rsk, &s.rsi, &s.rpos, &s.soff, &s.cin};
             for (const DevBuf* d : bufs) mix((uint64_t)(uintptr_t)d->p);
             uint32_t rbits = 0;
@@ -1438,6 +1443,8 @@ int batch_ensure(oct_ctx* c, BatchState& B) {
     HIP_TRY(c, s.leaves.ensure(sizeof(NodeRec) * (size_t)N));
     HIP_TRY(c, s.bmin.ensure(12 * (size_t)M));
     HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
+    HIP_TRY(c, s.rslots.ensure(4 * (size_t)refit_slot_words(N)));
+    HIP_TRY(c, s.ranc.ensure(4 * (size_t)OCT_MAX_LEVEL * (size_t)N));
     PairState& p = B.pair;
     HIP_TRY(c, p.flags.ensure(4 * BF_WORDS));
     HIP_TRY(c, p.lcount.ensure(4 * (size_t)N));

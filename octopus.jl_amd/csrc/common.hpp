@@ -151,6 +151,22 @@ __device__ inline int wave_incl_scan(int v) {
 }
 // sum across the wave (the same in every lane)
 __device__ inline int wave_sum(int v) { return __builtin_amdgcn_readlane(wave_incl_scan(v), OCT_WAVE - 1); }
+
+// Inclusive max-scan inside segments of consecutive lanes: lane i gets the
+// max over lanes seg .. i, where seg <= i is the first lane of its segment.
+// The steps of wave_incl_scan, each taken only where its source lane lies in
+// the segment (row_shr:s reads lane i - s, row_bcast:15 the lane before the
+// row, row_bcast:31 lane 31).  0 is the identity.  Every lane active.
+__device__ inline uint32_t seg_max_scan(uint32_t v, int lane, int seg) {
+    auto step = [&](uint32_t t, int src) { v = (src >= seg && t > v) ? t : v; };
+    step((uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true), lane - 1);
+    step((uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true), lane - 2);
+    step((uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true), lane - 4);
+    step((uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true), lane - 8);
+    step((uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false), (lane & ~15) - 1);
+    step((uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false), 31);
+    return v;
+}
 __device__ inline uint64_t wave_sum_u64(uint64_t v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, OCT_WAVE);
     return v;

@@ -163,6 +163,8 @@ struct BuildBufs {
     NodeRec* leaves;     // leaf records in Morton order
     uint8_t* nlevel;
     float *bmin, *bmax;
+    uint32_t* rslots;    // refit: the runs cut by a block's edges, refit_slot_words(n)
+    int32_t* ranc;       // refit: per level (OCT_MAX_LEVEL rows of n) each leaf's ancestor, -1 none
 };
 
 int bounds_blocks(int64_t n);
@@ -170,6 +172,7 @@ int64_t sort_tiles(int64_t n);
 int64_t topo_blocks(int64_t n);
 int64_t hist_words(int64_t n);
 int64_t max_nodes(int64_t n, int T);
+int64_t refit_slot_words(int64_t n);
 // Bounds, origin and key width of one set (per-block partials, then a fold);
 // with `copy`, the coordinates are copied there on the way.
 hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, float* copy, const BuildBufs& b,
