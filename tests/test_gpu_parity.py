@@ -76,6 +76,28 @@ def test_tree_parity(orc, case, leaf):
     assert t.n_leaves == int(np.sum(np.all(o.node_children == -1, axis=1)))
 
 
+def test_tree_parity_many_leaves(orc):
+    # 740k leaves at levels 12-21: the bounds refit strides over more leaf
+    # blocks than its grid holds, and the per-level merge of the runs cut at
+    # block edges goes through several chunks with the open run carried
+    import octohip
+
+    pts = np.concatenate([datasets.gaussian_blob(1_000_000, 31, 0.1, (0.25,) * 3),
+                          datasets.gaussian_blob(1_000_000, 32, 0.05, (-0.25,) * 3)])
+    r = 0.002
+    print(f"tree parity many leaves n={pts.shape[0]} r={r} leaf=8", flush=True)
+    tns, sid, _ = _build(pts, r, leaf=8)
+    t = octohip.get_tree(tns, sid)
+    o = orc.run(pts, r, leaf_size=8)
+    assert t.n_leaves > 600_000
+    assert t.n_nodes == o.n_nodes
+    assert np.array_equal(_np(t.node_first), o.node_first)
+    assert np.array_equal(_np(t.node_last), o.node_last)
+    assert np.array_equal(_np(t.node_children), o.node_children)
+    assert np.array_equal(_np(t.node_bounds_min), o.bmin)
+    assert np.array_equal(_np(t.node_bounds_max), o.bmax)
+
+
 def test_tree_index_base_one(orc):
     import octohip
 
