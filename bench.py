@@ -325,7 +325,7 @@ def bench_clustered(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
+        print(json.dumps({"error": "the benchmark measures one GPU (one process, one device); --gpus above 1 is not supported",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     if a.workload == "batched":

@@ -9,7 +9,7 @@
 # binding only needs a raw device pointer from them, so any device array type
 # that converts to Ptr{Float32} works (AMDGPU.jl's ROCArray does via
 # `pointer`).  Results come back as raw device pointers plus sizes; wrap them
-# with `unsafe_wrap(ROCArray, ...)` as in INTEGRATION.md.
+# with `unsafe_wrap(ROCArray, ptr, dims)` (the library keeps owning them).
 #
 module OctopusHIP
 
@@ -165,7 +165,7 @@ for_each_neighbor(f, t::TNS, q, s, i) =
 The HIP adjoint behind the `build_edges_diff` rrule
 (ext/OctopusChainRulesCoreExt.jl:137-192): device pointers in, grad (3, N)
 overwritten.  A ChainRulesCore `rrule` wraps this exactly as the reference's
-extension does; see INTEGRATION.md.
+extension does: forward through `build_edges`, pullback through this call.
 """
 function build_edges_diff_pullback!(t::TNS, n_points::Integer, e::OctEdges, d_disp, d_dist, radius::Real,
                                     grad)
@@ -176,6 +176,6 @@ function build_edges_diff_pullback!(t::TNS, n_points::Integer, e::OctEdges, d_di
     return grad
 end
 
-function build_edges_diff end  # the rrule lives in a ChainRulesCore extension (INTEGRATION.md)
+function build_edges_diff end  # the rrule lives in a ChainRulesCore package extension, not here
 
 end # module
