@@ -524,13 +524,14 @@ __device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
 // one level up at the same position or, for the first level, the last node
 // of the level above starting before k (the same ranks, minus one).
 // Writes node_first, the packed record (first child for internal nodes),
-// parent links and levels; leaves also get their range end and bounds, and
-// the nodes covering all n points (the chain down to the first split) get the
-// global bounds.  Child slots and counts come from k_link_refit, internal
-// bounds and range ends from k_link_refit and k_refit_merge, which find each
-// leaf's ancestors in ranc (per level, in leaf order).
+// parent links and levels; leaves also get their range end, and the nodes
+// covering all n points (the chain down to the first split) get the global
+// bounds.  Child slots and counts and the leaves' bounds come from
+// k_link_refit, internal bounds and range ends from k_link_refit and
+// k_refit_merge, which find each leaf's ancestors in ranc (per level, in leaf
+// order).
 __global__ void __launch_bounds__(TOPO_THREADS)
-k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, int64_t n, int T,
+k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
             const uint16_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
             const uint64_t* __restrict__ total, BuildParams* __restrict__ prm, Forest forest, int index_base,
             NodeRec* __restrict__ nodes, int32_t* __restrict__ nfirst, int32_t* __restrict__ nlast,
@@ -638,24 +639,9 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
                 e = lower_bound_pred(k + T + 1, n, [&](int64_t j) { return sk[j] != key; });
             const int32_t cnt = (int32_t)(e - k);
             nlast[id] = (int32_t)(e - 1) + index_base;
-            uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
-            uint32_t hi3[3] = {0u, 0u, 0u};
-            for (int64_t j0 = k; j0 < e; j0 += 4) {
-                float4 pp[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) pp[q] = scoord[j0 + q < e ? j0 + q : k];  // four loads in flight
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float v[3] = {pp[q].x, pp[q].y, pp[q].z};
-                    for (int d = 0; d < 3; ++d) {
-                        const uint32_t o = f2o(v[d]);
-                        lo3[d] = o < lo3[d] ? o : lo3[d];
-                        hi3[d] = o > hi3[d] ? o : hi3[d];
-                    }
-                }
-            }
-            rec.lo = make_float4(o2f(lo3[0]), o2f(lo3[1]), o2f(lo3[2]), __int_as_float((int)k));
-            rec.hi = make_float4(o2f(hi3[0]), o2f(hi3[1]), o2f(hi3[2]), __int_as_float(cnt));
+            // bounds from k_link_refit (one lane per leaf there, not one per position)
+            rec.lo = make_float4(0.f, 0.f, 0.f, __int_as_float((int)k));
+            rec.hi = make_float4(0.f, 0.f, 0.f, __int_as_float(cnt));
             const int32_t lr = before(OCT_NLEVELS) - M;
             // the leaf's ancestor at each level the refit visits (-1 from its
             // own level on): the last node of that level starting at or before k
@@ -675,7 +661,7 @@ k_topo_emit(const uint64_t* __restrict__ sk, const float4* __restrict__ scoord, 
             }
         }
         nodes[id] = rec;
-        if (L == lam || chain) {
+        if (L != lam && chain) {
             bmin[3 * (int64_t)id + 0] = rec.lo.x; bmin[3 * (int64_t)id + 1] = rec.lo.y; bmin[3 * (int64_t)id + 2] = rec.lo.z;
             bmax[3 * (int64_t)id + 0] = rec.hi.x; bmax[3 * (int64_t)id + 1] = rec.hi.y; bmax[3 * (int64_t)id + 2] = rec.hi.z;
         }
@@ -728,7 +714,8 @@ __global__ void __launch_bounds__(LR_THREADS)
 k_link_refit(const uint64_t* __restrict__ sk, const BuildParams* __restrict__ prm, int batched, int index_base,
              int link_blocks, const int32_t* __restrict__ parent, const int32_t* __restrict__ nfirst,
              const uint8_t* __restrict__ nlevel, const int32_t* __restrict__ ranc, int64_t astride,
-             const NodeRec* __restrict__ leaves, int32_t* __restrict__ nchild, NodeRec* __restrict__ nodes,
+             const int32_t* __restrict__ leaf_ids, const float4* __restrict__ scoord,
+             NodeRec* __restrict__ leaves, int32_t* __restrict__ nchild, NodeRec* __restrict__ nodes,
              int32_t* __restrict__ nlast, float* __restrict__ bmin, float* __restrict__ bmax,
              uint32_t* __restrict__ slots, int64_t bstride) {
     __shared__ int32_t s_anc[LR_WAVES][OCT_MAX_LEVEL][OCT_WAVE];  // the lanes' ancestors per level
@@ -776,11 +763,39 @@ k_link_refit(const uint64_t* __restrict__ sk, const BuildParams* __restrict__ pr
         }
         uint32_t v[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
         if (valid) {
-            const NodeRec r = leaves[lr];
-            const int32_t first = __float_as_int(r.lo.w);
-            v[0] = ~f2o(r.lo.x); v[1] = ~f2o(r.lo.y); v[2] = ~f2o(r.lo.z);
-            v[3] = f2o(r.hi.x);  v[4] = f2o(r.hi.y);  v[5] = f2o(r.hi.z);
-            v[6] = (uint32_t)(first + __float_as_int(r.hi.w) - 1 + index_base);
+            // the leaf's bounds: the exact min / max of its points, eight
+            // loads in flight (a short last group re-reads the first point)
+            const float4 rl = leaves[lr].lo, rh = leaves[lr].hi;
+            const int32_t id = leaf_ids[lr];
+            const int32_t first = __float_as_int(rl.w), cnt = __float_as_int(rh.w);
+            uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+            uint32_t hi3[3] = {0u, 0u, 0u};
+            for (int32_t j0 = 0; j0 < cnt; j0 += 8) {
+                float4 pp[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) pp[q] = scoord[(int64_t)first + (j0 + q < cnt ? j0 + q : 0)];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float c3[3] = {pp[q].x, pp[q].y, pp[q].z};
+                    for (int d = 0; d < 3; ++d) {
+                        const uint32_t o = f2o(c3[d]);
+                        lo3[d] = o < lo3[d] ? o : lo3[d];
+                        hi3[d] = o > hi3[d] ? o : hi3[d];
+                    }
+                }
+            }
+            float* lf = reinterpret_cast<float*>(leaves + lr);
+            float* nr = reinterpret_cast<float*>(nodes + id);
+            for (int d = 0; d < 3; ++d) {
+                const float lo = o2f(lo3[d]), hi = o2f(hi3[d]);
+                lf[d] = lo; lf[4 + d] = hi;
+                nr[d] = lo; nr[4 + d] = hi;
+                bmin[3 * (int64_t)id + d] = lo;
+                bmax[3 * (int64_t)id + d] = hi;
+                v[d] = ~lo3[d];
+                v[3 + d] = hi3[d];
+            }
+            v[6] = (uint32_t)(first + cnt - 1 + index_base);
         }
         __syncthreads();  // the last block's LDS reads are done
 #pragma unroll
@@ -1066,7 +1081,7 @@ hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base,
                        nblk);
     hipError_t e = exclusive_scan(b.hist, b.hist, (int64_t)(OCT_NLEVELS + 1) * nblk, b.scan_tmp, b.total, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, b.scoord, n, T, b.topo, b.hist,
+    hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, n, T, b.topo, b.hist,
                        nblk, b.total, b.prm, f, index_base, b.nodes, b.nfirst, b.nlast, b.nchild, b.bmin, b.bmax,
                        b.parent, b.nlevel, b.leaf_ids, b.leaves, b.ranc, n);
     // child links over the nodes, and the refit over the leaves (at most n of
@@ -1075,8 +1090,8 @@ hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base,
     const int gr = (int)std::min<int64_t>(refit_blocks(n), 512);
     const int64_t bstride = refit_blocks(n);  // refit blocks of leaves, at most
     hipLaunchKernelGGL(k_link_refit, dim3(gl + gr), dim3(LR_THREADS), 0, s, sk, b.prm, f.batched, index_base, gl,
-                       b.parent, b.nfirst, b.nlevel, b.ranc, n, b.leaves, b.nchild, b.nodes, b.nlast, b.bmin, b.bmax,
-                       b.rslots, bstride);
+                       b.parent, b.nfirst, b.nlevel, b.ranc, n, b.leaf_ids, b.scoord, b.leaves, b.nchild, b.nodes,
+                       b.nlast, b.bmin, b.bmax, b.rslots, bstride);
     hipLaunchKernelGGL(k_refit_merge, dim3(OCT_MAX_LEVEL), dim3(RM_THREADS), 0, s, b.prm, f.batched, b.rslots,
                        bstride, b.nodes, b.nlast, b.bmin, b.bmax);
     return hipGetLastError();
