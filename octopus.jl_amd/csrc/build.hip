@@ -100,11 +100,14 @@ k_bounds_partial(const float* __restrict__ coords, int64_t n, float* __restrict_
     }
 }
 
-// (also zeroes `zero[0 .. nz)`: the tile sums k_refit_keys adds up next)
+// (also zeroes `zero[0 .. nz)`: the tile sums k_refit_keys adds up next; and
+// with csrc, stores the coordinates' address there for the refit kernels,
+// which a replayed graph runs on whatever array the caller passed this time)
 __global__ void __launch_bounds__(BND_THREADS)
 k_bounds_final(const uint32_t* __restrict__ partials, int nblocks, float radius, BuildParams* __restrict__ prm,
-               uint64_t* __restrict__ zero, int64_t nz) {
+               uint64_t* __restrict__ zero, int64_t nz, const float* coords, const float** __restrict__ csrc) {
     for (int64_t i = threadIdx.x; i < nz; i += BND_THREADS) zero[i] = 0ull;
+    if (csrc && threadIdx.x == 0) *csrc = coords;
     bounds_fold(partials, 0, nblocks, radius, prm);
 }
 
@@ -268,9 +271,10 @@ __device__ inline bool key_idx_less(uint64_t ka, int32_t ia, uint64_t kb, int32_
 
 // K1: new keys in the previous order; vals0 = PI, vals1 = changed flag.
 __global__ void __launch_bounds__(RF_THREADS)
-k_refit_keys(const float* __restrict__ coords, int64_t n, const BuildParams* __restrict__ prm,
+k_refit_keys(const float* const* __restrict__ csrc, int64_t n, const BuildParams* __restrict__ prm,
              const uint64_t* __restrict__ pk, const float4* __restrict__ prev_sc, uint64_t* __restrict__ nk,
              int32_t* __restrict__ pi, uint32_t* __restrict__ chg, uint64_t* __restrict__ tile_sums) {
+    const float* __restrict__ coords = *csrc;  // this step's coordinates
     const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
     bool changed = false;
     if (i < n) {
@@ -377,11 +381,12 @@ __device__ inline void refit_write(int64_t pos, uint64_t key, int32_t idx, const
 
 // K5: unchanged points to their new positions; K6 (c < |C|): changed points.
 __global__ void __launch_bounds__(RF_THREADS)
-k_refit_place(const float* __restrict__ coords, int64_t n, const uint64_t* __restrict__ total,
+k_refit_place(const float* const* __restrict__ csrc, int64_t n, const uint64_t* __restrict__ total,
               const uint64_t* __restrict__ pk, const int32_t* __restrict__ pi, const uint32_t* __restrict__ cpos,
               const uint64_t* __restrict__ csk, const int32_t* __restrict__ csi, const uint32_t* __restrict__ cspos,
               uint64_t* __restrict__ kout, int32_t* __restrict__ perm, float4* __restrict__ sc, int index_base,
               const BuildParams* __restrict__ prm) {
+    const float* __restrict__ coords = *csrc;  // this step's coordinates
     const int64_t nc = (int64_t)*total;
     const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
     if (prm->refit_fail) {  // keep the previous order (perm and scoord stay as they are)
@@ -993,7 +998,7 @@ int64_t max_nodes(int64_t n, int T) { return n + 21 * (n / (T + 1)) + 22; }
 
 // Stage 1: bounds reduction (the caller reads prm back to learn key_bits).
 hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, float* copy, const BuildBufs& b,
-                               hipStream_t s) {
+                               hipStream_t s, const float** csrc) {
     const int nb = bounds_blocks(n);
     const bool vec = ((uintptr_t)coords & 15u) == 0 && (!copy || ((uintptr_t)copy & 15u) == 0);
     if (vec)
@@ -1001,7 +1006,7 @@ hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, flo
     else
         hipLaunchKernelGGL(k_bounds_partial<false>, dim3(nb), dim3(BND_THREADS), 0, s, coords, n, copy, b.partials);
     hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BND_THREADS), 0, s, b.partials, nb, radius, b.prm,
-                       reinterpret_cast<uint64_t*>(b.scan_tmp), scan_tiles(n));
+                       reinterpret_cast<uint64_t*>(b.scan_tmp), scan_tiles(n), coords, csrc);
     return hipGetLastError();
 }
 
@@ -1043,17 +1048,17 @@ hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int i
 // (keys[cur] and scoord hold the previous sort) and the changed count in
 // *b.total.  The caller reads the count and either finishes the refit or
 // runs the full sort.
-hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, const BuildBufs& b, hipStream_t s) {
+hipError_t launch_refit_prepare(const float* const* csrc, int64_t n, int cur, const BuildBufs& b, hipStream_t s) {
     const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
     // the tile sums in scan_tmp were zeroed by the bounds fold just before
-    hipLaunchKernelGGL(k_refit_keys, dim3(g), dim3(RF_THREADS), 0, s, coords, n, b.prm, b.keys[cur], b.scoord,
+    hipLaunchKernelGGL(k_refit_keys, dim3(g), dim3(RF_THREADS), 0, s, csrc, n, b.prm, b.keys[cur], b.scoord,
                        b.keys[cur ^ 1], b.vals[0], (uint32_t*)b.vals[1], reinterpret_cast<uint64_t*>(b.scan_tmp));
     return exclusive_scan_summed((const uint32_t*)b.vals[1], (uint32_t*)b.vals[1], n, b.scan_tmp, b.total, s);
 }
 
 // Part 2: the sorted keys end in keys[cur ^ 1].  Grids cover `limit` changed
 // points; the kernels read the real count (and refit_fail) on the device.
-hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t limit, int key_bits, int cur,
+hipError_t launch_refit_finish(const float* const* csrc, int64_t n, int64_t limit, int key_bits, int cur,
                                int index_base, const BuildBufs& b, const RefitBufs& r, hipStream_t s) {
     if (limit > REFIT_MAX_CHANGED) return hipErrorInvalidValue;  // the changed-point buffers hold that many
     const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
@@ -1065,7 +1070,7 @@ hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t limit, in
         hipLaunchKernelGGL(k_refit_rank, dim3(gc), dim3(RF_THREADS), 0, s, n, b.total, r.ck, r.ci, b.keys[cur],
                            b.vals[0], cpos, r.csk, r.csi, r.cspos, b.prm);
     }
-    hipLaunchKernelGGL(k_refit_place, dim3(g), dim3(RF_THREADS), 0, s, coords, n, b.total, b.keys[cur], b.vals[0],
+    hipLaunchKernelGGL(k_refit_place, dim3(g), dim3(RF_THREADS), 0, s, csrc, n, b.total, b.keys[cur], b.vals[0],
                        cpos, r.csk, r.csi, r.cspos, b.keys[cur ^ 1], b.perm, b.scoord, index_base, b.prm);
     return hipGetLastError();
 }

@@ -99,9 +99,10 @@ struct SetState {
     }
     // The refit step (bounds, refit, topology: some 25 launches) as a HIP
     // graph per parity of cur, replayed while its key (sizes, buffers,
-    // radius) holds.  It reads a copy of the caller's coordinates (cin), so
-    // the caller may pass a new array every step.
-    DevBuf cin;
+    // radius) holds.  It reads the caller's coordinates through csrc, a device
+    // word the bounds stage sets every step, so the caller may pass a new
+    // array every step.
+    DevBuf csrc;
     struct StepGraph {
         hipGraphExec_t exec = nullptr;
         uint64_t key = 0;
@@ -118,7 +119,7 @@ struct SetState {
         drop_graphs();
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
                          &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
-                         &rslots, &ranc, &rk, &ri, &rsk, &rsi, &rpos, &soff, &cin};
+                         &rslots, &ranc, &rk, &ri, &rsk, &rsi, &rpos, &soff, &csrc};
         for (DevBuf* d : all) d->release();
         soff_n = -1;
     }
@@ -513,19 +514,21 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
                           s.rpos.as<uint32_t>()};
         // the key width must stay s.key_bits (the levels the topology refits depend on it)
         // the graph starts after the bounds, which run eagerly on the caller's
-        // coordinates and copy them into cin (the graph reads that copy)
-        auto enqueue = [&](const float* coords) -> int {
+        // coordinates and leave their address in csrc (the graph reads it there)
+        HIP_TRY(c, s.csrc.ensure(sizeof(const float*)));
+        const float* const* csrc = s.csrc.as<const float*>();
+        auto enqueue = [&]() -> int {
             Phase ph(c, OCT_PROF_BUILD);
-            HIP_TRY(c, launch_refit_prepare(coords, n, s.cur, b, c->stream));
-            HIP_TRY(c, launch_refit_finish(coords, n, limit, s.key_bits, s.cur, c->index_base, b, r, c->stream));
+            HIP_TRY(c, launch_refit_prepare(csrc, n, s.cur, b, c->stream));
+            HIP_TRY(c, launch_refit_finish(csrc, n, limit, s.key_bits, s.cur, c->index_base, b, r, c->stream));
             HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, s.cur ^ 1, b, f, c->stream));
             return OCT_OK;
         };
         {
-            HIP_TRY(c, s.cin.ensure(12 * (size_t)n));
             {
                 Phase ph(c, OCT_PROF_BOUNDS);
-                HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, s.cin.as<float>(), b, c->stream));
+                HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, nullptr, b, c->stream,
+                                               s.csrc.as<const float*>()));
             }
             // the graph's key: everything its launches bake in
             uint64_t key = 1469598103934665603ull;
@@ -534,7 +537,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
                                     &s.total, &s.topo, &s.perm, &s.scoord, &s.nodes, &s.nfirst, &s.nlast, &s.nchild,
                                     &s.parent, &s.nlevel, &s.leaf_ids, &s.leaves, &s.bmin, &s.bmax, &s.rslots,
                                     &s.ranc, &s.rk, &s.ri,
-                                    &s.rsk, &s.rsi, &s.rpos, &s.soff, &s.cin};
+                                    &s.rsk, &s.rsi, &s.rpos, &s.soff, &s.csrc};
             for (const DevBuf* d : bufs) mix((uint64_t)(uintptr_t)d->p);
             uint32_t rbits = 0;
             memcpy(&rbits, &c->radius, 4);
@@ -548,7 +551,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
                 if (g.exec) (void)hipGraphExecDestroy(g.exec);
                 g.exec = nullptr;
                 g.key = 0;
-                HIP_TRY_RC(capture_graph(c, [&]() { return enqueue(s.cin.as<float>()); }, &g.exec));
+                HIP_TRY_RC(capture_graph(c, [&]() { return enqueue(); }, &g.exec));
                 g.key = key;
             }
             Phase ph(c, OCT_PROF_BUILD);  // refit and topology

@@ -174,9 +174,10 @@ int64_t hist_words(int64_t n);
 int64_t max_nodes(int64_t n, int T);
 int64_t refit_slot_words(int64_t n);
 // Bounds, origin and key width of one set (per-block partials, then a fold);
-// with `copy`, the coordinates are copied there on the way.
+// with `copy`, the coordinates are copied there on the way; with `csrc`, their
+// address is stored there (device memory) for the refit kernels.
 hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, float* copy, const BuildBufs& b,
-                               hipStream_t s);
+                               hipStream_t s, const float** csrc = nullptr);
 hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int index_base, const BuildBufs& b,
                              int* cur_out, hipStream_t s);
 constexpr int RADIX_BITS = 9;
@@ -202,8 +203,10 @@ struct RefitBufs {
     int32_t* csi;
     uint32_t* cspos; // their final positions
 };
-hipError_t launch_refit_prepare(const float* coords, int64_t n, int cur, const BuildBufs& b, hipStream_t s);
-hipError_t launch_refit_finish(const float* coords, int64_t n, int64_t limit, int key_bits, int cur,
+// The refit reads the coordinates through *csrc (device memory, written by
+// the bounds stage), so a captured refit follows the caller's array.
+hipError_t launch_refit_prepare(const float* const* csrc, int64_t n, int cur, const BuildBufs& b, hipStream_t s);
+hipError_t launch_refit_finish(const float* const* csrc, int64_t n, int64_t limit, int key_bits, int cur,
                                int index_base, const BuildBufs& b, const RefitBufs& r, hipStream_t s);
 
 // Edge outputs (see edges.hip).
