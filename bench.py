@@ -352,13 +352,7 @@ def main():
     dev = torch.device("cuda", 0)
     base = torch.from_numpy(pts).to(dev)
     total_steps = a.warmup + a.steps
-    g = torch.Generator(device=dev)
-    g.manual_seed(2)
-    snaps = [base]
-    for _ in range(total_steps):
-        snaps.append((snaps[-1].double() + torch.randn(base.shape, generator=g, device=dev, dtype=torch.float64)
-                      * (0.01 * h)).float().contiguous())
-    snaps = snaps[1:]
+    snaps = datasets.sph_snapshots(base, total_steps, h=h, seed=2)
     torch.cuda.synchronize()
 
     modes = ["learned", "exact"] if a.mode == "both" else [a.mode]

@@ -32,8 +32,12 @@ class Batch:
             raise ArgumentError("batch coords must be a float32 device tensor")
         if coords.dim() != 2 or coords.shape[1] != 3:
             raise DimensionMismatch("batch coords must have shape (N, 3)")
+        if not coords.is_contiguous():
+            # a private copy would freeze the coordinates: in-place updates of
+            # the caller's tensor would then never reach the batch
+            raise ArgumentError("batch coords must be contiguous (they are referenced, not copied)")
         self.tns = tns
-        self.coords = coords.contiguous()
+        self.coords = coords
         self.offsets = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
         self.radii = np.ascontiguousarray(np.asarray(radii, dtype=np.float32))
         if self.offsets.shape[0] != self.radii.shape[0] + 1 or int(self.offsets[-1]) != coords.shape[0]:

@@ -86,6 +86,25 @@ def sph_step(coords: np.ndarray, step: int, h: float = 0.01, seed: int = 2) -> n
     return (coords.astype(np.float64) + d).astype(np.float32)
 
 
+def sph_snapshots(base, steps: int, h: float = 0.01, seed: int = 2):
+    """The SPH block's time steps on the device of `base` (an (N, 3) float32
+    torch tensor): snapshot k is snapshot k-1 plus N(0, (0.01h)^2) per
+    coordinate, drawn in float64 by a torch generator seeded with `seed` on
+    that device and rounded to float32.  Returns `steps` contiguous tensors
+    (the first is one step past `base`)."""
+    import torch
+
+    g = torch.Generator(device=base.device)
+    g.manual_seed(seed)
+    snaps = []
+    cur = base
+    for _ in range(steps):
+        cur = (cur.double() + torch.randn(base.shape, generator=g, device=base.device, dtype=torch.float64)
+               * (0.01 * h)).float().contiguous()
+        snaps.append(cur)
+    return snaps
+
+
 def gaussian_mixture(n: int, seed: int, n_comp: int, sigma_lo: float, sigma_hi: float,
                      stream: int = 3) -> np.ndarray:
     centers = uniform(seed, stream, 3 * n_comp).astype(np.float64).reshape(n_comp, 3)

@@ -92,6 +92,12 @@ def lib():
         L.orc_build_edges.argtypes = [ctypes.POINTER(_Tree), _f32p, ctypes.c_int64, _f32p,
                                       ctypes.c_int, ctypes.c_int]
         L.orc_edges_free.argtypes = [ctypes.POINTER(_Edges)]
+        L.orc_count_edges.restype = ctypes.c_int64
+        L.orc_count_edges.argtypes = [ctypes.POINTER(_Tree), _f32p, ctypes.c_int64, _f32p, ctypes.c_int,
+                                      ctypes.c_int, _i32p]
+        L.orc_build_edges_rows.restype = ctypes.POINTER(_Edges)
+        L.orc_build_edges_rows.argtypes = [ctypes.POINTER(_Tree), _f32p, _i64p, ctypes.c_int64, _f32p,
+                                           ctypes.c_int, ctypes.c_int]
         L.orc_pullback.argtypes = [ctypes.c_int64, ctypes.c_int64, _i32p, _i32p, _f32p, _f32p,
                                    _f32p, _f32p, ctypes.c_float, _f32p]
         L.orc_max_threads.restype = ctypes.c_int
@@ -234,6 +240,45 @@ def build_edges(tree: Tree, coords_q, coords_t=None, exclude_self=None, nthreads
     nt = lib().orc_max_threads() if nthreads is None else int(nthreads)
     h = lib().orc_build_edges(tree._handle, _ptr(cq, _f32p), cq.shape[0], _ptr(ct, _f32p),
                               int(bool(exclude_self)), nt)
+    e = h.contents
+    ne, nq = int(e.n_edges), int(e.nq)
+    out = Edges(
+        offsets=_arr(e.offsets, nq + 1, np.int64),
+        senders=_arr(e.senders, ne, np.int32), receivers=_arr(e.receivers, ne, np.int32),
+        rel_displacement=_arr(e.rdisp, 3 * ne, np.float32, (ne, 3)),
+        rel_dist_norm=_arr(e.rdist, ne, np.float32),
+    )
+    lib().orc_edges_free(h)
+    return out
+
+
+def count_edges(tree: Tree, coords_q, coords_t=None, exclude_self=None, nthreads=None):
+    """Pass A of build_edges alone (src/cpu/edges_cpu.jl:36-61): per-query
+    counts (int32) and their total, without storing any edge."""
+    cq = _coords(coords_q)
+    ct = tree._coords if coords_t is None else _coords(coords_t)
+    if exclude_self is None:
+        exclude_self = coords_t is None
+    nt = lib().orc_max_threads() if nthreads is None else int(nthreads)
+    counts = np.zeros(cq.shape[0], np.int32)
+    total = lib().orc_count_edges(tree._handle, _ptr(cq, _f32p), cq.shape[0], _ptr(ct, _f32p),
+                                  int(bool(exclude_self)), nt, _ptr(counts, _i32p))
+    return counts, int(total)
+
+
+def build_edges_rows(tree: Tree, coords_q, rows, coords_t=None, exclude_self=None, nthreads=None) -> Edges:
+    """Pass B of build_edges (src/cpu/edges_cpu.jl:92-130) for the queries in
+    `rows` only, in that order; receivers are the queries' own indices."""
+    cq = _coords(coords_q)
+    ct = tree._coords if coords_t is None else _coords(coords_t)
+    if exclude_self is None:
+        exclude_self = coords_t is None
+    rw = np.ascontiguousarray(rows, np.int64)
+    if rw.size and (rw.min() < 0 or rw.max() >= cq.shape[0]):
+        raise ValueError("rows out of range")
+    nt = lib().orc_max_threads() if nthreads is None else int(nthreads)
+    h = lib().orc_build_edges_rows(tree._handle, _ptr(cq, _f32p), _ptr(rw, _i64p), rw.shape[0],
+                                   _ptr(ct, _f32p), int(bool(exclude_self)), nt)
     e = h.contents
     ne, nq = int(e.n_edges), int(e.nq)
     out = Edges(

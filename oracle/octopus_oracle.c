@@ -468,6 +468,72 @@ void orc_edges_free(orc_edges* e) {
     free(e);
 }
 
+/* Pass A of build_edges alone (src/cpu/edges_cpu.jl:36-61,
+ * _count_edges_range!): counts[i] for every query i and the total.  For
+ * edge sets too large to hold on the host (the 4M-point clustered workload
+ * has ~6e8 edges); dynamic chunks of 1024 queries, since per-query work is
+ * uneven there (the counts do not depend on the schedule). */
+int64_t orc_count_edges(const orc_tree* t, const float* coords_q, int64_t nq, const float* coords_t,
+                        int exclude_self, int nthreads, int32_t* counts) {
+    const float r = t->radius;
+    const float r_sq = r * r;
+    if (nthreads < 1) nthreads = 1;
+    const int64_t nblk = (nq + 1023) / 1024;
+    int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : total) if (nblk > 1)
+    for (int64_t b = 0; b < nblk; ++b) {
+        const int64_t hi = (b + 1) * 1024 < nq ? (b + 1) * 1024 : nq;
+        for (int64_t i = b * 1024; i < hi; ++i) {
+            count_ctx k = {(int32_t)i, exclude_self, 0};
+            const float* p = coords_q + 3 * i;
+            traverse(t, coords_t, p[0], p[1], p[2], r_sq, visit_count, &k);
+            counts[i] = k.c;
+            total += k.c;
+        }
+    }
+    return total;
+}
+
+/* Pass B of build_edges (src/cpu/edges_cpu.jl:92-130, _fill_edges_range!)
+ * restricted to the queries rows[0 .. nrows): their edges in that order,
+ * each query's edges in DFS visit order, receivers = the query's own index
+ * (so the self pair still excludes j == rows[k]).  A sample of receivers of
+ * an edge set too large to fill whole. */
+orc_edges* orc_build_edges_rows(const orc_tree* t, const float* coords_q, const int64_t* rows,
+                                int64_t nrows, const float* coords_t, int exclude_self, int nthreads) {
+    orc_edges* e = (orc_edges*)calloc(1, sizeof(orc_edges));
+    e->nq = nrows;
+    e->offsets = (int64_t*)calloc((size_t)nrows + 1, sizeof(int64_t));
+    const float r = t->radius;
+    const float r_sq = r * r;
+    const float inv_r = 1.0f / r;
+    if (nthreads < 1) nthreads = 1;
+    int32_t* counts = (int32_t*)calloc((size_t)nrows + 1, sizeof(int32_t));
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int64_t k = 0; k < nrows; ++k) {
+        count_ctx c = {(int32_t)rows[k], exclude_self, 0};
+        const float* p = coords_q + 3 * rows[k];
+        traverse(t, coords_t, p[0], p[1], p[2], r_sq, visit_count, &c);
+        counts[k] = c.c;
+    }
+    for (int64_t k = 0; k < nrows; ++k) e->offsets[k + 1] = e->offsets[k] + counts[k];
+    free(counts);
+    const int64_t ne = e->offsets[nrows];
+    e->n_edges = ne;
+    e->senders = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ne > 0 ? ne : 1));
+    e->receivers = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ne > 0 ? ne : 1));
+    e->rdisp = (float*)malloc(sizeof(float) * 3 * (size_t)(ne > 0 ? ne : 1));
+    e->rdist = (float*)malloc(sizeof(float) * (size_t)(ne > 0 ? ne : 1));
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int64_t k = 0; k < nrows; ++k) {
+        fill_ctx f = {(int32_t)rows[k], exclude_self, e->offsets[k], inv_r,
+                      e->senders, e->receivers, e->rdisp, e->rdist};
+        const float* p = coords_q + 3 * rows[k];
+        traverse(t, coords_t, p[0], p[1], p[2], r_sq, visit_fill, &f);
+    }
+    return e;
+}
+
 /* --------------------------------------------------- build_edges_diff rrule */
 
 /* ext/OctopusChainRulesCoreExt.jl:137-192 (CPU pullback): a serial loop over

@@ -67,6 +67,10 @@ int64_t orc_for_each_neighbor(const orc_tree* t, const float* coords_q, const fl
 orc_edges* orc_build_edges(const orc_tree* t, const float* coords_q, int64_t nq,
                            const float* coords_t, int exclude_self, int nthreads);
 void orc_edges_free(orc_edges* e);
+int64_t orc_count_edges(const orc_tree* t, const float* coords_q, int64_t nq, const float* coords_t,
+                        int exclude_self, int nthreads, int32_t* counts);
+orc_edges* orc_build_edges_rows(const orc_tree* t, const float* coords_q, const int64_t* rows,
+                                int64_t nrows, const float* coords_t, int exclude_self, int nthreads);
 
 void orc_pullback(int64_t n_points, int64_t n_edges, const int32_t* senders,
                   const int32_t* receivers, const float* rdisp, const float* rdist,

@@ -228,6 +228,36 @@ def test_edges_chunked_matches_serial(orc):
     assert np.array_equal(e1.rel_displacement, e4.rel_displacement)
 
 
+def test_count_and_rows_match_brute_force(orc):
+    """The count-only pass and the selected-rows fill (the checkers of the
+    4M-point workload) against the brute force, self pair and cross pair."""
+    pts = datasets.uniform_cube(3000, 13)
+    r = 0.07
+    t = orc.run(pts, r)
+    counts, total = orc.count_edges(t, pts, nthreads=3)
+    want = brute_edges(pts, pts, r, exclude_self=True)
+    assert total == want[0].shape[0]
+    assert np.array_equal(counts, np.bincount(want[1], minlength=pts.shape[0]))
+    rows = np.array([2999, 0, 17, 17, 1500], np.int64)  # any order, repeats allowed
+    e = orc.build_edges_rows(t, pts, rows, nthreads=2)
+    assert np.array_equal(np.diff(e.offsets), counts[rows])
+    for k, i in enumerate(rows):
+        a, b = int(e.offsets[k]), int(e.offsets[k + 1])
+        assert np.all(e.receivers[a:b] == i)
+        m = want[1] == i
+        got = sort_edges(e.senders[a:b], e.receivers[a:b], e.rel_displacement[a:b], e.rel_dist_norm[a:b])
+        ref = (want[0][m], want[1][m], want[2][m], want[3][m])
+        for x, y in zip(got, ref):
+            assert np.array_equal(x, y)
+    q = datasets.uniform_cube(400, 14)
+    cq, cross_total = orc.count_edges(t, q, pts)
+    wq = brute_edges(q, pts, r, exclude_self=False)
+    assert cross_total == wq[0].shape[0]
+    assert np.array_equal(cq, np.bincount(wq[1], minlength=q.shape[0]))
+    with pytest.raises(ValueError):
+        orc.build_edges_rows(t, pts, np.array([3000], np.int64))
+
+
 def test_boundary_exact_r(orc):
     for key in ("boundary_exact_r_included", "boundary_past_r_excluded"):
         g = GOLD[key]

@@ -67,6 +67,69 @@ def test_oracle_forward_special_cases():
     assert np.all(g[:-1] == 0)
 
 
+def test_dir_codes_27_placements():
+    """Every one of the 27 placements of a unit box around the unit query box
+    gets its own code, x weighing 1, y 3 and z 9, below = 0 / level = 1 /
+    above = 2 per axis; a centre offset of exactly a quarter of the two
+    extents is still level, one step more is not."""
+    f = np.float32
+    qlo, qhi = np.zeros(3, f), np.ones(3, f)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                off = np.array([2 * dx, 2 * dy, 2 * dz], f)
+                code = int(om.dir_codes(qlo, qhi, qlo + off, qhi + off))
+                assert code == (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1), (dx, dy, dz, code)
+    # threshold: tau = 0.25 * (1 + 1) = 0.5 on each axis
+    for shift, want in ((0.5, 1), (0.5625, 2), (-0.5, 1), (-0.5625, 0)):
+        clo = qlo + np.array([shift, 0, 0], f)
+        assert int(om.dir_codes(qlo, qhi, clo, clo + np.ones(3, f))) == want + 3 + 9, shift
+        clo = qlo + np.array([0, 0, shift], f)
+        assert int(om.dir_codes(qlo, qhi, clo, clo + np.ones(3, f))) == 1 + 3 + 9 * want, shift
+
+
+class _FakeTree:
+    """A hand-made tree: root 0 with four leaves (the attributes
+    leaf_features reads)."""
+
+    def __init__(self):
+        self.node_children = np.full((5, 8), -1, np.int32)
+        self.node_children[0, :4] = [1, 2, 3, 4]
+        self.node_first = np.array([0, 0, 2, 5, 6], np.int32)
+        self.node_last = np.array([9, 1, 4, 5, 9], np.int32)
+        self.node_level = np.array([0, 1, 2, 3, 2], np.int32)
+        self.bmin = np.array([[-1.5, -1.5, 0], [0, 0, 0], [2, 0, 0], [0, 0, 3], [-1.5, -1.5, 0]], np.float32)
+        self.bmax = np.array([[2.5, 1, 3], [1, 1, 1], [2.5, 0.5, 0.5], [0, 0, 3], [-0.5, -0.5, 1]], np.float32)
+
+
+def test_leaf_features_hand_worked():
+    """The 29 features of each leaf of a hand-made four-leaf tree (r = 1),
+    worked out by hand from the definitions: [0] log1p(points), [1] level / 21,
+    [2] max(log2(max(extent / r, 1e-30)), -20) with extent the largest box
+    side, [3 + k] log1p(points of the candidate leaves in direction k, the 27
+    codes without 13 = the leaf's own cell).  Candidates are leaves whose box
+    gap is <= r (L1-L2 gap exactly 1: included).
+      L1 box [0,1]^3, 2 points, level 1: sees L2 at code 14 (slot 13, 3 points)
+         and L4 at code 9 (slot 9, 4 points); L3 is 2 away in z.
+      L2 box [2,2.5]x[0,.5]^2, 3 points, level 2, extent .5: sees L1 at code 12
+         (slot 12, 2 points).
+      L3 a single point (0,0,3), level 3, extent 0: sees nothing.
+      L4 box [-1.5,-.5]^2 x [0,1], 4 points, level 2: sees L1 at code 17
+         (slot 16, 2 points)."""
+    want = np.zeros((4, 29), np.float64)
+    want[0, :3] = [math.log(3.0), 1 / 21, 0.0]
+    want[0, 3 + 9] = math.log(5.0)
+    want[0, 3 + 13] = math.log(4.0)
+    want[1, :3] = [math.log(4.0), 2 / 21, -1.0]
+    want[1, 3 + 12] = math.log(3.0)
+    want[2, :3] = [math.log(2.0), 3 / 21, -20.0]
+    want[3, :3] = [math.log(5.0), 2 / 21, 0.0]
+    want[3, 3 + 16] = math.log(3.0)
+    t = _FakeTree()
+    got = om.leaf_features(t, t, 1.0)
+    np.testing.assert_allclose(got, want, rtol=2e-7, atol=0)
+
+
 def test_oracle_candidates_cover_every_pair(orc):
     # every pair within r lies in a candidate leaf of the query's leaf, and the
     # direction code of a leaf seen from itself is 13
