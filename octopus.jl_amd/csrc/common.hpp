@@ -33,6 +33,12 @@ __host__ __device__ inline float o2f(uint32_t o) {
 __device__ inline float jmin(float a, float b) { return f2o(b) < f2o(a) ? b : a; }
 __device__ inline float jmax(float a, float b) { return f2o(b) > f2o(a) ? b : a; }
 
+// One rel_displacement component as the CPU path writes it
+// (src/cpu/edges_cpu.jl:111-113): the traversal's difference is target minus
+// query (src/cpu/iterator_cpu.jl:96) and the fill negates it.  Equal to
+// (q - t) * inv_r in every bit except when t == q, where it is -0.0.
+__device__ inline float rel_disp(float q, float t, float inv_r) { return -(t - q) * inv_r; }
+
 // ------------------------------------------------------------------ Morton
 
 // Magic-bits spread of the low 21 bits (src/morton.jl:13-23).

@@ -460,16 +460,16 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
                                 const uint32_t s = e - blk_base;
                                 L.sen[s] = j + index_base;
                                 L.rcv[s] = i + index_base;
-                                L.disp[3 * s + 0] = dx * inv_r;
-                                L.disp[3 * s + 1] = dy * inv_r;
-                                L.disp[3 * s + 2] = dz * inv_r;
+                                L.disp[3 * s + 0] = rel_disp(qx, tp[u].x, inv_r);
+                                L.disp[3 * s + 1] = rel_disp(qy, tp[u].y, inv_r);
+                                L.disp[3 * s + 2] = rel_disp(qz, tp[u].z, inv_r);
                                 L.dist[s] = sqrtf(d2) * inv_r;
                             } else {
                                 out.senders[e] = j + index_base;
                                 out.receivers[e] = i + index_base;
-                                out.rdisp[3 * (int64_t)e + 0] = dx * inv_r;
-                                out.rdisp[3 * (int64_t)e + 1] = dy * inv_r;
-                                out.rdisp[3 * (int64_t)e + 2] = dz * inv_r;
+                                out.rdisp[3 * (int64_t)e + 0] = rel_disp(qx, tp[u].x, inv_r);
+                                out.rdisp[3 * (int64_t)e + 1] = rel_disp(qy, tp[u].y, inv_r);
+                                out.rdisp[3 * (int64_t)e + 2] = rel_disp(qz, tp[u].z, inv_r);
                                 out.rdist[e] = sqrtf(d2) * inv_r;
                             }
                         }
@@ -672,7 +672,9 @@ __device__ inline void emit_hits(const uint32_t* hits, uint32_t n, uint32_t base
             const uint32_t e = base + t;
             out.senders[e] = __float_as_int(p[k].w) + index_base;
             out.receivers[e] = __float_as_int(q[k].w) + index_base;
-            *reinterpret_cast<float3*>(out.rdisp + 3 * (int64_t)e) = make_float3(dx * inv_r, dy * inv_r, dz * inv_r);
+            *reinterpret_cast<float3*>(out.rdisp + 3 * (int64_t)e) =
+                make_float3(rel_disp(q[k].x, p[k].x, inv_r), rel_disp(q[k].y, p[k].y, inv_r),
+                            rel_disp(q[k].z, p[k].z, inv_r));
             out.rdist[e] = sqrtf(d2) * inv_r;
         }
     }

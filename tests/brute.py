@@ -10,7 +10,9 @@ import numpy as np
 
 def brute_edges(cq, ct, r, exclude_self):
     """All (receiver i, sender j) with d2 <= r*r, sorted by (i, j); features
-    rdisp = (q_i - t_j) * inv_r and rdist = sqrt(d2) * inv_r in FP32."""
+    rdisp = -(t_j - q_i) * inv_r and rdist = sqrt(d2) * inv_r in FP32, the
+    expressions of the CPU fill (src/cpu/edges_cpu.jl:111-114; the negated
+    difference makes a zero component -0.0)."""
     cq = np.ascontiguousarray(cq, np.float32)
     ct = np.ascontiguousarray(ct, np.float32)
     r = np.float32(r)
@@ -18,7 +20,7 @@ def brute_edges(cq, ct, r, exclude_self):
     inv_r = np.float32(1.0) / r
     S, R, D, N = [], [], [], []
     for i in range(cq.shape[0]):
-        d = cq[i] - ct  # (nt, 3) FP32
+        d = ct - cq[i]  # (nt, 3) FP32, target minus query (src/cpu/iterator_cpu.jl:96)
         d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
         m = d2 <= r2
         if exclude_self and i < ct.shape[0]:
@@ -26,13 +28,20 @@ def brute_edges(cq, ct, r, exclude_self):
         j = np.nonzero(m)[0].astype(np.int32)
         S.append(j)
         R.append(np.full(j.shape[0], i, np.int32))
-        D.append(d[j] * inv_r)
+        D.append(-d[j] * inv_r)
         N.append(np.sqrt(d2[j]) * inv_r)
     if not S:
         return (np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros((0, 3), np.float32),
                 np.zeros(0, np.float32))
     return (np.concatenate(S), np.concatenate(R), np.concatenate(D).astype(np.float32),
             np.concatenate(N).astype(np.float32))
+
+
+def bits(a):
+    """The bits of a float32 array as int32 (array_equal on these tells -0.0
+    from +0.0); other arrays unchanged."""
+    a = np.asarray(a)
+    return np.ascontiguousarray(a).view(np.int32) if a.dtype == np.float32 else a
 
 
 def sort_edges(senders, receivers, rdisp, rdist):

@@ -12,7 +12,7 @@ import os
 import numpy as np
 import pytest
 
-from brute import brute_edges, sort_edges
+from brute import bits, brute_edges, sort_edges
 from octohip import datasets
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")))
@@ -194,8 +194,8 @@ def _check_edges(orc, cq, r, ct=None):
     assert np.array_equal(got[0], want[0])
     assert np.array_equal(got[1], want[1])
     # features bit-exact (stronger than the reference's atol=1f-5)
-    assert np.array_equal(got[2], want[2])
-    assert np.array_equal(got[3], want[3])
+    assert np.array_equal(bits(got[2]), bits(want[2]))
+    assert np.array_equal(bits(got[3]), bits(want[3]))
     assert np.all(got[3] <= np.float32(1.0) + np.float32(1e-5))
     return e
 
@@ -225,7 +225,7 @@ def test_edges_chunked_matches_serial(orc):
     e1 = orc.build_edges(t, pts, nthreads=1)
     e4 = orc.build_edges(t, pts, nthreads=4)
     assert np.array_equal(e1.senders, e4.senders)
-    assert np.array_equal(e1.rel_displacement, e4.rel_displacement)
+    assert np.array_equal(bits(e1.rel_displacement), bits(e4.rel_displacement))
 
 
 def test_count_and_rows_match_brute_force(orc):
@@ -248,7 +248,7 @@ def test_count_and_rows_match_brute_force(orc):
         got = sort_edges(e.senders[a:b], e.receivers[a:b], e.rel_displacement[a:b], e.rel_dist_norm[a:b])
         ref = (want[0][m], want[1][m], want[2][m], want[3][m])
         for x, y in zip(got, ref):
-            assert np.array_equal(x, y)
+            assert np.array_equal(bits(x), bits(y))
     q = datasets.uniform_cube(400, 14)
     cq, cross_total = orc.count_edges(t, q, pts)
     wq = brute_edges(q, pts, r, exclude_self=False)
@@ -365,3 +365,18 @@ def test_pullback_zero_tangents(orc):
     e = orc.build_edges(t, pts)
     g = orc.pullback(30, e.senders, e.receivers, e.rel_displacement, e.rel_dist_norm, None, None, 0.2)
     assert np.all(g == 0)
+
+
+def test_zero_displacement_component_is_negative_zero(orc):
+    """The CPU fill writes -dx * inv_r with dx = target - query
+    (src/cpu/edges_cpu.jl:111-113, src/cpu/iterator_cpu.jl:96): a component
+    where the two points agree is -(+0.0) * inv_r = -0.0, sign bit set."""
+    pts = np.array([[0.25, 0.0, 0.5], [0.25, 0.5, 0.5]], np.float32)
+    e = orc.build_edges(orc.run(pts, 1.0), pts)
+    assert e.n_edges == 2
+    d = e.rel_displacement.view(np.int32)
+    assert np.all(d[:, 0] == np.int32(-2**31)) and np.all(d[:, 2] == np.int32(-2**31))
+    assert np.all(e.rel_displacement[:, 1] == np.float32([-0.5, 0.5])[e.receivers])
+    b = brute_edges(pts, pts, 1.0, exclude_self=True)
+    assert np.array_equal(bits(sort_edges(*b)[2]), bits(sort_edges(e.senders, e.receivers, e.rel_displacement,
+                                                                     e.rel_dist_norm)[2]))
