@@ -186,6 +186,22 @@ int oct_profile_read(oct_ctx* ctx, int reset, double* ms, int64_t* calls);
 /* Block until all work queued on the context's stream is done. */
 int oct_synchronize(oct_ctx* ctx);
 
+/* Buffer lifetime.  Arrays the library hands out (oct_edges, oct_tree,
+ * oct_view, oct_edge_stats, oct_batch_edges) hold the last call's results
+ * until the next call that rewrites them, as the reference's EdgeBuffer and
+ * tree arrays do (test/contracts.jl:223-240).  When a later call needs a
+ * larger buffer, the old allocation is not freed but kept, with its stale
+ * contents, on the context's retire list, so a view of it stays readable
+ * memory (a replaced Julia array stays valid likewise); buffers grow by at
+ * least 1.5x, so the list holds at most about twice the live size.
+ * oct_trim frees the list (after waiting for the context's stream): views
+ * taken before a buffer was replaced must not be read after it.
+ * oct_destroy frees everything.  No counterpart in the reference, whose
+ * garbage collector frees unreachable arrays. */
+int oct_trim(oct_ctx* ctx);
+/* Allocations on the retire list. */
+int oct_retired_count(const oct_ctx* ctx, int64_t* count);
+
 /* ---- Learned per-leaf hit-count predictor ------------------------------
  * No counterpart in the reference (whose GPU build_edges always runs the
  * exact count pass, ext/OctopusCUDAExt.jl:373-387): a 29 -> 64 -> 64 -> 1
@@ -204,8 +220,8 @@ int oct_synchronize(oct_ctx* ctx);
 /* When OCT_EDGES_LEARNED runs the predictor: by default on a pair's first
  * call and again after a call that needed a second attempt, fell back to the
  * exact path or recounted more than 1/64 of its leaves (otherwise the call
- * reuses the buffers and fixed runs of 4 leaves per wave); always != 0: on
- * every call. */
+ * reuses the buffers and fixed runs of oct_set_run_leaves leaves per wave,
+ * 6 by default); always != 0: on every call. */
 int oct_set_predictor_policy(oct_ctx* ctx, int always);
 /* Install a parameter block from host memory (NULL removes the predictor). */
 int oct_predictor_load(oct_ctx* ctx, const float* host_params);

@@ -19,10 +19,15 @@
 namespace oct {
 
 // Grow-only device buffer (reused across calls, like ensure_capacity!,
-// src/util.jl:12-17).
+// src/util.jl:12-17).  A buffer the C-ABI hands out (edges, tree arrays,
+// device views, statistics) has `retire` set: when it grows, the old
+// allocation goes to the context's retire list instead of being freed, so a
+// view of it taken earlier stays readable memory (stale contents, as a
+// replaced Julia array stays valid) until oct_trim or oct_destroy.
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    std::vector<void*>* retire = nullptr;
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap && p) return hipSuccess;
         // grow by at least 1.5x (and 1/8 headroom, 64 KiB granules) so sizes
@@ -30,8 +35,12 @@ struct DevBuf {
         size_t want = std::max(bytes + bytes / 8, cap + cap / 2);
         want = (want + 65535) & ~(size_t)65535;
         if (p) {
-            hipError_t e = hipFree(p);
-            if (e != hipSuccess) return e;
+            if (retire) {
+                retire->push_back(p);
+            } else {
+                hipError_t e = hipFree(p);
+                if (e != hipSuccess) return e;
+            }
             p = nullptr;
             cap = 0;
         }
@@ -69,6 +78,11 @@ struct SetState {
     int64_t soff_n = -1;            // the n soff holds
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
         nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax, rslots, ranc;
+    // the buffers oct_get_tree and oct_device_view hand out
+    void export_to(std::vector<void*>* r) {
+        DevBuf* out[] = {&perm, &nfirst, &nlast, &nchild, &bmin, &bmax, &nodes, &scoord};
+        for (DevBuf* d : out) d->retire = r;
+    }
     BuildBufs bufs() const {
         BuildBufs b;
         b.prm = prm.as<BuildParams>();
@@ -148,6 +162,11 @@ struct PairState {
     int32_t* heavy_list() { return heavy.as<int32_t>(); }
     uint8_t* heavy_flags(int64_t n_leaves) { return (uint8_t*)(heavy.as<int32_t>() + n_leaves); }
     oct_edge_stats st{};
+    // the buffers oct_build_edges, oct_batch_run and oct_edge_stats_get hand out
+    void export_to(std::vector<void*>* r) {
+        DevBuf* out[] = {&senders, &receivers, &rdisp, &rdist, &feat, &pred, &lhits};
+        for (DevBuf* d : out) d->retire = r;
+    }
     void release_all() {
         DevBuf* all[] = {&senders, &receivers, &rdisp, &rdist, &lists, &lcount, &qcount, &qoff, &scan_tmp, &flags,
                          &tiles, &ldir, &feat, &pred, &lhits, &pwork, &pscan, &wave_first, &heavy, &eprm};
@@ -230,6 +249,7 @@ struct oct_ctx {
     std::vector<SetState> sets;
     std::vector<PairState> pairs;
     std::vector<BatchState> batches;
+    std::vector<void*> retired;  // grown-out allocations of handed-out buffers (DevBuf::retire)
     Predictor mlp;
     std::string err;
     void* pinned = nullptr;  // small host staging area for read-backs
@@ -635,6 +655,7 @@ int oct_destroy(oct_ctx* c) {
     for (auto& p : c->pairs) p.release_all();
     for (auto& b : c->batches) b.release_all();
     c->mlp.release_all();
+    for (void* p : c->retired) (void)hipFree(p);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->sig) (void)hipHostFree(c->sig);
     for (auto& m : c->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
@@ -708,6 +729,7 @@ int oct_add_point_set(oct_ctx* c, const float* coords, int64_t n, int* set_id) {
     if (n < 0) return fail(c, OCT_ERR_DIMENSION, "point count must be >= 0");
     if (n > 0 && !coords) return fail(c, OCT_ERR_ARGUMENT, "null coordinates");
     c->sets.emplace_back();
+    c->sets.back().export_to(&c->retired);
     c->sets.back().coords = coords;
     c->sets.back().n = n;
     *set_id = (int)c->sets.size() - 1 + c->index_base;
@@ -741,6 +763,7 @@ int oct_set_active_search(oct_ctx* c, int q, int t) {
     if (!valid_set(c, q) || !valid_set(c, t)) return fail(c, OCT_ERR_ARGUMENT, "unknown point set");
     if (find_pair(c, q, t)) return OCT_OK;  // idempotent (src/api.jl:152)
     c->pairs.emplace_back();
+    c->pairs.back().export_to(&c->retired);
     c->pairs.back().q = q;
     c->pairs.back().t = t;
     return OCT_OK;
@@ -1215,6 +1238,22 @@ int oct_synchronize(oct_ctx* c) {
     return OCT_OK;
 }
 
+int oct_trim(oct_ctx* c) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (c->retired.empty()) return OCT_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));  // no queued kernel still reads them
+    for (void* p : c->retired) HIP_TRY(c, hipFree(p));
+    c->retired.clear();
+    return OCT_OK;
+}
+
+int oct_retired_count(const oct_ctx* c, int64_t* count) {
+    if (!c || !count) return OCT_ERR_ARGUMENT;
+    *count = (int64_t)c->retired.size();
+    return OCT_OK;
+}
+
 // ------------------------------------------------------------ predictor
 
 namespace {
@@ -1619,6 +1658,8 @@ int oct_batch_create(oct_ctx* c, int n_sets, const float* coords, const int64_t*
     HIP_TRY(c, hipSetDevice(c->device));
     c->batches.emplace_back();
     BatchState& B = c->batches.back();
+    B.pair.export_to(&c->retired);
+    B.eoff.retire = &c->retired;
     B.live = true;
     B.S = n_sets;
     B.N = N;

@@ -24,7 +24,7 @@ __all__ = [
     "TNS", "ArgumentError", "DimensionMismatch", "OctopusError",
     "set_search_radius", "set_refit_mode", "set_leaf_size", "add_point_set", "update_point_set",
     "resize_point_set", "set_active_search", "run", "build_edges", "build_edges_diff",
-    "device_view", "for_each_neighbor", "count_neighbors_device", "get_tree",
+    "device_view", "for_each_neighbor", "count_neighbors_device", "get_tree", "trim", "retired_count",
 ]
 
 
@@ -55,16 +55,20 @@ def _torch():
 
 
 class _DeviceArray:
-    """Zero-copy handle on library-owned device memory for torch.as_tensor."""
+    """Zero-copy handle on library-owned device memory for torch.as_tensor.
+    The tensor keeps this handle alive, and the handle keeps `owner` (the TNS
+    or batch whose context owns the memory) alive, so the context outlives
+    every view of its buffers."""
 
-    def __init__(self, ptr, shape, typestr):
+    def __init__(self, ptr, shape, typestr, owner):
+        self.owner = owner
         self.__cuda_array_interface__ = {
             "shape": tuple(shape), "typestr": typestr, "data": (int(ptr), False),
             "version": 2, "strides": None,
         }
 
 
-def _wrap(ptr, shape, dtype, device):
+def _wrap(ptr, shape, dtype, device, owner):
     torch = _torch()
     n = 1
     for s in shape:
@@ -72,7 +76,7 @@ def _wrap(ptr, shape, dtype, device):
     if n == 0 or not ptr:
         return torch.empty(shape, dtype=dtype, device=device)
     typestr = {torch.int32: "<i4", torch.float32: "<f4", torch.int64: "<i8"}[dtype]
-    return torch.as_tensor(_DeviceArray(ptr, shape, typestr), device=device)
+    return torch.as_tensor(_DeviceArray(ptr, shape, typestr, owner), device=device)
 
 
 class TNS:
@@ -313,12 +317,12 @@ def get_tree(tns: TNS, set_id: int) -> Tree:
     tns._check(tns._lib.oct_get_tree(tns._c(), int(set_id), ctypes.byref(t)))
     m, n, dev = int(t.n_nodes), int(t.n_points), tns.device
     return Tree(
-        node_bounds_min=_wrap(t.node_bounds_min, (m, 3), torch.float32, dev),
-        node_bounds_max=_wrap(t.node_bounds_max, (m, 3), torch.float32, dev),
-        node_first=_wrap(t.node_first, (m,), torch.int32, dev),
-        node_last=_wrap(t.node_last, (m,), torch.int32, dev),
-        node_children=_wrap(t.node_children, (m, 8), torch.int32, dev),
-        permutation=_wrap(t.permutation, (n,), torch.int32, dev),
+        node_bounds_min=_wrap(t.node_bounds_min, (m, 3), torch.float32, dev, tns),
+        node_bounds_max=_wrap(t.node_bounds_max, (m, 3), torch.float32, dev, tns),
+        node_first=_wrap(t.node_first, (m,), torch.int32, dev, tns),
+        node_last=_wrap(t.node_last, (m,), torch.int32, dev, tns),
+        node_children=_wrap(t.node_children, (m, 8), torch.int32, dev, tns),
+        permutation=_wrap(t.permutation, (n,), torch.int32, dev, tns),
         n_nodes=m, n_leaves=int(t.n_leaves), origin=tuple(t.origin),
     )
 
@@ -352,10 +356,10 @@ def build_edges(tns: TNS, q: int, t: int, mode: str = "exact", order: str = "lea
     tns._leave()
     n, dev = int(e.n_edges), tns.device
     return Edges(
-        senders=_wrap(e.senders, (n,), torch.int32, dev),
-        receivers=_wrap(e.receivers, (n,), torch.int32, dev),
-        rel_displacement=_wrap(e.rel_displacement, (n, 3), torch.float32, dev),
-        rel_dist_norm=_wrap(e.rel_dist_norm, (n, 1), torch.float32, dev),
+        senders=_wrap(e.senders, (n,), torch.int32, dev, tns),
+        receivers=_wrap(e.receivers, (n,), torch.int32, dev, tns),
+        rel_displacement=_wrap(e.rel_displacement, (n, 3), torch.float32, dev, tns),
+        rel_dist_norm=_wrap(e.rel_dist_norm, (n, 1), torch.float32, dev, tns),
     )
 
 
@@ -449,6 +453,23 @@ def profile_read(tns: TNS, reset: bool = True) -> dict:
 
 def synchronize(tns: TNS):
     tns._check(tns._lib.oct_synchronize(tns._c()))
+
+
+def trim(tns: TNS) -> int:
+    """Frees the allocations that later, larger results replaced (they stay
+    allocated, stale, so that tensors taken from them remain readable
+    memory); returns how many were freed.  Tensors taken before their buffer
+    was replaced must not be read afterwards."""
+    n = retired_count(tns)
+    tns._check(tns._lib.oct_trim(tns._c()))
+    return n
+
+
+def retired_count(tns: TNS) -> int:
+    """Allocations kept on the context's retire list."""
+    n = ctypes.c_int64()
+    tns._check(tns._lib.oct_retired_count(tns._c(), ctypes.byref(n)))
+    return int(n.value)
 
 
 def for_each_neighbor(f, tns: TNS, q: int, t: int, i: int):

@@ -147,6 +147,8 @@ SIGNATURES = {
     "oct_profile_read": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int64)]),
     "oct_synchronize": (ctypes.c_int, [_P]),
+    "oct_trim": (ctypes.c_int, [_P]),
+    "oct_retired_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "oct_build_info_get": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(OctBuildInfo)]),
     "oct_set_predictor_policy": (ctypes.c_int, [_P, ctypes.c_int]),
     "oct_predictor_load": (ctypes.c_int, [_P, _P]),

@@ -65,11 +65,11 @@ class Batch:
         self.tns._leave()
         n, dev = int(out.n_edges), self.tns.device
         return BatchEdges(
-            senders=_wrap(out.senders, (n,), torch.int32, dev),
-            receivers=_wrap(out.receivers, (n,), torch.int32, dev),
-            rel_displacement=_wrap(out.rel_displacement, (n, 3), torch.float32, dev),
-            rel_dist_norm=_wrap(out.rel_dist_norm, (n, 1), torch.float32, dev),
-            set_offsets=_wrap(out.set_offsets, (self.n_sets + 1,), torch.int64, dev),
+            senders=_wrap(out.senders, (n,), torch.int32, dev, self),
+            receivers=_wrap(out.receivers, (n,), torch.int32, dev, self),
+            rel_displacement=_wrap(out.rel_displacement, (n, 3), torch.float32, dev, self),
+            rel_dist_norm=_wrap(out.rel_dist_norm, (n, 1), torch.float32, dev, self),
+            set_offsets=_wrap(out.set_offsets, (self.n_sets + 1,), torch.int64, dev, self),
             replayed=bool(out.replayed),
             recounted_leaves=int(out.recounted_leaves),
             recorded_leaves=int(out.recorded_leaves),

@@ -99,9 +99,9 @@ def edge_stats(tns, q: int, t: int, tensors: bool = True, wait: bool = True) -> 
     tns._check(get(tns._c(), int(q), int(t), ctypes.byref(s)))
     n, dev = int(s.n_leaves), tns.device
     return {
-        "features": _wrap(s.features, (n, FEATURES), torch.float32, dev) if tensors and s.features else None,
-        "predicted": _wrap(s.predicted, (n,), torch.float32, dev) if tensors and s.predicted else None,
-        "exact": _wrap(s.exact, (n,), torch.int32, dev) if tensors and s.exact and n else None,
+        "features": _wrap(s.features, (n, FEATURES), torch.float32, dev, tns) if tensors and s.features else None,
+        "predicted": _wrap(s.predicted, (n,), torch.float32, dev, tns) if tensors and s.predicted else None,
+        "exact": _wrap(s.exact, (n,), torch.int32, dev, tns) if tensors and s.exact and n else None,
         "n_leaves": n,
         "n_edges": int(s.n_edges),
         "recounted_leaves": int(s.recounted_leaves),

@@ -91,3 +91,12 @@ def test_sph_lattice_shape():
     assert pts.shape == (1000, 3) and np.float32(r) == np.float32(0.02)
     g = np.round(pts / 0.01)
     assert np.all(np.abs(pts - g * 0.01) <= 0.001 + 1e-7)
+
+
+def test_device_views_hold_their_owner():
+    """A wrapped result keeps the TNS (or batch) whose context owns its
+    memory alive: the handle torch keeps for the tensor refers to the owner."""
+    owner = object()
+    h = octohip._DeviceArray(0x1000, (4,), "<i4", owner)
+    assert h.owner is owner
+    assert h.__cuda_array_interface__["data"] == (0x1000, False)

@@ -563,3 +563,38 @@ def test_unaligned_coordinates(orc):
         cur = (cur.astype(np.float64) + rng.normal(0, 1e-4, cur.shape)).astype(np.float32)
         c.copy_(torch.from_numpy(cur).cuda())
         octohip.update_point_set(tns, sid, c)
+
+
+def test_views_survive_growth_and_their_tns():
+    """A result tensor taken before a larger result replaced its buffer keeps
+    its values (the old allocation is retired, not freed, until trim), and a
+    result outlives the Python TNS it came from."""
+    import gc
+
+    import octohip
+
+    pts = datasets.uniform_cube(4000, 61)
+    tns, sid, c = _build(pts, 0.05)
+    e1 = octohip.build_edges(tns, sid, sid)
+    v1 = [_np(x).copy() for x in e1]
+    p1 = e1.senders.data_ptr()
+    octohip.set_search_radius(tns, 0.12)  # ~14x the edges: every output buffer grows
+    octohip.run(tns)
+    e2 = octohip.build_edges(tns, sid, sid)
+    assert e2.senders.shape[0] > 8 * e1.senders.shape[0]
+    assert e2.senders.data_ptr() != p1
+    assert octohip.retired_count(tns) >= 4
+    for a, b in zip(e1, v1):  # the retired allocation still holds the first result
+        assert np.array_equal(bits(_np(a)), bits(b))
+    del e1
+    assert octohip.trim(tns) >= 4
+    assert octohip.retired_count(tns) == 0
+
+    def make():
+        t, s, cc = _build(pts, 0.05)
+        return octohip.build_edges(t, s, s), cc
+
+    e3, _ = make()
+    gc.collect()  # the TNS is only reachable through e3's tensors
+    for a, b in zip(e3, v1):
+        assert np.array_equal(bits(_np(a)), bits(b))
