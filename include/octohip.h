@@ -330,6 +330,19 @@ int oct_mlp_forward(oct_ctx* ctx, const float* params, const float* features, in
 int oct_mlp_gradient(oct_ctx* ctx, const float* params, const float* features, const float* targets, int n,
                      float* grad, float* loss);
 
+/* No counterpart in the reference.  oct_mlp_gradient for callers that cannot
+ * allocate (code under HIP graph capture, tight training loops, harnesses
+ * that forbid hipMalloc): it calls no allocating routine and no host wait;
+ * its row records and partial sums live in a fixed device array of the
+ * library (~10 MB on each GPU it runs on, held until the process ends), so
+ * n is at most OCT_MLP_STATIC_ROWS (more: OCT_ERR_CAPACITY).  grad and loss
+ * (loss may be NULL) are written in order on the context's stream.  Every
+ * call on one GPU uses that GPU's one array: calls that overlap in time on
+ * one GPU (other streams or contexts) race. */
+#define OCT_MLP_STATIC_ROWS 8192
+int oct_mlp_gradient_noalloc(oct_ctx* ctx, const float* params, const float* features, const float* targets, int n,
+                             float* grad, float* loss);
+
 #ifdef __cplusplus
 }
 #endif

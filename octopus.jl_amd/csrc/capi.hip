@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -232,6 +233,31 @@ struct BatchState {
 }  // namespace oct
 
 using namespace oct;
+
+// Fixed device scratch of oct_mlp_gradient_noalloc (the entry point that
+// allocates nothing): the row records and partial sums of one gradient over
+// at most OCT_MLP_STATIC_ROWS rows, ~10 MB on each GPU the library runs on,
+// held until the process ends.  Host code never passes `mlp_noalloc_scratch`
+// itself to a kernel (on the host it names a stand-in, not device memory):
+// oct_create takes each device's own address of it with hipGetSymbolAddress,
+// with that device current, and keeps it in noalloc_scratch_ptr[device].
+constexpr int64_t MLP_NOALLOC_FLOATS = mlp_scratch_floats_c(OCT_MLP_STATIC_ROWS);
+static __device__ __align__(256) float mlp_noalloc_scratch[MLP_NOALLOC_FLOATS];
+constexpr int OCT_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static float* noalloc_scratch_ptr[OCT_MAX_DEVICES] = {nullptr};
+
+// The current device's address of the fixed scratch (device is current).
+static hipError_t noalloc_scratch_setup(int device) {
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (noalloc_scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(mlp_noalloc_scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    noalloc_scratch_ptr[device] = (float*)p;
+    return hipSuccess;
+}
 
 struct oct_ctx {
     int device = 0;
@@ -618,10 +644,12 @@ int oct_create(oct_ctx** out, int device, int ndims, int index_base) {
     *out = nullptr;
     if (ndims != 3) return OCT_ERR_ARGUMENT;
     if (index_base != 0 && index_base != 1) return OCT_ERR_ARGUMENT;
+    if (device < 0 || device >= OCT_MAX_DEVICES) return OCT_ERR_ARGUMENT;
     oct_ctx* c = new oct_ctx();
     c->device = device;
     c->index_base = index_base;
     if (hipSetDevice(device) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
+    if (noalloc_scratch_setup(device) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
     c->own_stream = true;
     if (hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
@@ -1861,6 +1889,22 @@ int oct_autotune(oct_ctx* c, int q, int t, int mode, const int* leaf_sizes, int 
     if (rc == OCT_OK) rc = oct_set_run_chunk(c, chunks[bm]);
     if (rc != OCT_OK) return rc;
     if (best) *best = best_k;
+    return OCT_OK;
+}
+
+int oct_mlp_gradient_noalloc(oct_ctx* c, const float* params, const float* X, const float* T, int n, float* grad,
+                             float* loss) {
+    if (!c || !params || !X || !T || !grad || n < 1) return OCT_ERR_ARGUMENT;
+    if (n > OCT_MLP_STATIC_ROWS)
+        return fail(c, OCT_ERR_CAPACITY, "oct_mlp_gradient_noalloc: at most OCT_MLP_STATIC_ROWS (8192) rows");
+    HIP_TRY(c, hipSetDevice(c->device));
+    float* scratch = noalloc_scratch_ptr[c->device];  // set by oct_create
+    if (!scratch) return fail(c, OCT_ERR_STATE, "fixed scratch not set up for this device");
+    // every word the kernels read is written first in the same call (row
+    // records for rows < n, partial sums for chunks < ceil(n / 256)), so
+    // nothing needs clearing; stream-ordered, no host wait
+    HIP_TRY(c, launch_mlp_step(X, T, n, n, 0, const_cast<float*>(params), nullptr, nullptr, 1, 0.f, scratch, grad,
+                               loss, c->stream));
     return OCT_OK;
 }
 

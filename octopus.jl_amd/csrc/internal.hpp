@@ -309,6 +309,13 @@ hipError_t launch_leaf_features(const TreeView& q, const int2* lists, const uint
                                 int cap, float r, float* feat, hipStream_t s);
 hipError_t launch_mlp_infer(const float* X, int64_t n, const float* P, float* y, float* hits, hipStream_t s);
 int64_t mlp_scratch_floats(int batch);
+// The same as a compile-time constant: one row record (normalised inputs,
+// both layers' activations and deltas, the output delta and the loss) per
+// row, then the trained parameters' partial sums per chunk of rows.
+constexpr int64_t MLP_ROW_FLOATS = MLP_IN + 4 * MLP_H + 2;
+constexpr int64_t mlp_scratch_floats_c(int64_t batch) {
+    return batch * MLP_ROW_FLOATS + ((batch + MLP_ROWS_PER_CHUNK - 1) / MLP_ROWS_PER_CHUNK) * MLP_TRAINED;
+}
 // One training step on `batch` rows (seed 0: rows 0..batch-1, else hashed
 // rows); lr 0 computes the gradient (grad_out) without updating.
 hipError_t launch_mlp_step(const float* X, const float* T, int64_t n_data, int batch, uint64_t seed, float* P,

@@ -154,6 +154,7 @@ constexpr int TR_D2 = TR_H2 + MLP_H;
 constexpr int TR_G = TR_D2 + MLP_H;
 constexpr int TR_LOSS = TR_G + 1;
 constexpr int TR_ROW = TR_LOSS + 1;
+static_assert(TR_ROW == MLP_ROW_FLOATS, "row record size is shared with the fixed scratch of capi.hip");
 
 __device__ inline uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
@@ -381,10 +382,7 @@ hipError_t launch_mlp_infer(const float* X, int64_t n, const float* P, float* y,
     return hipGetLastError();
 }
 
-int64_t mlp_scratch_floats(int batch) {
-    const int chunks = (batch + MLP_ROWS_PER_CHUNK - 1) / MLP_ROWS_PER_CHUNK;
-    return (int64_t)batch * TR_ROW + (int64_t)chunks * MLP_TRAINED;
-}
+int64_t mlp_scratch_floats(int batch) { return mlp_scratch_floats_c(batch); }
 
 hipError_t launch_mlp_step(const float* X, const float* T, int64_t n_data, int batch, uint64_t seed, float* P,
                            float* m, float* v, int step, float lr, float* scratch, float* grad_out, float* loss_out,

@@ -172,6 +172,7 @@ SIGNATURES = {
     "oct_stream_join": (ctypes.c_int, [_P, _P, ctypes.c_int]),
     "oct_mlp_forward": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
     "oct_mlp_gradient": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int, _P, _P]),
+    "oct_mlp_gradient_noalloc": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int, _P, _P]),
 }
 
 PROF_PHASES = ["bounds", "build", "leaf_lists", "count", "scan", "fill", "backward", "single", "predict"]
