@@ -7,13 +7,17 @@ generated on the device before the timed region), then update_point_set +
 run! (refit mode: the almost-sorted path of time stepping) + build_edges on
 the self pair.  The metric is edges/s of build_edges with a bit-exact edge
 set, learned single pass (the headline `value`) against the exact two-pass
-path, both measured in the same process on the same snapshots.
+path and against the single pass without the predictor, all measured in the
+same process on the same snapshots.  The same line carries compact results
+of the other configurations (clustered 4M, differentiable 262k, batched 256
+sets) and the CPU baseline (the C oracle on this machine's host cores).
 
 Prints ONE JSON line.
 """
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -33,11 +37,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100, help="timed steps (the SPH configuration's 100 time steps)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="both", choices=["both", "exact", "learned"],
-                    help="both: learned (headline value) and exact, in that order")
-    ap.add_argument("--cpu-steps", type=int, default=30, help="steps of the CPU oracle baseline (0 = skip)")
+    ap.add_argument("--mode", default="all", choices=["all", "both", "exact", "learned", "learned_no_predictor"],
+                    help="all: learned (headline value), learned without the predictor, exact; both: learned "
+                         "and exact")
+    ap.add_argument("--cpu-runs", type=int, default=7,
+                    help="timed runs of the CPU oracle baseline after one warm-up (median reported; 0 = skip)")
     ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M points)")
     ap.add_argument("--no-profile", action="store_true", help="time without the per-phase HIP events")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="sph: skip the compact clustered / differentiable / batched results")
     ap.add_argument("--private-stream", action="store_true",
                     help="run the library on its own stream, joined to torch's current stream per call "
                          "(default: on torch's current stream)")
@@ -50,9 +58,10 @@ def parse():
                     help="full sort every step (default: refit mode, the almost-sorted path of time stepping)")
     ap.add_argument("--predict-always", action="store_true",
                     help="run the predictor on every step (default: first call and after overflows / recounts)")
-    ap.add_argument("--workload", default="sph", choices=["sph", "batched", "clustered"],
+    ap.add_argument("--workload", default="sph", choices=["sph", "batched", "clustered", "diff"],
                     help="sph: the headline 1M-point SPH block; batched: 256 uneven sets in one replayed graph; "
-                         "clustered: 4M clustered points (the learned predictor's own distribution)")
+                         "clustered: 4M clustered points (the learned predictor's own distribution); diff: "
+                         "build_edges_diff forward + HIP backward at 262,144 points")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched: keep --leaf-size / --run-leaves / --run-chunk")
     ap.add_argument("--leaf-size", type=int, default=32)
@@ -61,8 +70,64 @@ def parse():
     return ap.parse_args()
 
 
+def host_info():
+    """CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return model, nproc
+
+
+def cpu_baseline(host_snaps, r, runs, what):
+    """The C oracle (restating the reference CPU path: parallel binning,
+    serial sort / build / refit, two-pass edges over query chunks) on the
+    host: one warm-up, then `runs` runs on successive snapshots, run! and
+    build_edges timed separately; medians."""
+    import oracle
+
+    oracle.build()
+    threads = oracle.max_threads()
+    model, nproc = host_info()
+    run_ms, be_ms, edges = [], [], []
+    n = host_snaps[0].shape[0]
+    for k in range(runs + 1):
+        hp = host_snaps[k % len(host_snaps)]
+        t0 = time.perf_counter()
+        t = oracle.run(hp, float(r))
+        t1 = time.perf_counter()
+        e = oracle.build_edges(t, hp)
+        t2 = time.perf_counter()
+        if k == 0:
+            continue
+        run_ms.append(1e3 * (t1 - t0))
+        be_ms.append(1e3 * (t2 - t1))
+        edges.append(e.n_edges)
+        del e, t
+    mr, mb = statistics.median(run_ms), statistics.median(be_ms)
+    me = statistics.median(edges)
+    return {"value": me / ((mr + mb) / 1e3), "unit": "edges/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "nproc": nproc, "runs": runs,
+            "run_ms_median": mr, "build_edges_ms_median": mb,
+            "build_edges_edges_per_s": me / (mb / 1e3), "points_per_s": n / ((mr + mb) / 1e3),
+            "sample": f"median of {runs} runs after one warm-up, each one step of {what} (C oracle: run! then "
+                      f"build_edges, OpenMP {threads} threads)"}
+
+
 def run_mode(a, mode, snaps, r):
-    """Times a.steps steps of one build_edges mode on a fresh TNS."""
+    """Times a.steps steps of one build_edges mode on a fresh TNS.  mode
+    learned_no_predictor is the learned mode with no predictor installed
+    (the first call takes the exact path, later calls the single pass with
+    fixed runs)."""
     import torch
 
     import octohip
@@ -72,6 +137,7 @@ def run_mode(a, mode, snaps, r):
     octohip.set_refit_mode(tns, not a.no_refit)
     sid = octohip.add_point_set(tns, snaps[0])
     octohip.set_active_search(tns, sid, sid)
+    api_mode = "exact" if mode == "exact" else "learned"
     predictor = None
     if mode == "learned" and a.predictor != "none":
         if os.path.exists(os.path.join(octohip.predictor.SHIPPED, a.predictor + ".f32")):
@@ -81,27 +147,28 @@ def run_mode(a, mode, snaps, r):
                 octohip.predictor.set_policy(tns, True)
     used = [0]
     refits = [0]
+    single = [0]
 
     def step(k):
         octohip.update_point_set(tns, sid, snaps[k])
         octohip.run(tns)
-        e = octohip.build_edges(tns, sid, sid, mode=mode)
+        e = octohip.build_edges(tns, sid, sid, mode=api_mode)
         # (after build_edges: the refit's outcome is back on the host by then)
         refits[0] += int(octohip.build_info(tns, sid)["refit_used"])
-        if predictor:
-            # (known when the call returns: no wait for the stream)
-            used[0] += int(octohip.predictor.edge_stats(tns, sid, sid, tensors=False, wait=False)["used_predictor"])
+        # (known when the call returns: no wait for the stream)
+        st = octohip.predictor.edge_stats(tns, sid, sid, tensors=False, wait=False)
+        used[0] += int(st["used_predictor"])
+        single[0] += int(st["single_pass"])
         return e.senders.shape[0]
 
     step(0)
     first_call = octohip.predictor.edge_stats(tns, sid, sid)
     for k in range(1, a.warmup):
         step(k)
-    used[0] = 0
-    refits[0] = 0
+    used[0] = refits[0] = single[0] = 0
     # the timed loop times only the dominant kernel's phase with events (two
     # per step); the full per-phase breakdown comes from a separate pass
-    kphase = "single" if mode == "learned" else "fill"
+    kphase = "fill" if mode == "exact" else "single"
     torch.cuda.synchronize()
     if not a.no_profile:
         octohip.profile(tns, True, phases=[kphase])
@@ -114,7 +181,8 @@ def run_mode(a, mode, snaps, r):
     t1 = time.perf_counter()
     st = octohip.predictor.edge_stats(tns, sid, sid)
     kprof = octohip.profile_read(tns, reset=True)[kphase] if not a.no_profile else (0.0, 0)
-    used_timed, refits_timed = used[0], refits[0]
+    used_timed, refits_timed, single_timed = used[0], refits[0], single[0]
+    n_nodes = octohip.get_tree(tns, sid).n_nodes
     prof = {}
     if not a.no_profile:
         octohip.profile(tns, True)
@@ -130,24 +198,27 @@ def run_mode(a, mode, snaps, r):
         "value": edges / sec,
         "ms_per_step": 1e3 * sec / a.steps,
         "edges_per_step": edges / a.steps,
+        "n_nodes": n_nodes,
         "breakdown_ms_per_step": {k: v[0] / min(a.steps, 10) for k, v in prof.items() if v[1]},
         "timed_kernel": {"phase": kphase, "ms_total": kprof[0], "launches": kprof[1]},
         "predictor": predictor,
         "first_call": {k: first_call[k] for k in ("single_pass", "used_predictor", "attempts")},
+        "timed_steps_single_pass": single_timed,
         "timed_steps_using_predictor": used_timed,
         "timed_steps_refit": refits_timed,
         "last_step": {k: st[k] for k in ("single_pass", "used_predictor", "attempts", "recounted_leaves",
-                                         "recorded_leaves", "predicted_total", "n_edges", "n_leaves")},
+                                         "recorded_leaves", "heavy_leaves", "predicted_total", "n_edges",
+                                         "n_leaves")},
     }
     del tns
     return res
 
 
-def bench_batched(a):
+def batched(a, autotune, steps, with_baselines):
     """256 independent 3D sets, sizes log-uniform in [1000, 65536], each a
     4-component Gaussian mixture with its radius set for ~20 neighbours; one
     step = in-place coordinate update + run! + build_edges of every set as one
-    replayed graph.  Beside it: the same sets one at a time through the
+    replayed graph.  With baselines: the same sets one at a time through the
     single-set API (exact mode), and the C oracle on the host."""
     import numpy as np
     import torch
@@ -162,16 +233,20 @@ def bench_batched(a):
     base = torch.from_numpy(np.concatenate(sets)).to(dev)
     g = torch.Generator(device=dev)
     g.manual_seed(3)
-    total_steps = a.warmup + a.steps
+    total_steps = a.warmup + steps
     snaps = [(base.double() + torch.randn(base.shape, generator=g, device=dev, dtype=torch.float64) * 1e-4)
              .float().contiguous() for _ in range(total_steps)]
     coords = snaps[0].clone()
     tns = octohip.TNS()
     b = octohip.Batch(tns, coords, off, radii)
-    b.config(a.leaf_size, a.run_leaves, True, a.run_chunk)
-    tuned = None
-    if not a.no_autotune:
+    if autotune:
+        b.config(a.leaf_size, a.run_leaves, True, a.run_chunk)
         tuned = b.autotune(reps=3)
+        cfg = tuned["best"]
+    else:
+        tuned = None
+        cfg = (32, 4, 1)  # the autotuner's usual pick at this configuration
+        b.config(cfg[0], cfg[1], True, cfg[2])
 
     last = {}
 
@@ -180,6 +255,7 @@ def bench_batched(a):
         res = b.run()
         last["recounted_leaves"] = res.recounted_leaves
         last["recorded_leaves"] = res.recorded_leaves
+        last["heavy_leaves"] = res.heavy_leaves
         return res.senders.shape[0], res.replayed
 
     for k in range(a.warmup):
@@ -193,61 +269,56 @@ def bench_batched(a):
         replays += int(rp)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
-    # the same sets one at a time through the single-set API (one TNS per
-    # set, since a TNS has one radius), after one untimed warm-up step each
-    host_sets = [snaps[-1][off[i]:off[i + 1]].contiguous() for i in range(len(sets))]
-    seq = []
-    for i, s in enumerate(host_sets):
-        t = octohip.TNS()
-        octohip.set_search_radius(t, float(radii[i]))
-        sid = octohip.add_point_set(t, s)
-        octohip.set_active_search(t, sid, sid)
-        octohip.run(t)
-        octohip.build_edges(t, sid, sid)
-        seq.append((t, sid))
-    seq_edges = 0
+    out = {"value": edges / sec, "unit": "edges/s", "ms_per_step": 1e3 * sec / steps, "steps": steps,
+           "sets": len(sets), "points": n, "edges_per_step": edges / steps,
+           "leaf_size": cfg[0], "run_leaves": cfg[1], "run_chunk": cfg[2], "autotuned": bool(tuned),
+           "timed_replays": replays, "sets_per_s": len(sets) * steps / sec, "last_step": dict(last)}
+    if tuned:
+        out["autotune_ms_per_step"] = {f"leaf{k[0]}_run{k[1]}_chunk{k[2]}": v for k, v in tuned["ms"].items()}
+    if with_baselines:
+        # the same sets one at a time through the single-set API (one TNS per
+        # set, since a TNS has one radius), after one untimed warm-up step each
+        host_sets = [snaps[-1][off[i]:off[i + 1]].contiguous() for i in range(len(sets))]
+        seq = []
+        for i, s in enumerate(host_sets):
+            t = octohip.TNS()
+            octohip.set_search_radius(t, float(radii[i]))
+            sid = octohip.add_point_set(t, s)
+            octohip.set_active_search(t, sid, sid)
+            octohip.run(t)
+            octohip.build_edges(t, sid, sid)
+            seq.append((t, sid))
+        seq_edges = 0
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        for i, (t, sid) in enumerate(seq):
+            octohip.update_point_set(t, sid, host_sets[i])
+            octohip.run(t)
+            seq_edges += octohip.build_edges(t, sid, sid).senders.shape[0]
+        torch.cuda.synchronize()
+        seq_sec = time.perf_counter() - s0
+        out["sequential_single_set_api"] = {"ms_per_step": 1e3 * seq_sec, "edges_per_s": seq_edges / seq_sec}
+        if a.cpu_runs > 0:
+            import oracle
+
+            oracle.build()
+            host = [s.cpu().numpy() for s in host_sets]
+            model, nproc = host_info()
+            c0 = time.perf_counter()
+            ce = 0
+            for i, hp in enumerate(host):
+                ce += oracle.build_edges(oracle.run(hp, float(radii[i])), hp).n_edges
+            c1 = time.perf_counter()
+            out["cpu_baseline"] = {"value": ce / (c1 - c0), "unit": "edges/s", "cores": oracle.max_threads(),
+                                   "kind": "port", "cpu_model": model, "nproc": nproc,
+                                   "sample": f"one step of all 256 sets (run! + build_edges per set, C oracle), "
+                                             f"{c1 - c0:.1f} s"}
+    del b, tns
     torch.cuda.synchronize()
-    s0 = time.perf_counter()
-    for i, (t, sid) in enumerate(seq):
-        octohip.update_point_set(t, sid, host_sets[i])
-        octohip.run(t)
-        seq_edges += octohip.build_edges(t, sid, sid).senders.shape[0]
-    torch.cuda.synchronize()
-    seq_sec = time.perf_counter() - s0
-    cpu = None
-    if a.cpu_steps > 0:
-        import oracle
-
-        oracle.build()
-        host = [s.cpu().numpy() for s in host_sets]
-        c0 = time.perf_counter()
-        ce = 0
-        for i, hp in enumerate(host):
-            ce += oracle.build_edges(oracle.run(hp, float(radii[i])), hp).n_edges
-        c1 = time.perf_counter()
-        cpu = {"value": ce / (c1 - c0), "unit": "edges/s", "cores": oracle.max_threads(), "kind": "port",
-               "sample": f"one step of all 256 sets (run! + build_edges per set, C oracle), {c1 - c0:.1f} s"}
-    print(json.dumps({
-        "metric": "edges/s, batched 256 sets, one replayed graph per step",
-        "value": edges / sec, "unit": "edges/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": 1e3 * sec / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "256 sets, sizes log-uniform in [1000, 65536], 4-component Gaussian mixtures, "
-                               "radius per set for ~20 neighbours; in-place update + run! + build_edges per step",
-                   "sets": len(sets), "points": n, "edges_per_step": edges / a.steps,
-                   "leaf_size": tuned["best"][0] if tuned else a.leaf_size,
-                   "run_leaves": tuned["best"][1] if tuned else a.run_leaves,
-                   "run_chunk": tuned["best"][2] if tuned else a.run_chunk, "timed_replays": replays},
-        "autotune_ms_per_step": ({f"leaf{k[0]}_run{k[1]}_chunk{k[2]}": v for k, v in tuned["ms"].items()}
-                                 if tuned else None),
-        "sets_per_s": len(sets) * a.steps / sec,
-        "last_step": dict(last),
-        "sequential_single_set_api": {"ms_per_step": 1e3 * seq_sec, "edges_per_s": seq_edges / seq_sec},
-        "cpu_baseline": cpu,
-    }))
+    return out
 
 
-def bench_clustered(a):
+def clustered(a, steps):
     """4M points, 90% from a 16-cluster Gaussian mixture (sigma 0.05) and 10%
     uniform, radius 0.01 (about 155 neighbours per point, 6e8 edges): the
     learned predictor's own distribution.  A step is run! (refit of unmoved
@@ -261,7 +332,6 @@ def bench_clustered(a):
     pts, r = datasets.clustered(seed=4)
     n = pts.shape[0]
     coords = torch.from_numpy(pts).cuda()
-    steps = min(a.steps, 5)
     res = {}
     for mode in ("learned", "exact"):
         tns = octohip.TNS()
@@ -309,30 +379,110 @@ def bench_clustered(a):
         del tns
         torch.cuda.synchronize()
     head = res["learned"]
-    print(json.dumps({
-        "metric": "edges/s, learned vs exact build_edges, 4M clustered points",
-        "value": head["value"], "unit": "edges/s", "n_gpus": 1, "steps": steps, "warmup": 2,
-        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "4,000,000 points: 90% from a 16-cluster Gaussian mixture (sigma 0.05), 10% uniform, "
-                               "radius 0.01; run! + build_edges per step on unmoved points",
-                   "points": n, "edges_per_step": head["edges_per_step"]},
-        "learned_over_exact": head["value"] / res["exact"]["value"],
-        "modes": res,
-    }))
+    return {"value": head["value"], "unit": "edges/s", "steps": steps, "points": n,
+            "edges_per_step": head["edges_per_step"], "ms_per_step": head["ms_per_step"],
+            "heavy_leaf_share": head["last_step"]["heavy_leaves"] / max(head["last_step"]["n_leaves"], 1),
+            "learned_over_exact": head["value"] / res["exact"]["value"], "modes": res}
+
+
+def differentiable(a, steps):
+    """262,144 uniform points, radius for ~24 neighbours; a step is run! +
+    build_edges_diff forward + loss = sum(rel_displacement^2) +
+    0.5 sum(rel_dist_norm^2) + backward through the HIP adjoint.  The
+    pullback's roofline: it reads 40 B per edge (indices 8, the tangents 16,
+    rel_displacement and rel_dist_norm 16) and writes 12 B per point."""
+    import torch
+
+    import octohip
+    from octohip import datasets
+
+    pts, r = datasets.differentiable()
+    n = pts.shape[0]
+    x = torch.from_numpy(pts).cuda()
+    tns = octohip.TNS()
+    octohip.set_search_radius(tns, float(r))
+    sid = octohip.add_point_set(tns, x)
+    octohip.set_active_search(tns, sid, sid)
+
+    def step():
+        octohip.update_point_set(tns, sid, x)
+        octohip.run(tns)
+        xg = x.detach().requires_grad_(True)
+        e = octohip.build_edges_diff(xg, tns, sid, float(r))
+        loss = (e.rel_displacement ** 2).sum() + 0.5 * (e.rel_dist_norm ** 2).sum()
+        loss.backward()
+        return e.senders.shape[0]
+
+    for _ in range(max(a.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    octohip.profile(tns, True, phases=["backward"])
+    octohip.profile_read(tns, reset=True)
+    edges = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        edges += step()
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    bwd = octohip.profile_read(tns, reset=True)["backward"]
+    octohip.profile(tns, True)
+    step()
+    torch.cuda.synchronize()
+    prof = octohip.profile_read(tns, reset=True)
+    octohip.profile(tns, False)
+    e_avg = edges / steps
+    bwd_s = bwd[0] / max(bwd[1], 1) / 1e3
+    bwd_bytes = 40.0 * e_avg + 12.0 * n
+    achieved = bwd_bytes / bwd_s / 1e9 if bwd_s > 0 else 0.0
+    del tns
+    return {"value": edges / sec, "unit": "edges/s", "steps": steps, "points": n, "radius": float(r),
+            "edges_per_step": e_avg, "ms_per_step": 1e3 * sec / steps,
+            "breakdown_ms_per_step": {k: v[0] for k, v in prof.items() if v[1]},
+            "pullback_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": achieved / HBM_PEAK_GBS, "kernel": "k_pullback",
+                                  "kernel_avg_ms": bwd_s * 1e3, "alg_bytes_per_launch": bwd_bytes}}
+
+
+def envelope(metric, value, steps, warmup, ms, config, **extra):
+    out = {"metric": metric, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "the benchmark measures one GPU (one process, one device); --gpus above 1 is not supported",
-                          "n_gpus": a.gpus}))
+        print(json.dumps({"error": "the benchmark measures one GPU (one process, one device); --gpus above 1 is "
+                                   "not supported", "n_gpus": a.gpus}))
         sys.exit(2)
     if a.workload == "batched":
-        bench_batched(a)
+        b = batched(a, not a.no_autotune, a.steps, True)
+        print(json.dumps(envelope(
+            "edges/s, batched 256 sets, one replayed graph per step", b["value"], a.steps, a.warmup,
+            b["ms_per_step"],
+            {"workload": "256 sets, sizes log-uniform in [1000, 65536], 4-component Gaussian mixtures, radius per "
+                         "set for ~20 neighbours; in-place update + run! + build_edges per step", **b})))
         return
     if a.workload == "clustered":
-        bench_clustered(a)
+        c = clustered(a, min(a.steps, 5))
+        print(json.dumps(envelope(
+            "edges/s, learned vs exact build_edges, 4M clustered points", c["value"], c["steps"], 2,
+            c["ms_per_step"],
+            {"workload": "4,000,000 points: 90% from a 16-cluster Gaussian mixture (sigma 0.05), 10% uniform, "
+                         "radius 0.01; run! + build_edges per step on unmoved points", "points": c["points"],
+             "edges_per_step": c["edges_per_step"]},
+            learned_over_exact=c["learned_over_exact"], heavy_leaf_share=c["heavy_leaf_share"], modes=c["modes"])))
+        return
+    if a.workload == "diff":
+        d = differentiable(a, a.steps)
+        print(json.dumps(envelope(
+            "edges/s, build_edges_diff forward + HIP backward, 262,144 points", d["value"], a.steps, a.warmup,
+            d["ms_per_step"],
+            {"workload": "262,144 uniform points, radius for ~24 neighbours; run! + build_edges_diff + loss + "
+                         "backward per step", "points": d["points"], "edges_per_step": d["edges_per_step"]},
+            breakdown_ms_per_step=d["breakdown_ms_per_step"], roofline=d["pullback_roofline"])))
         return
     import torch
 
@@ -355,7 +505,7 @@ def main():
     snaps = datasets.sph_snapshots(base, total_steps, h=h, seed=2)
     torch.cuda.synchronize()
 
-    modes = ["learned", "exact"] if a.mode == "both" else [a.mode]
+    modes = {"all": ["learned", "learned_no_predictor", "exact"], "both": ["learned", "exact"]}.get(a.mode, [a.mode])
     results = {m: run_mode(a, m, snaps, r) for m in modes}
     head = results[modes[0]]
 
@@ -368,58 +518,45 @@ def main():
     alg_bytes = 24.0 * e_avg + 12.0 * n  # 24 B per emitted edge + 12 B per query point read
     achieved = alg_bytes / k_avg_s / 1e9 if k_avg_s > 0 else 0.0
     kernel_symbol = "k_edges_single" if kname == "single" else "k_edges<EDGE_FILL>"
-    traffic = None
+    traffic, traffic_edges = None, None
     if os.path.exists(TRAFFIC_FILE):
         tr = json.load(open(TRAFFIC_FILE))
         if kernel_symbol in tr.get("kernels", {}):
             traffic = tr["kernels"][kernel_symbol]["bytes_per_launch"]
-    step_bytes = 16.0 * n + 24.0 * e_avg + 12.0 * n  # coords + perm write, edge output, query reads
+            traffic_edges = tr.get("edges_per_launch")
+    # run! writes perm and the 64 B-per-node tree after reading the
+    # coordinates; build_edges reads the queries and writes 24 B per edge
+    step_bytes = 16.0 * n + 64.0 * head["n_nodes"] + 12.0 * n + 24.0 * e_avg
 
     cpu = None
-    if a.cpu_steps > 0:
-        import oracle
+    if a.cpu_runs > 0:
+        host = [snaps[k].cpu().numpy() for k in range(min(a.cpu_runs + 1, len(snaps)))]
+        cpu = cpu_baseline(host, r, a.cpu_runs, f"the same {n}-point SPH workload")
 
-        oracle.build()
-        threads = oracle.max_threads()
-        host = [snaps[k].cpu().numpy() for k in range(min(a.cpu_steps, len(snaps)))]
-        ce = 0
-        c0 = time.perf_counter()
-        for k in range(a.cpu_steps):
-            hp = host[k % len(host)]
-            t = oracle.run(hp, float(r))
-            ce += oracle.build_edges(t, hp).n_edges
-        c1 = time.perf_counter()
-        cpu = {"value": ce / (c1 - c0), "unit": "edges/s", "cores": threads, "kind": "port",
-               "sample": f"{a.cpu_steps} steps of the same 1M-point SPH workload (run! + build_edges, C oracle, "
-                         f"OpenMP {threads} threads), {c1 - c0:.1f} s"}
-
-    out = {
-        "metric": METRIC,
-        "value": head["value"],
-        "unit": "edges/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": head["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"SPH block: {n} FP32 points on a {a.side}^3 lattice, h=0.01, jitter 0.1h, "
-                               f"radius 2h, N(0,(0.01h)^2) step displacement; update + run! (refit mode) + "
-                               f"build_edges per step",
-                   "mode": head["mode"], "points": n, "edges_per_step": e_avg, "radius": float(r)},
-        "modes": results,
-        "points_per_s": n / (head["ms_per_step"] / 1e3),
-        "hbm_fraction_step": (step_bytes / (head["ms_per_step"] / 1e3)) / 1e9 / HBM_PEAK_GBS,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kernel_symbol, "kernel_avg_ms": k_avg_s * 1e3, "alg_bytes_per_launch": alg_bytes},
-        "cpu_baseline": cpu,
-    }
+    out = envelope(
+        METRIC, head["value"], a.steps, a.warmup, head["ms_per_step"],
+        {"workload": f"SPH block: {n} FP32 points on a {a.side}^3 lattice, h=0.01, jitter 0.1h, radius 2h, "
+                     f"N(0,(0.01h)^2) step displacement; update + run! (refit mode) + build_edges per step",
+         "mode": head["mode"], "points": n, "edges_per_step": e_avg, "radius": float(r)},
+        modes=results,
+        points_per_s=n / (head["ms_per_step"] / 1e3),
+        hbm_fraction_step=(step_bytes / (head["ms_per_step"] / 1e3)) / 1e9 / HBM_PEAK_GBS,
+        roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_measured_at_edges": traffic_edges,
+                  "kernel": kernel_symbol, "kernel_avg_ms": k_avg_s * 1e3, "alg_bytes_per_launch": alg_bytes},
+        cpu_baseline=cpu,
+    )
     if "learned" in results and "exact" in results:
         out["learned_over_exact"] = results["learned"]["value"] / results["exact"]["value"]
+    if "learned" in results and "learned_no_predictor" in results:
+        out["learned_over_no_predictor"] = results["learned"]["value"] / results["learned_no_predictor"]["value"]
+    if not a.no_extras and a.side == 100:
+        del snaps
+        torch.cuda.empty_cache()
+        out["clustered"] = clustered(a, 3)
+        torch.cuda.empty_cache()
+        out["differentiable"] = differentiable(a, 20)
+        out["batched"] = batched(a, False, 20, False)
     print(json.dumps(out))
 
 
