@@ -149,6 +149,43 @@ int oct_build_info_get(oct_ctx* ctx, int set_id, oct_build_info* out);
  * OCT_ORDER_* (OCT_ORDER_RECEIVER needs OCT_EDGES_EXACT). */
 int oct_build_edges(oct_ctx* ctx, int q, int t, int mode, int order, oct_edges* out);
 
+/* CSR neighbour lists of one active pair (tns.neighbor_buffers[pair],
+ * src/types.jl:44-48).  Query i's neighbours are flat[offsets[i] ..
+ * offsets[i+1]) (offsets are 0-based counts, offsets[0] = 0, as the
+ * reference's exclusive scan gives them; flat carries index_base), each
+ * slice in traversal order.  Device arrays owned by the context, valid until
+ * the next materialize on the pair. */
+typedef struct oct_neighbors {
+    int32_t* offsets;          /* n_queries + 1 */
+    int32_t* flat;             /* total */
+    int64_t n_queries;
+    int64_t total;
+    int32_t materialized;      /* 0 before the first materialize and after each oct_run */
+    int32_t pad;
+} oct_neighbors;
+/* materialize_all_neighbors! for the active pair (q, t) (src/api.jl:348-362,
+ * materialize_cpu! src/cpu/query_cpu.jl:139-192, which the reference runs on
+ * the CPU only, src/api.jl:350): count pass, exclusive scan, and a fill pass
+ * that writes only the target indices, in query order.  The self pair
+ * excludes j == i.  Int32 limit on the total as build_edges. */
+int oct_materialize_neighbors(oct_ctx* ctx, int q, int t, oct_neighbors* out);
+/* The pair's lists as last materialized (materialized = 0: none valid). */
+int oct_get_neighbors(oct_ctx* ctx, int q, int t, oct_neighbors* out);
+
+/* apply_zsort (src/api.jl:405-410; apply_zsort_cpu! src/cpu/zsort_cpu.jl:11-32,
+ * _apply_zsort_cuda ext/OctopusCUDAExt.jl:422-432): out[:, k] = in[:, perm[k]]
+ * with perm the set's z-order permutation.  in and out are distinct device
+ * arrays of n (the set's size) columns of `width` elements of elem_bytes
+ * (1, 2, 4 or 8) bytes each: a column-major (width, n) matrix, or a vector
+ * with width 1.  The permutation must be current (run! after the last
+ * change of the set, as prepare_zsort checks, src/api.jl:381-388):
+ * OCT_ERR_STATE otherwise. */
+int oct_apply_zsort(oct_ctx* ctx, int set_id, const void* in, void* out, int64_t width, int elem_bytes);
+/* Its adjoint, the rrule's scatter (ext/OctopusChainRulesCoreExt.jl:65-94):
+ * grad[:, perm[k]] = d_out[:, k]; every column of grad is written. */
+int oct_apply_zsort_adjoint(oct_ctx* ctx, int set_id, const void* d_out, void* grad, int64_t width,
+                            int elem_bytes);
+
 /* device_view (src/api.jl:451-454, ext/OctopusCUDAExt.jl:37-56). */
 int oct_device_view(oct_ctx* ctx, int q, int t, oct_view* out);
 

@@ -155,6 +155,10 @@ struct PairState {
     // (int32 per leaf) followed by a flag byte per leaf
     DevBuf heavy;
     DevBuf eprm;  // the query tree's build parameters with the leaf count clamped to the rows
+    // CSR neighbour lists (materialize_all_neighbors!): offsets (n_q + 1) and targets
+    DevBuf csr_off, csr_flat;
+    bool csr_ok = false;
+    int64_t csr_nq = 0, csr_total = 0;
     bool heavy_on = true;  // count heavy leaves ahead: first call, and while many leaves are not recorded
     // a call that returned on the single pass's early total: its recount
     // statistics are final only when the kernel ends (read later)
@@ -165,12 +169,13 @@ struct PairState {
     oct_edge_stats st{};
     // the buffers oct_build_edges, oct_batch_run and oct_edge_stats_get hand out
     void export_to(std::vector<void*>* r) {
-        DevBuf* out[] = {&senders, &receivers, &rdisp, &rdist, &feat, &pred, &lhits};
+        DevBuf* out[] = {&senders, &receivers, &rdisp, &rdist, &feat, &pred, &lhits, &csr_off, &csr_flat};
         for (DevBuf* d : out) d->retire = r;
     }
     void release_all() {
         DevBuf* all[] = {&senders, &receivers, &rdisp, &rdist, &lists, &lcount, &qcount, &qoff, &scan_tmp, &flags,
-                         &tiles, &ldir, &feat, &pred, &lhits, &pwork, &pscan, &wave_first, &heavy, &eprm};
+                         &tiles, &ldir, &feat, &pred, &lhits, &pwork, &pscan, &wave_first, &heavy, &eprm,
+                         &csr_off, &csr_flat};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -807,7 +812,10 @@ int oct_run(oct_ctx* c) {
         int rc = build_set(c, s);
         if (rc != OCT_OK) return rc;
     }
-    for (auto& p : c->pairs) p.materialized = false;
+    for (auto& p : c->pairs) {
+        p.materialized = false;
+        p.csr_ok = false;  // run! invalidates materialized lists (test/contracts.jl:252-270)
+    }
     return OCT_OK;
 }
 
@@ -845,9 +853,15 @@ void recount_policy(PairState& p, uint32_t recounted, int64_t heavy_n, int64_t n
     p.heavy_on = (int64_t)recounted * 64 > n_leaves;
 }
 
-// Flat-COO edges of pair (q, t).  collect: exact path that also keeps the
-// predictor's features and the exact per-leaf hits (training rows).
-static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collect, oct_edges* out) {
+// What an edges_impl call produces: the flat-COO edges; only the counts with
+// the predictor's features and the exact per-leaf hits (training rows); or
+// CSR neighbour lists (targets only, in query order).
+enum Emit { EMIT_EDGES = 0, EMIT_COLLECT = 1, EMIT_CSR = 2 };
+
+// Flat-COO edges (or counts, or CSR lists) of pair (q, t).
+static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, int emit, oct_edges* out) {
+    const bool collect = emit == EMIT_COLLECT;
+    const bool csr = emit == EMIT_CSR;
     if (!c || !out) return OCT_ERR_ARGUMENT;
     PairState* pp = find_pair(c, q, t);
     if (!pp) return fail(c, OCT_ERR_STATE, "build_edges!: pair not registered; call set_active_search! first");
@@ -863,14 +877,25 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     memset(out, 0, sizeof(*out));
     // the previous call's recount statistics (final once its kernels ended)
     // come back with this call's read-back, queued ahead of everything else
-    const bool prev_pending = p.stats_pending;
+    // (a CSR call leaves the edge buffers' state and statistics alone)
+    const bool prev_pending = !csr && p.stats_pending;
     if (prev_pending)
         HIP_TRY(c, hipMemcpyAsync(c->sig + SIG_STATS, p.flags.as<uint32_t>() + 9, 8, hipMemcpyDeviceToHost,
                                   c->stream));
-    p.stats_pending = false;
-    memset(&p.st, 0, sizeof(p.st));
-    p.n_edges = 0;
+    if (!csr) {
+        p.stats_pending = false;
+        memset(&p.st, 0, sizeof(p.st));
+        p.n_edges = 0;
+    }
     if (qs.n == 0 || ts.n == 0) {
+        if (csr) {  // every query has no neighbour: offsets all zero
+            HIP_TRY(c, p.csr_off.ensure(4 * (size_t)(qs.n + 1)));
+            HIP_TRY(c, launch_fill_u32(p.csr_off.as<uint32_t>(), qs.n + 1, 0u, c->stream));
+            p.csr_ok = true;
+            p.csr_nq = qs.n;
+            p.csr_total = 0;
+            return OCT_OK;
+        }
         p.materialized = true;
         return OCT_OK;
     }
@@ -880,7 +905,7 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     const float inv_r = 1.0f / r;
     // the predictor runs on the pair's first call and after a call that
     // overflowed or recounted many leaves (or on every call, if asked)
-    const bool predict = mode == OCT_EDGES_LEARNED && !collect && c->mlp.loaded &&
+    const bool predict = mode == OCT_EDGES_LEARNED && emit == EMIT_EDGES && c->mlp.loaded &&
                          (c->predict_always || p.cap_edges <= 0 || p.replan);
     const bool features = predict || collect;
     // Rows of the per-leaf arrays: the exact leaf count when it is on the
@@ -900,7 +925,7 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     HIP_TRY(c, p.qcount.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.qoff.ensure(4 * (size_t)qs.n));
     HIP_TRY(c, p.scan_tmp.ensure(4 * (size_t)scan_tmp_words(qs.n)));
-    const int by_receiver = order == OCT_ORDER_RECEIVER;
+    const int by_receiver = order == OCT_ORDER_RECEIVER || csr;
     // flags (uint32 words): [0] largest candidate count of an overflowing
     // query leaf, [1] a list walk ran out of stack, [2..3] exact-path edge
     // total, [4] single pass over capacity, [5] single-pass look-back gave up
@@ -914,7 +939,7 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     uint32_t hf[HF_WORDS] = {0};
     BuildParams* raw_prm = (BuildParams*)((char*)p.flags.p + HF_BYTES);
     int safe = 0;
-    bool single = mode == OCT_EDGES_LEARNED && !collect && (p.cap_edges > 0 || predict);
+    bool single = mode == OCT_EDGES_LEARNED && emit == EMIT_EDGES && (p.cap_edges > 0 || predict);
     const bool heavy = single && p.heavy_on;
     bool lists_ok = false;
     uint64_t total = 0;
@@ -1092,6 +1117,22 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
     if (total >= (uint64_t)INT32_MAX)
         return fail(c, OCT_ERR_CAPACITY, "edge count exceeds Int32 (the reference's limit)");
     const int64_t E = (int64_t)total;
+    if (csr) {  // targets only, each query's slice at its scanned offset
+        HIP_TRY(c, p.csr_off.ensure(4 * (size_t)(qs.n + 1)));
+        HIP_TRY(c, p.csr_flat.ensure(4 * (size_t)std::max<int64_t>(E, 1)));
+        if (E > 0)
+            HIP_TRY(c, launch_edges(EDGE_INDEX, qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r,
+                                    self_pair, 1, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), nullptr, E,
+                                    EdgeOut{p.csr_flat.as<int32_t>(), nullptr, nullptr, nullptr}, c->index_base,
+                                    c->stream));
+        HIP_TRY(c, hipMemcpyAsync(p.csr_off.p, p.qoff.p, 4 * (size_t)qs.n, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, launch_fill_u32(p.csr_off.as<uint32_t>() + qs.n, 1, (uint32_t)E, c->stream));
+        p.csr_ok = true;
+        p.csr_nq = qs.n;
+        p.csr_total = E;
+        out->n_edges = E;
+        return OCT_OK;
+    }
     if (collect) {  // counts only: the training rows need no edges
         p.n_edges = 0;
         p.materialized = false;
@@ -1161,7 +1202,34 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, bool collec
 }
 
 int oct_build_edges(oct_ctx* c, int q, int t, int mode, int order, oct_edges* out) {
-    return edges_impl(c, q, t, mode, order, false, out);
+    return edges_impl(c, q, t, mode, order, EMIT_EDGES, out);
+}
+
+static void neighbors_out(const PairState& p, oct_neighbors* out) {
+    memset(out, 0, sizeof(*out));
+    if (!p.csr_ok) return;
+    out->offsets = p.csr_off.as<int32_t>();
+    out->flat = p.csr_flat.as<int32_t>();
+    out->n_queries = p.csr_nq;
+    out->total = p.csr_total;
+    out->materialized = 1;
+}
+
+int oct_materialize_neighbors(oct_ctx* c, int q, int t, oct_neighbors* out) {
+    if (!c || !out) return OCT_ERR_ARGUMENT;
+    oct_edges e{};
+    const int rc = edges_impl(c, q, t, OCT_EDGES_EXACT, OCT_ORDER_RECEIVER, EMIT_CSR, &e);
+    if (rc != OCT_OK) return rc;
+    neighbors_out(*find_pair(c, q, t), out);
+    return OCT_OK;
+}
+
+int oct_get_neighbors(oct_ctx* c, int q, int t, oct_neighbors* out) {
+    if (!c || !out) return OCT_ERR_ARGUMENT;
+    PairState* p = find_pair(c, q, t);
+    if (!p) return fail(c, OCT_ERR_STATE, "neighbours: pair not registered; call set_active_search! first");
+    neighbors_out(*p, out);
+    return OCT_OK;
 }
 
 int oct_device_view(oct_ctx* c, int q, int t, oct_view* out) {
@@ -1203,6 +1271,32 @@ __global__ void k_count_neighbors_device(oct_view v, int32_t* counts) {
 }
 
 extern "C" {
+
+static int zsort_impl(oct_ctx* c, int id, const void* src, void* dst, int64_t width, int elem_bytes,
+                      bool scatter) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (!valid_set(c, id)) return fail(c, OCT_ERR_ARGUMENT, "unknown point set");
+    if (width < 1) return fail(c, OCT_ERR_ARGUMENT, "apply_zsort: width must be >= 1");
+    if (elem_bytes != 1 && elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8)
+        return fail(c, OCT_ERR_ARGUMENT, "apply_zsort: element size must be 1, 2, 4 or 8 bytes");
+    SetState& s = c->sets[set_index(c, id)];
+    // prepare_zsort (src/api.jl:381-388): the permutation must be current
+    if (!s.built || s.dirty) return fail(c, OCT_ERR_STATE, "apply_zsort: call run! first so the permutation is available");
+    if (s.n == 0) return OCT_OK;
+    if (!src || !dst) return fail(c, OCT_ERR_ARGUMENT, "apply_zsort: null array");
+    if (src == dst) return fail(c, OCT_ERR_ARGUMENT, "apply_zsort: the result must be a new array");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, launch_zsort(src, dst, s.perm.as<int32_t>(), s.n, width, elem_bytes, c->index_base, scatter, c->stream));
+    return OCT_OK;
+}
+
+int oct_apply_zsort(oct_ctx* c, int set_id, const void* in, void* out, int64_t width, int elem_bytes) {
+    return zsort_impl(c, set_id, in, out, width, elem_bytes, false);
+}
+
+int oct_apply_zsort_adjoint(oct_ctx* c, int set_id, const void* d_out, void* grad, int64_t width, int elem_bytes) {
+    return zsort_impl(c, set_id, d_out, grad, width, elem_bytes, true);
+}
 
 int oct_count_neighbors_device(oct_ctx* c, const oct_view* v, int32_t* counts) {
     if (!c || !v) return OCT_ERR_ARGUMENT;
@@ -1357,7 +1451,7 @@ int oct_predictor_collect(oct_ctx* c, int q, int t, int64_t* rows) {
     if (!c) return OCT_ERR_ARGUMENT;
     if (rows) *rows = 0;
     oct_edges e{};
-    int rc = edges_impl(c, q, t, OCT_EDGES_EXACT, OCT_ORDER_LEAF, true, &e);
+    int rc = edges_impl(c, q, t, OCT_EDGES_EXACT, OCT_ORDER_LEAF, EMIT_COLLECT, &e);
     if (rc != OCT_OK) return rc;
     PairState& p = *find_pair(c, q, t);
     const int64_t n = p.st.features ? p.st.n_leaves : 0;

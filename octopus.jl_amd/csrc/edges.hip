@@ -408,14 +408,14 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
         uint32_t my = 0;  // COUNT: count; FILL: next output position of this lane's query
         uint32_t blk_base = 0, blk_tot = 0;
         bool staged = false;
-        if (MODE == EDGE_FILL) {
+        if (MODE != EDGE_COUNT) {
             uint32_t c = 0;
             if (lane < nqb) {
                 my = qoff[slot];
                 c = qcount[slot];
             }
             // leaf order: this block of queries owns [blk_base, blk_base + blk_tot)
-            if (!by_receiver) {
+            if (MODE == EDGE_FILL && !by_receiver) {
                 blk_base = (uint32_t)__builtin_amdgcn_readfirstlane((int)my);
                 blk_tot = (uint32_t)wave_sum((int)c);
                 staged = blk_tot <= L.stage;
@@ -441,7 +441,7 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
                     const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
                     const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
                     const int32_t ix = self_pair ? i : -2;  // -2 matches no point: only the self pair drops j == i
-                    uint32_t run = MODE == EDGE_FILL ? (uint32_t)__builtin_amdgcn_readlane((int)my, qi) : 0u;
+                    uint32_t run = MODE != EDGE_COUNT ? (uint32_t)__builtin_amdgcn_readlane((int)my, qi) : 0u;
                     const uint32_t run0 = run;
 #pragma unroll
                     for (int u = 0; u < EW_TILE; ++u) {
@@ -454,6 +454,7 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
                         const bool in_r = d2 <= r_sq;
                         const bool other = j != ix;
                         const uint64_t m = ballot(in_r) & ballot(other);  // compare masks ANDed, no branch
+                        if (MODE == EDGE_INDEX && (in_r & other)) out.senders[run + (uint32_t)mbcnt(m)] = j + index_base;
                         if (MODE == EDGE_FILL && (in_r & other)) {
                             const uint32_t e = run + (uint32_t)mbcnt(m);
                             if (staged) {
@@ -1214,8 +1215,12 @@ hipError_t launch_edges(int mode, const TreeView& q, const TreeView& t, const in
         hipLaunchKernelGGL(k_edges<EDGE_COUNT>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, by_receiver, qcount, qoff,
                            lhits, out, index_base);
-    else
+    else if (mode == EDGE_FILL)
         hipLaunchKernelGGL(k_edges<EDGE_FILL>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
+                           q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, by_receiver, qcount, qoff,
+                           lhits, out, index_base);
+    else
+        hipLaunchKernelGGL(k_edges<EDGE_INDEX>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, by_receiver, qcount, qoff,
                            lhits, out, index_base);
     return hipGetLastError();

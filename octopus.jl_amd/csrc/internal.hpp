@@ -217,7 +217,9 @@ struct EdgeOut {
     float* rdist;
 };
 
-enum { EDGE_COUNT = 0, EDGE_FILL = 1 };
+// EDGE_INDEX: the fill pass writing only the target index of each hit
+// (CSR neighbour lists, materialize_all_neighbors!), into out.senders.
+enum { EDGE_COUNT = 0, EDGE_FILL = 1, EDGE_INDEX = 2 };
 // flags[0]: largest candidate count of an overflowing query leaf; flags[1]:
 // a walk ran out of stack (rerun with safe = 1)
 // Per-leaf set parameters of a forest (batched sets, self pairs): the set of
@@ -347,6 +349,11 @@ hipError_t launch_batch_check_bits(const BuildParams* sprm, const int64_t* off, 
 hipError_t launch_batch_edge_offsets(const uint32_t* loff, const uint64_t* total, const NodeRec* leaves,
                                      const BuildParams* gprm, const int64_t* off, int S, int64_t* eoff,
                                      hipStream_t s);
+
+// apply_zsort gather (scatter = its adjoint) of n columns of `width`
+// elements of elem_bytes (1, 2, 4 or 8) bytes (see zsort.hip).
+hipError_t launch_zsort(const void* src, void* dst, const int32_t* perm, int64_t n, int64_t width, int elem_bytes,
+                        int index_base, bool scatter, hipStream_t s);
 
 // Adjoint (see backward.hip).
 hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* senders,

@@ -25,6 +25,7 @@ __all__ = [
     "set_search_radius", "set_refit_mode", "set_leaf_size", "add_point_set", "update_point_set",
     "resize_point_set", "set_active_search", "run", "build_edges", "build_edges_diff",
     "device_view", "for_each_neighbor", "count_neighbors_device", "get_tree", "trim", "retired_count",
+    "materialize_all_neighbors", "neighbors", "get_neighborlist", "prepare_zsort", "apply_zsort",
 ]
 
 
@@ -363,6 +364,57 @@ def build_edges(tns: TNS, q: int, t: int, mode: str = "exact", order: str = "lea
     )
 
 
+Neighbors = namedtuple("Neighbors", ["offsets", "flat"])
+
+
+def _neighbors_of(tns, nb):
+    torch = _torch()
+    if not nb.materialized:
+        return None
+    n, total, dev = int(nb.n_queries), int(nb.total), tns.device
+    return Neighbors(offsets=_wrap(nb.offsets, (n + 1,), torch.int32, dev, tns),
+                     flat=_wrap(nb.flat, (total,), torch.int32, dev, tns))
+
+
+def materialize_all_neighbors(tns: TNS):
+    """materialize_all_neighbors!(tns) (src/api.jl:348-362): CSR neighbour
+    lists of every active pair, on the device (the reference builds them on
+    the CPU only, src/api.jl:350).  Read them with neighbors(tns, q, t)."""
+    ctx = tns._enter()
+    for q, t in tns.active_pairs:
+        nb = L.OctNeighbors()
+        tns._check(tns._lib.oct_materialize_neighbors(ctx, int(q), int(t), ctypes.byref(nb)))
+    tns._leave()
+    return tns
+
+
+def neighbors(tns: TNS, q: int, t: int):
+    """tns.neighbor_buffers[pair] (src/types.jl:44-48) as device tensors
+    (offsets (n_q + 1,), flat (total,)): query i's neighbours are
+    flat[offsets[i]:offsets[i + 1]]; None until materialized (run clears it)."""
+    nb = L.OctNeighbors()
+    tns._check(tns._lib.oct_get_neighbors(tns._c(), int(q), int(t), ctypes.byref(nb)))
+    return _neighbors_of(tns, nb)
+
+
+def get_neighborlist(tns: TNS, q: int, t: int, i: int):
+    """get_neighborlist(tns, q, t, i) (src/api.jl:282-292): the neighbours of
+    query i (self excluded for the self pair) as an int32 device tensor, from
+    the pair's materialized lists (materialized first when they are not)."""
+    if (q, t) not in tns.active_pairs:
+        raise OctopusError(f"pair ({q}, {t}) not registered; call set_active_search! first")
+    nb = neighbors(tns, q, t)
+    if nb is None:
+        ctx = tns._enter()
+        raw = L.OctNeighbors()
+        tns._check(tns._lib.oct_materialize_neighbors(ctx, int(q), int(t), ctypes.byref(raw)))
+        tns._leave()
+        nb = _neighbors_of(tns, raw)
+    k = int(i) - tns.index_base
+    lo, hi = (int(v) for v in nb.offsets[k:k + 2].cpu())
+    return nb.flat[lo:hi]
+
+
 def device_view(tns: TNS, q: int, t: int) -> L.OctView:
     """device_view(tns, q, t) (src/api.jl:451-454): a by-value handle for HIP
     kernels (include/octohip_device.h); invalid after the next run."""
@@ -477,6 +529,77 @@ def for_each_neighbor(f, tns: TNS, q: int, t: int, i: int):
     raises on device handles (src/cpu/iterator_cpu.jl:246); this engine's
     handles are device handles, so use the device iterator instead."""
     raise OctopusError("for_each_neighbor (host) requires a CPU TNS; use for_each_neighbor_device on GPU")
+
+
+# ------------------------------------------------------------------ z-sort
+
+
+def prepare_zsort(tns: TNS):
+    """prepare_zsort(tns) (src/api.jl:381-388): every point set's permutation
+    must be current (run! since the last change)."""
+    for k, d in enumerate(tns.dirty):
+        if d:
+            raise OctopusError(f"prepare_zsort: call run! first so the permutation for set "
+                               f"{k + tns.index_base} is available")
+    return tns
+
+
+def _zsort_call(tns, set_id, src, adjoint):
+    torch = _torch()
+    k = _sid(tns, set_id)
+    n = tns.point_sets[k].shape[0]
+    if src.dim() > 2:
+        raise ArgumentError(f"apply_zsort supports 1-D or 2-D arrays only; got {src.dim()}-D")
+    if src.shape[0] != n:
+        raise DimensionMismatch(f"perm length {n} does not match the point axis {src.shape[0]}")
+    if src.device != tns.device:
+        raise ArgumentError("apply_zsort: the array must live on the TNS device")
+    src = src.contiguous()
+    width = 1 if src.dim() == 1 else src.shape[1]
+    out = torch.empty_like(src)
+    if src.numel() == 0:
+        return out
+    fn = tns._lib.oct_apply_zsort_adjoint if adjoint else tns._lib.oct_apply_zsort
+    tns._check(fn(tns._enter(), int(set_id), ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                  int(width), int(src.element_size())))
+    tns._leave()
+    return out
+
+
+def _make_zsort_fn():
+    torch = _torch()
+
+    class ApplyZsort(torch.autograd.Function):
+        """rrule of apply_zsort (ext/OctopusChainRulesCoreExt.jl:65-94): the
+        pullback scatters the tangent back through the permutation."""
+
+        @staticmethod
+        def forward(ctx, user_array, tns, set_id):
+            ctx.tns, ctx.set_id = tns, set_id
+            return _zsort_call(tns, set_id, user_array, False)
+
+        @staticmethod
+        def backward(ctx, dy):
+            return _zsort_call(ctx.tns, ctx.set_id, dy, True), None, None
+
+    return ApplyZsort
+
+
+_ApplyZsort = None
+
+
+def apply_zsort(tns: TNS, set_id: int, user_array):
+    """apply_zsort(tns, set_id, user_array) (src/api.jl:405-410): a new array
+    whose entries along the point axis follow the z-order permutation of the
+    set (a (N,) vector or an (N, F) matrix, the bytes of the reference's
+    (F, N) column-major matrix); user_array is not written.  Differentiable:
+    the backward is the permutation's scatter (the reference's rrule)."""
+    global _ApplyZsort
+    if _ApplyZsort is None:
+        _ApplyZsort = _make_zsort_fn()
+    if user_array.requires_grad:
+        return _ApplyZsort.apply(user_array, tns, set_id)
+    return _zsort_call(tns, set_id, user_array, False)
 
 
 # ----------------------------------------------------------------- autograd

@@ -98,6 +98,17 @@ class OctBuildInfo(ctypes.Structure):
     ]
 
 
+class OctNeighbors(ctypes.Structure):
+    _fields_ = [
+        ("offsets", ctypes.c_void_p),
+        ("flat", ctypes.c_void_p),
+        ("n_queries", ctypes.c_int64),
+        ("total", ctypes.c_int64),
+        ("materialized", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
 class OctEdgeStats(ctypes.Structure):
     _fields_ = [
         ("features", ctypes.c_void_p),
@@ -139,6 +150,10 @@ SIGNATURES = {
     "oct_build_edges": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(OctEdges)]),
     "oct_device_view": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OctView)]),
+    "oct_materialize_neighbors": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OctNeighbors)]),
+    "oct_get_neighbors": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OctNeighbors)]),
+    "oct_apply_zsort": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_int64, ctypes.c_int]),
+    "oct_apply_zsort_adjoint": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_int64, ctypes.c_int]),
     "oct_count_neighbors_device": (ctypes.c_int, [_P, ctypes.POINTER(OctView), _P]),
     "oct_build_edges_backward": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(OctEdges), _P, _P,
                                                 ctypes.c_float, _P]),

@@ -291,6 +291,39 @@ def build_edges_rows(tree: Tree, coords_q, rows, coords_t=None, exclude_self=Non
     return out
 
 
+def apply_zsort(perm, a):
+    """apply_zsort_cpu! (src/cpu/zsort_cpu.jl:11-32): out[..., k] =
+    a[..., perm[k]] along the point axis (axis 0 of an (N,) or (N, F) array
+    here, the last axis of the reference's column-major (F, N))."""
+    a = np.asarray(a)
+    if a.ndim > 2:
+        raise ValueError("1-D or 2-D arrays only")
+    out = np.empty_like(a)
+    for k in range(perm.shape[0]):
+        out[k] = a[perm[k]]
+    return out
+
+
+def zsort_pullback(perm, dy):
+    """The rrule's scatter (ext/OctopusChainRulesCoreExt.jl:85-94): grad =
+    zero(a); grad[perm[k]] = dy[k]."""
+    dy = np.asarray(dy)
+    g = np.zeros_like(dy)
+    for k in range(perm.shape[0]):
+        g[perm[k]] = dy[k]
+    return g
+
+
+def materialize(tree: Tree, coords_q, coords_t=None, exclude_self=None, nthreads=None):
+    """materialize_cpu! (src/cpu/query_cpu.jl:139-192): CSR (offsets int32
+    of n_q + 1 starting at 0, flat int32 targets).  Its passes are those of
+    build_edges without the features (the same traversal, the same j != i
+    rule for the self pair, the same serial exclusive scan), so the lists
+    are build_edges' offsets and senders."""
+    e = build_edges(tree, coords_q, coords_t, exclude_self, nthreads)
+    return e.offsets.astype(np.int32), e.senders.copy()
+
+
 def for_each_neighbor(tree: Tree, coords_q, i: int, coords_t=None) -> np.ndarray:
     """Host for_each_neighbor (src/cpu/iterator_cpu.jl:245-300): every j within
     r of query i, traversal order, self not filtered."""

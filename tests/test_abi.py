@@ -46,6 +46,7 @@ def test_struct_sizes_match_header():
     assert ctypes.sizeof(_lib.OctView) == 10 * 8 + 4 * 4 + 8 + 8
     assert ctypes.sizeof(_lib.OctEdgeStats) == 3 * 8 + 4 * 8 + 8 + 4 * 4
     assert ctypes.sizeof(_lib.OctBuildInfo) == 16
+    assert ctypes.sizeof(_lib.OctNeighbors) == 2 * 8 + 2 * 8 + 2 * 4
     assert ctypes.sizeof(_lib.OctBatchEdges) == 4 * 8 + 8 + 8 + 2 * 4 + 3 * 8
 
 

@@ -380,3 +380,48 @@ def test_zero_displacement_component_is_negative_zero(orc):
     b = brute_edges(pts, pts, 1.0, exclude_self=True)
     assert np.array_equal(bits(sort_edges(*b)[2]), bits(sort_edges(e.senders, e.receivers, e.rel_displacement,
                                                                      e.rel_dist_norm)[2]))
+
+
+@pytest.mark.parametrize("n,r,seed", [(50, 0.15, 10), (500, 0.05, 20), (2000, 0.1, 30), (500, 0.08, 60)])
+def test_materialize_matches_brute_force(orc, n, r, seed):
+    """CSR lists (test/query_cpu.jl:14-52,83-106): each query's slice is its
+    brute-force neighbour set, self excluded."""
+    pts = datasets.uniform_cube(n, seed)
+    off, flat = orc.materialize(orc.run(pts, r), pts)
+    assert off.dtype == np.int32 and off[0] == 0 and off.shape[0] == n + 1
+    s, rc, _, _ = brute_edges(pts, pts, r, exclude_self=True)
+    assert np.array_equal(np.diff(off), np.bincount(rc, minlength=n))
+    for i in range(n):
+        assert np.array_equal(np.sort(flat[off[i]:off[i + 1]]), s[rc == i])
+
+
+def test_materialize_cross_pair_and_clustered(orc):
+    a = datasets.uniform_cube(200, 51)
+    b = datasets.uniform_cube(300, 52)
+    off, flat = orc.materialize(orc.run(b, 0.1), a, b)
+    s, rc, _, _ = brute_edges(a, b, 0.1, exclude_self=False)
+    for i in range(200):
+        assert np.array_equal(np.sort(flat[off[i]:off[i + 1]]), s[rc == i])
+    pts = (np.float32(0.1) * datasets.normal(40, 0, 15000).astype(np.float32)).reshape(5000, 3)
+    off, flat = orc.materialize(orc.run(pts, 0.03), pts)
+    s, rc, _, _ = brute_edges(pts, pts, 0.03, exclude_self=True)
+    assert np.array_equal(np.diff(off), np.bincount(rc, minlength=5000))
+    assert np.array_equal(np.sort(flat[off[100]:off[101]]), s[rc == 100])
+
+
+def test_zsort_gather_and_scatter(orc):
+    """apply_zsort on the reference's layout: the gather follows perm, the
+    scatter is its transpose (<S x, y> = <x, S^T y> for random x, y) and
+    undoes it."""
+    pts = datasets.uniform_cube(700, 71)
+    t = orc.run(pts, 0.05)
+    g = np.random.default_rng(3)
+    a = g.normal(size=(700, 5)).astype(np.float32)
+    z = orc.apply_zsort(t.perm, a)
+    assert np.array_equal(z[0], a[t.perm[0]]) and np.array_equal(z[-1], a[t.perm[-1]])
+    v = np.arange(700, dtype=np.int32)
+    assert np.array_equal(orc.apply_zsort(t.perm, v), t.perm)  # gathering the identity gives perm itself
+    y = g.normal(size=(700, 5))
+    x = g.normal(size=(700, 5))
+    assert np.isclose((orc.apply_zsort(t.perm, x) * y).sum(), (x * orc.zsort_pullback(t.perm, y)).sum())
+    assert np.array_equal(orc.zsort_pullback(t.perm, z), a)
