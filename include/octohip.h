@@ -200,6 +200,13 @@ int oct_count_neighbors_device(oct_ctx* ctx, const oct_view* view, int32_t* coun
  * zero tangent.  Edge indices carry the context's index_base. */
 int oct_build_edges_backward(oct_ctx* ctx, int64_t n_points, const oct_edges* edges,
                              const float* d_disp, const float* d_dist, float radius, float* grad);
+/* No counterpart in the reference (whose GPU adjoint pushes every term with
+ * atomics, ext/OctopusCUDAChainRulesCoreExt.jl:56-99).  pull != 0 (the
+ * default): oct_build_edges_backward pulls each edge's sender term from its
+ * reverse edge when every receiver's edges form one run (build_edges
+ * output, either order) and pushes the rest; 0: every sender term is pushed
+ * with an atomic.  Both give dL/dcoords up to FP32 summation order. */
+int oct_set_backward_form(oct_ctx* ctx, int pull);
 
 /* Kernel timing by phase with HIP events on the context's stream (off by
  * default).  oct_profile_read waits for the stream, then writes the summed

@@ -281,6 +281,8 @@ struct oct_ctx {
     std::vector<PairState> pairs;
     std::vector<BatchState> batches;
     std::vector<void*> retired;  // grown-out allocations of handed-out buffers (DevBuf::retire)
+    bool pull_backward = true;   // build_edges_diff adjoint in the pull form (run table), else the push form
+    DevBuf bw_scratch;           // its run table
     Predictor mlp;
     std::string err;
     void* pinned = nullptr;  // small host staging area for read-backs
@@ -689,6 +691,7 @@ int oct_destroy(oct_ctx* c) {
     for (auto& b : c->batches) b.release_all();
     c->mlp.release_all();
     for (void* p : c->retired) (void)hipFree(p);
+    c->bw_scratch.release();
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->sig) (void)hipHostFree(c->sig);
     for (auto& m : c->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
@@ -1314,9 +1317,21 @@ int oct_build_edges_backward(oct_ctx* c, int64_t n_points, const oct_edges* e, c
     if (n_points < 0 || e->n_edges < 0) return fail(c, OCT_ERR_ARGUMENT, "negative size");
     if (n_points > 0 && !grad) return fail(c, OCT_ERR_ARGUMENT, "null gradient buffer");
     HIP_TRY(c, hipSetDevice(c->device));
+    // the pull form's run table (edge indices must fit int32, as the reference's)
+    uint32_t* scratch = nullptr;
+    if (n_points > 0 && e->n_edges > 0 && e->n_edges < (int64_t)INT32_MAX && c->pull_backward) {
+        HIP_TRY(c, c->bw_scratch.ensure(4 * (size_t)pullback_scratch_words(n_points)));
+        scratch = c->bw_scratch.as<uint32_t>();
+    }
     Phase ph(c, OCT_PROF_BACKWARD);
     HIP_TRY(c, launch_pullback(n_points, e->n_edges, e->senders, e->receivers, e->rel_displacement,
-                               e->rel_dist_norm, d_disp, d_dist, radius, c->index_base, grad, c->stream));
+                               e->rel_dist_norm, d_disp, d_dist, radius, c->index_base, grad, scratch, c->stream));
+    return OCT_OK;
+}
+
+int oct_set_backward_form(oct_ctx* c, int pull) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    c->pull_backward = pull != 0;
     return OCT_OK;
 }
 

@@ -356,10 +356,14 @@ hipError_t launch_zsort(const void* src, void* dst, const int32_t* perm, int64_t
                         int index_base, bool scatter, hipStream_t s);
 
 // Adjoint (see backward.hip).
+// scratch (pullback_scratch_words(n_points) words, or null for the push
+// form): the pull form's run table; the result is the same up to FP32
+// summation order.
+int64_t pullback_scratch_words(int64_t n_points);
 hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* senders,
                            const int32_t* receivers, const float* rdisp, const float* rdist,
                            const float* d_disp, const float* d_dist, float r, int index_base,
-                           float* grad, hipStream_t s);
+                           float* grad, uint32_t* scratch, hipStream_t s);
 
 int64_t scan_tmp_words(int64_t n);
 // Tiles of the device scan (SCAN_TILE_INPUTS inputs each).

@@ -434,6 +434,13 @@ def count_neighbors_device(tns: TNS, view: L.OctView):
     return out
 
 
+def set_backward_form(tns: TNS, pull: bool = True):
+    """The build_edges_diff adjoint pulls each edge's sender term from its
+    reverse edge (pull=True, the default) or pushes every term with an
+    atomic (pull=False, the reference GPU kernel's form)."""
+    tns._check(tns._lib.oct_set_backward_form(tns._c(), int(bool(pull))))
+
+
 def set_run_leaves(tns: TNS, run_leaves: int):
     """Leaves per single-pass run without a plan (1..64, default 6)."""
     tns._check(tns._lib.oct_set_run_leaves(tns._c(), int(run_leaves)))

@@ -157,6 +157,7 @@ SIGNATURES = {
     "oct_count_neighbors_device": (ctypes.c_int, [_P, ctypes.POINTER(OctView), _P]),
     "oct_build_edges_backward": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(OctEdges), _P, _P,
                                                 ctypes.c_float, _P]),
+    "oct_set_backward_form": (ctypes.c_int, [_P, ctypes.c_int]),
     "oct_profile": (ctypes.c_int, [_P, ctypes.c_int]),
     "oct_profile_phases": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "oct_profile_read": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double),

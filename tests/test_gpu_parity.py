@@ -277,6 +277,37 @@ def test_backward_vs_oracle_pullback(orc):
     np.testing.assert_allclose(_np(g), want, rtol=1e-4, atol=1e-3)
 
 
+@pytest.mark.parametrize("layout", ["leaf", "receiver", "shuffled", "one_way"])
+def test_backward_pull_and_push_forms(orc, layout):
+    """The pull form (sender terms taken from the reverse edges) and the push
+    form (atomics) against the oracle pullback on the differentiable
+    configuration: build_edges' two layouts, a shuffled edge list (receivers
+    not in runs: the pull form must notice and push), and a list with half of
+    the reverse edges dropped (unpaired edges push)."""
+    import octohip
+
+    torch = _torch()
+    pts, r = datasets.differentiable()
+    tns, sid, c = _build(pts, r)
+    e = octohip.build_edges(tns, sid, sid, order="receiver" if layout == "receiver" else "leaf")
+    s, rc, d, n = (x.clone() for x in e)
+    if layout == "shuffled":
+        p = torch.randperm(s.shape[0], generator=torch.Generator().manual_seed(3)).cuda()
+        s, rc, d, n = s[p], rc[p], d[p], n[p]
+    if layout == "one_way":
+        keep = (s < rc) | (rc % 2 == 0)
+        s, rc, d, n = s[keep], rc[keep], d[keep], n[keep]
+    g0 = torch.Generator().manual_seed(0)
+    dd = torch.randn(d.shape, generator=g0).float()
+    dn = torch.randn(n.shape, generator=g0).float()
+    want = orc.pullback(pts.shape[0], _np(s), _np(rc), _np(d), _np(n).reshape(-1), dd.numpy(), dn.numpy(), r)
+    for pull in (True, False):
+        octohip.set_backward_form(tns, pull)
+        g = octohip._pullback(tns, pts.shape[0], s, rc, d, n, dd.cuda(), dn.cuda(), r)
+        print(f"backward layout={layout} pull={pull} edges={s.shape[0]}", flush=True)
+        np.testing.assert_allclose(_np(g), want, rtol=1e-4, atol=1e-3)
+
+
 def test_backward_closed_form():
     """loss = sum|rdisp|^2 + 0.5 sum rdist^2 -> g_i = (6/r^2) sum_j (x_i - x_j)."""
     import octohip
