@@ -46,19 +46,21 @@ typedef struct oct_ctx oct_ctx;
 typedef struct oct_edges {
     int32_t* senders;          /* E, target index j */
     int32_t* receivers;        /* E, query index i */
-    float* rel_displacement;   /* 3*E, (q_i - t_j) / r, xyz per edge */
+    float* rel_displacement;   /* D*E (D = the context's ndims), (q_i - t_j) / r, D components per edge */
     float* rel_dist_norm;      /* E, |q_i - t_j| / r */
     int64_t n_edges;
 } oct_edges;
 
 /* Tree of one point set, reference layout (src/types.jl:23-42), device arrays
- * owned by the context, valid until the next oct_run. */
+ * owned by the context, valid until the next oct_run.  In a 2D context
+ * (quadtrees) the bounds hold 2 floats and node_children 4 ids per node, and
+ * origin[2] is 0. */
 typedef struct oct_tree {
-    float* node_bounds_min;    /* 3*M */
-    float* node_bounds_max;    /* 3*M */
+    float* node_bounds_min;    /* D*M */
+    float* node_bounds_max;    /* D*M */
     int32_t* node_first;       /* M, inclusive first position into perm */
     int32_t* node_last;        /* M, inclusive last position */
-    int32_t* node_children;    /* 8*M, octant order, -1 empty; a leaf has all 8 empty */
+    int32_t* node_children;    /* 2^D*M, octant (quadrant) order, -1 empty; a leaf has all empty */
     int32_t* permutation;      /* N, sorted position -> point index */
     int32_t n_nodes;
     int32_t n_leaves;
@@ -70,8 +72,8 @@ typedef struct oct_tree {
  * DeviceView); pass it to oct_for_each_neighbor_device (octohip_device.h).
  * Invalidated by the next oct_run. */
 typedef struct oct_view {
-    const float* coords_q;     /* query coords, 3*Nq */
-    const float* coords_t;     /* target coords, 3*Nt */
+    const float* coords_q;     /* query coords, 3*Nq (2D: the (x, y, 2^20) copy) */
+    const float* coords_t;     /* target coords, 3*Nt (likewise) */
     const float* bounds_min;   /* reference-layout tree, as in oct_tree */
     const float* bounds_max;
     const int32_t* node_first;
@@ -89,8 +91,13 @@ typedef struct oct_view {
     int32_t pad;
 } oct_view;
 
-/* TNS(T; ndims) (src/types.jl:167-183).  ndims must be 3; device is a HIP
- * ordinal.  The context creates its own stream unless oct_set_stream is used. */
+/* TNS(T; ndims) (src/types.jl:167-183).  ndims 3 (octrees) or 2
+ * (quadtrees: point sets are (2, n) arrays, xy interleaved; the library
+ * builds a 2D set as the octree of (x, y, 2^20), whose edges are the 2D
+ * ones bit for bit and whose nodes are the quadtree's below its top ten
+ * one-child levels, and writes the reference's quadtree arrays from it;
+ * batches are 3D only).  device is a HIP ordinal.  The context creates its
+ * own stream unless oct_set_stream is used. */
 int oct_create(oct_ctx** out, int device, int ndims, int index_base);
 int oct_destroy(oct_ctx* ctx);
 const char* oct_last_error(const oct_ctx* ctx);
@@ -114,8 +121,9 @@ int oct_set_refit_mode(oct_ctx* ctx, int on);
 int oct_set_leaf_size(oct_ctx* ctx, int leaf_size);
 
 /* add_point_set! (src/api.jl:79-109): coords is a caller-owned device array of
- * n points, xyz interleaved (the bytes of a column-major (3, n) matrix); it is
- * referenced, never copied.  Writes the set id. */
+ * n points, xyz (2D: xy) interleaved (the bytes of a column-major (D, n)
+ * matrix); it is referenced, never copied (a 2D set's build reads a padded
+ * copy made at each run).  Writes the set id. */
 int oct_add_point_set(oct_ctx* ctx, const float* coords, int64_t n, int* set_id);
 /* update_point_set! (src/api.jl:131-140): same n required. */
 int oct_update_point_set(oct_ctx* ctx, int set_id, const float* coords, int64_t n);
@@ -195,8 +203,8 @@ int oct_device_view(oct_ctx* ctx, int q, int t, oct_view* out);
 int oct_count_neighbors_device(oct_ctx* ctx, const oct_view* view, int32_t* counts);
 
 /* Pullback of build_edges_diff (ext/OctopusChainRulesCoreExt.jl:137-192,
- * ext/OctopusCUDAChainRulesCoreExt.jl:56-167): grad (3*n_points, device) is
- * overwritten with dL/dcoords.  d_disp (3*E) or d_dist (E) may be NULL for a
+ * ext/OctopusCUDAChainRulesCoreExt.jl:56-167): grad (D*n_points, device) is
+ * overwritten with dL/dcoords.  d_disp (D*E) or d_dist (E) may be NULL for a
  * zero tangent.  Edge indices carry the context's index_base. */
 int oct_build_edges_backward(oct_ctx* ctx, int64_t n_points, const oct_edges* edges,
                              const float* d_disp, const float* d_dist, float radius, float* grad);

@@ -15,6 +15,7 @@
 
 namespace oct {
 
+template <int D>
 __global__ void __launch_bounds__(256)
 k_pullback(int64_t n_edges, const int32_t* __restrict__ senders, const int32_t* __restrict__ receivers,
            const float* __restrict__ rdisp, const float* __restrict__ rdist,
@@ -22,29 +23,29 @@ k_pullback(int64_t n_edges, const int32_t* __restrict__ senders, const int32_t* 
            float* __restrict__ grad) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = k < n_edges;
-    float v[3] = {0.f, 0.f, 0.f};
+    float v[D] = {};
     int32_t i = -1, j = -1;
     if (valid) {
         i = receivers[k] - index_base;
         j = senders[k] - index_base;
         if (d_disp) {
-            for (int d = 0; d < 3; ++d) v[d] = d_disp[3 * k + d] / r;
+            for (int d = 0; d < D; ++d) v[d] = d_disp[D * k + d] / r;
         }
         if (d_dist) {
             const float dn = rdist[k];
             if (dn > 1e-8f) {
                 const float inv_dnr = d_dist[k] / (dn * r);
-                for (int d = 0; d < 3; ++d) v[d] = v[d] + rdisp[3 * k + d] * inv_dnr;
+                for (int d = 0; d < D; ++d) v[d] = v[d] + rdisp[D * k + d] * inv_dnr;
             }
         }
-        for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (int64_t)j + d], -v[d]);
+        for (int d = 0; d < D; ++d) atomicAdd(&grad[D * (int64_t)j + d], -v[d]);
     }
     // receiver side: segmented sum over runs of equal receivers within the wave
     const int lane = lane_id();
     const int prev_i = __shfl_up(i, 1, OCT_WAVE);
     const bool head = valid && (lane == 0 || prev_i != i);
     const uint64_t heads = ballot(head);
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < D; ++d) {
         float s = v[d];
         // inclusive segmented scan: add from lane-o when no head lies in (lane-o, lane]
         for (int o = 1; o < OCT_WAVE; o <<= 1) {
@@ -58,7 +59,7 @@ k_pullback(int64_t n_edges, const int32_t* __restrict__ senders, const int32_t* 
     const bool next_valid = __shfl_down((int)valid, 1, OCT_WAVE);
     const bool tail = valid && (lane == OCT_WAVE - 1 || !next_valid || next_i != i);
     if (tail)
-        for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (int64_t)i + d], v[d]);
+        for (int d = 0; d < D; ++d) atomicAdd(&grad[D * (int64_t)i + d], v[d]);
 }
 
 // ---- pull form: no per-edge atomics on the sender side --------------------
@@ -92,21 +93,23 @@ k_pull_runs(int64_t n_edges, const int32_t* __restrict__ receivers, int index_ba
     if (k == n_edges - 1 || receivers[k + 1] != i) run_end[i - index_base] = (int32_t)(k + 1);
 }
 
+template <int D>
 __device__ inline void pull_value(int64_t k, const float* __restrict__ rdisp, const float* __restrict__ rdist,
                                   const float* __restrict__ d_disp, const float* __restrict__ d_dist, float r,
-                                  float (&v)[3]) {
-    v[0] = v[1] = v[2] = 0.f;
+                                  float (&v)[D]) {
+    for (int d = 0; d < D; ++d) v[d] = 0.f;
     if (d_disp)
-        for (int d = 0; d < 3; ++d) v[d] = d_disp[3 * k + d] / r;
+        for (int d = 0; d < D; ++d) v[d] = d_disp[D * k + d] / r;
     if (d_dist) {
         const float dn = rdist[k];
         if (dn > 1e-8f) {
             const float inv_dnr = d_dist[k] / (dn * r);
-            for (int d = 0; d < 3; ++d) v[d] = v[d] + rdisp[3 * k + d] * inv_dnr;
+            for (int d = 0; d < D; ++d) v[d] = v[d] + rdisp[D * k + d] * inv_dnr;
         }
     }
 }
 
+template <int D>
 __global__ void __launch_bounds__(256)
 k_pullback_pull(int64_t n_edges, const int32_t* __restrict__ senders, const int32_t* __restrict__ receivers,
                 const float* __restrict__ rdisp, const float* __restrict__ rdist, const float* __restrict__ d_disp,
@@ -115,13 +118,13 @@ k_pullback_pull(int64_t n_edges, const int32_t* __restrict__ senders, const int3
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = k < n_edges;
     const bool runs_ok = *flag == 0u;
-    float v[3] = {0.f, 0.f, 0.f};
+    float v[D] = {};
     int32_t i = -1;
     if (valid) {
         i = receivers[k] - index_base;
         const int32_t j = senders[k] - index_base;
-        float w[3];
-        pull_value(k, rdisp, rdist, d_disp, d_dist, r, w);
+        float w[D];
+        pull_value<D>(k, rdisp, rdist, d_disp, d_dist, r, w);
         int64_t kr = -1;
         if (runs_ok) {
             const int32_t si = run_start[i], sj = run_start[j], ej = run_end[j];
@@ -153,13 +156,13 @@ k_pullback_pull(int64_t n_edges, const int32_t* __restrict__ senders, const int3
             }
         }
         if (kr >= 0) {  // the partner's sender term belongs to i: pull it
-            float u[3];
-            pull_value(kr, rdisp, rdist, d_disp, d_dist, r, u);
-            for (int d = 0; d < 3; ++d) v[d] = w[d] - u[d];
+            float u[D];
+            pull_value<D>(kr, rdisp, rdist, d_disp, d_dist, r, u);
+            for (int d = 0; d < D; ++d) v[d] = w[d] - u[d];
         } else {  // no partner: push this edge's sender term to j
-            for (int d = 0; d < 3; ++d) {
+            for (int d = 0; d < D; ++d) {
                 v[d] = w[d];
-                atomicAdd(&grad[3 * (int64_t)j + d], -w[d]);
+                atomicAdd(&grad[D * (int64_t)j + d], -w[d]);
             }
         }
     }
@@ -168,7 +171,7 @@ k_pullback_pull(int64_t n_edges, const int32_t* __restrict__ senders, const int3
     const int prev_i = __shfl_up(i, 1, OCT_WAVE);
     const bool head = valid && (lane == 0 || prev_i != i);
     const uint64_t heads = ballot(head);
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < D; ++d) {
         float s = v[d];
         for (int o = 1; o < OCT_WAVE; o <<= 1) {
             const float w = __shfl_up(s, o, OCT_WAVE);
@@ -181,7 +184,7 @@ k_pullback_pull(int64_t n_edges, const int32_t* __restrict__ senders, const int3
     const bool next_valid = __shfl_down((int)valid, 1, OCT_WAVE);
     const bool tail = valid && (lane == OCT_WAVE - 1 || !next_valid || next_i != i);
     if (tail)
-        for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (int64_t)i + d], v[d]);
+        for (int d = 0; d < D; ++d) atomicAdd(&grad[D * (int64_t)i + d], v[d]);
 }
 
 __global__ void k_zero_f32(float* p, int64_t n) {
@@ -191,17 +194,18 @@ __global__ void k_zero_f32(float* p, int64_t n) {
 
 int64_t pullback_scratch_words(int64_t n_points) { return 3 * n_points + 64; }
 
-hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* senders,
-                           const int32_t* receivers, const float* rdisp, const float* rdist,
-                           const float* d_disp, const float* d_dist, float r, int index_base,
-                           float* grad, uint32_t* scratch, hipStream_t s) {
+template <int D>
+static hipError_t launch_pullback_d(int64_t n_points, int64_t n_edges, const int32_t* senders,
+                                    const int32_t* receivers, const float* rdisp, const float* rdist,
+                                    const float* d_disp, const float* d_dist, float r, int index_base, float* grad,
+                                    uint32_t* scratch, hipStream_t s) {
     if (n_points > 0)
-        hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((3 * n_points + 255) / 256)), dim3(256), 0, s, grad,
-                           3 * n_points);
+        hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((D * n_points + 255) / 256)), dim3(256), 0, s, grad,
+                           D * n_points);
     if (n_edges <= 0 || !(d_disp || d_dist)) return hipGetLastError();
     const unsigned g = (unsigned)((n_edges + 255) / 256);
     if (!scratch) {
-        hipLaunchKernelGGL(k_pullback, dim3(g), dim3(256), 0, s, n_edges, senders, receivers, rdisp, rdist, d_disp,
+        hipLaunchKernelGGL(k_pullback<D>, dim3(g), dim3(256), 0, s, n_edges, senders, receivers, rdisp, rdist, d_disp,
                            d_dist, r, index_base, grad);
         return hipGetLastError();
     }
@@ -213,9 +217,22 @@ hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* sen
     hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, (float*)scratch, words);
     hipLaunchKernelGGL(k_pull_runs, dim3(g), dim3(256), 0, s, n_edges, receivers, index_base, run_start, run_end,
                        run_count, flag);
-    hipLaunchKernelGGL(k_pullback_pull, dim3(g), dim3(256), 0, s, n_edges, senders, receivers, rdisp, rdist, d_disp,
-                       d_dist, r, index_base, run_start, run_end, flag, grad);
+    hipLaunchKernelGGL(k_pullback_pull<D>, dim3(g), dim3(256), 0, s, n_edges, senders, receivers, rdisp, rdist,
+                       d_disp, d_dist, r, index_base, run_start, run_end, flag, grad);
     return hipGetLastError();
+}
+
+hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* senders,
+                           const int32_t* receivers, const float* rdisp, const float* rdist,
+                           const float* d_disp, const float* d_dist, float r, int index_base,
+                           float* grad, uint32_t* scratch, int dims, hipStream_t s) {
+    if (dims == 2)
+        return launch_pullback_d<2>(n_points, n_edges, senders, receivers, rdisp, rdist, d_disp, d_dist, r,
+                                    index_base, grad, scratch, s);
+    if (dims == 3)
+        return launch_pullback_d<3>(n_points, n_edges, senders, receivers, rdisp, rdist, d_disp, d_dist, r,
+                                    index_base, grad, scratch, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace oct

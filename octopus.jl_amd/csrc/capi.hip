@@ -75,13 +75,16 @@ struct SetState {
     int cur = 0;
     oct_build_info info{};
     DevBuf rk, ri, rsk, rsi, rpos;  // refit: changed points
+    DevBuf pad;                     // 2D sets: the (x, y, PLANE_Z) copy the build reads (planar.hip)
+    DevBuf qfirst, qlast, qkids, qbmin, qbmax;  // 2D sets: the quadtree in the reference layout
     DevBuf soff;                    // {0, n}: the one-set forest descriptor
     int64_t soff_n = -1;            // the n soff holds
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
         nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax, rslots, ranc;
     // the buffers oct_get_tree and oct_device_view hand out
     void export_to(std::vector<void*>* r) {
-        DevBuf* out[] = {&perm, &nfirst, &nlast, &nchild, &bmin, &bmax, &nodes, &scoord};
+        DevBuf* out[] = {&perm, &nfirst, &nlast, &nchild, &bmin, &bmax, &nodes, &scoord, &pad,
+                         &qfirst, &qlast, &qkids, &qbmin, &qbmax};
         for (DevBuf* d : out) d->retire = r;
     }
     BuildBufs bufs() const {
@@ -134,7 +137,8 @@ struct SetState {
         drop_graphs();
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
                          &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
-                         &rslots, &ranc, &rk, &ri, &rsk, &rsi, &rpos, &soff, &csrc};
+                         &rslots, &ranc, &rk, &ri, &rsk, &rsi, &rpos, &soff, &csrc, &pad, &qfirst, &qlast,
+                         &qkids, &qbmin, &qbmax};
         for (DevBuf* d : all) d->release();
         soff_n = -1;
     }
@@ -157,6 +161,7 @@ struct PairState {
     DevBuf eprm;  // the query tree's build parameters with the leaf count clamped to the rows
     // CSR neighbour lists (materialize_all_neighbors!): offsets (n_q + 1) and targets
     DevBuf csr_off, csr_flat;
+    DevBuf rdisp2;  // 2D contexts: rel_displacement (2, E)
     bool csr_ok = false;
     int64_t csr_nq = 0, csr_total = 0;
     bool heavy_on = true;  // count heavy leaves ahead: first call, and while many leaves are not recorded
@@ -169,13 +174,13 @@ struct PairState {
     oct_edge_stats st{};
     // the buffers oct_build_edges, oct_batch_run and oct_edge_stats_get hand out
     void export_to(std::vector<void*>* r) {
-        DevBuf* out[] = {&senders, &receivers, &rdisp, &rdist, &feat, &pred, &lhits, &csr_off, &csr_flat};
+        DevBuf* out[] = {&senders, &receivers, &rdisp, &rdist, &feat, &pred, &lhits, &csr_off, &csr_flat, &rdisp2};
         for (DevBuf* d : out) d->retire = r;
     }
     void release_all() {
         DevBuf* all[] = {&senders, &receivers, &rdisp, &rdist, &lists, &lcount, &qcount, &qoff, &scan_tmp, &flags,
                          &tiles, &ldir, &feat, &pred, &lhits, &pwork, &pscan, &wave_first, &heavy, &eprm,
-                         &csr_off, &csr_flat};
+                         &csr_off, &csr_flat, &rdisp2};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -271,6 +276,7 @@ struct oct_ctx {
     hipEvent_t join[2] = {nullptr, nullptr};  // oct_stream_join: caller -> context, context -> caller
     hipStream_t cap_stream = nullptr;          // graph captures (the caller's stream may be the legacy default)
     int index_base = 0;
+    int ndims = 3;  // 2: quadtree sets, built as planar octrees (planar.hip)
     float radius = 0.f;
     int leaf_size = 32;
     bool refit_mode = false;
@@ -541,6 +547,13 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
     HIP_TRY(c, s.rslots.ensure(4 * (size_t)refit_slot_words(n)));
     HIP_TRY(c, s.ranc.ensure(4 * (size_t)OCT_MAX_LEVEL * (size_t)n));
     const BuildBufs b = s.bufs();
+    // a 2D set is built from its (x, y, PLANE_Z) copy
+    const float* src = s.coords;
+    if (c->ndims == 2) {
+        HIP_TRY(c, s.pad.ensure(12 * (size_t)n));
+        HIP_TRY(c, launch_pad_planar(s.coords, n, s.pad.as<float>(), c->stream));
+        src = s.pad.as<float>();
+    }
     // almost-sorted refit: the previous sort of the same n points is reused.
     // It runs without a host wait: the device decides whether it can
     // (prm->refit_fail), and whoever reads the build parameters back settles
@@ -580,7 +593,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
         {
             {
                 Phase ph(c, OCT_PROF_BOUNDS);
-                HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, nullptr, b, c->stream,
+                HIP_TRY(c, launch_bounds_stage(src, n, c->radius, nullptr, b, c->stream,
                                                s.csrc.as<const float*>()));
             }
             // the graph's key: everything its launches bake in
@@ -615,12 +628,12 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
     } else {
         {
             Phase ph(c, OCT_PROF_BOUNDS);
-            HIP_TRY(c, launch_bounds_stage(s.coords, n, c->radius, nullptr, b, c->stream));
+            HIP_TRY(c, launch_bounds_stage(src, n, c->radius, nullptr, b, c->stream));
         }
         Phase ph(c, OCT_PROF_BUILD);
         HIP_TRY_RC(read_params(c, s));  // the key width sizes the radix passes
         s.key_bits = s.hp.key_bits;
-        HIP_TRY(c, launch_sort_stage(s.coords, n, s.key_bits, c->index_base, b, &cur, c->stream));
+        HIP_TRY(c, launch_sort_stage(src, n, s.key_bits, c->index_base, b, &cur, c->stream));
         HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, cur, b, f, c->stream));
     }
     // no wait here: the tree counts come back with the next wait that needs them
@@ -649,12 +662,13 @@ extern "C" {
 int oct_create(oct_ctx** out, int device, int ndims, int index_base) {
     if (!out) return OCT_ERR_ARGUMENT;
     *out = nullptr;
-    if (ndims != 3) return OCT_ERR_ARGUMENT;
+    if (ndims != 3 && ndims != 2) return OCT_ERR_ARGUMENT;
     if (index_base != 0 && index_base != 1) return OCT_ERR_ARGUMENT;
     if (device < 0 || device >= OCT_MAX_DEVICES) return OCT_ERR_ARGUMENT;
     oct_ctx* c = new oct_ctx();
     c->device = device;
     c->index_base = index_base;
+    c->ndims = ndims;
     if (hipSetDevice(device) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
     if (noalloc_scratch_setup(device) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
@@ -841,6 +855,29 @@ int oct_get_tree(oct_ctx* c, int id, oct_tree* out) {
     out->n_nodes = s.hp.n_nodes;
     out->n_leaves = s.hp.n_leaves;
     for (int d = 0; d < 3; ++d) out->origin[d] = s.hp.origin[d];
+    if (c->ndims == 2) {  // the reference's quadtree: a one-child chain over the planar octree
+        if (s.hp.cmax > 0x1fffffu)
+            return fail(c, OCT_ERR_CAPACITY, "2D trees are exported for extents below 2^21 cells (radii) per axis");
+        const int32_t chain = s.n > (int64_t)s.built_leaf ? PLANAR_CHAIN : 0;
+        const int64_t m2 = (int64_t)s.hp.n_nodes + chain;
+        HIP_TRY(c, hipSetDevice(c->device));
+        HIP_TRY(c, s.qfirst.ensure(4 * (size_t)m2));
+        HIP_TRY(c, s.qlast.ensure(4 * (size_t)m2));
+        HIP_TRY(c, s.qkids.ensure(16 * (size_t)m2));
+        HIP_TRY(c, s.qbmin.ensure(8 * (size_t)m2));
+        HIP_TRY(c, s.qbmax.ensure(8 * (size_t)m2));
+        HIP_TRY(c, launch_quadtree(s.nfirst.as<int32_t>(), s.nlast.as<int32_t>(), s.nchild.as<int32_t>(),
+                                   s.bmin.as<float>(), s.bmax.as<float>(), s.hp.n_nodes, chain, s.n, c->index_base,
+                                   s.qfirst.as<int32_t>(), s.qlast.as<int32_t>(), s.qkids.as<int32_t>(),
+                                   s.qbmin.as<float>(), s.qbmax.as<float>(), c->stream));
+        out->node_bounds_min = s.qbmin.as<float>();
+        out->node_bounds_max = s.qbmax.as<float>();
+        out->node_first = s.qfirst.as<int32_t>();
+        out->node_last = s.qlast.as<int32_t>();
+        out->node_children = s.qkids.as<int32_t>();
+        out->n_nodes = (int32_t)m2;
+        out->origin[2] = 0.f;
+    }
     return OCT_OK;
 }
 
@@ -1201,6 +1238,11 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, int emit, o
     out->rel_displacement = eo.rdisp;
     out->rel_dist_norm = eo.rdist;
     out->n_edges = E;
+    if (c->ndims == 2) {  // rel_displacement (2, E): the z column of the planar build dropped
+        HIP_TRY(c, p.rdisp2.ensure(8 * (size_t)std::max<int64_t>(E, 1)));
+        HIP_TRY(c, launch_disp_planar(eo.rdisp, E, p.rdisp2.as<float>(), c->stream));
+        out->rel_displacement = p.rdisp2.as<float>();
+    }
     return OCT_OK;
 }
 
@@ -1244,8 +1286,9 @@ int oct_device_view(oct_ctx* c, int q, int t, oct_view* out) {
     int rc = ensure_hp(c, ts);
     if (rc != OCT_OK) return rc;
     memset(out, 0, sizeof(*out));
-    out->coords_q = qs.coords;
-    out->coords_t = ts.coords;
+    // 2D: the padded (x, y, PLANE_Z) copies, the layout the device iterator reads
+    out->coords_q = c->ndims == 2 ? qs.pad.as<float>() : qs.coords;
+    out->coords_t = c->ndims == 2 ? ts.pad.as<float>() : ts.coords;
     out->bounds_min = ts.bmin.as<float>();
     out->bounds_max = ts.bmax.as<float>();
     out->node_first = ts.nfirst.as<int32_t>();
@@ -1325,7 +1368,8 @@ int oct_build_edges_backward(oct_ctx* c, int64_t n_points, const oct_edges* e, c
     }
     Phase ph(c, OCT_PROF_BACKWARD);
     HIP_TRY(c, launch_pullback(n_points, e->n_edges, e->senders, e->receivers, e->rel_displacement,
-                               e->rel_dist_norm, d_disp, d_dist, radius, c->index_base, grad, scratch, c->stream));
+                               e->rel_dist_norm, d_disp, d_dist, radius, c->index_base, grad, scratch, c->ndims,
+                               c->stream));
     return OCT_OK;
 }
 
@@ -1781,6 +1825,7 @@ int batch_eager(oct_ctx* c, BatchState& B) {
 int oct_batch_create(oct_ctx* c, int n_sets, const float* coords, const int64_t* offsets, const float* radii,
                      int* batch_id) {
     if (!c || !batch_id || !offsets || !radii) return OCT_ERR_ARGUMENT;
+    if (c->ndims != 3) return fail(c, OCT_ERR_ARGUMENT, "batches hold 3D point sets");
     if (n_sets < 1 || n_sets > (1 << 20)) return fail(c, OCT_ERR_ARGUMENT, "batch: n_sets must be in 1..2^20");
     if (offsets[0] != 0) return fail(c, OCT_ERR_ARGUMENT, "batch: offsets[0] must be 0");
     for (int s = 0; s < n_sets; ++s) {

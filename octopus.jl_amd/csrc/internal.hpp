@@ -350,6 +350,15 @@ hipError_t launch_batch_edge_offsets(const uint32_t* loff, const uint64_t* total
                                      const BuildParams* gprm, const int64_t* off, int S, int64_t* eoff,
                                      hipStream_t s);
 
+// 2D sets on the 3D engine (see planar.hip): z = PLANE_Z for every point.
+constexpr float PLANE_Z = 1048576.0f;  // 2^20
+constexpr int PLANAR_CHAIN = 10;       // quadtree levels above the octree's root (31 - 21)
+hipError_t launch_pad_planar(const float* xy, int64_t n, float* xyz, hipStream_t s);
+hipError_t launch_disp_planar(const float* d3, int64_t n_edges, float* d2, hipStream_t s);
+hipError_t launch_quadtree(const int32_t* ofirst, const int32_t* olast, const int32_t* okids, const float* obmin,
+                           const float* obmax, int32_t m3, int32_t chain, int64_t n, int index_base, int32_t* qfirst,
+                           int32_t* qlast, int32_t* qkids, float* qbmin, float* qbmax, hipStream_t s);
+
 // apply_zsort gather (scatter = its adjoint) of n columns of `width`
 // elements of elem_bytes (1, 2, 4 or 8) bytes (see zsort.hip).
 hipError_t launch_zsort(const void* src, void* dst, const int32_t* perm, int64_t n, int64_t width, int elem_bytes,
@@ -363,7 +372,7 @@ int64_t pullback_scratch_words(int64_t n_points);
 hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* senders,
                            const int32_t* receivers, const float* rdisp, const float* rdist,
                            const float* d_disp, const float* d_dist, float r, int index_base,
-                           float* grad, uint32_t* scratch, hipStream_t s);
+                           float* grad, uint32_t* scratch, int dims, hipStream_t s);
 
 int64_t scan_tmp_words(int64_t n);
 // Tiles of the device scan (SCAN_TILE_INPUTS inputs each).

@@ -102,16 +102,17 @@ _current_stream() = nothing
 
 mutable struct TNS
     ctx::Ptr{Cvoid}
+    ndims::Int                   # 3 (octrees) or 2 (quadtrees)
     radius::Float32
     coords::Vector{Any}          # keeps user arrays alive (referenced, not copied)
     dirty::Vector{Bool}
     active_pairs::Vector{Tuple{Int32,Int32}}
     function TNS(::Type{Float32}=Float32; ndims::Int=3, device::Int=0)
-        ndims == 3 || throw(ArgumentError("the HIP engine builds octrees (ndims = 3)"))
+        ndims in (2, 3) || throw(ArgumentError("ndims must be 2 or 3"))
         h = Ref{Ptr{Cvoid}}(C_NULL)
-        rc = ccall((:oct_create, libocto), Cint, (Ptr{Ptr{Cvoid}}, Cint, Cint, Cint), h, device, 3, 1)
+        rc = ccall((:oct_create, libocto), Cint, (Ptr{Ptr{Cvoid}}, Cint, Cint, Cint), h, device, ndims, 1)
         rc == OCT_OK || error("oct_create failed ($rc)")
-        t = new(h[], 0f0, Any[], Bool[], Tuple{Int32,Int32}[])
+        t = new(h[], ndims, 0f0, Any[], Bool[], Tuple{Int32,Int32}[])
         # every device array handed out keeps `t` reachable (see _device_wrap),
         # so the context is destroyed only after the last of them is gone
         finalizer(x -> ccall((:oct_destroy, libocto), Cint, (Ptr{Cvoid},), x.ctx), t)
@@ -177,10 +178,11 @@ function set_leaf_size!(t::TNS, leaf_size::Integer)
     return t
 end
 
-_npoints(coords) = (size(coords, 1) == 3 || throw(DimensionMismatch("coords must be (3, N)")); size(coords, 2))
+_npoints(t::TNS, coords) =
+    (size(coords, 1) == t.ndims || throw(DimensionMismatch("coords must be ($(t.ndims), N)")); size(coords, 2))
 
 function add_point_set!(t::TNS, coords::AbstractMatrix{Float32})
-    n = _npoints(coords)
+    n = _npoints(t, coords)
     id = Ref{Cint}(0)
     _check(t, ccall((:oct_add_point_set, libocto), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int64, Ptr{Cint}),
                     t.ctx, pointer(coords), n, id))
@@ -190,7 +192,7 @@ function add_point_set!(t::TNS, coords::AbstractMatrix{Float32})
 end
 
 function update_point_set!(t::TNS, id::Integer, coords::AbstractMatrix{Float32})
-    n = _npoints(coords)
+    n = _npoints(t, coords)
     _check(t, ccall((:oct_update_point_set, libocto), Cint, (Ptr{Cvoid}, Cint, Ptr{Float32}, Int64),
                     t.ctx, id, pointer(coords), n))
     t.coords[id] = coords
@@ -199,7 +201,7 @@ function update_point_set!(t::TNS, id::Integer, coords::AbstractMatrix{Float32})
 end
 
 function resize_point_set!(t::TNS, id::Integer, coords::AbstractMatrix{Float32})
-    n = _npoints(coords)
+    n = _npoints(t, coords)
     _check(t, ccall((:oct_resize_point_set, libocto), Cint, (Ptr{Cvoid}, Cint, Ptr{Float32}, Int64),
                     t.ctx, id, pointer(coords), n))
     t.coords[id] = coords
@@ -230,16 +232,17 @@ function get_tree!(t::TNS, id::Integer)
 end
 
 """The tree of set `id` in the reference layout (src/types.jl:23-42) as device
-arrays: node_bounds_min/max (3, M), node_first/last (M), node_children (8, M),
-permutation (N); valid until the next run!."""
+arrays: node_bounds_min/max (D, M), node_first/last (M), node_children
+(2^D, M), permutation (N); valid until the next run!."""
 function get_tree(t::TNS, id::Integer)
     x = get_tree!(t, id)
     m, n = Int(x.n_nodes), Int(x.n_points)
-    return (node_bounds_min = _device_wrap(x.node_bounds_min, (3, m), t),
-            node_bounds_max = _device_wrap(x.node_bounds_max, (3, m), t),
+    d = t.ndims
+    return (node_bounds_min = _device_wrap(x.node_bounds_min, (d, m), t),
+            node_bounds_max = _device_wrap(x.node_bounds_max, (d, m), t),
             node_first = _device_wrap(x.node_first, (m,), t),
             node_last = _device_wrap(x.node_last, (m,), t),
-            node_children = _device_wrap(x.node_children, (8, m), t),
+            node_children = _device_wrap(x.node_children, (1 << d, m), t),
             permutation = _device_wrap(x.permutation, (n,), t),
             origin = x.origin)
 end
@@ -274,7 +277,7 @@ end
 """
     build_edges(tns, q, t; mode=:exact, order=...) -> NamedTuple
 
-`(senders, receivers, rel_displacement (3, E), rel_dist_norm (1, E))` as
+`(senders, receivers, rel_displacement (D, E), rel_dist_norm (1, E))` as
 device arrays aliasing the pair's buffers, as the reference's NamedTuple
 aliases its EdgeBuffer (src/api.jl:329-337, test/contracts.jl:223-240): the
 next build on the pair overwrites them.
@@ -284,7 +287,7 @@ function build_edges(t::TNS, q::Integer, s::Integer; kw...)
     n = Int(e.n_edges)
     return (senders = _device_wrap(e.senders, (n,), t),
             receivers = _device_wrap(e.receivers, (n,), t),
-            rel_displacement = _device_wrap(e.rel_displacement, (3, n), t),
+            rel_displacement = _device_wrap(e.rel_displacement, (t.ndims, n), t),
             rel_dist_norm = _device_wrap(e.rel_dist_norm, (1, n), t))
 end
 

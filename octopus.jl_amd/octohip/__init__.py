@@ -92,8 +92,8 @@ class TNS:
         dtype = torch.float32 if dtype is None else dtype
         if dtype != torch.float32:
             raise ArgumentError("the HIP engine computes in Float32 only")
-        if ndims != 3:
-            raise ArgumentError("the HIP engine builds octrees (ndims = 3)")
+        if ndims not in (2, 3):
+            raise ArgumentError("ndims must be 2 (quadtrees) or 3 (octrees)")
         if index_base not in (0, 1):
             raise ArgumentError("index_base must be 0 or 1")
         self.dtype = dtype
@@ -316,15 +316,15 @@ def get_tree(tns: TNS, set_id: int) -> Tree:
     _sid(tns, set_id)
     t = L.OctTree()
     tns._check(tns._lib.oct_get_tree(tns._c(), int(set_id), ctypes.byref(t)))
-    m, n, dev = int(t.n_nodes), int(t.n_points), tns.device
+    m, n, dev, d = int(t.n_nodes), int(t.n_points), tns.device, tns.ndims
     return Tree(
-        node_bounds_min=_wrap(t.node_bounds_min, (m, 3), torch.float32, dev, tns),
-        node_bounds_max=_wrap(t.node_bounds_max, (m, 3), torch.float32, dev, tns),
+        node_bounds_min=_wrap(t.node_bounds_min, (m, d), torch.float32, dev, tns),
+        node_bounds_max=_wrap(t.node_bounds_max, (m, d), torch.float32, dev, tns),
         node_first=_wrap(t.node_first, (m,), torch.int32, dev, tns),
         node_last=_wrap(t.node_last, (m,), torch.int32, dev, tns),
-        node_children=_wrap(t.node_children, (m, 8), torch.int32, dev, tns),
+        node_children=_wrap(t.node_children, (m, 1 << d), torch.int32, dev, tns),
         permutation=_wrap(t.permutation, (n,), torch.int32, dev, tns),
-        n_nodes=m, n_leaves=int(t.n_leaves), origin=tuple(t.origin),
+        n_nodes=m, n_leaves=int(t.n_leaves), origin=tuple(t.origin)[:d],
     )
 
 
@@ -359,7 +359,7 @@ def build_edges(tns: TNS, q: int, t: int, mode: str = "exact", order: str = "lea
     return Edges(
         senders=_wrap(e.senders, (n,), torch.int32, dev, tns),
         receivers=_wrap(e.receivers, (n,), torch.int32, dev, tns),
-        rel_displacement=_wrap(e.rel_displacement, (n, 3), torch.float32, dev, tns),
+        rel_displacement=_wrap(e.rel_displacement, (n, tns.ndims), torch.float32, dev, tns),
         rel_dist_norm=_wrap(e.rel_dist_norm, (n, 1), torch.float32, dev, tns),
     )
 
@@ -614,7 +614,7 @@ def apply_zsort(tns: TNS, set_id: int, user_array):
 
 def _pullback(tns: TNS, n_points: int, senders, receivers, rdisp, rdist, d_disp, d_dist, radius):
     torch = _torch()
-    grad = torch.empty((n_points, 3), dtype=torch.float32, device=tns.device)
+    grad = torch.empty((n_points, tns.ndims), dtype=torch.float32, device=tns.device)
     e = L.OctEdges()
     e.senders = senders.data_ptr() if senders.numel() else None
     e.receivers = receivers.data_ptr() if receivers.numel() else None

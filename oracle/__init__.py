@@ -41,6 +41,24 @@ class _Tree(ctypes.Structure):
     ]
 
 
+class _Tree2(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int64),
+        ("radius", ctypes.c_float),
+        ("leaf_size", ctypes.c_int32),
+        ("n_nodes", ctypes.c_int32),
+        ("origin", ctypes.c_float * 2),
+        ("keys", _u64p),
+        ("perm", _i32p),
+        ("node_first", _i32p),
+        ("node_last", _i32p),
+        ("node_level", _i32p),
+        ("node_children", _i32p),
+        ("bmin", _f32p),
+        ("bmax", _f32p),
+    ]
+
+
 class _Edges(ctypes.Structure):
     _fields_ = [
         ("nq", ctypes.c_int64),
@@ -101,6 +119,18 @@ def lib():
         L.orc_pullback.argtypes = [ctypes.c_int64, ctypes.c_int64, _i32p, _i32p, _f32p, _f32p,
                                    _f32p, _f32p, ctypes.c_float, _f32p]
         L.orc_max_threads.restype = ctypes.c_int
+        L.orc_morton_encode2.restype = ctypes.c_uint64
+        L.orc_morton_encode2.argtypes = [ctypes.c_uint32] * 2
+        L.orc_morton_decode2.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32)]
+        L.orc_bin_point2.argtypes = [ctypes.c_float] * 5 + [_i32p]
+        L.orc_run2.restype = ctypes.POINTER(_Tree2)
+        L.orc_run2.argtypes = [_f32p, ctypes.c_int64, ctypes.c_float, ctypes.c_int32]
+        L.orc_tree2_free.argtypes = [ctypes.POINTER(_Tree2)]
+        L.orc_build_edges2.restype = ctypes.POINTER(_Edges)
+        L.orc_build_edges2.argtypes = [ctypes.POINTER(_Tree2), _f32p, ctypes.c_int64, _f32p, ctypes.c_int,
+                                       ctypes.c_int]
+        L.orc_pullback2.argtypes = [ctypes.c_int64, ctypes.c_int64, _i32p, _i32p, _f32p, _f32p, _f32p, _f32p,
+                                    ctypes.c_float, _f32p]
         _lib = L
     return _lib
 
@@ -356,3 +386,110 @@ def pullback(n_points: int, senders, receivers, rdisp, rdist, d_disp, d_dist, r:
 
 def max_threads() -> int:
     return int(lib().orc_max_threads())
+
+
+# ------------------------------------------------------------------------ 2D
+
+
+def _coords2(c) -> np.ndarray:
+    c = np.ascontiguousarray(c, dtype=np.float32)
+    if c.ndim != 2 or c.shape[1] != 2:
+        raise ValueError("coords must be (N, 2) float32, one point per row (xy interleaved)")
+    return c
+
+
+def morton_encode2(ix: int, iy: int) -> int:
+    return int(lib().orc_morton_encode2(ix, iy))
+
+
+def morton_decode2(code: int):
+    out = (ctypes.c_uint32 * 2)()
+    lib().orc_morton_decode2(code, out)
+    return tuple(int(v) for v in out)
+
+
+def bin_point2(p, o, inv_cs):
+    out = (ctypes.c_int32 * 2)()
+    lib().orc_bin_point2(*(float(v) for v in p), *(float(v) for v in o), float(inv_cs), out)
+    return tuple(int(v) for v in out)
+
+
+@dataclass
+class Tree2:
+    n: int
+    radius: float
+    leaf_size: int
+    origin: np.ndarray
+    keys: np.ndarray
+    perm: np.ndarray
+    node_first: np.ndarray
+    node_last: np.ndarray
+    node_level: np.ndarray
+    node_children: np.ndarray  # (M, 4)
+    bmin: np.ndarray  # (M, 2)
+    bmax: np.ndarray  # (M, 2)
+    _handle: object = None
+    _coords: object = None
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.node_first.shape[0])
+
+    def __del__(self):
+        if self._handle is not None and _lib is not None:
+            _lib.orc_tree2_free(self._handle)
+            self._handle = None
+
+
+def run2(coords, radius: float, leaf_size: int = 32) -> Tree2:
+    """run! for one 2D point set (quadtree)."""
+    c = _coords2(coords)
+    h = lib().orc_run2(_ptr(c, _f32p), c.shape[0], float(radius), int(leaf_size))
+    t = h.contents
+    n, m = int(t.n), int(t.n_nodes)
+    return Tree2(
+        n=n, radius=float(np.float32(radius)), leaf_size=int(leaf_size),
+        origin=np.array(list(t.origin), np.float32),
+        keys=_arr(t.keys, n, np.uint64), perm=_arr(t.perm, n, np.int32),
+        node_first=_arr(t.node_first, m, np.int32), node_last=_arr(t.node_last, m, np.int32),
+        node_level=_arr(t.node_level, m, np.int32),
+        node_children=_arr(t.node_children, 4 * m, np.int32, (m, 4)),
+        bmin=_arr(t.bmin, 2 * m, np.float32, (m, 2)), bmax=_arr(t.bmax, 2 * m, np.float32, (m, 2)),
+        _handle=h, _coords=c,
+    )
+
+
+def build_edges2(tree: Tree2, coords_q, coords_t=None, exclude_self=None, nthreads=None) -> Edges:
+    """2D build_edges (src/cpu/edges_cpu.jl:132-170 fill): rel_displacement (E, 2)."""
+    cq = _coords2(coords_q)
+    ct = tree._coords if coords_t is None else _coords2(coords_t)
+    if exclude_self is None:
+        exclude_self = coords_t is None
+    nt = lib().orc_max_threads() if nthreads is None else int(nthreads)
+    h = lib().orc_build_edges2(tree._handle, _ptr(cq, _f32p), cq.shape[0], _ptr(ct, _f32p),
+                               int(bool(exclude_self)), nt)
+    e = h.contents
+    ne, nq = int(e.n_edges), int(e.nq)
+    out = Edges(
+        offsets=_arr(e.offsets, nq + 1, np.int64),
+        senders=_arr(e.senders, ne, np.int32), receivers=_arr(e.receivers, ne, np.int32),
+        rel_displacement=_arr(e.rdisp, 2 * ne, np.float32, (ne, 2)),
+        rel_dist_norm=_arr(e.rdist, ne, np.float32),
+    )
+    lib().orc_edges_free(h)
+    return out
+
+
+def pullback2(n_points: int, senders, receivers, rdisp, rdist, d_disp, d_dist, r: float):
+    """The CPU pullback with D = 2 (ext/OctopusChainRulesCoreExt.jl:137-192)."""
+    s = np.ascontiguousarray(senders, np.int32)
+    rc = np.ascontiguousarray(receivers, np.int32)
+    dp = np.ascontiguousarray(rdisp, np.float32).reshape(-1, 2)
+    dn = np.ascontiguousarray(rdist, np.float32).reshape(-1)
+    g = np.zeros((n_points, 2), np.float32)
+    a = None if d_disp is None else np.ascontiguousarray(d_disp, np.float32).reshape(-1, 2)
+    b = None if d_dist is None else np.ascontiguousarray(d_dist, np.float32).reshape(-1)
+    lib().orc_pullback2(n_points, s.shape[0], _ptr(s, _i32p), _ptr(rc, _i32p), _ptr(dp, _f32p), _ptr(dn, _f32p),
+                        None if a is None else _ptr(a, _f32p), None if b is None else _ptr(b, _f32p), float(r),
+                        _ptr(g, _f32p))
+    return g

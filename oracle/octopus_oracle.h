@@ -78,6 +78,39 @@ void orc_pullback(int64_t n_points, int64_t n_edges, const int32_t* senders,
 
 int orc_max_threads(void);
 
+/* ---- 2D (quadtrees): octopus_oracle2d.c -------------------------------- */
+typedef struct orc_tree2 {
+    int64_t n;
+    float radius;
+    int32_t leaf_size;
+    int32_t n_nodes;
+    float origin[2];
+    uint64_t* keys;       /* 62-bit 2D Morton key per point (input order) */
+    int32_t* perm;
+    int32_t* node_first;
+    int32_t* node_last;
+    int32_t* node_level;
+    int32_t* node_children; /* 4 per node, quadrant order, -1 = empty */
+    float* bmin;          /* 2 per node */
+    float* bmax;
+} orc_tree2;
+
+uint64_t orc_split_by_2(uint64_t x);
+uint64_t orc_compact_by_2(uint64_t x);
+uint64_t orc_morton_encode2(uint32_t ix, uint32_t iy);
+void orc_morton_decode2(uint64_t code, uint32_t out[2]);
+void orc_bin_point2(float px, float py, float ox, float oy, float inv_cs, int32_t out[2]);
+void orc_point_origin2(const float* coords, int64_t n, float origin[2]);
+void orc_bin2(const float* coords, int64_t n, const float origin[2], float cell_size, uint64_t* keys);
+orc_tree2* orc_run2(const float* coords, int64_t n, float radius, int32_t leaf_size);
+void orc_tree2_free(orc_tree2* t);
+/* edges with rdisp 2 per edge */
+orc_edges* orc_build_edges2(const orc_tree2* t, const float* coords_q, int64_t nq, const float* coords_t,
+                            int exclude_self, int nthreads);
+void orc_pullback2(int64_t n_points, int64_t n_edges, const int32_t* senders, const int32_t* receivers,
+                   const float* rdisp, const float* rdist, const float* d_disp, const float* d_dist, float r,
+                   float* grad);
+
 #ifdef __cplusplus
 }
 #endif

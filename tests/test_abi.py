@@ -62,7 +62,7 @@ def test_argument_errors_without_device():
     with pytest.raises(octohip.ArgumentError):
         octohip.run(tns)  # no point sets
     with pytest.raises(octohip.ArgumentError):
-        octohip.TNS(ndims=2)
+        octohip.TNS(ndims=4)
     with pytest.raises(octohip.OctopusError):
         octohip.for_each_neighbor(lambda j: None, tns, 0, 0, 0)
 
