@@ -99,6 +99,16 @@ typedef struct oct_view {
  * batches are 3D only).  device is a HIP ordinal.  The context creates its
  * own stream unless oct_set_stream is used. */
 int oct_create(oct_ctx** out, int device, int ndims, int index_base);
+/* TNS(Float64; ndims) (test/float64.jl): elem_bytes 8 makes every
+ * coordinate, radius, bound and feature a double (elem_bytes 4 is
+ * oct_create).  A Float64 context builds with double bounds, origin and
+ * cells, walks the tree in double one query per thread, and writes edges in
+ * receiver order; float* fields of the result structs (oct_edges features,
+ * oct_tree bounds) then point at doubles.  Its radius is set with
+ * oct_set_search_radius64, its adjoint is oct_build_edges_backward64; the
+ * learned single pass, the refit mode, batches, the device view and the
+ * predictor are Float32 only (OCT_ERR_ARGUMENT / OCT_ERR_STATE). */
+int oct_create_typed(oct_ctx** out, int device, int ndims, int index_base, int elem_bytes);
 int oct_destroy(oct_ctx* ctx);
 const char* oct_last_error(const oct_ctx* ctx);
 /* Run the context's work on this stream from now on (work already queued on
@@ -114,6 +124,8 @@ int oct_stream_join(oct_ctx* ctx, void* hip_stream, int direction);
 
 /* set_search_radius! (src/api.jl:24-29): r > 0, marks every set dirty. */
 int oct_set_search_radius(oct_ctx* ctx, float r);
+/* set_search_radius! with a double radius: T(r) of the context's type. */
+int oct_set_search_radius64(oct_ctx* ctx, double r);
 /* set_refit_mode! (src/api.jl:43-47): keep keys and perm between builds and
  * try the almost-sorted path first; marks every set dirty. */
 int oct_set_refit_mode(oct_ctx* ctx, int on);
@@ -208,6 +220,9 @@ int oct_count_neighbors_device(oct_ctx* ctx, const oct_view* view, int32_t* coun
  * zero tangent.  Edge indices carry the context's index_base. */
 int oct_build_edges_backward(oct_ctx* ctx, int64_t n_points, const oct_edges* edges,
                              const float* d_disp, const float* d_dist, float radius, float* grad);
+/* The same for a Float64 context: every array holds doubles. */
+int oct_build_edges_backward64(oct_ctx* ctx, int64_t n_points, const oct_edges* edges, const double* d_disp,
+                               const double* d_dist, double radius, double* grad);
 /* No counterpart in the reference (whose GPU adjoint pushes every term with
  * atomics, ext/OctopusCUDAChainRulesCoreExt.jl:56-99).  pull != 0 (the
  * default): oct_build_edges_backward pulls each edge's sender term from its

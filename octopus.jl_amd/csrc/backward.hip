@@ -15,15 +15,15 @@
 
 namespace oct {
 
-template <int D>
+template <int D, typename T>
 __global__ void __launch_bounds__(256)
 k_pullback(int64_t n_edges, const int32_t* __restrict__ senders, const int32_t* __restrict__ receivers,
-           const float* __restrict__ rdisp, const float* __restrict__ rdist,
-           const float* __restrict__ d_disp, const float* __restrict__ d_dist, float r, int index_base,
-           float* __restrict__ grad) {
+           const T* __restrict__ rdisp, const T* __restrict__ rdist,
+           const T* __restrict__ d_disp, const T* __restrict__ d_dist, T r, int index_base,
+           T* __restrict__ grad) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = k < n_edges;
-    float v[D] = {};
+    T v[D] = {};
     int32_t i = -1, j = -1;
     if (valid) {
         i = receivers[k] - index_base;
@@ -32,9 +32,9 @@ k_pullback(int64_t n_edges, const int32_t* __restrict__ senders, const int32_t* 
             for (int d = 0; d < D; ++d) v[d] = d_disp[D * k + d] / r;
         }
         if (d_dist) {
-            const float dn = rdist[k];
-            if (dn > 1e-8f) {
-                const float inv_dnr = d_dist[k] / (dn * r);
+            const T dn = rdist[k];
+            if (dn > T(1e-8)) {
+                const T inv_dnr = d_dist[k] / (dn * r);
                 for (int d = 0; d < D; ++d) v[d] = v[d] + rdisp[D * k + d] * inv_dnr;
             }
         }
@@ -46,10 +46,10 @@ k_pullback(int64_t n_edges, const int32_t* __restrict__ senders, const int32_t* 
     const bool head = valid && (lane == 0 || prev_i != i);
     const uint64_t heads = ballot(head);
     for (int d = 0; d < D; ++d) {
-        float s = v[d];
+        T s = v[d];
         // inclusive segmented scan: add from lane-o when no head lies in (lane-o, lane]
         for (int o = 1; o < OCT_WAVE; o <<= 1) {
-            const float w = __shfl_up(s, o, OCT_WAVE);
+            const T w = __shfl_up(s, o, OCT_WAVE);
             const uint64_t span = (lane >= o) ? (((2ull << lane) - 1) & ~((2ull << (lane - o)) - 1)) : 0ull;
             if (lane >= o && (heads & span) == 0) s = s + w;
         }
@@ -93,38 +93,37 @@ k_pull_runs(int64_t n_edges, const int32_t* __restrict__ receivers, int index_ba
     if (k == n_edges - 1 || receivers[k + 1] != i) run_end[i - index_base] = (int32_t)(k + 1);
 }
 
-template <int D>
-__device__ inline void pull_value(int64_t k, const float* __restrict__ rdisp, const float* __restrict__ rdist,
-                                  const float* __restrict__ d_disp, const float* __restrict__ d_dist, float r,
-                                  float (&v)[D]) {
-    for (int d = 0; d < D; ++d) v[d] = 0.f;
+template <int D, typename T>
+__device__ inline void pull_value(int64_t k, const T* __restrict__ rdisp, const T* __restrict__ rdist,
+                                  const T* __restrict__ d_disp, const T* __restrict__ d_dist, T r, T (&v)[D]) {
+    for (int d = 0; d < D; ++d) v[d] = T(0);
     if (d_disp)
         for (int d = 0; d < D; ++d) v[d] = d_disp[D * k + d] / r;
     if (d_dist) {
-        const float dn = rdist[k];
-        if (dn > 1e-8f) {
-            const float inv_dnr = d_dist[k] / (dn * r);
+        const T dn = rdist[k];
+        if (dn > T(1e-8)) {  // T(1e-8), as the reference's rrule
+            const T inv_dnr = d_dist[k] / (dn * r);
             for (int d = 0; d < D; ++d) v[d] = v[d] + rdisp[D * k + d] * inv_dnr;
         }
     }
 }
 
-template <int D>
+template <int D, typename T>
 __global__ void __launch_bounds__(256)
 k_pullback_pull(int64_t n_edges, const int32_t* __restrict__ senders, const int32_t* __restrict__ receivers,
-                const float* __restrict__ rdisp, const float* __restrict__ rdist, const float* __restrict__ d_disp,
-                const float* __restrict__ d_dist, float r, int index_base, const int32_t* __restrict__ run_start,
-                const int32_t* __restrict__ run_end, const uint32_t* __restrict__ flag, float* __restrict__ grad) {
+                const T* __restrict__ rdisp, const T* __restrict__ rdist, const T* __restrict__ d_disp,
+                const T* __restrict__ d_dist, T r, int index_base, const int32_t* __restrict__ run_start,
+                const int32_t* __restrict__ run_end, const uint32_t* __restrict__ flag, T* __restrict__ grad) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = k < n_edges;
     const bool runs_ok = *flag == 0u;
-    float v[D] = {};
+    T v[D] = {};
     int32_t i = -1;
     if (valid) {
         i = receivers[k] - index_base;
         const int32_t j = senders[k] - index_base;
-        float w[D];
-        pull_value<D>(k, rdisp, rdist, d_disp, d_dist, r, w);
+        T w[D];
+        pull_value<D, T>(k, rdisp, rdist, d_disp, d_dist, r, w);
         int64_t kr = -1;
         if (runs_ok) {
             const int32_t si = run_start[i], sj = run_start[j], ej = run_end[j];
@@ -156,8 +155,8 @@ k_pullback_pull(int64_t n_edges, const int32_t* __restrict__ senders, const int3
             }
         }
         if (kr >= 0) {  // the partner's sender term belongs to i: pull it
-            float u[D];
-            pull_value<D>(kr, rdisp, rdist, d_disp, d_dist, r, u);
+            T u[D];
+            pull_value<D, T>(kr, rdisp, rdist, d_disp, d_dist, r, u);
             for (int d = 0; d < D; ++d) v[d] = w[d] - u[d];
         } else {  // no partner: push this edge's sender term to j
             for (int d = 0; d < D; ++d) {
@@ -172,9 +171,9 @@ k_pullback_pull(int64_t n_edges, const int32_t* __restrict__ senders, const int3
     const bool head = valid && (lane == 0 || prev_i != i);
     const uint64_t heads = ballot(head);
     for (int d = 0; d < D; ++d) {
-        float s = v[d];
+        T s = v[d];
         for (int o = 1; o < OCT_WAVE; o <<= 1) {
-            const float w = __shfl_up(s, o, OCT_WAVE);
+            const T w = __shfl_up(s, o, OCT_WAVE);
             const uint64_t span = (lane >= o) ? (((2ull << lane) - 1) & ~((2ull << (lane - o)) - 1)) : 0ull;
             if (lane >= o && (heads & span) == 0) s = s + w;
         }
@@ -187,26 +186,27 @@ k_pullback_pull(int64_t n_edges, const int32_t* __restrict__ senders, const int3
         for (int d = 0; d < D; ++d) atomicAdd(&grad[D * (int64_t)i + d], v[d]);
 }
 
-__global__ void k_zero_f32(float* p, int64_t n) {
+template <typename T>
+__global__ void k_zero(T* p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.f;
+    if (i < n) p[i] = T(0);
 }
 
 int64_t pullback_scratch_words(int64_t n_points) { return 3 * n_points + 64; }
 
-template <int D>
+template <int D, typename T>
 static hipError_t launch_pullback_d(int64_t n_points, int64_t n_edges, const int32_t* senders,
-                                    const int32_t* receivers, const float* rdisp, const float* rdist,
-                                    const float* d_disp, const float* d_dist, float r, int index_base, float* grad,
-                                    uint32_t* scratch, hipStream_t s) {
+                                    const int32_t* receivers, const T* rdisp, const T* rdist, const T* d_disp,
+                                    const T* d_dist, T r, int index_base, T* grad, uint32_t* scratch,
+                                    hipStream_t s) {
     if (n_points > 0)
-        hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((D * n_points + 255) / 256)), dim3(256), 0, s, grad,
+        hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)((D * n_points + 255) / 256)), dim3(256), 0, s, grad,
                            D * n_points);
     if (n_edges <= 0 || !(d_disp || d_dist)) return hipGetLastError();
     const unsigned g = (unsigned)((n_edges + 255) / 256);
     if (!scratch) {
-        hipLaunchKernelGGL(k_pullback<D>, dim3(g), dim3(256), 0, s, n_edges, senders, receivers, rdisp, rdist, d_disp,
-                           d_dist, r, index_base, grad);
+        hipLaunchKernelGGL((k_pullback<D, T>), dim3(g), dim3(256), 0, s, n_edges, senders, receivers, rdisp, rdist,
+                           d_disp, d_dist, r, index_base, grad);
         return hipGetLastError();
     }
     int32_t* run_start = (int32_t*)scratch;
@@ -214,10 +214,10 @@ static hipError_t launch_pullback_d(int64_t n_points, int64_t n_edges, const int
     int32_t* run_count = run_end + n_points;
     uint32_t* flag = (uint32_t*)(run_count + n_points);
     const int64_t words = 3 * n_points + 1;
-    hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, (float*)scratch, words);
+    hipLaunchKernelGGL(k_zero<uint32_t>, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, scratch, words);
     hipLaunchKernelGGL(k_pull_runs, dim3(g), dim3(256), 0, s, n_edges, receivers, index_base, run_start, run_end,
                        run_count, flag);
-    hipLaunchKernelGGL(k_pullback_pull<D>, dim3(g), dim3(256), 0, s, n_edges, senders, receivers, rdisp, rdist,
+    hipLaunchKernelGGL((k_pullback_pull<D, T>), dim3(g), dim3(256), 0, s, n_edges, senders, receivers, rdisp, rdist,
                        d_disp, d_dist, r, index_base, run_start, run_end, flag, grad);
     return hipGetLastError();
 }
@@ -227,11 +227,23 @@ hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* sen
                            const float* d_disp, const float* d_dist, float r, int index_base,
                            float* grad, uint32_t* scratch, int dims, hipStream_t s) {
     if (dims == 2)
-        return launch_pullback_d<2>(n_points, n_edges, senders, receivers, rdisp, rdist, d_disp, d_dist, r,
-                                    index_base, grad, scratch, s);
+        return launch_pullback_d<2, float>(n_points, n_edges, senders, receivers, rdisp, rdist, d_disp, d_dist, r,
+                                           index_base, grad, scratch, s);
     if (dims == 3)
-        return launch_pullback_d<3>(n_points, n_edges, senders, receivers, rdisp, rdist, d_disp, d_dist, r,
-                                    index_base, grad, scratch, s);
+        return launch_pullback_d<3, float>(n_points, n_edges, senders, receivers, rdisp, rdist, d_disp, d_dist, r,
+                                           index_base, grad, scratch, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_pullback64(int64_t n_points, int64_t n_edges, const int32_t* senders, const int32_t* receivers,
+                             const double* rdisp, const double* rdist, const double* d_disp, const double* d_dist,
+                             double r, int index_base, double* grad, uint32_t* scratch, int dims, hipStream_t s) {
+    if (dims == 2)
+        return launch_pullback_d<2, double>(n_points, n_edges, senders, receivers, rdisp, rdist, d_disp, d_dist, r,
+                                            index_base, grad, scratch, s);
+    if (dims == 3)
+        return launch_pullback_d<3, double>(n_points, n_edges, senders, receivers, rdisp, rdist, d_disp, d_dist, r,
+                                            index_base, grad, scratch, s);
     return hipErrorInvalidValue;
 }
 

@@ -77,6 +77,8 @@ struct SetState {
     DevBuf rk, ri, rsk, rsi, rpos;  // refit: changed points
     DevBuf pad;                     // 2D sets: the (x, y, PLANE_Z) copy the build reads (planar.hip)
     DevBuf qfirst, qlast, qkids, qbmin, qbmax;  // 2D sets: the quadtree in the reference layout
+    // Float64 sets (f64.hip): double parameters and bounds, the rounded copy the sort reads
+    DevBuf p64, partials64, c32, bmin64, bmax64;
     DevBuf soff;                    // {0, n}: the one-set forest descriptor
     int64_t soff_n = -1;            // the n soff holds
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
@@ -84,7 +86,7 @@ struct SetState {
     // the buffers oct_get_tree and oct_device_view hand out
     void export_to(std::vector<void*>* r) {
         DevBuf* out[] = {&perm, &nfirst, &nlast, &nchild, &bmin, &bmax, &nodes, &scoord, &pad,
-                         &qfirst, &qlast, &qkids, &qbmin, &qbmax};
+                         &qfirst, &qlast, &qkids, &qbmin, &qbmax, &bmin64, &bmax64};
         for (DevBuf* d : out) d->retire = r;
     }
     BuildBufs bufs() const {
@@ -138,7 +140,7 @@ struct SetState {
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
                          &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
                          &rslots, &ranc, &rk, &ri, &rsk, &rsi, &rpos, &soff, &csrc, &pad, &qfirst, &qlast,
-                         &qkids, &qbmin, &qbmax};
+                         &qkids, &qbmin, &qbmax, &p64, &partials64, &c32, &bmin64, &bmax64};
         for (DevBuf* d : all) d->release();
         soff_n = -1;
     }
@@ -277,6 +279,8 @@ struct oct_ctx {
     hipStream_t cap_stream = nullptr;          // graph captures (the caller's stream may be the legacy default)
     int index_base = 0;
     int ndims = 3;  // 2: quadtree sets, built as planar octrees (planar.hip)
+    bool f64 = false;     // Float64 sets (f64.hip)
+    double radius64 = 0;  // the radius in double (Float64 contexts)
     float radius = 0.f;
     int leaf_size = 32;
     bool refit_mode = false;
@@ -504,20 +508,13 @@ int ensure_hp(oct_ctx* c, SetState& s) {
     return OCT_OK;
 }
 
-int build_set(oct_ctx* c, SetState& s, bool force_full) {
-    const int64_t n = s.n;
-    if (s.refit_pending) HIP_TRY_RC(ensure_hp(c, s));  // the previous build's outcome first
-    s.built = false;
-    if (n == 0) {
-        memset(&s.hp, 0, sizeof(s.hp));
-        s.hp_fresh = true;
-        s.built = true;
-        s.dirty = false;
-        return OCT_OK;
-    }
-    if (n >= (int64_t)INT32_MAX) return fail(c, OCT_ERR_CAPACITY, "point sets are limited to 2^31 - 1 points");
-    const int T = c->leaf_size;
-    const int64_t M = max_nodes(n, T);
+// run! of a Float64 set (f64.hip): double bounds, origin and keys, the
+// shared sort and topology, double node bounds.  Always the full sort (the
+// almost-sorted refit reads Float32 coordinates).
+int build_set_f64(oct_ctx* c, SetState& s);
+
+// The device buffers of one set's build (n points, at most M nodes).
+int ensure_build_bufs(oct_ctx* c, SetState& s, int64_t n, int64_t M) {
     const bool fresh_prm = s.prm.p == nullptr;
     HIP_TRY(c, s.prm.ensure(sizeof(BuildParams)));
     if (fresh_prm)
@@ -546,6 +543,25 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
     HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
     HIP_TRY(c, s.rslots.ensure(4 * (size_t)refit_slot_words(n)));
     HIP_TRY(c, s.ranc.ensure(4 * (size_t)OCT_MAX_LEVEL * (size_t)n));
+    return OCT_OK;
+}
+
+int build_set(oct_ctx* c, SetState& s, bool force_full) {
+    const int64_t n = s.n;
+    if (s.refit_pending) HIP_TRY_RC(ensure_hp(c, s));  // the previous build's outcome first
+    s.built = false;
+    if (n == 0) {
+        memset(&s.hp, 0, sizeof(s.hp));
+        s.hp_fresh = true;
+        s.built = true;
+        s.dirty = false;
+        return OCT_OK;
+    }
+    if (n >= (int64_t)INT32_MAX) return fail(c, OCT_ERR_CAPACITY, "point sets are limited to 2^31 - 1 points");
+    if (c->f64) return build_set_f64(c, s);
+    const int T = c->leaf_size;
+    const int64_t M = max_nodes(n, T);
+    HIP_TRY_RC(ensure_build_bufs(c, s, n, M));
     const BuildBufs b = s.bufs();
     // a 2D set is built from its (x, y, PLANE_Z) copy
     const float* src = s.coords;
@@ -647,6 +663,58 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
     return OCT_OK;
 }
 
+int build_set_f64(oct_ctx* c, SetState& s) {
+    const int64_t n = s.n;
+    const int T = c->leaf_size;
+    const int64_t M = max_nodes(n, T);
+    HIP_TRY_RC(ensure_build_bufs(c, s, n, M));
+    HIP_TRY(c, s.p64.ensure(sizeof(Params64)));
+    HIP_TRY(c, s.partials64.ensure(6 * sizeof(uint64_t) * (size_t)bounds64_blocks(n)));
+    HIP_TRY(c, s.c32.ensure(12 * (size_t)n));
+    HIP_TRY(c, s.bmin64.ensure(24 * (size_t)M));
+    HIP_TRY(c, s.bmax64.ensure(24 * (size_t)M));
+    HIP_TRY(c, s.soff.ensure(16));
+    if (s.soff_n != n) {
+        HIP_TRY(c, launch_single_set_offsets(s.soff.as<int64_t>(), n, c->stream));
+        s.soff_n = n;
+    }
+    const BuildBufs b = s.bufs();
+    const double* src = (const double*)s.coords;
+    if (c->ndims == 2) {  // the (x, y, 2^34) copy: its z cell is 0 in double
+        HIP_TRY(c, s.pad.ensure(24 * (size_t)n));
+        HIP_TRY(c, launch_pad_planar64((const double*)s.coords, n, s.pad.as<double>(), c->stream));
+        src = s.pad.as<double>();
+    }
+    memset(&s.info, 0, sizeof(s.info));
+    s.info.changed_points = -1;
+    s.refit_pending = false;
+    {
+        Phase ph(c, OCT_PROF_BOUNDS);
+        HIP_TRY(c, launch_build64(src, n, c->radius64, s.partials64.as<uint64_t>(), s.p64.as<Params64>(),
+                                  b.prm, c->stream));
+    }
+    Phase ph(c, OCT_PROF_BUILD);
+    HIP_TRY_RC(read_params(c, s));  // the key width sizes the radix passes
+    s.key_bits = s.hp.key_bits;
+    HIP_TRY(c, launch_keys64(src, n, s.p64.as<Params64>(), b.keys[0], b.vals[0], s.c32.as<float>(), c->stream));
+    int passes = std::max(1, (s.key_bits + RADIX_BITS - 1) / RADIX_BITS);
+    int shifts[8];
+    for (int k = 0; k < passes; ++k) shifts[k] = k * RADIX_BITS;
+    int cur = 0;
+    HIP_TRY(c, launch_sort_passes(s.c32.as<float>(), n, shifts, passes, c->index_base, b, &cur, c->stream));
+    const Forest f{0, s.soff.as<int64_t>(), s.prm.as<BuildParams>()};
+    HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, cur, b, f, c->stream));
+    HIP_TRY(c, launch_refit64(src, b.prm, M, b.nlevel, b.nfirst, b.nlast, b.nchild, b.perm, c->index_base,
+                              s.bmin64.as<double>(), s.bmax64.as<double>(), c->stream));
+    s.hp_fresh = false;
+    s.built_leaf = T;
+    s.cur = cur;
+    s.sorted_ok = false;
+    s.built = true;
+    s.dirty = false;
+    return OCT_OK;
+}
+
 PairState* find_pair(oct_ctx* c, int q, int t) {
     for (auto& p : c->pairs)
         if (p.q == q && p.t == t) return &p;
@@ -660,7 +728,12 @@ PairState* find_pair(oct_ctx* c, int q, int t) {
 extern "C" {
 
 int oct_create(oct_ctx** out, int device, int ndims, int index_base) {
+    return oct_create_typed(out, device, ndims, index_base, 4);
+}
+
+int oct_create_typed(oct_ctx** out, int device, int ndims, int index_base, int elem_bytes) {
     if (!out) return OCT_ERR_ARGUMENT;
+    if (elem_bytes != 4 && elem_bytes != 8) return OCT_ERR_ARGUMENT;
     *out = nullptr;
     if (ndims != 3 && ndims != 2) return OCT_ERR_ARGUMENT;
     if (index_base != 0 && index_base != 1) return OCT_ERR_ARGUMENT;
@@ -669,6 +742,7 @@ int oct_create(oct_ctx** out, int device, int ndims, int index_base) {
     c->device = device;
     c->index_base = index_base;
     c->ndims = ndims;
+    c->f64 = elem_bytes == 8;
     if (hipSetDevice(device) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
     if (noalloc_scratch_setup(device) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return OCT_ERR_DEVICE; }
@@ -758,6 +832,18 @@ int oct_set_search_radius(oct_ctx* c, float r) {
     // the kernels rely on a finite r * r (empty candidate slots sit at infinity)
     if (!std::isfinite(r * r)) return fail(c, OCT_ERR_ARGUMENT, "search radius squared must be finite in Float32");
     c->radius = r;
+    c->radius64 = (double)r;
+    return mark_all_dirty(c);
+}
+
+int oct_set_search_radius64(oct_ctx* c, double r) {
+    if (!c) return OCT_ERR_ARGUMENT;
+    if (!(r > 0.0)) return fail(c, OCT_ERR_ARGUMENT, "search radius must be positive");
+    if (!std::isfinite(r * r)) return fail(c, OCT_ERR_ARGUMENT, "search radius squared must be finite");
+    if (!c->f64 && !std::isfinite((float)r * (float)r))
+        return fail(c, OCT_ERR_ARGUMENT, "search radius squared must be finite in Float32");
+    c->radius64 = r;
+    c->radius = (float)r;  // T(r) of a Float32 context
     return mark_all_dirty(c);
 }
 
@@ -855,6 +941,10 @@ int oct_get_tree(oct_ctx* c, int id, oct_tree* out) {
     out->n_nodes = s.hp.n_nodes;
     out->n_leaves = s.hp.n_leaves;
     for (int d = 0; d < 3; ++d) out->origin[d] = s.hp.origin[d];
+    if (c->f64) {  // doubles
+        out->node_bounds_min = (float*)s.bmin64.p;
+        out->node_bounds_max = (float*)s.bmax64.p;
+    }
     if (c->ndims == 2) {  // the reference's quadtree: a one-child chain over the planar octree
         if (s.hp.cmax > 0x1fffffu)
             return fail(c, OCT_ERR_CAPACITY, "2D trees are exported for extents below 2^21 cells (radii) per axis");
@@ -864,12 +954,18 @@ int oct_get_tree(oct_ctx* c, int id, oct_tree* out) {
         HIP_TRY(c, s.qfirst.ensure(4 * (size_t)m2));
         HIP_TRY(c, s.qlast.ensure(4 * (size_t)m2));
         HIP_TRY(c, s.qkids.ensure(16 * (size_t)m2));
-        HIP_TRY(c, s.qbmin.ensure(8 * (size_t)m2));
-        HIP_TRY(c, s.qbmax.ensure(8 * (size_t)m2));
-        HIP_TRY(c, launch_quadtree(s.nfirst.as<int32_t>(), s.nlast.as<int32_t>(), s.nchild.as<int32_t>(),
-                                   s.bmin.as<float>(), s.bmax.as<float>(), s.hp.n_nodes, chain, s.n, c->index_base,
-                                   s.qfirst.as<int32_t>(), s.qlast.as<int32_t>(), s.qkids.as<int32_t>(),
-                                   s.qbmin.as<float>(), s.qbmax.as<float>(), c->stream));
+        HIP_TRY(c, s.qbmin.ensure((c->f64 ? 16 : 8) * (size_t)m2));
+        HIP_TRY(c, s.qbmax.ensure((c->f64 ? 16 : 8) * (size_t)m2));
+        if (c->f64)
+            HIP_TRY(c, launch_quadtree64(s.nfirst.as<int32_t>(), s.nlast.as<int32_t>(), s.nchild.as<int32_t>(),
+                                         s.bmin64.as<double>(), s.bmax64.as<double>(), s.hp.n_nodes, chain, s.n,
+                                         c->index_base, s.qfirst.as<int32_t>(), s.qlast.as<int32_t>(),
+                                         s.qkids.as<int32_t>(), s.qbmin.as<double>(), s.qbmax.as<double>(), c->stream));
+        else
+            HIP_TRY(c, launch_quadtree(s.nfirst.as<int32_t>(), s.nlast.as<int32_t>(), s.nchild.as<int32_t>(),
+                                       s.bmin.as<float>(), s.bmax.as<float>(), s.hp.n_nodes, chain, s.n,
+                                       c->index_base, s.qfirst.as<int32_t>(), s.qlast.as<int32_t>(),
+                                       s.qkids.as<int32_t>(), s.qbmin.as<float>(), s.qbmax.as<float>(), c->stream));
         out->node_bounds_min = s.qbmin.as<float>();
         out->node_bounds_max = s.qbmax.as<float>();
         out->node_first = s.qfirst.as<int32_t>();
@@ -891,6 +987,102 @@ constexpr int SIG_STATS = SP_SIG_MARK + 4;  // deferred recount statistics in th
 void recount_policy(PairState& p, uint32_t recounted, int64_t heavy_n, int64_t n_leaves) {
     if (((int64_t)recounted - heavy_n) * 64 > n_leaves) p.replan = true;
     p.heavy_on = (int64_t)recounted * 64 > n_leaves;
+}
+
+// build_edges / materialize of a Float64 pair (f64.hip): one thread per
+// query walks the tree in double; count, scan, then the fill at each query's
+// offset (receiver order, the reference's layout).  The learned single pass
+// computes in Float32, so a Float64 context takes the exact path only.
+static int edges_f64(oct_ctx* c, PairState& p, SetState& qs, SetState& ts, bool self_pair, bool csr,
+                     oct_edges* out) {
+    if (!csr) {
+        memset(&p.st, 0, sizeof(p.st));
+        p.n_edges = 0;
+    }
+    const int64_t nq = qs.n;
+    if (nq == 0 || ts.n == 0) {
+        if (csr) {
+            HIP_TRY(c, p.csr_off.ensure(4 * (size_t)(nq + 1)));
+            HIP_TRY(c, launch_fill_u32(p.csr_off.as<uint32_t>(), nq + 1, 0u, c->stream));
+            p.csr_ok = true;
+            p.csr_nq = nq;
+            p.csr_total = 0;
+        } else {
+            p.materialized = true;
+        }
+        return OCT_OK;
+    }
+    const double r = c->radius64, r_sq = r * r, inv_r = 1.0 / r;
+    const double* cq = c->ndims == 2 ? qs.pad.as<double>() : (const double*)qs.coords;
+    const double* ct = c->ndims == 2 ? ts.pad.as<double>() : (const double*)ts.coords;
+    HIP_TRY(c, p.flags.ensure(HF_BYTES + 2 * sizeof(BuildParams)));
+    HIP_TRY(c, p.qcount.ensure(4 * (size_t)nq));
+    HIP_TRY(c, p.qoff.ensure(4 * (size_t)nq));
+    HIP_TRY(c, p.scan_tmp.ensure(4 * (size_t)scan_tmp_words(nq)));
+    const int32_t* tf = ts.nfirst.as<int32_t>();
+    const int32_t* tl = ts.nlast.as<int32_t>();
+    const int32_t* tk = ts.nchild.as<int32_t>();
+    const int32_t* tp = ts.perm.as<int32_t>();
+    const double* bl = ts.bmin64.as<double>();
+    const double* bh = ts.bmax64.as<double>();
+    uint64_t* total_dev = (uint64_t*)(p.flags.as<uint32_t>() + 2);
+    {
+        Phase ph(c, OCT_PROF_COUNT);
+        HIP_TRY(c, launch_edges64(EDGE_COUNT, cq, nq, ct, bl, bh, tf, tl, tk, tp, c->index_base, r_sq, inv_r,
+                                  self_pair, p.qcount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                  c->stream));
+    }
+    {
+        Phase ph(c, OCT_PROF_SCAN);
+        HIP_TRY(c, exclusive_scan(p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), nq, p.scan_tmp.as<uint32_t>(),
+                                  total_dev, c->stream));
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->pinned, total_dev, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, wait_stream(c->stream));
+    uint64_t total = 0;
+    memcpy(&total, c->pinned, 8);
+    if (total >= (uint64_t)INT32_MAX)
+        return fail(c, OCT_ERR_CAPACITY, "edge count exceeds Int32 (the reference's limit)");
+    const int64_t E = (int64_t)total;
+    if (csr) {
+        HIP_TRY(c, p.csr_off.ensure(4 * (size_t)(nq + 1)));
+        HIP_TRY(c, p.csr_flat.ensure(4 * (size_t)std::max<int64_t>(E, 1)));
+        HIP_TRY(c, launch_edges64(EDGE_INDEX, cq, nq, ct, bl, bh, tf, tl, tk, tp, c->index_base, r_sq, inv_r,
+                                  self_pair, nullptr, p.qoff.as<uint32_t>(), p.csr_flat.as<int32_t>(), nullptr, nullptr,
+                                  nullptr, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(p.csr_off.p, p.qoff.p, 4 * (size_t)nq, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, launch_fill_u32(p.csr_off.as<uint32_t>() + nq, 1, (uint32_t)E, c->stream));
+        p.csr_ok = true;
+        p.csr_nq = nq;
+        p.csr_total = E;
+        out->n_edges = E;
+        return OCT_OK;
+    }
+    HIP_TRY(c, p.senders.ensure(4 * (size_t)std::max<int64_t>(E, 1)));
+    HIP_TRY(c, p.receivers.ensure(4 * (size_t)std::max<int64_t>(E, 1)));
+    HIP_TRY(c, p.rdisp.ensure(24 * (size_t)std::max<int64_t>(E, 1)));
+    HIP_TRY(c, p.rdist.ensure(8 * (size_t)std::max<int64_t>(E, 1)));
+    {
+        Phase ph(c, OCT_PROF_FILL);
+        HIP_TRY(c, launch_edges64(EDGE_FILL, cq, nq, ct, bl, bh, tf, tl, tk, tp, c->index_base, r_sq, inv_r,
+                                  self_pair, nullptr, p.qoff.as<uint32_t>(), p.senders.as<int32_t>(),
+                                  p.receivers.as<int32_t>(), p.rdisp.as<double>(), p.rdist.as<double>(), c->stream));
+    }
+    p.n_edges = E;
+    p.materialized = true;
+    p.st.n_edges = E;
+    p.st.attempts = 1;
+    out->senders = p.senders.as<int32_t>();
+    out->receivers = p.receivers.as<int32_t>();
+    out->rel_displacement = (float*)p.rdisp.p;  // doubles in a Float64 context
+    out->rel_dist_norm = (float*)p.rdist.p;
+    out->n_edges = E;
+    if (c->ndims == 2) {
+        HIP_TRY(c, p.rdisp2.ensure(16 * (size_t)std::max<int64_t>(E, 1)));
+        HIP_TRY(c, launch_disp_planar64(p.rdisp.as<double>(), E, p.rdisp2.as<double>(), c->stream));
+        out->rel_displacement = (float*)p.rdisp2.p;
+    }
+    return OCT_OK;
 }
 
 // What an edges_impl call produces: the flat-COO edges; only the counts with
@@ -915,6 +1107,12 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, int emit, o
     if (!qs.built || qs.dirty || !ts.built || ts.dirty) return fail(c, OCT_ERR_STATE, "run! before build_edges");
     HIP_TRY(c, hipSetDevice(c->device));
     memset(out, 0, sizeof(*out));
+    if (c->f64) {
+        if (mode != OCT_EDGES_EXACT)
+            return fail(c, OCT_ERR_ARGUMENT, "the learned single pass computes in Float32; Float64 sets use the exact mode");
+        if (collect) return fail(c, OCT_ERR_STATE, "the predictor's training rows are Float32 only");
+        return edges_f64(c, p, qs, ts, q == t, csr, out);
+    }
     // the previous call's recount statistics (final once its kernels ended)
     // come back with this call's read-back, queued ahead of everything else
     // (a CSR call leaves the edge buffers' state and statistics alone)
@@ -1283,6 +1481,7 @@ int oct_device_view(oct_ctx* c, int q, int t, oct_view* out) {
     SetState& qs = c->sets[set_index(c, q)];
     SetState& ts = c->sets[set_index(c, t)];
     if (!qs.built || qs.dirty || !ts.built || ts.dirty) return fail(c, OCT_ERR_STATE, "run! before device_view");
+    if (c->f64) return fail(c, OCT_ERR_STATE, "device_view: the device iterator walks Float32 trees");
     int rc = ensure_hp(c, ts);
     if (rc != OCT_OK) return rc;
     memset(out, 0, sizeof(*out));
@@ -1354,8 +1553,8 @@ int oct_count_neighbors_device(oct_ctx* c, const oct_view* v, int32_t* counts) {
     return OCT_OK;
 }
 
-int oct_build_edges_backward(oct_ctx* c, int64_t n_points, const oct_edges* e, const float* d_disp,
-                             const float* d_dist, float radius, float* grad) {
+static int backward_impl(oct_ctx* c, int64_t n_points, const oct_edges* e, const void* d_disp, const void* d_dist,
+                         double radius, void* grad) {
     if (!c || !e) return OCT_ERR_ARGUMENT;
     if (n_points < 0 || e->n_edges < 0) return fail(c, OCT_ERR_ARGUMENT, "negative size");
     if (n_points > 0 && !grad) return fail(c, OCT_ERR_ARGUMENT, "null gradient buffer");
@@ -1367,10 +1566,29 @@ int oct_build_edges_backward(oct_ctx* c, int64_t n_points, const oct_edges* e, c
         scratch = c->bw_scratch.as<uint32_t>();
     }
     Phase ph(c, OCT_PROF_BACKWARD);
-    HIP_TRY(c, launch_pullback(n_points, e->n_edges, e->senders, e->receivers, e->rel_displacement,
-                               e->rel_dist_norm, d_disp, d_dist, radius, c->index_base, grad, scratch, c->ndims,
-                               c->stream));
+    if (c->f64)  // every array holds doubles
+        HIP_TRY(c, launch_pullback64(n_points, e->n_edges, e->senders, e->receivers,
+                                     (const double*)e->rel_displacement, (const double*)e->rel_dist_norm,
+                                     (const double*)d_disp, (const double*)d_dist, radius, c->index_base,
+                                     (double*)grad, scratch, c->ndims, c->stream));
+    else
+        HIP_TRY(c, launch_pullback(n_points, e->n_edges, e->senders, e->receivers, e->rel_displacement,
+                                   e->rel_dist_norm, (const float*)d_disp, (const float*)d_dist, (float)radius,
+                                   c->index_base, (float*)grad, scratch, c->ndims, c->stream));
     return OCT_OK;
+}
+
+int oct_build_edges_backward(oct_ctx* c, int64_t n_points, const oct_edges* e, const float* d_disp,
+                             const float* d_dist, float radius, float* grad) {
+    if (c && c->f64)
+        return fail(c, OCT_ERR_ARGUMENT, "a Float64 context takes oct_build_edges_backward64 (double radius)");
+    return backward_impl(c, n_points, e, d_disp, d_dist, radius, grad);
+}
+
+int oct_build_edges_backward64(oct_ctx* c, int64_t n_points, const oct_edges* e, const double* d_disp,
+                               const double* d_dist, double radius, double* grad) {
+    if (c && !c->f64) return fail(c, OCT_ERR_ARGUMENT, "oct_build_edges_backward64 needs a Float64 context");
+    return backward_impl(c, n_points, e, d_disp, d_dist, radius, grad);
 }
 
 int oct_set_backward_form(oct_ctx* c, int pull) {
@@ -1825,7 +2043,7 @@ int batch_eager(oct_ctx* c, BatchState& B) {
 int oct_batch_create(oct_ctx* c, int n_sets, const float* coords, const int64_t* offsets, const float* radii,
                      int* batch_id) {
     if (!c || !batch_id || !offsets || !radii) return OCT_ERR_ARGUMENT;
-    if (c->ndims != 3) return fail(c, OCT_ERR_ARGUMENT, "batches hold 3D point sets");
+    if (c->ndims != 3 || c->f64) return fail(c, OCT_ERR_ARGUMENT, "batches hold 3D Float32 point sets");
     if (n_sets < 1 || n_sets > (1 << 20)) return fail(c, OCT_ERR_ARGUMENT, "batch: n_sets must be in 1..2^20");
     if (offsets[0] != 0) return fail(c, OCT_ERR_ARGUMENT, "batch: offsets[0] must be 0");
     for (int s = 0; s < n_sets; ++s) {

@@ -359,6 +359,33 @@ hipError_t launch_quadtree(const int32_t* ofirst, const int32_t* olast, const in
                            const float* obmax, int32_t m3, int32_t chain, int64_t n, int index_base, int32_t* qfirst,
                            int32_t* qlast, int32_t* qkids, float* qbmin, float* qbmax, hipStream_t s);
 
+// Float64 point sets (see f64.hip).
+struct Params64 {
+    double origin[3];  // min - Float64(1e-6)
+    double inv_cs;     // 1 / r
+    double radius;
+};
+constexpr double PLANE_Z64 = 17179869184.0;  // 2^34: 2^34 - 1e-6 rounds back to 2^34 in double
+int bounds64_blocks(int64_t n);
+hipError_t launch_build64(const double* coords, int64_t n, double radius, uint64_t* partials, Params64* p,
+                          BuildParams* prm, hipStream_t s);
+hipError_t launch_keys64(const double* coords, int64_t n, const Params64* p, uint64_t* keys, int32_t* vals,
+                         float* c32, hipStream_t s);
+hipError_t launch_refit64(const double* coords, const BuildParams* prm, int64_t max_nodes, const uint8_t* nlevel,
+                          const int32_t* nfirst, const int32_t* nlast, const int32_t* nchild, const int32_t* perm,
+                          int base, double* bmin, double* bmax, hipStream_t s);
+hipError_t launch_edges64(int mode, const double* cq, int64_t nq, const double* ct, const double* bmin,
+                          const double* bmax, const int32_t* nfirst, const int32_t* nlast, const int32_t* nchild,
+                          const int32_t* perm, int base, double r_sq, double inv_r, int self_pair, uint32_t* qcount,
+                          const uint32_t* qoff, int32_t* senders, int32_t* receivers, double* rdisp, double* rdist,
+                          hipStream_t s);
+hipError_t launch_pad_planar64(const double* xy, int64_t n, double* xyz, hipStream_t s);
+hipError_t launch_disp_planar64(const double* d3, int64_t n_edges, double* d2, hipStream_t s);
+hipError_t launch_quadtree64(const int32_t* ofirst, const int32_t* olast, const int32_t* okids, const double* obmin,
+                             const double* obmax, int32_t m3, int32_t chain, int64_t n, int index_base,
+                             int32_t* qfirst, int32_t* qlast, int32_t* qkids, double* qbmin, double* qbmax,
+                             hipStream_t s);
+
 // apply_zsort gather (scatter = its adjoint) of n columns of `width`
 // elements of elem_bytes (1, 2, 4 or 8) bytes (see zsort.hip).
 hipError_t launch_zsort(const void* src, void* dst, const int32_t* perm, int64_t n, int64_t width, int elem_bytes,
@@ -373,6 +400,10 @@ hipError_t launch_pullback(int64_t n_points, int64_t n_edges, const int32_t* sen
                            const int32_t* receivers, const float* rdisp, const float* rdist,
                            const float* d_disp, const float* d_dist, float r, int index_base,
                            float* grad, uint32_t* scratch, int dims, hipStream_t s);
+// The same in double (Float64 sets): every pointer holds doubles.
+hipError_t launch_pullback64(int64_t n_points, int64_t n_edges, const int32_t* senders, const int32_t* receivers,
+                             const double* rdisp, const double* rdist, const double* d_disp, const double* d_dist,
+                             double r, int index_base, double* grad, uint32_t* scratch, int dims, hipStream_t s);
 
 int64_t scan_tmp_words(int64_t n);
 // Tiles of the device scan (SCAN_TILE_INPUTS inputs each).

@@ -36,11 +36,12 @@ k_disp_planar(const float* __restrict__ d3, int64_t n_edges, float* __restrict__
 // Quadtree arrays (reference layout, index base applied like the octree's)
 // from the octree's: `chain` one-child nodes on top, then octree node k as
 // node k + chain with its first four children (the last four are empty).
+template <typename T>
 __global__ void __launch_bounds__(256)
 k_quadtree(const int32_t* __restrict__ ofirst, const int32_t* __restrict__ olast, const int32_t* __restrict__ okids,
-           const float* __restrict__ obmin, const float* __restrict__ obmax, int32_t m3, int32_t chain,
+           const T* __restrict__ obmin, const T* __restrict__ obmax, int32_t m3, int32_t chain,
            int64_t n, int index_base, int32_t* __restrict__ qfirst, int32_t* __restrict__ qlast,
-           int32_t* __restrict__ qkids, float* __restrict__ qbmin, float* __restrict__ qbmax) {
+           int32_t* __restrict__ qkids, T* __restrict__ qbmin, T* __restrict__ qbmax) {
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (int64_t)m3 + chain) return;
     const int64_t k = id < chain ? 0 : id - chain;
@@ -80,8 +81,19 @@ hipError_t launch_quadtree(const int32_t* ofirst, const int32_t* olast, const in
                            int32_t* qlast, int32_t* qkids, float* qbmin, float* qbmax, hipStream_t s) {
     const int64_t m2 = (int64_t)m3 + chain;
     if (m2 <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_quadtree, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, s, ofirst, olast, okids, obmin,
-                       obmax, m3, chain, n, index_base, qfirst, qlast, qkids, qbmin, qbmax);
+    hipLaunchKernelGGL(k_quadtree<float>, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, s, ofirst, olast, okids,
+                       obmin, obmax, m3, chain, n, index_base, qfirst, qlast, qkids, qbmin, qbmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_quadtree64(const int32_t* ofirst, const int32_t* olast, const int32_t* okids, const double* obmin,
+                             const double* obmax, int32_t m3, int32_t chain, int64_t n, int index_base,
+                             int32_t* qfirst, int32_t* qlast, int32_t* qkids, double* qbmin, double* qbmax,
+                             hipStream_t s) {
+    const int64_t m2 = (int64_t)m3 + chain;
+    if (m2 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_quadtree<double>, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, s, ofirst, olast,
+                       okids, obmin, obmax, m3, chain, n, index_base, qfirst, qlast, qkids, qbmin, qbmax);
     return hipGetLastError();
 }
 

@@ -13,7 +13,8 @@ using OctopusHIP: TNS, build_edges, build_edges_diff_pullback!, apply_zsort, app
 # Primal: fresh copies of the pair's edges, so the pullback closes over values
 # that a later build on the pair cannot overwrite (the reference snapshots the
 # same four arrays).
-function OctopusHIP.build_edges_diff(coords::AbstractMatrix{Float32}, t::TNS, id::Integer, radius::Real;
+function OctopusHIP.build_edges_diff(coords::AbstractMatrix{<:Union{Float32,Float64}}, t::TNS, id::Integer,
+                                     radius::Real;
                                      mode::Symbol=:exact)
     e = build_edges(t, id, id; mode=mode)
     return (senders = copy(e.senders), receivers = copy(e.receivers),
@@ -36,8 +37,8 @@ end
 _dense(::Nothing, like) = nothing
 _dense(x, like) = x isa typeof(like) ? x : copyto!(similar(like), x)
 
-function ChainRulesCore.rrule(::typeof(OctopusHIP.build_edges_diff), coords::AbstractMatrix{Float32}, t::TNS,
-                              id::Integer, radius::Real; mode::Symbol=:exact)
+function ChainRulesCore.rrule(::typeof(OctopusHIP.build_edges_diff), coords::AbstractMatrix{<:Union{Float32,Float64}},
+                              t::TNS, id::Integer, radius::Real; mode::Symbol=:exact)
     e = OctopusHIP.build_edges_diff(coords, t, id, radius; mode=mode)
     function build_edges_diff_pullback(d)
         dd = _dense(_field(d, :rel_displacement), e.rel_displacement)

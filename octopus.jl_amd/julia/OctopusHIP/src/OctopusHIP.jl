@@ -4,9 +4,11 @@
 # exception types as the reference; the backend symbol is :hip and every
 # computation runs in the HIP library on one MI355X.
 #
-# Coordinates are device arrays of Float32 with 3 rows (column-major (3, N),
-# i.e. xyz interleaved), referenced and never copied (src/types.jl:13).  The
-# binding needs only a raw device pointer from them (`pointer(A)`).
+# Coordinates are device arrays of the TNS's element type (Float32 or
+# Float64) with D = 3 or 2 rows (column-major (D, N), i.e. xyz interleaved),
+# referenced and never copied (src/types.jl:13).  The binding needs only a raw
+# device pointer from them (`pointer(A)`); the C ABI types such pointers as
+# float*, and a Float64 context reads them as double (oct_create_typed).
 #
 # Device arrays and streams come from a GPU array package through two hooks
 # that the AMDGPU extension (ext/OctopusHIPAMDGPUExt.jl) fills:
@@ -102,24 +104,26 @@ _current_stream() = nothing
 
 mutable struct TNS
     ctx::Ptr{Cvoid}
+    eltype::DataType             # Float32 or Float64 (TNS(T), src/types.jl:167-183)
     ndims::Int                   # 3 (octrees) or 2 (quadtrees)
-    radius::Float32
+    radius::Float64
     coords::Vector{Any}          # keeps user arrays alive (referenced, not copied)
     dirty::Vector{Bool}
     active_pairs::Vector{Tuple{Int32,Int32}}
-    function TNS(::Type{Float32}=Float32; ndims::Int=3, device::Int=0)
+    function TNS(::Type{T}=Float32; ndims::Int=3, device::Int=0) where {T<:Union{Float32,Float64}}
         ndims in (2, 3) || throw(ArgumentError("ndims must be 2 or 3"))
         h = Ref{Ptr{Cvoid}}(C_NULL)
-        rc = ccall((:oct_create, libocto), Cint, (Ptr{Ptr{Cvoid}}, Cint, Cint, Cint), h, device, ndims, 1)
+        rc = ccall((:oct_create_typed, libocto), Cint, (Ptr{Ptr{Cvoid}}, Cint, Cint, Cint, Cint), h, device, ndims, 1,
+                   sizeof(T))
         rc == OCT_OK || error("oct_create failed ($rc)")
-        t = new(h[], ndims, 0f0, Any[], Bool[], Tuple{Int32,Int32}[])
+        t = new(h[], T, ndims, 0.0, Any[], Bool[], Tuple{Int32,Int32}[])
         # every device array handed out keeps `t` reachable (see _device_wrap),
         # so the context is destroyed only after the last of them is gone
         finalizer(x -> ccall((:oct_destroy, libocto), Cint, (Ptr{Cvoid},), x.ctx), t)
         return t
     end
 end
-TNS(::Type{T}; kw...) where {T} = throw(MethodError(TNS, (T,)))  # Float32 only (test/float64.jl:138-148)
+TNS(::Type{T}; kw...) where {T} = throw(MethodError(TNS, (T,)))  # Float32 and Float64 only
 
 function _check(t::TNS, rc::Cint)
     rc == OCT_OK && return nothing
@@ -160,8 +164,8 @@ trim!(t::TNS) = (_check(t, ccall((:oct_trim, libocto), Cint, (Ptr{Cvoid},), t.ct
 
 function set_search_radius!(t::TNS, r::Real)
     r > 0 || throw(ArgumentError("search radius must be positive"))
-    t.radius = Float32(r)
-    _check(t, ccall((:oct_set_search_radius, libocto), Cint, (Ptr{Cvoid}, Cfloat), t.ctx, t.radius))
+    t.radius = Float64(t.eltype(r))  # T(r)
+    _check(t, ccall((:oct_set_search_radius64, libocto), Cint, (Ptr{Cvoid}, Cdouble), t.ctx, t.radius))
     fill!(t.dirty, true)
     return t
 end
@@ -178,32 +182,36 @@ function set_leaf_size!(t::TNS, leaf_size::Integer)
     return t
 end
 
-_npoints(t::TNS, coords) =
-    (size(coords, 1) == t.ndims || throw(DimensionMismatch("coords must be ($(t.ndims), N)")); size(coords, 2))
+function _npoints(t::TNS, coords)
+    # an eltype other than the TNS's is a MethodError, as in the reference (test/float64.jl:138-148)
+    eltype(coords) === t.eltype || throw(MethodError(add_point_set!, (t, coords)))
+    size(coords, 1) == t.ndims || throw(DimensionMismatch("coords must be ($(t.ndims), N)"))
+    return size(coords, 2)
+end
 
-function add_point_set!(t::TNS, coords::AbstractMatrix{Float32})
+function add_point_set!(t::TNS, coords::AbstractMatrix{<:Union{Float32,Float64}})
     n = _npoints(t, coords)
     id = Ref{Cint}(0)
     _check(t, ccall((:oct_add_point_set, libocto), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int64, Ptr{Cint}),
-                    t.ctx, pointer(coords), n, id))
+                    t.ctx, Ptr{Float32}(pointer(coords)), n, id))
     push!(t.coords, coords)
     push!(t.dirty, true)
     return Int(id[])
 end
 
-function update_point_set!(t::TNS, id::Integer, coords::AbstractMatrix{Float32})
+function update_point_set!(t::TNS, id::Integer, coords::AbstractMatrix{<:Union{Float32,Float64}})
     n = _npoints(t, coords)
     _check(t, ccall((:oct_update_point_set, libocto), Cint, (Ptr{Cvoid}, Cint, Ptr{Float32}, Int64),
-                    t.ctx, id, pointer(coords), n))
+                    t.ctx, id, Ptr{Float32}(pointer(coords)), n))
     t.coords[id] = coords
     t.dirty[id] = true
     return t
 end
 
-function resize_point_set!(t::TNS, id::Integer, coords::AbstractMatrix{Float32})
+function resize_point_set!(t::TNS, id::Integer, coords::AbstractMatrix{<:Union{Float32,Float64}})
     n = _npoints(t, coords)
     _check(t, ccall((:oct_resize_point_set, libocto), Cint, (Ptr{Cvoid}, Cint, Ptr{Float32}, Int64),
-                    t.ctx, id, pointer(coords), n))
+                    t.ctx, id, Ptr{Float32}(pointer(coords)), n))
     t.coords[id] = coords
     t.dirty[id] = true
     return t
@@ -237,9 +245,9 @@ arrays: node_bounds_min/max (D, M), node_first/last (M), node_children
 function get_tree(t::TNS, id::Integer)
     x = get_tree!(t, id)
     m, n = Int(x.n_nodes), Int(x.n_points)
-    d = t.ndims
-    return (node_bounds_min = _device_wrap(x.node_bounds_min, (d, m), t),
-            node_bounds_max = _device_wrap(x.node_bounds_max, (d, m), t),
+    d, T = t.ndims, t.eltype
+    return (node_bounds_min = _device_wrap(Ptr{T}(x.node_bounds_min), (d, m), t),
+            node_bounds_max = _device_wrap(Ptr{T}(x.node_bounds_max), (d, m), t),
             node_first = _device_wrap(x.node_first, (m,), t),
             node_last = _device_wrap(x.node_last, (m,), t),
             node_children = _device_wrap(x.node_children, (1 << d, m), t),
@@ -287,8 +295,8 @@ function build_edges(t::TNS, q::Integer, s::Integer; kw...)
     n = Int(e.n_edges)
     return (senders = _device_wrap(e.senders, (n,), t),
             receivers = _device_wrap(e.receivers, (n,), t),
-            rel_displacement = _device_wrap(e.rel_displacement, (t.ndims, n), t),
-            rel_dist_norm = _device_wrap(e.rel_dist_norm, (1, n), t))
+            rel_displacement = _device_wrap(Ptr{t.eltype}(e.rel_displacement), (t.ndims, n), t),
+            rel_dist_norm = _device_wrap(Ptr{t.eltype}(e.rel_dist_norm), (1, n), t))
 end
 
 # ---- neighbour lists (src/api.jl:282-292, 348-362) --------------------------------
@@ -385,12 +393,19 @@ returned, d_disp (3, E) / d_dist (1, E) device arrays or `nothing` (a zero
 tangent), grad (3, N) overwritten with dL/dcoords.
 """
 function build_edges_diff_pullback!(t::TNS, n_points::Integer, e, d_disp, d_dist, radius::Real, grad)
-    edges = OctEdges(pointer(e.senders), pointer(e.receivers), pointer(e.rel_displacement),
-                     pointer(e.rel_dist_norm), Int64(length(e.senders)))
-    _check(t, ccall((:oct_build_edges_backward, libocto), Cint,
-                    (Ptr{Cvoid}, Int64, Ref{OctEdges}, Ptr{Float32}, Ptr{Float32}, Cfloat, Ptr{Float32}),
-                    _enter(t), n_points, edges, d_disp === nothing ? C_NULL : pointer(d_disp),
-                    d_dist === nothing ? C_NULL : pointer(d_dist), Float32(radius), pointer(grad)))
+    edges = OctEdges(pointer(e.senders), pointer(e.receivers), Ptr{Float32}(pointer(e.rel_displacement)),
+                     Ptr{Float32}(pointer(e.rel_dist_norm)), Int64(length(e.senders)))
+    dd = d_disp === nothing ? C_NULL : pointer(d_disp)
+    dn = d_dist === nothing ? C_NULL : pointer(d_dist)
+    if t.eltype === Float64
+        _check(t, ccall((:oct_build_edges_backward64, libocto), Cint,
+                        (Ptr{Cvoid}, Int64, Ref{OctEdges}, Ptr{Float64}, Ptr{Float64}, Cdouble, Ptr{Float64}),
+                        _enter(t), n_points, edges, dd, dn, Float64(radius), pointer(grad)))
+    else
+        _check(t, ccall((:oct_build_edges_backward, libocto), Cint,
+                        (Ptr{Cvoid}, Int64, Ref{OctEdges}, Ptr{Float32}, Ptr{Float32}, Cfloat, Ptr{Float32}),
+                        _enter(t), n_points, edges, dd, dn, Float32(radius), pointer(grad)))
+    end
     return grad
 end
 

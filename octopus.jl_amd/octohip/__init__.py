@@ -76,7 +76,7 @@ def _wrap(ptr, shape, dtype, device, owner):
         n *= s
     if n == 0 or not ptr:
         return torch.empty(shape, dtype=dtype, device=device)
-    typestr = {torch.int32: "<i4", torch.float32: "<f4", torch.int64: "<i8"}[dtype]
+    typestr = {torch.int32: "<i4", torch.float32: "<f4", torch.int64: "<i8", torch.float64: "<f8"}[dtype]
     return torch.as_tensor(_DeviceArray(ptr, shape, typestr, owner), device=device)
 
 
@@ -90,8 +90,8 @@ class TNS:
                  private_stream: bool = False):
         torch = _torch()
         dtype = torch.float32 if dtype is None else dtype
-        if dtype != torch.float32:
-            raise ArgumentError("the HIP engine computes in Float32 only")
+        if dtype not in (torch.float32, torch.float64):
+            raise ArgumentError("TNS(T): T must be Float32 or Float64")
         if ndims not in (2, 3):
             raise ArgumentError("ndims must be 2 (quadtrees) or 3 (octrees)")
         if index_base not in (0, 1):
@@ -118,7 +118,8 @@ class TNS:
             if not torch.cuda.is_available():
                 raise OctopusError("no HIP device: the octohip engine runs on an MI355X only")
             h = ctypes.c_void_p()
-            rc = self._lib.oct_create(ctypes.byref(h), self.device.index, self.ndims, self.index_base)
+            rc = self._lib.oct_create_typed(ctypes.byref(h), self.device.index, self.ndims, self.index_base,
+                                            8 if self.dtype == torch.float64 else 4)
             if rc != L.OCT_OK:
                 raise OctopusError(f"oct_create failed ({rc})")
             self._ctx = h
@@ -132,7 +133,7 @@ class TNS:
             else:
                 self._check(self._lib.oct_set_stream(h, self._caller_stream()))
             if self.radius > 0:
-                self._check(self._lib.oct_set_search_radius(h, self.radius))
+                self._set_radius(h)
             if self.target_leaf_size != 32:
                 self._check(self._lib.oct_set_leaf_size(h, self.target_leaf_size))
             if self.refit_mode:
@@ -159,6 +160,16 @@ class TNS:
         if raw is not None:  # the handle without building a Stream object
             return ctypes.c_void_p(raw(self.device.index))
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    @property
+    def f64(self):
+        return self.dtype == _torch().float64
+
+    def _set_radius(self, h):
+        if self.f64:  # T(r) in double
+            self._check(self._lib.oct_set_search_radius64(h, self.radius))
+        else:
+            self._check(self._lib.oct_set_search_radius(h, self.radius))
 
     def _check(self, rc):
         if rc == L.OCT_OK:
@@ -226,10 +237,10 @@ def set_search_radius(tns: TNS, r):
         raise ArgumentError("search radius must be positive")
     import numpy as np
 
-    tns.radius = float(np.float32(r))
+    tns.radius = float(r) if tns.f64 else float(np.float32(r))
     tns.dirty = [True] * len(tns.dirty)
     if tns._ctx is not None:
-        tns._check(tns._lib.oct_set_search_radius(tns._ctx, tns.radius))
+        tns._set_radius(tns._ctx)
     return tns
 
 
@@ -318,8 +329,8 @@ def get_tree(tns: TNS, set_id: int) -> Tree:
     tns._check(tns._lib.oct_get_tree(tns._c(), int(set_id), ctypes.byref(t)))
     m, n, dev, d = int(t.n_nodes), int(t.n_points), tns.device, tns.ndims
     return Tree(
-        node_bounds_min=_wrap(t.node_bounds_min, (m, d), torch.float32, dev, tns),
-        node_bounds_max=_wrap(t.node_bounds_max, (m, d), torch.float32, dev, tns),
+        node_bounds_min=_wrap(t.node_bounds_min, (m, d), tns.dtype, dev, tns),
+        node_bounds_max=_wrap(t.node_bounds_max, (m, d), tns.dtype, dev, tns),
         node_first=_wrap(t.node_first, (m,), torch.int32, dev, tns),
         node_last=_wrap(t.node_last, (m,), torch.int32, dev, tns),
         node_children=_wrap(t.node_children, (m, 1 << d), torch.int32, dev, tns),
@@ -359,8 +370,8 @@ def build_edges(tns: TNS, q: int, t: int, mode: str = "exact", order: str = "lea
     return Edges(
         senders=_wrap(e.senders, (n,), torch.int32, dev, tns),
         receivers=_wrap(e.receivers, (n,), torch.int32, dev, tns),
-        rel_displacement=_wrap(e.rel_displacement, (n, tns.ndims), torch.float32, dev, tns),
-        rel_dist_norm=_wrap(e.rel_dist_norm, (n, 1), torch.float32, dev, tns),
+        rel_displacement=_wrap(e.rel_displacement, (n, tns.ndims), tns.dtype, dev, tns),
+        rel_dist_norm=_wrap(e.rel_dist_norm, (n, 1), tns.dtype, dev, tns),
     )
 
 
@@ -614,7 +625,7 @@ def apply_zsort(tns: TNS, set_id: int, user_array):
 
 def _pullback(tns: TNS, n_points: int, senders, receivers, rdisp, rdist, d_disp, d_dist, radius):
     torch = _torch()
-    grad = torch.empty((n_points, tns.ndims), dtype=torch.float32, device=tns.device)
+    grad = torch.empty((n_points, tns.ndims), dtype=tns.dtype, device=tns.device)
     e = L.OctEdges()
     e.senders = senders.data_ptr() if senders.numel() else None
     e.receivers = receivers.data_ptr() if receivers.numel() else None
@@ -623,8 +634,8 @@ def _pullback(tns: TNS, n_points: int, senders, receivers, rdisp, rdist, d_disp,
     e.n_edges = senders.numel()
     dd = None if d_disp is None else ctypes.c_void_p(d_disp.contiguous().data_ptr())
     dn = None if d_dist is None else ctypes.c_void_p(d_dist.contiguous().data_ptr())
-    tns._check(tns._lib.oct_build_edges_backward(tns._enter(), n_points, ctypes.byref(e), dd, dn, float(radius),
-                                                 ctypes.c_void_p(grad.data_ptr())))
+    fn = tns._lib.oct_build_edges_backward64 if tns.f64 else tns._lib.oct_build_edges_backward
+    tns._check(fn(tns._enter(), n_points, ctypes.byref(e), dd, dn, float(radius), ctypes.c_void_p(grad.data_ptr())))
     tns._leave()
     return grad
 

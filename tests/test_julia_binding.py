@@ -65,7 +65,7 @@ def header_structs():
 
 _JL_SCALAR = {"Cint": ("int", 0), "Int64": ("int64_t", 0), "Cfloat": ("float", 0), "Float32": ("float", 0),
               "Int32": ("int32_t", 0), "UInt32": ("uint32_t", 0), "UInt64": ("uint64_t", 0),
-              "Cdouble": ("double", 0), "Cstring": ("char", 1)}
+              "Cdouble": ("double", 0), "Float64": ("double", 0), "Cstring": ("char", 1)}
 _JL_STRUCT = {"OctEdges": "oct_edges", "OctTree": "oct_tree", "OctView": "oct_view",
               "OctNeighbors": "oct_neighbors", "OctBuildInfo": "oct_build_info"}
 
@@ -160,10 +160,10 @@ def test_every_ccall_matches_the_header():
 
 def test_binding_covers_the_drop_in_entry_points():
     names = {c[1] for c in julia_ccalls()}
-    for n in ("oct_create", "oct_destroy", "oct_set_stream", "oct_synchronize", "oct_set_search_radius",
+    for n in ("oct_create_typed", "oct_destroy", "oct_set_stream", "oct_synchronize", "oct_set_search_radius64",
               "oct_set_refit_mode", "oct_add_point_set", "oct_update_point_set", "oct_resize_point_set",
               "oct_set_active_search", "oct_run", "oct_get_tree", "oct_build_edges", "oct_device_view",
-              "oct_build_edges_backward", "oct_materialize_neighbors", "oct_get_neighbors", "oct_apply_zsort",
+              "oct_build_edges_backward", "oct_build_edges_backward64", "oct_materialize_neighbors", "oct_get_neighbors", "oct_apply_zsort",
               "oct_apply_zsort_adjoint", "oct_trim", "oct_last_error"):
         assert n in names, n
 
