@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--leaf-size", type=int, default=32)
     ap.add_argument("--run-leaves", type=int, default=6)
     ap.add_argument("--run-chunk", type=int, default=4, help="single-pass runs per block ticket (1..4)")
+    ap.add_argument("--single-form", default="fused", choices=["split", "fused"],
+                    help="learned mode: record + scan + emit kernels, or the fused look-back kernel")
     return ap.parse_args()
 
 
@@ -135,6 +137,7 @@ def run_mode(a, mode, snaps, r):
     tns = octohip.TNS(private_stream=a.private_stream)
     octohip.set_search_radius(tns, float(r))
     octohip.set_refit_mode(tns, not a.no_refit)
+    octohip.set_single_form(tns, a.single_form == "split")
     sid = octohip.add_point_set(tns, snaps[0])
     octohip.set_active_search(tns, sid, sid)
     api_mode = "exact" if mode == "exact" else "learned"

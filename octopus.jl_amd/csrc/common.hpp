@@ -187,6 +187,25 @@ __device__ inline int load_coherent(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Correctly rounded FP32 square root of x >= SQRT_DIRECT_MIN (finite) and of
+// +0: the hardware estimate (within 1 ulp), then the residual of each
+// neighbour, x - s_down * s and x - s_up * s, in one fused multiply-add:
+// x <= s_down * s puts sqrt(x) below the midpoint of s_down and s, x > s_up *
+// s above that of s and s_up (no float x lies between such a product and
+// the midpoint's square).  These are the steps sqrtf itself takes on this
+// target for such inputs; below SQRT_DIRECT_MIN it first scales x by 2^32 so
+// the residuals stay normal, which callers get by calling sqrtf there.
+constexpr float SQRT_DIRECT_MIN = 0x1p-96f;
+__device__ inline float sqrt_rn_direct(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sdn = __int_as_float(__float_as_int(s) - 1);
+    const float sup = __int_as_float(__float_as_int(s) + 1);
+    const float rdn = fmaf(-sdn, s, x);
+    const float rup = fmaf(-sup, s, x);
+    const float y = rdn <= 0.0f ? sdn : s;
+    return rup > 0.0f ? sup : y;
+}
+
 // FP32 squared distance from a point to a box, in the reference's order
 // (src/cpu/iterator_cpu.jl:31-40).
 __device__ inline float point_box_d2(float px, float py, float pz, float4 lo, float4 hi) {

@@ -12,6 +12,8 @@
 //   d2 = (dx*dx + dy*dy) + dz*dz <= r*r,
 // so the edge set is bit-identical to the CPU path.  The emit pass stages a
 // leaf's edges in LDS and writes them out with whole-wave contiguous stores.
+#include <type_traits>
+
 #include "internal.hpp"
 
 namespace oct {
@@ -499,14 +501,75 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
     return leaf_total;
 }
 
+// Hit-list entry: compact = query slot in the leaf (6 bits) << 26 | target
+// sorted position (needs fewer than 2^26 target points); wide = (query
+// sorted position, target sorted position).
+constexpr uint32_t HIT_POS_BITS = 26;
+constexpr uint32_t HIT_POS_MASK = (1u << HIT_POS_BITS) - 1u;
+
+// Filtered candidates one wave stages in LDS (x, y, z, sorted position); the
+// same space holds the slot -> sorted position table before that.
+constexpr int SP_STAGE_PTS = 128;
+
+// The query-outer sweep of one register tile (nu chunks of 64 candidates,
+// w = sorted position) for the leaf's queries: hits are appended at
+// hits[run ..] while they fit `room`; returns the leaf's exact hit count.
+// The self pair is the candidate at the query's own sorted position.  With
+// COUNTS the per-query counts go to qcount instead (the list overflowed).
+template <bool WIDE, bool COUNTS>
+__device__ inline uint32_t sweep_tile(int32_t qf, int nqb, float4 myq, const float4 (&tp)[EW_TILE],
+                                      const int32_t (&tpos)[EW_TILE], int nu, float r_sq, int self_pair,
+                                      uint32_t* hits, uint32_t room, uint32_t* __restrict__ qcount, int lane) {
+    uint32_t run = 0;
+    uint32_t mine = 0;
+    for (int qi = 0; qi < nqb; ++qi) {
+        const uint32_t run0 = run;
+        const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
+        const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
+        const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
+        const int32_t sp = self_pair ? qf + qi : -1;  // -1 matches no sorted position
+        const uint32_t tag = (uint32_t)qi << HIT_POS_BITS;
+        // one query against the tile; CHECK tests each hit against the room
+        // (only for a query whose hits might not all fit)
+        auto query = [&](auto check) {
+            constexpr bool CHECK = decltype(check)::value;
+#pragma unroll
+            for (int u = 0; u < EW_TILE; ++u) {
+                if (u >= nu) break;
+                const float dx = qx - tp[u].x;
+                const float dy = qy - tp[u].y;
+                const float dz = qz - tp[u].z;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                const bool in_r = d2 <= r_sq;
+                const bool other = tpos[u] != sp;
+                const uint64_t m = ballot(in_r) & ballot(other);  // compare masks ANDed, no branch
+                if (!COUNTS && (in_r & other)) {
+                    const uint32_t k = (uint32_t)mbcnt(m);
+                    if (!CHECK || run + k < room) {
+                        if (WIDE) (reinterpret_cast<int2*>(hits) + run)[k] = make_int2(qf + qi, tpos[u]);
+                        else (hits + run)[k] = tag | (uint32_t)tpos[u];
+                    }
+                }
+                run += (uint32_t)__popcll(m);
+            }
+        };
+        if (COUNTS || run + (uint32_t)nu * OCT_WAVE <= room) query(std::false_type{});
+        else query(std::true_type{});
+        if (COUNTS && lane == qi) mine = run - run0;
+    }
+    if (COUNTS && lane < nqb) qcount[qf + lane] = mine;
+    return run;
+}
+
 // A leaf whose candidates fit one register tile (at most 64 queries, 64
-// candidate leaves and 4 x 64 candidate points) is handled in one sweep: the
-// loop is query-outer, so the hits come out grouped by query, queries in
-// sorted order and each query's hits in candidate order, which is exactly the
-// output order.  Hits are appended to the wave's LDS hit list as (query
-// sorted position, target sorted position) while they fit; the return value
-// is the exact hit count either way (a count above `room` means the list
-// could not hold them all).
+// candidate leaves and 4 x 64 candidate points after the point-box filter)
+// is handled in one sweep: the loop is query-outer, so the hits come out
+// grouped by query, queries in sorted order and each query's hits in
+// candidate order, which is exactly the output order.  Hits are appended to
+// the wave's LDS hit list as (query slot, target sorted position) while they
+// fit; the return value is the exact hit count either way (a count above
+// `room` means the list could not hold them all, and the per-query counts are
+// then in qcount).
 constexpr uint32_t REC_NOT_TILE = 0xffffffffu;        // the filtered candidates do not fit one register tile
 
 __device__ inline bool recordable_leaf(int32_t qc, int32_t ncand, int total) {
@@ -565,17 +628,69 @@ __device__ inline int load_filtered_tile(int total, const float4* __restrict__ t
     return kept;
 }
 
-// Hit-list entry: compact = query slot in the leaf (6 bits) << 26 | target
-// sorted position (needs fewer than 2^26 target points); wide = (query
-// sorted position, target sorted position).
-constexpr uint32_t HIT_POS_BITS = 26;
-constexpr uint32_t HIT_POS_MASK = (1u << HIT_POS_BITS) - 1u;
+// Loads the leaf's candidates (the sequence of its candidate leaves, at most
+// one register tile of them) for leaf_sweep_record: a slot -> sorted
+// position table in LDS (each candidate leaf writes its own slots, so no
+// search), the gather, then the point-box filter (its FP32 distance to the
+// query box is a lower bound of the distance to every query in the box, so
+// no edge is lost).  When the filter saves at least one chunk of 64 and the
+// kept points fit the stage, they are compacted through LDS in sequence order
+// (no second gather).  Returns the number of candidates in tp / tpos.
+__device__ inline int load_record_tile(int2 lf, int incl, int total, const float4* __restrict__ tsc, float4* stage,
+                                       int lane, float4 qlo, float4 qhi, float r_sq, float4 (&tp)[EW_TILE],
+                                       int32_t (&tpos)[EW_TILE]) {
+    int32_t* tbl = reinterpret_cast<int32_t*>(stage);  // total <= 256 entries: 1 KB of the stage
+    const int ex = incl - lf.y;
+    for (int k = 0; ballot(k < lf.y) != 0; ++k)
+        if (k < lf.y) tbl[ex + k] = lf.x + k;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < EW_TILE; ++u) {
+        const int ci = u * OCT_WAVE + lane;
+        tpos[u] = ci < total ? tbl[ci] : 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    bool keep[EW_TILE];
+    uint64_t km[EW_TILE];
+    int kept = 0;
+#pragma unroll
+    for (int u = 0; u < EW_TILE; ++u) {
+        const int ci = u * OCT_WAVE + lane;
+        tp[u] = make_float4(INFINITY, INFINITY, INFINITY, 0.f);  // never within r
+        if (ci < total) tp[u] = tsc[tpos[u]];
+        keep[u] = ci < total && point_box_d2(tp[u].x, tp[u].y, tp[u].z, qlo, qhi) <= r_sq;
+        km[u] = ballot(keep[u]);
+        kept += __popcll(km[u]);
+    }
+    const int nu_all = (total + OCT_WAVE - 1) / OCT_WAVE;
+    if (kept > SP_STAGE_PTS || (kept + OCT_WAVE - 1) / OCT_WAVE >= nu_all) return total;  // wave-uniform
+    int base = 0;
+#pragma unroll
+    for (int u = 0; u < EW_TILE; ++u) {
+        if (keep[u])
+            stage[base + mbcnt(km[u])] = make_float4(tp[u].x, tp[u].y, tp[u].z, __int_as_float(tpos[u]));
+        base += __popcll(km[u]);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < EW_TILE; ++u) {
+        const int k = u * OCT_WAVE + lane;
+        tp[u] = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+        tpos[u] = 0;
+        if (u * OCT_WAVE < SP_STAGE_PTS && k < kept) {
+            tp[u] = stage[k];
+            tpos[u] = __float_as_int(tp[u].w);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    return kept;
+}
 
 template <bool WIDE>
 __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, float4 qlo, float4 qhi, int32_t ncand, int2 lf0,
                                              const float4* __restrict__ qsc, const float4* __restrict__ tsc,
-                                             float r_sq, int self_pair, const WaveLds& L, int lane, uint32_t* hits,
-                                             uint32_t room, uint32_t* __restrict__ qcount) {
+                                             float r_sq, int self_pair, const WaveLds& L, float4* stage, int lane,
+                                             uint32_t* hits, uint32_t room, uint32_t* __restrict__ qcount) {
     const int nqb = qc;
     float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);
     if (lane < nqb) myq = qsc[qf + lane];
@@ -583,54 +698,25 @@ __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, float4 qlo,
     if (lane < ncand) lf = lf0;
     const int incl = wave_incl_scan(lf.y);
     const int total = __builtin_amdgcn_readlane(incl, OCT_WAVE - 1);
-    L.start[lane + 1] = incl;
-    if (lane == 0) L.start[0] = 0;
-    L.first[lane] = lf.x;
-    __builtin_amdgcn_wave_barrier();
     float4 tp[EW_TILE];
     int32_t tpos[EW_TILE];
-    // candidates that fit one tile as they are skip the filter and its second
-    // gather (measured: single pass 446 -> 433 us at the SPH workload; the
-    // two-pass fill is faster with the filter, 337 against 378 us, so it keeps it)
-    const int got = total <= EW_TILE * OCT_WAVE
-                        ? load_tile(0, total, tsc, L, lane, false, qlo, qhi, r_sq, tp, tpos)
-                        : load_filtered_tile(total, tsc, L, lane, qlo, qhi, r_sq, tp, tpos);
-    if (got < 0) return REC_NOT_TILE;
-    const int nu = (got + OCT_WAVE - 1) / OCT_WAVE;
-    uint32_t run = 0;
-    uint32_t mine = 0;  // hits of this lane's query
-    for (int qi = 0; qi < nqb; ++qi) {
-        const uint32_t run0 = run;
-        const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
-        const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
-        const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), qi));
-        const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), qi);
-        const int32_t ix = self_pair ? i : -2;  // -2 matches no point: only the self pair drops j == i
-#pragma unroll
-        for (int u = 0; u < EW_TILE; ++u) {
-            if (u >= nu) break;
-            const int32_t j = __float_as_int(tp[u].w);
-            const float dx = qx - tp[u].x;
-            const float dy = qy - tp[u].y;
-            const float dz = qz - tp[u].z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            const bool in_r = d2 <= r_sq;
-            const bool other = j != ix;
-            const uint64_t m = ballot(in_r) & ballot(other);  // compare masks ANDed, no branch
-            if (in_r & other) {
-                const uint32_t h = run + (uint32_t)mbcnt(m);
-                if (h < room) {
-                    if (WIDE) reinterpret_cast<int2*>(hits)[h] = make_int2(qf + qi, tpos[u]);
-                    else hits[h] = ((uint32_t)qi << HIT_POS_BITS) | (uint32_t)tpos[u];
-                }
-            }
-            run += (uint32_t)__popcll(m);
-        }
-        if (lane == qi) mine = run - run0;
+    int got;
+    if (total <= EW_TILE * OCT_WAVE) {
+        got = load_record_tile(lf, incl, total, tsc, stage, lane, qlo, qhi, r_sq, tp, tpos);
+    } else {
+        L.start[lane + 1] = incl;
+        if (lane == 0) L.start[0] = 0;
+        L.first[lane] = lf.x;
+        __builtin_amdgcn_wave_barrier();
+        got = load_filtered_tile(total, tsc, L, lane, qlo, qhi, r_sq, tp, tpos);
+        if (got < 0) return REC_NOT_TILE;
     }
+    const int nu = (got + OCT_WAVE - 1) / OCT_WAVE;
+    const uint32_t run = sweep_tile<WIDE, false>(qf, nqb, myq, tp, tpos, nu, r_sq, self_pair, hits, room, qcount,
+                                                 lane);
     // a list that could not hold every hit: the per-query counts let the
     // caller write this leaf with the FILL body without another count sweep
-    if (run > room && lane < nqb) qcount[qf + lane] = mine;
+    if (run > room) sweep_tile<WIDE, true>(qf, nqb, myq, tp, tpos, nu, r_sq, self_pair, hits, room, qcount, lane);
     return run;
 }
 
@@ -638,31 +724,36 @@ __device__ inline uint32_t leaf_sweep_record(int32_t qf, int32_t qc, float4 qlo,
 // every lane takes consecutive hits, so the stores of one instruction are
 // contiguous (the displacement as one 12-byte store per edge).  EMIT_BATCH
 // hits per lane are loaded before any is written, so their gathers overlap.
+// The leaf's output range is addressed from scalar bases with 32-bit
+// per-lane byte offsets.
 constexpr int EMIT_BATCH = 4;
 
 template <bool WIDE>
 __device__ inline void emit_hits(const uint32_t* hits, uint32_t n, uint32_t base, int32_t qf,
                                  const float4* __restrict__ qsc, const float4* __restrict__ tsc, float inv_r,
                                  EdgeOut out, int index_base, int lane) {
+    char* const sen = reinterpret_cast<char*>(out.senders + base);
+    char* const rcv = reinterpret_cast<char*>(out.receivers + base);
+    char* const dsp = reinterpret_cast<char*>(out.rdisp + 3 * (int64_t)base);
+    char* const dst = reinterpret_cast<char*>(out.rdist + base);
+    const float4* const qb = qsc + qf;
     for (uint32_t t0 = 0; t0 < n; t0 += EMIT_BATCH * OCT_WAVE) {
         float4 q[EMIT_BATCH], p[EMIT_BATCH];
 #pragma unroll
         for (int k = 0; k < EMIT_BATCH; ++k) {
             const uint32_t t = t0 + (uint32_t)(k * OCT_WAVE + lane);
-            int32_t qp = qf, tp = 0;
-            if (t < n) {
-                if (WIDE) {
-                    const int2 h = reinterpret_cast<const int2*>(hits)[t];
-                    qp = h.x;
-                    tp = h.y;
-                } else {
-                    const uint32_t h = hits[t];
-                    qp = qf + (int32_t)(h >> HIT_POS_BITS);
-                    tp = (int32_t)(h & HIT_POS_MASK);
-                }
+            if (WIDE) {
+                int2 h = make_int2(qf, 0);
+                if (t < n) h = reinterpret_cast<const int2*>(hits)[t];
+                q[k] = qsc[h.x];
+                p[k] = tsc[h.y];
+            } else {
+                uint32_t h = 0;
+                if (t < n) h = hits[t];
+                // the query slot (< 64) and the target position (< 2^26) as byte offsets that fit 32 bits
+                q[k] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(qb) + ((h >> HIT_POS_BITS) << 4));
+                p[k] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(tsc) + ((h & HIT_POS_MASK) << 4));
             }
-            q[k] = qsc[qp];
-            p[k] = tsc[tp];
         }
 #pragma unroll
         for (int k = 0; k < EMIT_BATCH; ++k) {
@@ -670,13 +761,13 @@ __device__ inline void emit_hits(const uint32_t* hits, uint32_t n, uint32_t base
             if (t >= n) break;
             const float dx = q[k].x - p[k].x, dy = q[k].y - p[k].y, dz = q[k].z - p[k].z;
             const float d2 = dx * dx + dy * dy + dz * dz;
-            const uint32_t e = base + t;
-            out.senders[e] = __float_as_int(p[k].w) + index_base;
-            out.receivers[e] = __float_as_int(q[k].w) + index_base;
-            *reinterpret_cast<float3*>(out.rdisp + 3 * (int64_t)e) =
+            const float sd = ballot(d2 < SQRT_DIRECT_MIN) ? sqrtf(d2) : sqrt_rn_direct(d2);
+            *reinterpret_cast<int32_t*>(sen + (t << 2)) = __float_as_int(p[k].w) + index_base;
+            *reinterpret_cast<int32_t*>(rcv + (t << 2)) = __float_as_int(q[k].w) + index_base;
+            *reinterpret_cast<float3*>(dsp + t * 12u) =
                 make_float3(rel_disp(q[k].x, p[k].x, inv_r), rel_disp(q[k].y, p[k].y, inv_r),
                             rel_disp(q[k].z, p[k].z, inv_r));
-            out.rdist[e] = sqrtf(d2) * inv_r;
+            *reinterpret_cast<float*>(dst + (t << 2)) = sd * inv_r;
         }
     }
 }
@@ -854,6 +945,7 @@ constexpr int SP_HIT_WORDS = 1536;                 // hit list of one wave: 6 KB
 static_assert(SP_SIG_MARK % 2 == 0, "the marker is a 64-bit word");
 constexpr uint32_t SP_STAGE = SP_HIT_WORDS * 4 / 24;  // FILL stage laid over the hit list: 170 edges
 static_assert(SP_PLAN_BUDGET == SP_HIT_WORDS, "the plan packs one wave's compact hit list");
+static_assert(SP_STAGE_PTS * 4 >= EW_TILE * OCT_WAVE, "the stage holds the slot table and the filtered positions");
 
 // Run hand-out of the single pass.  A block takes `chunk` (1..EW_WAVES)
 // consecutive runs per global ticket (neighbouring runs share candidate
@@ -992,17 +1084,18 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
     __shared__ uint32_t s_hits[EW_WAVES][SP_HIT_WORDS];
     __shared__ uint32_t s_cnt[EW_WAVES][SP_MAX_LEAVES];
     __shared__ int32_t s_qf[EW_WAVES][SP_MAX_LEAVES];
-    __shared__ int32_t s_cand[EW_WAVES][EW_TILE * OCT_WAVE];
+    __shared__ float4 s_stage[EW_WAVES][SP_STAGE_PTS];  // record tiles; positions of the filtered candidates
     __shared__ unsigned long long s_q;  // run hand-out: ticket << SP_Q_SHIFT | runs taken from it
     __shared__ uint32_t s_stat[2];
-    const int wv = threadIdx.x / OCT_WAVE;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / OCT_WAVE));  // scalar LDS bases
     const int lane = lane_id();
     uint32_t* hits = s_hits[wv];
     uint32_t* cntw = s_cnt[wv];
     int32_t* qfw = s_qf[wv];
     int32_t* stage = reinterpret_cast<int32_t*>(hits);
     const WaveLds L{s_start[wv], s_first[wv], stage, stage + SP_STAGE, reinterpret_cast<float*>(stage + 2 * SP_STAGE),
-                    reinterpret_cast<float*>(stage + 5 * SP_STAGE), SP_STAGE, s_cand[wv]};
+                    reinterpret_cast<float*>(stage + 5 * SP_STAGE), SP_STAGE,
+                    reinterpret_cast<int32_t*>(s_stage[wv])};
     const int32_t nleaves = qprm->n_leaves;
     const int32_t nruns = wave_first ? *plan_waves : (nleaves + leaves_per_run - 1) / leaves_per_run;
     if (threadIdx.x == 0) s_q = (SP_Q_NONE << SP_Q_SHIFT) | (unsigned long long)run_chunk;
@@ -1052,7 +1145,7 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
                 c = REC_NOT_TILE;
                 if (recordable_leaf(qc, ncand, total))
                     c = leaf_sweep_record<WIDE>(qf, qc, qa.lo, qa.hi, ncand, lf0, qsc, tsc, lr_sq, self_pair, L,
-                                                lane, WIDE ? hits + 2 * hfill : hits + hfill, HITS - hfill, qcount);
+                                                s_stage[wv], lane, WIDE ? hits + 2 * hfill : hits + hfill, HITS - hfill, qcount);
                 if (c != REC_NOT_TILE) {
                     if (c <= HITS - hfill) {  // else the list is full: qcount holds the per-query counts
                         recorded |= 1ull << k;
@@ -1106,16 +1199,18 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
         if (!ok || incl > cap_edges) continue;  // wave-uniform
         __builtin_amdgcn_wave_barrier();
         // phase B: recorded leaves straight from the hit list ...
-        uint32_t pos = (uint32_t)excl;
+        // (scalar offsets, so the emit addresses from scalar bases)
+        uint32_t pos = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)excl);
         uint32_t hread = 0;
         for (int32_t r = r_lo; r < r_hi; ++r) {
             const int k = r - r_lo;
-            const uint32_t c = cntw[k];
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)cntw[k]);
             if (recorded & (1ull << k)) {
+                const int32_t qf = __builtin_amdgcn_readfirstlane(qfw[k]);
                 float lr_sq, linv_r;
                 int libase;
-                leaf_set_params(sets, qfw[k], r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
-                emit_hits<WIDE>(WIDE ? hits + 2 * hread : hits + hread, c, pos, qfw[k], qsc, tsc, linv_r, out,
+                leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
+                emit_hits<WIDE>(WIDE ? hits + 2 * hread : hits + hread, c, pos, qf, qsc, tsc, linv_r, out,
                                 libase, lane);
                 hread += c;
             }
@@ -1255,6 +1350,307 @@ hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2*
                            q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first,
                            plan_waves, lhits, tile_state, ticket, flags, stats, cap_edges, out, index_base,
                            leaves_per_run, run_chunk, sets, lheavy, sig, sig_src, sig_words);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ split single pass
+//
+// The single pass as two kernels with a scan of the run totals between them,
+// so that no run's emit waits on the runs before it (in the fused kernel the
+// look-back held each wave until the slowest phase A of the runs in flight
+// had published).  k_sp_record sweeps every leaf of its runs once: a leaf
+// whose candidates fit one register tile records its hits in the run's slot
+// of a hit buffer in HBM (SP_HIT_WORDS words per run, compact or wide
+// entries as in the LDS hit list) while they fit, any other leaf keeps its
+// per-query counts; it writes each leaf's exact count (lhits), the run's total
+// and its recorded-leaf mask.  An exclusive scan of the run totals gives each
+// run's output offset and the call's total.  k_sp_emit writes every run's
+// edges: recorded leaves straight from the hit buffer, the others re-swept
+// with the FILL body, heavy leaves left to k_edges_heavy_fill as in the
+// fused kernel.  Both kernels take XCD-local contiguous shares of the runs,
+// and none of their waves waits on another.
+
+// Runs of the call: the plan's, or fixed runs of leaves_per_run leaves.
+__device__ inline bool sp_run_range(int64_t w, int32_t nleaves, const int32_t* __restrict__ wave_first,
+                                    int leaves_per_run, int32_t& r_lo, int32_t& r_hi) {
+    if (wave_first) {
+        r_lo = wave_first[w];
+        r_hi = wave_first[w + 1];
+    } else {
+        r_lo = (int32_t)w * leaves_per_run;
+        r_hi = min(nleaves, r_lo + leaves_per_run);
+    }
+    r_lo = __builtin_amdgcn_readfirstlane(r_lo);
+    r_hi = __builtin_amdgcn_readfirstlane(r_hi);
+    // a malformed plan sweeps nothing: the host then takes the exact path
+    return r_lo >= 0 && r_hi <= nleaves && r_hi >= r_lo && r_hi - r_lo <= SP_MAX_LEAVES;
+}
+
+__device__ inline int32_t sp_run_count(int32_t nleaves, const int32_t* __restrict__ plan_waves, int leaves_per_run) {
+    return plan_waves ? *plan_waves : (nleaves + leaves_per_run - 1) / leaves_per_run;
+}
+
+template <bool WIDE>
+__global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
+k_sp_record(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm,
+            const float4* __restrict__ tsc, const int2* __restrict__ lists, const int32_t* __restrict__ lcount,
+            int cap, float r_sq, float inv_r, int self_pair, uint32_t* __restrict__ qcount,
+            const int32_t* __restrict__ wave_first, const int32_t* __restrict__ plan_waves,
+            uint32_t* __restrict__ lhits, uint32_t* __restrict__ run_tot, uint64_t* __restrict__ run_mask,
+            int64_t n_bound, uint32_t* __restrict__ hitbuf, int64_t n_slots, uint32_t* __restrict__ flags,
+            uint32_t* __restrict__ stats, int leaves_per_run, SetLookup sets, const uint8_t* __restrict__ lheavy) {
+    constexpr uint32_t HITS = WIDE ? SP_HIT_WORDS / 2 : SP_HIT_WORDS;
+    __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
+    __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
+    __shared__ float4 s_stage[EW_WAVES][SP_STAGE_PTS];  // record tiles; positions of the filtered candidates
+    __shared__ uint32_t s_stat[2];
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / OCT_WAVE));
+    const int lane = lane_id();
+    const WaveLds L{s_start[wv], s_first[wv], nullptr, nullptr, nullptr, nullptr, 0,
+                    reinterpret_cast<int32_t*>(s_stage[wv])};
+    const int32_t nleaves = qprm->n_leaves;
+    const int32_t nruns = sp_run_count(nleaves, plan_waves, leaves_per_run);
+    const XcdShare sh = xcd_share(n_bound);
+    uint32_t n_recount = 0, n_record = 0;
+    for (int64_t w = sh.lo + (int64_t)sh.k * EW_WAVES + wv; w < sh.hi; w += (int64_t)sh.nb * EW_WAVES) {
+        if (w >= nruns) {  // past the call's runs: a zero total for the scan
+            if (lane == 0) {
+                run_tot[w] = 0u;
+                run_mask[w] = 0ull;
+            }
+            continue;
+        }
+        int32_t r_lo, r_hi;
+        if (!sp_run_range(w, nleaves, wave_first, leaves_per_run, r_lo, r_hi)) {
+            if (lane == 0) {
+                atomicOr(&flags[1], 2u);
+                run_tot[w] = 0u;
+                run_mask[w] = 0ull;
+            }
+            continue;
+        }
+        // the run's slot of the hit buffer (runs past the slots record nothing)
+        uint32_t* hits = w < n_slots ? hitbuf + w * (int64_t)SP_HIT_WORDS : nullptr;
+        const uint32_t hcap = hits ? HITS : 0u;
+        uint64_t recorded = 0;  // bit k: leaf r_lo + k sits in the hit buffer
+        uint32_t hfill = 0, wtot = 0, nrec = 0;
+        for (int32_t r = r_lo; r < r_hi; ++r) {
+            const int k = r - r_lo;
+            uint32_t c;
+            if (lheavy && __builtin_amdgcn_readfirstlane(lheavy[r]) != 0) {
+                c = __builtin_amdgcn_readfirstlane(lhits[r]);  // counted by the heavy-leaf pass
+            } else {
+                const NodeRec qa = qleaves[r];
+                const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+                const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+                const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+                const int2* lst = lists + (int64_t)r * cap;
+                const int2 lf0 = lst[lane < ncand ? lane : 0];
+                const int total = __builtin_amdgcn_readfirstlane(wave_sum(lane < ncand ? lf0.y : 0));
+                float lr_sq, linv_r;
+                int libase;
+                leaf_set_params(sets, qf, r_sq, inv_r, 0, lr_sq, linv_r, libase);
+                c = REC_NOT_TILE;
+                if (recordable_leaf(qc, ncand, total))
+                    c = leaf_sweep_record<WIDE>(qf, qc, qa.lo, qa.hi, ncand, lf0, qsc, tsc, lr_sq, self_pair, L,
+                                                s_stage[wv], lane, hits ? (WIDE ? hits + 2 * hfill : hits + hfill)
+                                                                        : nullptr,
+                                                hcap - hfill, qcount);
+                if (c != REC_NOT_TILE) {
+                    if (c <= hcap - hfill) {  // else the slot is full: qcount holds the per-query counts
+                        recorded |= 1ull << k;
+                        hfill += c;
+                        ++nrec;
+                    }
+                } else {
+                    c = leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r,
+                                              self_pair, 0, qcount, nullptr, EdgeOut{}, libase, L, lane);
+                }
+                if (lane == 0) lhits[r] = c;
+            }
+            wtot += c;
+        }
+        n_recount += (uint32_t)(r_hi - r_lo) - nrec;
+        n_record += nrec;
+        if (lane == 0) {
+            run_tot[w] = wtot;
+            run_mask[w] = recorded;
+        }
+    }
+    // one pair of global atomics per block
+    if (threadIdx.x == 0) s_stat[0] = s_stat[1] = 0;
+    __syncthreads();
+    if (lane == 0) {
+        atomicAdd(&s_stat[0], n_recount);
+        atomicAdd(&s_stat[1], n_record);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && (s_stat[0] | s_stat[1])) {
+        atomicAdd(&stats[0], s_stat[0]);
+        atomicAdd(&stats[1], s_stat[1]);
+    }
+}
+
+// Writes the edges of every run at its scanned offset.  Block 0's first wave
+// first settles the call: a total above cap_edges raises flags[0] (nothing
+// is written, the host regrows and redoes the call), then the flag words go
+// to the host's signal block with the marker (the host returns on them while
+// the emit runs).  The total itself is already at flags[2..3] (the scan's).
+template <bool WIDE>
+__global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
+k_sp_emit(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm,
+          const float4* __restrict__ tsc, const int2* __restrict__ lists, const int32_t* __restrict__ lcount, int cap,
+          float r_sq, float inv_r, int self_pair, uint32_t* __restrict__ qcount, uint32_t* __restrict__ qoff,
+          const int32_t* __restrict__ wave_first, const int32_t* __restrict__ plan_waves,
+          const uint32_t* __restrict__ lhits, const uint32_t* __restrict__ run_off,
+          const uint64_t* __restrict__ run_mask, int64_t n_bound, const uint32_t* __restrict__ hitbuf,
+          int64_t n_slots, uint32_t* __restrict__ flags, uint64_t cap_edges, EdgeOut out, int index_base0,
+          int leaves_per_run, SetLookup sets, const uint8_t* __restrict__ lheavy, uint32_t* __restrict__ sig,
+          const uint32_t* __restrict__ sig_src, int sig_words) {
+    constexpr uint32_t STAGE_WORDS = 1024;                // FILL stage of one wave
+    constexpr uint32_t STAGE = STAGE_WORDS * 4 / 24;      // 170 edges
+    __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
+    __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
+    __shared__ int32_t s_fill[EW_WAVES][STAGE_WORDS];
+    __shared__ int32_t s_cand[EW_WAVES][EW_TILE * OCT_WAVE];
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / OCT_WAVE));
+    const int lane = lane_id();
+    int32_t* fs = s_fill[wv];
+    const WaveLds L{s_start[wv], s_first[wv], fs, fs + STAGE, reinterpret_cast<float*>(fs + 2 * STAGE),
+                    reinterpret_cast<float*>(fs + 5 * STAGE), STAGE, s_cand[wv]};
+    const uint64_t total = *reinterpret_cast<const uint64_t*>(flags + 2);
+    const bool over = total > cap_edges;
+    if (blockIdx.x == 0 && wv == 0) {
+        if (over && lane == 0) atomicOr(&flags[0], 1u);
+        if (sig) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+            for (int i = lane; i < sig_words; i += OCT_WAVE)
+                sig[i] = __hip_atomic_load(&sig_src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            if (lane == 0)
+                __hip_atomic_store(reinterpret_cast<unsigned long long*>(sig + SP_SIG_MARK), 1ull + total,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+    if (over || flags[1] != 0) return;  // capacity, or a malformed plan: the host redoes the call
+    const int32_t nleaves = qprm->n_leaves;
+    const int32_t nruns = sp_run_count(nleaves, plan_waves, leaves_per_run);
+    const XcdShare sh = xcd_share(n_bound);
+    for (int64_t w = sh.lo + (int64_t)sh.k * EW_WAVES + wv; w < sh.hi; w += (int64_t)sh.nb * EW_WAVES) {
+        if (w >= nruns) break;  // (runs ascend within the share)
+        int32_t r_lo, r_hi;
+        if (!sp_run_range(w, nleaves, wave_first, leaves_per_run, r_lo, r_hi)) continue;
+        const uint64_t recorded = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(run_mask[w] >> 32))
+                                   << 32) |
+                                  (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)run_mask[w]);
+        const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)run_off[w]);
+        const uint32_t* hits = w < n_slots ? hitbuf + w * (int64_t)SP_HIT_WORDS : nullptr;
+        // recorded leaves straight from the hit buffer ...
+        uint32_t pos = base, hread = 0;
+        for (int32_t r = r_lo; r < r_hi; ++r) {
+            const int k = r - r_lo;
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)lhits[r]);
+            if (hits && (recorded & (1ull << k))) {
+                const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qleaves[r]));
+                float lr_sq, linv_r;
+                int libase;
+                leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
+                emit_hits<WIDE>(WIDE ? hits + 2 * hread : hits + hread, c, pos, qf, qsc, tsc, linv_r, out, libase,
+                                lane);
+                hread += c;
+            }
+            pos += c;
+        }
+        // ... then the others re-swept (a heavy leaf only gets its output
+        // position, in qoff at its first query, for k_edges_heavy_fill)
+        if (recorded == (r_hi - r_lo == 64 ? ~0ull : ((1ull << (r_hi - r_lo)) - 1ull))) continue;
+        pos = base;
+        for (int32_t r = r_lo; r < r_hi; ++r) {
+            const int k = r - r_lo;
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)lhits[r]);
+            // (a leaf without edges writes nothing; its qcount may be stale)
+            if (c != 0 && !(hits && (recorded & (1ull << k)))) {
+                const NodeRec qa = qleaves[r];
+                const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
+                if (lheavy && __builtin_amdgcn_readfirstlane(lheavy[r]) != 0) {
+                    if (lane == 0) qoff[qf] = pos;
+                } else {
+                    const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
+                    uint32_t run = pos;
+                    for (int qb = 0; qb < qc; qb += OCT_WAVE) {
+                        const int nqb = min(OCT_WAVE, qc - qb);
+                        const uint32_t cq = lane < nqb ? qcount[qf + qb + lane] : 0u;  // this lane's, from the record
+                        const uint32_t inc = (uint32_t)wave_incl_scan((int)cq);
+                        if (lane < nqb) qoff[qf + qb + lane] = run + inc - cq;
+                        run += (uint32_t)__builtin_amdgcn_readlane((int)inc, OCT_WAVE - 1);
+                    }
+                    const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
+                    const int2* lst = lists + (int64_t)r * cap;
+                    const int2 lf0 = lst[lane < ncand ? lane : 0];
+                    float lr_sq, linv_r;
+                    int libase;
+                    leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
+                    leaf_pass<EDGE_FILL>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r, self_pair, 0,
+                                         qcount, qoff, out, libase, L, lane);
+                }
+            }
+            pos += c;
+        }
+    }
+}
+
+
+// Hit-buffer slots the split single pass records into: one per fixed run, or
+// for a plan (whose runs the host does not know) twice the fixed-run count;
+// runs past the slots record nothing and are re-swept (exact, slower).
+int64_t split_bound(int64_t n_leaves, int leaves_per_run, bool plan) {
+    return plan ? n_leaves + 1 : (n_leaves + leaves_per_run - 1) / leaves_per_run;
+}
+int64_t split_slots(int64_t n_leaves, int leaves_per_run, bool plan) {
+    const int64_t fixed = (n_leaves + leaves_per_run - 1) / leaves_per_run;
+    return plan ? std::min<int64_t>(n_leaves + 1, 2 * fixed) : fixed;
+}
+
+hipError_t launch_edges_split(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount, int cap,
+                              float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
+                              const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
+                              uint32_t* run_tot, uint32_t* run_off, uint64_t* run_mask, uint32_t* hitbuf,
+                              uint32_t* scan_tmp, uint32_t* flags, uint32_t* stats, uint64_t cap_edges, EdgeOut out,
+                              int index_base, int leaves_per_run, const SetLookup& sets, const uint8_t* lheavy,
+                              uint32_t* sig, const uint32_t* sig_src, int sig_words, hipStream_t s) {
+    if (sig && (sig_words < 0 || sig_words > SP_SIG_MARK)) return hipErrorInvalidValue;
+    if (leaves_per_run < 1 || leaves_per_run > SP_MAX_LEAVES) return hipErrorInvalidValue;
+    if (!lhits || (wave_first && !plan_waves)) return hipErrorInvalidValue;
+    const bool plan = wave_first != nullptr;
+    const int64_t n_bound = split_bound(q.n_leaves, leaves_per_run, plan);
+    const int64_t n_slots = split_slots(q.n_leaves, leaves_per_run, plan);
+    if (n_bound <= 0) return hipSuccess;
+    const int64_t want = (n_bound + EW_WAVES - 1) / EW_WAVES;
+    const unsigned g = (unsigned)std::max<int64_t>(OCT_NXCD, std::min<int64_t>(want, 16384));
+    const bool wide = t.n >= ((int64_t)1 << HIT_POS_BITS);
+    if (wide)
+        hipLaunchKernelGGL(k_sp_record<true>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
+                           t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, wave_first, plan_waves, lhits,
+                           run_tot, run_mask, n_bound, hitbuf, n_slots, flags, stats, leaves_per_run, sets, lheavy);
+    else
+        hipLaunchKernelGGL(k_sp_record<false>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
+                           q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, wave_first,
+                           plan_waves, lhits, run_tot, run_mask, n_bound, hitbuf, n_slots, flags, stats,
+                           leaves_per_run, sets, lheavy);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = exclusive_scan(run_tot, run_off, n_bound, scan_tmp, reinterpret_cast<uint64_t*>(flags + 2), s);
+    if (e != hipSuccess) return e;
+    if (wide)
+        hipLaunchKernelGGL(k_sp_emit<true>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
+                           t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first, plan_waves,
+                           lhits, run_off, run_mask, n_bound, hitbuf, n_slots, flags, cap_edges, out, index_base,
+                           leaves_per_run, sets, lheavy, sig, sig_src, sig_words);
+    else
+        hipLaunchKernelGGL(k_sp_emit<false>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
+                           t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first, plan_waves,
+                           lhits, run_off, run_mask, n_bound, hitbuf, n_slots, flags, cap_edges, out, index_base,
+                           leaves_per_run, sets, lheavy, sig, sig_src, sig_words);
     return hipGetLastError();
 }
 

@@ -460,6 +460,73 @@ def test_single_pass_heavy_leaves(orc):
     assert heavy[2] == 0
 
 
+FORM_CASES = {
+    "plain100k_leaf32": ("plain100k", 32),
+    "plain100k_leaf8": ("plain100k", 8),
+    "clustered2000": ("clustered2000", 32),
+    "coincident100": ("coincident100", 32),
+    "huge_radius": ("huge_radius", 32),
+}
+
+
+@pytest.mark.parametrize("case", sorted(FORM_CASES))
+@pytest.mark.parametrize("plan", [False, True])
+def test_single_pass_forms_agree(orc, case, plan):
+    """The split single pass (record kernel, scan, emit kernel) and the fused
+    look-back kernel write the same edges in the same positions, and both are
+    the oracle's set: with fixed runs and with the predictor's plan on every
+    call (runs of uneven length, hit-buffer slots by plan)."""
+    import octohip
+    from octohip import predictor
+
+    name, leaf = FORM_CASES[case]
+    make, r = EDGE_CASES[name]
+    pts = make()
+    print(f"single forms case={case} plan={plan}", flush=True)
+    o = orc.build_edges(orc.run(pts, r, leaf_size=leaf), pts)
+    want = (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm)
+    outs = []
+    for split in (True, False):
+        tns, sid, _ = _build(pts, r, leaf=leaf)
+        octohip.set_single_form(tns, split)
+        if plan:
+            predictor.load(tns, "sph_lattice")
+            predictor.set_policy(tns, True)
+        for call in range(2):
+            e = octohip.build_edges(tns, sid, sid, mode="learned")
+            got = tuple(_np(x).copy() for x in (e.senders, e.receivers, e.rel_displacement, e.rel_dist_norm))
+            _edges_equal(_gpu_edges(e), want)
+        st = predictor.edge_stats(tns, sid, sid)
+        assert st["single_pass"], st
+        outs.append(got)
+    for a, b in zip(*outs):
+        assert np.array_equal(bits(a), bits(b))
+
+
+def test_split_single_pass_overflow_and_heavy(orc):
+    """Leaves whose hits overflow their run's hit-buffer slot (long runs of
+    dense leaves) are re-swept by the emit kernel from their recorded
+    per-query counts, and heavy leaves are filled after it: the edges stay
+    the exact path's, in the same order."""
+    import octohip
+    from octohip import predictor
+
+    pts = datasets.gaussian_mixture(30000, 43, 4, 0.02, 0.1)
+    r = float(datasets.calibrate_radius(pts, 40.0, seed=3))
+    tns, sid, _ = _build(pts, r)
+    ex = _gpu_edges(octohip.build_edges(tns, sid, sid, mode="exact"))
+    octohip.set_run_leaves(tns, 64)  # runs far beyond one slot's hits
+    for call in range(3):
+        got = _gpu_edges(octohip.build_edges(tns, sid, sid, mode="learned"))
+        st = predictor.edge_stats(tns, sid, sid)
+        print(f"overflow call {call}: {st['recounted_leaves']} recounted, {st['heavy_leaves']} heavy", flush=True)
+        for a, b in zip(got, ex):
+            assert np.array_equal(bits(a), bits(b))
+    assert st["recounted_leaves"] > 0
+    o = orc.build_edges(orc.run(pts, r), pts)
+    _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+
+
 @pytest.mark.parametrize("mode", ["exact", "learned"])
 def test_leaf_count_growth_between_builds(orc, mode):
     """run! does not wait for the tree counts; build_edges sizes its per-leaf
