@@ -63,12 +63,10 @@ def parse():
                          "clustered: 4M clustered points (the learned predictor's own distribution); diff: "
                          "build_edges_diff forward + HIP backward at 262,144 points")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="batched: keep --leaf-size / --run-leaves / --run-chunk")
+                    help="batched / clustered: keep --leaf-size / --run-leaves / --run-chunk (default: each mode autotuned)")
     ap.add_argument("--leaf-size", type=int, default=32)
     ap.add_argument("--run-leaves", type=int, default=6)
     ap.add_argument("--run-chunk", type=int, default=4, help="single-pass runs per block ticket (1..4)")
-    ap.add_argument("--single-form", default="fused", choices=["split", "fused"],
-                    help="learned mode: record + scan + emit kernels, or the fused look-back kernel")
     return ap.parse_args()
 
 
@@ -137,7 +135,6 @@ def run_mode(a, mode, snaps, r):
     tns = octohip.TNS(private_stream=a.private_stream)
     octohip.set_search_radius(tns, float(r))
     octohip.set_refit_mode(tns, not a.no_refit)
-    octohip.set_single_form(tns, a.single_form == "split")
     sid = octohip.add_point_set(tns, snaps[0])
     octohip.set_active_search(tns, sid, sid)
     api_mode = "exact" if mode == "exact" else "learned"
@@ -342,8 +339,17 @@ def clustered(a, steps):
         octohip.set_refit_mode(tns, True)
         sid = octohip.add_point_set(tns, coords)
         octohip.set_active_search(tns, sid, sid)
+        tuned = None
+        leaf = 32
+        if not a.no_autotune:  # each mode at its own best configuration (leaf size; runs for the single pass)
+            t = octohip.autotune(tns, sid, sid, mode=mode, leaf_sizes=(8, 16, 32),
+                                 runs=(6, 12, 48) if mode == "learned" else (6,), chunks=(4,), reps=2)
+            tuned = {"leaf_size": t["best"][0], "run_leaves": t["best"][1], "run_chunk": t["best"][2]}
+            leaf = t["best"][0]
         pred = None
-        if mode == "learned" and os.path.exists(os.path.join(octohip.predictor.SHIPPED, "clustered4m.f32")):
+        # the shipped predictor's features were taken at leaf size 32: it plans only such trees
+        if (mode == "learned" and leaf == 32
+                and os.path.exists(os.path.join(octohip.predictor.SHIPPED, "clustered4m.f32"))):
             octohip.predictor.load(tns, "clustered4m")
             pred = "clustered4m"
         first = None
@@ -373,7 +379,7 @@ def clustered(a, steps):
         octohip.profile(tns, False)
         res[mode] = {"value": edges / sec, "ms_per_step": 1e3 * sec / steps, "edges_per_step": edges / steps,
                      "breakdown_ms_per_step": {k: v[0] for k, v in prof.items() if v[1]},
-                     "predictor": pred,
+                     "predictor": pred, "autotuned": tuned,
                      "first_call": {k: first[k] for k in ("single_pass", "used_predictor", "attempts",
                                                           "predicted_total", "n_edges")},
                      "last_step": {k: st[k] for k in ("single_pass", "used_predictor", "attempts",

@@ -230,14 +230,6 @@ int oct_build_edges_backward64(oct_ctx* ctx, int64_t n_points, const oct_edges* 
  * output, either order) and pushes the rest; 0: every sender term is pushed
  * with an atomic.  Both give dL/dcoords up to FP32 summation order. */
 int oct_set_backward_form(oct_ctx* ctx, int pull);
-/* No counterpart in the reference (which has no single pass).  split = 0
- * (the default): OCT_EDGES_LEARNED runs as one fused kernel whose runs find
- * their output offsets by a decoupled look-back and keep their hits in LDS;
- * split != 0: a record kernel (each run of leaves sweeps its leaves once and
- * keeps their hits in a hit buffer in device memory), a scan of the run
- * totals and an emit kernel, with no wait between runs.  Edges are
- * identical. */
-int oct_set_single_form(oct_ctx* ctx, int split);
 
 /* Kernel timing by phase with HIP events on the context's stream (off by
  * default).  oct_profile_read waits for the stream, then writes the summed

@@ -164,8 +164,6 @@ struct PairState {
     // CSR neighbour lists (materialize_all_neighbors!): offsets (n_q + 1) and targets
     DevBuf csr_off, csr_flat;
     DevBuf rdisp2;  // 2D contexts: rel_displacement (2, E)
-    // the split single pass: run totals and offsets, hit-buffer slots, scan scratch
-    DevBuf run_tot, run_off, hitbuf, split_tmp;
     bool csr_ok = false;
     int64_t csr_nq = 0, csr_total = 0;
     bool heavy_on = true;  // count heavy leaves ahead: first call, and while many leaves are not recorded
@@ -184,7 +182,7 @@ struct PairState {
     void release_all() {
         DevBuf* all[] = {&senders, &receivers, &rdisp, &rdist, &lists, &lcount, &qcount, &qoff, &scan_tmp, &flags,
                          &tiles, &ldir, &feat, &pred, &lhits, &pwork, &pscan, &wave_first, &heavy, &eprm,
-                         &csr_off, &csr_flat, &rdisp2, &run_tot, &run_off, &hitbuf, &split_tmp};
+                         &csr_off, &csr_flat, &rdisp2};
         for (DevBuf* d : all) d->release();
     }
 };
@@ -294,7 +292,6 @@ struct oct_ctx {
     std::vector<BatchState> batches;
     std::vector<void*> retired;  // grown-out allocations of handed-out buffers (DevBuf::retire)
     bool pull_backward = true;   // build_edges_diff adjoint in the pull form (run table), else the push form
-    bool split_single = false;   // learned mode: record kernel + scan + emit kernel, else the fused look-back kernel
     DevBuf bw_scratch;           // its run table
     Predictor mlp;
     std::string err;
@@ -1259,36 +1256,15 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, int emit, o
             if (early) {  // cleared before the launch; the single pass's last run sets it
                 __atomic_store_n((uint64_t*)(c->sig + SP_SIG_MARK), 0ull, __ATOMIC_RELEASE);
             }
-            if (c->split_single) {
-                const int64_t nb = split_bound(nql, c->run_leaves, predict);
-                const int64_t ns = split_slots(nql, c->run_leaves, predict);
-                HIP_TRY(c, p.run_tot.ensure(4 * (size_t)nb));
-                HIP_TRY(c, p.run_off.ensure(4 * (size_t)nb));
-                HIP_TRY(c, p.tiles.ensure(8 * (size_t)nb));
-                HIP_TRY(c, p.hitbuf.ensure(4 * (size_t)SP_PLAN_BUDGET * (size_t)ns));
-                HIP_TRY(c, p.split_tmp.ensure(4 * (size_t)scan_tmp_words(nb)));
-                HIP_TRY(c, launch_edges_split(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq, inv_r,
-                                              self_pair, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(),
-                                              predict ? p.wave_first.as<int32_t>() : nullptr,
-                                              predict ? (const int32_t*)(flags + 12) : nullptr,
-                                              p.lhits.as<uint32_t>(), p.run_tot.as<uint32_t>(),
-                                              p.run_off.as<uint32_t>(), p.tiles.as<uint64_t>(),
-                                              p.hitbuf.as<uint32_t>(), p.split_tmp.as<uint32_t>(), flags + 4,
-                                              flags + 9, (uint64_t)p.cap_edges, eo, c->index_base, c->run_leaves,
-                                              SetLookup{}, heavy ? p.heavy_flags(nql) : nullptr,
-                                              early ? c->sig_dev : nullptr, flags,
-                                              (int)((HF_BYTES + 2 * sizeof(BuildParams)) / 4), c->stream));
-            } else {
-                HIP_TRY(c, launch_edges_single(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
-                                               inv_r, self_pair, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(),
-                                               predict ? p.wave_first.as<int32_t>() : nullptr,
-                                               predict ? (const int32_t*)(flags + 12) : nullptr,
-                                               p.lhits.as<uint32_t>(), p.tiles.as<uint64_t>(), flags + 8, flags + 4,
-                                               flags + 9, (uint64_t)p.cap_edges, eo, c->index_base, c->run_leaves,
-                                               c->run_chunk, SetLookup{}, heavy ? p.heavy_flags(nql) : nullptr,
-                                               early ? c->sig_dev : nullptr, flags,
-                                               (int)((HF_BYTES + 2 * sizeof(BuildParams)) / 4), c->stream));
-            }
+            HIP_TRY(c, launch_edges_single(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
+                                           inv_r, self_pair, p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(),
+                                           predict ? p.wave_first.as<int32_t>() : nullptr,
+                                           predict ? (const int32_t*)(flags + 12) : nullptr,
+                                           p.lhits.as<uint32_t>(), p.tiles.as<uint64_t>(), flags + 8, flags + 4,
+                                           flags + 9, (uint64_t)p.cap_edges, eo, c->index_base, c->run_leaves,
+                                           c->run_chunk, SetLookup{}, heavy ? p.heavy_flags(nql) : nullptr,
+                                           early ? c->sig_dev : nullptr, flags,
+                                           (int)((HF_BYTES + 2 * sizeof(BuildParams)) / 4), c->stream));
             if (heavy)
                 HIP_TRY(c, launch_edges_heavy_fill(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
                                                    inv_r, self_pair, p.heavy_list(), flags + 16,
@@ -1621,11 +1597,6 @@ int oct_set_backward_form(oct_ctx* c, int pull) {
     return OCT_OK;
 }
 
-int oct_set_single_form(oct_ctx* c, int split) {
-    if (!c) return OCT_ERR_ARGUMENT;
-    c->split_single = split != 0;
-    return OCT_OK;
-}
 
 int oct_profile_phases(oct_ctx* c, uint32_t mask) {
     if (!c) return OCT_ERR_ARGUMENT;

@@ -1353,307 +1353,6 @@ hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2*
     return hipGetLastError();
 }
 
-// ------------------------------------------------------ split single pass
-//
-// The single pass as two kernels with a scan of the run totals between them,
-// so that no run's emit waits on the runs before it (in the fused kernel the
-// look-back held each wave until the slowest phase A of the runs in flight
-// had published).  k_sp_record sweeps every leaf of its runs once: a leaf
-// whose candidates fit one register tile records its hits in the run's slot
-// of a hit buffer in HBM (SP_HIT_WORDS words per run, compact or wide
-// entries as in the LDS hit list) while they fit, any other leaf keeps its
-// per-query counts; it writes each leaf's exact count (lhits), the run's total
-// and its recorded-leaf mask.  An exclusive scan of the run totals gives each
-// run's output offset and the call's total.  k_sp_emit writes every run's
-// edges: recorded leaves straight from the hit buffer, the others re-swept
-// with the FILL body, heavy leaves left to k_edges_heavy_fill as in the
-// fused kernel.  Both kernels take XCD-local contiguous shares of the runs,
-// and none of their waves waits on another.
-
-// Runs of the call: the plan's, or fixed runs of leaves_per_run leaves.
-__device__ inline bool sp_run_range(int64_t w, int32_t nleaves, const int32_t* __restrict__ wave_first,
-                                    int leaves_per_run, int32_t& r_lo, int32_t& r_hi) {
-    if (wave_first) {
-        r_lo = wave_first[w];
-        r_hi = wave_first[w + 1];
-    } else {
-        r_lo = (int32_t)w * leaves_per_run;
-        r_hi = min(nleaves, r_lo + leaves_per_run);
-    }
-    r_lo = __builtin_amdgcn_readfirstlane(r_lo);
-    r_hi = __builtin_amdgcn_readfirstlane(r_hi);
-    // a malformed plan sweeps nothing: the host then takes the exact path
-    return r_lo >= 0 && r_hi <= nleaves && r_hi >= r_lo && r_hi - r_lo <= SP_MAX_LEAVES;
-}
-
-__device__ inline int32_t sp_run_count(int32_t nleaves, const int32_t* __restrict__ plan_waves, int leaves_per_run) {
-    return plan_waves ? *plan_waves : (nleaves + leaves_per_run - 1) / leaves_per_run;
-}
-
-template <bool WIDE>
-__global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
-k_sp_record(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm,
-            const float4* __restrict__ tsc, const int2* __restrict__ lists, const int32_t* __restrict__ lcount,
-            int cap, float r_sq, float inv_r, int self_pair, uint32_t* __restrict__ qcount,
-            const int32_t* __restrict__ wave_first, const int32_t* __restrict__ plan_waves,
-            uint32_t* __restrict__ lhits, uint32_t* __restrict__ run_tot, uint64_t* __restrict__ run_mask,
-            int64_t n_bound, uint32_t* __restrict__ hitbuf, int64_t n_slots, uint32_t* __restrict__ flags,
-            uint32_t* __restrict__ stats, int leaves_per_run, SetLookup sets, const uint8_t* __restrict__ lheavy) {
-    constexpr uint32_t HITS = WIDE ? SP_HIT_WORDS / 2 : SP_HIT_WORDS;
-    __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
-    __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
-    __shared__ float4 s_stage[EW_WAVES][SP_STAGE_PTS];  // record tiles; positions of the filtered candidates
-    __shared__ uint32_t s_stat[2];
-    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / OCT_WAVE));
-    const int lane = lane_id();
-    const WaveLds L{s_start[wv], s_first[wv], nullptr, nullptr, nullptr, nullptr, 0,
-                    reinterpret_cast<int32_t*>(s_stage[wv])};
-    const int32_t nleaves = qprm->n_leaves;
-    const int32_t nruns = sp_run_count(nleaves, plan_waves, leaves_per_run);
-    const XcdShare sh = xcd_share(n_bound);
-    uint32_t n_recount = 0, n_record = 0;
-    for (int64_t w = sh.lo + (int64_t)sh.k * EW_WAVES + wv; w < sh.hi; w += (int64_t)sh.nb * EW_WAVES) {
-        if (w >= nruns) {  // past the call's runs: a zero total for the scan
-            if (lane == 0) {
-                run_tot[w] = 0u;
-                run_mask[w] = 0ull;
-            }
-            continue;
-        }
-        int32_t r_lo, r_hi;
-        if (!sp_run_range(w, nleaves, wave_first, leaves_per_run, r_lo, r_hi)) {
-            if (lane == 0) {
-                atomicOr(&flags[1], 2u);
-                run_tot[w] = 0u;
-                run_mask[w] = 0ull;
-            }
-            continue;
-        }
-        // the run's slot of the hit buffer (runs past the slots record nothing)
-        uint32_t* hits = w < n_slots ? hitbuf + w * (int64_t)SP_HIT_WORDS : nullptr;
-        const uint32_t hcap = hits ? HITS : 0u;
-        uint64_t recorded = 0;  // bit k: leaf r_lo + k sits in the hit buffer
-        uint32_t hfill = 0, wtot = 0, nrec = 0;
-        for (int32_t r = r_lo; r < r_hi; ++r) {
-            const int k = r - r_lo;
-            uint32_t c;
-            if (lheavy && __builtin_amdgcn_readfirstlane(lheavy[r]) != 0) {
-                c = __builtin_amdgcn_readfirstlane(lhits[r]);  // counted by the heavy-leaf pass
-            } else {
-                const NodeRec qa = qleaves[r];
-                const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
-                const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
-                const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
-                const int2* lst = lists + (int64_t)r * cap;
-                const int2 lf0 = lst[lane < ncand ? lane : 0];
-                const int total = __builtin_amdgcn_readfirstlane(wave_sum(lane < ncand ? lf0.y : 0));
-                float lr_sq, linv_r;
-                int libase;
-                leaf_set_params(sets, qf, r_sq, inv_r, 0, lr_sq, linv_r, libase);
-                c = REC_NOT_TILE;
-                if (recordable_leaf(qc, ncand, total))
-                    c = leaf_sweep_record<WIDE>(qf, qc, qa.lo, qa.hi, ncand, lf0, qsc, tsc, lr_sq, self_pair, L,
-                                                s_stage[wv], lane, hits ? (WIDE ? hits + 2 * hfill : hits + hfill)
-                                                                        : nullptr,
-                                                hcap - hfill, qcount);
-                if (c != REC_NOT_TILE) {
-                    if (c <= hcap - hfill) {  // else the slot is full: qcount holds the per-query counts
-                        recorded |= 1ull << k;
-                        hfill += c;
-                        ++nrec;
-                    }
-                } else {
-                    c = leaf_pass<EDGE_COUNT>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r,
-                                              self_pair, 0, qcount, nullptr, EdgeOut{}, libase, L, lane);
-                }
-                if (lane == 0) lhits[r] = c;
-            }
-            wtot += c;
-        }
-        n_recount += (uint32_t)(r_hi - r_lo) - nrec;
-        n_record += nrec;
-        if (lane == 0) {
-            run_tot[w] = wtot;
-            run_mask[w] = recorded;
-        }
-    }
-    // one pair of global atomics per block
-    if (threadIdx.x == 0) s_stat[0] = s_stat[1] = 0;
-    __syncthreads();
-    if (lane == 0) {
-        atomicAdd(&s_stat[0], n_recount);
-        atomicAdd(&s_stat[1], n_record);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && (s_stat[0] | s_stat[1])) {
-        atomicAdd(&stats[0], s_stat[0]);
-        atomicAdd(&stats[1], s_stat[1]);
-    }
-}
-
-// Writes the edges of every run at its scanned offset.  Block 0's first wave
-// first settles the call: a total above cap_edges raises flags[0] (nothing
-// is written, the host regrows and redoes the call), then the flag words go
-// to the host's signal block with the marker (the host returns on them while
-// the emit runs).  The total itself is already at flags[2..3] (the scan's).
-template <bool WIDE>
-__global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
-k_sp_emit(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc, const BuildParams* __restrict__ qprm,
-          const float4* __restrict__ tsc, const int2* __restrict__ lists, const int32_t* __restrict__ lcount, int cap,
-          float r_sq, float inv_r, int self_pair, uint32_t* __restrict__ qcount, uint32_t* __restrict__ qoff,
-          const int32_t* __restrict__ wave_first, const int32_t* __restrict__ plan_waves,
-          const uint32_t* __restrict__ lhits, const uint32_t* __restrict__ run_off,
-          const uint64_t* __restrict__ run_mask, int64_t n_bound, const uint32_t* __restrict__ hitbuf,
-          int64_t n_slots, uint32_t* __restrict__ flags, uint64_t cap_edges, EdgeOut out, int index_base0,
-          int leaves_per_run, SetLookup sets, const uint8_t* __restrict__ lheavy, uint32_t* __restrict__ sig,
-          const uint32_t* __restrict__ sig_src, int sig_words) {
-    constexpr uint32_t STAGE_WORDS = 1024;                // FILL stage of one wave
-    constexpr uint32_t STAGE = STAGE_WORDS * 4 / 24;      // 170 edges
-    __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
-    __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
-    __shared__ int32_t s_fill[EW_WAVES][STAGE_WORDS];
-    __shared__ int32_t s_cand[EW_WAVES][EW_TILE * OCT_WAVE];
-    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / OCT_WAVE));
-    const int lane = lane_id();
-    int32_t* fs = s_fill[wv];
-    const WaveLds L{s_start[wv], s_first[wv], fs, fs + STAGE, reinterpret_cast<float*>(fs + 2 * STAGE),
-                    reinterpret_cast<float*>(fs + 5 * STAGE), STAGE, s_cand[wv]};
-    const uint64_t total = *reinterpret_cast<const uint64_t*>(flags + 2);
-    const bool over = total > cap_edges;
-    if (blockIdx.x == 0 && wv == 0) {
-        if (over && lane == 0) atomicOr(&flags[0], 1u);
-        if (sig) {
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-            for (int i = lane; i < sig_words; i += OCT_WAVE)
-                sig[i] = __hip_atomic_load(&sig_src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-            if (lane == 0)
-                __hip_atomic_store(reinterpret_cast<unsigned long long*>(sig + SP_SIG_MARK), 1ull + total,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-    }
-    if (over || flags[1] != 0) return;  // capacity, or a malformed plan: the host redoes the call
-    const int32_t nleaves = qprm->n_leaves;
-    const int32_t nruns = sp_run_count(nleaves, plan_waves, leaves_per_run);
-    const XcdShare sh = xcd_share(n_bound);
-    for (int64_t w = sh.lo + (int64_t)sh.k * EW_WAVES + wv; w < sh.hi; w += (int64_t)sh.nb * EW_WAVES) {
-        if (w >= nruns) break;  // (runs ascend within the share)
-        int32_t r_lo, r_hi;
-        if (!sp_run_range(w, nleaves, wave_first, leaves_per_run, r_lo, r_hi)) continue;
-        const uint64_t recorded = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(run_mask[w] >> 32))
-                                   << 32) |
-                                  (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)run_mask[w]);
-        const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)run_off[w]);
-        const uint32_t* hits = w < n_slots ? hitbuf + w * (int64_t)SP_HIT_WORDS : nullptr;
-        // recorded leaves straight from the hit buffer ...
-        uint32_t pos = base, hread = 0;
-        for (int32_t r = r_lo; r < r_hi; ++r) {
-            const int k = r - r_lo;
-            const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)lhits[r]);
-            if (hits && (recorded & (1ull << k))) {
-                const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qleaves[r]));
-                float lr_sq, linv_r;
-                int libase;
-                leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
-                emit_hits<WIDE>(WIDE ? hits + 2 * hread : hits + hread, c, pos, qf, qsc, tsc, linv_r, out, libase,
-                                lane);
-                hread += c;
-            }
-            pos += c;
-        }
-        // ... then the others re-swept (a heavy leaf only gets its output
-        // position, in qoff at its first query, for k_edges_heavy_fill)
-        if (recorded == (r_hi - r_lo == 64 ? ~0ull : ((1ull << (r_hi - r_lo)) - 1ull))) continue;
-        pos = base;
-        for (int32_t r = r_lo; r < r_hi; ++r) {
-            const int k = r - r_lo;
-            const uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)lhits[r]);
-            // (a leaf without edges writes nothing; its qcount may be stale)
-            if (c != 0 && !(hits && (recorded & (1ull << k)))) {
-                const NodeRec qa = qleaves[r];
-                const int32_t qf = __builtin_amdgcn_readfirstlane(rec_a(qa));
-                if (lheavy && __builtin_amdgcn_readfirstlane(lheavy[r]) != 0) {
-                    if (lane == 0) qoff[qf] = pos;
-                } else {
-                    const int32_t qc = __builtin_amdgcn_readfirstlane(rec_b(qa));
-                    uint32_t run = pos;
-                    for (int qb = 0; qb < qc; qb += OCT_WAVE) {
-                        const int nqb = min(OCT_WAVE, qc - qb);
-                        const uint32_t cq = lane < nqb ? qcount[qf + qb + lane] : 0u;  // this lane's, from the record
-                        const uint32_t inc = (uint32_t)wave_incl_scan((int)cq);
-                        if (lane < nqb) qoff[qf + qb + lane] = run + inc - cq;
-                        run += (uint32_t)__builtin_amdgcn_readlane((int)inc, OCT_WAVE - 1);
-                    }
-                    const int32_t ncand = __builtin_amdgcn_readfirstlane(min(lcount[r], cap));
-                    const int2* lst = lists + (int64_t)r * cap;
-                    const int2 lf0 = lst[lane < ncand ? lane : 0];
-                    float lr_sq, linv_r;
-                    int libase;
-                    leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
-                    leaf_pass<EDGE_FILL>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r, self_pair, 0,
-                                         qcount, qoff, out, libase, L, lane);
-                }
-            }
-            pos += c;
-        }
-    }
-}
-
-
-// Hit-buffer slots the split single pass records into: one per fixed run, or
-// for a plan (whose runs the host does not know) twice the fixed-run count;
-// runs past the slots record nothing and are re-swept (exact, slower).
-int64_t split_bound(int64_t n_leaves, int leaves_per_run, bool plan) {
-    return plan ? n_leaves + 1 : (n_leaves + leaves_per_run - 1) / leaves_per_run;
-}
-int64_t split_slots(int64_t n_leaves, int leaves_per_run, bool plan) {
-    const int64_t fixed = (n_leaves + leaves_per_run - 1) / leaves_per_run;
-    return plan ? std::min<int64_t>(n_leaves + 1, 2 * fixed) : fixed;
-}
-
-hipError_t launch_edges_split(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount, int cap,
-                              float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
-                              const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
-                              uint32_t* run_tot, uint32_t* run_off, uint64_t* run_mask, uint32_t* hitbuf,
-                              uint32_t* scan_tmp, uint32_t* flags, uint32_t* stats, uint64_t cap_edges, EdgeOut out,
-                              int index_base, int leaves_per_run, const SetLookup& sets, const uint8_t* lheavy,
-                              uint32_t* sig, const uint32_t* sig_src, int sig_words, hipStream_t s) {
-    if (sig && (sig_words < 0 || sig_words > SP_SIG_MARK)) return hipErrorInvalidValue;
-    if (leaves_per_run < 1 || leaves_per_run > SP_MAX_LEAVES) return hipErrorInvalidValue;
-    if (!lhits || (wave_first && !plan_waves)) return hipErrorInvalidValue;
-    const bool plan = wave_first != nullptr;
-    const int64_t n_bound = split_bound(q.n_leaves, leaves_per_run, plan);
-    const int64_t n_slots = split_slots(q.n_leaves, leaves_per_run, plan);
-    if (n_bound <= 0) return hipSuccess;
-    const int64_t want = (n_bound + EW_WAVES - 1) / EW_WAVES;
-    const unsigned g = (unsigned)std::max<int64_t>(OCT_NXCD, std::min<int64_t>(want, 16384));
-    const bool wide = t.n >= ((int64_t)1 << HIT_POS_BITS);
-    if (wide)
-        hipLaunchKernelGGL(k_sp_record<true>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
-                           t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, wave_first, plan_waves, lhits,
-                           run_tot, run_mask, n_bound, hitbuf, n_slots, flags, stats, leaves_per_run, sets, lheavy);
-    else
-        hipLaunchKernelGGL(k_sp_record<false>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord,
-                           q.params, t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, wave_first,
-                           plan_waves, lhits, run_tot, run_mask, n_bound, hitbuf, n_slots, flags, stats,
-                           leaves_per_run, sets, lheavy);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    e = exclusive_scan(run_tot, run_off, n_bound, scan_tmp, reinterpret_cast<uint64_t*>(flags + 2), s);
-    if (e != hipSuccess) return e;
-    if (wide)
-        hipLaunchKernelGGL(k_sp_emit<true>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
-                           t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first, plan_waves,
-                           lhits, run_off, run_mask, n_bound, hitbuf, n_slots, flags, cap_edges, out, index_base,
-                           leaves_per_run, sets, lheavy, sig, sig_src, sig_words);
-    else
-        hipLaunchKernelGGL(k_sp_emit<false>, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
-                           t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, qcount, qoff, wave_first, plan_waves,
-                           lhits, run_off, run_mask, n_bound, hitbuf, n_slots, flags, cap_edges, out, index_base,
-                           leaves_per_run, sets, lheavy, sig, sig_src, sig_words);
-    return hipGetLastError();
-}
-
 hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
                               int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
                               const uint32_t* n_heavy, uint32_t* qcount, uint32_t* lhits, const SetLookup& sets,

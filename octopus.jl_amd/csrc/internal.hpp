@@ -276,19 +276,6 @@ int64_t single_tiles(int64_t n_leaves);
 // sig (optional, fixed runs only): host-visible memory; the last run copies
 // sig_words words of sig_src there and then writes 1 + the edge total as the
 // 64-bit word at sig + SP_SIG_MARK, while the other runs' phase B goes on.
-// The single pass as record kernel, scan of the run totals and emit kernel
-// (edges.hip).  run_tot / run_off need split_bound words, run_mask
-// split_bound 64-bit words, hitbuf split_slots x SP_PLAN_BUDGET words,
-// scan_tmp scan_tmp_words(split_bound) words.
-int64_t split_bound(int64_t n_leaves, int leaves_per_run, bool plan);
-int64_t split_slots(int64_t n_leaves, int leaves_per_run, bool plan);
-hipError_t launch_edges_split(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount, int cap,
-                              float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
-                              const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,
-                              uint32_t* run_tot, uint32_t* run_off, uint64_t* run_mask, uint32_t* hitbuf,
-                              uint32_t* scan_tmp, uint32_t* flags, uint32_t* stats, uint64_t cap_edges, EdgeOut out,
-                              int index_base, int leaves_per_run, const SetLookup& sets, const uint8_t* lheavy,
-                              uint32_t* sig, const uint32_t* sig_src, int sig_words, hipStream_t s);
 hipError_t launch_edges_single(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
                                int cap, float r_sq, float inv_r, int self_pair, uint32_t* qcount, uint32_t* qoff,
                                const int32_t* wave_first, const int32_t* plan_waves, uint32_t* lhits,

@@ -452,14 +452,6 @@ def set_backward_form(tns: TNS, pull: bool = True):
     tns._check(tns._lib.oct_set_backward_form(tns._c(), int(bool(pull))))
 
 
-def set_single_form(tns: TNS, split: bool = False):
-    """The learned mode's single pass as one fused kernel whose runs find
-    their output offsets by a decoupled look-back (split=False, the default)
-    or as record kernel + scan + emit kernel (split=True).  Edges are the
-    same either way."""
-    tns._check(tns._lib.oct_set_single_form(tns._c(), int(bool(split))))
-
-
 def set_run_leaves(tns: TNS, run_leaves: int):
     """Leaves per single-pass run without a plan (1..64, default 6)."""
     tns._check(tns._lib.oct_set_run_leaves(tns._c(), int(run_leaves)))

@@ -158,7 +158,6 @@ SIGNATURES = {
     "oct_build_edges_backward": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(OctEdges), _P, _P,
                                                 ctypes.c_float, _P]),
     "oct_set_backward_form": (ctypes.c_int, [_P, ctypes.c_int]),
-    "oct_set_single_form": (ctypes.c_int, [_P, ctypes.c_int]),
     "oct_create_typed": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int]),
     "oct_set_search_radius64": (ctypes.c_int, [_P, ctypes.c_double]),

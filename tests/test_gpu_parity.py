@@ -460,7 +460,7 @@ def test_single_pass_heavy_leaves(orc):
     assert heavy[2] == 0
 
 
-FORM_CASES = {
+PLAN_CASES = {
     "plain100k_leaf32": ("plain100k", 32),
     "plain100k_leaf8": ("plain100k", 8),
     "clustered2000": ("clustered2000", 32),
@@ -469,26 +469,23 @@ FORM_CASES = {
 }
 
 
-@pytest.mark.parametrize("case", sorted(FORM_CASES))
-@pytest.mark.parametrize("plan", [False, True])
-def test_single_pass_forms_agree(orc, case, plan):
-    """The split single pass (record kernel, scan, emit kernel) and the fused
-    look-back kernel write the same edges in the same positions, and both are
-    the oracle's set: with fixed runs and with the predictor's plan on every
-    call (runs of uneven length, hit-buffer slots by plan)."""
+@pytest.mark.parametrize("case", sorted(PLAN_CASES))
+def test_single_pass_plan_every_call(orc, case):
+    """The single pass under the predictor's plan on every call (runs of
+    uneven length, packed by predicted hits) writes the same edges in the
+    same positions as with fixed runs, and both are the oracle's set."""
     import octohip
     from octohip import predictor
 
-    name, leaf = FORM_CASES[case]
+    name, leaf = PLAN_CASES[case]
     make, r = EDGE_CASES[name]
     pts = make()
-    print(f"single forms case={case} plan={plan}", flush=True)
+    print(f"single pass plan case={case}", flush=True)
     o = orc.build_edges(orc.run(pts, r, leaf_size=leaf), pts)
     want = (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm)
     outs = []
-    for split in (True, False):
+    for plan in (False, True):
         tns, sid, _ = _build(pts, r, leaf=leaf)
-        octohip.set_single_form(tns, split)
         if plan:
             predictor.load(tns, "sph_lattice")
             predictor.set_policy(tns, True)
@@ -498,16 +495,17 @@ def test_single_pass_forms_agree(orc, case, plan):
             _edges_equal(_gpu_edges(e), want)
         st = predictor.edge_stats(tns, sid, sid)
         assert st["single_pass"], st
+        assert st["used_predictor"] == plan, st
         outs.append(got)
     for a, b in zip(*outs):
         assert np.array_equal(bits(a), bits(b))
 
 
-def test_split_single_pass_overflow_and_heavy(orc):
-    """Leaves whose hits overflow their run's hit-buffer slot (long runs of
-    dense leaves) are re-swept by the emit kernel from their recorded
-    per-query counts, and heavy leaves are filled after it: the edges stay
-    the exact path's, in the same order."""
+def test_single_pass_list_overflow_and_heavy(orc):
+    """Leaves whose hits overflow their run's hit list (long runs of dense
+    leaves) are re-swept in phase B from the per-query counts of their phase
+    A sweep, and heavy leaves are filled after the pass: the edges stay the
+    exact path's, in the same order."""
     import octohip
     from octohip import predictor
 
