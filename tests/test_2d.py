@@ -201,3 +201,26 @@ def test_api_2d_checks():
         octohip.update_point_set(t2, sid, torch.zeros(11, 2, device="cuda"))
     with pytest.raises((octohip.ArgumentError, octohip.DimensionMismatch)):  # batches are 3D
         octohip.Batch(t2, torch.zeros(10, 2, device="cuda"), [0, 10], [0.1])
+
+
+def test_device_view_2d():
+    """device_view of a 2D set (test/query_cuda_2d.jl pattern): a user kernel
+    iterating the view counts, per query, the points within r in the 2D
+    predicate (dx*dx + dy*dy <= r*r in FP32, self included); the view's
+    coordinates are the stored (x, y, constant z) copy, whose z differences
+    are exactly 0."""
+    import octohip
+
+    pts = _pts(4000, 51)
+    r = 0.03
+    tns, sid, _ = _tns(pts, r)
+    v = octohip.device_view(tns, sid, sid)
+    cnt = _np(octohip.count_neighbors_device(tns, v))
+    p = np.ascontiguousarray(pts, np.float32)
+    r2 = np.float32(r) * np.float32(r)
+    want = np.empty(p.shape[0], np.int64)
+    for i in range(p.shape[0]):
+        dx = p[:, 0] - p[i, 0]
+        dy = p[:, 1] - p[i, 1]
+        want[i] = np.count_nonzero(dx * dx + dy * dy <= r2)
+    assert np.array_equal(cnt, want)
