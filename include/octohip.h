@@ -166,7 +166,9 @@ int oct_build_info_get(oct_ctx* ctx, int set_id, oct_build_info* out);
 /* build_edges / build_edges! (src/api.jl:304-337,
  * ext/OctopusCUDAExt.jl:345-415): flat-COO edges of the active pair (q, t);
  * the self pair (q == t) excludes j == i.  mode: OCT_EDGES_*; order:
- * OCT_ORDER_* (OCT_ORDER_RECEIVER needs OCT_EDGES_EXACT). */
+ * OCT_ORDER_* (OCT_ORDER_RECEIVER needs OCT_EDGES_EXACT).  A set never run
+ * has an empty tree and gives no edges (test/backend_errors_cpu.jl:40-57);
+ * a set changed since its last run is OCT_ERR_STATE. */
 int oct_build_edges(oct_ctx* ctx, int q, int t, int mode, int order, oct_edges* out);
 
 /* CSR neighbour lists of one active pair (tns.neighbor_buffers[pair],

@@ -1104,6 +1104,24 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, int emit, o
     PairState& p = *pp;
     SetState& qs = c->sets[set_index(c, q)];
     SetState& ts = c->sets[set_index(c, t)];
+    if (emit == EMIT_EDGES && (!qs.built || !ts.built)) {
+        // never run: the tree is empty and the traversal finds no edges, as
+        // the reference's does (test/backend_errors_cpu.jl:40-57)
+        HIP_TRY(c, hipSetDevice(c->device));
+        const size_t w = c->f64 ? 8 : 4;
+        HIP_TRY(c, p.senders.ensure(4));
+        HIP_TRY(c, p.receivers.ensure(4));
+        HIP_TRY(c, p.rdisp.ensure(3 * w));
+        HIP_TRY(c, p.rdist.ensure(w));
+        memset(out, 0, sizeof(*out));
+        out->senders = p.senders.as<int32_t>();
+        out->receivers = p.receivers.as<int32_t>();
+        out->rel_displacement = (float*)p.rdisp.p;
+        out->rel_dist_norm = (float*)p.rdist.p;
+        p.n_edges = 0;
+        p.materialized = true;
+        return OCT_OK;
+    }
     if (!qs.built || qs.dirty || !ts.built || ts.dirty) return fail(c, OCT_ERR_STATE, "run! before build_edges");
     HIP_TRY(c, hipSetDevice(c->device));
     memset(out, 0, sizeof(*out));
