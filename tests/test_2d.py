@@ -224,3 +224,24 @@ def test_device_view_2d():
         dy = p[:, 1] - p[i, 1]
         want[i] = np.count_nonzero(dx * dx + dy * dy <= r2)
     assert np.array_equal(cnt, want)
+
+
+def test_jittered_lattice_2d(orc):
+    """The configuration of test/edges_cuda.jl:147-200 (D = 2, N = 40,000,
+    r = 0.072, leaf 32: a jittered lattice at spacing r / 1.6, dense enough
+    that the reference's first node estimate overflowed): tree and edges
+    against the oracle."""
+    import octohip
+
+    r = 0.072
+    s = np.float32(r / 1.6)
+    g = np.stack(np.meshgrid(np.arange(200), np.arange(200), indexing="ij"), -1).reshape(-1, 2)
+    jit = datasets.uniform(77, 0, 2 * g.shape[0]).astype(np.float32).reshape(-1, 2) - np.float32(0.5)
+    pts = ((g.astype(np.float32) + np.float32(0.25) * jit) * s).astype(np.float32)
+    tns, sid, _ = _tns(pts, r)
+    o = orc.run2(pts, r)
+    t = octohip.get_tree(tns, sid)
+    assert t.n_nodes == o.n_nodes
+    assert np.array_equal(_np(t.node_children), o.node_children)
+    assert np.array_equal(_np(t.permutation), o.perm)
+    _same_edges(octohip.build_edges(tns, sid, sid), orc.build_edges2(o, pts))

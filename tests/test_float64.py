@@ -129,3 +129,27 @@ def test_float64_refuses_float32_only_modes():
         octohip.build_edges(tns, sid, sid, mode="learned")
     with pytest.raises(octohip.OctopusError):
         octohip.device_view(tns, sid, sid)
+
+
+def test_float64_apply_zsort_and_cross_rejection():
+    """test/float64.jl:115-148: apply_zsort of Float64 arrays (1-D and 2-D)
+    follows the set's permutation, and a Float64 TNS takes no Float32
+    coordinates nor a Float32 TNS Float64 ones."""
+    import octohip
+
+    torch = _torch()
+    pts = np.random.default_rng(4005).random((200, 3))
+    tns, sid, _ = _tns(pts, 0.1)
+    perm = _np(octohip.get_tree(tns, sid).permutation)
+    masses = torch.rand(200, dtype=torch.float64, device="cuda")
+    out = octohip.apply_zsort(tns, sid, masses)
+    assert out.dtype == torch.float64
+    assert np.array_equal(_np(out), _np(masses)[perm])
+    feats = torch.rand(200, 4, dtype=torch.float64, device="cuda")
+    assert np.array_equal(_np(octohip.apply_zsort(tns, sid, feats)), _np(feats)[perm])
+    with pytest.raises(octohip.ArgumentError):
+        octohip.add_point_set(tns, torch.rand(10, 3, device="cuda"))
+    t32 = octohip.TNS()
+    octohip.set_search_radius(t32, 0.1)
+    with pytest.raises(octohip.ArgumentError):
+        octohip.add_point_set(t32, torch.rand(10, 3, dtype=torch.float64, device="cuda"))
