@@ -414,88 +414,99 @@ k_refit_place(const float* const* __restrict__ csrc, int64_t n, const uint64_t* 
 constexpr int TOPO_THREADS = 256;
 constexpr int TOPO_MAX_T = 256;  // largest leaf size the window supports
 
-// Per sorted position: m (shared levels with the previous key), lambda, and
-// whether nodes start here; per-block counts of nodes per level (rows
-// 0..21) and of leaves (row 22).
+// Per sorted position k: m_k (levels shared with the previous key), lambda
+// and whether nodes start at k, plus the length of the leaf starting at k;
+// per-block counts of nodes per level (rows 0..21) and of leaves (row 22).
+// Every window test is one comparison of two keys: for sorted keys the levels
+// keys i and j share (i < j) are the minimum of m over (i, j], so the minimum
+// of m over (k, k+T] is key_shared(k, k+T), and the level-P node holding k
+// spans the positions whose keys share at least P levels with key k, found by
+// binary searches over the block's keys in LDS.
+constexpr uint32_t TOPO_LEN_SHIFT = 11;  // topo bits 11..19: leaf length (0: more than T)
 __global__ void __launch_bounds__(TOPO_THREADS)
-k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, int batched, uint16_t* __restrict__ topo,
+k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, int batched, uint32_t* __restrict__ topo,
              uint32_t* __restrict__ hist, int nblk) {
-    __shared__ int8_t m[TOPO_THREADS + 2 * TOPO_MAX_T + 1];
+    __shared__ uint64_t kk[TOPO_THREADS + 2 * TOPO_MAX_T + 1];  // keys of positions lo .. lo + span - 1
     // the block's node starts per level as differences: a position starting
     // nodes at levels mk+1 .. lam adds 1 at mk+1 and takes 1 at lam+1; [23]
     // counts the leaves (one per starting position)
     __shared__ int32_t dl[OCT_NLEVELS + 2];
     if (threadIdx.x < OCT_NLEVELS + 2) dl[threadIdx.x] = 0;
     const int64_t base = (int64_t)blockIdx.x * TOPO_THREADS;
-    const int64_t lo = base - T;  // m[i] holds m_{lo + i}
+    const int64_t lo = base - T - 1;
     const int span = TOPO_THREADS + 2 * T + 1;
     for (int i = threadIdx.x; i < span; i += TOPO_THREADS) {
         const int64_t j = lo + i;
-        int8_t v = -1;
-        if (j > 0 && j < n) v = (int8_t)key_shared(sk[j - 1], sk[j], batched);  // -1 where a set starts
-        else if (j >= n) v = -2;  // past the end: never read as a real value
-        m[i] = v;
+        kk[i] = (j >= 0 && j < n) ? sk[j] : 0ull;  // outside [0, n): never compared
     }
     __syncthreads();
     const int64_t k = base + threadIdx.x;
-    const int li = threadIdx.x + T;
-    int mk = -1, lam = -1;
+    const int li = threadIdx.x + T + 1;  // kk[li] is key k
+    int mk = -1, lam = -1, len = 0;
     bool emit = false;
     if (k < n) {
-        mk = m[li];  // -1 at k == 0
+        const uint64_t key = kk[li];
+        mk = k > 0 ? key_shared(kk[li - 1], key, batched) : -1;  // -1 where a set starts
         // lambda: first level whose range starting at k holds <= T points.
-        // A repeat of the previous key (mk == 21) starts no node, so its
-        // window minimum is never needed.
-        int mw = 127;
-        if (mk >= OCT_MAX_LEVEL) {
-            mw = OCT_MAX_LEVEL;
-        } else if (k + T >= n) {
-            mw = -1;
-        } else {
-            // eight independent LDS reads per step (the window never passes
-            // the staged span)
-            int j = 1;
-            for (; j + 7 <= T; j += 8) {
-                int v = m[li + j];
-#pragma unroll
-                for (int q = 1; q < 8; ++q) v = min(v, (int)m[li + j + q]);
-                mw = min(mw, v);
-            }
-            for (; j <= T; ++j) mw = min(mw, (int)m[li + j]);
-        }
+        // A repeat of the previous key (mk == 21) starts no node.
+        int mw;
+        if (mk >= OCT_MAX_LEVEL) mw = OCT_MAX_LEVEL;
+        else if (k + T >= n) mw = -1;
+        else mw = key_shared(key, kk[li + T], batched);
         lam = min(OCT_MAX_LEVEL, max(mk, mw) + 1);
+        // keys sharing at least P levels with key k lie in [key & mask, key | ~mask]
+        // (a forest compares its set bits from level 0 and its Morton bits,
+        // which start at level 7, from there on)
+        auto mask = [&](int P) {
+            const int sh = batched ? min(63 - 3 * P, 42) : 63 - 3 * P;
+            return ~0ull << sh;
+        };
+        const int e_n = (int)min<int64_t>(n - lo, span);  // index of position n, if staged
         if (mk < 0) {  // position 0 or the first point of a set: its root chain starts here
             emit = true;
         } else if (lam > mk) {
-            // parent range (level mk, contains k) must hold more than T points:
-            // a = distance back to its start, b = distance on to its end,
-            // both searched eight positions per step
-            const int P = mk;
-            int a = T + 1, b = T + 1;
-            for (int j0 = 1; j0 <= T; j0 += 8) {
-                uint32_t hit = 0;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int j = j0 + q;
-                    if (j <= T && k - j >= 0 && ((int)m[li - j] < P || k - j == 0)) hit |= 1u << q;
+            // nodes start here iff the level-mk node [s, e) holding k has
+            // more than T points: decided by two keys when s < k - T or
+            // e > k + T, else s and e are searched (both within T of k)
+            const uint64_t mp = mask(mk), klo = key & mp, khi = key | ~mp;
+            emit = (k + T < n && mw >= mk) || (k > T && kk[li - T - 1] >= klo);
+            if (!emit) {
+                int a = li - (int)min<int64_t>(k, T), b = li;  // s: first index with key >= klo
+                while (a < b) {
+                    const int mid = (a + b) >> 1;
+                    if (kk[mid] >= klo) b = mid;
+                    else a = mid + 1;
                 }
-                if (hit) { a = j0 + __builtin_ctz(hit); break; }
-                if (k - (j0 + 7) <= 0) break;  // reached position 0 without a start: a stays T + 1
-            }
-            for (int j0 = 1; j0 <= T; j0 += 8) {
-                uint32_t hit = 0;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int j = j0 + q;
-                    if (j <= T && (k + j >= n || (int)m[li + j] < P)) hit |= 1u << q;
+                const int s_i = a;
+                a = li + 1;  // e: first index past li that is n or holds a key > khi
+                b = min(li + T, e_n);
+                while (a < b) {
+                    const int mid = (a + b) >> 1;
+                    if (kk[mid] > khi) b = mid;
+                    else a = mid + 1;
                 }
-                if (hit) { b = j0 + __builtin_ctz(hit); break; }
+                emit = a - s_i > T;
             }
-            emit = (a + b) > T;
+        }
+        if (emit) {
+            // the leaf starting at k (level lam) ends at the first position
+            // past k that is n or holds a key above the leaf's range; within
+            // T unless lam == 21 holds more than T equal keys
+            const uint64_t khi = key | ~mask(lam);
+            int a = li + 1, b = min(li + T, e_n);
+            if (b == e_n || kk[b] > khi) {
+                while (a < b) {
+                    const int mid = (a + b) >> 1;
+                    if (kk[mid] > khi) b = mid;
+                    else a = mid + 1;
+                }
+                len = a - li;
+            }
         }
     }
     // m is stored +1 so k == 0 (m = -1) fits the low 5 bits
-    if (k < n) topo[k] = (uint16_t)((mk + 1) | (lam << 5) | (emit ? (1 << 10) : 0));
+    if (k < n)
+        topo[k] = (uint32_t)((mk + 1) | (lam << 5) | (emit ? (1 << 10) : 0)) | ((uint32_t)len << TOPO_LEN_SHIFT);
     if (emit) {  // mk + 1 >= 0 and lam + 1 <= OCT_NLEVELS
         atomicAdd(&dl[mk + 1], 1);
         atomicSub(&dl[lam + 1], 1);
@@ -537,7 +548,7 @@ __device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
 // order).
 __global__ void __launch_bounds__(TOPO_THREADS)
 k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
-            const uint16_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
+            const uint32_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
             const uint64_t* __restrict__ total, BuildParams* __restrict__ prm, Forest forest, int index_base,
             NodeRec* __restrict__ nodes, int32_t* __restrict__ nfirst, int32_t* __restrict__ nlast,
             int32_t* __restrict__ nchild, float* __restrict__ bmin, float* __restrict__ bmax,
@@ -547,7 +558,6 @@ k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
     __shared__ uint32_t wc[NW][OCT_NLEVELS + 1];
     __shared__ uint64_t wmask[NW][OCT_NLEVELS + 1];  // per wave and level: lanes starting a node
     __shared__ uint32_t woff[NW][OCT_NLEVELS + 1];   // ids before the wave's first such lane
-    __shared__ int8_t mm[TOPO_THREADS + TOPO_MAX_T + 1];  // m of positions base .. base + 256 + T
     const int64_t base = (int64_t)blockIdx.x * TOPO_THREADS;
     // loads that do not depend on the ranks are issued first, so their
     // latencies overlap the ballots and barriers below
@@ -562,16 +572,13 @@ k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
         s_hi1 = forest.set_off[1];
         L01 = key_shared(sk[s_lo1], sk[s_hi1 - 1], 0);
     }
-    for (int i = threadIdx.x; i < TOPO_THREADS + T + 1; i += TOPO_THREADS) {
-        const int64_t j = base + i;
-        mm[i] = j < n ? (int8_t)((int)(topo[j] & 31) - 1) : (int8_t)-2;
-    }
-    int mk = -1, lam = -1;
+    int mk = -1, lam = -1, len = 0;
     bool emit = false;
     if (k < n) {
-        const uint16_t t = topo[k];
+        const uint32_t t = topo[k];
         mk = (int)(t & 31) - 1;
         lam = (t >> 5) & 31;
+        len = (int)(t >> TOPO_LEN_SHIFT);
         emit = (t >> 10) & 1;
     }
     const int w = threadIdx.x / OCT_WAVE;
@@ -624,24 +631,10 @@ k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
         const bool chain = (k == s_lo && L <= L0);
         NodeRec rec;
         if (L == lam) {
-            // leaf: the range ends at the first j > k with m_j < lam
-            int64_t e = -1;
-            const int li = threadIdx.x;
-            for (int j0 = 1; j0 <= T; j0 += 8) {
-                uint32_t hit = 0;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int j = j0 + q;
-                    if (j <= T && (k + j >= n || (int)mm[li + j] < lam)) hit |= 1u << q;
-                }
-                if (hit) {
-                    e = k + j0 + __builtin_ctz(hit);
-                    if (e > n) e = n;
-                    break;
-                }
-            }
-            if (e < 0)  // level-21 leaf holding more than T equal keys
-                e = lower_bound_pred(k + T + 1, n, [&](int64_t j) { return sk[j] != key; });
+            // leaf: its length from k_topo_count, else (0) a level-21 leaf
+            // holding more than T equal keys
+            const int64_t e = len > 0 ? k + len
+                                      : lower_bound_pred(k + T + 1, n, [&](int64_t j) { return sk[j] != key; });
             const int32_t cnt = (int32_t)(e - k);
             nlast[id] = (int32_t)(e - 1) + index_base;
             // bounds from k_link_refit (one lane per leaf there, not one per position)

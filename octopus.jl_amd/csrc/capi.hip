@@ -100,7 +100,7 @@ struct SetState {
         b.hist = hist.as<uint32_t>();
         b.scan_tmp = scan_tmp.as<uint32_t>();
         b.total = total.as<uint64_t>();
-        b.topo = topo.as<uint16_t>();
+        b.topo = topo.as<uint32_t>();
         b.perm = perm.as<int32_t>();
         b.scoord = scoord.as<float4>();
         b.nodes = nodes.as<NodeRec>();
@@ -528,7 +528,7 @@ int ensure_build_bufs(oct_ctx* c, SetState& s, int64_t n, int64_t M) {
     // the scans: radix and topology histograms, and (refit) the n changed flags
     HIP_TRY(c, s.scan_tmp.ensure(4 * (size_t)scan_tmp_words(std::max<int64_t>(n, hist_words(n)))));
     HIP_TRY(c, s.total.ensure(16));  // uint64 total
-    HIP_TRY(c, s.topo.ensure(2 * (size_t)n));
+    HIP_TRY(c, s.topo.ensure(4 * (size_t)n));
     HIP_TRY(c, s.perm.ensure(4 * (size_t)n));
     HIP_TRY(c, s.scoord.ensure(16 * (size_t)n));
     HIP_TRY(c, s.nodes.ensure(sizeof(NodeRec) * (size_t)M));
@@ -1890,7 +1890,7 @@ int batch_ensure(oct_ctx* c, BatchState& B) {
     HIP_TRY(c, s.hist.ensure(4 * (size_t)hist_words(N)));
     HIP_TRY(c, s.scan_tmp.ensure(4 * (size_t)scan_tmp_words(hist_words(N))));
     HIP_TRY(c, s.total.ensure(16));
-    HIP_TRY(c, s.topo.ensure(2 * (size_t)N));
+    HIP_TRY(c, s.topo.ensure(4 * (size_t)N));
     HIP_TRY(c, s.perm.ensure(4 * (size_t)N));
     HIP_TRY(c, s.scoord.ensure(16 * (size_t)N));
     HIP_TRY(c, s.nodes.ensure(sizeof(NodeRec) * (size_t)M));

@@ -155,7 +155,7 @@ struct BuildBufs {
     uint32_t* hist;      // radix histograms, reused for the topology table
     uint32_t* scan_tmp;
     uint64_t* total;     // scan total of the topology table
-    uint16_t* topo;      // per sorted position: m + 1, lambda, emit flag
+    uint32_t* topo;      // per sorted position: m + 1, lambda, emit flag, leaf length
     int32_t* perm;       // exported, index base applied
     float4* scoord;      // Morton-sorted coords, w = original index bits
     NodeRec* nodes;
