@@ -423,6 +423,7 @@ constexpr int TOPO_MAX_T = 256;  // largest leaf size the window supports
 // spans the positions whose keys share at least P levels with key k, found by
 // binary searches over the block's keys in LDS.
 constexpr uint32_t TOPO_LEN_SHIFT = 11;  // topo bits 11..19: leaf length (0: more than T)
+constexpr uint32_t EMIT_LIST = 512;      // k_topo_emit's nodes per round
 __global__ void __launch_bounds__(TOPO_THREADS)
 k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, int batched, uint32_t* __restrict__ topo,
              uint32_t* __restrict__ hist, int nblk) {
@@ -536,35 +537,44 @@ __device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
 
 // Emits every node starting at each sorted position, with no search: the
 // BFS id of the node at (level L, position k) is the scanned (L, block)
-// offset plus the ballot rank inside the block; its parent is either the node
-// one level up at the same position or, for the first level, the last node
-// of the level above starting before k (the same ranks, minus one).
-// Writes node_first, the packed record (first child for internal nodes),
-// parent links and levels; leaves also get their range end, and the nodes
-// covering all n points (the chain down to the first split) get the global
-// bounds.  Child slots and counts and the leaves' bounds come from
-// k_link_refit, internal bounds and range ends from k_link_refit and
-// k_refit_merge, which find each leaf's ancestors in ranc (per level, in leaf
-// order).
+// offset plus its rank among the block's level-L starts; its parent is either
+// the node one level up at the same position or, for the first level, the
+// last node of the level above starting before k (the same ranks, minus one).
+// The block lists its nodes in LDS, level-major and in position order (in
+// rounds of EMIT_LIST entries), and one thread per node writes it, so
+// consecutive threads write consecutive ids.  Per node: node_first, level, parent, the packed
+// record, its octant in its parent and the id its first child would have (a
+// node's children are the ids from its own to the next node's, so
+// k_link_refit writes the child slots and counts from these two arrays);
+// leaves also get their range end, their bounds (the exact min / max of their
+// points) and their ancestor at every refit level (ranc, level-major, leaf
+// order), and the nodes covering all of a set's points (its chain down to the
+// first split) get the set's bounds.  Internal bounds and range ends come
+// from k_link_refit and k_refit_merge.
 __global__ void __launch_bounds__(TOPO_THREADS)
 k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
             const uint32_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
             const uint64_t* __restrict__ total, BuildParams* __restrict__ prm, Forest forest, int index_base,
-            NodeRec* __restrict__ nodes, int32_t* __restrict__ nfirst, int32_t* __restrict__ nlast,
-            int32_t* __restrict__ nchild, float* __restrict__ bmin, float* __restrict__ bmax,
-            int32_t* __restrict__ parent, uint8_t* __restrict__ nlevel, int32_t* __restrict__ leaf_ids,
-            NodeRec* __restrict__ leaves, int32_t* __restrict__ ranc, int64_t astride) {
+            const float4* __restrict__ scoord, NodeRec* __restrict__ nodes, int32_t* __restrict__ nfirst,
+            int32_t* __restrict__ nlast, float* __restrict__ bmin, float* __restrict__ bmax,
+            int32_t* __restrict__ parent, uint8_t* __restrict__ nlevel, int32_t* __restrict__ nfc,
+            uint8_t* __restrict__ noct, int32_t* __restrict__ leaf_ids, NodeRec* __restrict__ leaves,
+            int32_t* __restrict__ ranc, int64_t astride) {
     constexpr int NW = TOPO_THREADS / OCT_WAVE;
     __shared__ uint32_t wc[NW][OCT_NLEVELS + 1];
     __shared__ uint64_t wmask[NW][OCT_NLEVELS + 1];  // per wave and level: lanes starting a node
     __shared__ uint32_t woff[NW][OCT_NLEVELS + 1];   // ids before the wave's first such lane
+    __shared__ uint32_t s_loff[OCT_NLEVELS + 1];     // the block's nodes at levels below L
+    __shared__ uint32_t s_topo[TOPO_THREADS];
+    __shared__ uint64_t s_key[TOPO_THREADS];
+    __shared__ uint16_t s_list[EMIT_LIST];           // position | level << 8, one round's nodes
     const int64_t base = (int64_t)blockIdx.x * TOPO_THREADS;
     // loads that do not depend on the ranks are issued first, so their
     // latencies overlap the ballots and barriers below
     const uint32_t hrow = threadIdx.x <= OCT_NLEVELS ? hoff[(int64_t)threadIdx.x * nblk + blockIdx.x] : 0u;
     const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
-    const int64_t k = base + threadIdx.x;
-    const uint64_t key = k < n ? sk[k] : 0ull;
+    const int64_t k0 = base + threadIdx.x;
+    const uint64_t key0 = k0 < n ? sk[k0] : 0ull;
     int64_t s_lo1 = 0, s_hi1 = 0;  // one set: its range and chain depth
     int L01 = 0;
     if (!forest.batched && n > 0) {
@@ -572,97 +582,139 @@ k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
         s_hi1 = forest.set_off[1];
         L01 = key_shared(sk[s_lo1], sk[s_hi1 - 1], 0);
     }
-    int mk = -1, lam = -1, len = 0;
-    bool emit = false;
-    if (k < n) {
-        const uint32_t t = topo[k];
-        mk = (int)(t & 31) - 1;
-        lam = (t >> 5) & 31;
-        len = (int)(t >> TOPO_LEN_SHIFT);
-        emit = (t >> 10) & 1;
-    }
-    const int w = threadIdx.x / OCT_WAVE;
+    uint32_t t0 = 0;
+    if (k0 < n) t0 = topo[k0];
+    s_topo[threadIdx.x] = t0;
+    s_key[threadIdx.x] = key0;
+    const int mk0 = (int)(t0 & 31) - 1, lam0 = (t0 >> 5) & 31;
+    const bool emit0 = k0 < n && ((t0 >> 10) & 1);
+    const int w0 = threadIdx.x / OCT_WAVE;
     const int lane = lane_id();
     for (int L = 0; L <= OCT_NLEVELS; ++L) {
-        const bool ind = L < OCT_NLEVELS ? (emit && mk < L && L <= lam) : emit;
+        const bool ind = L < OCT_NLEVELS ? (emit0 && mk0 < L && L <= lam0) : emit0;
         const uint64_t b = ballot(ind);
         if (lane == 0) {
-            wmask[w][L] = b;
-            wc[w][L] = (uint32_t)__popcll(b);
+            wmask[w0][L] = b;
+            wc[w0][L] = (uint32_t)__popcll(b);
         }
     }
     __syncthreads();
-    if (threadIdx.x <= OCT_NLEVELS) {  // the block's offset per level, then each wave's
+    if (threadIdx.x < OCT_WAVE) {  // the block's offset per level, then each wave's
         const int L = threadIdx.x;
         uint32_t s = hrow;
-        for (int q = 0; q < NW; ++q) {
-            woff[q][L] = s;
-            s += wc[q][L];
-        }
+        if (L <= OCT_NLEVELS)
+            for (int q = 0; q < NW; ++q) {
+                woff[q][L] = s;
+                s += wc[q][L];
+            }
+        // the block's nodes at levels below L (row 22, the leaves, is no
+        // level); the scan runs with the whole wave active
+        const int c = L < OCT_NLEVELS ? (int)(s - hrow) : 0;
+        const int incl = wave_incl_scan(c);
+        if (L <= OCT_NLEVELS) s_loff[L] = (uint32_t)(incl - c);
     }
     __syncthreads();
-    if (k == 0) {
+    if (k0 == 0) {
         prm->n_nodes = M;
         prm->n_leaves = (int32_t)(*total) - M;  // the scan's last row counts the leaves
+        nfc[M] = M;                             // the first child of a node past the last
     }
-    if (!emit) return;
-    // the set of this position, its point range and chain depth (levels
-    // 0..L0 hold all of the set's points)
-    const int set = key_set(key, forest.batched);
-    int64_t s_lo = s_lo1, s_hi = s_hi1;
-    int L0 = L01;
-    if (forest.batched) {
-        s_lo = forest.set_off[set];
-        s_hi = forest.set_off[set + 1];
-        L0 = key_shared(sk[s_lo], sk[s_hi - 1], 1);
-    }
-    BuildParams* sp = forest.sprm + set;
-    if (k == s_lo) sp->chain_level = L0;
-    // count of level-L node starts before position k, plus the level's offset
-    // (LDS reads: a per-lane level cannot index registers without a select chain)
-    auto before = [&](int L) -> int32_t { return (int32_t)(woff[w][L] + (uint32_t)mbcnt(wmask[w][L])); };
-    const int Lfirst = mk + 1;  // 0 where a set starts
-    for (int L = Lfirst; L <= lam; ++L) {
-        const int32_t id = before(L);
-        nlevel[id] = (uint8_t)L;
-        nfirst[id] = (int32_t)k + index_base;
-        for (int o = 0; o < 8; ++o) nchild[8 * (int64_t)id + o] = -1;
-        parent[id] = L == 0 ? -1 : (L > Lfirst ? before(L - 1) : before(L - 1) - 1);
-        const bool chain = (k == s_lo && L <= L0);
-        NodeRec rec;
-        if (L == lam) {
-            // leaf: its length from k_topo_count, else (0) a level-21 leaf
-            // holding more than T equal keys
-            const int64_t e = len > 0 ? k + len
-                                      : lower_bound_pred(k + T + 1, n, [&](int64_t j) { return sk[j] != key; });
-            const int32_t cnt = (int32_t)(e - k);
-            nlast[id] = (int32_t)(e - 1) + index_base;
-            // bounds from k_link_refit (one lane per leaf there, not one per position)
-            rec.lo = make_float4(0.f, 0.f, 0.f, __int_as_float((int)k));
-            rec.hi = make_float4(0.f, 0.f, 0.f, __int_as_float(cnt));
-            const int32_t lr = before(OCT_NLEVELS) - M;
-            // the leaf's ancestor at each level the refit visits (-1 from its
-            // own level on): the last node of that level starting at or before k
-            for (int La = forest.batched ? 0 : L0 + 1; La < OCT_MAX_LEVEL; ++La)
-                ranc[(int64_t)La * astride + lr] = La >= lam ? -1 : before(La) - (La >= Lfirst ? 0 : 1);
-            leaf_ids[lr] = id;
-            leaves[lr] = rec;
-            if (chain) sp->branch_root = id;  // the whole set sits in one leaf
-        } else {
-            rec.lo = make_float4(0.f, 0.f, 0.f, __int_as_float(before(L + 1)));  // first child
-            rec.hi = make_float4(0.f, 0.f, 0.f, __int_as_float(0));             // -children, from k_link_refit
-            if (chain) {
-                rec.lo.x = o2f(sp->omin[0]); rec.lo.y = o2f(sp->omin[1]); rec.lo.z = o2f(sp->omin[2]);
-                rec.hi.x = o2f(sp->omax[0]); rec.hi.y = o2f(sp->omax[1]); rec.hi.z = o2f(sp->omax[2]);
-                nlast[id] = (int32_t)(s_hi - 1) + index_base;
-                if (L == L0) sp->branch_root = id;
+    // count of level-L starts before position base + p, plus the level's offset
+    auto before = [&](int L, int p) -> int32_t {
+        const int w = p / OCT_WAVE, l = p % OCT_WAVE;
+        return (int32_t)(woff[w][L] + (uint32_t)__popcll(wmask[w][L] & ((1ull << l) - 1ull)));
+    };
+    const uint32_t nb = s_loff[OCT_NLEVELS];  // the block's nodes (block-uniform)
+    for (uint32_t r0 = 0; r0 < nb; r0 += EMIT_LIST) {
+        if (emit0)
+            for (int L = mk0 + 1; L <= lam0; ++L) {
+                const uint32_t slot = s_loff[L] + (uint32_t)(before(L, threadIdx.x) - (int32_t)woff[0][L]);
+                if (slot >= r0 && slot < r0 + EMIT_LIST) s_list[slot - r0] = (uint16_t)(threadIdx.x | (L << 8));
+            }
+        __syncthreads();
+        const uint32_t nr = min(nb - r0, (uint32_t)EMIT_LIST);
+        for (uint32_t j = threadIdx.x; j < nr; j += TOPO_THREADS) {
+            const int p = s_list[j] & 255, L = s_list[j] >> 8;
+            const int32_t id = before(L, p);
+            const uint32_t t = s_topo[p];
+            const int mk = (int)(t & 31) - 1, lam = (t >> 5) & 31, len = (int)(t >> TOPO_LEN_SHIFT);
+            const int64_t k = base + p;
+            const uint64_t key = s_key[p];
+            const int Lfirst = mk + 1;  // 0 where a set starts
+            // the set of this position, its first point and (at its first
+            // point) its end and chain depth: levels 0..L0 hold all its points
+            const int set = key_set(key, forest.batched);
+            int64_t s_lo = s_lo1, s_hi = s_hi1;
+            int L0 = L01;
+            if (forest.batched) {
+                s_lo = forest.set_off[set];
+                if (k == s_lo) {
+                    s_hi = forest.set_off[set + 1];
+                    L0 = key_shared(sk[s_lo], sk[s_hi - 1], 1);
+                }
+            }
+            BuildParams* sp = forest.sprm + set;
+            if (k == s_lo && L == 0) sp->chain_level = L0;
+            const bool chain = k == s_lo && L <= L0;
+            nlevel[id] = (uint8_t)L;
+            nfirst[id] = (int32_t)k + index_base;
+            parent[id] = L == 0 ? -1 : (L > Lfirst ? before(L - 1, p) : before(L - 1, p) - 1);
+            noct[id] = L == 0 ? (uint8_t)0 : (uint8_t)((key_morton(key, forest.batched) >> level_shift(L - 1)) & 7ull);
+            const int32_t fc = L < OCT_MAX_LEVEL ? before(L + 1, p) : M;
+            nfc[id] = fc;
+            NodeRec rec;
+            if (L == lam) {
+                // leaf: its length from k_topo_count, else (0) a level-21 leaf
+                // holding more than T equal keys
+                const int64_t e = len > 0 ? k + len
+                                          : lower_bound_pred(k + T + 1, n, [&](int64_t j) { return sk[j] != key; });
+                const int32_t cnt_pts = (int32_t)(e - k);
+                nlast[id] = (int32_t)(e - 1) + index_base;
+                // the exact min / max of its points, eight loads in flight (a
+                // short last group re-reads the first point)
+                uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+                uint32_t hi3[3] = {0u, 0u, 0u};
+                for (int32_t j0 = 0; j0 < cnt_pts; j0 += 8) {
+                    float4 pp[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) pp[q] = scoord[k + (j0 + q < cnt_pts ? j0 + q : 0)];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const float c3[3] = {pp[q].x, pp[q].y, pp[q].z};
+                        for (int d = 0; d < 3; ++d) {
+                            const uint32_t o = f2o(c3[d]);
+                            lo3[d] = o < lo3[d] ? o : lo3[d];
+                            hi3[d] = o > hi3[d] ? o : hi3[d];
+                        }
+                    }
+                }
+                rec.lo = make_float4(o2f(lo3[0]), o2f(lo3[1]), o2f(lo3[2]), __int_as_float((int)k));
+                rec.hi = make_float4(o2f(hi3[0]), o2f(hi3[1]), o2f(hi3[2]), __int_as_float(cnt_pts));
+                const int32_t lr = before(OCT_NLEVELS, p) - M;
+                // the leaf's ancestor at each level the refit visits (-1 from its
+                // own level on): the last node of that level starting at or before k
+                for (int La = forest.batched ? 0 : L0 + 1; La < OCT_MAX_LEVEL; ++La)
+                    ranc[(int64_t)La * astride + lr] = La >= lam ? -1 : before(La, p) - (La >= Lfirst ? 0 : 1);
+                leaf_ids[lr] = id;
+                leaves[lr] = rec;
+                if (chain) sp->branch_root = id;  // the whole set sits in one leaf
+            } else {
+                rec.lo = make_float4(0.f, 0.f, 0.f, __int_as_float(fc));  // first child
+                rec.hi = make_float4(0.f, 0.f, 0.f, __int_as_float(0));   // -children, from k_link_refit
+                if (chain) {
+                    rec.lo.x = o2f(sp->omin[0]); rec.lo.y = o2f(sp->omin[1]); rec.lo.z = o2f(sp->omin[2]);
+                    rec.hi.x = o2f(sp->omax[0]); rec.hi.y = o2f(sp->omax[1]); rec.hi.z = o2f(sp->omax[2]);
+                    nlast[id] = (int32_t)(s_hi - 1) + index_base;
+                    if (L == L0) sp->branch_root = id;
+                }
+            }
+            nodes[id] = rec;
+            if (L == lam || chain) {
+                bmin[3 * (int64_t)id + 0] = rec.lo.x; bmin[3 * (int64_t)id + 1] = rec.lo.y; bmin[3 * (int64_t)id + 2] = rec.lo.z;
+                bmax[3 * (int64_t)id + 0] = rec.hi.x; bmax[3 * (int64_t)id + 1] = rec.hi.y; bmax[3 * (int64_t)id + 2] = rec.hi.z;
             }
         }
-        nodes[id] = rec;
-        if (L != lam && chain) {
-            bmin[3 * (int64_t)id + 0] = rec.lo.x; bmin[3 * (int64_t)id + 1] = rec.lo.y; bmin[3 * (int64_t)id + 2] = rec.lo.z;
-            bmax[3 * (int64_t)id + 0] = rec.hi.x; bmax[3 * (int64_t)id + 1] = rec.hi.y; bmax[3 * (int64_t)id + 2] = rec.hi.z;
-        }
+        __syncthreads();  // s_list is rewritten by the next round
     }
 }
 
@@ -694,186 +746,150 @@ __device__ inline void put_slot(uint32_t* p, int32_t id, const uint32_t* v) {
 
 // Two jobs in one launch, both on k_topo_emit's output and independent of
 // each other.
-// The last link_blocks blocks: every non-root node writes its id into its
-// parent's octant slot and counts itself into the parent's record
-// (b = -children).
-// The blocks before them: internal bounds and range ends from the leaves.  A block
-// takes LR_THREADS consecutive leaves (Morton order) and their ancestors per
-// level (ranc), then goes down the levels: at each level the lanes under one
-// node form a run of consecutive lanes; a segmented wave scan leaves the
-// maxima of each wave's part of a run in its last lane, and the parts that
-// cross into later waves are carried over in LDS.  A run that holds all of
-// its node's leaves (the leaves either side of the block tell) stores the
-// results; a run cut by an edge of the block goes to the block's slot for
-// that level and edge (slots[L][block][edge]; a run over the whole block
-// fills both), and k_refit_merge finishes those nodes.  The chain of a single
-// set keeps its set's bounds.
+// The last link_blocks blocks, one thread per node: a node's children are
+// the ids from its first child to the next node's first child (BFS order
+// keeps siblings together, and a node without children has the same first
+// child as the node after it), so the thread writes the node's eight child
+// slots from its children's octants and, for an internal node, its record's
+// b = -children.  No slot is written twice and no atomics are needed.
+// The blocks before them: internal bounds and range ends from the leaves'
+// bounds, one (block of LR_THREADS consecutive leaves, level) pair per
+// block, all levels at once.  The lanes under one node of the level form a
+// run of consecutive lanes; a segmented wave scan leaves the maxima of each
+// wave's part of a run in its last lane, and the parts that cross into later
+// waves are carried over in LDS.  A run that holds all of its node's leaves
+// (the leaves either side of the block tell) stores the results; a run cut
+// by an edge of the block goes to the block's slot for that level and edge
+// (slots[L][block][edge]; a run over the whole block fills both), and
+// k_refit_merge finishes those nodes.  The chain of a single set keeps its
+// set's bounds.
 __global__ void __launch_bounds__(LR_THREADS)
-k_link_refit(const uint64_t* __restrict__ sk, const BuildParams* __restrict__ prm, int batched, int index_base,
-             int link_blocks, const int32_t* __restrict__ parent, const int32_t* __restrict__ nfirst,
-             const uint8_t* __restrict__ nlevel, const int32_t* __restrict__ ranc, int64_t astride,
-             const int32_t* __restrict__ leaf_ids, const float4* __restrict__ scoord,
-             NodeRec* __restrict__ leaves, int32_t* __restrict__ nchild, NodeRec* __restrict__ nodes,
-             int32_t* __restrict__ nlast, float* __restrict__ bmin, float* __restrict__ bmax,
-             uint32_t* __restrict__ slots, int64_t bstride) {
-    __shared__ int32_t s_anc[LR_WAVES][OCT_MAX_LEVEL][OCT_WAVE];  // the lanes' ancestors per level
-    __shared__ int32_t s_edge[OCT_MAX_LEVEL][2];                   // those of the leaves before / after the block
+k_link_refit(const BuildParams* __restrict__ prm, int batched, int index_base, int link_blocks, int refit_chunks,
+             const int32_t* __restrict__ nfc, const uint8_t* __restrict__ noct, const int32_t* __restrict__ ranc,
+             int64_t astride, const NodeRec* __restrict__ leaves, int32_t* __restrict__ nchild,
+             NodeRec* __restrict__ nodes, int32_t* __restrict__ nlast, float* __restrict__ bmin,
+             float* __restrict__ bmax, uint32_t* __restrict__ slots, int64_t bstride) {
+    __shared__ int32_t s_edge[2];                                  // ancestors of the leaves before / after the block
     __shared__ int32_t s_fid[LR_WAVES], s_lid[LR_WAVES], s_one[LR_WAVES];  // per wave: ids at lanes 0 / 63, one run
     __shared__ uint32_t s_lv[LR_WAVES][7];                         // the maxima at lane 63
-    __shared__ int32_t s_cut[2];                                   // the level's edge slots were written
+    __shared__ int32_t s_cut[2];                                   // the edge slots were written
     const int nrb = (int)gridDim.x - link_blocks;  // the refit's blocks come first
     if ((int)blockIdx.x >= nrb) {
         const int32_t M = prm->n_nodes;
-        for (int32_t id = 1 + (int32_t)((blockIdx.x - nrb) * blockDim.x + threadIdx.x); id < M;
+        for (int32_t id = (int32_t)((blockIdx.x - nrb) * blockDim.x + threadIdx.x); id < M;
              id += (int32_t)(link_blocks * blockDim.x)) {
-            const int32_t p = parent[id];
-            if (p < 0) continue;  // the root of another set
-            const int L = nlevel[id];
-            const uint64_t key = key_morton(sk[nfirst[id] - index_base], batched);
-            const int o = (int)((key >> level_shift(L - 1)) & 7ull);
-            nchild[8 * (int64_t)p + o] = id + index_base;
-            atomicSub((int*)&nodes[p].hi.w, 1);
+            const int32_t f0 = nfc[id], f1 = nfc[id + 1];
+            int32_t row[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+            for (int32_t c = f0; c < f1; ++c) {
+                const int o = noct[c];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (q == o) row[q] = c + index_base;
+            }
+            int4* dst = reinterpret_cast<int4*>(nchild + 8 * (int64_t)id);
+            dst[0] = make_int4(row[0], row[1], row[2], row[3]);
+            dst[1] = make_int4(row[4], row[5], row[6], row[7]);
+            if (f1 > f0) reinterpret_cast<int32_t*>(nodes + id)[7] = f0 - f1;
         }
         return;
     }
+    const int L = (int)blockIdx.x / refit_chunks;  // levels 0..20; level-21 nodes are leaves
     const int32_t nl = prm->n_leaves;
     const int L0 = batched ? -1 : prm->chain_level;  // levels 0..L0: the chain, not refit
+    if (L <= L0) return;
     const int lane = lane_id();
     const int wv = threadIdx.x / OCT_WAVE;
     const uint64_t upto = lane == OCT_WAVE - 1 ? ~0ull : ((2ull << lane) - 1ull);
-    for (int64_t blk = blockIdx.x; blk * LR_THREADS < nl; blk += nrb) {  // block-uniform
+    const int32_t* anc = ranc + (int64_t)L * astride;
+    uint32_t* lslots = slots + (int64_t)L * bstride * 2 * REFIT_SLOT_WORDS;
+    for (int64_t blk = (int)blockIdx.x % refit_chunks; blk * LR_THREADS < nl; blk += refit_chunks) {  // block-uniform
         const int32_t bbase = (int32_t)(blk * LR_THREADS);
         const int32_t lr = bbase + (int32_t)threadIdx.x;
         const bool valid = lr < nl;
-        // every level's ancestor load in flight at once; the first thread also
-        // reads the leaf before the block's, the last one the leaf after
+        // the first thread also reads the leaf before the block's, the last
+        // one the leaf after
         const bool edge = (threadIdx.x == 0 && bbase > 0) || (threadIdx.x == LR_THREADS - 1 && lr + 1 < nl);
         const int64_t ej = threadIdx.x == 0 ? (int64_t)bbase - 1 : (int64_t)lr + 1;
-        int32_t a[OCT_MAX_LEVEL], x[OCT_MAX_LEVEL];
-#pragma unroll
-        for (int L = 0; L < OCT_MAX_LEVEL; ++L) {
-            a[L] = -1;
-            x[L] = -1;
-            if (L > L0) {
-                if (valid) a[L] = ranc[(int64_t)L * astride + lr];
-                if (edge) x[L] = ranc[(int64_t)L * astride + ej];
-            }
-        }
+        const int32_t cur = valid ? anc[lr] : -1;
+        const int32_t x = edge ? anc[ej] : -1;
         uint32_t v[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
         if (valid) {
-            // the leaf's bounds: the exact min / max of its points, eight
-            // loads in flight (a short last group re-reads the first point)
-            const float4 rl = leaves[lr].lo, rh = leaves[lr].hi;
-            const int32_t id = leaf_ids[lr];
-            const int32_t first = __float_as_int(rl.w), cnt = __float_as_int(rh.w);
-            uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
-            uint32_t hi3[3] = {0u, 0u, 0u};
-            for (int32_t j0 = 0; j0 < cnt; j0 += 8) {
-                float4 pp[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) pp[q] = scoord[(int64_t)first + (j0 + q < cnt ? j0 + q : 0)];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const float c3[3] = {pp[q].x, pp[q].y, pp[q].z};
-                    for (int d = 0; d < 3; ++d) {
-                        const uint32_t o = f2o(c3[d]);
-                        lo3[d] = o < lo3[d] ? o : lo3[d];
-                        hi3[d] = o > hi3[d] ? o : hi3[d];
-                    }
-                }
-            }
-            float* lf = reinterpret_cast<float*>(leaves + lr);
-            float* nr = reinterpret_cast<float*>(nodes + id);
-            for (int d = 0; d < 3; ++d) {
-                const float lo = o2f(lo3[d]), hi = o2f(hi3[d]);
-                lf[d] = lo; lf[4 + d] = hi;
-                nr[d] = lo; nr[4 + d] = hi;
-                bmin[3 * (int64_t)id + d] = lo;
-                bmax[3 * (int64_t)id + d] = hi;
-                v[d] = ~lo3[d];
-                v[3 + d] = hi3[d];
-            }
-            v[6] = (uint32_t)(first + cnt - 1 + index_base);
+            const NodeRec rec = leaves[lr];  // bounds from k_topo_emit
+            v[0] = ~f2o(rec.lo.x); v[1] = ~f2o(rec.lo.y); v[2] = ~f2o(rec.lo.z);
+            v[3] = f2o(rec.hi.x); v[4] = f2o(rec.hi.y); v[5] = f2o(rec.hi.z);
+            v[6] = (uint32_t)(__float_as_int(rec.lo.w) + __float_as_int(rec.hi.w) - 1 + index_base);
         }
+        uint32_t* sl = lslots + blk * 2 * REFIT_SLOT_WORDS;
         __syncthreads();  // the last block's LDS reads are done
-#pragma unroll
-        for (int L = 0; L < OCT_MAX_LEVEL; ++L) {
-            s_anc[wv][L][lane] = a[L];
-            if (threadIdx.x == 0) s_edge[L][0] = x[L];
-            if (threadIdx.x == LR_THREADS - 1) s_edge[L][1] = x[L];
+        if (threadIdx.x == 0) {
+            s_edge[0] = x;
+            s_cut[0] = 0;
+            s_cut[1] = 0;
         }
-        for (int L = OCT_MAX_LEVEL - 1; L > L0; --L) {
-            uint32_t* sl = slots + ((int64_t)L * bstride + blk) * 2 * REFIT_SLOT_WORDS;
-            if (threadIdx.x == 0) {
-                s_cut[0] = 0;
-                s_cut[1] = 0;
-            }
-            __syncthreads();  // s_anc written; the last level's carries read
-            const int32_t cur = s_anc[wv][L][lane];
-            const bool active = cur >= 0;
-            const uint64_t am = ballot(active);
-            const int32_t prev = __shfl_up(cur, 1, OCT_WAVE), next = __shfl_down(cur, 1, OCT_WAVE);
-            const bool head = active && (lane == 0 || !((am >> (lane - 1)) & 1ull) || prev != cur);
-            const bool tail = active && (lane == OCT_WAVE - 1 || !((am >> (lane + 1)) & 1ull) || next != cur);
-            const uint64_t hm = ballot(head) & upto;
-            // an inactive lane keeps its values: its segment starts past itself
-            const int seg = active ? 63 - __clzll((long long)hm) : lane + 1;
+        if (threadIdx.x == LR_THREADS - 1) s_edge[1] = x;
+        const bool active = cur >= 0;
+        const uint64_t am = ballot(active);
+        const int32_t prev = __shfl_up(cur, 1, OCT_WAVE), next = __shfl_down(cur, 1, OCT_WAVE);
+        const bool head = active && (lane == 0 || !((am >> (lane - 1)) & 1ull) || prev != cur);
+        const bool tail = active && (lane == OCT_WAVE - 1 || !((am >> (lane + 1)) & 1ull) || next != cur);
+        const uint64_t hm = ballot(head) & upto;
+        // an inactive lane keeps its values: its segment starts past itself
+        const int seg = active ? 63 - __clzll((long long)hm) : lane + 1;
 #pragma unroll
-            for (int q = 0; q < 7; ++q) v[q] = seg_max_scan(v[q], lane, seg);
-            if (lane == 0) s_fid[wv] = cur;
-            if (lane == OCT_WAVE - 1) {
-                s_lid[wv] = cur;
-                s_one[wv] = active && seg == 0;
-                for (int q = 0; q < 7; ++q) s_lv[wv][q] = v[q];
+        for (int q = 0; q < 7; ++q) v[q] = seg_max_scan(v[q], lane, seg);
+        if (lane == 0) s_fid[wv] = cur;
+        if (lane == OCT_WAVE - 1) {
+            s_lid[wv] = cur;
+            s_one[wv] = active && seg == 0;
+            for (int q = 0; q < 7; ++q) s_lv[wv][q] = v[q];
+        }
+        __syncthreads();
+        // the run open into this wave from the block's earlier ones, and
+        // whether it reaches back to the block's first lane
+        int32_t cid = -1;
+        bool creach = false;
+        uint32_t cv[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
+        for (int y = 0; y < wv; ++y) {
+            if (s_one[y] && s_lid[y] == cid) {
+                for (int q = 0; q < 7; ++q) cv[q] = s_lv[y][q] > cv[q] ? s_lv[y][q] : cv[q];
+            } else {
+                cid = s_lid[y];
+                creach = y == 0 && s_one[y];
+                for (int q = 0; q < 7; ++q) cv[q] = s_lv[y][q];
             }
-            __syncthreads();
-            // the run open into this wave from the block's earlier ones, and
-            // whether it reaches back to the block's first lane
-            int32_t cid = -1;
-            bool creach = false;
-            uint32_t cv[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
-            for (int y = 0; y < wv; ++y) {
-                if (s_one[y] && s_lid[y] == cid) {
-                    for (int q = 0; q < 7; ++q) cv[q] = s_lv[y][q] > cv[q] ? s_lv[y][q] : cv[q];
-                } else {
-                    cid = s_lid[y];
-                    creach = y == 0 && s_one[y];
-                    for (int q = 0; q < 7; ++q) cv[q] = s_lv[y][q];
+        }
+        bool reach0 = false;  // this lane's run reaches the block's first lane
+        if (active && seg == 0) {
+            if (cid == cur) {
+                for (int q = 0; q < 7; ++q) v[q] = cv[q] > v[q] ? cv[q] : v[q];
+                reach0 = creach;
+            } else {
+                reach0 = wv == 0;
+            }
+        }
+        const bool cont = tail && lane == OCT_WAVE - 1 && wv < LR_WAVES - 1 && s_fid[wv + 1] == cur;
+        const bool reachE = tail && lane == OCT_WAVE - 1 && wv == LR_WAVES - 1;  // to the block's last lane
+        // all of the node's leaves lie in this run: the leaves either side
+        // of the block are not the node's where the run reaches that edge
+        const bool whole = (!reach0 || s_edge[0] != cur) && (!reachE || s_edge[1] != cur);
+        if (tail && !cont) {
+            if (whole) {
+                store_refit(cur, v, nodes, nlast, bmin, bmax);
+            } else {
+                if (reach0) {
+                    put_slot(sl, cur, v);
+                    s_cut[0] = 1;
+                }
+                if (reachE) {
+                    put_slot(sl + REFIT_SLOT_WORDS, cur, v);
+                    s_cut[1] = 1;
                 }
             }
-            bool reach0 = false;  // this lane's run reaches the block's first lane
-            if (active && seg == 0) {
-                if (cid == cur) {
-                    for (int q = 0; q < 7; ++q) v[q] = cv[q] > v[q] ? cv[q] : v[q];
-                    reach0 = creach;
-                } else {
-                    reach0 = wv == 0;
-                }
-            }
-            const bool cont = tail && lane == OCT_WAVE - 1 && wv < LR_WAVES - 1 && s_fid[wv + 1] == cur;
-            const bool reachE = tail && lane == OCT_WAVE - 1 && wv == LR_WAVES - 1;  // to the block's last lane
-            // all of the node's leaves lie in this run: the leaves either side
-            // of the block are not the node's where the run reaches that edge
-            const bool whole = (!reach0 || s_edge[L][0] != cur) && (!reachE || s_edge[L][1] != cur);
-            if (tail && !cont) {
-                if (whole) {
-                    store_refit(cur, v, nodes, nlast, bmin, bmax);
-                } else {
-                    if (reach0) {
-                        put_slot(sl, cur, v);
-                        s_cut[0] = 1;
-                    }
-                    if (reachE) {
-                        put_slot(sl + REFIT_SLOT_WORDS, cur, v);
-                        s_cut[1] = 1;
-                    }
-                }
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                if (!s_cut[0]) sl[0] = ~0u;
-                if (!s_cut[1]) sl[REFIT_SLOT_WORDS] = ~0u;
-            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (!s_cut[0]) sl[0] = ~0u;
+            if (!s_cut[1]) sl[REFIT_SLOT_WORDS] = ~0u;
         }
     }
 }
@@ -1080,16 +1096,17 @@ hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base,
     hipError_t e = exclusive_scan(b.hist, b.hist, (int64_t)(OCT_NLEVELS + 1) * nblk, b.scan_tmp, b.total, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, n, T, b.topo, b.hist,
-                       nblk, b.total, b.prm, f, index_base, b.nodes, b.nfirst, b.nlast, b.nchild, b.bmin, b.bmax,
-                       b.parent, b.nlevel, b.leaf_ids, b.leaves, b.ranc, n);
-    // child links over the nodes, and the refit over the leaves (at most n of
-    // each; both loops stride over the real counts)
+                       nblk, b.total, b.prm, f, index_base, b.scoord, b.nodes, b.nfirst, b.nlast, b.bmin, b.bmax,
+                       b.parent, b.nlevel, b.nfc, b.noct, b.leaf_ids, b.leaves, b.ranc, n);
+    // child links over the nodes, and the refit over (block of leaves, level)
+    // pairs (at most max_nodes nodes and n leaves; both loops stride over the
+    // real counts)
     const int gl = (int)std::min<int64_t>((max_nodes(n, T) + LR_THREADS - 1) / LR_THREADS, 1024);
-    const int gr = (int)std::min<int64_t>(refit_blocks(n), 512);
+    const int rc = (int)std::min<int64_t>(refit_blocks(n), 256);
     const int64_t bstride = refit_blocks(n);  // refit blocks of leaves, at most
-    hipLaunchKernelGGL(k_link_refit, dim3(gl + gr), dim3(LR_THREADS), 0, s, sk, b.prm, f.batched, index_base, gl,
-                       b.parent, b.nfirst, b.nlevel, b.ranc, n, b.leaf_ids, b.scoord, b.leaves, b.nchild, b.nodes,
-                       b.nlast, b.bmin, b.bmax, b.rslots, bstride);
+    hipLaunchKernelGGL(k_link_refit, dim3(rc * OCT_MAX_LEVEL + gl), dim3(LR_THREADS), 0, s, b.prm, f.batched,
+                       index_base, gl, rc, b.nfc, b.noct, b.ranc, n, b.leaves, b.nchild, b.nodes, b.nlast, b.bmin,
+                       b.bmax, b.rslots, bstride);
     hipLaunchKernelGGL(k_refit_merge, dim3(OCT_MAX_LEVEL), dim3(RM_THREADS), 0, s, b.prm, f.batched, b.rslots,
                        bstride, b.nodes, b.nlast, b.bmin, b.bmax);
     return hipGetLastError();

@@ -82,7 +82,7 @@ struct SetState {
     DevBuf soff;                    // {0, n}: the one-set forest descriptor
     int64_t soff_n = -1;            // the n soff holds
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
-        nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax, rslots, ranc;
+        nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax, rslots, ranc, nfc, noct;
     // the buffers oct_get_tree and oct_device_view hand out
     void export_to(std::vector<void*>* r) {
         DevBuf* out[] = {&perm, &nfirst, &nlast, &nchild, &bmin, &bmax, &nodes, &scoord, &pad,
@@ -115,6 +115,8 @@ struct SetState {
         b.bmax = bmax.as<float>();
         b.rslots = rslots.as<uint32_t>();
         b.ranc = ranc.as<int32_t>();
+        b.nfc = nfc.as<int32_t>();
+        b.noct = noct.as<uint8_t>();
         return b;
     }
     // The refit step (bounds, refit, topology: some 25 launches) as a HIP
@@ -139,7 +141,7 @@ struct SetState {
         drop_graphs();
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
                          &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
-                         &rslots, &ranc, &rk, &ri, &rsk, &rsi, &rpos, &soff, &csrc, &pad, &qfirst, &qlast,
+                         &rslots, &ranc, &nfc, &noct, &rk, &ri, &rsk, &rsi, &rpos, &soff, &csrc, &pad, &qfirst, &qlast,
                          &qkids, &qbmin, &qbmax, &p64, &partials64, &c32, &bmin64, &bmax64};
         for (DevBuf* d : all) d->release();
         soff_n = -1;
@@ -543,6 +545,8 @@ int ensure_build_bufs(oct_ctx* c, SetState& s, int64_t n, int64_t M) {
     HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
     HIP_TRY(c, s.rslots.ensure(4 * (size_t)refit_slot_words(n)));
     HIP_TRY(c, s.ranc.ensure(4 * (size_t)OCT_MAX_LEVEL * (size_t)n));
+    HIP_TRY(c, s.nfc.ensure(4 * ((size_t)M + 1)));
+    HIP_TRY(c, s.noct.ensure((size_t)M));
     return OCT_OK;
 }
 
@@ -618,7 +622,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
             const DevBuf* bufs[] = {&s.prm, &s.partials, &s.keys0, &s.keys1, &s.vals0, &s.vals1, &s.hist, &s.scan_tmp,
                                     &s.total, &s.topo, &s.perm, &s.scoord, &s.nodes, &s.nfirst, &s.nlast, &s.nchild,
                                     &s.parent, &s.nlevel, &s.leaf_ids, &s.leaves, &s.bmin, &s.bmax, &s.rslots,
-                                    &s.ranc, &s.rk, &s.ri,
+                                    &s.ranc, &s.nfc, &s.noct, &s.rk, &s.ri,
                                     &s.rsk, &s.rsi, &s.rpos, &s.soff, &s.csrc};
             for (const DevBuf* d : bufs) mix((uint64_t)(uintptr_t)d->p);
             uint32_t rbits = 0;
@@ -1905,6 +1909,8 @@ int batch_ensure(oct_ctx* c, BatchState& B) {
     HIP_TRY(c, s.bmax.ensure(12 * (size_t)M));
     HIP_TRY(c, s.rslots.ensure(4 * (size_t)refit_slot_words(N)));
     HIP_TRY(c, s.ranc.ensure(4 * (size_t)OCT_MAX_LEVEL * (size_t)N));
+    HIP_TRY(c, s.nfc.ensure(4 * ((size_t)M + 1)));
+    HIP_TRY(c, s.noct.ensure((size_t)M));
     PairState& p = B.pair;
     HIP_TRY(c, p.flags.ensure(4 * BF_WORDS));
     HIP_TRY(c, p.lcount.ensure(4 * (size_t)N));

@@ -165,6 +165,8 @@ struct BuildBufs {
     float *bmin, *bmax;
     uint32_t* rslots;    // refit: the runs cut by a block's edges, refit_slot_words(n)
     int32_t* ranc;       // refit: per level (OCT_MAX_LEVEL rows of n) each leaf's ancestor, -1 none
+    int32_t* nfc;        // per node (and one past the last): the id its first child has or would have
+    uint8_t* noct;       // per node: its octant in its parent
 };
 
 int bounds_blocks(int64_t n);
