@@ -272,13 +272,14 @@ __device__ inline bool key_idx_less(uint64_t ka, int32_t ia, uint64_t kb, int32_
 // K1: new keys in the previous order; vals0 = PI, vals1 = changed flag.
 __global__ void __launch_bounds__(RF_THREADS)
 k_refit_keys(const float* const* __restrict__ csrc, int64_t n, const BuildParams* __restrict__ prm,
-             const uint64_t* __restrict__ pk, const float4* __restrict__ prev_sc, uint64_t* __restrict__ nk,
-             int32_t* __restrict__ pi, uint32_t* __restrict__ chg, uint64_t* __restrict__ tile_sums) {
+             const uint64_t* __restrict__ pk, const int32_t* __restrict__ prev_perm, int index_base,
+             uint64_t* __restrict__ nk, int32_t* __restrict__ pi, uint32_t* __restrict__ chg,
+             uint64_t* __restrict__ tile_sums) {
     const float* __restrict__ coords = *csrc;  // this step's coordinates
     const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
     bool changed = false;
     if (i < n) {
-        const int32_t idx = __float_as_int(prev_sc[i].w);
+        const int32_t idx = prev_perm[i] - index_base;
         const float ox = prm->origin[0], oy = prm->origin[1], oz = prm->origin[2];
         const float inv = prm->inv_cs;
         const float px = coords[3 * (int64_t)idx], py = coords[3 * (int64_t)idx + 1],
@@ -328,8 +329,15 @@ k_refit_compact(int64_t n, const uint32_t* __restrict__ cpos, const uint64_t* __
 }
 
 // K4: rank of each changed point within C (all-pairs count, C staged in LDS
-// tiles) and its number of unchanged predecessors (binary search of the
-// previous order); writes C sorted (csk, csi) with final positions cspos.
+// tiles) and its number of unchanged predecessors (a search of the previous
+// order); writes C sorted (csk, csi) with final positions cspos.  While the
+// grid holds RF_LANES threads per changed point (the usual case: a time step
+// changes few keys), the RF_LANES consecutive lanes of a point split its
+// count and search the previous order RF_LANES-ary: each round probes
+// RF_LANES positions at once and keeps the gap between the last probe below
+// the point and the first one not below, so 1M positions take 5 rounds of
+// dependent loads instead of 20.  Otherwise one thread per point.
+constexpr int RF_LANES = 16;
 __global__ void __launch_bounds__(RF_THREADS)
 k_refit_rank(int64_t n, const uint64_t* __restrict__ total, const uint64_t* __restrict__ ck,
              const int32_t* __restrict__ ci, const uint64_t* __restrict__ pk, const int32_t* __restrict__ pi,
@@ -338,10 +346,14 @@ k_refit_rank(int64_t n, const uint64_t* __restrict__ total, const uint64_t* __re
     __shared__ uint64_t tk[RF_TILE];
     __shared__ int32_t ti[RF_TILE];
     const int64_t nc = (int64_t)*total;
+    const bool wide = nc * RF_LANES <= (int64_t)gridDim.x * RF_THREADS;
+    const int S = wide ? RF_LANES : 1;
+    const int64_t first = (int64_t)blockIdx.x * (RF_THREADS / S);  // this block's first point
     // block-uniform exits (before any barrier): a failed refit, or a block past the changed points
-    if (prm->refit_fail || (int64_t)blockIdx.x * RF_THREADS >= nc) return;
-    const int64_t c = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
-    const bool mine = c < nc;
+    if (prm->refit_fail || first >= nc) return;
+    const int sub = (int)threadIdx.x % S;
+    const int64_t c = first + (int64_t)threadIdx.x / S;
+    const bool mine = c < nc;  // the same for the S lanes of a point
     const uint64_t key = mine ? ck[c] : 0ull;
     const int32_t idx = mine ? ci[c] : 0;
     uint32_t rank = 0;
@@ -354,15 +366,32 @@ k_refit_rank(int64_t n, const uint64_t* __restrict__ total, const uint64_t* __re
         }
         __syncthreads();
         if (mine)
-            for (int j = 0; j < tn; ++j) rank += key_idx_less(tk[j], ti[j], key, idx) ? 1u : 0u;
+            for (int j = sub; j < tn; j += S) rank += key_idx_less(tk[j], ti[j], key, idx) ? 1u : 0u;
+    }
+    int64_t lo = 0, hi = n;  // first position of the previous order not below (key, idx)
+    if (wide) {              // block-uniform
+#pragma unroll
+        for (int o = 1; o < RF_LANES; o <<= 1) rank += (uint32_t)__shfl_xor((int)rank, o, RF_LANES);
+        const int g0 = (lane_id() / RF_LANES) * RF_LANES;  // the point's first lane in the wave
+        while (mine && lo < hi) {  // the same for the point's lanes
+            const int64_t m = hi - lo;
+            const int64_t p = lo + ((int64_t)(sub + 1) * m) / (RF_LANES + 1);  // in [lo, hi), ascending
+            const bool below = key_idx_less(pk[p], pi[p], key, idx);
+            const int t = __popc((uint32_t)((ballot(below) >> g0) & ((1ull << RF_LANES) - 1ull)));
+            const int64_t nlo = t > 0 ? lo + ((int64_t)t * m) / (RF_LANES + 1) + 1 : lo;
+            const int64_t nhi = t < RF_LANES ? lo + ((int64_t)(t + 1) * m) / (RF_LANES + 1) : hi;
+            lo = nlo;
+            hi = nhi;
+        }
+        if (sub != 0) return;
+    } else {
+        while (mine && lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (key_idx_less(pk[mid], pi[mid], key, idx)) lo = mid + 1;
+            else hi = mid;
+        }
     }
     if (!mine) return;
-    int64_t lo = 0, hi = n;  // first position of the previous order not below (key, idx)
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (key_idx_less(pk[mid], pi[mid], key, idx)) lo = mid + 1;
-        else hi = mid;
-    }
     const uint32_t before_changed = lo < n ? cpos[lo] : (uint32_t)nc;
     const uint32_t unchanged_before = (uint32_t)lo - before_changed;
     csk[rank] = key;
@@ -1057,11 +1086,13 @@ hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int i
 // (keys[cur] and scoord hold the previous sort) and the changed count in
 // *b.total.  The caller reads the count and either finishes the refit or
 // runs the full sort.
-hipError_t launch_refit_prepare(const float* const* csrc, int64_t n, int cur, const BuildBufs& b, hipStream_t s) {
+hipError_t launch_refit_prepare(const float* const* csrc, int64_t n, int cur, int index_base, const BuildBufs& b,
+                                hipStream_t s) {
     const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
     // the tile sums in scan_tmp were zeroed by the bounds fold just before
-    hipLaunchKernelGGL(k_refit_keys, dim3(g), dim3(RF_THREADS), 0, s, csrc, n, b.prm, b.keys[cur], b.scoord,
-                       b.keys[cur ^ 1], b.vals[0], (uint32_t*)b.vals[1], reinterpret_cast<uint64_t*>(b.scan_tmp));
+    hipLaunchKernelGGL(k_refit_keys, dim3(g), dim3(RF_THREADS), 0, s, csrc, n, b.prm, b.keys[cur], b.perm,
+                       index_base, b.keys[cur ^ 1], b.vals[0], (uint32_t*)b.vals[1],
+                       reinterpret_cast<uint64_t*>(b.scan_tmp));
     return exclusive_scan_summed((const uint32_t*)b.vals[1], (uint32_t*)b.vals[1], n, b.scan_tmp, b.total, s);
 }
 

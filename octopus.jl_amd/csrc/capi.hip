@@ -605,7 +605,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
         const float* const* csrc = s.csrc.as<const float*>();
         auto enqueue = [&]() -> int {
             Phase ph(c, OCT_PROF_BUILD);
-            HIP_TRY(c, launch_refit_prepare(csrc, n, s.cur, b, c->stream));
+            HIP_TRY(c, launch_refit_prepare(csrc, n, s.cur, c->index_base, b, c->stream));
             HIP_TRY(c, launch_refit_finish(csrc, n, limit, s.key_bits, s.cur, c->index_base, b, r, c->stream));
             HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, s.cur ^ 1, b, f, c->stream));
             return OCT_OK;

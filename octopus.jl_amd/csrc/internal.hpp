@@ -207,7 +207,8 @@ struct RefitBufs {
 };
 // The refit reads the coordinates through *csrc (device memory, written by
 // the bounds stage), so a captured refit follows the caller's array.
-hipError_t launch_refit_prepare(const float* const* csrc, int64_t n, int cur, const BuildBufs& b, hipStream_t s);
+hipError_t launch_refit_prepare(const float* const* csrc, int64_t n, int cur, int index_base, const BuildBufs& b,
+                                hipStream_t s);
 hipError_t launch_refit_finish(const float* const* csrc, int64_t n, int64_t limit, int key_bits, int cur,
                                int index_base, const BuildBufs& b, const RefitBufs& r, hipStream_t s);
 

@@ -81,6 +81,16 @@ def test_refit_sph_steps_match_full_rebuild(orc):
     assert all(0 < i["changed_points"] < pts.shape[0] // 8 for i in infos)
 
 
+def test_refit_many_changed_points(orc):
+    # more changed points than the rank kernel's grid holds at 16 lanes per
+    # point (64,000 points: 8,192 threads, so more than 512 changed), still
+    # within the refit limit of N / 8
+    pts, r = datasets.sph_lattice(seed=7, side=40)
+    infos = _steps(pts, r, 1, 2e-3, 5, check_oracle=orc)
+    assert all(i["refit_used"] for i in infos)
+    assert all(512 < i["changed_points"] < pts.shape[0] // 8 for i in infos)
+
+
 def test_refit_ties_and_coincident_points(orc):
     # many exactly coincident points (equal keys, order by index) with movers
     base = np.repeat(datasets.uniform_cube(200, 3), 20, axis=0)
