@@ -96,8 +96,30 @@ def test_sph_lattice_shape():
 
 def test_device_views_hold_their_owner():
     """A wrapped result keeps the TNS (or batch) whose context owns its
-    memory alive: the handle torch keeps for the tensor refers to the owner."""
-    owner = object()
-    h = octohip._DeviceArray(0x1000, (4,), "<i4", owner)
-    assert h.owner is owner
-    assert h.__cuda_array_interface__["data"] == (0x1000, False)
+    memory alive: the DLPack struct torch holds for the tensor keeps the
+    owner referenced until torch's deleter releases it."""
+    import ctypes
+    import gc
+    import weakref
+
+    class Owner:
+        pass
+
+    owner = Owner()
+    ref = weakref.ref(owner)
+    cap = octohip._dl_capsule(0x1000, (4, 3), "<f4", 0, owner)
+    get = ctypes.pythonapi.PyCapsule_GetPointer
+    get.restype = ctypes.c_void_p
+    get.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    addr = get(cap, b"dltensor")
+    dm = octohip._DLManagedTensor.from_address(addr)
+    assert dm.dl_tensor.data == 0x1000 and dm.dl_tensor.ndim == 2
+    assert (dm.dl_tensor.device.device_type, dm.dl_tensor.device.device_id) == (10, 0)
+    assert (dm.dl_tensor.dtype.code, dm.dl_tensor.dtype.bits, dm.dl_tensor.dtype.lanes) == (2, 32, 1)
+    assert [dm.dl_tensor.shape[i] for i in range(2)] == [4, 3] and not dm.dl_tensor.strides
+    del owner
+    gc.collect()
+    assert ref() is not None  # held by the handed-out struct
+    dm.deleter(ctypes.pointer(dm))  # what torch calls when the tensor dies
+    gc.collect()
+    assert ref() is None
