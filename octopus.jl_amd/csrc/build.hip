@@ -553,6 +553,49 @@ k_topo_count(const uint64_t* __restrict__ sk, int64_t n, int T, int batched, uin
     }
 }
 
+// The topology table holds one row per level (0..21) and one for the leaves
+// (22), each with one count per k_topo_count block.  One block per row scans
+// it in place (exclusive, in chunks of TR_THREADS * TR_ITEMS with the carry
+// kept) and writes the row's total after the table (row 23): k_topo_emit
+// turns the totals of the rows before a level into the level's first id.
+constexpr int TR_THREADS = 1024;
+constexpr int TR_ITEMS = 4;
+__global__ void __launch_bounds__(TR_THREADS) k_topo_rows(uint32_t* __restrict__ hist, int nblk) {
+    __shared__ uint32_t s_w[TR_THREADS / OCT_WAVE];
+    uint32_t* row = hist + (int64_t)blockIdx.x * nblk;
+    const int lane = lane_id(), w = (int)threadIdx.x / OCT_WAVE;
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < nblk; c0 += TR_THREADS * TR_ITEMS) {  // block-uniform
+        uint32_t v[TR_ITEMS];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < TR_ITEMS; ++q) {
+            const int i = c0 + (int)threadIdx.x * TR_ITEMS + q;
+            v[q] = i < nblk ? row[i] : 0u;
+            sum += v[q];
+        }
+        const uint32_t incl = (uint32_t)wave_incl_scan((int)sum);
+        if (lane == OCT_WAVE - 1) s_w[w] = incl;
+        __syncthreads();
+        uint32_t wpre = 0, tot = 0;
+        for (int i = 0; i < TR_THREADS / OCT_WAVE; ++i) {
+            const uint32_t x = s_w[i];
+            wpre += i < w ? x : 0u;
+            tot += x;
+        }
+        uint32_t pre = carry + wpre + incl - sum;
+#pragma unroll
+        for (int q = 0; q < TR_ITEMS; ++q) {
+            const int i = c0 + (int)threadIdx.x * TR_ITEMS + q;
+            if (i < nblk) row[i] = pre;
+            pre += v[q];
+        }
+        carry += tot;
+        __syncthreads();  // s_w is rewritten by the next chunk
+    }
+    if (threadIdx.x == 0) hist[(int64_t)(OCT_NLEVELS + 1) * nblk + blockIdx.x] = carry;
+}
+
 // first j in [lo, hi) with pred(j) true, pred monotone false..true; hi if none
 template <typename P>
 __device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
@@ -583,8 +626,7 @@ __device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
 __global__ void __launch_bounds__(TOPO_THREADS)
 k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
             const uint32_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
-            const uint64_t* __restrict__ total, BuildParams* __restrict__ prm, Forest forest, int index_base,
-            const float4* __restrict__ scoord, NodeRec* __restrict__ nodes, int32_t* __restrict__ nfirst,
+            BuildParams* __restrict__ prm, Forest forest, int index_base, const float4* __restrict__ scoord, NodeRec* __restrict__ nodes, int32_t* __restrict__ nfirst,
             int32_t* __restrict__ nlast, float* __restrict__ bmin, float* __restrict__ bmax,
             int32_t* __restrict__ parent, uint8_t* __restrict__ nlevel, int32_t* __restrict__ nfc,
             uint8_t* __restrict__ noct, int32_t* __restrict__ leaf_ids, NodeRec* __restrict__ leaves,
@@ -594,14 +636,16 @@ k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
     __shared__ uint64_t wmask[NW][OCT_NLEVELS + 1];  // per wave and level: lanes starting a node
     __shared__ uint32_t woff[NW][OCT_NLEVELS + 1];   // ids before the wave's first such lane
     __shared__ uint32_t s_loff[OCT_NLEVELS + 1];     // the block's nodes at levels below L
+    __shared__ int32_t s_M, s_nl;                    // nodes and leaves in the tree
     __shared__ uint32_t s_topo[TOPO_THREADS];
     __shared__ uint64_t s_key[TOPO_THREADS];
     __shared__ uint16_t s_list[EMIT_LIST];           // position | level << 8, one round's nodes
     const int64_t base = (int64_t)blockIdx.x * TOPO_THREADS;
     // loads that do not depend on the ranks are issued first, so their
     // latencies overlap the ballots and barriers below
-    const uint32_t hrow = threadIdx.x <= OCT_NLEVELS ? hoff[(int64_t)threadIdx.x * nblk + blockIdx.x] : 0u;
-    const int32_t M = (int32_t)hoff[(int64_t)OCT_NLEVELS * nblk];
+    // row L of the table: this block's offset within the row, and the row's total
+    const uint32_t hrel = threadIdx.x <= OCT_NLEVELS ? hoff[(int64_t)threadIdx.x * nblk + blockIdx.x] : 0u;
+    const uint32_t rtot = threadIdx.x <= OCT_NLEVELS ? hoff[(int64_t)(OCT_NLEVELS + 1) * nblk + threadIdx.x] : 0u;
     const int64_t k0 = base + threadIdx.x;
     const uint64_t key0 = k0 < n ? sk[k0] : 0ull;
     int64_t s_lo1 = 0, s_hi1 = 0;  // one set: its range and chain depth
@@ -630,6 +674,14 @@ k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
     __syncthreads();
     if (threadIdx.x < OCT_WAVE) {  // the block's offset per level, then each wave's
         const int L = threadIdx.x;
+        // ids before level L: the totals of the rows before it (row 22, the
+        // leaves, then starts at the node count)
+        const int rb = wave_incl_scan((int)rtot) - (int)rtot;
+        if (L == OCT_NLEVELS) {
+            s_M = rb;
+            s_nl = (int)rtot;
+        }
+        const uint32_t hrow = hrel + (uint32_t)rb;
         uint32_t s = hrow;
         if (L <= OCT_NLEVELS)
             for (int q = 0; q < NW; ++q) {
@@ -643,10 +695,11 @@ k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
         if (L <= OCT_NLEVELS) s_loff[L] = (uint32_t)(incl - c);
     }
     __syncthreads();
+    const int32_t M = s_M;
     if (k0 == 0) {
         prm->n_nodes = M;
-        prm->n_leaves = (int32_t)(*total) - M;  // the scan's last row counts the leaves
-        nfc[M] = M;                             // the first child of a node past the last
+        prm->n_leaves = s_nl;
+        nfc[M] = M;  // the first child of a node past the last
     }
     // count of level-L starts before position base + p, plus the level's offset
     auto before = [&](int L, int p) -> int32_t {
@@ -1028,7 +1081,7 @@ int64_t sort_tiles(int64_t n) { return (n + RS_TILE - 1) / RS_TILE; }
 int64_t topo_blocks(int64_t n) { return (n + TOPO_THREADS - 1) / TOPO_THREADS; }
 int64_t hist_words(int64_t n) {
     int64_t a = (int64_t)RS_BUCKETS * sort_tiles(n);
-    int64_t b = (int64_t)(OCT_NLEVELS + 1) * topo_blocks(n);
+    int64_t b = (int64_t)(OCT_NLEVELS + 1) * topo_blocks(n) + OCT_NLEVELS + 1;  // and the row totals
     return a > b ? a : b;
 }
 // Nodes: at most n leaves, and at most n / (T + 1) internal nodes per level.
@@ -1124,10 +1177,9 @@ hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base,
     const int nblk = (int)topo_blocks(n);
     hipLaunchKernelGGL(k_topo_count, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, n, T, f.batched, b.topo, b.hist,
                        nblk);
-    hipError_t e = exclusive_scan(b.hist, b.hist, (int64_t)(OCT_NLEVELS + 1) * nblk, b.scan_tmp, b.total, s);
-    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_topo_rows, dim3(OCT_NLEVELS + 1), dim3(TR_THREADS), 0, s, b.hist, nblk);
     hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, n, T, b.topo, b.hist,
-                       nblk, b.total, b.prm, f, index_base, b.scoord, b.nodes, b.nfirst, b.nlast, b.bmin, b.bmax,
+                       nblk, b.prm, f, index_base, b.scoord, b.nodes, b.nfirst, b.nlast, b.bmin, b.bmax,
                        b.parent, b.nlevel, b.nfc, b.noct, b.leaf_ids, b.leaves, b.ranc, n);
     // child links over the nodes, and the refit over (block of leaves, level)
     // pairs (at most max_nodes nodes and n leaves; both loops stride over the
