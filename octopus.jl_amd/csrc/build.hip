@@ -269,12 +269,17 @@ __device__ inline bool key_idx_less(uint64_t ka, int32_t ia, uint64_t kb, int32_
     return ka < kb || (ka == kb && ia < ib);
 }
 
-// K1: new keys in the previous order; vals0 = PI, vals1 = changed flag.
+// K1: new keys in the previous order (pi = PI, and per group of 64
+// positions the mask of those whose key changed, wmask) and the changed
+// count of each tile of RF_TILE_POS positions (tile sums, zeroed by the
+// bounds fold).
+constexpr int RF_TILE_POS = 4096;
+constexpr int RF_TILE_GROUPS = RF_TILE_POS / OCT_WAVE;  // 64: one lane each in K3
+static_assert(RF_TILE_GROUPS == OCT_WAVE, "K3 gives each group of a tile one lane");
 __global__ void __launch_bounds__(RF_THREADS)
 k_refit_keys(const float* const* __restrict__ csrc, int64_t n, const BuildParams* __restrict__ prm,
              const uint64_t* __restrict__ pk, const int32_t* __restrict__ prev_perm, int index_base,
-             uint64_t* __restrict__ nk, int32_t* __restrict__ pi, uint32_t* __restrict__ chg,
-             uint64_t* __restrict__ tile_sums) {
+             int32_t* __restrict__ pi, RefitLook lk) {
     const float* __restrict__ coords = *csrc;  // this step's coordinates
     const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
     bool changed = false;
@@ -286,46 +291,79 @@ k_refit_keys(const float* const* __restrict__ csrc, int64_t n, const BuildParams
                     pz = coords[3 * (int64_t)idx + 2];
         const uint64_t k = morton3(floor_clamp((px - ox) * inv, 0x1fffffu),
                                    floor_clamp((py - oy) * inv, 0x1fffffu), floor_clamp((pz - oz) * inv, 0x1fffffu));
-        nk[i] = k;
         pi[i] = idx;
         changed = k != pk[i];
-        chg[i] = changed ? 1u : 0u;
     }
-    // the changed count of this block into its scan tile's sum (the scan of
-    // chg then needs no pass of its own to sum the tiles)
+    // this block's changed count into its tile's sum
     __shared__ uint32_t bc;
     if (threadIdx.x == 0) bc = 0;
     __syncthreads();
     const uint64_t m = ballot(changed);
-    if (lane_id() == 0 && m) atomicAdd(&bc, (uint32_t)__popcll(m));
+    const int64_t g = (int64_t)blockIdx.x * (RF_THREADS / OCT_WAVE) + (int)threadIdx.x / OCT_WAVE;
+    if (lane_id() == 0) {
+        if (g * OCT_WAVE < n) lk.wmask[g] = m;
+        if (m) atomicAdd(&bc, (uint32_t)__popcll(m));
+    }
     __syncthreads();
     if (threadIdx.x == 0 && bc) {
-        const int64_t tile = ((int64_t)blockIdx.x * RF_THREADS) / SCAN_TILE_INPUTS;
-        atomicAdd(reinterpret_cast<unsigned long long*>(tile_sums + tile), (unsigned long long)bc);
+        const int64_t tile = ((int64_t)blockIdx.x * RF_THREADS) / RF_TILE_POS;
+        atomicAdd(reinterpret_cast<unsigned long long*>(lk.tsum + tile), (unsigned long long)bc);
     }
 }
 
-// K3: whether the refit can run (the changed count within limit and the key
-// width unchanged; thread 0 records both in prm for the later kernels and the
-// host), then compact the changed points (their new key and index).
-__global__ void __launch_bounds__(RF_THREADS)
-k_refit_compact(int64_t n, const uint32_t* __restrict__ cpos, const uint64_t* __restrict__ total,
-                const uint64_t* __restrict__ nk, const int32_t* __restrict__ pi, uint64_t* __restrict__ ck,
-                int32_t* __restrict__ ci, int64_t limit, int key_bits, BuildParams* __restrict__ prm) {
-    const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
-    const uint64_t nc = *total;
-    const bool fail = nc > (uint64_t)limit || prm->key_bits != key_bits;
-    if (i == 0) {
-        prm->changed = (uint32_t)(nc < 0xffffffffull ? nc : 0xffffffffull);
-        prm->refit_fail = fail ? 1u : 0u;
+// K3: one wave per tile: the changed count before the tile (its tile sums
+// below it), before each of its groups (wpre), and the changed points C
+// compacted in the previous order (ck, ci; their new keys computed again, as
+// K1 did).  Tile 0 records the total and whether the refit can run (within
+// `limit`, key width unchanged) for the kernels after it and the host.
+__global__ void __launch_bounds__(OCT_WAVE)
+k_refit_compact(const float* const* __restrict__ csrc, int64_t n, BuildParams* __restrict__ prm,
+                const int32_t* __restrict__ pi, RefitLook lk, uint64_t* __restrict__ ck, int32_t* __restrict__ ci,
+                int64_t limit, int key_bits, uint64_t* __restrict__ total) {
+    const int64_t ntiles = (n + RF_TILE_POS - 1) / RF_TILE_POS;
+    const int64_t t = blockIdx.x;
+    const int lane = lane_id();
+    uint64_t below = 0, all = 0;
+    for (int64_t u = lane; u < ntiles; u += OCT_WAVE) {
+        const uint64_t v = lk.tsum[u];
+        all += v;
+        below += u < t ? v : 0ull;
     }
-    if (i >= n || fail) return;
-    const uint32_t c = cpos[i];
-    const uint32_t next = i + 1 < n ? cpos[i + 1] : (uint32_t)*total;
-    if (next != c) {
-        ck[c] = nk[i];
-        ci[c] = pi[i];
+    below = wave_sum_u64(below);
+    all = wave_sum_u64(all);
+    const int64_t g = t * RF_TILE_GROUPS + lane;
+    const uint64_t m = g * OCT_WAVE < n ? lk.wmask[g] : 0ull;
+    const int c = __popcll(m);
+    const uint64_t gpre = below + (uint64_t)(wave_incl_scan(c) - c);
+    if (g * OCT_WAVE < n) lk.wpre[g] = (uint32_t)gpre;
+    if (t == 0 && lane == 0) {
+        *total = all;
+        prm->changed = (uint32_t)(all < 0xffffffffull ? all : 0xffffffffull);
+        prm->refit_fail = (all > (uint64_t)limit || prm->key_bits != key_bits) ? 1u : 0u;
     }
+    if (!m || all > (uint64_t)limit) return;
+    const float* __restrict__ coords = *csrc;
+    const float ox = prm->origin[0], oy = prm->origin[1], oz = prm->origin[2];
+    const float inv = prm->inv_cs;
+    uint64_t rest = m;
+    for (uint64_t pos = gpre; rest; ++pos) {
+        const int b = __builtin_ctzll(rest);
+        rest &= rest - 1ull;
+        const int64_t i = g * OCT_WAVE + b;
+        const int32_t idx = pi[i];
+        const float* p = coords + 3 * (int64_t)idx;
+        ck[pos] = morton3(floor_clamp((p[0] - ox) * inv, 0x1fffffu), floor_clamp((p[1] - oy) * inv, 0x1fffffu),
+                          floor_clamp((p[2] - oz) * inv, 0x1fffffu));
+        ci[pos] = idx;
+    }
+}
+
+// changed positions of the previous order before position i (i <= n)
+__device__ inline uint32_t changed_before(const RefitLook& lk, int64_t i, int64_t n, uint32_t nc) {
+    if (i >= n) return nc;
+    const uint64_t m = lk.wmask[i / OCT_WAVE];
+    const int l = (int)(i % OCT_WAVE);
+    return lk.wpre[i / OCT_WAVE] + (uint32_t)__popcll(m & ((1ull << l) - 1ull));
 }
 
 // K4: rank of each changed point within C (all-pairs count, C staged in LDS
@@ -341,7 +379,7 @@ constexpr int RF_LANES = 16;
 __global__ void __launch_bounds__(RF_THREADS)
 k_refit_rank(int64_t n, const uint64_t* __restrict__ total, const uint64_t* __restrict__ ck,
              const int32_t* __restrict__ ci, const uint64_t* __restrict__ pk, const int32_t* __restrict__ pi,
-             const uint32_t* __restrict__ cpos, uint64_t* __restrict__ csk, int32_t* __restrict__ csi,
+             RefitLook lk, uint64_t* __restrict__ csk, int32_t* __restrict__ csi,
              uint32_t* __restrict__ cspos, const BuildParams* __restrict__ prm) {
     __shared__ uint64_t tk[RF_TILE];
     __shared__ int32_t ti[RF_TILE];
@@ -392,7 +430,7 @@ k_refit_rank(int64_t n, const uint64_t* __restrict__ total, const uint64_t* __re
         }
     }
     if (!mine) return;
-    const uint32_t before_changed = lo < n ? cpos[lo] : (uint32_t)nc;
+    const uint32_t before_changed = changed_before(lk, lo, n, (uint32_t)nc);
     const uint32_t unchanged_before = (uint32_t)lo - before_changed;
     csk[rank] = key;
     csi[rank] = idx;
@@ -411,7 +449,7 @@ __device__ inline void refit_write(int64_t pos, uint64_t key, int32_t idx, const
 // K5: unchanged points to their new positions; K6 (c < |C|): changed points.
 __global__ void __launch_bounds__(RF_THREADS)
 k_refit_place(const float* const* __restrict__ csrc, int64_t n, const uint64_t* __restrict__ total,
-              const uint64_t* __restrict__ pk, const int32_t* __restrict__ pi, const uint32_t* __restrict__ cpos,
+              const uint64_t* __restrict__ pk, const int32_t* __restrict__ pi, RefitLook lk,
               const uint64_t* __restrict__ csk, const int32_t* __restrict__ csi, const uint32_t* __restrict__ cspos,
               uint64_t* __restrict__ kout, int32_t* __restrict__ perm, float4* __restrict__ sc, int index_base,
               const BuildParams* __restrict__ prm) {
@@ -424,9 +462,10 @@ k_refit_place(const float* const* __restrict__ csrc, int64_t n, const uint64_t* 
     }
     if (i < nc) refit_write(cspos[i], csk[i], csi[i], coords, kout, perm, sc, index_base);
     if (i >= n) return;
-    const uint32_t c = cpos[i];
-    const uint32_t next = i + 1 < n ? cpos[i + 1] : (uint32_t)nc;
-    if (next != c) return;  // changed: placed above
+    const uint64_t wm = lk.wmask[i / OCT_WAVE];
+    const int l = (int)(i % OCT_WAVE);
+    if ((wm >> l) & 1ull) return;  // changed: placed above
+    const uint32_t c = lk.wpre[i / OCT_WAVE] + (uint32_t)__popcll(wm & ((1ull << l) - 1ull));
     const uint64_t key = pk[i];
     const int32_t idx = pi[i];
     int64_t lo = 0, hi = nc;  // changed points below (key, idx)
@@ -1096,8 +1135,8 @@ hipError_t launch_bounds_stage(const float* coords, int64_t n, float radius, flo
         hipLaunchKernelGGL(k_bounds_partial<true>, dim3(nb), dim3(BND_THREADS), 0, s, coords, n, copy, b.partials);
     else
         hipLaunchKernelGGL(k_bounds_partial<false>, dim3(nb), dim3(BND_THREADS), 0, s, coords, n, copy, b.partials);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BND_THREADS), 0, s, b.partials, nb, radius, b.prm,
-                       reinterpret_cast<uint64_t*>(b.scan_tmp), scan_tiles(n), coords, csrc);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BND_THREADS), 0, s, b.partials, nb, radius, b.prm, b.rlook,
+                       b.rlook ? refit_look_zero_words(n) : 0, coords, csrc);
     return hipGetLastError();
 }
 
@@ -1136,35 +1175,47 @@ hipError_t launch_sort_stage(const float* coords, int64_t n, int key_bits, int i
 }
 
 // Stage 2a' (almost-sorted path), part 1: new keys in the previous order
-// (keys[cur] and scoord hold the previous sort) and the changed count in
-// *b.total.  The caller reads the count and either finishes the refit or
-// runs the full sort.
-hipError_t launch_refit_prepare(const float* const* csrc, int64_t n, int cur, int index_base, const BuildBufs& b,
-                                hipStream_t s) {
-    const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
-    // the tile sums in scan_tmp were zeroed by the bounds fold just before
-    hipLaunchKernelGGL(k_refit_keys, dim3(g), dim3(RF_THREADS), 0, s, csrc, n, b.prm, b.keys[cur], b.perm,
-                       index_base, b.keys[cur ^ 1], b.vals[0], (uint32_t*)b.vals[1],
-                       reinterpret_cast<uint64_t*>(b.scan_tmp));
-    return exclusive_scan_summed((const uint32_t*)b.vals[1], (uint32_t*)b.vals[1], n, b.scan_tmp, b.total, s);
+// (keys[cur] and perm hold the previous sort), the changed points compacted,
+// their count in *b.total and, in prm, whether the refit can run.
+RefitLook refit_look(uint64_t* base, int64_t n) {
+    const int64_t ntiles = (n + RF_TILE_POS - 1) / RF_TILE_POS, ngroups = (n + OCT_WAVE - 1) / OCT_WAVE;
+    RefitLook lk;
+    lk.tsum = base;
+    lk.wmask = base + ntiles;
+    lk.wpre = reinterpret_cast<uint32_t*>(base + ntiles + ngroups);
+    return lk;
+}
+int64_t refit_look_zero_words(int64_t n) { return (n + RF_TILE_POS - 1) / RF_TILE_POS; }
+int64_t refit_look_words(int64_t n) {
+    const int64_t ngroups = (n + OCT_WAVE - 1) / OCT_WAVE;
+    return refit_look_zero_words(n) + ngroups + (ngroups + 1) / 2;
+}
+
+hipError_t launch_refit_prepare(const float* const* csrc, int64_t n, int64_t limit, int key_bits, int cur,
+                                int index_base, const BuildBufs& b, const RefitBufs& r, hipStream_t s) {
+    if (limit > REFIT_MAX_CHANGED) return hipErrorInvalidValue;  // the changed-point buffers hold that many
+    const RefitLook lk = refit_look(b.rlook, n);
+    // the tile sums were zeroed by the bounds fold just before
+    hipLaunchKernelGGL(k_refit_keys, dim3((unsigned)((n + RF_THREADS - 1) / RF_THREADS)), dim3(RF_THREADS), 0, s,
+                       csrc, n, b.prm, b.keys[cur], b.perm, index_base, b.vals[0], lk);
+    hipLaunchKernelGGL(k_refit_compact, dim3((unsigned)((n + RF_TILE_POS - 1) / RF_TILE_POS)), dim3(OCT_WAVE), 0, s,
+                       csrc, n, b.prm, b.vals[0], lk, r.ck, r.ci, limit, key_bits, b.total);
+    return hipGetLastError();
 }
 
 // Part 2: the sorted keys end in keys[cur ^ 1].  Grids cover `limit` changed
 // points; the kernels read the real count (and refit_fail) on the device.
-hipError_t launch_refit_finish(const float* const* csrc, int64_t n, int64_t limit, int key_bits, int cur,
-                               int index_base, const BuildBufs& b, const RefitBufs& r, hipStream_t s) {
-    if (limit > REFIT_MAX_CHANGED) return hipErrorInvalidValue;  // the changed-point buffers hold that many
+hipError_t launch_refit_finish(const float* const* csrc, int64_t n, int64_t limit, int cur, int index_base,
+                               const BuildBufs& b, const RefitBufs& r, hipStream_t s) {
     const unsigned g = (unsigned)((n + RF_THREADS - 1) / RF_THREADS);
-    const uint32_t* cpos = (const uint32_t*)b.vals[1];
-    hipLaunchKernelGGL(k_refit_compact, dim3(g), dim3(RF_THREADS), 0, s, n, cpos, b.total, b.keys[cur ^ 1], b.vals[0],
-                       r.ck, r.ci, limit, key_bits, b.prm);
+    const RefitLook lk = refit_look(b.rlook, n);
     if (limit > 0) {
         const unsigned gc = (unsigned)((limit + RF_THREADS - 1) / RF_THREADS);
         hipLaunchKernelGGL(k_refit_rank, dim3(gc), dim3(RF_THREADS), 0, s, n, b.total, r.ck, r.ci, b.keys[cur],
-                           b.vals[0], cpos, r.csk, r.csi, r.cspos, b.prm);
+                           b.vals[0], lk, r.csk, r.csi, r.cspos, b.prm);
     }
     hipLaunchKernelGGL(k_refit_place, dim3(g), dim3(RF_THREADS), 0, s, csrc, n, b.total, b.keys[cur], b.vals[0],
-                       cpos, r.csk, r.csi, r.cspos, b.keys[cur ^ 1], b.perm, b.scoord, index_base, b.prm);
+                       lk, r.csk, r.csi, r.cspos, b.keys[cur ^ 1], b.perm, b.scoord, index_base, b.prm);
     return hipGetLastError();
 }
 

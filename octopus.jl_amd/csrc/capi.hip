@@ -82,7 +82,7 @@ struct SetState {
     DevBuf soff;                    // {0, n}: the one-set forest descriptor
     int64_t soff_n = -1;            // the n soff holds
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
-        nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax, rslots, ranc, nfc, noct;
+        nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax, rslots, ranc, nfc, noct, rlook;
     // the buffers oct_get_tree and oct_device_view hand out
     void export_to(std::vector<void*>* r) {
         DevBuf* out[] = {&perm, &nfirst, &nlast, &nchild, &bmin, &bmax, &nodes, &scoord, &pad,
@@ -117,6 +117,7 @@ struct SetState {
         b.ranc = ranc.as<int32_t>();
         b.nfc = nfc.as<int32_t>();
         b.noct = noct.as<uint8_t>();
+        b.rlook = rlook.as<uint64_t>();
         return b;
     }
     // The refit step (bounds, refit, topology: some 25 launches) as a HIP
@@ -141,7 +142,7 @@ struct SetState {
         drop_graphs();
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
                          &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
-                         &rslots, &ranc, &nfc, &noct, &rk, &ri, &rsk, &rsi, &rpos, &soff, &csrc, &pad, &qfirst, &qlast,
+                         &rslots, &ranc, &nfc, &noct, &rlook, &rk, &ri, &rsk, &rsi, &rpos, &soff, &csrc, &pad, &qfirst, &qlast,
                          &qkids, &qbmin, &qbmax, &p64, &partials64, &c32, &bmin64, &bmax64};
         for (DevBuf* d : all) d->release();
         soff_n = -1;
@@ -527,8 +528,8 @@ int ensure_build_bufs(oct_ctx* c, SetState& s, int64_t n, int64_t M) {
     HIP_TRY(c, s.vals0.ensure(4 * (size_t)n));
     HIP_TRY(c, s.vals1.ensure(4 * (size_t)n));
     HIP_TRY(c, s.hist.ensure(4 * (size_t)hist_words(n)));
-    // the scans: radix and topology histograms, and (refit) the n changed flags
-    HIP_TRY(c, s.scan_tmp.ensure(4 * (size_t)scan_tmp_words(std::max<int64_t>(n, hist_words(n)))));
+    // the scan of the radix histograms
+    HIP_TRY(c, s.scan_tmp.ensure(4 * (size_t)scan_tmp_words(hist_words(n))));
     HIP_TRY(c, s.total.ensure(16));  // uint64 total
     HIP_TRY(c, s.topo.ensure(4 * (size_t)n));
     HIP_TRY(c, s.perm.ensure(4 * (size_t)n));
@@ -547,6 +548,7 @@ int ensure_build_bufs(oct_ctx* c, SetState& s, int64_t n, int64_t M) {
     HIP_TRY(c, s.ranc.ensure(4 * (size_t)OCT_MAX_LEVEL * (size_t)n));
     HIP_TRY(c, s.nfc.ensure(4 * ((size_t)M + 1)));
     HIP_TRY(c, s.noct.ensure((size_t)M));
+    HIP_TRY(c, s.rlook.ensure(8 * (size_t)refit_look_words(n)));
     return OCT_OK;
 }
 
@@ -605,8 +607,8 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
         const float* const* csrc = s.csrc.as<const float*>();
         auto enqueue = [&]() -> int {
             Phase ph(c, OCT_PROF_BUILD);
-            HIP_TRY(c, launch_refit_prepare(csrc, n, s.cur, c->index_base, b, c->stream));
-            HIP_TRY(c, launch_refit_finish(csrc, n, limit, s.key_bits, s.cur, c->index_base, b, r, c->stream));
+            HIP_TRY(c, launch_refit_prepare(csrc, n, limit, s.key_bits, s.cur, c->index_base, b, r, c->stream));
+            HIP_TRY(c, launch_refit_finish(csrc, n, limit, s.cur, c->index_base, b, r, c->stream));
             HIP_TRY(c, launch_topology_stage(n, s.key_bits, T, c->index_base, s.cur ^ 1, b, f, c->stream));
             return OCT_OK;
         };
@@ -622,7 +624,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
             const DevBuf* bufs[] = {&s.prm, &s.partials, &s.keys0, &s.keys1, &s.vals0, &s.vals1, &s.hist, &s.scan_tmp,
                                     &s.total, &s.topo, &s.perm, &s.scoord, &s.nodes, &s.nfirst, &s.nlast, &s.nchild,
                                     &s.parent, &s.nlevel, &s.leaf_ids, &s.leaves, &s.bmin, &s.bmax, &s.rslots,
-                                    &s.ranc, &s.nfc, &s.noct, &s.rk, &s.ri,
+                                    &s.ranc, &s.nfc, &s.noct, &s.rlook, &s.rk, &s.ri,
                                     &s.rsk, &s.rsi, &s.rpos, &s.soff, &s.csrc};
             for (const DevBuf* d : bufs) mix((uint64_t)(uintptr_t)d->p);
             uint32_t rbits = 0;

@@ -167,6 +167,7 @@ struct BuildBufs {
     int32_t* ranc;       // refit: per level (OCT_MAX_LEVEL rows of n) each leaf's ancestor, -1 none
     int32_t* nfc;        // per node (and one past the last): the id its first child has or would have
     uint8_t* noct;       // per node: its octant in its parent
+    uint64_t* rlook;     // refit: tile sums, group masks and prefixes (refit_look_words(n))
 };
 
 int bounds_blocks(int64_t n);
@@ -198,6 +199,17 @@ hipError_t launch_single_set_offsets(int64_t* off, int64_t n, hipStream_t s);
 // perm and scoord left as they were, a valid tree of the previous order that
 // the host replaces by a full rebuild.  No host wait in between.
 constexpr int64_t REFIT_MAX_CHANGED = 32768;
+// The refit's changed-point bookkeeping in BuildBufs::rlook: the changed
+// count per tile of 4096 positions (zeroed by the bounds fold,
+// refit_look_zero_words), and per group of 64 positions the changed mask and
+// the changed count before the group.
+struct RefitLook {
+    uint64_t* tsum;
+    uint64_t* wmask;
+    uint32_t* wpre;
+};
+int64_t refit_look_words(int64_t n);
+int64_t refit_look_zero_words(int64_t n);
 struct RefitBufs {
     uint64_t* ck;    // changed keys (compacted)
     int32_t* ci;     // changed point indices
@@ -207,10 +219,10 @@ struct RefitBufs {
 };
 // The refit reads the coordinates through *csrc (device memory, written by
 // the bounds stage), so a captured refit follows the caller's array.
-hipError_t launch_refit_prepare(const float* const* csrc, int64_t n, int cur, int index_base, const BuildBufs& b,
-                                hipStream_t s);
-hipError_t launch_refit_finish(const float* const* csrc, int64_t n, int64_t limit, int key_bits, int cur,
-                               int index_base, const BuildBufs& b, const RefitBufs& r, hipStream_t s);
+hipError_t launch_refit_prepare(const float* const* csrc, int64_t n, int64_t limit, int key_bits, int cur,
+                                int index_base, const BuildBufs& b, const RefitBufs& r, hipStream_t s);
+hipError_t launch_refit_finish(const float* const* csrc, int64_t n, int64_t limit, int cur, int index_base,
+                               const BuildBufs& b, const RefitBufs& r, hipStream_t s);
 
 // Edge outputs (see edges.hip).
 struct EdgeOut {
@@ -409,16 +421,9 @@ hipError_t launch_pullback64(int64_t n_points, int64_t n_edges, const int32_t* s
                              double r, int index_base, double* grad, uint32_t* scratch, int dims, hipStream_t s);
 
 int64_t scan_tmp_words(int64_t n);
-// Tiles of the device scan (SCAN_TILE_INPUTS inputs each).
-constexpr int SCAN_TILE_INPUTS = 4096;
-int64_t scan_tiles(int64_t n);
 // Exclusive scan of n uint32 counts; the optional 64-bit total is exact even
 // when the uint32 offsets would wrap (callers check it first).
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
                           uint64_t* total_out, hipStream_t s);
-// The same scan when tmp already holds the 64-bit sums of the input's tiles
-// (its producer added them up): one launch instead of two.
-hipError_t exclusive_scan_summed(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
-                                 uint64_t* total_out, hipStream_t s);
 
 }  // namespace oct

@@ -147,19 +147,6 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles_carry(const uint32_
 }
 
 // words of uint32 scratch exclusive_scan needs for n inputs (64-bit tile sums)
-static_assert(SCAN_TILE == SCAN_TILE_INPUTS, "producers that add up tile sums use SCAN_TILE_INPUTS");
-int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
-
-hipError_t exclusive_scan_summed(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
-                                 uint64_t* total_out, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    const int64_t tiles = scan_tiles(n);
-    if (tiles > SCAN_CARRY_MAX) return exclusive_scan(in, out, n, tmp, total_out, s);  // recomputes the sums
-    hipLaunchKernelGGL(k_scan_tiles_carry, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, in, out, n,
-                       (const uint64_t*)tmp, total_out);
-    return hipGetLastError();
-}
-
 int64_t scan_tmp_words(int64_t n) { return 2 * ((n + SCAN_TILE - 1) / SCAN_TILE + 1); }
 
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp,
