@@ -78,7 +78,9 @@ class _DLManagedTensor(ctypes.Structure):
     pass
 
 
-_DL_DELETER = ctypes.CFUNCTYPE(None, ctypes.POINTER(_DLManagedTensor))
+# The deleter takes the struct's address (a pointer argument, so the C ABI
+# is dlpack.h's).
+_DL_DELETER = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
 _DLManagedTensor._fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", ctypes.c_void_p),
                              ("deleter", _DL_DELETER)]
 _KDL_ROCM = 10
@@ -86,9 +88,22 @@ _DL_TYPES = {"<i4": (0, 32), "<i8": (0, 64), "<f4": (2, 32), "<f8": (2, 64)}  # 
 _dl_live = {}  # address of a handed-out DLManagedTensor -> (struct, shape, owner) until torch deletes it
 
 
-@_DL_DELETER
-def _dl_delete(p):
-    _dl_live.pop(ctypes.addressof(p.contents), None)
+def _make_dl_deleter(live):
+    @_DL_DELETER
+    def delete(addr):
+        live.pop(addr, None)
+
+    return delete
+
+
+_dl_delete = _make_dl_deleter(_dl_live)
+# A tensor can outlive this module: at interpreter exit a caller's global is
+# freed after the module's dict has been cleared.  torch then reads the
+# struct (kept by _dl_live) and calls the deleter, so neither may be freed
+# before the process ends: both get a reference nothing releases.  (The
+# deleter reaches the dict through its closure, not a module global.)
+ctypes.pythonapi.Py_IncRef(ctypes.py_object(_dl_live))
+ctypes.pythonapi.Py_IncRef(ctypes.py_object(_dl_delete))
 
 
 _PyCapsule_New = ctypes.pythonapi.PyCapsule_New

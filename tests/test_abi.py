@@ -123,3 +123,24 @@ def test_device_views_hold_their_owner():
     dm.deleter(ctypes.pointer(dm))  # what torch calls when the tensor dies
     gc.collect()
     assert ref() is None
+
+
+def test_views_held_at_interpreter_exit():
+    """Result tensors a script still holds when the interpreter exits are
+    freed after the module's own dict is cleared: their deleter and the
+    structs torch reads must still exist then (the process used to end with
+    a segmentation fault).  Host memory stands in for the device buffers
+    (kDLCPU), so this runs without a GPU."""
+    import subprocess
+    import sys
+
+    code = (
+        "import sys; sys.path[:0] = [{pkg!r}]\n"
+        "import numpy as np, torch, octohip\n"
+        "octohip._KDL_ROCM = 1  # kDLCPU: host memory in place of device buffers\n"
+        "buf = np.arange(64, dtype=np.float32)\n"
+        "held = [torch.from_dlpack(octohip._dl_capsule(buf.ctypes.data, (64,), '<f4', 0, buf)) for _ in range(4)]\n"
+        "assert float(held[3][5]) == 5.0\n"
+    ).format(pkg=os.path.join(ROOT, "octopus.jl_amd"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
