@@ -143,6 +143,26 @@ def _wrap(ptr, shape, dtype, device, owner):
     return torch.from_dlpack(_dl_capsule(ptr, shape, typestr, index, owner))
 
 
+class _Core:
+    """Owns the C context.  Library memory views hold this object rather
+    than their TNS, so the context (and the memory) lives until the TNS and
+    every such view are gone, and a TNS may cache views without a cycle."""
+
+    __slots__ = ("lib", "h")
+
+    def __init__(self, lib, h):
+        self.lib = lib
+        self.h = h
+
+    def __del__(self):
+        if self.h is not None:
+            try:
+                self.lib.oct_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+
 class TNS:
     """TNS(T = Float32; ndims = 3) (src/types.jl:167-183) on one HIP device.
 
@@ -169,14 +189,19 @@ class TNS:
         self.point_sets = []        # coords tensors, kept alive (referenced, not copied)
         self.active_pairs = []
         self.dirty = []
-        self._ctx = None
+        self._core = None
+        self._views = {}  # (pair, field) -> (address, view of the buffer): build_edges slices these
         self._stream = None
         self._private_stream = bool(private_stream)
         self._lib = L.load()
 
     # ------------------------------------------------------------ context
+    @property
+    def _ctx(self):
+        return self._core.h if self._core is not None else None
+
     def _c(self):
-        if self._ctx is None:
+        if self._core is None:
             torch = _torch()
             if not torch.cuda.is_available():
                 raise OctopusError("no HIP device: the octohip engine runs on an MI355X only")
@@ -185,7 +210,7 @@ class TNS:
                                             8 if self.dtype == torch.float64 else 4)
             if rc != L.OCT_OK:
                 raise OctopusError(f"oct_create failed ({rc})")
-            self._ctx = h
+            self._core = _Core(self._lib, h)
             # By default the library runs on the caller's current stream, like
             # any torch op (set again on every call: free while it is the same
             # stream).  private_stream: its own stream (default priority),
@@ -246,13 +271,20 @@ class TNS:
             raise OctopusError(msg)
         raise OctopusError(f"octohip error {rc}: {msg}")
 
-    def __del__(self):
-        if getattr(self, "_ctx", None) is not None:
-            try:
-                self._lib.oct_destroy(self._ctx)
-            except Exception:
-                pass
-            self._ctx = None
+    def _view(self, key, ptr, n, tail, dtype):
+        """The first n rows of the library buffer at address ptr, as a slice
+        of a view kept per key: wrapping a buffer (a DLPack capsule, then
+        torch.from_dlpack) costs several microseconds per tensor, a slice
+        about one.  A buffer keeps its address while it does not grow (a
+        grown one gets a new address), and n never exceeds what the buffer
+        at ptr holds now, so the slice lies within it."""
+        if n == 0 or not ptr:
+            return _torch().empty((n,) + tail, dtype=dtype, device=self.device)
+        ent = self._views.get(key)
+        if ent is None or ent[0] != ptr or ent[1].shape[0] < n:
+            ent = (ptr, _wrap(ptr, (n,) + tail, dtype, self.device, self._core))
+            self._views[key] = ent
+        return ent[1][:n]
 
     # method spellings of the free functions
     def set_search_radius(self, r):
@@ -429,12 +461,12 @@ def build_edges(tns: TNS, q: int, t: int, mode: str = "exact", order: str = "lea
     e = L.OctEdges()
     tns._check(tns._lib.oct_build_edges(tns._enter(), int(q), int(t), _MODES[mode], _ORDERS[order], ctypes.byref(e)))
     tns._leave()
-    n, dev = int(e.n_edges), tns.device
+    n = int(e.n_edges)
     return Edges(
-        senders=_wrap(e.senders, (n,), torch.int32, dev, tns),
-        receivers=_wrap(e.receivers, (n,), torch.int32, dev, tns),
-        rel_displacement=_wrap(e.rel_displacement, (n, tns.ndims), tns.dtype, dev, tns),
-        rel_dist_norm=_wrap(e.rel_dist_norm, (n, 1), tns.dtype, dev, tns),
+        senders=tns._view((q, t, 0), e.senders, n, (), torch.int32),
+        receivers=tns._view((q, t, 1), e.receivers, n, (), torch.int32),
+        rel_displacement=tns._view((q, t, 2), e.rel_displacement, n, (tns.ndims,), tns.dtype),
+        rel_dist_norm=tns._view((q, t, 3), e.rel_dist_norm, n, (1,), tns.dtype),
     )
 
 
