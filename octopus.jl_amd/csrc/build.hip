@@ -447,12 +447,19 @@ __device__ inline void refit_write(int64_t pos, uint64_t key, int32_t idx, const
 }
 
 // K5: unchanged points to their new positions; K6 (c < |C|): changed points.
+// An unchanged point moves by the changed points sorted below it, found by
+// binary search of C sorted; while C has at most RF_PLACE_LDS points (a time
+// step changes few) each block stages it in LDS first, so the search's ten
+// or so dependent steps are LDS reads rather than global loads.
+constexpr int RF_PLACE_LDS = 1024;
 __global__ void __launch_bounds__(RF_THREADS)
 k_refit_place(const float* const* __restrict__ csrc, int64_t n, const uint64_t* __restrict__ total,
               const uint64_t* __restrict__ pk, const int32_t* __restrict__ pi, RefitLook lk,
               const uint64_t* __restrict__ csk, const int32_t* __restrict__ csi, const uint32_t* __restrict__ cspos,
               uint64_t* __restrict__ kout, int32_t* __restrict__ perm, float4* __restrict__ sc, int index_base,
               const BuildParams* __restrict__ prm) {
+    __shared__ uint64_t s_ck[RF_PLACE_LDS];
+    __shared__ int32_t s_ci[RF_PLACE_LDS];
     const float* __restrict__ coords = *csrc;  // this step's coordinates
     const int64_t nc = (int64_t)*total;
     const int64_t i = (int64_t)blockIdx.x * RF_THREADS + threadIdx.x;
@@ -460,19 +467,40 @@ k_refit_place(const float* const* __restrict__ csrc, int64_t n, const uint64_t* 
         if (i < n) kout[i] = pk[i];
         return;
     }
+    // loads that do not depend on C first
+    uint64_t wm = 0, key = 0;
+    int32_t idx = 0;
+    if (i < n) {
+        wm = lk.wmask[i / OCT_WAVE];
+        key = pk[i];
+        idx = pi[i];
+    }
+    const bool staged = nc <= RF_PLACE_LDS;  // block-uniform
+    if (staged) {
+        for (int j = threadIdx.x; j < (int)nc; j += RF_THREADS) {
+            s_ck[j] = csk[j];
+            s_ci[j] = csi[j];
+        }
+        __syncthreads();
+    }
     if (i < nc) refit_write(cspos[i], csk[i], csi[i], coords, kout, perm, sc, index_base);
     if (i >= n) return;
-    const uint64_t wm = lk.wmask[i / OCT_WAVE];
     const int l = (int)(i % OCT_WAVE);
     if ((wm >> l) & 1ull) return;  // changed: placed above
     const uint32_t c = lk.wpre[i / OCT_WAVE] + (uint32_t)__popcll(wm & ((1ull << l) - 1ull));
-    const uint64_t key = pk[i];
-    const int32_t idx = pi[i];
     int64_t lo = 0, hi = nc;  // changed points below (key, idx)
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (key_idx_less(csk[mid], csi[mid], key, idx)) lo = mid + 1;
-        else hi = mid;
+    if (staged) {
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (key_idx_less(s_ck[mid], s_ci[mid], key, idx)) lo = mid + 1;
+            else hi = mid;
+        }
+    } else {
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (key_idx_less(csk[mid], csi[mid], key, idx)) lo = mid + 1;
+            else hi = mid;
+        }
     }
     refit_write((int64_t)(i - c) + lo, key, idx, coords, kout, perm, sc, index_base);
 }
@@ -648,12 +676,10 @@ __device__ inline int64_t lower_bound_pred(int64_t lo, int64_t hi, P pred) {
 
 // Emits every node starting at each sorted position, with no search: the
 // BFS id of the node at (level L, position k) is the scanned (L, block)
-// offset plus its rank among the block's level-L starts; its parent is either
-// the node one level up at the same position or, for the first level, the
-// last node of the level above starting before k (the same ranks, minus one).
+// offset plus its rank among the block's level-L starts.
 // The block lists its nodes in LDS, level-major and in position order (in
 // rounds of EMIT_LIST entries), and one thread per node writes it, so
-// consecutive threads write consecutive ids.  Per node: node_first, level, parent, the packed
+// consecutive threads write consecutive ids.  Per node: node_first, level, the packed
 // record, its octant in its parent and the id its first child would have (a
 // node's children are the ids from its own to the next node's, so
 // k_link_refit writes the child slots and counts from these two arrays);
@@ -667,7 +693,7 @@ k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
             const uint32_t* __restrict__ topo, const uint32_t* __restrict__ hoff, int nblk,
             BuildParams* __restrict__ prm, Forest forest, int index_base, const float4* __restrict__ scoord, NodeRec* __restrict__ nodes, int32_t* __restrict__ nfirst,
             int32_t* __restrict__ nlast, float* __restrict__ bmin, float* __restrict__ bmax,
-            int32_t* __restrict__ parent, uint8_t* __restrict__ nlevel, int32_t* __restrict__ nfc,
+            uint8_t* __restrict__ nlevel, int32_t* __restrict__ nfc,
             uint8_t* __restrict__ noct, int32_t* __restrict__ leaf_ids, NodeRec* __restrict__ leaves,
             int32_t* __restrict__ ranc, int64_t astride) {
     constexpr int NW = TOPO_THREADS / OCT_WAVE;
@@ -779,7 +805,6 @@ k_topo_emit(const uint64_t* __restrict__ sk, int64_t n, int T,
             const bool chain = k == s_lo && L <= L0;
             nlevel[id] = (uint8_t)L;
             nfirst[id] = (int32_t)k + index_base;
-            parent[id] = L == 0 ? -1 : (L > Lfirst ? before(L - 1, p) : before(L - 1, p) - 1);
             noct[id] = L == 0 ? (uint8_t)0 : (uint8_t)((key_morton(key, forest.batched) >> level_shift(L - 1)) & 7ull);
             const int32_t fc = L < OCT_MAX_LEVEL ? before(L + 1, p) : M;
             nfc[id] = fc;
@@ -1231,7 +1256,7 @@ hipError_t launch_topology_stage(int64_t n, int key_bits, int T, int index_base,
     hipLaunchKernelGGL(k_topo_rows, dim3(OCT_NLEVELS + 1), dim3(TR_THREADS), 0, s, b.hist, nblk);
     hipLaunchKernelGGL(k_topo_emit, dim3(nblk), dim3(TOPO_THREADS), 0, s, sk, n, T, b.topo, b.hist,
                        nblk, b.prm, f, index_base, b.scoord, b.nodes, b.nfirst, b.nlast, b.bmin, b.bmax,
-                       b.parent, b.nlevel, b.nfc, b.noct, b.leaf_ids, b.leaves, b.ranc, n);
+                       b.nlevel, b.nfc, b.noct, b.leaf_ids, b.leaves, b.ranc, n);
     // child links over the nodes, and the refit over (block of leaves, level)
     // pairs (at most max_nodes nodes and n leaves; both loops stride over the
     // real counts)

@@ -82,7 +82,7 @@ struct SetState {
     DevBuf soff;                    // {0, n}: the one-set forest descriptor
     int64_t soff_n = -1;            // the n soff holds
     DevBuf prm, partials, keys0, keys1, vals0, vals1, hist, scan_tmp, total, topo, perm, scoord, nodes,
-        nfirst, nlast, nchild, parent, nlevel, leaf_ids, leaves, bmin, bmax, rslots, ranc, nfc, noct, rlook;
+        nfirst, nlast, nchild, nlevel, leaf_ids, leaves, bmin, bmax, rslots, ranc, nfc, noct, rlook;
     // the buffers oct_get_tree and oct_device_view hand out
     void export_to(std::vector<void*>* r) {
         DevBuf* out[] = {&perm, &nfirst, &nlast, &nchild, &bmin, &bmax, &nodes, &scoord, &pad,
@@ -107,7 +107,6 @@ struct SetState {
         b.nfirst = nfirst.as<int32_t>();
         b.nlast = nlast.as<int32_t>();
         b.nchild = nchild.as<int32_t>();
-        b.parent = parent.as<int32_t>();
         b.nlevel = nlevel.as<uint8_t>();
         b.leaf_ids = leaf_ids.as<int32_t>();
         b.leaves = leaves.as<NodeRec>();
@@ -141,7 +140,7 @@ struct SetState {
     void release_all() {
         drop_graphs();
         DevBuf* all[] = {&prm, &partials, &keys0, &keys1, &vals0, &vals1, &hist, &scan_tmp, &total, &topo, &perm,
-                         &scoord, &nodes, &nfirst, &nlast, &nchild, &parent, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
+                         &scoord, &nodes, &nfirst, &nlast, &nchild, &nlevel, &leaf_ids, &leaves, &bmin, &bmax,
                          &rslots, &ranc, &nfc, &noct, &rlook, &rk, &ri, &rsk, &rsi, &rpos, &soff, &csrc, &pad, &qfirst, &qlast,
                          &qkids, &qbmin, &qbmax, &p64, &partials64, &c32, &bmin64, &bmax64};
         for (DevBuf* d : all) d->release();
@@ -538,7 +537,6 @@ int ensure_build_bufs(oct_ctx* c, SetState& s, int64_t n, int64_t M) {
     HIP_TRY(c, s.nfirst.ensure(4 * (size_t)M));
     HIP_TRY(c, s.nlast.ensure(4 * (size_t)M));
     HIP_TRY(c, s.nchild.ensure(32 * (size_t)M));
-    HIP_TRY(c, s.parent.ensure(4 * (size_t)M));
     HIP_TRY(c, s.nlevel.ensure((size_t)M));
     HIP_TRY(c, s.leaf_ids.ensure(4 * (size_t)n));
     HIP_TRY(c, s.leaves.ensure(sizeof(NodeRec) * (size_t)n));
@@ -623,7 +621,7 @@ int build_set(oct_ctx* c, SetState& s, bool force_full) {
             auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
             const DevBuf* bufs[] = {&s.prm, &s.partials, &s.keys0, &s.keys1, &s.vals0, &s.vals1, &s.hist, &s.scan_tmp,
                                     &s.total, &s.topo, &s.perm, &s.scoord, &s.nodes, &s.nfirst, &s.nlast, &s.nchild,
-                                    &s.parent, &s.nlevel, &s.leaf_ids, &s.leaves, &s.bmin, &s.bmax, &s.rslots,
+                                    &s.nlevel, &s.leaf_ids, &s.leaves, &s.bmin, &s.bmax, &s.rslots,
                                     &s.ranc, &s.nfc, &s.noct, &s.rlook, &s.rk, &s.ri,
                                     &s.rsk, &s.rsi, &s.rpos, &s.soff, &s.csrc};
             for (const DevBuf* d : bufs) mix((uint64_t)(uintptr_t)d->p);
@@ -1903,7 +1901,6 @@ int batch_ensure(oct_ctx* c, BatchState& B) {
     HIP_TRY(c, s.nfirst.ensure(4 * (size_t)M));
     HIP_TRY(c, s.nlast.ensure(4 * (size_t)M));
     HIP_TRY(c, s.nchild.ensure(32 * (size_t)M));
-    HIP_TRY(c, s.parent.ensure(4 * (size_t)M));
     HIP_TRY(c, s.nlevel.ensure((size_t)M));
     HIP_TRY(c, s.leaf_ids.ensure(4 * (size_t)N));
     HIP_TRY(c, s.leaves.ensure(sizeof(NodeRec) * (size_t)N));

@@ -159,7 +159,7 @@ struct BuildBufs {
     int32_t* perm;       // exported, index base applied
     float4* scoord;      // Morton-sorted coords, w = original index bits
     NodeRec* nodes;
-    int32_t *nfirst, *nlast, *nchild, *parent, *leaf_ids;
+    int32_t *nfirst, *nlast, *nchild, *leaf_ids;
     NodeRec* leaves;     // leaf records in Morton order
     uint8_t* nlevel;
     float *bmin, *bmax;
