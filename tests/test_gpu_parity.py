@@ -630,7 +630,8 @@ def test_single_pass_returns_on_its_total(orc):
 def test_unaligned_coordinates(orc):
     """Coordinates that start 4 bytes past a 16-byte boundary (a view into a
     larger buffer) take the bounds kernel's 4-byte load path; tree and edges
-    must be the oracle's, in refit steps too (the copy the step graph reads)."""
+    must be the oracle's, in refit steps too (the step graph reads the caller's
+    array through the address the bounds fold leaves on the device)."""
     import octohip
 
     torch = _torch()
