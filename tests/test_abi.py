@@ -144,3 +144,29 @@ def test_views_held_at_interpreter_exit():
     ).format(pkg=os.path.join(ROOT, "octopus.jl_amd"))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
+
+
+def test_result_views_are_slices_of_kept_views(monkeypatch):
+    """build_edges returns slices of a view kept per (pair, field): the same
+    address and a length within the kept view give a slice of it; a longer
+    length or a new address (a grown buffer) give a new view.  Host memory
+    stands in for the device buffers (kDLCPU)."""
+    import torch
+
+    monkeypatch.setattr(octohip, "_KDL_ROCM", 1)
+    tns = octohip.TNS()
+    tns.device = torch.device("cpu", 0)
+    a = np.arange(400, dtype=np.float32)
+    b = np.arange(400, dtype=np.float32) + 1000
+    pa, pb = a.ctypes.data, b.ctypes.data
+    v1 = tns._view((0, 0, 2), pa, 100, (3,), torch.float32)
+    assert v1.shape == (100, 3) and v1.data_ptr() == pa and float(v1[1, 0]) == 3.0
+    base = tns._views[(0, 0, 2)][1]
+    v2 = tns._view((0, 0, 2), pa, 60, (3,), torch.float32)
+    assert v2.shape == (60, 3) and tns._views[(0, 0, 2)][1] is base  # a slice of the kept view
+    v3 = tns._view((0, 0, 2), pa, 130, (3,), torch.float32)
+    assert v3.shape == (130, 3) and tns._views[(0, 0, 2)][1] is not base  # longer: wrapped again
+    v4 = tns._view((0, 0, 2), pb, 10, (3,), torch.float32)
+    assert v4.data_ptr() == pb and float(v4[0, 0]) == 1000.0  # a new address: wrapped again
+    assert tns._view((0, 0, 0), pa, 0, (), torch.int32).shape == (0,)
+    assert float(v1[2, 2]) == 8.0  # earlier results still view the first buffer
