@@ -25,7 +25,6 @@ constexpr int LL_SAFE_STACK = 160;  // entries of the safe walk (scratch)
 constexpr int LL_FAST_STACK = 32;   // stack entries per query leaf of the cooperative walk (LDS, 4 bytes each)
 constexpr int LL_THREADS = 256;
 constexpr int EW_WAVES = 4;     // waves per block, each owns one query leaf at a time
-constexpr int EW_STAGE = 256;   // edges one wave stages in LDS (6 KB)
 constexpr int EW_TILE = 4;      // candidate chunks of 64 held in registers per tile
 // A query leaf the single pass records in one sweep has at most 64 queries,
 // 64 candidate leaves and REC_MAX_CAND candidate points, of which at most
@@ -325,13 +324,46 @@ k_leaf_lists_coop(const NodeRec* __restrict__ qleaves, const BuildParams* __rest
 struct WaveLds {
     int32_t* start;  // 65: prefix of candidate-leaf sizes
     int32_t* first;  // 64: first sorted position of each candidate leaf
-    int32_t* sen;    // `stage` staged senders
-    int32_t* rcv;
-    float* disp;     // 3 * stage
-    float* dist;
-    uint32_t stage;  // edges the stage holds
     int32_t* cand;   // EW_TILE * 64: sorted positions of the candidates that pass the query-box filter
+    float4* tile;    // FILL: EW_TILE * 64, the candidate tile being swept
+    int2* rec;       // FILL: the tile's edges as (output position, query slot << 8 | tile slot)
+    uint32_t nrec;   // FILL: records rec holds
 };
+
+// Writes the n edges recorded in L.rec (output position, query slot << 8 |
+// tile slot) against the tile in L.tile: one lane per edge, the query's
+// coordinates taken from the lane that holds it (myq of lane `slot`) with a
+// permute, the target's from LDS.  The FP32 operations are the sweep's, in
+// the same order, and the square root is correctly rounded (as sqrtf), so
+// the edges are the ones a per-hit store from the sweep would write.
+__device__ inline void write_tile_edges(const WaveLds& L, uint32_t n, float4 myq, float inv_r, EdgeOut out,
+                                        int index_base, int lane) {
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t t0 = 0; t0 < n; t0 += OCT_WAVE) {  // wave-uniform
+        const uint32_t t = t0 + (uint32_t)lane;
+        const bool valid = t < n;
+        int2 h = make_int2(0, 0);
+        if (valid) h = L.rec[t];
+        const int qs = (int)((uint32_t)h.y >> 8) << 2;  // the query's lane, as a byte address
+        const float4 p = L.tile[h.y & 255];
+        const float qx = __int_as_float(__builtin_amdgcn_ds_bpermute(qs, __float_as_int(myq.x)));
+        const float qy = __int_as_float(__builtin_amdgcn_ds_bpermute(qs, __float_as_int(myq.y)));
+        const float qz = __int_as_float(__builtin_amdgcn_ds_bpermute(qs, __float_as_int(myq.z)));
+        const int32_t qi = __builtin_amdgcn_ds_bpermute(qs, __float_as_int(myq.w));
+        const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        const float sd = ballot(valid && d2 < SQRT_DIRECT_MIN) ? sqrtf(d2) : sqrt_rn_direct(d2);
+        if (valid) {
+            const int64_t e = (int64_t)(uint32_t)h.x;
+            out.senders[e] = __float_as_int(p.w) + index_base;
+            out.receivers[e] = qi + index_base;
+            *reinterpret_cast<float3*>(out.rdisp + 3 * e) =
+                make_float3(rel_disp(qx, p.x, inv_r), rel_disp(qy, p.y, inv_r), rel_disp(qz, p.z, inv_r));
+            out.rdist[e] = sd * inv_r;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
 
 // Loads candidate points t0 .. t0 + 255 of the sequence that L.start /
 // L.first describe into tp / tpos (w = point index; -1 marks an empty slot).
@@ -388,9 +420,9 @@ __device__ inline int load_tile(int t0, int total, const float4* __restrict__ ts
 // broadcast with readlane, query-outer, so each (query, 64 candidates) step
 // is one ballot.
 //   COUNT: writes the per-query counts to qcount[slot]; returns the leaf total.
-//   FILL:  reads qcount/qoff[slot] and writes the edges; a query block whose
-//          edges fit the LDS stage is staged and written with contiguous
-//          whole-wave stores.
+//   FILL:  reads qoff[slot] and writes the edges: each tile's hits are
+//          recorded in LDS (output position, query slot, tile slot) and
+//          written by write_tile_edges, one lane per edge.
 // slot = sorted position (leaf order) or original index (receiver order).
 template <int MODE>
 __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 qhi, int32_t ncand, int2 lf0,
@@ -408,21 +440,18 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
         if (lane < nqb) myq = qsc[qf + qb + lane];
         const int32_t slot = by_receiver ? __float_as_int(myq.w) : qf + qb + lane;
         uint32_t my = 0;  // COUNT: count; FILL: next output position of this lane's query
-        uint32_t blk_base = 0, blk_tot = 0;
-        bool staged = false;
-        if (MODE != EDGE_COUNT) {
-            uint32_t c = 0;
-            if (lane < nqb) {
-                my = qoff[slot];
-                c = qcount[slot];
-            }
-            // leaf order: this block of queries owns [blk_base, blk_base + blk_tot)
-            if (MODE == EDGE_FILL && !by_receiver) {
-                blk_base = (uint32_t)__builtin_amdgcn_readfirstlane((int)my);
-                blk_tot = (uint32_t)wave_sum((int)c);
-                staged = blk_tot <= L.stage;
-            }
-        }
+        if (MODE != EDGE_COUNT && lane < nqb) my = qoff[slot];
+        // FILL: the sweep records each hit as (output position, query slot,
+        // tile slot) and write_tile_edges computes and stores the edges with
+        // one lane per edge, from the tile in LDS and the queries in
+        // registers: the features are computed by whole waves, not by the
+        // few lanes of a 64-candidate step that hit, and the stores of one
+        // instruction are consecutive edges.
+        uint32_t nrec = 0;
+        auto flush = [&]() {
+            write_tile_edges(L, nrec, myq, inv_r, out, index_base, lane);
+            nrec = 0;
+        };
         for (int g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
             int2 lf = make_int2(0, 0);
             if (g0 + lane < ncand) lf = (g0 == 0) ? lf0 : lst[g0 + lane];
@@ -437,6 +466,11 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
                 int32_t tpos[EW_TILE];
                 const int got = load_tile(t0, total, tsc, L, lane, filter, qlo, qhi, r_sq, tp, tpos);
                 const int nu = (got + OCT_WAVE - 1) / OCT_WAVE;
+                if (MODE == EDGE_FILL) {
+#pragma unroll
+                    for (int u = 0; u < EW_TILE; ++u) L.tile[u * OCT_WAVE + lane] = tp[u];
+                    __builtin_amdgcn_wave_barrier();
+                }
                 for (int qi = 0; qi < nqb; ++qi) {
                     const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), qi));
                     const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), qi));
@@ -448,6 +482,7 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
 #pragma unroll
                     for (int u = 0; u < EW_TILE; ++u) {
                         if (u >= nu) break;
+                        if (MODE == EDGE_FILL && nrec + (uint32_t)OCT_WAVE > L.nrec) flush();  // wave-uniform
                         const int32_t j = __float_as_int(tp[u].w);
                         const float dx = qx - tp[u].x;
                         const float dy = qy - tp[u].y;
@@ -457,45 +492,21 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
                         const bool other = j != ix;
                         const uint64_t m = ballot(in_r) & ballot(other);  // compare masks ANDed, no branch
                         if (MODE == EDGE_INDEX && (in_r & other)) out.senders[run + (uint32_t)mbcnt(m)] = j + index_base;
-                        if (MODE == EDGE_FILL && (in_r & other)) {
-                            const uint32_t e = run + (uint32_t)mbcnt(m);
-                            if (staged) {
-                                const uint32_t s = e - blk_base;
-                                L.sen[s] = j + index_base;
-                                L.rcv[s] = i + index_base;
-                                L.disp[3 * s + 0] = rel_disp(qx, tp[u].x, inv_r);
-                                L.disp[3 * s + 1] = rel_disp(qy, tp[u].y, inv_r);
-                                L.disp[3 * s + 2] = rel_disp(qz, tp[u].z, inv_r);
-                                L.dist[s] = sqrtf(d2) * inv_r;
-                            } else {
-                                out.senders[e] = j + index_base;
-                                out.receivers[e] = i + index_base;
-                                out.rdisp[3 * (int64_t)e + 0] = rel_disp(qx, tp[u].x, inv_r);
-                                out.rdisp[3 * (int64_t)e + 1] = rel_disp(qy, tp[u].y, inv_r);
-                                out.rdisp[3 * (int64_t)e + 2] = rel_disp(qz, tp[u].z, inv_r);
-                                out.rdist[e] = sqrtf(d2) * inv_r;
-                            }
-                        }
+                        if (MODE == EDGE_FILL && (in_r & other))
+                            L.rec[nrec + (uint32_t)mbcnt(m)] =
+                                make_int2((int)(run + (uint32_t)mbcnt(m)), (qi << 8) | (u * OCT_WAVE + lane));
                         run += (uint32_t)__popcll(m);
+                        if (MODE == EDGE_FILL) nrec += (uint32_t)__popcll(m);
                     }
                     if (lane == qi) my += run - run0;
                 }
+                if (MODE == EDGE_FILL && nrec) flush();  // before the tile is replaced
             }
             __builtin_amdgcn_wave_barrier();
         }
         if (MODE == EDGE_COUNT) {
             if (lane < nqb) qcount[slot] = my;
             leaf_total += (uint32_t)wave_sum(lane < nqb ? (int)my : 0);
-        }
-        if (MODE == EDGE_FILL && staged) {
-            __builtin_amdgcn_wave_barrier();
-            for (uint32_t t = lane; t < blk_tot; t += OCT_WAVE) {
-                out.senders[blk_base + t] = L.sen[t];
-                out.receivers[blk_base + t] = L.rcv[t];
-                out.rdist[blk_base + t] = L.dist[t];
-            }
-            for (uint32_t t = lane; t < 3 * blk_tot; t += OCT_WAVE) out.rdisp[3 * (int64_t)blk_base + t] = L.disp[t];
-            __builtin_amdgcn_wave_barrier();
         }
     }
     return leaf_total;
@@ -774,17 +785,13 @@ __device__ inline void emit_hits(const uint32_t* hits, uint32_t n, uint32_t base
 
 template <int MODE>
 struct EdgeLds {
-    static constexpr int STAGE = MODE == EDGE_FILL ? EW_STAGE : 1;
+    static constexpr int FT = MODE == EDGE_FILL ? EW_TILE * OCT_WAVE : 1;  // tile slots and records
     int32_t start[EW_WAVES][OCT_WAVE + 1];
     int32_t first[EW_WAVES][OCT_WAVE];
-    int32_t sen[EW_WAVES][STAGE];
-    int32_t rcv[EW_WAVES][STAGE];
-    float disp[EW_WAVES][3 * STAGE];
-    float dist[EW_WAVES][STAGE];
     int32_t cand[EW_WAVES][EW_TILE * OCT_WAVE];
-    __device__ WaveLds wave(int w) {
-        return WaveLds{start[w], first[w], sen[w], rcv[w], disp[w], dist[w], (uint32_t)STAGE, cand[w]};
-    }
+    float4 tile[EW_WAVES][FT];
+    int2 rec[EW_WAVES][FT];
+    __device__ WaveLds wave(int w) { return WaveLds{start[w], first[w], cand[w], tile[w], rec[w], (uint32_t)FT}; }
 };
 
 // Two-pass driver: one wave per query leaf over a contiguous, XCD-local range
@@ -943,7 +950,7 @@ constexpr uint32_t SP_SPIN_LIMIT = 1u << 22;   // bounded wait: a stuck look-bac
 
 constexpr int SP_HIT_WORDS = 1536;                 // hit list of one wave: 6 KB
 static_assert(SP_SIG_MARK % 2 == 0, "the marker is a 64-bit word");
-constexpr uint32_t SP_STAGE = SP_HIT_WORDS * 4 / 24;  // FILL stage laid over the hit list: 170 edges
+static_assert(SP_HIT_WORDS * 4 >= EW_TILE * OCT_WAVE * (16 + 8), "phase B's FILL tile and records fit the hit list");
 static_assert(SP_PLAN_BUDGET == SP_HIT_WORDS, "the plan packs one wave's compact hit list");
 static_assert(SP_STAGE_PTS * 4 >= EW_TILE * OCT_WAVE, "the stage holds the slot table and the filtered positions");
 
@@ -1081,7 +1088,7 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
     constexpr uint32_t HITS = WIDE ? SP_HIT_WORDS / 2 : SP_HIT_WORDS;
     __shared__ int32_t s_start[EW_WAVES][OCT_WAVE + 1];
     __shared__ int32_t s_first[EW_WAVES][OCT_WAVE];
-    __shared__ uint32_t s_hits[EW_WAVES][SP_HIT_WORDS];
+    __shared__ alignas(16) uint32_t s_hits[EW_WAVES][SP_HIT_WORDS];
     __shared__ uint32_t s_cnt[EW_WAVES][SP_MAX_LEAVES];
     __shared__ int32_t s_qf[EW_WAVES][SP_MAX_LEAVES];
     __shared__ float4 s_stage[EW_WAVES][SP_STAGE_PTS];  // record tiles; positions of the filtered candidates
@@ -1092,10 +1099,9 @@ k_edges_single(const NodeRec* __restrict__ qleaves, const float4* __restrict__ q
     uint32_t* hits = s_hits[wv];
     uint32_t* cntw = s_cnt[wv];
     int32_t* qfw = s_qf[wv];
-    int32_t* stage = reinterpret_cast<int32_t*>(hits);
-    const WaveLds L{s_start[wv], s_first[wv], stage, stage + SP_STAGE, reinterpret_cast<float*>(stage + 2 * SP_STAGE),
-                    reinterpret_cast<float*>(stage + 5 * SP_STAGE), SP_STAGE,
-                    reinterpret_cast<int32_t*>(s_stage[wv])};
+    // phase B's FILL sweep lays its tile (4 KB) and records (2 KB) over the hit list
+    const WaveLds L{s_start[wv], s_first[wv], reinterpret_cast<int32_t*>(s_stage[wv]), reinterpret_cast<float4*>(hits),
+                    reinterpret_cast<int2*>(hits + 4 * EW_TILE * OCT_WAVE), (uint32_t)(EW_TILE * OCT_WAVE)};
     const int32_t nleaves = qprm->n_leaves;
     const int32_t nruns = wave_first ? *plan_waves : (nleaves + leaves_per_run - 1) / leaves_per_run;
     if (threadIdx.x == 0) s_q = (SP_Q_NONE << SP_Q_SHIFT) | (unsigned long long)run_chunk;
