@@ -490,7 +490,9 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
                         const float d2 = dx * dx + dy * dy + dz * dz;
                         const bool in_r = d2 <= r_sq;
                         const bool other = j != ix;
-                        const uint64_t m = ballot(in_r) & ballot(other);  // compare masks ANDed, no branch
+                        // compare masks ANDed, no branch; COUNT counts the self pair
+                        // here and takes it off once per query below
+                        const uint64_t m = MODE == EDGE_COUNT ? ballot(in_r) : ballot(in_r) & ballot(other);
                         if (MODE == EDGE_INDEX && (in_r & other)) out.senders[run + (uint32_t)mbcnt(m)] = j + index_base;
                         if (MODE == EDGE_FILL && (in_r & other))
                             L.rec[nrec + (uint32_t)mbcnt(m)] =
@@ -505,6 +507,14 @@ __device__ inline uint32_t leaf_pass(int32_t qf, int32_t qc, float4 qlo, float4 
             __builtin_amdgcn_wave_barrier();
         }
         if (MODE == EDGE_COUNT) {
+            // The self pair was counted: a query's own point is among its
+            // candidates (its leaf is in its list at box distance 0, and the
+            // point-box filter keeps it) and within r, unless a coordinate is
+            // not finite (q - q is then NaN, as the sweep computed it).
+            if (self_pair && lane < nqb) {
+                const float sx = myq.x - myq.x, sy = myq.y - myq.y, sz = myq.z - myq.z;
+                if (sx * sx + sy * sy + sz * sz <= r_sq) my -= 1u;
+            }
             if (lane < nqb) qcount[slot] = my;
             leaf_total += (uint32_t)wave_sum(lane < nqb ? (int)my : 0);
         }
