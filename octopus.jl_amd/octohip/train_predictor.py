@@ -9,7 +9,8 @@ the training seeds):
   clustered  4,000,000 points, 90% from a 16-cluster Gaussian mixture
              (sigma 0.05) and 10% uniform, radius 0.01
   sph        the 100^3 SPH lattice (h = 0.01, jitter 0.1h), radius 2h, each
-             instance advanced a seeded number of N(0, (0.01h)^2) steps
+             instance advanced 10 * (seed % 11) steps of N(0, (0.01h)^2), so
+             consecutive seeds cover the configuration's 100 time steps
 The exact per-leaf hit counts come from the count pass (no edges are
 written).  Prints one JSON line with the training loss and the held-out
 error against the best constant predictor.
@@ -36,7 +37,7 @@ def instance(workload: str, seed: int, n: int):
     if workload == "sph":
         side = round(n ** (1 / 3))
         pts, r = datasets.sph_lattice(seed=seed, side=side)
-        for k in range(seed % 5):
+        for k in range(10 * (seed % 11)):
             pts = datasets.sph_step(pts, k, seed=seed)
         return pts, float(r)
     raise SystemExit(f"unknown workload {workload}")
