@@ -1287,11 +1287,12 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, int emit, o
                                            c->run_chunk, SetLookup{}, heavy ? p.heavy_flags(nql) : nullptr,
                                            early ? c->sig_dev : nullptr, flags,
                                            (int)((HF_BYTES + 2 * sizeof(BuildParams)) / 4), c->stream));
-            if (heavy)
-                HIP_TRY(c, launch_edges_heavy_fill(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
+            if (heavy) {  // one block per heavy leaf
+                HIP_TRY(c, launch_edges_dense_fill(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
                                                    inv_r, self_pair, p.heavy_list(), flags + 16,
                                                    p.qcount.as<uint32_t>(), p.qoff.as<uint32_t>(), flags + 4, eo,
                                                    c->index_base, SetLookup{}, c->stream));
+            }
         } else {
             {
                 Phase ph(c, OCT_PROF_COUNT);

@@ -905,10 +905,11 @@ k_edges_heavy(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qs
 }
 
 // Writes the edges of the heavy leaves after the single pass, one wave per
-// leaf: the single pass left each one's output position in qoff at its first
-// query, and its per-query counts in qcount came from k_edges_heavy.  Nothing
-// runs when the single pass raised flags[0] (capacity) or flags[1]
-// (look-back or plan): the host then redoes the call.
+// leaf (the batched sets, whose heavy leaves are seldom dense; a single pair
+// takes k_edges_dense_fill): the single pass left each one's output position
+// in qoff at its first query, and its per-query counts in qcount came from
+// k_edges_heavy.  Nothing runs when the single pass raised flags[0]
+// (capacity) or flags[1] (look-back or plan): the host then redoes the call.
 __global__ void __launch_bounds__(EW_WAVES * OCT_WAVE)
 k_edges_heavy_fill(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
                    const BuildParams* __restrict__ qprm, const float4* __restrict__ tsc,
@@ -947,6 +948,174 @@ k_edges_heavy_fill(const NodeRec* __restrict__ qleaves, const float4* __restrict
         leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
         leaf_pass<EDGE_FILL>(qf, qc, qa.lo, qa.hi, ncand, lf0, lst, qsc, tsc, lr_sq, linv_r, self_pair, 0, qcount,
                              qoff, out, libase, L, lane);
+    }
+}
+
+// Writes the edges of the heavy leaves after the single pass, one block per
+// leaf, query by query.  A heavy leaf is dense: hundreds of queries, each
+// with hundreds of hits among thousands of candidates.  Swept tile by tile
+// with every query of a 64-query block (leaf_pass), each query's output is
+// written a piece per tile, and the pieces of 64 rows land in HBM as partial
+// lines; here the block stages the leaf's candidates (those within r of the
+// leaf box, DF_STAGE at a time, in sequence order) in LDS, and each wave takes
+// one query at a time through all of them, recording its hits (stage slots)
+// and writing them out as consecutive edges of that query, whole waves at a
+// time.  The edges and their order (candidate sequence order per query) are
+// leaf_pass's.  The single pass left each leaf's output position in qoff at
+// its first query, and k_edges_heavy the per-query counts in qcount; this
+// kernel turns them into per-query cursors in qoff.  Nothing runs when the
+// single pass raised flags[0] (capacity) or flags[1] (look-back or plan).
+constexpr int DF_THREADS = 256;
+constexpr int DF_WAVES = DF_THREADS / OCT_WAVE;
+constexpr int DF_STAGE = 2048;  // candidates staged per round (32 KB)
+constexpr int DF_RECS = 512;    // hit records per wave (stage slots)
+
+__global__ void __launch_bounds__(DF_THREADS)
+k_edges_dense_fill(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
+                   const BuildParams* __restrict__ qprm, const float4* __restrict__ tsc,
+                   const int2* __restrict__ lists, const int32_t* __restrict__ lcount, int cap, float r_sq,
+                   float inv_r, int self_pair, const int32_t* __restrict__ heavy_list,
+                   const uint32_t* __restrict__ n_heavy, const uint32_t* __restrict__ qcount,
+                   uint32_t* __restrict__ qoff, const uint32_t* __restrict__ flags, EdgeOut out, int index_base0,
+                   SetLookup sets) {
+    __shared__ float4 s_pts[DF_STAGE];
+    __shared__ uint16_t s_rec[DF_WAVES][DF_RECS];
+    __shared__ int32_t s_start[OCT_WAVE + 1];
+    __shared__ int32_t s_first[OCT_WAVE];
+    __shared__ uint32_t s_wsum[DF_WAVES];
+    __shared__ uint32_t s_carry;
+    if (flags[0] != 0 || flags[1] != 0) return;  // block-uniform
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / OCT_WAVE));
+    const int lane = lane_id();
+    const int32_t nleaves = qprm->n_leaves;
+    const int32_t nh = (int32_t)min(*n_heavy, (uint32_t)max(nleaves, 0));
+    uint16_t* rec = s_rec[wv];
+    for (int32_t hi = (int32_t)blockIdx.x; hi < nh; hi += (int32_t)gridDim.x) {  // block-uniform
+        const int32_t r = heavy_list[hi];
+        if (r < 0 || r >= nleaves) continue;  // the candidate walk lists only leaves
+        const NodeRec qa = qleaves[r];
+        const int32_t qf = rec_a(qa), qc = rec_b(qa);
+        const int32_t ncand = min(lcount[r], cap);
+        const int2* lst = lists + (int64_t)r * cap;
+        float lr_sq, linv_r;
+        int libase;
+        leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
+        // per-query cursors: the leaf's position plus the counts before each query
+        const uint32_t pos0 = qoff[qf];
+        __syncthreads();  // every thread has read qoff[qf] before it is rewritten
+        if (threadIdx.x == 0) s_carry = pos0;
+        __syncthreads();
+        for (int32_t q0 = 0; q0 < qc; q0 += DF_THREADS) {  // block-uniform
+            const int32_t qi = q0 + (int32_t)threadIdx.x;
+            const uint32_t c = qi < qc ? qcount[qf + qi] : 0u;
+            const uint32_t inc = (uint32_t)wave_incl_scan((int)c);
+            if (lane == OCT_WAVE - 1) s_wsum[wv] = inc;
+            __syncthreads();
+            uint32_t before = s_carry;
+            for (int k = 0; k < wv; ++k) before += s_wsum[k];
+            if (qi < qc) qoff[qf + qi] = before + inc - c;
+            __syncthreads();
+            if (threadIdx.x == DF_THREADS - 1) s_carry = before + inc;
+            __syncthreads();
+        }
+        // the candidate sequence, one candidate leaf group of 64 at a time
+        for (int32_t g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
+            if (wv == 0) {
+                int2 lf = make_int2(0, 0);
+                if (g0 + lane < ncand) lf = lst[g0 + lane];
+                const int incl = wave_incl_scan(lf.y);
+                s_start[lane + 1] = incl;
+                if (lane == 0) s_start[0] = 0;
+                s_first[lane] = lf.x;
+            }
+            __syncthreads();
+            const int total = s_start[OCT_WAVE];
+            for (int t0 = 0; t0 < total;) {  // block-uniform
+                // stage the candidates within r of the leaf box, in order
+                int kept = 0;
+                int t = t0;
+                while (t < total && kept + DF_THREADS <= DF_STAGE) {  // block-uniform
+                    const int ci = t + (int)threadIdx.x;
+                    bool keep = false;
+                    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (ci < total) {
+                        int lo = 0, hi2 = OCT_WAVE;  // owner: last o with start[o] <= ci
+                        while (hi2 - lo > 1) {
+                            const int mid = (lo + hi2) >> 1;
+                            if (s_start[mid] <= ci) lo = mid;
+                            else hi2 = mid;
+                        }
+                        p = tsc[s_first[lo] + (ci - s_start[lo])];
+                        keep = point_box_d2(p.x, p.y, p.z, qa.lo, qa.hi) <= lr_sq;
+                    }
+                    const uint64_t m = ballot(keep);
+                    if (lane == 0) s_wsum[wv] = (uint32_t)__popcll(m);
+                    __syncthreads();
+                    int wbase = kept;
+                    int add = 0;
+                    for (int k = 0; k < DF_WAVES; ++k) {
+                        const int v = (int)s_wsum[k];
+                        wbase += k < wv ? v : 0;
+                        add += v;
+                    }
+                    if (keep) s_pts[wbase + mbcnt(m)] = p;
+                    __syncthreads();
+                    kept += add;
+                    t += DF_THREADS;
+                }
+                t0 = t;
+                // each wave: its queries through the staged candidates
+                for (int32_t qi = wv; qi < qc; qi += DF_WAVES) {
+                    const float4 qp = qsc[qf + qi];  // every lane the same query
+                    const float qx = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(qp.x)));
+                    const float qy = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(qp.y)));
+                    const float qz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(qp.z)));
+                    const int32_t i = __builtin_amdgcn_readfirstlane(__float_as_int(qp.w));
+                    const int32_t ix = self_pair ? i : -2;  // -2 matches no point: only the self pair drops j == i
+                    uint32_t cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)qoff[qf + qi]);
+                    uint32_t nrec = 0;
+                    auto flush = [&]() {
+                        __builtin_amdgcn_wave_barrier();
+                        for (uint32_t u0 = 0; u0 < nrec; u0 += OCT_WAVE) {
+                            const uint32_t u = u0 + (uint32_t)lane;
+                            if (u < nrec) {
+                                const float4 pp = s_pts[rec[u]];
+                                const float dx = qx - pp.x, dy = qy - pp.y, dz = qz - pp.z;
+                                const float d2 = dx * dx + dy * dy + dz * dz;
+                                const int64_t e = (int64_t)(cur + u);
+                                out.senders[e] = __float_as_int(pp.w) + libase;
+                                out.receivers[e] = i + libase;
+                                *reinterpret_cast<float3*>(out.rdisp + 3 * e) =
+                                    make_float3(rel_disp(qx, pp.x, linv_r), rel_disp(qy, pp.y, linv_r),
+                                                rel_disp(qz, pp.z, linv_r));
+                                out.rdist[e] = sqrtf(d2) * linv_r;
+                            }
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        cur += nrec;
+                        nrec = 0;
+                    };
+                    for (int s0 = 0; s0 < kept; s0 += OCT_WAVE) {  // wave-uniform
+                        if (nrec + (uint32_t)OCT_WAVE > (uint32_t)DF_RECS) flush();
+                        const int sl = s0 + lane;
+                        bool hit = false;
+                        if (sl < kept) {
+                            const float4 pp = s_pts[sl];
+                            const float dx = qx - pp.x, dy = qy - pp.y, dz = qz - pp.z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            hit = d2 <= lr_sq && __float_as_int(pp.w) != ix;
+                        }
+                        const uint64_t m = ballot(hit);
+                        if (hit) rec[nrec + (uint32_t)mbcnt(m)] = (uint16_t)sl;
+                        nrec += (uint32_t)__popcll(m);
+                    }
+                    if (nrec) flush();
+                    if (lane == 0) qoff[qf + qi] = cur;  // this wave's query: the next round continues here
+                }
+                __syncthreads();  // the stage is rewritten next round
+            }
+            __syncthreads();  // s_start is rewritten for the next group
+        }
     }
 }
 
@@ -1420,6 +1589,19 @@ hipError_t launch_edges_heavy_fill(const TreeView& q, const TreeView& t, const i
     hipLaunchKernelGGL(k_edges_heavy_fill, dim3(g), dim3(EW_WAVES * OCT_WAVE), 0, s, q.leaves, q.scoord, q.params,
                        t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, heavy_list, n_heavy, qcount, qoff,
                        flags, out, index_base, sets);
+    return hipGetLastError();
+}
+
+hipError_t launch_edges_dense_fill(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
+                                   int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
+                                   const uint32_t* n_heavy, const uint32_t* qcount, uint32_t* qoff,
+                                   const uint32_t* flags, EdgeOut out, int index_base, const SetLookup& sets,
+                                   hipStream_t s) {
+    // a fixed grid (the count is on the device); blocks loop over the heavy leaves
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(q.n_leaves, 2048));
+    hipLaunchKernelGGL(k_edges_dense_fill, dim3(g), dim3(DF_THREADS), 0, s, q.leaves, q.scoord, q.params, t.scoord,
+                       lists, lcount, cap, r_sq, inv_r, self_pair, heavy_list, n_heavy, qcount, qoff, flags, out,
+                       index_base, sets);
     return hipGetLastError();
 }
 

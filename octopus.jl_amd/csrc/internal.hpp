@@ -270,6 +270,14 @@ hipError_t launch_edges_heavy_fill(const TreeView& q, const TreeView& t, const i
                                    int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
                                    const uint32_t* n_heavy, uint32_t* qcount, uint32_t* qoff, const uint32_t* flags,
                                    EdgeOut out, int index_base, const SetLookup& sets, hipStream_t s);
+// The heavy leaves' edges after the single pass, one block per leaf, query by
+// query through the leaf's candidates staged in LDS (k_edges_dense_fill):
+// the same edges in the same order as launch_edges_heavy_fill.
+hipError_t launch_edges_dense_fill(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
+                                   int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
+                                   const uint32_t* n_heavy, const uint32_t* qcount, uint32_t* qoff,
+                                   const uint32_t* flags, EdgeOut out, int index_base, const SetLookup& sets,
+                                   hipStream_t s);
 // Exact counts of the listed heavy leaves ahead of the single pass: per-query
 // counts into qcount, leaf totals into lhits.
 hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
