@@ -460,6 +460,35 @@ def test_single_pass_heavy_leaves(orc):
     assert heavy[2] == 0
 
 
+def test_dense_fill_many_queries_and_candidates(orc):
+    """A dense core (2,500 points in a cube of side r/4, so every pair is an
+    edge) over a uniform background: its leaves hold hundreds to thousands of
+    queries (the per-query cursors are scanned in several block-wide
+    chunks) and more candidates within r of the leaf box than one staging
+    round of the dense fill takes.  The learned call (heavy leaves through
+    the dense fill) writes exactly the exact path's edges, in the same order,
+    and both are the oracle's set."""
+    import octohip
+    from octohip import predictor
+
+    r = 0.02
+    core = 0.5 + (datasets.uniform(71, 0, 3 * 2500).reshape(2500, 3) * (r / 4))
+    back = datasets.uniform(72, 0, 3 * 3000).reshape(3000, 3)
+    pts = np.concatenate([core, back]).astype(np.float32)
+    tns, sid, _ = _build(pts, r)
+    ex = _gpu_edges(octohip.build_edges(tns, sid, sid, mode="exact"))
+    octohip.build_edges(tns, sid, sid, mode="learned")  # sizes the single pass's buffers
+    got = _gpu_edges(octohip.build_edges(tns, sid, sid, mode="learned"))
+    st = predictor.edge_stats(tns, sid, sid)
+    print(f"dense core: {st['heavy_leaves']} heavy leaves, {st['n_edges']} edges", flush=True)
+    assert st["single_pass"] and st["heavy_leaves"] > 0
+    assert st["n_edges"] > 2500 * 2499
+    for a, b in zip(got, ex):  # same edges in the same order
+        assert np.array_equal(bits(a), bits(b))
+    o = orc.build_edges(orc.run(pts, r), pts)
+    _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+
+
 PLAN_CASES = {
     "plain100k_leaf32": ("plain100k", 32),
     "plain100k_leaf8": ("plain100k", 8),
