@@ -1272,9 +1272,10 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, int emit, o
             eo = EdgeOut{p.senders.as<int32_t>(), p.receivers.as<int32_t>(), p.rdisp.as<float>(), p.rdist.as<float>()};
             Phase ph(c, OCT_PROF_SINGLE);
             if (heavy)
-                HIP_TRY(c, launch_edges_heavy(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap, r_sq,
-                                              inv_r, self_pair, p.heavy_list(), flags + 16, p.qcount.as<uint32_t>(),
-                                              p.lhits.as<uint32_t>(), SetLookup{}, c->stream));
+                HIP_TRY(c, launch_edges_dense_count(qv, tv, p.lists.as<int2>(), p.lcount.as<int32_t>(), p.cap,
+                                                    r_sq, inv_r, self_pair, p.heavy_list(), flags + 16,
+                                                    p.qcount.as<uint32_t>(), p.lhits.as<uint32_t>(), SetLookup{},
+                                                    c->stream));
             if (early) {  // cleared before the launch; the single pass's last run sets it
                 __atomic_store_n((uint64_t*)(c->sig + SP_SIG_MARK), 0ull, __ATOMIC_RELEASE);
             }

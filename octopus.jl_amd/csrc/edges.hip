@@ -32,11 +32,15 @@ constexpr int EW_TILE = 4;      // candidate chunks of 64 held in registers per 
 constexpr int REC_MAX_CAND = 1024;
 // Heavy leaf: one that will not be recorded in one sweep (more than 64
 // queries or candidate leaves, an estimated filtered candidate count above
-// one tile) or whose estimated query x candidate pairs exceed HEAVY_PAIRS
-// (about four hit lists of hits at the usual hit rate of a quarter).  The
-// single pass takes their counts from a parallel count pass run before it,
-// so no run's aggregate waits on them.
+// one tile), whose estimated query x candidate pairs exceed HEAVY_PAIRS
+// (about four hit lists of hits at the usual hit rate of a quarter), or whose
+// candidate leaves hold more than HEAVY_RAW points (a small leaf beside a
+// dense region: the single pass, one query per lane, would load and filter
+// them all for a few queries, where the heavy-leaf kernels spread the
+// candidates over the lanes).  The single pass takes their counts from a
+// parallel count pass run before it, so no run's aggregate waits on them.
 constexpr float HEAVY_PAIRS = 4096.f;
+constexpr int HEAVY_RAW = 512;
 
 // Direction of box c seen from box q, one of 27 codes sx + 3 sy + 9 sz with
 // s = 0 (below), 1 (level), 2 (above) per axis: the centres must differ by
@@ -77,10 +81,13 @@ __device__ inline float filtered_estimate(const float4& qlo, const float4& qhi, 
 
 template <bool DIRS, bool HEAVY, typename Push>
 __device__ inline bool ll_visit(const float4& qlo, const float4& qhi, const NodeRec& ch, float r_sq, float r,
-                                int2* out, uint8_t* dout, int cap, int& cnt, float& est, Push push) {
+                                int2* out, uint8_t* dout, int cap, int& cnt, float& est, int& raw, Push push) {
     if (box_box_d2(qlo, qhi, ch.lo, ch.hi) > r_sq) return true;
     if (rec_b(ch) > 0) {
-        if (HEAVY) est += filtered_estimate(qlo, qhi, ch, r);
+        if (HEAVY) {
+            est += filtered_estimate(qlo, qhi, ch, r);
+            raw += rec_b(ch);
+        }
         if (cnt < cap) {
             out[cnt] = make_int2(rec_a(ch), rec_b(ch));
             if (DIRS) dout[cnt] = dir_code(qlo, qhi, ch.lo, ch.hi);
@@ -124,12 +131,16 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
         uint8_t* dout = DIRS ? ldir + (int64_t)r * cap : nullptr;
         int cnt = 0;
         float est = 0.f;  // HEAVY: estimated filtered candidate points
+        int raw = 0;      // HEAVY: points of the candidate leaves
         const float rad = HEAVY ? sqrtf(r_sq) : 0.f;
         bool full = false;
         const NodeRec rr = tnodes[root];
         if (box_box_d2(qa.lo, qa.hi, rr.lo, rr.hi) <= r_sq) {
             if (rec_b(rr) > 0) {
-                if (HEAVY) est += filtered_estimate(qa.lo, qa.hi, rr, rad);
+                if (HEAVY) {
+                    est += filtered_estimate(qa.lo, qa.hi, rr, rad);
+                    raw += rec_b(rr);
+                }
                 if (cnt < cap) {
                     out[cnt] = make_int2(rec_a(rr), rec_b(rr));
                     if (DIRS) dout[cnt] = dir_code(qa.lo, qa.hi, rr.lo, rr.hi);
@@ -152,7 +163,7 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
                         if (c < ent.y) ch[c] = tnodes[ent.x + c];
 #pragma unroll
                     for (int c = 0; c < 8; ++c)
-                        if (c < ent.y && !ll_visit<DIRS, HEAVY>(qa.lo, qa.hi, ch[c], r_sq, rad, out, dout, cap, cnt, est, push)) full = true;
+                        if (c < ent.y && !ll_visit<DIRS, HEAVY>(qa.lo, qa.hi, ch[c], r_sq, rad, out, dout, cap, cnt, est, raw, push)) full = true;
                 }
             }
         }
@@ -162,7 +173,7 @@ k_leaf_lists(const NodeRec* __restrict__ qleaves,
         if (HEAVY) {  // heavy leaves: flagged, and listed compactly (one atomic per wave)
             const int qc = rec_b(qa);
             const bool heavy = qc > OCT_WAVE || cnt > OCT_WAVE || est > (float)(EW_TILE * OCT_WAVE) ||
-                               (float)qc * est > HEAVY_PAIRS;
+                               (float)qc * est > HEAVY_PAIRS || raw > HEAVY_RAW;
             lheavy[r] = heavy ? 1 : 0;
             const uint64_t hm = ballot(heavy);
             if (hm) {
@@ -220,12 +231,16 @@ k_leaf_lists_coop(const NodeRec* __restrict__ qleaves, const BuildParams* __rest
         uint8_t* dout = DIRS ? ldir + (int64_t)r * cap : nullptr;
         int cnt = 0;
         float est = 0.f;  // HEAVY: estimated filtered candidate points
+        int raw = 0;      // HEAVY: points of the candidate leaves
         const float rad = HEAVY ? sqrtf(r_sq) : 0.f;
         bool full = false;
         const NodeRec rr = tnodes[root];
         if (box_box_d2(qa.lo, qa.hi, rr.lo, rr.hi) <= r_sq) {
             if (rec_b(rr) > 0) {
-                if (HEAVY) est += filtered_estimate(qa.lo, qa.hi, rr, rad);
+                if (HEAVY) {
+                    est += filtered_estimate(qa.lo, qa.hi, rr, rad);
+                    raw += rec_b(rr);
+                }
                 if (c == 0 && cap > 0) {
                     out[0] = make_int2(rec_a(rr), rec_b(rr));
                     if (DIRS) dout[0] = dir_code(qa.lo, qa.hi, rr.lo, rr.hi);
@@ -257,10 +272,15 @@ k_leaf_lists_coop(const NodeRec* __restrict__ qleaves, const BuildParams* __rest
                     }
                     if (HEAVY && lm) {  // the group's sum (a fixed order: the estimate is a heuristic)
                         float e = leaf ? filtered_estimate(qa.lo, qa.hi, ch, rad) : 0.f;
+                        int w = leaf ? rec_b(ch) : 0;
                         e += __shfl_xor(e, 1, OCT_WAVE);
                         e += __shfl_xor(e, 2, OCT_WAVE);
                         e += __shfl_xor(e, 4, OCT_WAVE);
+                        w += __shfl_xor(w, 1, OCT_WAVE);
+                        w += __shfl_xor(w, 2, OCT_WAVE);
+                        w += __shfl_xor(w, 4, OCT_WAVE);
                         est += e;
+                        raw += w;
                     }
                     cnt += __popc(lm);
                     const int np = __popc(im);
@@ -281,7 +301,7 @@ k_leaf_lists_coop(const NodeRec* __restrict__ qleaves, const BuildParams* __rest
         if (HEAVY && c == 0) {  // heavy leaves: flagged here, listed below
             const int qc = rec_b(qa);
             const bool heavy = qc > OCT_WAVE || cnt > OCT_WAVE || est > (float)(EW_TILE * OCT_WAVE) ||
-                               (float)qc * est > HEAVY_PAIRS;
+                               (float)qc * est > HEAVY_PAIRS || raw > HEAVY_RAW;
             lheavy[r] = heavy ? 1 : 0;
         }
     }
@@ -952,44 +972,112 @@ k_edges_heavy_fill(const NodeRec* __restrict__ qleaves, const float4* __restrict
 }
 
 // Writes the edges of the heavy leaves after the single pass, one block per
-// leaf, query by query.  A heavy leaf is dense: hundreds of queries, each
-// with hundreds of hits among thousands of candidates.  Swept tile by tile
-// with every query of a 64-query block (leaf_pass), each query's output is
-// written a piece per tile, and the pieces of 64 rows land in HBM as partial
-// lines; here the block stages the leaf's candidates (those within r of the
-// leaf box, DF_STAGE at a time, in sequence order) in LDS, and each wave takes
-// one query at a time through all of them, recording its hits (stage slots)
-// and writing them out as consecutive edges of that query, whole waves at a
-// time.  The edges and their order (candidate sequence order per query) are
-// leaf_pass's.  The single pass left each leaf's output position in qoff at
-// its first query, and k_edges_heavy the per-query counts in qcount; this
-// kernel turns them into per-query cursors in qoff.  Nothing runs when the
+// leaf, query by query.  Heavy leaves are dense: each query has hundreds of
+// hits among thousands of candidates.  Swept tile by tile with every query of
+// a 64-query block (leaf_pass), each query's output is written a piece per
+// tile, and the pieces of 64 rows land in HBM as partial lines; here the block
+// stages the leaf's candidates (those within r of the leaf box, in sequence
+// order, across candidate leaf groups, DF_STAGE at a time) in LDS, and each
+// wave takes one query at a time through all of them, recording its hits
+// (stage slots) and writing them out as consecutive edges of that query, whole
+// waves at a time.  The edges and their order (candidate sequence order per
+// query) are leaf_pass's.  Queries go 256 at a time: their cursors (the
+// leaf's output position, left in qoff at its first query by the single pass,
+// plus the counts of the COUNT launch in qcount) live in LDS, each wave takes
+// a contiguous share of them, and lane j holds the j-th query's position in
+// registers, so a query's turn reads no global memory.  Nothing runs when the
 // single pass raised flags[0] (capacity) or flags[1] (look-back or plan).
 constexpr int DF_THREADS = 256;
 constexpr int DF_WAVES = DF_THREADS / OCT_WAVE;
-constexpr int DF_STAGE = 2048;  // candidates staged per round (32 KB)
+constexpr int DF_STAGE = 1984;  // candidates staged per round
+constexpr int DF_UNROLL = 4;    // candidate chunks of 64 per step of the query sweep
+// the stage array: 64 slots past the last staged candidate hold points at
+// infinity, so that the sweep's last chunk reads them and finds no hit: 32 KB
+constexpr int DF_SLOTS = DF_STAGE + OCT_WAVE;
 constexpr int DF_RECS = 512;    // hit records per wave (stage slots)
+constexpr int DF_LOADS = 4;     // candidates each thread loads per staging step
 
+// Per-block LDS of k_edges_dense_fill.
+struct DenseLds {
+    float4 pts[DF_SLOTS];
+    uint16_t rec[DF_WAVES][DF_RECS];
+    uint32_t cur[DF_THREADS];
+    int32_t start[OCT_WAVE + 1];
+    int32_t first[OCT_WAVE];
+    uint32_t wsum[DF_LOADS][DF_WAVES];
+    uint32_t carry;
+};
+
+// One staging step: candidates [t, t + SU * DF_THREADS) of the current group
+// (clipped to total), those within r of the box appended to L.pts in sequence
+// order.  Block-uniform; returns the new stage size.
+template <int SU>
+__device__ inline int df_stage_step(DenseLds& L, int t, int total, int kept, const float4* __restrict__ tsc,
+                                    float4 lo, float4 hi, float lr_sq, int wv) {
+    float4 p[SU];
+    bool keep[SU];
+#pragma unroll
+    for (int k = 0; k < SU; ++k) {
+        const int ci = t + k * DF_THREADS + (int)threadIdx.x;
+        keep[k] = false;
+        p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ci < total) {
+            int o = 0, hi2 = OCT_WAVE;  // owner: last o with start[o] <= ci
+            while (hi2 - o > 1) {
+                const int mid = (o + hi2) >> 1;
+                if (L.start[mid] <= ci) o = mid;
+                else hi2 = mid;
+            }
+            p[k] = tsc[L.first[o] + (ci - L.start[o])];
+        }
+    }
+    uint64_t m[SU];
+#pragma unroll
+    for (int k = 0; k < SU; ++k) {
+        const int ci = t + k * DF_THREADS + (int)threadIdx.x;
+        keep[k] = ci < total && point_box_d2(p[k].x, p[k].y, p[k].z, lo, hi) <= lr_sq;
+        m[k] = ballot(keep[k]);
+        if (lane_id() == 0) L.wsum[k][wv] = (uint32_t)__popcll(m[k]);
+    }
+    __syncthreads();
+    int base = kept;
+#pragma unroll
+    for (int k = 0; k < SU; ++k) {
+        int wb = base;
+        for (int w = 0; w < DF_WAVES; ++w) {
+            const int v = (int)L.wsum[k][w];
+            wb += w < wv ? v : 0;
+            base += v;
+        }
+        if (keep[k]) L.pts[wb + mbcnt(m[k])] = p[k];
+    }
+    if (threadIdx.x < OCT_WAVE)  // the padding after the new end (base <= DF_STAGE)
+        L.pts[base + (int)threadIdx.x] = make_float4(INFINITY, INFINITY, INFINITY, __int_as_float(-1));
+    __syncthreads();  // the stage is complete and wsum free again
+    return base;
+}
+
+// MODE EDGE_COUNT: the same sweep counts each query's hits (qcount) and the
+// leaf's total (lhits[r]) before the single pass, in place of k_edges_heavy's
+// one wave per leaf (whose lanes hold queries: a heavy leaf of 20 queries
+// leaves 44 lanes idle, where here the lanes hold candidates).
+template <int MODE>
 __global__ void __launch_bounds__(DF_THREADS)
-k_edges_dense_fill(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
-                   const BuildParams* __restrict__ qprm, const float4* __restrict__ tsc,
-                   const int2* __restrict__ lists, const int32_t* __restrict__ lcount, int cap, float r_sq,
-                   float inv_r, int self_pair, const int32_t* __restrict__ heavy_list,
-                   const uint32_t* __restrict__ n_heavy, const uint32_t* __restrict__ qcount,
-                   uint32_t* __restrict__ qoff, const uint32_t* __restrict__ flags, EdgeOut out, int index_base0,
-                   SetLookup sets) {
-    __shared__ float4 s_pts[DF_STAGE];
-    __shared__ uint16_t s_rec[DF_WAVES][DF_RECS];
-    __shared__ int32_t s_start[OCT_WAVE + 1];
-    __shared__ int32_t s_first[OCT_WAVE];
-    __shared__ uint32_t s_wsum[DF_WAVES];
-    __shared__ uint32_t s_carry;
-    if (flags[0] != 0 || flags[1] != 0) return;  // block-uniform
+k_edges_dense(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qsc,
+              const BuildParams* __restrict__ qprm, const float4* __restrict__ tsc, const int2* __restrict__ lists,
+              const int32_t* __restrict__ lcount, int cap, float r_sq, float inv_r, int self_pair,
+              const int32_t* __restrict__ heavy_list, const uint32_t* __restrict__ n_heavy,
+              uint32_t* __restrict__ qcount, const uint32_t* __restrict__ qoff, uint32_t* __restrict__ lhits,
+              const uint32_t* __restrict__ flags, EdgeOut out, int index_base0, SetLookup sets) {
+    static_assert(MODE == EDGE_COUNT || MODE == EDGE_FILL, "count or fill");
+    constexpr bool FILL = MODE == EDGE_FILL;
+    __shared__ DenseLds L;
+    if (FILL && (flags[0] != 0 || flags[1] != 0)) return;  // block-uniform
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / OCT_WAVE));
     const int lane = lane_id();
     const int32_t nleaves = qprm->n_leaves;
     const int32_t nh = (int32_t)min(*n_heavy, (uint32_t)max(nleaves, 0));
-    uint16_t* rec = s_rec[wv];
+    uint16_t* rec = L.rec[wv];
     for (int32_t hi = (int32_t)blockIdx.x; hi < nh; hi += (int32_t)gridDim.x) {  // block-uniform
         const int32_t r = heavy_list[hi];
         if (r < 0 || r >= nleaves) continue;  // the candidate walk lists only leaves
@@ -1000,121 +1088,155 @@ k_edges_dense_fill(const NodeRec* __restrict__ qleaves, const float4* __restrict
         float lr_sq, linv_r;
         int libase;
         leaf_set_params(sets, qf, r_sq, inv_r, index_base0, lr_sq, linv_r, libase);
-        // per-query cursors: the leaf's position plus the counts before each query
-        const uint32_t pos0 = qoff[qf];
-        __syncthreads();  // every thread has read qoff[qf] before it is rewritten
-        if (threadIdx.x == 0) s_carry = pos0;
-        __syncthreads();
-        for (int32_t q0 = 0; q0 < qc; q0 += DF_THREADS) {  // block-uniform
-            const int32_t qi = q0 + (int32_t)threadIdx.x;
-            const uint32_t c = qi < qc ? qcount[qf + qi] : 0u;
-            const uint32_t inc = (uint32_t)wave_incl_scan((int)c);
-            if (lane == OCT_WAVE - 1) s_wsum[wv] = inc;
-            __syncthreads();
-            uint32_t before = s_carry;
-            for (int k = 0; k < wv; ++k) before += s_wsum[k];
-            if (qi < qc) qoff[qf + qi] = before + inc - c;
-            __syncthreads();
-            if (threadIdx.x == DF_THREADS - 1) s_carry = before + inc;
-            __syncthreads();
-        }
-        // the candidate sequence, one candidate leaf group of 64 at a time
-        for (int32_t g0 = 0; g0 < ncand; g0 += OCT_WAVE) {
-            if (wv == 0) {
-                int2 lf = make_int2(0, 0);
-                if (g0 + lane < ncand) lf = lst[g0 + lane];
-                const int incl = wave_incl_scan(lf.y);
-                s_start[lane + 1] = incl;
-                if (lane == 0) s_start[0] = 0;
-                s_first[lane] = lf.x;
+        if (FILL && threadIdx.x == 0) L.carry = qoff[qf];
+        uint32_t mytotal = 0;  // COUNT: hits of this thread's queries
+        for (int32_t b0 = 0; b0 < qc; b0 += DF_THREADS) {  // block-uniform: a batch of 256 queries
+            const int32_t nb = min(qc - b0, DF_THREADS);
+            if (FILL) {  // cursors of the batch, in query order
+                const int32_t qi = (int32_t)threadIdx.x;
+                const uint32_t c = qi < nb ? qcount[qf + b0 + qi] : 0u;
+                const uint32_t inc = (uint32_t)wave_incl_scan((int)c);
+                if (lane == OCT_WAVE - 1) L.wsum[0][wv] = inc;
+                __syncthreads();  // wsum and carry written
+                uint32_t before = L.carry;
+                for (int k = 0; k < wv; ++k) before += L.wsum[0][k];
+                L.cur[qi] = before + inc - c;
+                __syncthreads();  // every thread has read carry
+                if (threadIdx.x == DF_THREADS - 1) L.carry = before + inc;
+            } else {
+                L.cur[threadIdx.x] = 0u;  // counts; read by others only after the syncs below
             }
-            __syncthreads();
-            const int total = s_start[OCT_WAVE];
-            for (int t0 = 0; t0 < total;) {  // block-uniform
-                // stage the candidates within r of the leaf box, in order
+            // wave w takes a contiguous share of the batch's queries, lane j its
+            // j-th, so that each wave's output is one run (partial lines only at
+            // the ends of the shares)
+            const int32_t per = (nb + DF_WAVES - 1) / DF_WAVES;  // <= 64
+            const int32_t qw0 = wv * per;
+            const int nmine = max(0, min(nb, qw0 + per) - qw0);  // wave-uniform
+            const float4 myq = lane < nmine ? qsc[qf + b0 + qw0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+            // the candidate sequence, streamed through the stage
+            int32_t g0 = -OCT_WAVE;  // current group of candidate leaves
+            int total = 0, t = 0;
+            bool more = ncand > 0;
+            while (more) {  // block-uniform
                 int kept = 0;
-                int t = t0;
-                while (t < total && kept + DF_THREADS <= DF_STAGE) {  // block-uniform
-                    const int ci = t + (int)threadIdx.x;
-                    bool keep = false;
-                    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (ci < total) {
-                        int lo = 0, hi2 = OCT_WAVE;  // owner: last o with start[o] <= ci
-                        while (hi2 - lo > 1) {
-                            const int mid = (lo + hi2) >> 1;
-                            if (s_start[mid] <= ci) lo = mid;
-                            else hi2 = mid;
+                while (true) {  // fill the stage
+                    if (t >= total) {  // next group of 64 candidate leaves
+                        g0 += OCT_WAVE;
+                        if (g0 >= ncand) {
+                            more = false;
+                            break;
                         }
-                        p = tsc[s_first[lo] + (ci - s_start[lo])];
-                        keep = point_box_d2(p.x, p.y, p.z, qa.lo, qa.hi) <= lr_sq;
+                        __syncthreads();  // every thread is done with start
+                        if (wv == 0) {
+                            int2 lf = make_int2(0, 0);
+                            if (g0 + lane < ncand) lf = lst[g0 + lane];
+                            const int incl = wave_incl_scan(lf.y);
+                            L.start[lane + 1] = incl;
+                            if (lane == 0) L.start[0] = 0;
+                            L.first[lane] = lf.x;
+                        }
+                        __syncthreads();
+                        total = L.start[OCT_WAVE];
+                        t = 0;
+                        continue;
                     }
-                    const uint64_t m = ballot(keep);
-                    if (lane == 0) s_wsum[wv] = (uint32_t)__popcll(m);
-                    __syncthreads();
-                    int wbase = kept;
-                    int add = 0;
-                    for (int k = 0; k < DF_WAVES; ++k) {
-                        const int v = (int)s_wsum[k];
-                        wbase += k < wv ? v : 0;
-                        add += v;
+                    if (kept + DF_LOADS * DF_THREADS <= DF_STAGE) {
+                        kept = df_stage_step<DF_LOADS>(L, t, total, kept, tsc, qa.lo, qa.hi, lr_sq, wv);
+                        t += DF_LOADS * DF_THREADS;
+                    } else if (kept + DF_THREADS <= DF_STAGE) {
+                        kept = df_stage_step<1>(L, t, total, kept, tsc, qa.lo, qa.hi, lr_sq, wv);
+                        t += DF_THREADS;
+                    } else {
+                        break;
                     }
-                    if (keep) s_pts[wbase + mbcnt(m)] = p;
-                    __syncthreads();
-                    kept += add;
-                    t += DF_THREADS;
                 }
-                t0 = t;
+                if (kept == 0) continue;  // block-uniform; the loop ends when more is false
                 // each wave: its queries through the staged candidates
-                for (int32_t qi = wv; qi < qc; qi += DF_WAVES) {
-                    const float4 qp = qsc[qf + qi];  // every lane the same query
-                    const float qx = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(qp.x)));
-                    const float qy = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(qp.y)));
-                    const float qz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(qp.z)));
-                    const int32_t i = __builtin_amdgcn_readfirstlane(__float_as_int(qp.w));
+                for (int j = 0; j < nmine; ++j) {
+                    const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), j));
+                    const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), j));
+                    const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), j));
+                    const int32_t i = __builtin_amdgcn_readlane(__float_as_int(myq.w), j);
                     const int32_t ix = self_pair ? i : -2;  // -2 matches no point: only the self pair drops j == i
-                    uint32_t cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)qoff[qf + qi]);
+                    const int qslot = qw0 + j;
+                    uint32_t cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.cur[qslot]);
                     uint32_t nrec = 0;
                     auto flush = [&]() {
                         __builtin_amdgcn_wave_barrier();
+                        // the query's next nrec edges, from wave-uniform bases
+                        int32_t* const os = out.senders + cur;
+                        int32_t* const orc = out.receivers + cur;
+                        float* const od = out.rdisp + 3 * (int64_t)cur;
+                        float* const on = out.rdist + cur;
                         for (uint32_t u0 = 0; u0 < nrec; u0 += OCT_WAVE) {
                             const uint32_t u = u0 + (uint32_t)lane;
                             if (u < nrec) {
-                                const float4 pp = s_pts[rec[u]];
+                                const float4 pp = L.pts[rec[u]];
                                 const float dx = qx - pp.x, dy = qy - pp.y, dz = qz - pp.z;
                                 const float d2 = dx * dx + dy * dy + dz * dz;
-                                const int64_t e = (int64_t)(cur + u);
-                                out.senders[e] = __float_as_int(pp.w) + libase;
-                                out.receivers[e] = i + libase;
-                                *reinterpret_cast<float3*>(out.rdisp + 3 * e) =
+                                const float sd = ballot(d2 < SQRT_DIRECT_MIN) ? sqrtf(d2) : sqrt_rn_direct(d2);
+                                os[u] = __float_as_int(pp.w) + libase;
+                                orc[u] = i + libase;
+                                *reinterpret_cast<float3*>(od + 3 * u) =
                                     make_float3(rel_disp(qx, pp.x, linv_r), rel_disp(qy, pp.y, linv_r),
                                                 rel_disp(qz, pp.z, linv_r));
-                                out.rdist[e] = sqrtf(d2) * linv_r;
+                                on[u] = sd * linv_r;
                             }
                         }
                         __builtin_amdgcn_wave_barrier();
                         cur += nrec;
                         nrec = 0;
                     };
-                    for (int s0 = 0; s0 < kept; s0 += OCT_WAVE) {  // wave-uniform
-                        if (nrec + (uint32_t)OCT_WAVE > (uint32_t)DF_RECS) flush();
-                        const int sl = s0 + lane;
-                        bool hit = false;
-                        if (sl < kept) {
-                            const float4 pp = s_pts[sl];
-                            const float dx = qx - pp.x, dy = qy - pp.y, dz = qz - pp.z;
-                            const float d2 = dx * dx + dy * dy + dz * dz;
-                            hit = d2 <= lr_sq && __float_as_int(pp.w) != ix;
-                        }
+                    // one chunk of 64 staged candidates: hits recorded (FILL) and counted;
+                    // the padding past kept is at infinity, so no slot past kept hits
+                    auto chunk = [&](const float4& p, int sl) {
+                        const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        const bool hit = (d2 <= lr_sq) & (__float_as_int(p.w) != ix);
                         const uint64_t m = ballot(hit);
-                        if (hit) rec[nrec + (uint32_t)mbcnt(m)] = (uint16_t)sl;
+                        if (FILL && hit) rec[nrec + (uint32_t)mbcnt(m)] = (uint16_t)sl;
                         nrec += (uint32_t)__popcll(m);
+                    };
+                    // DF_UNROLL chunks at a time while they lie inside the stage: their
+                    // LDS reads are issued together, so one latency serves them all
+                    int s0 = 0;
+                    for (; s0 + DF_UNROLL * OCT_WAVE <= kept; s0 += DF_UNROLL * OCT_WAVE) {  // wave-uniform
+                        if (FILL && nrec + (uint32_t)(DF_UNROLL * OCT_WAVE) > (uint32_t)DF_RECS) flush();
+                        float4 pp[DF_UNROLL];
+#pragma unroll
+                        for (int k = 0; k < DF_UNROLL; ++k) pp[k] = L.pts[s0 + k * OCT_WAVE + lane];
+#pragma unroll
+                        for (int k = 0; k < DF_UNROLL; ++k) chunk(pp[k], s0 + k * OCT_WAVE + lane);
                     }
-                    if (nrec) flush();
-                    if (lane == 0) qoff[qf + qi] = cur;  // this wave's query: the next round continues here
+                    for (; s0 < kept; s0 += OCT_WAVE) {  // wave-uniform; reads end inside the padding
+                        if (FILL && nrec + (uint32_t)OCT_WAVE > (uint32_t)DF_RECS) flush();
+                        chunk(L.pts[s0 + lane], s0 + lane);
+                    }
+                    if (FILL) {
+                        if (nrec) flush();
+                        if (lane == 0) L.cur[qslot] = cur;  // this wave's query: the next round continues here
+                    } else {
+                        if (lane == 0) L.cur[qslot] = cur + nrec;
+                    }
                 }
                 __syncthreads();  // the stage is rewritten next round
             }
-            __syncthreads();  // s_start is rewritten for the next group
+            __syncthreads();  // cur and start are rewritten for the next batch or leaf
+            if (!FILL && (int32_t)threadIdx.x < nb) {
+                const uint32_t c = L.cur[threadIdx.x];
+                qcount[qf + b0 + (int32_t)threadIdx.x] = c;
+                mytotal += c;
+            }
+        }
+        if (!FILL) {  // the leaf's total
+            const uint32_t ws = (uint32_t)wave_sum((int)mytotal);
+            if (lane == 0) L.wsum[0][wv] = ws;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t tot = 0;
+                for (int k = 0; k < DF_WAVES; ++k) tot += L.wsum[0][k];
+                lhits[r] = tot;
+            }
         }
     }
 }
@@ -1594,14 +1716,25 @@ hipError_t launch_edges_heavy_fill(const TreeView& q, const TreeView& t, const i
 
 hipError_t launch_edges_dense_fill(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
                                    int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
-                                   const uint32_t* n_heavy, const uint32_t* qcount, uint32_t* qoff,
+                                   const uint32_t* n_heavy, uint32_t* qcount, const uint32_t* qoff,
                                    const uint32_t* flags, EdgeOut out, int index_base, const SetLookup& sets,
                                    hipStream_t s) {
     // a fixed grid (the count is on the device); blocks loop over the heavy leaves
     const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(q.n_leaves, 2048));
-    hipLaunchKernelGGL(k_edges_dense_fill, dim3(g), dim3(DF_THREADS), 0, s, q.leaves, q.scoord, q.params, t.scoord,
-                       lists, lcount, cap, r_sq, inv_r, self_pair, heavy_list, n_heavy, qcount, qoff, flags, out,
-                       index_base, sets);
+    hipLaunchKernelGGL(k_edges_dense<EDGE_FILL>, dim3(g), dim3(DF_THREADS), 0, s, q.leaves, q.scoord, q.params,
+                       t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, heavy_list, n_heavy, qcount, qoff,
+                       nullptr, flags, out, index_base, sets);
+    return hipGetLastError();
+}
+
+hipError_t launch_edges_dense_count(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
+                                    int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
+                                    const uint32_t* n_heavy, uint32_t* qcount, uint32_t* lhits,
+                                    const SetLookup& sets, hipStream_t s) {
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(q.n_leaves, 2048));
+    hipLaunchKernelGGL(k_edges_dense<EDGE_COUNT>, dim3(g), dim3(DF_THREADS), 0, s, q.leaves, q.scoord, q.params,
+                       t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, heavy_list, n_heavy, qcount, nullptr,
+                       lhits, nullptr, EdgeOut{}, 0, sets);
     return hipGetLastError();
 }
 

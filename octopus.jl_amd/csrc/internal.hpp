@@ -275,9 +275,15 @@ hipError_t launch_edges_heavy_fill(const TreeView& q, const TreeView& t, const i
 // the same edges in the same order as launch_edges_heavy_fill.
 hipError_t launch_edges_dense_fill(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
                                    int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
-                                   const uint32_t* n_heavy, const uint32_t* qcount, uint32_t* qoff,
+                                   const uint32_t* n_heavy, uint32_t* qcount, const uint32_t* qoff,
                                    const uint32_t* flags, EdgeOut out, int index_base, const SetLookup& sets,
                                    hipStream_t s);
+// The same staging and sweep counting instead: per-query counts into qcount,
+// each heavy leaf's total into lhits (in place of launch_edges_heavy).
+hipError_t launch_edges_dense_count(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
+                                    int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
+                                    const uint32_t* n_heavy, uint32_t* qcount, uint32_t* lhits,
+                                    const SetLookup& sets, hipStream_t s);
 // Exact counts of the listed heavy leaves ahead of the single pass: per-query
 // counts into qcount, leaf totals into lhits.
 hipError_t launch_edges_heavy(const TreeView& q, const TreeView& t, const int2* lists, const int32_t* lcount,
