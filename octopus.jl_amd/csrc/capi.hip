@@ -1250,7 +1250,8 @@ static int edges_impl(oct_ctx* c, int q, int t, int mode, int order, int emit, o
                                                p.pred.as<float>(), SP_PLAN_BUDGET, SP_PLAN_MAX_LEAVES,
                                                p.pwork.as<uint32_t>(), p.pscan.as<uint32_t>(),
                                                p.wave_first.as<int32_t>(), (int32_t*)(flags + 12),
-                                               (uint64_t*)(flags + 14), c->stream));
+                                               (uint64_t*)(flags + 14), heavy ? p.heavy_flags(nql) : nullptr,
+                                               c->stream));
         }
         if (single && p.cap_edges <= 0) {
             // first call on the pair: size the outputs from the predicted total

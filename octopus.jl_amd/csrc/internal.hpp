@@ -357,13 +357,14 @@ hipError_t launch_log1p_u32(const uint32_t* in, int64_t n, float* out, hipStream
 // Predictions and emission plan from per-leaf features: pred = predicted
 // hits; runs of consecutive leaves packed greedily within chunks of 64 leaves
 // so that a run's predicted hits fit `budget` hit-list entries and it holds at
-// most max_leaves leaves; wave_first[n_runs + 1]; pred_total = sum of the
-// rounded-up predictions.  work: 4 * plan_chunks(n) words; scan_tmp:
-// scan_tmp_words(plan_chunks(n)) words.
+// most max_leaves leaves (leaves flagged in lheavy, when given, cost nothing:
+// the single pass does not record them); wave_first[n_runs + 1]; pred_total =
+// sum of the rounded-up predictions, heavy leaves included.  work: 4 *
+// plan_chunks(n) words; scan_tmp: scan_tmp_words(plan_chunks(n)) words.
 int64_t plan_chunks(int64_t n_leaves);
 hipError_t launch_predict_plan(const float* feat, int64_t n, const float* P, float* pred, uint32_t budget,
                                int max_leaves, uint32_t* work, uint32_t* scan_tmp, int32_t* wave_first,
-                               int32_t* n_runs, uint64_t* pred_total, hipStream_t s);
+                               int32_t* n_runs, uint64_t* pred_total, const uint8_t* lheavy, hipStream_t s);
 constexpr uint32_t SP_PLAN_BUDGET = 1536;  // hit-list entries of one wave (edges.hip)
 constexpr int SP_PLAN_MAX_LEAVES = 64;     // leaves one wave may own (edges.hip)
 

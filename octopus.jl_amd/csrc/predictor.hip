@@ -96,11 +96,12 @@ __device__ inline uint32_t plan_cost(float p, uint32_t budget) {
 // y[l] = network output (log1p of the predicted hits); hits[l] (optional) =
 // expm1(y) clamped at 0.  With runs != null each wave (64 consecutive leaves,
 // one plan chunk) also counts the runs of its greedy packing into runs[chunk]
-// and the rounded-up sum of its predictions into psum[chunk] (see k_plan_write).
+// and the rounded-up sum of its predictions into psum[chunk] (see k_plan_write;
+// lheavy as there).
 __global__ void __launch_bounds__(MLP_THREADS)
 k_mlp_infer(const float* __restrict__ X, int64_t n, const float* __restrict__ P, float* __restrict__ y,
             float* __restrict__ hits, uint32_t* __restrict__ runs, uint32_t* __restrict__ psum, uint32_t budget,
-            int max_leaves) {
+            int max_leaves, const uint8_t* __restrict__ lheavy) {
     __shared__ float sp[MLP_PARAMS];
     load_params(sp, P);
     const int64_t l = (int64_t)blockIdx.x * MLP_THREADS + threadIdx.x;
@@ -121,7 +122,7 @@ k_mlp_infer(const float* __restrict__ X, int64_t n, const float* __restrict__ P,
     const int64_t l0 = chunk * PLAN_CHUNK;
     if (l0 >= n) return;  // wave-uniform
     const int cnt = (int)(n - l0 < PLAN_CHUNK ? n - l0 : PLAN_CHUNK);
-    const uint32_t c = valid ? plan_cost(hv, budget) : 0u;
+    const uint32_t c = valid && !(lheavy && lheavy[l]) ? plan_cost(hv, budget) : 0u;
     uint32_t nr = 0, fill = budget + 1;
     int len = 0;
     for (int i = 0; i < cnt; ++i) {
@@ -329,19 +330,23 @@ __global__ void k_log1p_u32(const uint32_t* __restrict__ in, int64_t n, float* _
 // before the leaf that would take its predicted cost past the budget or its
 // length past max_leaves.  A leaf's cost is its prediction with 1/16 headroom
 // plus 2 entries (a prediction that is not a number costs the whole budget).
-// Runs never cross a chunk of PLAN_CHUNK leaves, so the packing is the same
-// on every call.  One wave per chunk: the run starts at run_base[chunk]; the
-// last chunk closes the table.
+// A heavy leaf (lheavy[l] != 0, when given) costs nothing: the single pass
+// takes its count from the heavy-leaf pass and never records its hits, and
+// a run per heavy leaf would only multiply the runs.  The predicted total
+// (psum) still counts it: it sizes the output.  Runs never cross a chunk of
+// PLAN_CHUNK leaves, so the packing is the same on every call.  One wave per
+// chunk: the run starts at run_base[chunk]; the last chunk closes the table.
 __global__ void __launch_bounds__(256)
 k_plan_write(const float* __restrict__ pred, int64_t n, uint32_t budget, int max_leaves,
-             const uint32_t* __restrict__ run_base, int32_t* __restrict__ wave_first, int32_t* __restrict__ n_runs) {
+             const uint32_t* __restrict__ run_base, int32_t* __restrict__ wave_first, int32_t* __restrict__ n_runs,
+             const uint8_t* __restrict__ lheavy) {
     const int64_t chunk = ((int64_t)blockIdx.x * 256 + threadIdx.x) / PLAN_CHUNK;
     const int lane = lane_id();
     const int64_t l0 = chunk * PLAN_CHUNK;
     if (l0 >= n) return;  // wave-uniform
     const int64_t l = l0 + lane;
     const int cnt = (int)(n - l0 < PLAN_CHUNK ? n - l0 : PLAN_CHUNK);
-    const uint32_t c = l < n ? plan_cost(pred[l], budget) : 0u;
+    const uint32_t c = l < n && !(lheavy && lheavy[l]) ? plan_cost(pred[l], budget) : 0u;
     uint64_t starts = 0;
     uint32_t fill = budget + 1;
     int len = 0;
@@ -378,7 +383,7 @@ hipError_t launch_leaf_features(const TreeView& q, const int2* lists, const uint
 hipError_t launch_mlp_infer(const float* X, int64_t n, const float* P, float* y, float* hits, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + MLP_THREADS - 1) / MLP_THREADS)), dim3(MLP_THREADS), 0, s,
-                       X, n, P, y, hits, nullptr, nullptr, 0u, 0);
+                       X, n, P, y, hits, nullptr, nullptr, 0u, 0, nullptr);
     return hipGetLastError();
 }
 
@@ -415,7 +420,7 @@ int64_t plan_chunks(int64_t n_leaves) { return (n_leaves + PLAN_CHUNK - 1) / PLA
 
 hipError_t launch_predict_plan(const float* feat, int64_t n, const float* P, float* pred, uint32_t budget,
                                int max_leaves, uint32_t* work, uint32_t* scan_tmp, int32_t* wave_first,
-                               int32_t* n_runs, uint64_t* pred_total, hipStream_t s) {
+                               int32_t* n_runs, uint64_t* pred_total, const uint8_t* lheavy, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int64_t nc = plan_chunks(n);
     uint32_t* runs = work;
@@ -423,13 +428,13 @@ hipError_t launch_predict_plan(const float* feat, int64_t n, const float* P, flo
     uint32_t* run_base = work + 2 * nc;
     uint32_t* psum_pre = work + 3 * nc;
     hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + MLP_THREADS - 1) / MLP_THREADS)), dim3(MLP_THREADS), 0, s,
-                       feat, n, P, nullptr, pred, runs, psum, budget, max_leaves);
+                       feat, n, P, nullptr, pred, runs, psum, budget, max_leaves, lheavy);
     hipError_t e = exclusive_scan(runs, run_base, nc, scan_tmp, nullptr, s);
     if (e != hipSuccess) return e;
     e = exclusive_scan(psum, psum_pre, nc, scan_tmp, pred_total, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_plan_write, dim3((unsigned)((nc * PLAN_CHUNK + 255) / 256)), dim3(256), 0, s, pred, n, budget,
-                       max_leaves, run_base, wave_first, n_runs);
+                       max_leaves, run_base, wave_first, n_runs, lheavy);
     return hipGetLastError();
 }
 
