@@ -489,6 +489,30 @@ def test_dense_fill_many_queries_and_candidates(orc):
     _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
 
 
+def test_planned_first_call_with_heavy_leaves(orc):
+    """A pair's first learned call under the predictor's plan where many
+    leaves are heavy (1M points of the clustered distribution, the shipped
+    clustered4m predictor): heavy leaves cost nothing in the plan's packing,
+    their edges come from the dense fill, and the edges equal the exact
+    path's position for position and the oracle's set."""
+    import octohip
+    from octohip import predictor
+
+    pts, r = datasets.clustered(n=1_000_000, seed=9)
+    tns, sid, _ = _build(pts, float(r))
+    predictor.load(tns, "clustered4m")
+    got = _gpu_edges(octohip.build_edges(tns, sid, sid, mode="learned"))
+    st = predictor.edge_stats(tns, sid, sid)
+    ex = _gpu_edges(octohip.build_edges(tns, sid, sid, mode="exact"))
+    print(f"clustered 1M: {st['heavy_leaves']} heavy of {st['n_leaves']} leaves, {st['n_edges']} edges, "
+          f"predicted {st['predicted_total']:.0f}", flush=True)
+    assert st["single_pass"] and st["used_predictor"] and st["heavy_leaves"] > 0
+    for a, b in zip(got, ex):  # same edges in the same order
+        assert np.array_equal(bits(a), bits(b))
+    o = orc.build_edges(orc.run(pts, float(r)), pts)
+    _edges_equal(got, (o.senders, o.receivers, o.rel_displacement, o.rel_dist_norm))
+
+
 PLAN_CASES = {
     "plain100k_leaf32": ("plain100k", 32),
     "plain100k_leaf8": ("plain100k", 8),
