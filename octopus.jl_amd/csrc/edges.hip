@@ -1163,11 +1163,12 @@ k_edges_dense(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qs
                     uint32_t nrec = 0;
                     auto flush = [&]() {
                         __builtin_amdgcn_wave_barrier();
-                        // the query's next nrec edges, from wave-uniform bases
-                        int32_t* const os = out.senders + cur;
-                        int32_t* const orc = out.receivers + cur;
-                        float* const od = out.rdisp + 3 * (int64_t)cur;
-                        float* const on = out.rdist + cur;
+                        // the query's next nrec (< 2^16) edges, from wave-uniform bases
+                        // with 32-bit per-lane byte offsets
+                        char* const os = reinterpret_cast<char*>(out.senders + cur);
+                        char* const orc = reinterpret_cast<char*>(out.receivers + cur);
+                        char* const od = reinterpret_cast<char*>(out.rdisp + 3 * (int64_t)cur);
+                        char* const on = reinterpret_cast<char*>(out.rdist + cur);
                         for (uint32_t u0 = 0; u0 < nrec; u0 += OCT_WAVE) {
                             const uint32_t u = u0 + (uint32_t)lane;
                             if (u < nrec) {
@@ -1175,12 +1176,12 @@ k_edges_dense(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qs
                                 const float dx = qx - pp.x, dy = qy - pp.y, dz = qz - pp.z;
                                 const float d2 = dx * dx + dy * dy + dz * dz;
                                 const float sd = ballot(d2 < SQRT_DIRECT_MIN) ? sqrtf(d2) : sqrt_rn_direct(d2);
-                                os[u] = __float_as_int(pp.w) + libase;
-                                orc[u] = i + libase;
-                                *reinterpret_cast<float3*>(od + 3 * u) =
+                                *reinterpret_cast<int32_t*>(os + 4u * u) = __float_as_int(pp.w) + libase;
+                                *reinterpret_cast<int32_t*>(orc + 4u * u) = i + libase;
+                                *reinterpret_cast<float3*>(od + 12u * u) =
                                     make_float3(rel_disp(qx, pp.x, linv_r), rel_disp(qy, pp.y, linv_r),
                                                 rel_disp(qz, pp.z, linv_r));
-                                on[u] = sd * linv_r;
+                                *reinterpret_cast<float*>(on + 4u * u) = sd * linv_r;
                             }
                         }
                         __builtin_amdgcn_wave_barrier();
@@ -1192,9 +1193,11 @@ k_edges_dense(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qs
                     auto chunk = [&](const float4& p, int sl) {
                         const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
                         const float d2 = dx * dx + dy * dy + dz * dz;
-                        const bool hit = (d2 <= lr_sq) & (__float_as_int(p.w) != ix);
-                        const uint64_t m = ballot(hit);
-                        if (FILL && hit) rec[nrec + (uint32_t)mbcnt(m)] = (uint16_t)sl;
+                        const bool in_r = d2 <= lr_sq, other = __float_as_int(p.w) != ix;
+                        // the mask from the two compares' own masks (a ballot of
+                        // their conjunction would rebuild it through a select)
+                        const uint64_t m = ballot(in_r) & ballot(other);
+                        if (FILL && (in_r & other)) rec[nrec + (uint32_t)mbcnt(m)] = (uint16_t)sl;
                         nrec += (uint32_t)__popcll(m);
                     };
                     // DF_UNROLL chunks at a time while they lie inside the stage: their
