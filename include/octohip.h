@@ -324,7 +324,8 @@ typedef struct oct_edge_stats {
     int32_t single_pass;       /* the edges came from the single pass */
     int32_t used_predictor;
     int32_t attempts;          /* passes the call needed (1 when nothing overflowed) */
-    int32_t heavy_leaves;      /* single pass: leaves counted ahead (too large to record in one sweep) */
+    int32_t heavy_leaves;      /* single pass: leaves counted ahead and filled after it (too large to
+                                  record in one sweep, or beside a dense region) */
 } oct_edge_stats;
 /* A learned call that returned on the single pass's early total knows its
  * recount counters (recounted / recorded leaves) only when its kernels end:
@@ -358,7 +359,8 @@ typedef struct oct_batch_edges {
     int32_t replayed;             /* 1: this call replayed the captured graph */
     int64_t recounted_leaves;     /* single pass: leaves swept twice */
     int64_t recorded_leaves;      /* single pass: leaves swept once */
-    int64_t heavy_leaves;         /* single pass: leaves counted ahead (too large to record in one sweep) */
+    int64_t heavy_leaves;         /* single pass: leaves counted ahead and filled after it (too large to
+                                     record in one sweep, or beside a dense region) */
 } oct_batch_edges;
 int oct_batch_create(oct_ctx* ctx, int n_sets, const float* coords, const int64_t* offsets, const float* radii,
                      int* batch_id);
