@@ -1722,8 +1722,11 @@ hipError_t launch_edges_dense_fill(const TreeView& q, const TreeView& t, const i
                                    const uint32_t* n_heavy, uint32_t* qcount, const uint32_t* qoff,
                                    const uint32_t* flags, EdgeOut out, int index_base, const SetLookup& sets,
                                    hipStream_t s) {
-    // a fixed grid (the count is on the device); blocks loop over the heavy leaves
-    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(q.n_leaves, 2048));
+    // a block per leaf row (the heavy count is on the device; blocks past it
+    // exit at once): the dispatcher then hands each free slot the next heavy
+    // leaf, which balances their uneven sizes (a grid of 2,048 blocks looping
+    // over the leaves was 20% slower at the clustered workload)
+    const unsigned g = (unsigned)std::max<int64_t>(1, q.n_leaves);
     hipLaunchKernelGGL(k_edges_dense<EDGE_FILL>, dim3(g), dim3(DF_THREADS), 0, s, q.leaves, q.scoord, q.params,
                        t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, heavy_list, n_heavy, qcount, qoff,
                        nullptr, flags, out, index_base, sets);
@@ -1734,7 +1737,7 @@ hipError_t launch_edges_dense_count(const TreeView& q, const TreeView& t, const 
                                     int cap, float r_sq, float inv_r, int self_pair, const int32_t* heavy_list,
                                     const uint32_t* n_heavy, uint32_t* qcount, uint32_t* lhits,
                                     const SetLookup& sets, hipStream_t s) {
-    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(q.n_leaves, 2048));
+    const unsigned g = (unsigned)std::max<int64_t>(1, q.n_leaves);  // as the fill
     hipLaunchKernelGGL(k_edges_dense<EDGE_COUNT>, dim3(g), dim3(DF_THREADS), 0, s, q.leaves, q.scoord, q.params,
                        t.scoord, lists, lcount, cap, r_sq, inv_r, self_pair, heavy_list, n_heavy, qcount, nullptr,
                        lhits, nullptr, EdgeOut{}, 0, sets);
