@@ -977,7 +977,7 @@ k_edges_heavy_fill(const NodeRec* __restrict__ qleaves, const float4* __restrict
 // a 64-query block (leaf_pass), each query's output is written a piece per
 // tile, and the pieces of 64 rows land in HBM as partial lines; here the block
 // stages the leaf's candidates (those within r of the leaf box, in sequence
-// order, across candidate leaf groups, DF_STAGE at a time) in LDS, and each
+// order, across candidate leaf groups, a stage at a time) in LDS, and each
 // wave takes one query at a time through all of them, recording its hits
 // (stage slots) and writing them out as consecutive edges of that query, whole
 // waves at a time.  The edges and their order (candidate sequence order per
@@ -989,31 +989,38 @@ k_edges_heavy_fill(const NodeRec* __restrict__ qleaves, const float4* __restrict
 // single pass raised flags[0] (capacity) or flags[1] (look-back or plan).
 constexpr int DF_THREADS = 256;
 constexpr int DF_WAVES = DF_THREADS / OCT_WAVE;
-constexpr int DF_STAGE = 1984;  // candidates staged per round
+// Candidates staged per round: FILL keeps rounds long (each round costs every
+// query a flush), COUNT keeps blocks small (it has no records; 960 against
+// 1,984 took its launch 1.48 -> 1.24 ms at the clustered workload, where the
+// fill went 3.99 -> 5.02 ms).
+constexpr int DF_STAGE_FILL = 1984;
+constexpr int DF_STAGE_COUNT = 960;
 constexpr int DF_UNROLL = 4;    // candidate chunks of 64 per step of the query sweep
-// the stage array: 64 slots past the last staged candidate hold points at
-// infinity, so that the sweep's last chunk reads them and finds no hit: 32 KB
-constexpr int DF_SLOTS = DF_STAGE + OCT_WAVE;
 constexpr int DF_RECS = 512;    // hit records per wave (stage slots)
-constexpr int DF_LOADS = 4;     // candidates each thread loads per staging step
+constexpr int DF_LOADS_MAX = 4;
 
-// Per-block LDS of k_edges_dense_fill.
+// Per-block LDS of k_edges_dense.  The stage array: 64 slots past the last
+// staged candidate hold points at infinity, so that the sweep's last chunk
+// reads them and finds no hit (32 KB for FILL).
+template <int STAGE, bool RECS>
 struct DenseLds {
-    float4 pts[DF_SLOTS];
-    uint16_t rec[DF_WAVES][DF_RECS];
+    static constexpr int stage = STAGE;
+    float4 pts[STAGE + OCT_WAVE];
+    uint16_t rec[DF_WAVES][RECS ? DF_RECS : 1];
     uint32_t cur[DF_THREADS];
     int32_t start[OCT_WAVE + 1];
     int32_t first[OCT_WAVE];
-    uint32_t wsum[DF_LOADS][DF_WAVES];
+    uint32_t wsum[DF_LOADS_MAX][DF_WAVES];
     uint32_t carry;
 };
 
 // One staging step: candidates [t, t + SU * DF_THREADS) of the current group
 // (clipped to total), those within r of the box appended to L.pts in sequence
 // order.  Block-uniform; returns the new stage size.
-template <int SU>
-__device__ inline int df_stage_step(DenseLds& L, int t, int total, int kept, const float4* __restrict__ tsc,
+template <int SU, typename LDS>
+__device__ inline int df_stage_step(LDS& L, int t, int total, int kept, const float4* __restrict__ tsc,
                                     float4 lo, float4 hi, float lr_sq, int wv) {
+    static_assert(SU <= DF_LOADS_MAX, "wsum holds DF_LOADS_MAX steps");
     float4 p[SU];
     bool keep[SU];
 #pragma unroll
@@ -1051,7 +1058,7 @@ __device__ inline int df_stage_step(DenseLds& L, int t, int total, int kept, con
         }
         if (keep[k]) L.pts[wb + mbcnt(m[k])] = p[k];
     }
-    if (threadIdx.x < OCT_WAVE)  // the padding after the new end (base <= DF_STAGE)
+    if (threadIdx.x < OCT_WAVE)  // the padding after the new end (base <= stage)
         L.pts[base + (int)threadIdx.x] = make_float4(INFINITY, INFINITY, INFINITY, __int_as_float(-1));
     __syncthreads();  // the stage is complete and wsum free again
     return base;
@@ -1071,7 +1078,10 @@ k_edges_dense(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qs
               const uint32_t* __restrict__ flags, EdgeOut out, int index_base0, SetLookup sets) {
     static_assert(MODE == EDGE_COUNT || MODE == EDGE_FILL, "count or fill");
     constexpr bool FILL = MODE == EDGE_FILL;
-    __shared__ DenseLds L;
+    constexpr int STAGE = FILL ? DF_STAGE_FILL : DF_STAGE_COUNT;
+    constexpr int LOADS = FILL ? 4 : 2;  // a first step of LOADS * 256 fits the empty stage
+    static_assert(LOADS * DF_THREADS <= STAGE, "a first staging step fits");
+    __shared__ DenseLds<STAGE, FILL> L;
     if (FILL && (flags[0] != 0 || flags[1] != 0)) return;  // block-uniform
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / OCT_WAVE));
     const int lane = lane_id();
@@ -1140,10 +1150,10 @@ k_edges_dense(const NodeRec* __restrict__ qleaves, const float4* __restrict__ qs
                         t = 0;
                         continue;
                     }
-                    if (kept + DF_LOADS * DF_THREADS <= DF_STAGE) {
-                        kept = df_stage_step<DF_LOADS>(L, t, total, kept, tsc, qa.lo, qa.hi, lr_sq, wv);
-                        t += DF_LOADS * DF_THREADS;
-                    } else if (kept + DF_THREADS <= DF_STAGE) {
+                    if (kept + LOADS * DF_THREADS <= STAGE) {
+                        kept = df_stage_step<LOADS>(L, t, total, kept, tsc, qa.lo, qa.hi, lr_sq, wv);
+                        t += LOADS * DF_THREADS;
+                    } else if (kept + DF_THREADS <= STAGE) {
                         kept = df_stage_step<1>(L, t, total, kept, tsc, qa.lo, qa.hi, lr_sq, wv);
                         t += DF_THREADS;
                     } else {
